@@ -1,0 +1,160 @@
+/*
+ * qgybj.h - C ABI of the MI355X (gfx950) QG-YBJ+ ETD-RK2 library.
+ *
+ * One handle is one model: the whole grid is resident in HBM and every entry
+ * point is serialised on the handle's HIP stream.  A handle is not
+ * thread-safe.  No exceptions cross the ABI: every function returns a status
+ * code (QG_OK = 0) and qg_last_error() returns the message of the last
+ * failure on that handle.
+ *
+ * Field exchange uses the reference's array layout: Julia column-major
+ * (z, x, y) with z fastest, i.e. element (k, i, j) at k + nz*(i + nx*j).
+ * Complex fields (q, B, psi, A, C) are interleaved (re, im) doubles; real
+ * fields (u, v, w) are doubles.  Spectral fields follow FFTW ordering of
+ * (kx, ky).  Inside the library fields are stored planar [z][y][x].
+ *
+ * Citations name the reference (subhk/QGYBJplus.jl v1.0.27) function each
+ * entry point replaces.
+ */
+#ifndef QGYBJ_H
+#define QGYBJ_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define QG_OK 0
+#define QG_ERR_ARGUMENT 1     /* ArgumentError / DimensionMismatch in the reference */
+#define QG_ERR_DEVICE 2       /* HIP runtime failure */
+#define QG_ERR_SINGULAR 3     /* "Singular matrix in Thomas solver" (src/elliptic.jl:1177-1193) */
+#define QG_ERR_UNSUPPORTED 4  /* configuration outside the built path (e.g. normal YBJ) */
+#define QG_ERR_GTSV 5         /* "LAPACK gtsv failed" (src/operators.jl:447-464) */
+
+/* field ids: ModelFields (src/core/fields.jl:11-24) */
+#define QG_FIELD_Q 0
+#define QG_FIELD_B 1
+#define QG_FIELD_PSI 2
+#define QG_FIELD_A 3
+#define QG_FIELD_C 4
+#define QG_FIELD_U 5
+#define QG_FIELD_V 6
+#define QG_FIELD_W 7
+
+/* wave formulations: src/core/components.jl:59-67 */
+#define QG_YBJ_PLUS 0
+#define QG_PASSIVE 1
+#define QG_YBJ 2 /* normal YBJ: rejected with QG_ERR_UNSUPPORTED */
+
+/* precision modes */
+#define QG_PRECISION_FP64 0  /* every transform, product and solve in FP64 */
+#define QG_PRECISION_MIXED 1 /* FP32 transforms and products; FP64 state, solves and ETD coefficients */
+
+/* space of a field handed to qg_set_field */
+#define QG_SPACE_SPECTRAL 0
+#define QG_SPACE_PHYSICAL 1
+
+typedef struct qg_handle qg_handle;
+
+/*
+ * Model configuration.  The host computes kx, ky, N2 and a_ell with the
+ * reference's own code (src/core/grid.jl:122-125, src/stratification.jl:308-318,
+ * src/physics.jl:2-25) so the coefficients are bit-identical; dz is the
+ * solver spacing G.z[2]-G.z[1] (1.0 when nz == 1).
+ * Closures: HorizontalHyperdiffusivity fills both term lists,
+ * FlowHyperdiffusivity only the flow list, WaveHyperdiffusivity only the wave
+ * list (src/core/components.jl:105-208, src/nonlinear.jl:1160-1183).
+ */
+typedef struct qg_config {
+  int nx, ny, nz;
+  double f;                 /* Coriolis parameter of FPlane */
+  double dz;                /* G.z[2]-G.z[1] */
+  const double* kx;         /* [nx] */
+  const double* ky;         /* [ny] */
+  const double* N2;         /* [nz] runtime N^2 profile (OperatorCoefficients.N²) */
+  const double* a_ell;      /* [nz] f^2/max(N^2, sqrt(eps)) */
+  int n_flow_terms;
+  const double* flow_coefficients;
+  const int* flow_orders;
+  int n_wave_terms;
+  const double* wave_coefficients;
+  const int* wave_orders;
+  double nu_z;              /* VerticalDiffusivity coefficient */
+  int fixed_flow;           /* FixedFlow() */
+  int wave_feedback;        /* WaveMeanFeedback() (NoFeedback/NoWaveFeedback = 0) */
+  int formulation;          /* QG_YBJ_PLUS | QG_PASSIVE | QG_YBJ */
+  int inviscid;             /* Inviscid() */
+  int linear;               /* LinearDynamics() */
+  int no_dispersion;        /* NoDispersion() */
+  int precision;            /* QG_PRECISION_FP64 | QG_PRECISION_MIXED */
+} qg_config;
+
+/* Replaces build_runtime + allocate_fields (src/core/runtime.jl:34-83,
+ * src/core/model.jl:79-121): allocates the model on the current device. */
+int qg_create(const qg_config* cfg, qg_handle** out);
+
+/* Replaces finalize_model! (src/core/model.jl:124). Safe on NULL. */
+int qg_destroy(qg_handle* h);
+
+/* Message of the last failing call on h (or of the last failed qg_create when
+ * h is NULL). */
+const char* qg_last_error(const qg_handle* h);
+
+/* Replaces _assign_initial_field! / FieldArray upload (src/simulation.jl:405-427):
+ * copies one field from host memory.  For complex fields `space` selects
+ * spectral values (copied) or physical values (forward FFT on the device, as
+ * fft_forward!).  Real fields (u, v, w) are physical. */
+int qg_set_field(qg_handle* h, int field, const double* host, int space);
+
+/* Copies one field to host memory (spectral for q, B, psi, A, C). */
+int qg_get_field(qg_handle* h, int field, double* host);
+
+/* Replaces step!(model, ::ExponentialRungeKutta2) (src/core/runtime.jl:239-263)
+ * -> _advance_etdrk2! (src/timestep.jl:335-415), nsteps times with step size
+ * dt, including the double-buffer swap of the prognostic fields. */
+int qg_step(qg_handle* h, double dt, int nsteps);
+
+/* Replaces _check_termination_conditions! (src/simulation.jl:25-44): any
+ * non-finite value in q, B or psi, and max |ifft(psi)| over the grid. */
+int qg_check(qg_handle* h, int* nonfinite, double* max_abs_psi);
+
+/* Model-level operators (src/core/runtime.jl:169-206). */
+int qg_invert_q_to_psi(qg_handle* h);                 /* invert_q_to_psi!(model) */
+int qg_invert_B_to_A(qg_handle* h);                   /* invert_B_to_A!(model) */
+int qg_compute_velocities(qg_handle* h, int compute_w); /* compute_velocities!(model) */
+/* q from psi: method 0 = compute_q_from_psi! (src/initialization.jl:597-644),
+ * 1 = compute_barotropic_q_from_psi! (src/initialization.jl:653-668). */
+int qg_compute_q_from_psi(qg_handle* h, int method);
+/* compute_qw_complex!(qw, B, ...) (src/nonlinear.jl:859-944) on the current
+ * B; result copied to host (complex, (z,x,y)). */
+int qg_compute_qw(qg_handle* h, double* host_out);
+
+/* fft_forward! (direction -1) / fft_backward! (+1) of one complex (z,x,y)
+ * host array (src/transforms.jl:19-23, src/parallel_mpi.jl:735-797). */
+int qg_fft(qg_handle* h, const double* host_in, double* host_out, int direction);
+
+/* Waits for the handle's stream. */
+int qg_synchronize(qg_handle* h);
+
+/* Measurement hooks (no reference counterpart: the reference has no
+ * profiler, docs/src/advanced/performance.md:47-51).  mode bit 1 records HIP
+ * events around every kernel launch on the handle's stream, bit 2 one event
+ * per step.  qg_profile_end writes a JSON summary
+ * {"kernels": {name: {"count", "ms", "bytes"}}, "steps_ms": [...]} where
+ * "bytes" is the algorithmic HBM traffic of those launches; *needed receives
+ * the buffer size the summary requires. */
+int qg_profile_begin(qg_handle* h, int mode);
+int qg_profile_end(qg_handle* h, char* buf, size_t len, size_t* needed);
+
+/* Largest |q| met on the kh=0 column by invert_q_to_psi since the last call
+ * (the reference warns once when it exceeds 1e-10, src/elliptic.jl:257-265);
+ * resets the record. */
+int qg_take_mean_q_warning(qg_handle* h, double* max_abs_mean_q);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QGYBJ_H */

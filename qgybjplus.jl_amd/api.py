@@ -1,0 +1,636 @@
+"""Host-side mirror of the reference's public model API on the GPU library.
+
+Names, argument meaning and error behaviour follow subhk/QGYBJplus.jl:
+``RectilinearGrid`` (src/core/grid.jl), the typed components
+(src/core/components.jl), ``QGYBJModel`` (src/core/model.jl), ``set!``
+(src/simulation.jl:461-503), ``ExponentialRungeKutta2``/``step!``
+(src/timestep.jl:60-70, src/core/runtime.jl:239-263), ``Simulation``/``run!``
+(src/simulation.jl:204-662).  Julia's bang functions are spelled ``set_``,
+``step_`` and ``run_``; ``ArgumentError`` is ``ValueError``.
+
+The model state lives in GPU memory; every numerical operation runs in
+``libqgybj.so``.  Only user callbacks (an analytical psi(x, y, z), a surface
+wave profile) are evaluated on the host, as in the reference.
+"""
+
+from __future__ import annotations
+
+import enum
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Callable, Optional, Sequence, Tuple, Union
+
+import numpy as np
+
+from . import _lib
+
+SQRT_EPS = math.sqrt(np.finfo(np.float64).eps)
+
+
+# ---------------------------------------------------------------------------
+# Geometry (src/core/grid.jl)
+# ---------------------------------------------------------------------------
+
+def _spectral_wavenumbers(n: int, length: float) -> np.ndarray:
+    """FFTW ordering, src/core/grid.jl:122-125."""
+    scale = (2.0 * math.pi) / length
+    m = np.arange(n)
+    return scale * np.where(m <= (n + 1) // 2 - 1, m, m - n).astype(np.float64)
+
+
+class RectilinearGrid:
+    """``RectilinearGrid(; size, extent, x, y, z, centered)``, src/core/grid.jl:127-204."""
+
+    def __init__(self, *, size, extent=None, x=None, y=None, z=None, centered=False):
+        if not (isinstance(size, tuple) and len(size) == 3):
+            raise ValueError("size must be a three-element tuple")
+        if not all(isinstance(v, (int, np.integer)) for v in size):
+            raise ValueError("grid dimensions must be integers")
+        nx, ny, nz = (int(v) for v in size)
+        if min(nx, ny, nz) <= 0:
+            raise ValueError("all grid dimensions must be positive")
+        if centered and (x is not None or y is not None):
+            raise ValueError("centered=true cannot be combined with explicit x or y ranges")
+
+        def bounds(b, name):
+            if b is None:
+                return None
+            if not (isinstance(b, tuple) and len(b) == 2):
+                raise ValueError(f"{name} must be a two-element tuple")
+            lo, hi = float(b[0]), float(b[1])
+            if not (math.isfinite(lo) and math.isfinite(hi)):
+                raise ValueError(f"{name} bounds must be finite")
+            if not hi > lo:
+                raise ValueError(f"{name} upper bound must exceed its lower bound")
+            return lo, hi
+
+        xb, yb, zb = bounds(x, "x"), bounds(y, "y"), bounds(z, "z")
+        if extent is None:
+            if xb is None or yb is None or zb is None:
+                raise ValueError("provide extent=(Lx, Ly, Lz) or explicit x, y and z bounds")
+            Lx, Ly, Lz = xb[1] - xb[0], yb[1] - yb[0], zb[1] - zb[0]
+        else:
+            if not (isinstance(extent, tuple) and len(extent) == 3):
+                raise ValueError("extent must be a three-element tuple")
+            Lx, Ly, Lz = (float(v) for v in extent)
+            if not all(math.isfinite(v) for v in (Lx, Ly, Lz)):
+                raise ValueError("all grid extents must be finite")
+            if not all(v > 0 for v in (Lx, Ly, Lz)):
+                raise ValueError("all grid extents must be positive")
+            for b, L, name in ((xb, Lx, "x"), (yb, Ly, "y"), (zb, Lz, "z")):
+                if b is not None and not math.isclose(b[1] - b[0], L, rel_tol=10 * 2.2e-16,
+                                                      abs_tol=10 * 2.2e-16 * max(abs(L), 1.0)):
+                    raise ValueError(f"{name} range length does not match extent")
+        x0 = xb[0] if xb is not None else (-Lx / 2 if centered else 0.0)
+        y0 = yb[0] if yb is not None else (-Ly / 2 if centered else 0.0)
+        zbot = zb[0] if zb is not None else -Lz
+        self.size = (nx, ny, nz)
+        self.extent = (Lx, Ly, Lz)
+        self.origin = (x0, y0)
+        self.dx, self.dy, self.dz = Lx / nx, Ly / ny, Lz / nz
+        self.x = np.array([x0 + i * self.dx for i in range(nx)])
+        self.y = np.array([y0 + j * self.dy for j in range(ny)])
+        self.z_faces = np.array([zbot + k * self.dz for k in range(nz + 1)])
+        self.z = np.array([(self.z_faces[k] + self.z_faces[k + 1]) / 2 for k in range(nz)])
+        self.x_faces = np.array([x0 + i * self.dx for i in range(nx + 1)])
+        self.y_faces = np.array([y0 + j * self.dy for j in range(ny + 1)])
+        self.kx = _spectral_wavenumbers(nx, Lx)
+        self.ky = _spectral_wavenumbers(ny, Ly)
+        self.kh2 = self.kx[:, None] ** 2 + self.ky[None, :] ** 2
+
+    @property
+    def dz_solver(self) -> float:
+        """Spacing the vertical operators use (``G.z[2]-G.z[1]``)."""
+        return float(self.z[1] - self.z[0]) if self.size[2] > 1 else 1.0
+
+    def __repr__(self):
+        return f"RectilinearGrid(size={self.size}, extent={self.extent}, origin={self.origin})"
+
+
+def dealias_mask(grid: RectilinearGrid) -> np.ndarray:
+    """Radial 2/3 mask (src/physics.jl:28-43); shape (nx, ny)."""
+    nx, ny = grid.size[:2]
+    kmax = min(nx, ny) // 3
+    i, j = np.arange(nx), np.arange(ny)
+    ix = np.where(i <= nx // 2, i, i - nx)
+    jy = np.where(j <= ny // 2, j, j - ny)
+    return (ix[:, None] ** 2 + jy[None, :] ** 2) <= kmax ** 2
+
+
+# ---------------------------------------------------------------------------
+# Components (src/core/components.jl, src/stratification.jl)
+# ---------------------------------------------------------------------------
+
+@dataclass(frozen=True)
+class FPlane:
+    f: float
+
+    def __post_init__(self):
+        v = float(self.f)
+        if not math.isfinite(v):
+            raise ValueError(f"f must be finite (got {self.f})")
+        if v == 0:
+            raise ValueError("f must be non-zero")
+        object.__setattr__(self, "f", v)
+
+
+@dataclass(frozen=True)
+class ConstantStratification:
+    N2: float
+
+    def __post_init__(self):
+        v = float(self.N2)
+        if not math.isfinite(v):
+            raise ValueError(f"N² must be finite (got {self.N2})")
+        if not v > 0:
+            raise ValueError(f"N² must be positive (got {self.N2})")
+        object.__setattr__(self, "N2", v)
+
+    def evaluate(self, z: float) -> float:
+        """Through ``ConstantN(sqrt(N²))`` (src/core/runtime.jl:29-30)."""
+        n0 = math.sqrt(self.N2)
+        return n0 * n0
+
+
+@dataclass(frozen=True)
+class AnalyticalProfile:
+    """``AnalyticalProfile(func; returns=:N²)`` (src/stratification.jl:95-111,263-270)."""
+    func: Callable[[float], float]
+    returns: str = "N2"
+
+    def __post_init__(self):
+        if self.returns not in ("N", "N2"):
+            raise ValueError("returns must be :N or :N²")
+
+    def evaluate(self, z: float) -> float:
+        val = float(self.func(z))
+        return max(val, 0.0) if self.returns == "N2" else max(val * val, 0.0)
+
+
+class FlowEvolution: ...
+class FixedFlow(FlowEvolution): ...
+class EvolvingFlow(FlowEvolution): ...
+class FeedbackMode: ...
+class NoFeedback(FeedbackMode): ...
+class WaveMeanFeedback(FeedbackMode): ...
+class NoWaveFeedback(FeedbackMode): ...
+class WaveFormulation: ...
+class YBJPlus(WaveFormulation): ...
+class YBJ(WaveFormulation): ...
+class PassiveWave(WaveFormulation): ...
+class DissipationMode: ...
+class Dissipative(DissipationMode): ...
+class Inviscid(DissipationMode): ...
+class DynamicsMode: ...
+class NonlinearDynamics(DynamicsMode): ...
+class LinearDynamics(DynamicsMode): ...
+class DispersionMode: ...
+class Dispersive(DispersionMode): ...
+class NoDispersion(DispersionMode): ...
+
+
+def _terms(field_name, coefficients, orders):
+    if len(coefficients) == 0:
+        raise ValueError(f"{field_name} hyperdiffusivity requires at least one term")
+    if len(coefficients) != len(orders):
+        raise ValueError(f"{field_name} hyperdiffusivity coefficients and orders must have equal lengths")
+    vals = tuple(float(c) for c in coefficients)
+    if not all(math.isfinite(v) and v >= 0 for v in vals):
+        raise ValueError(f"{field_name} hyperdiffusivity coefficients must be finite and non-negative")
+    ords = tuple(int(o) for o in orders)
+    if not all(o > 0 and o % 2 == 0 for o in ords):
+        raise ValueError(f"{field_name} hyperdiffusivity orders must be positive and even")
+    return vals, ords
+
+
+class FlowHyperdiffusivity:
+    """Balanced-flow damping only (src/core/components.jl:105-160)."""
+
+    def __init__(self, coefficient=None, *, order: int = 4, coefficients=None, orders=None):
+        if coefficients is None:
+            if coefficient is None:
+                raise ValueError("coefficient is required")
+            coefficients, orders = (coefficient,), (order,)
+        self.coefficients, self.orders = _terms("flow", coefficients, orders)
+
+
+class WaveHyperdiffusivity:
+    """Wave damping only (src/core/components.jl:162-192)."""
+
+    def __init__(self, coefficient=None, *, order: int = 4, coefficients=None, orders=None):
+        if coefficients is None:
+            if coefficient is None:
+                raise ValueError("coefficient is required")
+            coefficients, orders = (coefficient,), (order,)
+        self.coefficients, self.orders = _terms("wave", coefficients, orders)
+
+
+class HorizontalHyperdiffusivity:
+    """Flow + wave damping; default 4th+12th order (src/core/components.jl:194-208)."""
+
+    def __init__(self, *, flow: Optional[FlowHyperdiffusivity] = None,
+                 wave: Optional[WaveHyperdiffusivity] = None):
+        self.flow = flow if flow is not None else FlowHyperdiffusivity(
+            coefficients=(0.01, 10.0), orders=(4, 12))
+        self.wave = wave if wave is not None else WaveHyperdiffusivity(
+            coefficients=(0.0, 10.0), orders=(4, 12))
+
+
+@dataclass(frozen=True)
+class VerticalDiffusivity:
+    coefficient: float = 0.0
+
+    def __post_init__(self):
+        v = float(self.coefficient)
+        if not math.isfinite(v):
+            raise ValueError("vertical diffusivity must be finite")
+        if v < 0:
+            raise ValueError("vertical diffusivity must be non-negative")
+        object.__setattr__(self, "coefficient", v)
+
+
+@dataclass(frozen=True)
+class SurfaceWave:
+    """Horizontally uniform surface wave (src/core/components.jl:211-215,274-283)."""
+    amplitude: float
+    scale: float
+    profile: str = "gaussian"
+
+    def __post_init__(self):
+        if not (math.isfinite(self.amplitude) and math.isfinite(self.scale)):
+            raise ValueError("wave amplitude and scale must be finite")
+        if not self.scale > 0:
+            raise ValueError(f"wave scale must be positive (got {self.scale})")
+        if self.profile not in ("gaussian", "exponential"):
+            raise ValueError("profile must be :gaussian or :exponential")
+
+
+@dataclass(frozen=True)
+class FieldArray:
+    """A 3-D field for ``set_`` (src/core/components.jl:225-241)."""
+    values: np.ndarray
+    space: str = "physical"
+    layout: str = "zxy"
+
+    def __post_init__(self):
+        if np.ndim(self.values) != 3:
+            raise ValueError("field arrays must be three-dimensional")
+        if self.space not in ("physical", "spectral"):
+            raise ValueError("field space must be :physical or :spectral")
+        if self.layout not in ("zxy", "xyz"):
+            raise ValueError("field layout must be :zxy or :xyz")
+
+
+@dataclass(frozen=True)
+class GPU:
+    """Architecture selecting this library: one MI355X holds the whole grid.
+
+    ``precision`` is "fp64" (plain) or "mixed" (FP32 transforms and products,
+    FP64 state, column solves and ETD coefficients)."""
+    precision: str = "fp64"
+
+    def __post_init__(self):
+        if self.precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+
+
+def _component(value, kinds, sym_map, bool_map, what):
+    if isinstance(value, kinds):
+        return value
+    if isinstance(value, bool) and bool_map is not None:
+        return bool_map[value]()
+    if isinstance(value, str) and value in sym_map:
+        return sym_map[value]()
+    raise ValueError(f"{what} must be one of {sorted(sym_map)} or a component instance")
+
+
+# ---------------------------------------------------------------------------
+# Model (src/core/model.jl, src/core/runtime.jl)
+# ---------------------------------------------------------------------------
+
+class _Fields:
+    """Accessor mirroring ``model.fields`` (ModelFields, src/core/fields.jl).
+
+    Reading an attribute downloads the field in (z, x, y) order; assigning one
+    uploads it (spectral values for q, B, psi, A, C)."""
+
+    def __init__(self, model):
+        object.__setattr__(self, "_m", model)
+
+    def __getattr__(self, name):
+        if name in _lib.FIELD_IDS:
+            return self._m._h.get_field(name)
+        raise AttributeError(name)
+
+    def __setattr__(self, name, value):
+        if name in _lib.FIELD_IDS:
+            self._m._h.set_field(name, value, _lib.SPACE_SPECTRAL)
+        else:
+            raise AttributeError(name)
+
+
+class QGYBJModel:
+    """``QGYBJModel(; grid, coriolis, stratification, closure, ...)`` on one GPU."""
+
+    def __init__(self, *, grid: RectilinearGrid, coriolis=None, stratification=None,
+                 closure=None, vertical_diffusion=None, flow="evolving", feedback="none",
+                 formulation=None, ybj_plus: bool = True, inviscid=False, linear=False,
+                 no_dispersion=False, architecture: GPU = GPU(), verbose: bool = False):
+        if not isinstance(grid, RectilinearGrid):
+            raise ValueError("grid must be a RectilinearGrid")
+        self.grid = grid
+        self.coriolis = coriolis if isinstance(coriolis, FPlane) else FPlane(1e-4 if coriolis is None else coriolis)
+        if stratification is None:
+            stratification = ConstantStratification(1e-5)
+        elif isinstance(stratification, (int, float)):
+            stratification = ConstantStratification(stratification)
+        self.stratification = stratification
+        self.closure = closure if closure is not None else HorizontalHyperdiffusivity()
+        vd = vertical_diffusion if vertical_diffusion is not None else VerticalDiffusivity()
+        self.vertical_diffusion = vd if isinstance(vd, VerticalDiffusivity) else VerticalDiffusivity(vd)
+        self.flow = _component(flow, FlowEvolution, {"fixed": FixedFlow, "evolving": EvolvingFlow},
+                               {True: FixedFlow, False: EvolvingFlow}, "flow")
+        self.feedback = _component(feedback, FeedbackMode,
+                                   {"none": NoFeedback, "off": NoFeedback, "wave_mean": WaveMeanFeedback,
+                                    "on": WaveMeanFeedback, "no_wave_feedback": NoWaveFeedback},
+                                   {True: WaveMeanFeedback, False: NoFeedback}, "feedback")
+        if formulation is None:
+            self.formulation = YBJPlus() if ybj_plus else YBJ()
+        else:
+            self.formulation = _component(formulation, WaveFormulation,
+                                          {"ybj_plus": YBJPlus, "ybj": YBJ, "passive": PassiveWave},
+                                          None, "formulation")
+        self.dissipation = _component(inviscid, DissipationMode, {}, {True: Inviscid, False: Dissipative},
+                                      "inviscid")
+        self.dynamics = _component(linear, DynamicsMode, {}, {True: LinearDynamics, False: NonlinearDynamics},
+                                   "linear")
+        self.dispersion = _component(no_dispersion, DispersionMode, {},
+                                     {True: NoDispersion, False: Dispersive}, "no_dispersion")
+        self.architecture = architecture
+
+        nx, ny, nz = grid.size
+        # N² on the bottom face of each cell (src/stratification.jl:308-318)
+        self.N2 = np.array([float(self.stratification.evaluate(grid.z[k] - grid.dz / 2))
+                            for k in range(nz)])
+        if not np.all(np.isfinite(self.N2)) or not np.all(self.N2 > 0):
+            raise ValueError("N² must be finite and positive at every level")
+        f = self.coriolis.f
+        self.a_ell = (f * f) / np.maximum(self.N2, SQRT_EPS)  # src/physics.jl:2-25
+        self.dealias_mask = dealias_mask(grid)
+
+        cl = self.closure
+        if isinstance(cl, HorizontalHyperdiffusivity):
+            flow_terms = list(zip(cl.flow.coefficients, cl.flow.orders))
+            wave_terms = list(zip(cl.wave.coefficients, cl.wave.orders))
+        elif isinstance(cl, FlowHyperdiffusivity):
+            flow_terms, wave_terms = list(zip(cl.coefficients, cl.orders)), []
+        elif isinstance(cl, WaveHyperdiffusivity):
+            flow_terms, wave_terms = [], list(zip(cl.coefficients, cl.orders))
+        else:
+            raise ValueError("closure must be a hyperdiffusivity component")
+        form = {YBJPlus: _lib.YBJ_PLUS, PassiveWave: _lib.PASSIVE, YBJ: _lib.YBJ}[type(self.formulation)]
+        self._h = _lib.Handle(
+            nx=nx, ny=ny, nz=nz, f=f, dz=grid.dz_solver, kx=grid.kx, ky=grid.ky, N2=self.N2,
+            a_ell=self.a_ell, flow_terms=flow_terms, wave_terms=wave_terms,
+            nu_z=self.vertical_diffusion.coefficient,
+            fixed_flow=isinstance(self.flow, FixedFlow),
+            wave_feedback=isinstance(self.feedback, WaveMeanFeedback),
+            formulation=form, inviscid=isinstance(self.dissipation, Inviscid),
+            linear=isinstance(self.dynamics, LinearDynamics),
+            no_dispersion=isinstance(self.dispersion, NoDispersion),
+            precision=_lib.PRECISION_MIXED if architecture.precision == "mixed" else _lib.PRECISION_FP64)
+        self.fields = _Fields(self)
+        self.finalized = False
+
+    # model-level operators (src/core/runtime.jl:169-206)
+    def invert_q_to_psi(self):
+        self._h.invert_q_to_psi()
+        return self
+
+    def invert_B_to_A(self):
+        self._h.invert_B_to_A()
+        return self
+
+    def compute_velocities(self, compute_w: bool = True):
+        self._h.compute_velocities(compute_w)
+        return self
+
+    def fft_forward(self, physical) -> np.ndarray:
+        return self._h.fft(physical, -1)
+
+    def fft_backward(self, spectral) -> np.ndarray:
+        return self._h.fft(spectral, +1)
+
+    def compute_qw(self) -> np.ndarray:
+        return self._h.compute_qw()
+
+    def finalize(self):
+        if not self.finalized:
+            self._h.close()
+            self.finalized = True
+        return self
+
+    def __repr__(self):
+        return f"QGYBJModel(grid={self.grid.size}, architecture={self.architecture})"
+
+
+def finalize_model_(model: QGYBJModel):
+    return model.finalize()
+
+
+# ---------------------------------------------------------------------------
+# set! (src/simulation.jl:230-329, 405-503)
+# ---------------------------------------------------------------------------
+
+def _evaluate_on_nodes(grid, fn) -> np.ndarray:
+    nx, ny, nz = grid.size
+    X, Y = np.meshgrid(grid.x, grid.y, indexing="ij")
+    out = np.empty((nz, nx, ny), dtype=np.complex128)
+    for k in range(nz):
+        try:
+            vals = fn(X, Y, grid.z[k])
+            vals = np.broadcast_to(np.asarray(vals, dtype=np.complex128), (nx, ny))
+        except Exception:
+            vals = np.array([[fn(grid.x[i], grid.y[j], grid.z[k]) for j in range(ny)]
+                             for i in range(nx)], dtype=np.complex128)
+        out[k] = vals
+    return out
+
+
+def _field_values_zxy(model, source) -> Tuple[np.ndarray, str]:
+    if isinstance(source, FieldArray):
+        vals = np.asarray(source.values)
+        if source.layout == "xyz":
+            vals = np.transpose(vals, (2, 0, 1))
+        space = source.space
+    else:
+        vals, space = np.asarray(source), "physical"
+    nx, ny, nz = model.grid.size
+    if vals.shape != (nz, nx, ny):
+        raise ValueError(f"initial field has size {vals.shape}; expected {(nz, nx, ny)} in z-x-y layout")
+    return vals.astype(np.complex128), space
+
+
+def set_(model, *, psi=None, mean_flow=None, pv_method: str = "qg", waves=None, B=None,
+         verbose: bool = False):
+    """``set!(model; ψ, pv_method, waves)`` (src/simulation.jl:461-503)."""
+    if isinstance(model, Simulation):
+        model._assert_mutable()
+        model = model.model
+    if model.finalized:
+        raise RuntimeError("cannot modify a finalized model")
+    h = model._h
+    flow = mean_flow if mean_flow is not None else psi
+    wave = waves if waves is not None else B
+    if flow is not None:
+        if callable(flow):
+            h.set_field("psi", _evaluate_on_nodes(model.grid, flow), _lib.SPACE_PHYSICAL)
+        elif isinstance(flow, (FieldArray, np.ndarray)):
+            vals, space = _field_values_zxy(model, flow)
+            h.set_field("psi", vals, _lib.SPACE_PHYSICAL if space == "physical" else _lib.SPACE_SPECTRAL)
+        else:
+            raise ValueError("mean_flow/ψ must be a function, array or FieldArray")
+        if pv_method in ("qg", "balanced"):
+            h.compute_q_from_psi(barotropic=False)
+        elif pv_method in ("barotropic", "asselin"):
+            h.compute_q_from_psi(barotropic=True)
+        elif pv_method != "none":
+            raise ValueError("pv_method must be :qg, :barotropic, or :none")
+        h.compute_velocities(compute_w=False)
+    if wave is not None:
+        if isinstance(wave, SurfaceWave):
+            g = model.grid
+            nx, ny, nz = g.size
+            phys = np.empty((nz, nx, ny), dtype=np.complex128)
+            for k in range(nz):
+                depth = max(0.0, -g.z[k] - g.dz / 2)
+                prof = math.exp(-(depth ** 2) / wave.scale ** 2) if wave.profile == "gaussian" \
+                    else math.exp(-depth / wave.scale)
+                phys[k] = complex(wave.amplitude * prof)
+            h.set_field("B", phys, _lib.SPACE_PHYSICAL)
+        elif isinstance(wave, (FieldArray, np.ndarray)):
+            vals, space = _field_values_zxy(model, wave)
+            h.set_field("B", vals, _lib.SPACE_PHYSICAL if space == "physical" else _lib.SPACE_SPECTRAL)
+        else:
+            raise ValueError("waves/B must be a SurfaceWave, array or FieldArray")
+        if isinstance(model.formulation, YBJ):
+            raise NotImplementedError("the normal YBJ formulation is not part of this library")
+        h.invert_B_to_A()
+    return model
+
+
+# ---------------------------------------------------------------------------
+# Time stepping (src/timestep.jl:60-70, src/core/runtime.jl:239-263)
+# ---------------------------------------------------------------------------
+
+class ExponentialRungeKutta2:
+    def __init__(self, *, dt: float):
+        v = float(dt)
+        if not math.isfinite(v):
+            raise ValueError("Δt must be finite")
+        if not v > 0:
+            raise ValueError("Δt must be positive")
+        self.dt = v
+        self.workspace = None
+
+
+def step_(model: QGYBJModel, timestepper: ExponentialRungeKutta2, nsteps: int = 1):
+    """``step!(model, timestepper)``; the workspace lives on the device."""
+    if model.finalized:
+        raise RuntimeError("cannot use operators after model finalization")
+    model._h.step(timestepper.dt, nsteps)
+    timestepper.workspace = "device"
+    return model
+
+
+class SimulationState(enum.Enum):
+    Ready = 0
+    Running = 1
+    Completed = 2
+    Failed = 3
+    Finalized = 4
+
+
+@dataclass
+class Clock:
+    time: float = 0.0
+    iteration: int = 0
+
+
+class Simulation:
+    """``Simulation(model; Δt, stop_time, stop_iteration)`` (src/simulation.jl:204-227)."""
+
+    def __init__(self, model: QGYBJModel, *, dt: float = 1.0, stop_time=None, stop_iteration=None,
+                 check_every: int = 1):
+        if model.finalized:
+            raise RuntimeError("cannot create a simulation from a finalized model")
+        self.model = model
+        self.timestepper = ExponentialRungeKutta2(dt=dt)
+        self.clock = Clock()
+        self.stop_time = stop_time
+        self.stop_iteration = stop_iteration
+        self.check_every = int(check_every)
+        self.state = SimulationState.Ready
+
+    def _assert_mutable(self):
+        if self.state in (SimulationState.Running, SimulationState.Failed, SimulationState.Finalized):
+            raise RuntimeError(f"cannot modify a {self.state.name.lower()} simulation")
+
+    def _before_stop_time(self) -> bool:
+        if self.stop_time is None:
+            return True
+        t = self.clock.time
+        tol = 8 * np.finfo(float).eps * max(abs(t), abs(self.stop_time), abs(self.timestepper.dt), 1.0)
+        return t < self.stop_time - tol
+
+
+def check_termination_(simulation: Simulation):
+    """``_check_termination_conditions!`` (src/simulation.jl:25-44)."""
+    bad, m = simulation.model._h.check()
+    if bad:
+        raise FloatingPointError("non-finite value detected in the model state")
+    if not m <= 1e10:
+        raise FloatingPointError(f"solution appears to be blowing up (max |psi| = {m})")
+    return simulation
+
+
+def run_(simulation: Simulation, *, stop_iteration=None, stop_time=None, progress: bool = False):
+    """``run!(simulation)`` (src/simulation.jl:570-662) without output writers."""
+    if simulation.state is SimulationState.Finalized:
+        raise RuntimeError("cannot run a finalized simulation")
+    if stop_iteration is not None:
+        simulation.stop_iteration = stop_iteration
+    if stop_time is not None:
+        simulation.stop_time = stop_time
+    if simulation.stop_iteration is None and simulation.stop_time is None:
+        raise ValueError("set stop_iteration or stop_time before run!")
+    simulation.state = SimulationState.Running
+    model = simulation.model
+    ts = simulation.timestepper
+    try:
+        while True:
+            it = simulation.clock.iteration
+            if simulation.stop_iteration is not None and it >= simulation.stop_iteration:
+                break
+            if not simulation._before_stop_time():
+                break
+            step_(model, ts)
+            if simulation.check_every and (it + 1) % simulation.check_every == 0:
+                check_termination_(simulation)
+            simulation.clock.iteration += 1
+            simulation.clock.time += ts.dt
+            if progress:
+                print(f"iteration={simulation.clock.iteration} | time={simulation.clock.time} s", flush=True)
+        simulation.state = SimulationState.Completed
+    except Exception:
+        simulation.state = SimulationState.Failed
+        raise
+    return simulation
+
+
+def finalize_simulation_(simulation: Simulation):
+    if simulation.state is not SimulationState.Finalized:
+        simulation.model.finalize()
+        simulation.state = SimulationState.Finalized
+    return simulation

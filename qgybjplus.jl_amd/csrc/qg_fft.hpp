@@ -1,0 +1,235 @@
+// LDS-staged mixed-radix Stockham FFT for one sequence held by a team of
+// threads.  Written for CDNA4 wave64: a sequence of length N is processed by
+// T lanes; the first stage reads from registers and the last stage leaves its
+// result in registers with the SAME element distribution, so a kernel can run
+// an inverse transform, do pointwise physics in registers, and feed a forward
+// transform without another LDS round trip.
+//
+// Register slot v[it][t] of lane `lane` holds element  (lane + it*T) + t*(N/R0).
+//
+// Stage algorithm (decimation in time, self-sorting): with span P (product of
+// the previous radices) and radix r, butterfly i in [0, N/r) reads
+// x[i + t*N/r], multiplies by w_{rP}^{t*(i mod P)}, applies an r-point DFT and
+// writes y[(i - i mod P)*r + i mod P + t*P].
+#pragma once
+
+#include "qg_common.hpp"
+
+namespace qg {
+
+// cos(2*pi*m/48) for m in [0, 12]; every radix used here divides 48.
+constexpr double kCos48[13] = {
+    1.0, 0.9914448613738104, 0.9659258262890683, 0.9238795325112867,
+    0.8660254037844386, 0.7933533402912352, 0.7071067811865476,
+    0.6087614290087207, 0.5, 0.3826834323650898, 0.25881904510252074,
+    0.1305261922200516, 0.0};
+
+__host__ __device__ constexpr double cos48(int m) {
+  m = ((m % 48) + 48) % 48;
+  if (m <= 12) return kCos48[m];
+  if (m <= 24) return -kCos48[24 - m];
+  if (m <= 36) return -kCos48[m - 24];
+  return kCos48[48 - m];
+}
+__host__ __device__ constexpr double sin48(int m) { return cos48(m - 12); }
+
+// exp(DIR * 2*pi*i * m / N) for N dividing 48.
+template <typename R, int N, int DIR>
+__device__ inline cplx<R> unit_root(int m) {
+  constexpr int f = 48 / N;
+  return mk<R>((R)cos48(m * f), (R)(DIR * sin48(m * f)));
+}
+
+// ---------------------------------------------------------------------------
+// In-register DFTs.  DIR = -1 forward (exp(-i..)), +1 inverse (unnormalised).
+// ---------------------------------------------------------------------------
+template <int N, int DIR>
+struct Dft;
+
+template <int DIR>
+struct Dft<1, DIR> {
+  template <typename R>
+  __device__ static inline void run(cplx<R>*) {}
+};
+
+template <int DIR>
+struct Dft<2, DIR> {
+  template <typename R>
+  __device__ static inline void run(cplx<R>* v) {
+    cplx<R> a = v[0], b = v[1];
+    v[0] = a + b;
+    v[1] = a - b;
+  }
+};
+
+template <int DIR>
+struct Dft<3, DIR> {
+  template <typename R>
+  __device__ static inline void run(cplx<R>* v) {
+    const R h = (R)0.8660254037844386;  // sin(2*pi/3)
+    cplx<R> a = v[0], b = v[1], c = v[2];
+    cplx<R> s = b + c;
+    cplx<R> d = b - c;
+    cplx<R> m = mk<R>(a.re - (R)0.5 * s.re, a.im - (R)0.5 * s.im);
+    // DIR * i * h * d
+    cplx<R> r = mk<R>(-(R)DIR * h * d.im, (R)DIR * h * d.re);
+    v[0] = a + s;
+    v[1] = m + r;
+    v[2] = m - r;
+  }
+};
+
+template <int DIR>
+struct Dft<4, DIR> {
+  template <typename R>
+  __device__ static inline void run(cplx<R>* v) {
+    cplx<R> a = v[0], b = v[1], c = v[2], d = v[3];
+    cplx<R> s0 = a + c, d0 = a - c, s1 = b + d, d1 = b - d;
+    cplx<R> r = mk<R>(-(R)DIR * d1.im, (R)DIR * d1.re);  // DIR*i*d1
+    v[0] = s0 + s1;
+    v[2] = s0 - s1;
+    v[1] = d0 + r;
+    v[3] = d0 - r;
+  }
+};
+
+// Composite DFT of size N1*N2 (Cooley-Tukey, decimation in time):
+// input n = N2*n1 + n2, output k = k1 + N1*k2.
+template <int N1, int N2, int DIR>
+struct DftCT {
+  template <typename R>
+  __device__ static inline void run(cplx<R>* v) {
+    constexpr int N = N1 * N2;
+    cplx<R> y[N2][N1];
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) {
+#pragma unroll
+      for (int n1 = 0; n1 < N1; ++n1) y[n2][n1] = v[N2 * n1 + n2];
+      Dft<N1, DIR>::run(y[n2]);
+#pragma unroll
+      for (int k1 = 1; k1 < N1; ++k1) {
+        if (n2 != 0) y[n2][k1] = cmul(y[n2][k1], unit_root<R, N, DIR>(n2 * k1));
+      }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) {
+      cplx<R> z[N2];
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) z[n2] = y[n2][k1];
+      Dft<N2, DIR>::run(z);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) v[k1 + N1 * k2] = z[k2];
+    }
+  }
+};
+
+template <int DIR> struct Dft<6, DIR> : DftCT<2, 3, DIR> {};
+template <int DIR> struct Dft<8, DIR> : DftCT<2, 4, DIR> {};
+template <int DIR> struct Dft<12, DIR> : DftCT<4, 3, DIR> {};
+template <int DIR> struct Dft<16, DIR> : DftCT<4, 4, DIR> {};
+
+// ---------------------------------------------------------------------------
+// Plans: radices per stage (first == last so the register distribution of the
+// input and of the output coincide) and lanes per sequence T.
+// ---------------------------------------------------------------------------
+template <int N> struct FftPlan;
+template <> struct FftPlan<8>   { static constexpr int S = 1, r0 = 8, r1 = 1, T = 1; };
+template <> struct FftPlan<16>  { static constexpr int S = 2, r0 = 4, r1 = 1, T = 4; };
+template <> struct FftPlan<32>  { static constexpr int S = 3, r0 = 4, r1 = 2, T = 8; };
+template <> struct FftPlan<64>  { static constexpr int S = 2, r0 = 8, r1 = 1, T = 8; };
+template <> struct FftPlan<128> { static constexpr int S = 3, r0 = 4, r1 = 8, T = 16; };
+template <> struct FftPlan<256> { static constexpr int S = 3, r0 = 8, r1 = 4, T = 32; };
+template <> struct FftPlan<384> { static constexpr int S = 3, r0 = 8, r1 = 6, T = 48; };
+template <> struct FftPlan<512> { static constexpr int S = 3, r0 = 8, r1 = 8, T = 64; };
+
+// Twiddle multiply for one butterfly: v[t] *= exp(DIR*2*pi*i*t*k/(r*P)),
+// read from the length-N forward table tw[m] = exp(-2*pi*i*m/N).
+template <typename R, int N, int r, int P, int DIR>
+__device__ inline void twiddle(cplx<R>* v, int k, const cplx<R>* __restrict__ tw) {
+  constexpr int step = N / (r * P);
+#pragma unroll
+  for (int t = 1; t < r; ++t) {
+    cplx<R> w = tw[t * k * step];
+    if (DIR > 0) w.im = -w.im;
+    v[t] = cmul(v[t], w);
+  }
+}
+
+template <typename R, int N, int DIR>
+struct Fft {
+  using P = FftPlan<N>;
+  static constexpr int R0 = P::r0;
+  static constexpr int R1 = P::r1;
+  static constexpr int T = P::T;
+  static constexpr int NB = N / R0;
+  static constexpr int IT = (NB + T - 1) / T;
+  static constexpr int NE = IT * R0;  // register elements per lane
+
+  // Element index held in register slot [it][t].
+  __device__ static inline int elem(int lane, int it, int t) { return lane + it * T + t * NB; }
+  __device__ static inline bool valid(int lane, int it) { return (NB % T == 0) || (lane + it * T < NB); }
+
+  // s: this sequence's LDS (N elements, unit stride).  Contains barriers: every
+  // thread of the block must call it the same number of times.
+  __device__ static inline void run(cplx<R> (&v)[IT][R0], cplx<R>* s, int lane,
+                                    const cplx<R>* __restrict__ tw) {
+    if constexpr (P::S == 1) {
+#pragma unroll
+      for (int it = 0; it < IT; ++it)
+        if (valid(lane, it)) Dft<R0, DIR>::run(v[it]);
+    } else {
+      // first stage: registers -> LDS
+      __syncthreads();
+#pragma unroll
+      for (int it = 0; it < IT; ++it) {
+        if (valid(lane, it)) {
+          int i = lane + it * T;
+          Dft<R0, DIR>::run(v[it]);
+#pragma unroll
+          for (int t = 0; t < R0; ++t) s[i * R0 + t] = v[it][t];
+        }
+      }
+      __syncthreads();
+      if constexpr (P::S == 3) {
+        constexpr int NB1 = N / R1;
+        constexpr int IT1 = (NB1 + T - 1) / T;
+        cplx<R> u[IT1][R1];
+#pragma unroll
+        for (int it = 0; it < IT1; ++it) {
+          int i = lane + it * T;
+          if (NB1 % T == 0 || i < NB1) {
+#pragma unroll
+            for (int t = 0; t < R1; ++t) u[it][t] = s[i + t * NB1];
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < IT1; ++it) {
+          int i = lane + it * T;
+          if (NB1 % T == 0 || i < NB1) {
+            int k = i % R0;
+            twiddle<R, N, R1, R0, DIR>(u[it], k, tw);
+            Dft<R1, DIR>::run(u[it]);
+            int j = (i - k) * R1 + k;
+#pragma unroll
+            for (int t = 0; t < R1; ++t) s[j + t * R0] = u[it][t];
+          }
+        }
+        __syncthreads();
+      }
+      // last stage: LDS -> registers, span NB = N/R0, k = i
+#pragma unroll
+      for (int it = 0; it < IT; ++it) {
+        if (valid(lane, it)) {
+          int i = lane + it * T;
+#pragma unroll
+          for (int t = 0; t < R0; ++t) v[it][t] = s[i + t * NB];
+          twiddle<R, N, R0, NB, DIR>(v[it], i, tw);
+          Dft<R0, DIR>::run(v[it]);
+        }
+      }
+    }
+  }
+};
+
+}  // namespace qg

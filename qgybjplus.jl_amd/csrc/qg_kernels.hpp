@@ -1,0 +1,915 @@
+// Fused FFT-pass kernels of the ETD-RK2 step.
+//
+// A 2-D transform of one z-plane is two passes: the Y-pass transforms columns
+// (stride nx) for a tile of S adjacent columns of one level, the X-pass
+// transforms rows (unit stride).  Spectral-space arithmetic (masks, i*k
+// multipliers, Hermitian parts, unpacking, the RHS combine and the ETD update)
+// is fused into the Y-pass that touches the spectral data; physical-space
+// products are fused into the X-pass.  Real physical fields travel as
+// half-spectrum intermediates (columns 0..nx/2 of the y-transformed data).
+//
+// Sign convention: forward = exp(-i k.x), unnormalised; inverse = exp(+i k.x)
+// scaled by 1/(nx*ny) (applied when the spectral input is loaded).
+#pragma once
+
+#include "qg_fft.hpp"
+#include "qg_flags.hpp"
+
+namespace qg {
+
+template <typename R, int N>
+struct Tile {
+  static constexpr int T = FftPlan<N>::T;
+  static constexpr int TARGET = sizeof(R) == 8 ? 256 : 512;
+  static constexpr int S0 = TARGET / T;
+  static constexpr int S = S0 < 1 ? 1 : (S0 > 64 ? 64 : S0);
+  static constexpr int THREADS = S * T;
+  static constexpr int LD = N + 1;
+};
+
+template <typename R>
+__device__ inline cplx<R> czero() { return mk<R>((R)0, (R)0); }
+
+// Hermitian part (X(k) + conj X(-k))/2, pre-scaled by `s`.
+__device__ inline cplx<double> herm(cplx<double> a, cplx<double> am, double s) {
+  return mk<double>(0.5 * s * (a.re + am.re), 0.5 * s * (a.im - am.im));
+}
+
+// ===========================================================================
+// Wave feedback q_w  (compute_qw_complex!, nonlinear.jl:859-944)
+// ===========================================================================
+
+// Y-pass: Bm = mask*B (scaled 1/N) -> T1 = IFFTy(Bm), T3 = IFFTy(i*ky*Bm).
+template <typename R, int NY>
+__global__ void __launch_bounds__((Tile<R, NY>::THREADS))
+k_ya_qw(Geo g, const cplx<double>* __restrict__ B, cplx<R>* __restrict__ T1,
+        cplx<R>* __restrict__ T3, const cplx<R>* __restrict__ tw) {
+  using TL = Tile<R, NY>;
+  using F = Fft<R, NY, +1>;
+  __shared__ cplx<R> lds[TL::S * TL::LD];
+  const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
+  const int x = blockIdx.x * TL::S + s, z = blockIdx.y;
+  const bool ok = x < g.nx;
+  cplx<R>* sl = lds + s * TL::LD;
+  cplx<R> v[F::IT][F::R0], w[F::IT][F::R0];
+#pragma unroll
+  for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < F::R0; ++t) {
+      v[it][t] = czero<R>();
+      w[it][t] = czero<R>();
+      int y = F::elem(lane, it, t);
+      if (ok && F::valid(lane, it)) {
+        size_t a = ((size_t)z * NY + y) * g.nx + x;
+        cplx<double> b = g.mask[y * g.nx + x] ? scale(B[a], g.inv_n) : czero<double>();
+        v[it][t] = cvt<R>(b);
+        w[it][t] = cvt<R>(ik_mul(g.ky[y], b));
+      }
+    }
+  F::run(v, sl, lane, tw);
+  if (T3) F::run(w, sl, lane, tw);  // uniform: T3 == nullptr computes T1 only
+#pragma unroll
+  for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < F::R0; ++t) {
+      int y = F::elem(lane, it, t);
+      if (ok && F::valid(lane, it)) {
+        size_t a = ((size_t)z * NY + y) * g.nx + x;
+        T1[a] = v[it][t];
+        if (T3) T3[a] = w[it][t];
+      }
+    }
+}
+
+// X-pass: B = IFFTx(T1), Bx = IFFTx(i*kx*T1), By = IFFTx(T3);
+// qwr = Re(i/2 (conj(Bx)By - conj(By)Bx)) = Bx.im*By.re - Bx.re*By.im,
+// |B|^2; forward real FFTs along x -> Qa (qwr), Qb (|B|^2), half width.
+template <typename R, int NX>
+__global__ void __launch_bounds__((Tile<R, NX>::THREADS))
+k_xa_qw(Geo g, const cplx<R>* __restrict__ T1, const cplx<R>* __restrict__ T3,
+        cplx<R>* __restrict__ Qa, cplx<R>* __restrict__ Qb, const cplx<R>* __restrict__ tw) {
+  using TL = Tile<R, NX>;
+  using FI = Fft<R, NX, +1>;
+  using FF = Fft<R, NX, -1>;
+  __shared__ cplx<R> lds[TL::S * TL::LD];
+  const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
+  const int row = blockIdx.x * TL::S + r;
+  const bool ok = row < g.nz * g.ny;
+  cplx<R>* sl = lds + r * TL::LD;
+  const size_t base = (size_t)row * NX;
+  cplx<R> v[FI::IT][FI::R0], w[FI::IT][FI::R0];
+  R mag[FI::IT][FI::R0];
+  // B and Bx
+#pragma unroll
+  for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < FI::R0; ++t) {
+      int x = FI::elem(lane, it, t);
+      cplx<R> a = (ok && FI::valid(lane, it)) ? T1[base + x] : czero<R>();
+      v[it][t] = a;
+      w[it][t] = ik_mul((R)g.kx[x], a);
+    }
+  FI::run(v, sl, lane, tw);
+#pragma unroll
+  for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < FI::R0; ++t) mag[it][t] = v[it][t].re * v[it][t].re + v[it][t].im * v[it][t].im;
+  FI::run(w, sl, lane, tw);  // w = Bx
+#pragma unroll
+  for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < FI::R0; ++t) {
+      int x = FI::elem(lane, it, t);
+      v[it][t] = (ok && FI::valid(lane, it)) ? T3[base + x] : czero<R>();
+    }
+  FI::run(v, sl, lane, tw);  // v = By
+#pragma unroll
+  for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < FI::R0; ++t) {
+      R q = w[it][t].im * v[it][t].re - w[it][t].re * v[it][t].im;
+      v[it][t] = mk<R>(q, (R)0);
+      w[it][t] = mk<R>(mag[it][t], (R)0);
+    }
+  FF::run(v, sl, lane, tw);
+  FF::run(w, sl, lane, tw);
+  const size_t hb = (size_t)row * g.Wp;
+#pragma unroll
+  for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < FI::R0; ++t) {
+      int x = FI::elem(lane, it, t);
+      if (ok && FI::valid(lane, it) && x < g.W) {
+        Qa[hb + x] = v[it][t];
+        Qb[hb + x] = w[it][t];
+      }
+    }
+}
+
+// Y-pass (forward, half width): q_w = mask*(F(qwr) - kh2/4 F(|B|^2))/f and
+// q_eff = mask ? q - q_w : 0  (replace_q_with_wave_feedback_rhs!, timestep.jl:122-140).
+template <typename R, int NY>
+__global__ void __launch_bounds__((Tile<R, NY>::THREADS))
+k_ya2_qeff(Geo g, const cplx<R>* __restrict__ Qa, const cplx<R>* __restrict__ Qb,
+           const cplx<double>* __restrict__ q, cplx<double>* __restrict__ qeff,
+           cplx<double>* __restrict__ qw_out, double inv_f, const cplx<R>* __restrict__ tw) {
+  using TL = Tile<R, NY>;
+  using F = Fft<R, NY, -1>;
+  __shared__ cplx<R> lds[TL::S * TL::LD];
+  const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
+  const int x = blockIdx.x * TL::S + s, z = blockIdx.y;
+  const bool ok = x < g.W;
+  cplx<R>* sl = lds + s * TL::LD;
+  cplx<R> v[F::IT][F::R0], w[F::IT][F::R0];
+#pragma unroll
+  for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < F::R0; ++t) {
+      int y = F::elem(lane, it, t);
+      bool in = ok && F::valid(lane, it);
+      size_t a = ((size_t)z * NY + y) * g.Wp + x;
+      v[it][t] = in ? Qa[a] : czero<R>();
+      w[it][t] = in ? Qb[a] : czero<R>();
+    }
+  F::run(v, sl, lane, tw);
+  F::run(w, sl, lane, tw);
+  const int mx = mirror(x, g.nx);
+#pragma unroll
+  for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < F::R0; ++t) {
+      int y = F::elem(lane, it, t);
+      if (!(ok && F::valid(lane, it))) continue;
+      const bool keep = g.mask[y * g.nx + x] != 0;
+      const double kh2 = g.kx[x] * g.kx[x] + g.ky[y] * g.ky[y];
+      cplx<double> fq = cvt<double>(v[it][t]), fm = cvt<double>(w[it][t]);
+      cplx<double> qw = keep ? scale(fq - scale(fm, 0.25 * kh2), inv_f) : czero<double>();
+      size_t a = ((size_t)z * NY + y) * g.nx + x;
+      if (qeff) qeff[a] = keep ? q[a] - qw : czero<double>();
+      if (qw_out) qw_out[a] = qw;
+      if (mx != x) {
+        const int my = mirror(y, NY);
+        size_t am = ((size_t)z * NY + my) * g.nx + mx;
+        cplx<double> qwm = conj(qw);
+        if (qeff) qeff[am] = keep ? q[am] - qwm : czero<double>();
+        if (qw_out) qw_out[am] = qwm;
+      }
+    }
+}
+
+// ===========================================================================
+// Advection and refraction  (nonlinear.jl:414-479, 621-676; operators.jl:195-274)
+// ===========================================================================
+
+// Y-pass (inverse, half width) producing the y-transformed Hermitian parts of
+// u = Re ifft(-i ky psi), v = Re ifft(i kx psi) (unmasked psi), of
+// Re ifft(mask q) and of zeta = Re ifft(mask (-kh2 psi)).
+template <typename R, int NY>
+__global__ void __launch_bounds__((Tile<R, NY>::THREADS))
+k_yb_adv(Geo g, int outs, const cplx<double>* __restrict__ psi, const cplx<double>* __restrict__ q,
+         cplx<R>* __restrict__ Pu, cplx<R>* __restrict__ Pv, cplx<R>* __restrict__ Pq,
+         cplx<R>* __restrict__ Pz, const cplx<R>* __restrict__ tw) {
+  using TL = Tile<R, NY>;
+  using F = Fft<R, NY, +1>;
+  __shared__ cplx<R> lds[TL::S * TL::LD];
+  const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
+  const int x = blockIdx.x * TL::S + s, z = blockIdx.y;
+  const bool ok = x < g.W;
+  const int mx = mirror(x, g.nx);
+  cplx<R>* sl = lds + s * TL::LD;
+  const double kxv = ok ? g.kx[x] : 0.0, kxm = ok ? g.kx[mx] : 0.0;
+  cplx<double> p[F::IT][F::R0], pm[F::IT][F::R0];
+  cplx<R> v[F::IT][F::R0];
+#pragma unroll
+  for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < F::R0; ++t) {
+      int y = F::elem(lane, it, t), my = mirror(y, NY);
+      bool in = ok && F::valid(lane, it);
+      p[it][t] = in ? psi[((size_t)z * NY + y) * g.nx + x] : czero<double>();
+      pm[it][t] = in ? psi[((size_t)z * NY + my) * g.nx + mx] : czero<double>();
+    }
+  for (int o = 0; o < 4; ++o) {
+    const int bit = 1 << o;
+    if (!(outs & bit)) continue;  // uniform across the block
+    cplx<R>* dst = o == 0 ? Pu : o == 1 ? Pv : o == 2 ? Pq : Pz;
+#pragma unroll
+    for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < F::R0; ++t) {
+        int y = F::elem(lane, it, t), my = mirror(y, NY);
+        cplx<double> a, am;
+        if (o == 0) {
+          a = ik_mul(-g.ky[y], p[it][t]);
+          am = ik_mul(-g.ky[my], pm[it][t]);
+        } else if (o == 1) {
+          a = ik_mul(kxv, p[it][t]);
+          am = ik_mul(kxm, pm[it][t]);
+        } else if (o == 2) {
+          bool in = ok && F::valid(lane, it);
+          bool keep = in && g.mask[y * g.nx + x];
+          a = keep ? q[((size_t)z * NY + y) * g.nx + x] : czero<double>();
+          am = keep ? q[((size_t)z * NY + my) * g.nx + mx] : czero<double>();
+        } else {
+          bool in = ok && F::valid(lane, it);
+          bool keep = in && g.mask[y * g.nx + x];
+          double kh2 = kxv * kxv + g.ky[y] * g.ky[y];
+          double kh2m = kxm * kxm + g.ky[my] * g.ky[my];
+          a = keep ? scale(p[it][t], -kh2) : czero<double>();
+          am = keep ? scale(pm[it][t], -kh2m) : czero<double>();
+        }
+        v[it][t] = cvt<R>(herm(a, am, g.inv_n));
+      }
+    F::run(v, sl, lane, tw);
+#pragma unroll
+    for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < F::R0; ++t) {
+        int y = F::elem(lane, it, t);
+        if (ok && F::valid(lane, it)) dst[((size_t)z * NY + y) * g.Wp + x] = v[it][t];
+      }
+  }
+}
+
+// Load one half-width row as a full Hermitian row (element x >= W is
+// conj(element nx-x)).
+template <typename R, class F>
+__device__ inline void load_herm_row(cplx<R> (&v)[F::IT][F::R0], const cplx<R>* __restrict__ src,
+                                     size_t hb, int nx, int lane, bool ok) {
+  const int W = nx / 2 + 1;
+#pragma unroll
+  for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < F::R0; ++t) {
+      int x = F::elem(lane, it, t);
+      cplx<R> a = czero<R>();
+      if (ok && F::valid(lane, it)) a = x < W ? src[hb + x] : conj(src[hb + (nx - x)]);
+      v[it][t] = a;
+    }
+}
+
+// X-pass: real rows from half-width intermediates (C2R), written to `outs`.
+// Used for u, v (velocities) and w.
+template <typename R, int NX>
+__global__ void __launch_bounds__((Tile<R, NX>::THREADS))
+k_x_c2r(Geo g, int nfields, const cplx<R>* __restrict__ P0, const cplx<R>* __restrict__ P1,
+        R* __restrict__ o0, R* __restrict__ o1, const cplx<R>* __restrict__ tw) {
+  using TL = Tile<R, NX>;
+  using FI = Fft<R, NX, +1>;
+  __shared__ cplx<R> lds[TL::S * TL::LD];
+  const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
+  const int row = blockIdx.x * TL::S + r;
+  const bool ok = row < g.nz * g.ny;
+  cplx<R>* sl = lds + r * TL::LD;
+  cplx<R> v[FI::IT][FI::R0];
+  for (int f = 0; f < nfields; ++f) {
+    const cplx<R>* src = f == 0 ? P0 : P1;
+    R* dst = f == 0 ? o0 : o1;
+    load_herm_row<R, FI>(v, src, (size_t)row * g.Wp, NX, lane, ok);
+    FI::run(v, sl, lane, tw);
+#pragma unroll
+    for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < FI::R0; ++t) {
+        int x = FI::elem(lane, it, t);
+        if (ok && FI::valid(lane, it)) dst[(size_t)row * NX + x] = v[it][t].re;
+      }
+  }
+}
+
+// X-pass for q advection: q = C2R(Pq); u*q, v*q -> forward real FFTs -> G1, G2.
+template <typename R, int NX>
+__global__ void __launch_bounds__((Tile<R, NX>::THREADS))
+k_xb_q(Geo g, const cplx<R>* __restrict__ Pq, const R* __restrict__ u, const R* __restrict__ vv,
+       cplx<R>* __restrict__ G1, cplx<R>* __restrict__ G2, const cplx<R>* __restrict__ tw) {
+  using TL = Tile<R, NX>;
+  using FI = Fft<R, NX, +1>;
+  using FF = Fft<R, NX, -1>;
+  __shared__ cplx<R> lds[TL::S * TL::LD];
+  const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
+  const int row = blockIdx.x * TL::S + r;
+  const bool ok = row < g.nz * g.ny;
+  cplx<R>* sl = lds + r * TL::LD;
+  cplx<R> v[FI::IT][FI::R0];
+  R qv[FI::IT][FI::R0];
+  load_herm_row<R, FI>(v, Pq, (size_t)row * g.Wp, NX, lane, ok);
+  FI::run(v, sl, lane, tw);
+#pragma unroll
+  for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < FI::R0; ++t) qv[it][t] = v[it][t].re;
+  const size_t hb = (size_t)row * g.Wp;
+  for (int f = 0; f < 2; ++f) {
+    const R* vel = f == 0 ? u : vv;
+    cplx<R>* dst = f == 0 ? G1 : G2;
+#pragma unroll
+    for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < FI::R0; ++t) {
+        int x = FI::elem(lane, it, t);
+        R c = (ok && FI::valid(lane, it)) ? vel[(size_t)row * NX + x] : (R)0;
+        v[it][t] = mk<R>(c * qv[it][t], (R)0);
+      }
+    FF::run(v, sl, lane, tw);
+#pragma unroll
+    for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < FI::R0; ++t) {
+        int x = FI::elem(lane, it, t);
+        if (ok && FI::valid(lane, it) && x < g.W) dst[hb + x] = v[it][t];
+      }
+  }
+}
+
+// X-pass for B: B = IFFTx(T1), zeta = C2R(Pz); u*B, v*B, zeta*B -> forward
+// complex FFTs -> G3, G4, G5 (full width).
+template <typename R, int NX>
+__global__ void __launch_bounds__((Tile<R, NX>::THREADS))
+k_xb_B(Geo g, int flags, const cplx<R>* __restrict__ T1, const cplx<R>* __restrict__ Pz,
+       const R* __restrict__ u, const R* __restrict__ vv, cplx<R>* __restrict__ G3,
+       cplx<R>* __restrict__ G4, cplx<R>* __restrict__ G5, const cplx<R>* __restrict__ tw) {
+  using TL = Tile<R, NX>;
+  using FI = Fft<R, NX, +1>;
+  using FF = Fft<R, NX, -1>;
+  __shared__ cplx<R> lds[TL::S * TL::LD];
+  const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
+  const int row = blockIdx.x * TL::S + r;
+  const bool ok = row < g.nz * g.ny;
+  cplx<R>* sl = lds + r * TL::LD;
+  const size_t base = (size_t)row * NX;
+  cplx<R> b[FI::IT][FI::R0], v[FI::IT][FI::R0];
+  R zt[FI::IT][FI::R0];
+#pragma unroll
+  for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < FI::R0; ++t) {
+      int x = FI::elem(lane, it, t);
+      b[it][t] = (ok && FI::valid(lane, it)) ? T1[base + x] : czero<R>();
+    }
+  FI::run(b, sl, lane, tw);
+  if (flags & XB_REFR) {
+    load_herm_row<R, FI>(v, Pz, (size_t)row * g.Wp, NX, lane, ok);
+    FI::run(v, sl, lane, tw);
+#pragma unroll
+    for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < FI::R0; ++t) zt[it][t] = v[it][t].re;
+  }
+  for (int f = 0; f < 3; ++f) {
+    if (f < 2 && !(flags & XB_ADV)) continue;
+    if (f == 2 && !(flags & XB_REFR)) continue;
+    cplx<R>* dst = f == 0 ? G3 : f == 1 ? G4 : G5;
+#pragma unroll
+    for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < FI::R0; ++t) {
+        int x = FI::elem(lane, it, t);
+        R c = (R)0;
+        if (f == 2) c = zt[it][t];
+        else if (ok && FI::valid(lane, it)) c = (f == 0 ? u : vv)[base + x];
+        v[it][t] = mk<R>(c * b[it][t].re, c * b[it][t].im);
+      }
+    FF::run(v, sl, lane, tw);
+#pragma unroll
+    for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < FI::R0; ++t) {
+        int x = FI::elem(lane, it, t);
+        if (ok && FI::valid(lane, it)) dst[base + x] = v[it][t];
+      }
+  }
+}
+
+// ETD tables per column, layout [6][ny][nx]: Eq, hphi1q, hphi2q, EB, hphi1B, hphi2B.
+struct Etd {
+  const double* tab;
+  size_t plane;  // nx*ny
+  __device__ inline double at(int which, int col) const { return tab[which * plane + col]; }
+};
+
+// Vertical diffusion nu_z * d2q/dz2 with Neumann ends (dissipation_q_nv!, nonlinear.jl:1010-1044).
+__device__ inline cplx<double> vdiff(const cplx<double>* __restrict__ q, int z, int nz, size_t col,
+                                     size_t plane, double nu, double inv_dz2) {
+  if (nz <= 1) return czero<double>();
+  cplx<double> c = q[(size_t)z * plane + col];
+  cplx<double> r;
+  if (z == 0) {
+    cplx<double> up = q[plane + col];
+    r = up - c;
+  } else if (z == nz - 1) {
+    cplx<double> dn = q[(size_t)(z - 1) * plane + col];
+    r = dn - c;
+  } else {
+    cplx<double> up = q[(size_t)(z + 1) * plane + col], dn = q[(size_t)(z - 1) * plane + col];
+    r = mk<double>(up.re - 2 * c.re + dn.re, up.im - 2 * c.im + dn.im);
+  }
+  return scale(scale(r, nu), inv_dz2);
+}
+
+// Y-pass (forward, half width): nq = mask*(i kx F(uq) + i ky F(vq)); rhs_q and
+// the ETD stage/final update of q (timestep.jl:286-295, 362-404).
+template <typename R, int NY>
+__global__ void __launch_bounds__((Tile<R, NY>::THREADS))
+k_yb2_q(Geo g, int mode, int flags, const cplx<R>* __restrict__ G1, const cplx<R>* __restrict__ G2,
+        const cplx<double>* __restrict__ qd, const cplx<double>* __restrict__ qn,
+        cplx<double>* __restrict__ rhs0,
+        cplx<double>* __restrict__ qout, Etd etd, double nu, double inv_dz2,
+        const cplx<R>* __restrict__ tw) {
+  using TL = Tile<R, NY>;
+  using F = Fft<R, NY, -1>;
+  __shared__ cplx<R> lds[TL::S * TL::LD];
+  const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
+  const int x = blockIdx.x * TL::S + s, z = blockIdx.y;
+  const bool ok = x < g.W;
+  cplx<R>* sl = lds + s * TL::LD;
+  cplx<R> a[F::IT][F::R0], b[F::IT][F::R0];
+  if (flags & YQ_ADV) {  // uniform
+#pragma unroll
+    for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < F::R0; ++t) {
+        int y = F::elem(lane, it, t);
+        bool in = ok && F::valid(lane, it);
+        size_t ad = ((size_t)z * NY + y) * g.Wp + x;
+        a[it][t] = in ? G1[ad] : czero<R>();
+        b[it][t] = in ? G2[ad] : czero<R>();
+      }
+    F::run(a, sl, lane, tw);
+    F::run(b, sl, lane, tw);
+  }
+  const int mx = mirror(x, g.nx);
+  const size_t plane = (size_t)g.nx * NY;
+#pragma unroll
+  for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < F::R0; ++t) {
+      int y = F::elem(lane, it, t);
+      if (!(ok && F::valid(lane, it))) continue;
+      const bool keep = g.mask[y * g.nx + x] != 0;
+      cplx<double> nq = czero<double>();
+      if ((flags & YQ_ADV) && keep)
+        nq = ik_mul(g.kx[x], cvt<double>(a[it][t])) + ik_mul(g.ky[y], cvt<double>(b[it][t]));
+      const int my = mirror(y, NY);
+      for (int side = 0; side < 2; ++side) {
+        if (side == 1 && mx == x) break;
+        const int cx = side ? mx : x, cy = side ? my : y;
+        const size_t col = (size_t)cy * g.nx + cx;
+        const size_t ad = (size_t)z * plane + col;
+        cplx<double> n = side ? conj(nq) : nq;
+        cplx<double> q0 = qn[ad];
+        cplx<double> rhs = czero<double>();
+        if (keep && !(flags & YQ_FIXED)) {
+          cplx<double> dq = (flags & YQ_DIFF) ? vdiff(qd, z, g.nz, col, plane, nu, inv_dz2) : czero<double>();
+          rhs = mk<double>(-n.re + dq.re, -n.im + dq.im);
+        }
+        cplx<double> out = czero<double>();
+        if (keep) {
+          if (flags & YQ_FIXED) {
+            out = q0;
+          } else {
+            const double E = etd.at(0, col), h1 = etd.at(1, col);
+            if (mode == MODE_STAGE) {
+              out = scale(q0, E) + scale(rhs, h1);
+            } else {
+              const double h2 = etd.at(2, col);
+              cplx<double> r0 = rhs0[ad];
+              out = (scale(q0, E) + scale(r0, h1)) + scale(rhs - r0, h2);
+            }
+          }
+        }
+        if (mode == MODE_STAGE) rhs0[ad] = rhs;
+        qout[ad] = out;
+      }
+    }
+}
+
+// Y-pass (forward, full width): nB = mask*(i kx F(uB) + i ky F(vB)), rB =
+// mask*F(zeta B); rhs_B = -nB + i (f/2) kh2 A - (i/2) rB; ETD update of B.
+template <typename R, int NY>
+__global__ void __launch_bounds__((Tile<R, NY>::THREADS))
+k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R>* __restrict__ G4,
+        const cplx<R>* __restrict__ G5, const cplx<double>* __restrict__ A,
+        const cplx<double>* __restrict__ Bn, cplx<double>* __restrict__ rhs0,
+        cplx<double>* __restrict__ Bout, Etd etd, double alpha, const cplx<R>* __restrict__ tw) {
+  using TL = Tile<R, NY>;
+  using F = Fft<R, NY, -1>;
+  __shared__ cplx<R> lds[TL::S * TL::LD];
+  const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
+  const int x = blockIdx.x * TL::S + s, z = blockIdx.y;
+  const bool ok = x < g.nx;
+  cplx<R>* sl = lds + s * TL::LD;
+  cplx<double> acc[F::IT][F::R0];
+  cplx<R> v[F::IT][F::R0];
+#pragma unroll
+  for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < F::R0; ++t) acc[it][t] = czero<double>();
+  const double kxv = ok ? g.kx[x] : 0.0;
+  for (int f = 0; f < 3; ++f) {
+    if (f < 2 && !(flags & YB_ADV)) continue;
+    if (f == 2 && !(flags & YB_REFR)) continue;
+    const cplx<R>* src = f == 0 ? G3 : f == 1 ? G4 : G5;
+#pragma unroll
+    for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < F::R0; ++t) {
+        int y = F::elem(lane, it, t);
+        bool in = ok && F::valid(lane, it);
+        v[it][t] = in ? src[((size_t)z * NY + y) * g.nx + x] : czero<R>();
+      }
+    F::run(v, sl, lane, tw);
+    if (f == 2) break;  // v keeps F(zeta*B) for the combine below
+#pragma unroll
+    for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < F::R0; ++t) {
+        int y = F::elem(lane, it, t);
+        cplx<double> c = cvt<double>(v[it][t]);
+        if (f == 0) acc[it][t] = ik_mul(kxv, c);                      // i kx F(uB)
+        else acc[it][t] = acc[it][t] + ik_mul(g.ky[y], c);           // + i ky F(vB)
+      }
+  }
+  const size_t plane = (size_t)g.nx * NY;
+#pragma unroll
+  for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < F::R0; ++t) {
+      int y = F::elem(lane, it, t);
+      if (!(ok && F::valid(lane, it))) continue;
+      const bool keep = g.mask[y * g.nx + x] != 0;
+      const size_t col = (size_t)y * g.nx + x;
+      const size_t ad = (size_t)z * plane + col;
+      cplx<double> rhs = czero<double>();
+      if (keep) {
+        cplx<double> nB = (flags & YB_ADV) ? acc[it][t] : czero<double>();
+        cplx<double> rB = (flags & YB_REFR) ? cvt<double>(v[it][t]) : czero<double>();
+        cplx<double> t1 = czero<double>();
+        if (flags & YB_DISP) {
+          const double kh2 = kxv * kxv + g.ky[y] * g.ky[y];
+          const double c = alpha * kh2;
+          cplx<double> Av = A[ad];
+          t1 = mk<double>(-(c * Av.im), c * Av.re);
+        }
+        cplx<double> t2 = mk<double>(-(0.5 * rB.im), 0.5 * rB.re);
+        rhs = mk<double>(-nB.re + t1.re, -nB.im + t1.im) - t2;
+      }
+      cplx<double> out = czero<double>();
+      if (keep) {
+        const double E = etd.at(3, col), h1 = etd.at(4, col);
+        cplx<double> b0 = Bn[ad];
+        if (mode == MODE_STAGE) {
+          out = scale(b0, E) + scale(rhs, h1);
+        } else {
+          const double h2 = etd.at(5, col);
+          cplx<double> r0 = rhs0[ad];
+          out = (scale(b0, E) + scale(r0, h1)) + scale(rhs - r0, h2);
+        }
+      }
+      if (mode == MODE_STAGE) rhs0[ad] = rhs;
+      Bout[ad] = out;
+    }
+}
+
+// ===========================================================================
+// Finalize: velocities and the QG omega equation (operators.jl:195-274, 365-493;
+// diagnostics.jl:149-229)
+// ===========================================================================
+
+// Y-pass (inverse, half width) of u, v and the four omega-RHS factors.
+template <typename R, int NY>
+__global__ void __launch_bounds__((Tile<R, NY>::THREADS))
+k_yf(Geo g, int want_omega, const cplx<double>* __restrict__ psi, cplx<R>* __restrict__ Pu,
+     cplx<R>* __restrict__ Pv, cplx<R>* __restrict__ Pbx, cplx<R>* __restrict__ Pby,
+     cplx<R>* __restrict__ Pxx, cplx<R>* __restrict__ Pxy, const cplx<R>* __restrict__ tw) {
+  using TL = Tile<R, NY>;
+  using F = Fft<R, NY, +1>;
+  __shared__ cplx<R> lds[TL::S * TL::LD];
+  const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
+  const int x = blockIdx.x * TL::S + s, z = blockIdx.y;
+  const bool ok = x < g.W;
+  const int mx = mirror(x, g.nx);
+  cplx<R>* sl = lds + s * TL::LD;
+  const double kxv = ok ? g.kx[x] : 0.0, kxm = ok ? g.kx[mx] : 0.0;
+  const bool top = z == g.nz - 1;
+  const double dz = g.dz;
+  cplx<double> p[F::IT][F::R0], pm[F::IT][F::R0];
+  cplx<R> v[F::IT][F::R0];
+#pragma unroll
+  for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < F::R0; ++t) {
+      int y = F::elem(lane, it, t), my = mirror(y, NY);
+      bool in = ok && F::valid(lane, it);
+      p[it][t] = in ? psi[((size_t)z * NY + y) * g.nx + x] : czero<double>();
+      pm[it][t] = in ? psi[((size_t)z * NY + my) * g.nx + mx] : czero<double>();
+    }
+  const int nout = want_omega ? 6 : 2;
+  for (int o = 0; o < nout; ++o) {
+    cplx<R>* dst = o == 0 ? Pu : o == 1 ? Pv : o == 2 ? Pbx : o == 3 ? Pby : o == 4 ? Pxx : Pxy;
+#pragma unroll
+    for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < F::R0; ++t) {
+        int y = F::elem(lane, it, t), my = mirror(y, NY);
+        bool in = ok && F::valid(lane, it);
+        cplx<double> a, am;
+        if (o == 0) {
+          a = ik_mul(-g.ky[y], p[it][t]);
+          am = ik_mul(-g.ky[my], pm[it][t]);
+        } else if (o == 1) {
+          a = ik_mul(kxv, p[it][t]);
+          am = ik_mul(kxm, pm[it][t]);
+        } else {
+          cplx<double> pz = czero<double>(), pzm = czero<double>(), pa = p[it][t], pam = pm[it][t];
+          if (!top && in) {
+            cplx<double> up = psi[((size_t)(z + 1) * NY + y) * g.nx + x];
+            cplx<double> upm = psi[((size_t)(z + 1) * NY + my) * g.nx + mx];
+            pz = mk<double>((up.re - p[it][t].re) / dz, (up.im - p[it][t].im) / dz);
+            pzm = mk<double>((upm.re - pm[it][t].re) / dz, (upm.im - pm[it][t].im) / dz);
+            pa = scale(up + p[it][t], 0.5);
+            pam = scale(upm + pm[it][t], 0.5);
+          }
+          const double ky = g.ky[y], kym = g.ky[my];
+          const double kh2 = kxv * kxv + ky * ky, kh2m = kxm * kxm + kym * kym;
+          if (o == 2) {
+            a = ik_mul(kxv, pz);
+            am = ik_mul(kxm, pzm);
+          } else if (o == 3) {
+            a = ik_mul(ky, pz);
+            am = ik_mul(kym, pzm);
+          } else if (o == 4) {
+            a = ik_mul(-(kxv * kh2), pa);
+            am = ik_mul(-(kxm * kh2m), pam);
+          } else {
+            a = ik_mul(-(ky * kh2), pa);
+            am = ik_mul(-(kym * kh2m), pam);
+          }
+        }
+        v[it][t] = cvt<R>(herm(a, am, g.inv_n));
+      }
+    F::run(v, sl, lane, tw);
+#pragma unroll
+    for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < F::R0; ++t) {
+        int y = F::elem(lane, it, t);
+        if (ok && F::valid(lane, it)) dst[((size_t)z * NY + y) * g.Wp + x] = v[it][t];
+      }
+  }
+}
+
+// X-pass: u, v out; omega RHS 2(bx*xy - by*xx) -> forward real FFT -> Rw (half).
+template <typename R, int NX>
+__global__ void __launch_bounds__((Tile<R, NX>::THREADS))
+k_xf(Geo g, int want_omega, const cplx<R>* __restrict__ Pu, const cplx<R>* __restrict__ Pv,
+     const cplx<R>* __restrict__ Pbx, const cplx<R>* __restrict__ Pby, const cplx<R>* __restrict__ Pxx,
+     const cplx<R>* __restrict__ Pxy, R* __restrict__ u, R* __restrict__ vv, cplx<R>* __restrict__ Rw,
+     const cplx<R>* __restrict__ tw) {
+  using TL = Tile<R, NX>;
+  using FI = Fft<R, NX, +1>;
+  using FF = Fft<R, NX, -1>;
+  __shared__ cplx<R> lds[TL::S * TL::LD];
+  const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
+  const int row = blockIdx.x * TL::S + r;
+  const bool ok = row < g.nz * g.ny;
+  cplx<R>* sl = lds + r * TL::LD;
+  const size_t hb = (size_t)row * g.Wp;
+  cplx<R> v[FI::IT][FI::R0];
+  for (int f = 0; f < 2; ++f) {
+    load_herm_row<R, FI>(v, f == 0 ? Pu : Pv, hb, NX, lane, ok);
+    FI::run(v, sl, lane, tw);
+    R* dst = f == 0 ? u : vv;
+#pragma unroll
+    for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < FI::R0; ++t) {
+        int x = FI::elem(lane, it, t);
+        if (ok && FI::valid(lane, it)) dst[(size_t)row * NX + x] = v[it][t].re;
+      }
+  }
+  if (!want_omega) return;  // uniform
+  R p1[FI::IT][FI::R0], tmp[FI::IT][FI::R0];
+  load_herm_row<R, FI>(v, Pbx, hb, NX, lane, ok);
+  FI::run(v, sl, lane, tw);
+#pragma unroll
+  for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < FI::R0; ++t) tmp[it][t] = v[it][t].re;
+  load_herm_row<R, FI>(v, Pxy, hb, NX, lane, ok);
+  FI::run(v, sl, lane, tw);
+#pragma unroll
+  for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < FI::R0; ++t) p1[it][t] = tmp[it][t] * v[it][t].re;
+  load_herm_row<R, FI>(v, Pby, hb, NX, lane, ok);
+  FI::run(v, sl, lane, tw);
+#pragma unroll
+  for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < FI::R0; ++t) tmp[it][t] = v[it][t].re;
+  load_herm_row<R, FI>(v, Pxx, hb, NX, lane, ok);
+  FI::run(v, sl, lane, tw);
+#pragma unroll
+  for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < FI::R0; ++t) v[it][t] = mk<R>((R)2 * (p1[it][t] - tmp[it][t] * v[it][t].re), (R)0);
+  FF::run(v, sl, lane, tw);
+#pragma unroll
+  for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < FI::R0; ++t) {
+      int x = FI::elem(lane, it, t);
+      if (ok && FI::valid(lane, it) && x < g.W) Rw[hb + x] = v[it][t];
+    }
+}
+
+// Y-pass (forward, half width) of a real field, masked, expanded to a full
+// Hermitian spectrum in double precision.
+template <typename R, int NY>
+__global__ void __launch_bounds__((Tile<R, NY>::THREADS))
+k_y_r2c_mask(Geo g, const cplx<R>* __restrict__ Rw, cplx<double>* __restrict__ out,
+             const cplx<R>* __restrict__ tw) {
+  using TL = Tile<R, NY>;
+  using F = Fft<R, NY, -1>;
+  __shared__ cplx<R> lds[TL::S * TL::LD];
+  const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
+  const int x = blockIdx.x * TL::S + s, z = blockIdx.y;
+  const bool ok = x < g.W;
+  cplx<R>* sl = lds + s * TL::LD;
+  cplx<R> v[F::IT][F::R0];
+#pragma unroll
+  for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < F::R0; ++t) {
+      int y = F::elem(lane, it, t);
+      v[it][t] = (ok && F::valid(lane, it)) ? Rw[((size_t)z * NY + y) * g.Wp + x] : czero<R>();
+    }
+  F::run(v, sl, lane, tw);
+  const int mx = mirror(x, g.nx);
+#pragma unroll
+  for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < F::R0; ++t) {
+      int y = F::elem(lane, it, t);
+      if (!(ok && F::valid(lane, it))) continue;
+      const bool keep = g.mask[y * g.nx + x] != 0;
+      cplx<double> c = keep ? cvt<double>(v[it][t]) : czero<double>();
+      out[((size_t)z * NY + y) * g.nx + x] = c;
+      if (mx != x) out[((size_t)z * NY + mirror(y, NY)) * g.nx + mx] = conj(c);
+    }
+}
+
+// Y-pass (inverse, half width): Hermitian part of a double spectrum.
+template <typename R, int NY>
+__global__ void __launch_bounds__((Tile<R, NY>::THREADS))
+k_y_herm(Geo g, const cplx<double>* __restrict__ in, cplx<R>* __restrict__ P,
+         const cplx<R>* __restrict__ tw) {
+  using TL = Tile<R, NY>;
+  using F = Fft<R, NY, +1>;
+  __shared__ cplx<R> lds[TL::S * TL::LD];
+  const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
+  const int x = blockIdx.x * TL::S + s, z = blockIdx.y;
+  const bool ok = x < g.W;
+  const int mx = mirror(x, g.nx);
+  cplx<R>* sl = lds + s * TL::LD;
+  cplx<R> v[F::IT][F::R0];
+#pragma unroll
+  for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < F::R0; ++t) {
+      int y = F::elem(lane, it, t);
+      cplx<double> a = czero<double>(), am = czero<double>();
+      if (ok && F::valid(lane, it)) {
+        a = in[((size_t)z * NY + y) * g.nx + x];
+        am = in[((size_t)z * NY + mirror(y, NY)) * g.nx + mx];
+      }
+      v[it][t] = cvt<R>(herm(a, am, g.inv_n));
+    }
+  F::run(v, sl, lane, tw);
+#pragma unroll
+  for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < F::R0; ++t) {
+      int y = F::elem(lane, it, t);
+      if (ok && F::valid(lane, it)) P[((size_t)z * NY + y) * g.Wp + x] = v[it][t];
+    }
+}
+
+// ===========================================================================
+// Plain complex transforms (set!/diagnostics/check): double in and out.
+// ===========================================================================
+
+// Y-pass C2C of full double fields; inverse applies 1/(nx*ny).
+template <typename R, int NY, int DIR>
+__global__ void __launch_bounds__((Tile<R, NY>::THREADS))
+k_y_c2c(Geo g, const cplx<double>* __restrict__ in, cplx<double>* __restrict__ out,
+        const cplx<R>* __restrict__ tw) {
+  using TL = Tile<R, NY>;
+  using F = Fft<R, NY, DIR>;
+  __shared__ cplx<R> lds[TL::S * TL::LD];
+  const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
+  const int x = blockIdx.x * TL::S + s, z = blockIdx.y;
+  const bool ok = x < g.nx;
+  cplx<R>* sl = lds + s * TL::LD;
+  cplx<R> v[F::IT][F::R0];
+  const double sc = DIR > 0 ? g.inv_n : 1.0;
+#pragma unroll
+  for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < F::R0; ++t) {
+      int y = F::elem(lane, it, t);
+      v[it][t] = (ok && F::valid(lane, it)) ? cvt<R>(scale(in[((size_t)z * NY + y) * g.nx + x], sc))
+                                            : czero<R>();
+    }
+  F::run(v, sl, lane, tw);
+#pragma unroll
+  for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < F::R0; ++t) {
+      int y = F::elem(lane, it, t);
+      if (ok && F::valid(lane, it)) out[((size_t)z * NY + y) * g.nx + x] = cvt<double>(v[it][t]);
+    }
+}
+
+// X-pass C2C of full double fields.  If `absmax` is set, also folds max|out|
+// into it (non-negative doubles compare like their bit patterns).
+template <typename R, int NX, int DIR>
+__global__ void __launch_bounds__((Tile<R, NX>::THREADS))
+k_x_c2c(Geo g, const cplx<double>* __restrict__ in, cplx<double>* __restrict__ out,
+        unsigned long long* __restrict__ absmax, const cplx<R>* __restrict__ tw) {
+  using TL = Tile<R, NX>;
+  using F = Fft<R, NX, DIR>;
+  __shared__ cplx<R> lds[TL::S * TL::LD];
+  const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
+  const int row = blockIdx.x * TL::S + r;
+  const bool ok = row < g.nz * g.ny;
+  cplx<R>* sl = lds + r * TL::LD;
+  cplx<R> v[F::IT][F::R0];
+#pragma unroll
+  for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < F::R0; ++t) {
+      int x = F::elem(lane, it, t);
+      v[it][t] = (ok && F::valid(lane, it)) ? cvt<R>(in[(size_t)row * NX + x]) : czero<R>();
+    }
+  F::run(v, sl, lane, tw);
+  double m = 0.0;
+#pragma unroll
+  for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < F::R0; ++t) {
+      int x = F::elem(lane, it, t);
+      if (ok && F::valid(lane, it)) {
+        cplx<double> c = cvt<double>(v[it][t]);
+        if (out) out[(size_t)row * NX + x] = c;
+        double a = hypot(c.re, c.im);
+        m = (a > m || a != a) ? a : m;
+      }
+    }
+  if (absmax) {
+    if (m != m) m = __builtin_huge_val();
+    atomicMax(absmax, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+}  // namespace qg

@@ -1,0 +1,50 @@
+// Host launchers of the fused FFT-pass kernels, one instantiation per
+// intermediate precision R (double: plain FP64 mode, float: mixed mode).
+#pragma once
+
+#include "qg_common.hpp"
+#include "qg_flags.hpp"
+
+namespace qg {
+
+struct EtdView {
+  const double* tab;
+  size_t plane;
+};
+
+bool fft_size_supported(int n);
+
+template <typename R>
+struct Ops {
+  using C = cplx<R>;
+  using D = cplx<double>;
+  static void ya_qw(const Geo& g, hipStream_t st, const D* B, C* T1, C* T3, const C* twy);
+  static void xa_qw(const Geo& g, hipStream_t st, const C* T1, const C* T3, C* Qa, C* Qb, const C* twx);
+  static void ya2_qeff(const Geo& g, hipStream_t st, const C* Qa, const C* Qb, const D* q, D* qeff, D* qw,
+                       double inv_f, const C* twy);
+  static void yb_adv(const Geo& g, hipStream_t st, int outs, const D* psi, const D* q, C* Pu, C* Pv, C* Pq,
+                     C* Pz, const C* twy);
+  static void x_c2r(const Geo& g, hipStream_t st, int nfields, const C* P0, const C* P1, R* o0, R* o1,
+                    const C* twx);
+  static void xb_q(const Geo& g, hipStream_t st, const C* Pq, const R* u, const R* v, C* G1, C* G2,
+                   const C* twx);
+  static void xb_B(const Geo& g, hipStream_t st, int flags, const C* T1, const C* Pz, const R* u, const R* v,
+                   C* G3, C* G4, C* G5, const C* twx);
+  static void yb2_q(const Geo& g, hipStream_t st, int mode, int flags, const C* G1, const C* G2, const D* qd,
+                    const D* qn,
+                    D* rhs0, D* qout, EtdView etd, double nu, double inv_dz2, const C* twy);
+  static void yb2_B(const Geo& g, hipStream_t st, int mode, int flags, const C* G3, const C* G4, const C* G5,
+                    const D* A, const D* Bn, D* rhs0, D* Bout, EtdView etd, double alpha, const C* twy);
+  static void yf(const Geo& g, hipStream_t st, int want_omega, const D* psi, C* Pu, C* Pv, C* Pbx, C* Pby,
+                 C* Pxx, C* Pxy, const C* twy);
+  static void xf(const Geo& g, hipStream_t st, int want_omega, const C* Pu, const C* Pv, const C* Pbx,
+                 const C* Pby, const C* Pxx, const C* Pxy, R* u, R* v, C* Rw, const C* twx);
+  static void y_r2c_mask(const Geo& g, hipStream_t st, const C* Rw, D* out, const C* twy);
+  static void y_herm(const Geo& g, hipStream_t st, const D* in, C* P, const C* twy);
+  // plain complex 2-D transforms of double fields (dir = -1 forward, +1 inverse)
+  static void y_c2c(const Geo& g, hipStream_t st, int dir, const D* in, D* out, const C* twy);
+  static void x_c2c(const Geo& g, hipStream_t st, int dir, const D* in, D* out, unsigned long long* absmax,
+                    const C* twx);
+};
+
+}  // namespace qg

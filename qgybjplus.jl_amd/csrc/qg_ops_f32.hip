@@ -1,0 +1,7 @@
+// FP32 instantiation of the fused FFT-pass kernels (mixed mode: FP32
+// transforms and products, FP64 state, column solves and ETD coefficients).
+#include "qg_ops_impl.hpp"
+
+namespace qg {
+template struct Ops<float>;
+}  // namespace qg

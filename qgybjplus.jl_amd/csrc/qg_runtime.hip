@@ -1,0 +1,858 @@
+// Host runtime of the QG-YBJ+ library: model handle, HBM buffers, the
+// ETD-RK2 step orchestration and the C ABI declared in include/qgybj.h.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/qgybj.h"
+#include "qg_columns.hpp"
+#include "qg_ops.hpp"
+
+using qg::cplx;
+using D = cplx<double>;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct QgError : std::runtime_error {
+  int code;
+  QgError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHECK(expr)                                                                 \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      throw QgError(QG_ERR_DEVICE, std::string(#expr " failed: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+}  // namespace
+
+struct qg_handle {
+  int nx = 0, ny = 0, nz = 0, precision = 0;
+  double f = 0, dz = 1, nu_z = 0;
+  bool fixed_flow = false, feedback = false, passive = false, inviscid = false, linear = false,
+       no_disp = false;
+  std::vector<double> kx, ky, N2, a, fc, wc;
+  std::vector<int> fo, wo;
+  hipStream_t st = nullptr;
+  qg::Geo g{};
+  size_t plane = 0, npts = 0, nhalf = 0;
+  std::vector<void*> allocs;
+  // device
+  double *d_kx = nullptr, *d_ky = nullptr, *d_a = nullptr, *d_cz = nullptr, *d_fN2 = nullptr;
+  unsigned char* d_mask = nullptr;
+  double* d_etd = nullptr;
+  double etd_dt = std::nan("");
+  D *q[2] = {nullptr, nullptr}, *B[2] = {nullptr, nullptr};
+  int cur = 0;
+  D *psi = nullptr, *A = nullptr, *C = nullptr, *qs = nullptr, *Bs = nullptr, *psis = nullptr,
+    *As = nullptr, *rq0 = nullptr, *rB0 = nullptr, *qeff = nullptr, *stg = nullptr;
+  double *s0 = nullptr, *s1 = nullptr, *s2 = nullptr;
+  void *u = nullptr, *v = nullptr, *w = nullptr;
+  void *T1 = nullptr, *T3 = nullptr, *G3 = nullptr, *G4 = nullptr, *G5 = nullptr;
+  void *Qa = nullptr, *Qb = nullptr, *Pu = nullptr, *Pv = nullptr, *Pq = nullptr, *Pz = nullptr,
+       *G1 = nullptr, *G2 = nullptr;
+  void *twx = nullptr, *twy = nullptr;  // intermediate precision
+  D *tw64x = nullptr, *tw64y = nullptr;
+  int* d_err = nullptr;
+  unsigned long long* d_warn = nullptr;
+  unsigned long long* d_absmax = nullptr;
+  int* d_flag = nullptr;
+  bool diag_valid = false, t1_valid = false;
+  std::string err;
+  // profiling: bit 1 = events around every kernel, bit 2 = one event per step
+  struct ProfRec {
+    const char* name;
+    double bytes;
+    hipEvent_t a, b;
+  };
+  int prof_mode = 0;
+  std::vector<ProfRec> prof;
+  std::vector<hipEvent_t> step_events;
+  std::vector<hipEvent_t> pool;
+  size_t pool_used = 0;
+  hipEvent_t event() {
+    if (pool_used == pool.size()) {
+      hipEvent_t e;
+      HIPCHECK(hipEventCreate(&e));
+      pool.push_back(e);
+    }
+    return pool[pool_used++];
+  }
+
+  template <typename T>
+  T* alloc(size_t count) {
+    void* p = nullptr;
+    HIPCHECK(hipMalloc(&p, count * sizeof(T) + 256));
+    allocs.push_back(p);
+    return static_cast<T*>(p);
+  }
+  size_t rsize() const { return precision == QG_PRECISION_MIXED ? 4 : 8; }
+  void* ralloc(size_t count) {  // count of R scalars
+    void* p = nullptr;
+    HIPCHECK(hipMalloc(&p, count * rsize() + 256));
+    allocs.push_back(p);
+    return p;
+  }
+  ~qg_handle() {
+    if (st) (void)hipStreamSynchronize(st);
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    for (void* p : allocs) (void)hipFree(p);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+namespace {
+
+// exp(-2*pi*i*m/n) correctly rounded from long double.
+template <typename R>
+std::vector<cplx<R>> twiddles(int n) {
+  std::vector<cplx<R>> t(n);
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (int m = 0; m < n; ++m) {
+    long double ang = -2.0L * pi * (long double)m / (long double)n;
+    t[m].re = (R)cosl(ang);
+    t[m].im = (R)sinl(ang);
+  }
+  return t;
+}
+
+// int_factor (src/nonlinear.jl:91-97, 1160-1183) and _etd_coefficients
+// (src/timestep.jl:177-190), per (kx, ky) column.
+double component_lambda(double kh2, double dt, const std::vector<double>& c, const std::vector<int>& o) {
+  double total = 0.0;
+  for (size_t i = 0; i < c.size(); ++i) {
+    double p = (double)powl((long double)kh2, (long double)(o[i] / 2));
+    total = total + c[i] * p;
+  }
+  return dt * total;
+}
+
+void etd_coefficients(double x, double h, double& E, double& h1, double& h2) {
+  E = std::exp(-x);
+  if (std::fabs(x) < 1e-6) {
+    double x2 = x * x;
+    h1 = h * (1 - x / 2 + x2 / 6 - x2 * x / 24 + x2 * x2 / 120);
+    h2 = h * (1.0 / 2 - x / 6 + x2 / 24 - x2 * x / 120 + x2 * x2 / 720);
+    return;
+  }
+  double em1 = std::expm1(-x);
+  h1 = h * (-em1) / x;
+  h2 = h * (em1 + x) / (x * x);
+}
+
+void build_etd(qg_handle& h, double dt) {
+  if (h.etd_dt == dt) return;
+  std::vector<double> tab(6 * h.plane);
+  for (int y = 0; y < h.ny; ++y)
+    for (int x = 0; x < h.nx; ++x) {
+      size_t col = (size_t)y * h.nx + x;
+      double kh2 = h.kx[x] * h.kx[x] + h.ky[y] * h.ky[y];
+      double lq = 0.0, lB = 0.0;
+      if (!h.inviscid) {
+        lq = h.fc.empty() ? 0.0 : component_lambda(kh2, dt, h.fc, h.fo);
+        lB = h.wc.empty() ? 0.0 : component_lambda(kh2, dt, h.wc, h.wo);
+      }
+      double e, a1, a2;
+      etd_coefficients(lq, dt, e, a1, a2);
+      tab[0 * h.plane + col] = e;
+      tab[1 * h.plane + col] = a1;
+      tab[2 * h.plane + col] = a2;
+      etd_coefficients(lB, dt, e, a1, a2);
+      tab[3 * h.plane + col] = e;
+      tab[4 * h.plane + col] = a1;
+      tab[5 * h.plane + col] = a2;
+    }
+  HIPCHECK(hipMemcpyAsync(h.d_etd, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, h.st));
+  HIPCHECK(hipStreamSynchronize(h.st));
+  h.etd_dt = dt;
+}
+
+bool is_dealiased(int i, int j, int nx, int ny) {  // src/physics.jl:36-43 (0-based)
+  int kmax = std::min(nx, ny) / 3;
+  int ix = i <= nx / 2 ? i : i - nx;
+  int jy = j <= ny / 2 ? j : j - ny;
+  return ix * ix + jy * jy <= kmax * kmax;
+}
+
+void check_status(qg_handle& h) {
+  int e[3] = {0, 0, 0};
+  HIPCHECK(hipMemcpyAsync(e, h.d_err, sizeof(e), hipMemcpyDeviceToHost, h.st));
+  HIPCHECK(hipStreamSynchronize(h.st));
+  if (e[0] != 0) {
+    HIPCHECK(hipMemsetAsync(h.d_err, 0, 3 * sizeof(int), h.st));
+    HIPCHECK(hipStreamSynchronize(h.st));
+    h.diag_valid = false;
+    h.t1_valid = false;
+    char buf[256];
+    if (e[0] == 1) {
+      snprintf(buf, sizeof buf,
+               "Singular matrix in Thomas solver at level %d of column %d (|pivot| < sqrt(eps)). "
+               "This may indicate an ill-conditioned elliptic problem from N^2~0 or a degenerate wavenumber.",
+               e[1] + 1, e[2]);
+      throw QgError(QG_ERR_SINGULAR, buf);
+    }
+    snprintf(buf, sizeof buf, "LAPACK-style gtsv failed for vertical velocity at row %d of column %d",
+             e[1] + 1, e[2]);
+    throw QgError(QG_ERR_GTSV, buf);
+  }
+}
+
+template <typename R>
+struct Engine {
+  using C = cplx<R>;
+  using O = qg::Ops<R>;
+  qg_handle& h;
+  // algorithmic bytes per point: complex R, real R, complex double; P = all
+  // points, Ph = points of a half-width (real-field) intermediate
+  const double cR = 2.0 * sizeof(R), rR = sizeof(R), cD = 16.0;
+  const double P, Ph;
+  explicit Engine(qg_handle& hh)
+      : h(hh), P((double)hh.npts), Ph((double)hh.g.W * hh.ny * hh.nz) {}
+  C* c(void* p) const { return static_cast<C*>(p); }
+  R* r(void* p) const { return static_cast<R*>(p); }
+  const C* twx() const { return static_cast<const C*>(h.twx); }
+  const C* twy() const { return static_cast<const C*>(h.twy); }
+  qg::EtdView etd() const { return qg::EtdView{h.d_etd, h.plane}; }
+
+  template <class F>
+  void K(const char* name, double bytes, F&& fn) {
+    if (h.prof_mode & 1) {
+      hipEvent_t a = h.event(), b = h.event();
+      HIPCHECK(hipEventRecord(a, h.st));
+      fn();
+      HIPCHECK(hipEventRecord(b, h.st));
+      h.prof.push_back({name, bytes, a, b});
+    } else {
+      fn();
+    }
+  }
+
+  void t1_only(const D* Bsrc) {
+    K("ya_T1", P * (cD + cR), [&] { O::ya_qw(h.g, h.st, Bsrc, c(h.T1), nullptr, twy()); });
+  }
+
+  // _diagnose_flow! (src/timestep.jl:192-215) minus the velocities: q_eff with
+  // wave feedback, then q -> psi.  Leaves T1 = IFFTy(mask B) when need_t1.
+  void diag(const D* qsrc, const D* Bsrc, D* psi_out, bool need_t1) {
+    bool t1 = false;
+    if (!h.fixed_flow) {
+      const D* src = qsrc;
+      if (h.feedback) {
+        K("ya_qw", P * (cD + 2 * cR), [&] { O::ya_qw(h.g, h.st, Bsrc, c(h.T1), c(h.T3), twy()); });
+        K("xa_qw", P * 2 * cR + Ph * 2 * cR,
+          [&] { O::xa_qw(h.g, h.st, c(h.T1), c(h.T3), c(h.Qa), c(h.Qb), twx()); });
+        K("ya2_qeff", Ph * 2 * cR + P * 2 * cD,
+          [&] { O::ya2_qeff(h.g, h.st, c(h.Qa), c(h.Qb), qsrc, h.qeff, nullptr, 1.0 / h.f, twy()); });
+        src = h.qeff;
+        t1 = true;
+      }
+      K("thomas_psi", P * 2 * cD,
+        [&] { qg::launch_thomas(h.g, h.st, src, psi_out, nullptr, h.s0, h.d_a, 1, h.d_err, h.d_warn); });
+    }
+    if (need_t1 && !t1) t1_only(Bsrc);
+  }
+
+  void wave_inv(const D* Bsrc, D* Aout, D* Cout) {
+    if (h.passive || h.no_disp) {
+      qg::launch_zero(Aout, h.npts, h.st);
+      if (Cout) qg::launch_zero(Cout, h.npts, h.st);
+      return;
+    }
+    K(Cout ? "thomas_AC" : "thomas_A", P * (Cout ? 3 : 2) * cD,
+      [&] { qg::launch_thomas(h.g, h.st, Bsrc, Aout, Cout, h.s0, h.d_a, 4, h.d_err, nullptr); });
+  }
+
+  void velocities(const D* psi) {
+    K("yb_uv", P * cD + Ph * 2 * cR, [&] {
+      O::yb_adv(h.g, h.st, qg::OUT_U | qg::OUT_V, psi, nullptr, c(h.Pu), c(h.Pv), nullptr, nullptr, twy());
+    });
+    K("x_c2r_uv", Ph * 2 * cR + P * 2 * rR,
+      [&] { O::x_c2r(h.g, h.st, 2, c(h.Pu), c(h.Pv), r(h.u), r(h.v), twx()); });
+  }
+
+  // u, v and the QG omega-equation w (compute_velocities!(compute_w=true)).
+  void velocities_w(const D* psi) {
+    K("yf", P * cD + Ph * 6 * cR, [&] {
+      O::yf(h.g, h.st, 1, psi, c(h.Pu), c(h.Pv), c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), twy());
+    });
+    K("xf", Ph * 7 * cR + P * 2 * rR, [&] {
+      O::xf(h.g, h.st, 1, c(h.Pu), c(h.Pv), c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), r(h.u), r(h.v), c(h.Pq),
+            twx());
+    });
+    K("y_r2c_omega", Ph * cR + P * cD, [&] { O::y_r2c_mask(h.g, h.st, c(h.Pq), h.rq0, twy()); });
+    K("omega_gtsv", P * 2 * cD, [&] {
+      qg::launch_omega(h.g, h.st, h.rq0, h.qeff, h.s0, h.s1, h.s2, h.d_cz, h.d_fN2, h.d_err);
+    });
+    K("y_herm_w", P * cD + Ph * cR, [&] { O::y_herm(h.g, h.st, h.qeff, c(h.Pz), twy()); });
+    K("x_c2r_w", Ph * cR + P * rR, [&] { O::x_c2r(h.g, h.st, 1, c(h.Pz), nullptr, r(h.w), nullptr, twx()); });
+  }
+
+  // One RHS evaluation at (qS, psiS, AS, T1, u, v) fused with the ETD update
+  // from the base state (qn, Bn): _compute_etdrk2_rhs! (src/timestep.jl:230-298)
+  // and the stage/final loops (src/timestep.jl:362-404).
+  void rhs_update(int mode, const D* qS, const D* psiS, const D* AS, const D* qn, const D* Bn, D* qout,
+                  D* Bout) {
+    const bool adv = !h.linear;
+    const bool refr = !h.passive;
+    const bool qdyn = !h.fixed_flow;
+    int outs = 0;
+    if (adv && qdyn) outs |= qg::OUT_Q;
+    if (refr) outs |= qg::OUT_Z;
+    const int nouts = ((outs & qg::OUT_Q) ? 1 : 0) + ((outs & qg::OUT_Z) ? 1 : 0);
+    if (outs)
+      K("yb_adv", P * (((outs & qg::OUT_Q) ? cD : 0) + ((outs & qg::OUT_Z) ? cD : 0)) + Ph * nouts * cR,
+        [&] { O::yb_adv(h.g, h.st, outs, psiS, qS, nullptr, nullptr, c(h.Pq), c(h.Pz), twy()); });
+    if (adv && qdyn)
+      K("xb_q", Ph * 3 * cR + P * 2 * rR,
+        [&] { O::xb_q(h.g, h.st, c(h.Pq), r(h.u), r(h.v), c(h.G1), c(h.G2), twx()); });
+    int xbf = (adv ? qg::XB_ADV : 0) | (refr ? qg::XB_REFR : 0);
+    if (xbf)
+      K("xb_B", P * cR + (refr ? Ph * cR : 0) + (adv ? P * 2 * rR : 0) + P * cR * ((adv ? 2 : 0) + (refr ? 1 : 0)),
+        [&] { O::xb_B(h.g, h.st, xbf, c(h.T1), c(h.Pz), r(h.u), r(h.v), c(h.G3), c(h.G4), c(h.G5), twx()); });
+    const bool diff = h.nu_z != 0.0 && !h.inviscid && h.nz > 1;
+    int qf = ((adv && qdyn) ? qg::YQ_ADV : 0) | (h.fixed_flow ? qg::YQ_FIXED : 0) | (diff ? qg::YQ_DIFF : 0);
+    const double inv_dz2 = h.nz > 1 ? 1.0 / (h.dz * h.dz) : 0.0;
+    K(mode == qg::MODE_STAGE ? "yb2_q_stage" : "yb2_q_final",
+      ((qf & qg::YQ_ADV) ? Ph * 2 * cR : 0) + P * cD * (3 + (diff ? 1 : 0)),
+      [&] { O::yb2_q(h.g, h.st, mode, qf, c(h.G1), c(h.G2), qS, qn, h.rq0, qout, etd(), h.nu_z, inv_dz2, twy()); });
+    const bool disp = !(h.passive || h.no_disp);
+    int bf = (adv ? qg::YB_ADV : 0) | (refr ? qg::YB_REFR : 0) | (disp ? qg::YB_DISP : 0);
+    K(mode == qg::MODE_STAGE ? "yb2_B_stage" : "yb2_B_final",
+      P * cR * ((adv ? 2 : 0) + (refr ? 1 : 0)) + P * cD * (3 + (disp ? 1 : 0)),
+      [&] { O::yb2_B(h.g, h.st, mode, bf, c(h.G3), c(h.G4), c(h.G5), AS, Bn, h.rB0, Bout, etd(), h.f / 2, twy()); });
+  }
+
+  // step!(model, ExponentialRungeKutta2) -> _advance_etdrk2! (src/timestep.jl:335-415).
+  void step() {
+    D* qn = h.q[h.cur];
+    D* Bn = h.B[h.cur];
+    D* q1 = h.q[1 - h.cur];
+    D* B1 = h.B[1 - h.cur];
+    const bool need_uv = !h.linear;
+    // RHS at S_n.  psi, A, u, v and T1 left by the previous finalize are the
+    // values this evaluation would recompute from (q_n, B_n).
+    if (!h.diag_valid) {
+      diag(qn, Bn, h.psi, true);
+      wave_inv(Bn, h.A, nullptr);
+      if (need_uv) velocities(h.psi);
+    } else if (!h.t1_valid) {
+      t1_only(Bn);
+    }
+    rhs_update(qg::MODE_STAGE, qn, h.psi, h.A, qn, Bn, h.qs, h.Bs);
+    // RHS at the stage
+    D* psiS = h.fixed_flow ? h.psi : h.psis;
+    diag(h.qs, h.Bs, psiS, true);
+    wave_inv(h.Bs, h.As, nullptr);
+    if (need_uv && !h.fixed_flow) velocities(psiS);
+    rhs_update(qg::MODE_FINAL, h.qs, psiS, h.As, qn, Bn, q1, B1);
+    // finalize (src/timestep.jl:300-332)
+    diag(q1, B1, h.psi, false);
+    wave_inv(B1, h.A, h.C);
+    velocities_w(h.psi);
+    h.cur = 1 - h.cur;
+    h.diag_valid = true;
+    h.t1_valid = h.feedback && !h.fixed_flow;
+  }
+
+  void compute_qw(D* out) {
+    O::ya_qw(h.g, h.st, h.B[h.cur], c(h.T1), c(h.T3), twy());
+    O::xa_qw(h.g, h.st, c(h.T1), c(h.T3), c(h.Qa), c(h.Qb), twx());
+    O::ya2_qeff(h.g, h.st, c(h.Qa), c(h.Qb), h.q[h.cur], nullptr, out, 1.0 / h.f, twy());
+    h.t1_valid = false;
+  }
+
+  void compute_velocities(bool compute_w) {
+    if (compute_w) {
+      velocities_w(h.psi);
+    } else {
+      velocities(h.psi);
+      qg::launch_zero_bytes(h.w, h.npts * h.rsize(), h.st);
+    }
+  }
+};
+
+template <typename F>
+int guarded(qg_handle* h, F&& fn) {
+  if (!h) return QG_ERR_ARGUMENT;
+  try {
+    fn();
+    h->err.clear();
+    return QG_OK;
+  } catch (const QgError& e) {
+    h->err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    h->err = e.what();
+    return QG_ERR_UNSUPPORTED;
+  }
+}
+
+template <typename Fn>
+void with_engine(qg_handle& h, Fn&& fn) {
+  if (h.precision == QG_PRECISION_MIXED) {
+    Engine<float> e(h);
+    fn(e);
+  } else {
+    Engine<double> e(h);
+    fn(e);
+  }
+}
+
+D* complex_field(qg_handle& h, int field) {
+  switch (field) {
+    case QG_FIELD_Q: return h.q[h.cur];
+    case QG_FIELD_B: return h.B[h.cur];
+    case QG_FIELD_PSI: return h.psi;
+    case QG_FIELD_A: return h.A;
+    case QG_FIELD_C: return h.C;
+    default: return nullptr;
+  }
+}
+void* real_field(qg_handle& h, int field) {
+  switch (field) {
+    case QG_FIELD_U: return h.u;
+    case QG_FIELD_V: return h.v;
+    case QG_FIELD_W: return h.w;
+    default: return nullptr;
+  }
+}
+
+// planar double field -> device planar field in place: forward FFT.
+void fft2_inplace(qg_handle& h, D* f, int dir, unsigned long long* absmax = nullptr, D* out = nullptr) {
+  // y first for the inverse (spectral -> physical), x first for the forward
+  if (dir < 0) {
+    qg::Ops<double>::x_c2c(h.g, h.st, -1, f, f, nullptr, h.tw64x);
+    qg::Ops<double>::y_c2c(h.g, h.st, -1, f, out ? out : f, h.tw64y);
+  } else {
+    D* o = out ? out : f;
+    qg::Ops<double>::y_c2c(h.g, h.st, +1, f, o, h.tw64y);
+    qg::Ops<double>::x_c2c(h.g, h.st, +1, o, absmax ? nullptr : o, absmax, h.tw64x);
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* qg_last_error(const qg_handle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+int qg_create(const qg_config* cfg, qg_handle** out) {
+  g_create_error.clear();
+  if (!cfg || !out) {
+    g_create_error = "qg_create: null argument";
+    return QG_ERR_ARGUMENT;
+  }
+  *out = nullptr;
+  qg_handle* h = nullptr;
+  try {
+    if (cfg->nx <= 0 || cfg->ny <= 0 || cfg->nz <= 0)
+      throw QgError(QG_ERR_ARGUMENT, "grid dimensions must be positive");
+    if (!qg::fft_size_supported(cfg->nx) || !qg::fft_size_supported(cfg->ny))
+      throw QgError(QG_ERR_UNSUPPORTED,
+                    "horizontal sizes must be one of 8,16,32,64,128,256,384,512 (got " +
+                        std::to_string(cfg->nx) + "x" + std::to_string(cfg->ny) + ")");
+    if (cfg->formulation == QG_YBJ)
+      throw QgError(QG_ERR_UNSUPPORTED, "the normal YBJ formulation is not part of this library; use YBJ+");
+    if (cfg->formulation != QG_YBJ_PLUS && cfg->formulation != QG_PASSIVE)
+      throw QgError(QG_ERR_ARGUMENT, "unknown wave formulation");
+    if (cfg->precision != QG_PRECISION_FP64 && cfg->precision != QG_PRECISION_MIXED)
+      throw QgError(QG_ERR_ARGUMENT, "unknown precision mode");
+    if (!cfg->kx || !cfg->ky || !cfg->N2 || !cfg->a_ell)
+      throw QgError(QG_ERR_ARGUMENT, "kx, ky, N2 and a_ell are required");
+    if (!(std::isfinite(cfg->dz) && cfg->dz > 0)) throw QgError(QG_ERR_ARGUMENT, "dz must be finite and positive");
+    if (!(std::isfinite(cfg->nu_z) && cfg->nu_z >= 0))
+      throw QgError(QG_ERR_ARGUMENT, "vertical diffusivity must be finite and non-negative");
+    const bool feedback = cfg->wave_feedback && !cfg->fixed_flow;
+    if (feedback && !(std::isfinite(cfg->f) && cfg->f != 0))
+      throw QgError(QG_ERR_ARGUMENT, "wave feedback requires a finite, nonzero Coriolis frequency");
+    for (int i = 0; i < cfg->n_flow_terms; ++i)
+      if (cfg->flow_orders[i] <= 0 || cfg->flow_orders[i] % 2)
+        throw QgError(QG_ERR_ARGUMENT, "flow hyperdiffusivity orders must be positive and even");
+    for (int i = 0; i < cfg->n_wave_terms; ++i)
+      if (cfg->wave_orders[i] <= 0 || cfg->wave_orders[i] % 2)
+        throw QgError(QG_ERR_ARGUMENT, "wave hyperdiffusivity orders must be positive and even");
+
+    h = new qg_handle();
+    h->nx = cfg->nx;
+    h->ny = cfg->ny;
+    h->nz = cfg->nz;
+    h->precision = cfg->precision;
+    h->f = cfg->f;
+    h->dz = cfg->nz > 1 ? cfg->dz : 1.0;
+    h->nu_z = cfg->nu_z;
+    h->fixed_flow = cfg->fixed_flow != 0;
+    h->feedback = feedback;
+    h->passive = cfg->formulation == QG_PASSIVE;
+    h->inviscid = cfg->inviscid != 0;
+    h->linear = cfg->linear != 0;
+    h->no_disp = cfg->no_dispersion != 0;
+    h->kx.assign(cfg->kx, cfg->kx + cfg->nx);
+    h->ky.assign(cfg->ky, cfg->ky + cfg->ny);
+    h->N2.assign(cfg->N2, cfg->N2 + cfg->nz);
+    h->a.assign(cfg->a_ell, cfg->a_ell + cfg->nz);
+    if (cfg->n_flow_terms > 0) {
+      h->fc.assign(cfg->flow_coefficients, cfg->flow_coefficients + cfg->n_flow_terms);
+      h->fo.assign(cfg->flow_orders, cfg->flow_orders + cfg->n_flow_terms);
+    }
+    if (cfg->n_wave_terms > 0) {
+      h->wc.assign(cfg->wave_coefficients, cfg->wave_coefficients + cfg->n_wave_terms);
+      h->wo.assign(cfg->wave_orders, cfg->wave_orders + cfg->n_wave_terms);
+    }
+    HIPCHECK(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
+    const int nx = h->nx, ny = h->ny, nz = h->nz;
+    h->plane = (size_t)nx * ny;
+    h->npts = h->plane * nz;
+    const int W = nx / 2 + 1;
+    const int Wp = (W + 15) / 16 * 16;
+    h->nhalf = (size_t)Wp * ny * nz;
+
+    h->d_kx = h->alloc<double>(nx);
+    h->d_ky = h->alloc<double>(ny);
+    h->d_a = h->alloc<double>(nz);
+    h->d_cz = h->alloc<double>(nz);
+    h->d_fN2 = h->alloc<double>(nz);
+    h->d_mask = h->alloc<unsigned char>(h->plane);
+    h->d_etd = h->alloc<double>(6 * h->plane);
+    std::vector<unsigned char> mask(h->plane);
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) mask[(size_t)y * nx + x] = is_dealiased(x, y, nx, ny) ? 1 : 0;
+    std::vector<double> cz(nz), fN2(nz);
+    const double f2 = h->f * h->f;
+    for (int k = 0; k < nz; ++k) {
+      cz[k] = (f2 / h->N2[k]) / (h->dz * h->dz);  // src/operators.jl:428
+      fN2[k] = h->f / h->N2[k];                   // src/operators.jl:439
+    }
+    HIPCHECK(hipMemcpy(h->d_kx, h->kx.data(), nx * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(h->d_ky, h->ky.data(), ny * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(h->d_a, h->a.data(), nz * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(h->d_cz, cz.data(), nz * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(h->d_fN2, fN2.data(), nz * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(h->d_mask, mask.data(), h->plane, hipMemcpyHostToDevice));
+
+    for (int i = 0; i < 2; ++i) {
+      h->q[i] = h->alloc<D>(h->npts);
+      h->B[i] = h->alloc<D>(h->npts);
+    }
+    h->psi = h->alloc<D>(h->npts);
+    h->A = h->alloc<D>(h->npts);
+    h->C = h->alloc<D>(h->npts);
+    h->qs = h->alloc<D>(h->npts);
+    h->Bs = h->alloc<D>(h->npts);
+    h->psis = h->alloc<D>(h->npts);
+    h->As = h->alloc<D>(h->npts);
+    h->rq0 = h->alloc<D>(h->npts);
+    h->rB0 = h->alloc<D>(h->npts);
+    h->qeff = h->alloc<D>(h->npts);
+    h->stg = h->alloc<D>(h->npts);
+    h->s0 = h->alloc<double>(h->npts);
+    h->s1 = h->alloc<double>(h->npts);
+    h->s2 = h->alloc<double>(h->npts);
+    h->u = h->ralloc(h->npts);
+    h->v = h->ralloc(h->npts);
+    h->w = h->ralloc(h->npts);
+    for (void** p : {&h->T1, &h->T3, &h->G3, &h->G4, &h->G5}) *p = h->ralloc(2 * h->npts);
+    for (void** p : {&h->Qa, &h->Qb, &h->Pu, &h->Pv, &h->Pq, &h->Pz, &h->G1, &h->G2}) *p = h->ralloc(2 * h->nhalf);
+    h->d_err = h->alloc<int>(4);
+    h->d_warn = h->alloc<unsigned long long>(1);
+    h->d_absmax = h->alloc<unsigned long long>(1);
+    h->d_flag = h->alloc<int>(1);
+    HIPCHECK(hipMemset(h->d_err, 0, 4 * sizeof(int)));
+    HIPCHECK(hipMemset(h->d_warn, 0, sizeof(unsigned long long)));
+    for (D* p : {h->q[0], h->q[1], h->B[0], h->B[1], h->psi, h->A, h->C})
+      HIPCHECK(hipMemset(p, 0, h->npts * sizeof(D)));
+    for (void* p : {h->u, h->v, h->w}) HIPCHECK(hipMemset(p, 0, h->npts * h->rsize()));
+
+    h->tw64x = h->alloc<D>(nx);
+    h->tw64y = h->alloc<D>(ny);
+    {
+      auto tx = twiddles<double>(nx), ty = twiddles<double>(ny);
+      HIPCHECK(hipMemcpy(h->tw64x, tx.data(), nx * sizeof(D), hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(h->tw64y, ty.data(), ny * sizeof(D), hipMemcpyHostToDevice));
+    }
+    if (h->precision == QG_PRECISION_MIXED) {
+      auto tx = twiddles<float>(nx), ty = twiddles<float>(ny);
+      h->twx = h->alloc<cplx<float>>(nx);
+      h->twy = h->alloc<cplx<float>>(ny);
+      HIPCHECK(hipMemcpy(h->twx, tx.data(), nx * sizeof(cplx<float>), hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(h->twy, ty.data(), ny * sizeof(cplx<float>), hipMemcpyHostToDevice));
+    } else {
+      h->twx = h->tw64x;
+      h->twy = h->tw64y;
+    }
+
+    qg::Geo& g = h->g;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.W = W;
+    g.Wp = Wp;
+    g.kx = h->d_kx;
+    g.ky = h->d_ky;
+    g.mask = h->d_mask;
+    g.inv_n = 1.0 / ((double)nx * (double)ny);
+    g.dz = h->dz;
+    HIPCHECK(hipDeviceSynchronize());
+    *out = h;
+    return QG_OK;
+  } catch (const QgError& e) {
+    g_create_error = e.what();
+    delete h;
+    return e.code;
+  } catch (const std::exception& e) {
+    g_create_error = e.what();
+    delete h;
+    return QG_ERR_DEVICE;
+  }
+}
+
+int qg_destroy(qg_handle* h) {
+  delete h;
+  return QG_OK;
+}
+
+int qg_set_field(qg_handle* h, int field, const double* host, int space) {
+  return guarded(h, [&] {
+    if (!host) throw QgError(QG_ERR_ARGUMENT, "null host buffer");
+    qg_handle& H = *h;
+    if (D* dst = complex_field(H, field)) {
+      HIPCHECK(hipMemcpyAsync(H.stg, host, H.npts * sizeof(D), hipMemcpyHostToDevice, H.st));
+      qg::launch_transpose<D>(H.stg, dst, H.nx, H.ny, H.nz, true, H.st);
+      if (space == QG_SPACE_PHYSICAL) fft2_inplace(H, dst, -1);
+      else if (space != QG_SPACE_SPECTRAL) throw QgError(QG_ERR_ARGUMENT, "space must be spectral or physical");
+    } else if (void* rdst = real_field(H, field)) {
+      double* tmp = reinterpret_cast<double*>(H.stg);
+      double* pl = reinterpret_cast<double*>(H.qeff);
+      HIPCHECK(hipMemcpyAsync(tmp, host, H.npts * sizeof(double), hipMemcpyHostToDevice, H.st));
+      if (H.precision == QG_PRECISION_MIXED) {
+        qg::launch_transpose<double>(tmp, pl, H.nx, H.ny, H.nz, true, H.st);
+        qg::launch_convert<double, float>(pl, static_cast<float*>(rdst), H.npts, H.st);
+      } else {
+        qg::launch_transpose<double>(tmp, static_cast<double*>(rdst), H.nx, H.ny, H.nz, true, H.st);
+      }
+    } else {
+      throw QgError(QG_ERR_ARGUMENT, "unknown field id " + std::to_string(field));
+    }
+    HIPCHECK(hipStreamSynchronize(H.st));
+    H.diag_valid = false;
+    H.t1_valid = false;
+  });
+}
+
+int qg_get_field(qg_handle* h, int field, double* host) {
+  return guarded(h, [&] {
+    if (!host) throw QgError(QG_ERR_ARGUMENT, "null host buffer");
+    qg_handle& H = *h;
+    if (D* src = complex_field(H, field)) {
+      qg::launch_transpose<D>(src, H.stg, H.nx, H.ny, H.nz, false, H.st);
+      HIPCHECK(hipMemcpyAsync(host, H.stg, H.npts * sizeof(D), hipMemcpyDeviceToHost, H.st));
+    } else if (void* rsrc = real_field(H, field)) {
+      double* tmp = reinterpret_cast<double*>(H.stg);
+      const double* pl = static_cast<const double*>(rsrc);
+      if (H.precision == QG_PRECISION_MIXED) {
+        double* conv = reinterpret_cast<double*>(H.qeff);
+        qg::launch_convert<float, double>(static_cast<const float*>(rsrc), conv, H.npts, H.st);
+        pl = conv;
+      }
+      qg::launch_transpose<double>(pl, tmp, H.nx, H.ny, H.nz, false, H.st);
+      HIPCHECK(hipMemcpyAsync(host, tmp, H.npts * sizeof(double), hipMemcpyDeviceToHost, H.st));
+    } else {
+      throw QgError(QG_ERR_ARGUMENT, "unknown field id " + std::to_string(field));
+    }
+    HIPCHECK(hipStreamSynchronize(H.st));
+  });
+}
+
+int qg_step(qg_handle* h, double dt, int nsteps) {
+  return guarded(h, [&] {
+    if (!(std::isfinite(dt) && dt > 0)) throw QgError(QG_ERR_ARGUMENT, "Δt must be finite and positive");
+    if (nsteps < 0) throw QgError(QG_ERR_ARGUMENT, "nsteps must be non-negative");
+    build_etd(*h, dt);
+    with_engine(*h, [&](auto& e) {
+      for (int i = 0; i < nsteps; ++i) {
+        if ((h->prof_mode & 2) && h->step_events.empty()) {
+          hipEvent_t ev = h->event();
+          HIPCHECK(hipEventRecord(ev, h->st));
+          h->step_events.push_back(ev);
+        }
+        e.step();
+        if (h->prof_mode & 2) {
+          hipEvent_t ev = h->event();
+          HIPCHECK(hipEventRecord(ev, h->st));
+          h->step_events.push_back(ev);
+        }
+      }
+    });
+    HIPCHECK(hipGetLastError());
+    check_status(*h);
+  });
+}
+
+int qg_synchronize(qg_handle* h) {
+  return guarded(h, [&] { HIPCHECK(hipStreamSynchronize(h->st)); });
+}
+
+int qg_check(qg_handle* h, int* nonfinite, double* max_abs_psi) {
+  return guarded(h, [&] {
+    qg_handle& H = *h;
+    HIPCHECK(hipMemsetAsync(H.d_flag, 0, sizeof(int), H.st));
+    HIPCHECK(hipMemsetAsync(H.d_absmax, 0, sizeof(unsigned long long), H.st));
+    qg::launch_nonfinite(H.q[H.cur], H.npts, H.d_flag, H.st);
+    qg::launch_nonfinite(H.B[H.cur], H.npts, H.d_flag, H.st);
+    qg::launch_nonfinite(H.psi, H.npts, H.d_flag, H.st);
+    fft2_inplace(H, H.psi, +1, H.d_absmax, H.stg);
+    int flag = 0;
+    unsigned long long bits = 0;
+    HIPCHECK(hipMemcpyAsync(&flag, H.d_flag, sizeof(int), hipMemcpyDeviceToHost, H.st));
+    HIPCHECK(hipMemcpyAsync(&bits, H.d_absmax, sizeof(bits), hipMemcpyDeviceToHost, H.st));
+    HIPCHECK(hipStreamSynchronize(H.st));
+    double m;
+    std::memcpy(&m, &bits, sizeof m);
+    if (nonfinite) *nonfinite = flag;
+    if (max_abs_psi) *max_abs_psi = m;
+  });
+}
+
+int qg_invert_q_to_psi(qg_handle* h) {
+  return guarded(h, [&] {
+    qg::launch_thomas(h->g, h->st, h->q[h->cur], h->psi, nullptr, h->s0, h->d_a, 1, h->d_err, h->d_warn);
+    h->diag_valid = false;
+    check_status(*h);
+  });
+}
+
+int qg_invert_B_to_A(qg_handle* h) {
+  return guarded(h, [&] {
+    qg::launch_thomas(h->g, h->st, h->B[h->cur], h->A, h->C, h->s0, h->d_a, 4, h->d_err, nullptr);
+    h->diag_valid = false;
+    check_status(*h);
+  });
+}
+
+int qg_compute_velocities(qg_handle* h, int compute_w) {
+  return guarded(h, [&] {
+    with_engine(*h, [&](auto& e) { e.compute_velocities(compute_w != 0); });
+    h->diag_valid = false;
+    check_status(*h);
+  });
+}
+
+int qg_compute_q_from_psi(qg_handle* h, int method) {
+  return guarded(h, [&] {
+    if (method != 0 && method != 1) throw QgError(QG_ERR_ARGUMENT, "pv method must be 0 (qg) or 1 (barotropic)");
+    qg::launch_q_from_psi(h->g, h->st, h->psi, h->q[h->cur], h->d_a, method);
+    h->diag_valid = false;
+    HIPCHECK(hipStreamSynchronize(h->st));
+  });
+}
+
+int qg_compute_qw(qg_handle* h, double* host_out) {
+  return guarded(h, [&] {
+    if (!host_out) throw QgError(QG_ERR_ARGUMENT, "null host buffer");
+    if (!(std::isfinite(h->f) && h->f != 0))
+      throw QgError(QG_ERR_ARGUMENT, "wave feedback requires a finite, nonzero Coriolis frequency");
+    with_engine(*h, [&](auto& e) { e.compute_qw(h->qeff); });
+    qg::launch_transpose<D>(h->qeff, h->stg, h->nx, h->ny, h->nz, false, h->st);
+    HIPCHECK(hipMemcpyAsync(host_out, h->stg, h->npts * sizeof(D), hipMemcpyDeviceToHost, h->st));
+    HIPCHECK(hipStreamSynchronize(h->st));
+  });
+}
+
+int qg_fft(qg_handle* h, const double* host_in, double* host_out, int direction) {
+  return guarded(h, [&] {
+    if (!host_in || !host_out) throw QgError(QG_ERR_ARGUMENT, "null host buffer");
+    if (direction != -1 && direction != 1) throw QgError(QG_ERR_ARGUMENT, "direction must be -1 or +1");
+    qg_handle& H = *h;
+    HIPCHECK(hipMemcpyAsync(H.stg, host_in, H.npts * sizeof(D), hipMemcpyHostToDevice, H.st));
+    qg::launch_transpose<D>(H.stg, H.qeff, H.nx, H.ny, H.nz, true, H.st);
+    fft2_inplace(H, H.qeff, direction);
+    qg::launch_transpose<D>(H.qeff, H.stg, H.nx, H.ny, H.nz, false, H.st);
+    HIPCHECK(hipMemcpyAsync(host_out, H.stg, H.npts * sizeof(D), hipMemcpyDeviceToHost, H.st));
+    HIPCHECK(hipStreamSynchronize(H.st));
+  });
+}
+
+int qg_profile_begin(qg_handle* h, int mode) {
+  return guarded(h, [&] {
+    HIPCHECK(hipStreamSynchronize(h->st));
+    h->prof.clear();
+    h->step_events.clear();
+    h->pool_used = 0;
+    h->prof_mode = mode & 3;
+  });
+}
+
+int qg_profile_end(qg_handle* h, char* buf, size_t len, size_t* needed) {
+  return guarded(h, [&] {
+    HIPCHECK(hipStreamSynchronize(h->st));
+    struct Agg {
+      std::string name;
+      int count = 0;
+      double ms = 0, bytes = 0;
+    };
+    std::vector<Agg> agg;
+    for (auto& r : h->prof) {
+      float ms = 0;
+      HIPCHECK(hipEventElapsedTime(&ms, r.a, r.b));
+      Agg* a = nullptr;
+      for (auto& x : agg)
+        if (x.name == r.name) a = &x;
+      if (!a) {
+        agg.push_back(Agg{r.name});
+        a = &agg.back();
+      }
+      a->count += 1;
+      a->ms += ms;
+      a->bytes += r.bytes;
+    }
+    std::string js = "{\"kernels\": {";
+    char tmp[256];
+    for (size_t i = 0; i < agg.size(); ++i) {
+      snprintf(tmp, sizeof tmp, "%s\"%s\": {\"count\": %d, \"ms\": %.6f, \"bytes\": %.1f}", i ? ", " : "",
+               agg[i].name.c_str(), agg[i].count, agg[i].ms, agg[i].bytes);
+      js += tmp;
+    }
+    js += "}, \"steps_ms\": [";
+    for (size_t i = 1; i < h->step_events.size(); ++i) {
+      float ms = 0;
+      HIPCHECK(hipEventElapsedTime(&ms, h->step_events[i - 1], h->step_events[i]));
+      snprintf(tmp, sizeof tmp, "%s%.6f", i > 1 ? ", " : "", ms);
+      js += tmp;
+    }
+    js += "]}";
+    h->prof_mode = 0;
+    h->prof.clear();
+    h->step_events.clear();
+    h->pool_used = 0;
+    if (needed) *needed = js.size() + 1;
+    if (buf && len > 0) {
+      size_t n = std::min(len - 1, js.size());
+      std::memcpy(buf, js.data(), n);
+      buf[n] = 0;
+      if (js.size() + 1 > len) throw QgError(QG_ERR_ARGUMENT, "profile buffer too small");
+    }
+  });
+}
+
+int qg_take_mean_q_warning(qg_handle* h, double* max_abs_mean_q) {
+  return guarded(h, [&] {
+    unsigned long long bits = 0;
+    HIPCHECK(hipMemcpyAsync(&bits, h->d_warn, sizeof bits, hipMemcpyDeviceToHost, h->st));
+    HIPCHECK(hipMemsetAsync(h->d_warn, 0, sizeof bits, h->st));
+    HIPCHECK(hipStreamSynchronize(h->st));
+    double m;
+    std::memcpy(&m, &bits, sizeof m);
+    if (max_abs_mean_q) *max_abs_mean_q = m;
+  });
+}
+
+}  // extern "C"
