@@ -11,6 +11,10 @@
 // the previous radices) and radix r, butterfly i in [0, N/r) reads
 // x[i + t*N/r], multiplies by w_{rP}^{t*(i mod P)}, applies an r-point DFT and
 // writes y[(i - i mod P)*r + i mod P + t*P].
+//
+// A lane's butterflies are the same in every transform it runs, so its stage
+// twiddles are loaded once per kernel (TwReg) and reused by every forward and
+// inverse transform of that kernel.
 #pragma once
 
 #include "qg_common.hpp"
@@ -71,8 +75,7 @@ struct Dft<3, DIR> {
     cplx<R> s = b + c;
     cplx<R> d = b - c;
     cplx<R> m = mk<R>(a.re - (R)0.5 * s.re, a.im - (R)0.5 * s.im);
-    // DIR * i * h * d
-    cplx<R> r = mk<R>(-(R)DIR * h * d.im, (R)DIR * h * d.re);
+    cplx<R> r = mk<R>(-(R)DIR * h * d.im, (R)DIR * h * d.re);  // DIR*i*h*d
     v[0] = a + s;
     v[1] = m + r;
     v[2] = m - r;
@@ -142,17 +145,49 @@ template <> struct FftPlan<256> { static constexpr int S = 3, r0 = 8, r1 = 4, T 
 template <> struct FftPlan<384> { static constexpr int S = 3, r0 = 8, r1 = 6, T = 48; };
 template <> struct FftPlan<512> { static constexpr int S = 3, r0 = 8, r1 = 8, T = 64; };
 
-// Twiddle multiply for one butterfly: v[t] *= exp(DIR*2*pi*i*t*k/(r*P)),
-// read from the length-N forward table tw[m] = exp(-2*pi*i*m/N).
-template <typename R, int N, int r, int P, int DIR>
-__device__ inline void twiddle(cplx<R>* v, int k, const cplx<R>* __restrict__ tw) {
-  constexpr int step = N / (r * P);
+// Stage twiddles of one lane, forward sign; the inverse uses the conjugates.
+template <typename R, int N>
+struct TwReg {
+  using P = FftPlan<N>;
+  static constexpr int R0 = P::r0, R1 = P::r1, T = P::T;
+  static constexpr int NB = N / R0;
+  static constexpr int IT = (NB + T - 1) / T;
+  static constexpr int NB1 = N / R1;
+  static constexpr int IT1 = (NB1 + T - 1) / T;
+  static constexpr int NMID = (P::S == 3) ? IT1 * (R1 - 1) : 1;
+  static constexpr int NLAST = (P::S >= 2) ? IT * (R0 - 1) : 1;
+  cplx<R> mid[NMID];
+  cplx<R> last[NLAST];
+
+  // tw: length-N table, tw[m] = exp(-2*pi*i*m/N)
+  __device__ inline void load(int lane, const cplx<R>* __restrict__ tw) {
+    if constexpr (P::S == 3) {
+      constexpr int step = N / (R1 * R0);
 #pragma unroll
-  for (int t = 1; t < r; ++t) {
-    cplx<R> w = tw[t * k * step];
-    if (DIR > 0) w.im = -w.im;
-    v[t] = cmul(v[t], w);
+      for (int it = 0; it < IT1; ++it) {
+        const int i = lane + it * T;
+        const int k = i % R0;
+#pragma unroll
+        for (int t = 1; t < R1; ++t)
+          mid[it * (R1 - 1) + t - 1] = (NB1 % T == 0 || i < NB1) ? tw[t * k * step] : mk<R>((R)1, (R)0);
+      }
+    }
+    if constexpr (P::S >= 2) {
+#pragma unroll
+      for (int it = 0; it < IT; ++it) {
+        const int i = lane + it * T;  // span NB, k = i, step = 1
+#pragma unroll
+        for (int t = 1; t < R0; ++t)
+          last[it * (R0 - 1) + t - 1] = (NB % T == 0 || i < NB) ? tw[t * i] : mk<R>((R)1, (R)0);
+      }
+    }
   }
+};
+
+template <typename R, int DIR>
+__device__ inline cplx<R> twmul(cplx<R> v, cplx<R> w) {
+  if (DIR > 0) w.im = -w.im;
+  return cmul(v, w);
 }
 
 template <typename R, int N, int DIR>
@@ -164,15 +199,18 @@ struct Fft {
   static constexpr int NB = N / R0;
   static constexpr int IT = (NB + T - 1) / T;
   static constexpr int NE = IT * R0;  // register elements per lane
+  // LDS index of element `idx`: one pad slot every R0 elements keeps the
+  // stride-R0 first-stage writes of neighbouring lanes in distinct banks.
+  static constexpr int LDSZ = N + N / R0;
+  __device__ static inline int pad(int idx) { return idx + idx / R0; }
 
   // Element index held in register slot [it][t].
   __device__ static inline int elem(int lane, int it, int t) { return lane + it * T + t * NB; }
   __device__ static inline bool valid(int lane, int it) { return (NB % T == 0) || (lane + it * T < NB); }
 
-  // s: this sequence's LDS (N elements, unit stride).  Contains barriers: every
-  // thread of the block must call it the same number of times.
-  __device__ static inline void run(cplx<R> (&v)[IT][R0], cplx<R>* s, int lane,
-                                    const cplx<R>* __restrict__ tw) {
+  // s: this sequence's LDS (LDSZ elements).  Contains barriers: every thread
+  // of the block must call it the same number of times.
+  __device__ static inline void run(cplx<R> (&v)[IT][R0], cplx<R>* s, int lane, const TwReg<R, N>& w) {
     if constexpr (P::S == 1) {
 #pragma unroll
       for (int it = 0; it < IT; ++it)
@@ -186,7 +224,7 @@ struct Fft {
           int i = lane + it * T;
           Dft<R0, DIR>::run(v[it]);
 #pragma unroll
-          for (int t = 0; t < R0; ++t) s[i * R0 + t] = v[it][t];
+          for (int t = 0; t < R0; ++t) s[pad(i * R0 + t)] = v[it][t];
         }
       }
       __syncthreads();
@@ -199,7 +237,7 @@ struct Fft {
           int i = lane + it * T;
           if (NB1 % T == 0 || i < NB1) {
 #pragma unroll
-            for (int t = 0; t < R1; ++t) u[it][t] = s[i + t * NB1];
+            for (int t = 0; t < R1; ++t) u[it][t] = s[pad(i + t * NB1)];
           }
         }
         __syncthreads();
@@ -208,11 +246,12 @@ struct Fft {
           int i = lane + it * T;
           if (NB1 % T == 0 || i < NB1) {
             int k = i % R0;
-            twiddle<R, N, R1, R0, DIR>(u[it], k, tw);
+#pragma unroll
+            for (int t = 1; t < R1; ++t) u[it][t] = twmul<R, DIR>(u[it][t], w.mid[it * (R1 - 1) + t - 1]);
             Dft<R1, DIR>::run(u[it]);
             int j = (i - k) * R1 + k;
 #pragma unroll
-            for (int t = 0; t < R1; ++t) s[j + t * R0] = u[it][t];
+            for (int t = 0; t < R1; ++t) s[pad(j + t * R0)] = u[it][t];
           }
         }
         __syncthreads();
@@ -223,8 +262,9 @@ struct Fft {
         if (valid(lane, it)) {
           int i = lane + it * T;
 #pragma unroll
-          for (int t = 0; t < R0; ++t) v[it][t] = s[i + t * NB];
-          twiddle<R, N, R0, NB, DIR>(v[it], i, tw);
+          for (int t = 0; t < R0; ++t) v[it][t] = s[pad(i + t * NB)];
+#pragma unroll
+          for (int t = 1; t < R0; ++t) v[it][t] = twmul<R, DIR>(v[it][t], w.last[it * (R0 - 1) + t - 1]);
           Dft<R0, DIR>::run(v[it]);
         }
       }

@@ -24,7 +24,8 @@ struct Tile {
   static constexpr int S0 = TARGET / T;
   static constexpr int S = S0 < 1 ? 1 : (S0 > 64 ? 64 : S0);
   static constexpr int THREADS = S * T;
-  static constexpr int LD = N + 1;
+  static constexpr int LD = N + N / FftPlan<N>::r0 + 1;  // padded sequence stride in LDS
+  static constexpr int NN = N;
 };
 
 template <typename R>
@@ -48,6 +49,8 @@ k_ya_qw(Geo g, const cplx<double>* __restrict__ B, cplx<R>* __restrict__ T1,
   using F = Fft<R, NY, +1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
+  TwReg<R, TL::NN> twr;
+  twr.load(lane, tw);
   const int x = blockIdx.x * TL::S + s, z = blockIdx.y;
   const bool ok = x < g.nx;
   cplx<R>* sl = lds + s * TL::LD;
@@ -66,8 +69,8 @@ k_ya_qw(Geo g, const cplx<double>* __restrict__ B, cplx<R>* __restrict__ T1,
         w[it][t] = cvt<R>(ik_mul(g.ky[y], b));
       }
     }
-  F::run(v, sl, lane, tw);
-  if (T3) F::run(w, sl, lane, tw);  // uniform: T3 == nullptr computes T1 only
+  F::run(v, sl, lane, twr);
+  if (T3) F::run(w, sl, lane, twr);  // uniform: T3 == nullptr computes T1 only
 #pragma unroll
   for (int it = 0; it < F::IT; ++it)
 #pragma unroll
@@ -93,6 +96,8 @@ k_xa_qw(Geo g, const cplx<R>* __restrict__ T1, const cplx<R>* __restrict__ T3,
   using FF = Fft<R, NX, -1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
+  TwReg<R, TL::NN> twr;
+  twr.load(lane, tw);
   const int row = blockIdx.x * TL::S + r;
   const bool ok = row < g.nz * g.ny;
   cplx<R>* sl = lds + r * TL::LD;
@@ -109,12 +114,12 @@ k_xa_qw(Geo g, const cplx<R>* __restrict__ T1, const cplx<R>* __restrict__ T3,
       v[it][t] = a;
       w[it][t] = ik_mul((R)g.kx[x], a);
     }
-  FI::run(v, sl, lane, tw);
+  FI::run(v, sl, lane, twr);
 #pragma unroll
   for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
     for (int t = 0; t < FI::R0; ++t) mag[it][t] = v[it][t].re * v[it][t].re + v[it][t].im * v[it][t].im;
-  FI::run(w, sl, lane, tw);  // w = Bx
+  FI::run(w, sl, lane, twr);  // w = Bx
 #pragma unroll
   for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
@@ -122,7 +127,7 @@ k_xa_qw(Geo g, const cplx<R>* __restrict__ T1, const cplx<R>* __restrict__ T3,
       int x = FI::elem(lane, it, t);
       v[it][t] = (ok && FI::valid(lane, it)) ? T3[base + x] : czero<R>();
     }
-  FI::run(v, sl, lane, tw);  // v = By
+  FI::run(v, sl, lane, twr);  // v = By
 #pragma unroll
   for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
@@ -131,8 +136,8 @@ k_xa_qw(Geo g, const cplx<R>* __restrict__ T1, const cplx<R>* __restrict__ T3,
       v[it][t] = mk<R>(q, (R)0);
       w[it][t] = mk<R>(mag[it][t], (R)0);
     }
-  FF::run(v, sl, lane, tw);
-  FF::run(w, sl, lane, tw);
+  FF::run(v, sl, lane, twr);
+  FF::run(w, sl, lane, twr);
   const size_t hb = (size_t)row * g.Wp;
 #pragma unroll
   for (int it = 0; it < FI::IT; ++it)
@@ -157,6 +162,8 @@ k_ya2_qeff(Geo g, const cplx<R>* __restrict__ Qa, const cplx<R>* __restrict__ Qb
   using F = Fft<R, NY, -1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
+  TwReg<R, TL::NN> twr;
+  twr.load(lane, tw);
   const int x = blockIdx.x * TL::S + s, z = blockIdx.y;
   const bool ok = x < g.W;
   cplx<R>* sl = lds + s * TL::LD;
@@ -171,8 +178,8 @@ k_ya2_qeff(Geo g, const cplx<R>* __restrict__ Qa, const cplx<R>* __restrict__ Qb
       v[it][t] = in ? Qa[a] : czero<R>();
       w[it][t] = in ? Qb[a] : czero<R>();
     }
-  F::run(v, sl, lane, tw);
-  F::run(w, sl, lane, tw);
+  F::run(v, sl, lane, twr);
+  F::run(w, sl, lane, twr);
   const int mx = mirror(x, g.nx);
 #pragma unroll
   for (int it = 0; it < F::IT; ++it)
@@ -213,6 +220,8 @@ k_yb_adv(Geo g, int outs, const cplx<double>* __restrict__ psi, const cplx<doubl
   using F = Fft<R, NY, +1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
+  TwReg<R, TL::NN> twr;
+  twr.load(lane, tw);
   const int x = blockIdx.x * TL::S + s, z = blockIdx.y;
   const bool ok = x < g.W;
   const int mx = mirror(x, g.nx);
@@ -260,7 +269,7 @@ k_yb_adv(Geo g, int outs, const cplx<double>* __restrict__ psi, const cplx<doubl
         }
         v[it][t] = cvt<R>(herm(a, am, g.inv_n));
       }
-    F::run(v, sl, lane, tw);
+    F::run(v, sl, lane, twr);
 #pragma unroll
     for (int it = 0; it < F::IT; ++it)
 #pragma unroll
@@ -298,6 +307,8 @@ k_x_c2r(Geo g, int nfields, const cplx<R>* __restrict__ P0, const cplx<R>* __res
   using FI = Fft<R, NX, +1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
+  TwReg<R, TL::NN> twr;
+  twr.load(lane, tw);
   const int row = blockIdx.x * TL::S + r;
   const bool ok = row < g.nz * g.ny;
   cplx<R>* sl = lds + r * TL::LD;
@@ -306,7 +317,7 @@ k_x_c2r(Geo g, int nfields, const cplx<R>* __restrict__ P0, const cplx<R>* __res
     const cplx<R>* src = f == 0 ? P0 : P1;
     R* dst = f == 0 ? o0 : o1;
     load_herm_row<R, FI>(v, src, (size_t)row * g.Wp, NX, lane, ok);
-    FI::run(v, sl, lane, tw);
+    FI::run(v, sl, lane, twr);
 #pragma unroll
     for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
@@ -327,13 +338,15 @@ k_xb_q(Geo g, const cplx<R>* __restrict__ Pq, const R* __restrict__ u, const R* 
   using FF = Fft<R, NX, -1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
+  TwReg<R, TL::NN> twr;
+  twr.load(lane, tw);
   const int row = blockIdx.x * TL::S + r;
   const bool ok = row < g.nz * g.ny;
   cplx<R>* sl = lds + r * TL::LD;
   cplx<R> v[FI::IT][FI::R0];
   R qv[FI::IT][FI::R0];
   load_herm_row<R, FI>(v, Pq, (size_t)row * g.Wp, NX, lane, ok);
-  FI::run(v, sl, lane, tw);
+  FI::run(v, sl, lane, twr);
 #pragma unroll
   for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
@@ -350,7 +363,7 @@ k_xb_q(Geo g, const cplx<R>* __restrict__ Pq, const R* __restrict__ u, const R* 
         R c = (ok && FI::valid(lane, it)) ? vel[(size_t)row * NX + x] : (R)0;
         v[it][t] = mk<R>(c * qv[it][t], (R)0);
       }
-    FF::run(v, sl, lane, tw);
+    FF::run(v, sl, lane, twr);
 #pragma unroll
     for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
@@ -373,6 +386,8 @@ k_xb_B(Geo g, int flags, const cplx<R>* __restrict__ T1, const cplx<R>* __restri
   using FF = Fft<R, NX, -1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
+  TwReg<R, TL::NN> twr;
+  twr.load(lane, tw);
   const int row = blockIdx.x * TL::S + r;
   const bool ok = row < g.nz * g.ny;
   cplx<R>* sl = lds + r * TL::LD;
@@ -386,10 +401,10 @@ k_xb_B(Geo g, int flags, const cplx<R>* __restrict__ T1, const cplx<R>* __restri
       int x = FI::elem(lane, it, t);
       b[it][t] = (ok && FI::valid(lane, it)) ? T1[base + x] : czero<R>();
     }
-  FI::run(b, sl, lane, tw);
+  FI::run(b, sl, lane, twr);
   if (flags & XB_REFR) {
     load_herm_row<R, FI>(v, Pz, (size_t)row * g.Wp, NX, lane, ok);
-    FI::run(v, sl, lane, tw);
+    FI::run(v, sl, lane, twr);
 #pragma unroll
     for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
@@ -409,7 +424,7 @@ k_xb_B(Geo g, int flags, const cplx<R>* __restrict__ T1, const cplx<R>* __restri
         else if (ok && FI::valid(lane, it)) c = (f == 0 ? u : vv)[base + x];
         v[it][t] = mk<R>(c * b[it][t].re, c * b[it][t].im);
       }
-    FF::run(v, sl, lane, tw);
+    FF::run(v, sl, lane, twr);
 #pragma unroll
     for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
@@ -459,6 +474,8 @@ k_yb2_q(Geo g, int mode, int flags, const cplx<R>* __restrict__ G1, const cplx<R
   using F = Fft<R, NY, -1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
+  TwReg<R, TL::NN> twr;
+  twr.load(lane, tw);
   const int x = blockIdx.x * TL::S + s, z = blockIdx.y;
   const bool ok = x < g.W;
   cplx<R>* sl = lds + s * TL::LD;
@@ -474,8 +491,8 @@ k_yb2_q(Geo g, int mode, int flags, const cplx<R>* __restrict__ G1, const cplx<R
         a[it][t] = in ? G1[ad] : czero<R>();
         b[it][t] = in ? G2[ad] : czero<R>();
       }
-    F::run(a, sl, lane, tw);
-    F::run(b, sl, lane, tw);
+    F::run(a, sl, lane, twr);
+    F::run(b, sl, lane, twr);
   }
   const int mx = mirror(x, g.nx);
   const size_t plane = (size_t)g.nx * NY;
@@ -535,6 +552,8 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
   using F = Fft<R, NY, -1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
+  TwReg<R, TL::NN> twr;
+  twr.load(lane, tw);
   const int x = blockIdx.x * TL::S + s, z = blockIdx.y;
   const bool ok = x < g.nx;
   cplx<R>* sl = lds + s * TL::LD;
@@ -557,7 +576,7 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
         bool in = ok && F::valid(lane, it);
         v[it][t] = in ? src[((size_t)z * NY + y) * g.nx + x] : czero<R>();
       }
-    F::run(v, sl, lane, tw);
+    F::run(v, sl, lane, twr);
     if (f == 2) break;  // v keeps F(zeta*B) for the combine below
 #pragma unroll
     for (int it = 0; it < F::IT; ++it)
@@ -625,6 +644,8 @@ k_yf(Geo g, int want_omega, const cplx<double>* __restrict__ psi, cplx<R>* __res
   using F = Fft<R, NY, +1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
+  TwReg<R, TL::NN> twr;
+  twr.load(lane, tw);
   const int x = blockIdx.x * TL::S + s, z = blockIdx.y;
   const bool ok = x < g.W;
   const int mx = mirror(x, g.nx);
@@ -662,8 +683,9 @@ k_yf(Geo g, int want_omega, const cplx<double>* __restrict__ psi, cplx<R>* __res
         } else {
           cplx<double> pz = czero<double>(), pzm = czero<double>(), pa = p[it][t], pam = pm[it][t];
           if (!top && in) {
-            cplx<double> up = psi[((size_t)(z + 1) * NY + y) * g.nx + x];
-            cplx<double> upm = psi[((size_t)(z + 1) * NY + my) * g.nx + mx];
+            // psi one level up: psi_z forward difference and two-level average
+            const cplx<double> up = psi[((size_t)(z + 1) * NY + y) * g.nx + x];
+            const cplx<double> upm = psi[((size_t)(z + 1) * NY + my) * g.nx + mx];
             pz = mk<double>((up.re - p[it][t].re) / dz, (up.im - p[it][t].im) / dz);
             pzm = mk<double>((upm.re - pm[it][t].re) / dz, (upm.im - pm[it][t].im) / dz);
             pa = scale(up + p[it][t], 0.5);
@@ -687,7 +709,7 @@ k_yf(Geo g, int want_omega, const cplx<double>* __restrict__ psi, cplx<R>* __res
         }
         v[it][t] = cvt<R>(herm(a, am, g.inv_n));
       }
-    F::run(v, sl, lane, tw);
+    F::run(v, sl, lane, twr);
 #pragma unroll
     for (int it = 0; it < F::IT; ++it)
 #pragma unroll
@@ -710,6 +732,8 @@ k_xf(Geo g, int want_omega, const cplx<R>* __restrict__ Pu, const cplx<R>* __res
   using FF = Fft<R, NX, -1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
+  TwReg<R, TL::NN> twr;
+  twr.load(lane, tw);
   const int row = blockIdx.x * TL::S + r;
   const bool ok = row < g.nz * g.ny;
   cplx<R>* sl = lds + r * TL::LD;
@@ -717,7 +741,7 @@ k_xf(Geo g, int want_omega, const cplx<R>* __restrict__ Pu, const cplx<R>* __res
   cplx<R> v[FI::IT][FI::R0];
   for (int f = 0; f < 2; ++f) {
     load_herm_row<R, FI>(v, f == 0 ? Pu : Pv, hb, NX, lane, ok);
-    FI::run(v, sl, lane, tw);
+    FI::run(v, sl, lane, twr);
     R* dst = f == 0 ? u : vv;
 #pragma unroll
     for (int it = 0; it < FI::IT; ++it)
@@ -730,30 +754,30 @@ k_xf(Geo g, int want_omega, const cplx<R>* __restrict__ Pu, const cplx<R>* __res
   if (!want_omega) return;  // uniform
   R p1[FI::IT][FI::R0], tmp[FI::IT][FI::R0];
   load_herm_row<R, FI>(v, Pbx, hb, NX, lane, ok);
-  FI::run(v, sl, lane, tw);
+  FI::run(v, sl, lane, twr);
 #pragma unroll
   for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
     for (int t = 0; t < FI::R0; ++t) tmp[it][t] = v[it][t].re;
   load_herm_row<R, FI>(v, Pxy, hb, NX, lane, ok);
-  FI::run(v, sl, lane, tw);
+  FI::run(v, sl, lane, twr);
 #pragma unroll
   for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
     for (int t = 0; t < FI::R0; ++t) p1[it][t] = tmp[it][t] * v[it][t].re;
   load_herm_row<R, FI>(v, Pby, hb, NX, lane, ok);
-  FI::run(v, sl, lane, tw);
+  FI::run(v, sl, lane, twr);
 #pragma unroll
   for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
     for (int t = 0; t < FI::R0; ++t) tmp[it][t] = v[it][t].re;
   load_herm_row<R, FI>(v, Pxx, hb, NX, lane, ok);
-  FI::run(v, sl, lane, tw);
+  FI::run(v, sl, lane, twr);
 #pragma unroll
   for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
     for (int t = 0; t < FI::R0; ++t) v[it][t] = mk<R>((R)2 * (p1[it][t] - tmp[it][t] * v[it][t].re), (R)0);
-  FF::run(v, sl, lane, tw);
+  FF::run(v, sl, lane, twr);
 #pragma unroll
   for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
@@ -773,6 +797,8 @@ k_y_r2c_mask(Geo g, const cplx<R>* __restrict__ Rw, cplx<double>* __restrict__ o
   using F = Fft<R, NY, -1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
+  TwReg<R, TL::NN> twr;
+  twr.load(lane, tw);
   const int x = blockIdx.x * TL::S + s, z = blockIdx.y;
   const bool ok = x < g.W;
   cplx<R>* sl = lds + s * TL::LD;
@@ -784,7 +810,7 @@ k_y_r2c_mask(Geo g, const cplx<R>* __restrict__ Rw, cplx<double>* __restrict__ o
       int y = F::elem(lane, it, t);
       v[it][t] = (ok && F::valid(lane, it)) ? Rw[((size_t)z * NY + y) * g.Wp + x] : czero<R>();
     }
-  F::run(v, sl, lane, tw);
+  F::run(v, sl, lane, twr);
   const int mx = mirror(x, g.nx);
 #pragma unroll
   for (int it = 0; it < F::IT; ++it)
@@ -808,6 +834,8 @@ k_y_herm(Geo g, const cplx<double>* __restrict__ in, cplx<R>* __restrict__ P,
   using F = Fft<R, NY, +1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
+  TwReg<R, TL::NN> twr;
+  twr.load(lane, tw);
   const int x = blockIdx.x * TL::S + s, z = blockIdx.y;
   const bool ok = x < g.W;
   const int mx = mirror(x, g.nx);
@@ -825,7 +853,7 @@ k_y_herm(Geo g, const cplx<double>* __restrict__ in, cplx<R>* __restrict__ P,
       }
       v[it][t] = cvt<R>(herm(a, am, g.inv_n));
     }
-  F::run(v, sl, lane, tw);
+  F::run(v, sl, lane, twr);
 #pragma unroll
   for (int it = 0; it < F::IT; ++it)
 #pragma unroll
@@ -848,6 +876,8 @@ k_y_c2c(Geo g, const cplx<double>* __restrict__ in, cplx<double>* __restrict__ o
   using F = Fft<R, NY, DIR>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
+  TwReg<R, TL::NN> twr;
+  twr.load(lane, tw);
   const int x = blockIdx.x * TL::S + s, z = blockIdx.y;
   const bool ok = x < g.nx;
   cplx<R>* sl = lds + s * TL::LD;
@@ -861,7 +891,7 @@ k_y_c2c(Geo g, const cplx<double>* __restrict__ in, cplx<double>* __restrict__ o
       v[it][t] = (ok && F::valid(lane, it)) ? cvt<R>(scale(in[((size_t)z * NY + y) * g.nx + x], sc))
                                             : czero<R>();
     }
-  F::run(v, sl, lane, tw);
+  F::run(v, sl, lane, twr);
 #pragma unroll
   for (int it = 0; it < F::IT; ++it)
 #pragma unroll
@@ -881,6 +911,8 @@ k_x_c2c(Geo g, const cplx<double>* __restrict__ in, cplx<double>* __restrict__ o
   using F = Fft<R, NX, DIR>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
+  TwReg<R, TL::NN> twr;
+  twr.load(lane, tw);
   const int row = blockIdx.x * TL::S + r;
   const bool ok = row < g.nz * g.ny;
   cplx<R>* sl = lds + r * TL::LD;
@@ -892,7 +924,7 @@ k_x_c2c(Geo g, const cplx<double>* __restrict__ in, cplx<double>* __restrict__ o
       int x = F::elem(lane, it, t);
       v[it][t] = (ok && F::valid(lane, it)) ? cvt<R>(in[(size_t)row * NX + x]) : czero<R>();
     }
-  F::run(v, sl, lane, tw);
+  F::run(v, sl, lane, twr);
   double m = 0.0;
 #pragma unroll
   for (int it = 0; it < F::IT; ++it)

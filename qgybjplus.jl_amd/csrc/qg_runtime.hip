@@ -54,6 +54,14 @@ struct qg_handle {
   D *psi = nullptr, *A = nullptr, *C = nullptr, *qs = nullptr, *Bs = nullptr, *psis = nullptr,
     *As = nullptr, *rq0 = nullptr, *rB0 = nullptr, *qeff = nullptr, *stg = nullptr;
   double *s0 = nullptr, *s1 = nullptr, *s2 = nullptr;
+  // Thomas elimination factors of the q->psi and B->A operators, and where
+  // (if anywhere) their pivots fall below sqrt(eps): {code, level, column}
+  double *th_den[2] = {nullptr, nullptr}, *th_inv[2] = {nullptr, nullptr}, *th_cp[2] = {nullptr, nullptr};
+  int sing_psi[3] = {0, 0, 0}, sing_A[3] = {0, 0, 0};
+  // recorded dgtsv elimination of the omega-equation columns
+  double *om_fact = nullptr, *om_dp = nullptr, *om_dinv = nullptr;
+  unsigned char* om_piv = nullptr;
+  int sing_omega[3] = {0, 0, 0};
   void *u = nullptr, *v = nullptr, *w = nullptr;
   void *T1 = nullptr, *T3 = nullptr, *G3 = nullptr, *G4 = nullptr, *G5 = nullptr;
   void *Qa = nullptr, *Qb = nullptr, *Pu = nullptr, *Pv = nullptr, *Pq = nullptr, *Pz = nullptr,
@@ -204,6 +212,27 @@ void check_status(qg_handle& h) {
   }
 }
 
+[[noreturn]] void raise_singular(const int* s) {
+  char buf[256];
+  snprintf(buf, sizeof buf,
+           "Singular matrix in Thomas solver at level %d of column %d (|pivot| < sqrt(eps)). "
+           "This may indicate an ill-conditioned elliptic problem from N^2~0 or a degenerate wavenumber.",
+           s[1] + 1, s[2]);
+  throw QgError(QG_ERR_SINGULAR, buf);
+}
+
+// invert_q_to_psi! (src/elliptic.jl:189-314) on device arrays.
+void solve_psi(qg_handle& h, const D* q, D* psi) {
+  if (h.sing_psi[0]) raise_singular(h.sing_psi);
+  qg::launch_thomas_lds(h.g, h.st, q, psi, nullptr, h.th_den[0], h.th_inv[0], h.th_cp[0], h.d_a, 1, h.d_warn);
+}
+
+// invert_B_to_A! (src/elliptic.jl:836-990) on device arrays; C optional.
+void solve_A(qg_handle& h, const D* B, D* A, D* C) {
+  if (h.sing_A[0]) raise_singular(h.sing_A);
+  qg::launch_thomas_lds(h.g, h.st, B, A, C, h.th_den[1], h.th_inv[1], h.th_cp[1], h.d_a, 4, nullptr);
+}
+
 template <typename R>
 struct Engine {
   using C = cplx<R>;
@@ -254,7 +283,7 @@ struct Engine {
         t1 = true;
       }
       K("thomas_psi", P * 2 * cD,
-        [&] { qg::launch_thomas(h.g, h.st, src, psi_out, nullptr, h.s0, h.d_a, 1, h.d_err, h.d_warn); });
+        [&] { solve_psi(h, src, psi_out); });
     }
     if (need_t1 && !t1) t1_only(Bsrc);
   }
@@ -266,7 +295,7 @@ struct Engine {
       return;
     }
     K(Cout ? "thomas_AC" : "thomas_A", P * (Cout ? 3 : 2) * cD,
-      [&] { qg::launch_thomas(h.g, h.st, Bsrc, Aout, Cout, h.s0, h.d_a, 4, h.d_err, nullptr); });
+      [&] { solve_A(h, Bsrc, Aout, Cout); });
   }
 
   void velocities(const D* psi) {
@@ -287,8 +316,15 @@ struct Engine {
             twx());
     });
     K("y_r2c_omega", Ph * cR + P * cD, [&] { O::y_r2c_mask(h.g, h.st, c(h.Pq), h.rq0, twy()); });
+    if (h.sing_omega[0]) {
+      char buf[160];
+      snprintf(buf, sizeof buf, "LAPACK-style gtsv failed for vertical velocity at row %d of column %d",
+               h.sing_omega[1] + 1, h.sing_omega[2]);
+      throw QgError(QG_ERR_GTSV, buf);
+    }
     K("omega_gtsv", P * 2 * cD, [&] {
-      qg::launch_omega(h.g, h.st, h.rq0, h.qeff, h.s0, h.s1, h.s2, h.d_cz, h.d_fN2, h.d_err);
+      qg::launch_omega_tile(h.g, h.st, h.rq0, h.qeff, h.om_fact, h.om_dp, h.om_dinv, h.om_piv, h.d_cz, h.d_fN2, h.s0,
+                            h.s1, h.s2);
     });
     K("y_herm_w", P * cD + Ph * cR, [&] { O::y_herm(h.g, h.st, h.qeff, c(h.Pz), twy()); });
     K("x_c2r_w", Ph * cR + P * rR, [&] { O::x_c2r(h.g, h.st, 1, c(h.Pz), nullptr, r(h.w), nullptr, twx()); });
@@ -601,6 +637,35 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
     g.mask = h->d_mask;
     g.inv_n = 1.0 / ((double)nx * (double)ny);
     g.dz = h->dz;
+    // Thomas factors, computed once: they depend only on (kh2, a, dz)
+    if (!qg::thomas_lds_setup() || !qg::omega_lds_setup())
+      throw QgError(QG_ERR_DEVICE, "cannot reserve the LDS of the column solvers");
+    {
+      const size_t tsz = qg::thomas_table_size(g);
+      for (int o = 0; o < 2; ++o) {
+        h->th_den[o] = h->alloc<double>(tsz);
+        h->th_inv[o] = h->alloc<double>(tsz);
+        h->th_cp[o] = h->alloc<double>(tsz);
+      }
+      int* e = h->alloc<int>(8);
+      HIPCHECK(hipMemset(e, 0, 8 * sizeof(int)));
+      qg::launch_thomas_factor(g, h->st, h->d_a, 1, h->th_den[0], h->th_inv[0], h->th_cp[0], e);
+      qg::launch_thomas_factor(g, h->st, h->d_a, 4, h->th_den[1], h->th_inv[1], h->th_cp[1], e + 4);
+      int he[8];
+      HIPCHECK(hipMemcpyAsync(he, e, sizeof he, hipMemcpyDeviceToHost, h->st));
+      HIPCHECK(hipStreamSynchronize(h->st));
+      std::memcpy(h->sing_psi, he, 3 * sizeof(int));
+      std::memcpy(h->sing_A, he + 4, 3 * sizeof(int));
+      h->om_fact = h->alloc<double>(h->npts);
+      h->om_dp = h->alloc<double>(h->npts);
+      h->om_dinv = h->alloc<double>(h->npts);
+      h->om_piv = h->alloc<unsigned char>(h->plane);
+      HIPCHECK(hipMemset(e, 0, 8 * sizeof(int)));
+      qg::launch_omega_factor(g, h->st, h->d_cz, h->om_fact, h->om_dp, h->om_dinv, h->om_piv, e);
+      HIPCHECK(hipMemcpyAsync(he, e, sizeof he, hipMemcpyDeviceToHost, h->st));
+      HIPCHECK(hipStreamSynchronize(h->st));
+      std::memcpy(h->sing_omega, he, 3 * sizeof(int));
+    }
     HIPCHECK(hipDeviceSynchronize());
     *out = h;
     return QG_OK;
@@ -724,7 +789,7 @@ int qg_check(qg_handle* h, int* nonfinite, double* max_abs_psi) {
 
 int qg_invert_q_to_psi(qg_handle* h) {
   return guarded(h, [&] {
-    qg::launch_thomas(h->g, h->st, h->q[h->cur], h->psi, nullptr, h->s0, h->d_a, 1, h->d_err, h->d_warn);
+    solve_psi(*h, h->q[h->cur], h->psi);
     h->diag_valid = false;
     check_status(*h);
   });
@@ -732,7 +797,7 @@ int qg_invert_q_to_psi(qg_handle* h) {
 
 int qg_invert_B_to_A(qg_handle* h) {
   return guarded(h, [&] {
-    qg::launch_thomas(h->g, h->st, h->B[h->cur], h->A, h->C, h->s0, h->d_a, 4, h->d_err, nullptr);
+    solve_A(*h, h->B[h->cur], h->A, h->C);
     h->diag_valid = false;
     check_status(*h);
   });

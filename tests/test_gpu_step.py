@@ -124,7 +124,8 @@ def test_config1_plumbing_100_steps(qg):
 
 
 @pytest.mark.parametrize("variant", ["feedback", "nofeedback", "fixed", "passive", "linear",
-                                     "nodisp", "inviscid", "nu_z", "flow_closure", "nonsquare"])
+                                     "nodisp", "inviscid", "nu_z", "flow_closure", "nonsquare",
+                                     "omega_pivot"])
 def test_step_variants_match_oracle(qg, variant):
     base = dict(size=(32, 32, 12), extent=(500e3, 500e3, 4000.0), N2_func=EXP_N2, feedback=True)
     if variant == "nofeedback":
@@ -146,6 +147,9 @@ def test_step_variants_match_oracle(qg, variant):
     elif variant == "nonsquare":
         base["size"] = (64, 32, 8)
         base["extent"] = (600e3, 300e3, 3000.0)
+    elif variant == "omega_pivot":
+        # N² falling 2.3x per level towards the surface: dgtsv interchanges rows
+        base["N2_func"] = lambda z: 1e-5 * math.exp(-z / 400.0)
     c = Case(**base)
     print(f"case step variant={variant}", flush=True)
     m, o = gpu_model(qg, c), oracle_model(c)
