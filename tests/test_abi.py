@@ -1,0 +1,78 @@
+"""The C ABI boundary without a GPU: the library loads, exports every symbol
+include/qgybj.h declares, and the ctypes mirror of qg_config has the C
+layout.  No compute call is made."""
+
+import ctypes
+import os
+import re
+import subprocess
+import tempfile
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "qgybj.h")
+LIB = os.path.join(ROOT, "qgybjplus.jl_amd", "libqgybj.so")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(qg_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_declares_the_entry_points():
+    names = declared_functions()
+    assert "qg_create" in names and "qg_step" in names and "qg_check" in names
+    assert len(names) >= 17
+
+
+def test_binding_lists_every_declared_symbol(qg):
+    from importlib import import_module
+    lib_mod = import_module(qg.__name__ + "._lib")
+    assert sorted(lib_mod.EXPORTED) == declared_functions()
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libqgybj.so not built")
+def test_library_exports_every_declared_symbol():
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    missing = [n for n in declared_functions() if n not in exported]
+    assert not missing, missing
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libqgybj.so not built")
+def test_library_loads_and_resolves_symbols():
+    lib = ctypes.CDLL(LIB)
+    for n in declared_functions():
+        assert getattr(lib, n) is not None
+
+
+def test_qg_config_layout_matches_c(qg):
+    """Compile a probe against the header and compare sizeof/offsetof with the
+    ctypes structure the binding passes to qg_create."""
+    from importlib import import_module
+    lib_mod = import_module(qg.__name__ + "._lib")
+    fields = [f for f, _ in lib_mod.QgConfig._fields_]
+    body = "\n".join(f'  printf("{f} %zu\\n", offsetof(qg_config, {f}));' for f in fields)
+    src = ("#include <stdio.h>\n#include <stddef.h>\n#include \"qgybj.h\"\nint main(void) {\n"
+           f'  printf("sizeof %zu\\n", sizeof(qg_config));\n{body}\n  return 0;\n}}\n')
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "probe.c")
+        exe = os.path.join(d, "probe")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout
+    got = dict((k, int(v)) for k, v in (line.split() for line in out.splitlines()))
+    assert got["sizeof"] == ctypes.sizeof(lib_mod.QgConfig)
+    for f in fields:
+        assert got[f] == getattr(lib_mod.QgConfig, f).offset, f
+
+
+def test_header_compiles_as_c_and_cpp():
+    with tempfile.TemporaryDirectory() as d:
+        for comp, ext in (("gcc", "c"), ("g++", "cpp")):
+            p = os.path.join(d, "inc." + ext)
+            open(p, "w").write('#include "qgybj.h"\nint main(void) { return QG_OK; }\n')
+            subprocess.run([comp, "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), p, "-o",
+                            os.path.join(d, "inc_" + ext)], check=True)
