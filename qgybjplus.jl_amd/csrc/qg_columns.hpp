@@ -5,29 +5,33 @@
 
 namespace qg {
 
-// LDS budget of one column-solver tile (TC columns x nz levels, FP64 complex,
-// plus the per-level coefficients); two tiles fit in a CU's 160 KB.
-constexpr size_t kThomasLdsBytes = 72 * 1024;
-int thomas_tile_columns(int nz);
-bool thomas_lds_setup();
-// Thomas elimination tables (pivot, its reciprocal, c') of every column,
-// stored for the quarter of wavenumbers |ix| <= nx/2, |iy| <= ny/2
-// (kdiv 1: q->psi, 4: B->A); thomas_table_size() entries each.
+// Column solves run as mirror tiles (qg_mirror.hpp); depths up to 1024
+// levels are supported.
+bool column_depth_supported(int nz);
+// Thomas elimination tables (1/pivot, c') of the quarter columns
+// |ix| <= nx/2, |iy| <= ny/2, level-indexed [k][rc] (kdiv 1: q->psi,
+// 4: B->A); thomas_table_size() entries each.
 size_t thomas_table_size(const Geo& g);
-void launch_thomas_factor(const Geo& g, hipStream_t st, const double* a, int kdiv, double* den, double* inv,
-                          double* cp, int* err);
+void launch_thomas_factor(const Geo& g, hipStream_t st, const double* a, int kdiv, double* inv, double* cp,
+                          int* err);
+bool thomas_setup();
+// `al` is a shifted by one level (al[k] = a[k-1], al[0] = 0), zero-padded to
+// thomas_shifted_a_size(nz) entries.
+int thomas_shifted_a_size(int nz);
 // Thomas solves of all columns; C (= dA/dz) is written when Cout is set.
-void launch_thomas_lds(const Geo& g, hipStream_t st, const cplx<double>* rhs, cplx<double>* out,
-                       cplx<double>* Cout, const double* den, const double* inv, const double* cp,
-                       const double* a, int kdiv, unsigned long long* warn);
-// QG omega-equation interior solves: elimination recorded once per model,
-// then applied to every step's right-hand sides.
-void launch_omega_factor(const Geo& g, hipStream_t st, const double* cz, double* fact, double* dp, double* dinv,
+void launch_thomas(const Geo& g, hipStream_t st, const cplx<double>* rhs, cplx<double>* out,
+                   cplx<double>* Cout, const double* inv, const double* cp, const double* al, int kdiv,
+                   unsigned long long* warn);
+// QG omega-equation interior solves: elimination recorded once per model
+// (tables of thomas_table_size() entries, pivoted flags per quarter column),
+// then applied to every step's right-hand sides.  cu: upper diagonal by level,
+// zero-padded to thomas_shifted_a_size(nz) entries.
+void launch_omega_factor(const Geo& g, hipStream_t st, const double* cz, double* M, double* D,
                          unsigned char* pivoted, int* err);
-bool omega_lds_setup();
-void launch_omega_tile(const Geo& g, hipStream_t st, const cplx<double>* rhsw, cplx<double>* wk,
-                       const double* fact, const double* dp, const double* dinv, const unsigned char* pivoted,
-                       const double* cz, const double* fN2, double* sd, double* sdu, double* sfill);
+bool omega_setup();
+void launch_omega(const Geo& g, hipStream_t st, const cplx<double>* rhsw, cplx<double>* wk, const double* M,
+                  const double* D, const unsigned char* pivoted, const double* cz, const double* cu,
+                  const double* fN2, double* sd, double* sdu, double* sfill);
 void launch_q_from_psi(const Geo& g, hipStream_t st, const cplx<double>* psi, cplx<double>* q,
                        const double* a, int barotropic);
 template <typename T>

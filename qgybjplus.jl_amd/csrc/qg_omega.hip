@@ -1,33 +1,25 @@
-// Interior solve of the QG omega equation (operators.jl:415-470), batched
-// over columns.
+// Interior solve of the QG omega equation (compute_vertical_velocity!,
+// src/operators.jl:415-470), batched over columns.
 //
 // The reference calls LAPACK dgtsv (Gaussian elimination with partial
 // pivoting) once for the real and once for the imaginary part of each column.
 // The elimination depends only on (kh2, f^2/N^2, dz), so k_omega_factor runs
-// it once per model and records, per interior row, the multiplier and the
-// eliminated diagonal; a column whose elimination interchanged rows is marked
-// and solved by the general path.  Per step, k_omega_tile stages a tile of
-// right-hand sides in LDS, applies the recorded elimination to them and back
-// substitutes: the same operations on the right-hand side, in the same order,
-// as dgtsv.  FMA contraction is off.
+// it once per model for the quarter columns and records, level-indexed, the
+// multiplier applied to each row and the reciprocal of its eliminated pivot;
+// a column whose elimination interchanged rows is marked and solved by the
+// general path, a restatement of dgtsv.  Per step, k_omega is a mirror-tile
+// kernel (qg_mirror.hpp): it applies the recorded elimination to the
+// right-hand sides, b_k -= m_k b_{k-1}, and back substitutes
+// x_k = (b_k - u_k x_{k+1}) / d_k as fma(-(u_k / d_k), x_{k+1}, b_k / d_k),
+// with the quotients formed from the stored reciprocals.
 #include "qg_columns.hpp"
+#include "qg_mirror.hpp"
 
 #pragma clang fp contract(off)
 
 namespace qg {
 
 namespace {
-
-constexpr int kThreads = 256;
-constexpr int kPF = 16;
-constexpr int kBatch = 16;
-
-// n / d given y = RN(1/d): q0 = n*y, then one residual correction.
-__device__ inline double quot(double n, double d, double y) {
-  const double q0 = n * y;
-  const double e = fma(-d, q0, n);
-  return fma(e, y, q0);
-}
 
 __device__ inline void flag_error(int* err, int code, int level, int col) {
   if (atomicCAS(err, 0, code) == 0) {
@@ -36,22 +28,30 @@ __device__ inline void flag_error(int* err, int code, int level, int col) {
   }
 }
 
-// Per column, rows iz = 0..n-1 (levels k = iz+1), n = nz-2:
-//   fact[iz] (iz < n-1): multiplier applied to row iz+1,
-//   dp[iz]: eliminated diagonal,  pivoted[col]: any row interchange.
-__global__ void k_omega_factor(Geo g, const double* __restrict__ cz, double* __restrict__ fact,
-                               double* __restrict__ dp, double* __restrict__ dinv,
-                               unsigned char* __restrict__ pivoted, int* __restrict__ err) {
-  const int plane = g.nx * g.ny;
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= plane) return;
-  const int x = col % g.nx, y = col / g.nx;
+// Tables [k][rc] over levels k = 0..nz-1 (interior row iz = k-1):
+//   M[k]: multiplier dgtsv applies to row k-1 from row k-2 (0 unless
+//         2 <= k <= nz-2),
+//   D[k]: 1 / eliminated pivot of row k-1 (0 unless 1 <= k <= nz-2 and
+//         kh2 > 0),
+//   pivoted[rc]: any row interchange (dgtsv INFO > 0 is recorded in err,
+//   code 2).
+__global__ void k_omega_factor(Geo g, Quarter qr, const double* __restrict__ cz, double* __restrict__ M,
+                               double* __restrict__ D, unsigned char* __restrict__ pivoted,
+                               int* __restrict__ err) {
+  const int rplane = qr.size();
+  const int rc = blockIdx.x * blockDim.x + threadIdx.x;
+  if (rc >= rplane) return;
+  const int x = rc % qr.Wr, y = rc / qr.Wr;
   const int nz = g.nz;
+  auto at = [&](int k) { return (size_t)k * rplane + rc; };
+  for (int k = 0; k < nz; ++k) {
+    M[at(k)] = 0.0;
+    D[at(k)] = 0.0;
+  }
+  pivoted[rc] = 0;
   const double kh2 = g.kx[x] * g.kx[x] + g.ky[y] * g.ky[y];
-  pivoted[col] = 0;
   if (!(kh2 > 0.0 && nz > 2)) return;
   const int n = nz - 2;
-  auto at = [&](int iz) { return (size_t)iz * plane + col; };
   double d = -2 * cz[1] - kh2;
   double du = (n > 1) ? cz[1] : 0.0;
   bool piv = false;
@@ -60,10 +60,10 @@ __global__ void k_omega_factor(Geo g, const double* __restrict__ cz, double* __r
     double dn = -2 * cz[i + 2] - kh2;
     double dun = (i + 1 < n - 1) ? cz[i + 2] : 0.0;
     if (fabs(d) >= fabs(dl)) {
-      if (d == 0.0) flag_error(err, 2, i, col);
+      if (d == 0.0) flag_error(err, 2, i, y * g.nx + x);
       const double f = dl / d;
       dn = dn - f * du;
-      fact[at(i)] = f;
+      M[at(i + 2)] = f;
     } else {
       piv = true;
       const double f = d / dl;
@@ -72,17 +72,14 @@ __global__ void k_omega_factor(Geo g, const double* __restrict__ cz, double* __r
       if (i < n - 2) dun = -f * dun;
       du = temp;
       d = dl;
-      fact[at(i)] = f;
     }
-    dp[at(i)] = d;
-    dinv[at(i)] = 1.0 / d;
+    D[at(i + 1)] = 1.0 / d;
     d = dn;
     du = dun;
   }
-  if (d == 0.0) flag_error(err, 2, n - 1, col);
-  dp[at(n - 1)] = d;
-  dinv[at(n - 1)] = 1.0 / d;
-  pivoted[col] = piv ? 1 : 0;
+  if (d == 0.0) flag_error(err, 2, n - 1, y * g.nx + x);
+  D[at(n)] = 1.0 / d;
+  pivoted[rc] = piv ? 1 : 0;
 }
 
 // General dgtsv path (row interchanges) for one column; b (complex) in LDS
@@ -158,139 +155,95 @@ __device__ void gtsv_general(double* b0, int TC2, int n, double kh2, const doubl
   }
 }
 
-__global__ void __launch_bounds__(kThreads)
-k_omega_tile(Geo g, const cplx<double>* __restrict__ rhsw, cplx<double>* __restrict__ wk,
-             const double* __restrict__ fact, const double* __restrict__ dp,
-             const double* __restrict__ dinv, const unsigned char* __restrict__ pivoted, const double* __restrict__ cz,
-             const double* __restrict__ fN2, double* __restrict__ sd, double* __restrict__ sdu,
-             double* __restrict__ sfill, int TC) {
-  extern __shared__ cplx<double> bs[];  // [nz][TC] then cz[nz]
-  const int plane = g.nx * g.ny;
-  const int nz = g.nz;
-  const int n = nz - 2;
-  const int col0 = blockIdx.x * TC;
-  const int ncol = min(TC, plane - col0);
-  double* czl = reinterpret_cast<double*>(bs + (size_t)nz * TC);
-
-  // phase 1: b = (f/N2[k]) * RHS[k] on interior levels (operators.jl:439)
-  for (int base = 0; base < nz * TC; base += kThreads * kBatch) {
-    cplx<double> v[kBatch];
-#pragma unroll
-    for (int u = 0; u < kBatch; ++u) {
-      const int e = base + u * kThreads + threadIdx.x;
-      const int k = e / TC, c = e % TC;
-      v[u] = (e < nz * TC && c < ncol && k >= 1 && k <= nz - 2) ? rhsw[(size_t)k * plane + col0 + c]
-                                                                 : mk(0.0, 0.0);
-    }
-#pragma unroll
-    for (int u = 0; u < kBatch; ++u) {
-      const int e = base + u * kThreads + threadIdx.x;
-      if (e < nz * TC) {
-        const double s = fN2[e / TC];
-        bs[e] = mk(s * v[u].re, s * v[u].im);
-      }
-    }
-  }
-  for (int k = threadIdx.x; k < nz; k += kThreads) czl[k] = cz[k];
+// rhsw: the omega right-hand side (complex, all levels); wk: w-hat, zero on
+// the end levels and where kh = 0.  cu[k] = cz[k] for 1 <= k <= nz-3 (the
+// upper diagonal of row k-1), zero elsewhere, padded to mirror_pad(nz).
+template <int QC>
+__global__ void __launch_bounds__(kMirrorThreads)
+k_omega(Geo g, Quarter qr, const cplx<double>* __restrict__ rhsw, cplx<double>* __restrict__ wk,
+        const double* __restrict__ M, const double* __restrict__ D, const unsigned char* __restrict__ pivoted,
+        const double* __restrict__ cz, const double* __restrict__ cu, const double* __restrict__ fN2,
+        double* __restrict__ sd, double* __restrict__ sdu, double* __restrict__ sfill) {
+  constexpr int NC = 4 * QC;
+  extern __shared__ cplx<double> xs[];  // [kpad][NC], then tm[kpad][QC], td[kpad][QC]
+  const int nz = g.nz, kpad = mirror_pad(nz);
+  const int rc0 = blockIdx.x * QC;
+  double* tm = reinterpret_cast<double*>(xs + (size_t)kpad * NC);
+  double* td = tm + (size_t)kpad * QC;
+  const MirrorSlot<QC> sl(g, qr, rc0);
+  // b = (f / N2[k]) * RHS[k] on the interior levels (src/operators.jl:439)
+  mirror_stage<QC>(g, qr, rc0, sl, rhsw, M, D, xs, tm, td, kpad,
+                   [&](int k) { return (k >= 1 && k <= nz - 2) ? fN2[k] : 0.0; });
   __syncthreads();
 
   const int lane = threadIdx.x;
-  if (lane < 2 * TC && n >= 1) {
-    const int c = lane % TC, part = lane / TC;
-    const int col = col0 + c;
-    if (c < ncol) {
-      const int x = col % g.nx, y = col / g.nx;
-      const double kh2 = g.kx[x] * g.kx[x] + g.ky[y] * g.ky[y];
-      double* b0 = reinterpret_cast<double*>(bs + (size_t)1 * TC + c) + part;  // row iz -> b0[iz*2TC]
-      const int TC2 = 2 * TC;
-      auto B = [&](int iz) -> double& { return b0[(size_t)iz * TC2]; };
-      auto at = [&](int iz) { return (size_t)iz * plane + col; };
-      if (!(kh2 > 0.0)) {
-        for (int iz = 0; iz < n; ++iz) B(iz) = 0.0;
-      } else if (pivoted[col]) {
-        if (part == 0) gtsv_general(b0, TC2, n, kh2, czl, sd, sdu, sfill, plane, col);
+  if (lane < 2 * NC) {
+    const int cs = lane >> 1, part = lane & 1;
+    const int rc = rc0 + cs % QC;
+    const QuadCol q = quad_col(g, qr, rc, cs / QC);
+    if (q.active && nz > 2) {
+      double* b0 = reinterpret_cast<double*>(xs + cs) + part;  // level k at b0[k * 2NC]
+      const int st = 2 * NC;
+      if (pivoted[rc]) {
+        const double kh2 = g.kx[q.x] * g.kx[q.x] + g.ky[q.y] * g.ky[q.y];
+        if (part == 0) gtsv_general(b0 + st, st, nz - 2, kh2, cz, sd, sdu, sfill, g.nx * g.ny, q.y * g.nx + q.x);
       } else {
-        // forward: b[i+1] -= fact[i] * b[i]; kPF rows between LDS and
-        // registers at a time
-        double bprev = B(0);
-        for (int i0 = 0; i0 < n - 1; i0 += kPF) {
-          double ff[kPF], bb[kPF];
-#pragma unroll
-          for (int j = 0; j < kPF; ++j) {
-            const bool in = i0 + j < n - 1;
-            ff[j] = in ? fact[at(i0 + j)] : 0.0;
-            bb[j] = in ? B(i0 + j + 1) : 0.0;
-          }
-#pragma unroll
-          for (int j = 0; j < kPF; ++j) {
-            const double bn = bb[j] - ff[j] * bprev;
-            if (i0 + j < n - 1) bprev = bn;
-            bb[j] = bn;
-          }
-#pragma unroll
-          for (int j = 0; j < kPF; ++j)
-            if (i0 + j < n - 1) B(i0 + j + 1) = bb[j];
-        }
-        // back substitution: du[i] = c_i = cz[i+1] (no interchange)
-        double x1 = quot(bprev, dp[at(n - 1)], dinv[at(n - 1)]);
-        B(n - 1) = x1;
-        for (int i0 = n - 2; i0 >= 0; i0 -= kPF) {
-          double dd[kPF], yy[kPF], bb[kPF], uu[kPF];
-#pragma unroll
-          for (int j = 0; j < kPF; ++j) {
-            const int i = i0 - j;
-            const bool in = i >= 0;
-            dd[j] = in ? dp[at(i)] : 1.0;
-            yy[j] = in ? dinv[at(i)] : 1.0;
-            bb[j] = in ? B(i) : 0.0;
-            uu[j] = in ? czl[i + 1] : 0.0;
-          }
-#pragma unroll
-          for (int j = 0; j < kPF; ++j) {
-            const double xv = quot(bb[j] - uu[j] * x1, dd[j], yy[j]);
-            if (i0 - j >= 0) x1 = xv;
-            bb[j] = xv;
-          }
-#pragma unroll
-          for (int j = 0; j < kPF; ++j)
-            if (i0 - j >= 0) B(i0 - j) = bb[j];
-        }
+        const double* tM = tm + cs % QC;
+        const double* tD = td + cs % QC;
+        mirror_sweeps(
+            b0, st, kpad, [&](int k, double b) { return make_double2(tM[k * QC], b); },
+            [&](int k, double bp) {
+              const double r = tD[k * QC];
+              return make_double2(cu[k] * r, bp * r);
+            });
       }
     }
   }
   __syncthreads();
 
-  // phase 3: w-hat; zero on the end levels and on kh = 0
-  for (int e = threadIdx.x; e < nz * TC; e += kThreads) {
-    const int k = e / TC, c = e % TC;
-    if (c >= ncol) continue;
-    wk[(size_t)k * plane + col0 + c] = bs[e];
-  }
+  if (!sl.q.active) return;
+  const size_t plane = (size_t)g.nx * g.ny;
+  for (int k = sl.kf; k < nz; k += MirrorSlot<QC>::kstep) wk[(size_t)k * plane + sl.off] = xs[k * NC + sl.c];
+}
+
+template <int QC>
+void launch_qc(const Geo& g, hipStream_t st, const cplx<double>* rhsw, cplx<double>* wk, const double* M,
+               const double* D, const unsigned char* pivoted, const double* cz, const double* cu, const double* fN2,
+               double* sd, double* sdu, double* sfill) {
+  const Quarter qr = quarter(g);
+  const int blocks = (qr.size() + QC - 1) / QC;
+  hipLaunchKernelGGL((k_omega<QC>), dim3(blocks), dim3(kMirrorThreads), mirror_lds_bytes(mirror_pad(g.nz), QC), st,
+                     g, qr, rhsw, wk, M, D, pivoted, cz, cu, fN2, sd, sdu, sfill);
+}
+
+template <int QC>
+bool set_lds_attr() {
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(k_omega<QC>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)kMirrorLdsBytes) == hipSuccess;
 }
 
 }  // namespace
 
-void launch_omega_factor(const Geo& g, hipStream_t st, const double* cz, double* fact, double* dp, double* dinv,
+void launch_omega_factor(const Geo& g, hipStream_t st, const double* cz, double* M, double* D,
                          unsigned char* pivoted, int* err) {
-  const int plane = g.nx * g.ny;
+  const Quarter qr = quarter(g);
   const int bs = 64;
-  hipLaunchKernelGGL(k_omega_factor, dim3((plane + bs - 1) / bs), dim3(bs), 0, st, g, cz, fact, dp, dinv,
-                     pivoted, err);
+  hipLaunchKernelGGL(k_omega_factor, dim3((qr.size() + bs - 1) / bs), dim3(bs), 0, st, g, qr, cz, M, D, pivoted,
+                     err);
 }
 
-bool omega_lds_setup() {
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(k_omega_tile),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)kThomasLdsBytes) == hipSuccess;
-}
+bool omega_setup() { return set_lds_attr<8>() && set_lds_attr<4>() && set_lds_attr<2>() && set_lds_attr<1>(); }
 
-void launch_omega_tile(const Geo& g, hipStream_t st, const cplx<double>* rhsw, cplx<double>* wk,
-                       const double* fact, const double* dp, const double* dinv, const unsigned char* pivoted,
-                       const double* cz, const double* fN2, double* sd, double* sdu, double* sfill) {
-  const int plane = g.nx * g.ny;
-  const int tc = thomas_tile_columns(g.nz);
-  const size_t lds = (size_t)tc * g.nz * sizeof(cplx<double>) + g.nz * sizeof(double);
-  hipLaunchKernelGGL(k_omega_tile, dim3((plane + tc - 1) / tc), dim3(kThreads), lds, st, g, rhsw, wk, fact, dp,
-                     dinv, pivoted, cz, fN2, sd, sdu, sfill, tc);
+void launch_omega(const Geo& g, hipStream_t st, const cplx<double>* rhsw, cplx<double>* wk, const double* M,
+                  const double* D, const unsigned char* pivoted, const double* cz, const double* cu,
+                  const double* fN2, double* sd, double* sdu, double* sfill) {
+  switch (mirror_qc(g.nz)) {
+    case 8: launch_qc<8>(g, st, rhsw, wk, M, D, pivoted, cz, cu, fN2, sd, sdu, sfill); return;
+    case 4: launch_qc<4>(g, st, rhsw, wk, M, D, pivoted, cz, cu, fN2, sd, sdu, sfill); return;
+    case 2: launch_qc<2>(g, st, rhsw, wk, M, D, pivoted, cz, cu, fN2, sd, sdu, sfill); return;
+    case 1: launch_qc<1>(g, st, rhsw, wk, M, D, pivoted, cz, cu, fN2, sd, sdu, sfill); return;
+    default: return;  // rejected by column_depth_supported at model creation
+  }
 }
 
 }  // namespace qg

@@ -45,7 +45,8 @@ struct qg_handle {
   size_t plane = 0, npts = 0, nhalf = 0;
   std::vector<void*> allocs;
   // device
-  double *d_kx = nullptr, *d_ky = nullptr, *d_a = nullptr, *d_cz = nullptr, *d_fN2 = nullptr;
+  double *d_kx = nullptr, *d_ky = nullptr, *d_a = nullptr, *d_al = nullptr, *d_cz = nullptr, *d_cu = nullptr,
+         *d_fN2 = nullptr;
   unsigned char* d_mask = nullptr;
   double* d_etd = nullptr;
   double etd_dt = std::nan("");
@@ -56,10 +57,10 @@ struct qg_handle {
   double *s0 = nullptr, *s1 = nullptr, *s2 = nullptr;
   // Thomas elimination factors of the q->psi and B->A operators, and where
   // (if anywhere) their pivots fall below sqrt(eps): {code, level, column}
-  double *th_den[2] = {nullptr, nullptr}, *th_inv[2] = {nullptr, nullptr}, *th_cp[2] = {nullptr, nullptr};
+  double *th_inv[2] = {nullptr, nullptr}, *th_cp[2] = {nullptr, nullptr};
   int sing_psi[3] = {0, 0, 0}, sing_A[3] = {0, 0, 0};
   // recorded dgtsv elimination of the omega-equation columns
-  double *om_fact = nullptr, *om_dp = nullptr, *om_dinv = nullptr;
+  double *om_M = nullptr, *om_D = nullptr;
   unsigned char* om_piv = nullptr;
   int sing_omega[3] = {0, 0, 0};
   void *u = nullptr, *v = nullptr, *w = nullptr;
@@ -224,13 +225,13 @@ void check_status(qg_handle& h) {
 // invert_q_to_psi! (src/elliptic.jl:189-314) on device arrays.
 void solve_psi(qg_handle& h, const D* q, D* psi) {
   if (h.sing_psi[0]) raise_singular(h.sing_psi);
-  qg::launch_thomas_lds(h.g, h.st, q, psi, nullptr, h.th_den[0], h.th_inv[0], h.th_cp[0], h.d_a, 1, h.d_warn);
+  qg::launch_thomas(h.g, h.st, q, psi, nullptr, h.th_inv[0], h.th_cp[0], h.d_al, 1, h.d_warn);
 }
 
 // invert_B_to_A! (src/elliptic.jl:836-990) on device arrays; C optional.
 void solve_A(qg_handle& h, const D* B, D* A, D* C) {
   if (h.sing_A[0]) raise_singular(h.sing_A);
-  qg::launch_thomas_lds(h.g, h.st, B, A, C, h.th_den[1], h.th_inv[1], h.th_cp[1], h.d_a, 4, nullptr);
+  qg::launch_thomas(h.g, h.st, B, A, C, h.th_inv[1], h.th_cp[1], h.d_al, 4, nullptr);
 }
 
 template <typename R>
@@ -323,8 +324,8 @@ struct Engine {
       throw QgError(QG_ERR_GTSV, buf);
     }
     K("omega_gtsv", P * 2 * cD, [&] {
-      qg::launch_omega_tile(h.g, h.st, h.rq0, h.qeff, h.om_fact, h.om_dp, h.om_dinv, h.om_piv, h.d_cz, h.d_fN2, h.s0,
-                            h.s1, h.s2);
+      qg::launch_omega(h.g, h.st, h.rq0, h.qeff, h.om_M, h.om_D, h.om_piv, h.d_cz, h.d_cu, h.d_fN2, h.s0, h.s1,
+                       h.s2);
     });
     K("y_herm_w", P * cD + Ph * cR, [&] { O::y_herm(h.g, h.st, h.qeff, c(h.Pz), twy()); });
     K("x_c2r_w", Ph * cR + P * rR, [&] { O::x_c2r(h.g, h.st, 1, c(h.Pz), nullptr, r(h.w), nullptr, twx()); });
@@ -497,6 +498,8 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
       throw QgError(QG_ERR_UNSUPPORTED,
                     "horizontal sizes must be one of 8,16,32,64,128,256,384,512 (got " +
                         std::to_string(cfg->nx) + "x" + std::to_string(cfg->ny) + ")");
+    if (!qg::column_depth_supported(cfg->nz))
+      throw QgError(QG_ERR_UNSUPPORTED, "nz must be between 1 and 1024 for the column solvers");
     if (cfg->formulation == QG_YBJ)
       throw QgError(QG_ERR_UNSUPPORTED, "the normal YBJ formulation is not part of this library; use YBJ+");
     if (cfg->formulation != QG_YBJ_PLUS && cfg->formulation != QG_PASSIVE)
@@ -571,7 +574,20 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
     HIPCHECK(hipMemcpy(h->d_kx, h->kx.data(), nx * sizeof(double), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(h->d_ky, h->ky.data(), ny * sizeof(double), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(h->d_a, h->a.data(), nz * sizeof(double), hipMemcpyHostToDevice));
+    {
+      std::vector<double> al(qg::thomas_shifted_a_size(nz), 0.0);
+      for (int k = 1; k < nz; ++k) al[k] = h->a[k - 1];
+      h->d_al = h->alloc<double>(al.size());
+      HIPCHECK(hipMemcpy(h->d_al, al.data(), al.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     HIPCHECK(hipMemcpy(h->d_cz, cz.data(), nz * sizeof(double), hipMemcpyHostToDevice));
+    {
+      // upper diagonal of the omega interior row k-1, by level
+      std::vector<double> cu(qg::thomas_shifted_a_size(nz), 0.0);
+      for (int k = 1; k <= nz - 3; ++k) cu[k] = cz[k];
+      h->d_cu = h->alloc<double>(cu.size());
+      HIPCHECK(hipMemcpy(h->d_cu, cu.data(), cu.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     HIPCHECK(hipMemcpy(h->d_fN2, fN2.data(), nz * sizeof(double), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(h->d_mask, mask.data(), h->plane, hipMemcpyHostToDevice));
 
@@ -638,30 +654,28 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
     g.inv_n = 1.0 / ((double)nx * (double)ny);
     g.dz = h->dz;
     // Thomas factors, computed once: they depend only on (kh2, a, dz)
-    if (!qg::thomas_lds_setup() || !qg::omega_lds_setup())
+    if (!qg::thomas_setup() || !qg::omega_setup())
       throw QgError(QG_ERR_DEVICE, "cannot reserve the LDS of the column solvers");
     {
       const size_t tsz = qg::thomas_table_size(g);
       for (int o = 0; o < 2; ++o) {
-        h->th_den[o] = h->alloc<double>(tsz);
         h->th_inv[o] = h->alloc<double>(tsz);
         h->th_cp[o] = h->alloc<double>(tsz);
       }
       int* e = h->alloc<int>(8);
       HIPCHECK(hipMemset(e, 0, 8 * sizeof(int)));
-      qg::launch_thomas_factor(g, h->st, h->d_a, 1, h->th_den[0], h->th_inv[0], h->th_cp[0], e);
-      qg::launch_thomas_factor(g, h->st, h->d_a, 4, h->th_den[1], h->th_inv[1], h->th_cp[1], e + 4);
+      qg::launch_thomas_factor(g, h->st, h->d_a, 1, h->th_inv[0], h->th_cp[0], e);
+      qg::launch_thomas_factor(g, h->st, h->d_a, 4, h->th_inv[1], h->th_cp[1], e + 4);
       int he[8];
       HIPCHECK(hipMemcpyAsync(he, e, sizeof he, hipMemcpyDeviceToHost, h->st));
       HIPCHECK(hipStreamSynchronize(h->st));
       std::memcpy(h->sing_psi, he, 3 * sizeof(int));
       std::memcpy(h->sing_A, he + 4, 3 * sizeof(int));
-      h->om_fact = h->alloc<double>(h->npts);
-      h->om_dp = h->alloc<double>(h->npts);
-      h->om_dinv = h->alloc<double>(h->npts);
-      h->om_piv = h->alloc<unsigned char>(h->plane);
+      h->om_M = h->alloc<double>(tsz);
+      h->om_D = h->alloc<double>(tsz);
+      h->om_piv = h->alloc<unsigned char>(tsz / nz);
       HIPCHECK(hipMemset(e, 0, 8 * sizeof(int)));
-      qg::launch_omega_factor(g, h->st, h->d_cz, h->om_fact, h->om_dp, h->om_dinv, h->om_piv, e);
+      qg::launch_omega_factor(g, h->st, h->d_cz, h->om_M, h->om_D, h->om_piv, e);
       HIPCHECK(hipMemcpyAsync(he, e, sizeof he, hipMemcpyDeviceToHost, h->st));
       HIPCHECK(hipStreamSynchronize(h->st));
       std::memcpy(h->sing_omega, he, 3 * sizeof(int));

@@ -1,20 +1,17 @@
-// Batched Thomas solves of the q->psi and B->A column operators.
+// Batched Thomas solves of the q->psi and B->A column operators
+// (thomas_solve!, src/elliptic.jl:1168-1202, as called by invert_q_to_psi!,
+// src/elliptic.jl:189-314, and invert_B_to_A!, src/elliptic.jl:836-990).
 //
-// The elimination of each column depends only on (kh2, a, dz), so
+// The elimination of a column depends only on (kh2, a, dz), so
 // k_thomas_factor runs it once per model and records per level the pivot
-// denom_k, its correctly rounded reciprocal and c'_k (the reference's
-// thomas_solve! recomputes them on every call, src/elliptic.jl:1168-1202).
-// Columns (kx, ky) and (-kx, -ky), (-kx, ky), (kx, -ky) share kh2, so the
-// tables are kept for the quarter (|ix| <= nx/2, |iy| <= ny/2) only.
-//
-// Per solve, a block owns a tile of TC adjacent columns: all 256 threads stage
-// the right-hand sides [nz][TC] into LDS with coalesced loads, 2*TC lanes run
-// the recurrences in place (one lane for the real and one for the imaginary
-// part, which the reference solves separately), and all threads write back.
-// Each forward step's quotient (b_k - dl_k x_{k-1}) / denom_k is formed from
-// the stored reciprocal with one FMA correction, which is the correctly
-// rounded quotient, so the division leaves the loop-carried chain.
+// denom_k, its reciprocal and c'_k for the quarter columns (the reference
+// recomputes them on every call).  A solve is a mirror-tile kernel: the
+// forward sweep x'_k = (dz2 b_k - a_{k-1} x'_{k-1}) / denom_k is evaluated as
+// fma(-(a_{k-1} inv_k), x'_{k-1}, (dz2 b_k) inv_k) and the back substitution
+// as fma(-c'_k, x_{k+1}, x'_k), one FMA per level on the serial chain; the
+// results differ from the reference's division by a few ulp.
 #include "qg_columns.hpp"
+#include "qg_mirror.hpp"
 
 #pragma clang fp contract(off)
 
@@ -22,9 +19,6 @@ namespace qg {
 
 namespace {
 
-constexpr int kThreads = 256;
-constexpr int kPF = 16;     // table entries loaded ahead in the sweeps
-constexpr int kBatch = 16;  // staged loads in flight per thread
 constexpr double kSqrtEps = 1.4901161193847656e-08;  // sqrt(eps(Float64))
 
 __device__ inline void flag_error(int* err, int code, int level, int col) {
@@ -34,28 +28,11 @@ __device__ inline void flag_error(int* err, int code, int level, int col) {
   }
 }
 
-// n / d given y = RN(1/d): q0 = n*y, then one residual correction.
-__device__ inline double quot(double n, double d, double y) {
-  const double q0 = n * y;
-  const double e = fma(-d, q0, n);
-  return fma(e, y, q0);
-}
-
-struct Quarter {
-  int Wr, Hr;  // reduced widths nx/2+1, ny/2+1
-  __device__ inline int rcol(int x, int y, int nx, int ny) const {
-    const int rx = x <= nx / 2 ? x : nx - x;
-    const int ry = y <= ny / 2 ? y : ny - y;
-    return ry * Wr + rx;
-  }
-};
-
-// Tables [k][rcol]: den (pivot), inv = 1/den, cp = c'.  Pivots below
-// sqrt(eps) are recorded in err (code 1) where thomas_solve! raises.
+// Tables [k][rc]: inv = 1/denom, cp = c'.  Pivots below sqrt(eps) are
+// recorded in err (code 1) where thomas_solve! raises.
 __global__ void k_thomas_factor(Geo g, Quarter qr, const double* __restrict__ a, int kdiv,
-                                double* __restrict__ den, double* __restrict__ inv, double* __restrict__ cp,
-                                int* __restrict__ err) {
-  const int rplane = qr.Wr * qr.Hr;
+                                double* __restrict__ inv, double* __restrict__ cp, int* __restrict__ err) {
+  const int rplane = qr.size();
   const int rc = blockIdx.x * blockDim.x + threadIdx.x;
   if (rc >= rplane) return;
   const int x = rc % qr.Wr, y = rc / qr.Wr;
@@ -65,7 +42,6 @@ __global__ void k_thomas_factor(Geo g, Quarter qr, const double* __restrict__ a,
   auto at = [&](int k) { return (size_t)k * rplane + rc; };
   if (nz == 1 || (kh2 == 0.0 && kdiv == 1)) {
     for (int k = 0; k < nz; ++k) {
-      den[at(k)] = 1.0;
       inv[at(k)] = 1.0;
       cp[at(k)] = 0.0;
     }
@@ -77,8 +53,7 @@ __global__ void k_thomas_factor(Geo g, Quarter qr, const double* __restrict__ a,
   const double du0 = gauge ? 0.0 : a[0];
   if (fabs(d0) < kSqrtEps) flag_error(err, 1, 0, y * g.nx + x);
   double c = du0 / d0;
-  den[at(0)] = d0;
-  inv[at(0)] = 1.0 / d0;
+  inv[at(0)] = gauge ? 0.0 : 1.0 / d0;  // gauge row: x'_0 = 0 * rhs
   cp[at(0)] = c;
   for (int i = 1; i < nz; ++i) {
     const double dl = a[i - 1];
@@ -89,195 +64,145 @@ __global__ void k_thomas_factor(Geo g, Quarter qr, const double* __restrict__ a,
     if (fabs(denom) < kSqrtEps) flag_error(err, 1, i, y * g.nx + x);
     const double du = (i < nz - 1) ? a[i] : 0.0;
     c = du / denom;
-    den[at(i)] = denom;
     inv[at(i)] = 1.0 / denom;
     cp[at(i)] = c;
   }
 }
 
-__device__ inline double& part_ref(cplx<double>* xs, int k, int TC, int c, int part) {
-  cplx<double>& v = xs[(size_t)k * TC + c];
-  return part ? v.im : v.re;
-}
-
-__global__ void __launch_bounds__(kThreads)
-k_thomas_tile(Geo g, Quarter qr, const cplx<double>* __restrict__ rhs, cplx<double>* __restrict__ out,
-              cplx<double>* __restrict__ Cout, const double* __restrict__ den, const double* __restrict__ inv,
-              const double* __restrict__ cp, const double* __restrict__ a, int kdiv, int TC,
-              unsigned long long* __restrict__ warn) {
-  extern __shared__ cplx<double> xs[];  // [nz][TC] then a[nz]
-  const int plane = g.nx * g.ny;
-  const int rplane = qr.Wr * qr.Hr;
-  const int nz = g.nz;
-  const int col0 = blockIdx.x * TC;
-  const int ncol = min(TC, plane - col0);
-  const double dz = g.dz;
-  const double dz2 = dz * dz;
-  double* al = reinterpret_cast<double*>(xs + (size_t)nz * TC);
-
-  // phase 1: stage the right-hand sides (coalesced along x) and a(z)
-  for (int base = 0; base < nz * TC; base += kThreads * kBatch) {
-    cplx<double> v[kBatch];
-#pragma unroll
-    for (int u = 0; u < kBatch; ++u) {
-      const int e = base + u * kThreads + threadIdx.x;
-      const int k = e / TC, c = e % TC;
-      v[u] = (e < nz * TC && c < ncol) ? rhs[(size_t)k * plane + col0 + c] : mk(0.0, 0.0);
-    }
-#pragma unroll
-    for (int u = 0; u < kBatch; ++u) {
-      const int e = base + u * kThreads + threadIdx.x;
-      if (e < nz * TC) xs[e] = v[u];
-    }
-  }
-  for (int k = threadIdx.x; k < nz; k += kThreads) al[k] = a[k];
+// KIND 0: q -> psi; 1: B -> A; 2: B -> A and C = dA/dz.
+// al[k] = a[k-1] (al[0] = 0), zero-padded to mirror_pad(nz) entries.
+template <int KIND, int QC>
+__global__ void __launch_bounds__(kMirrorThreads)
+k_thomas(Geo g, Quarter qr, const cplx<double>* __restrict__ rhs, cplx<double>* __restrict__ out,
+         cplx<double>* __restrict__ Cout, const double* __restrict__ inv, const double* __restrict__ cp,
+         const double* __restrict__ al, unsigned long long* __restrict__ warn) {
+  constexpr int NC = 4 * QC;
+  extern __shared__ cplx<double> xs[];  // [kpad][NC], then ti[kpad][QC], tc[kpad][QC]
+  const int nz = g.nz, kpad = mirror_pad(nz);
+  const int rc0 = blockIdx.x * QC;
+  double* ti = reinterpret_cast<double*>(xs + (size_t)kpad * NC);
+  double* tc = ti + (size_t)kpad * QC;
+  const MirrorSlot<QC> sl(g, qr, rc0);
+  mirror_stage<QC>(g, qr, rc0, sl, rhs, inv, cp, xs, ti, tc, kpad, [](int) { return 1.0; });
   __syncthreads();
 
-  // phase 2: recurrences, lanes [0, 2*TC)
   const int lane = threadIdx.x;
-  if (lane < 2 * TC) {
-    const int c = lane % TC, part = lane / TC;
-    const int col = col0 + c;
-    if (c < ncol) {
-      const int x = col % g.nx, y = col / g.nx;
-      const double kh2 = g.kx[x] * g.kx[x] + g.ky[y] * g.ky[y];
-      const int rc = qr.rcol(x, y, g.nx, g.ny);
-      auto at = [&](int k) { return (size_t)k * rplane + rc; };
+  if (lane < 2 * NC) {
+    const int cs = lane >> 1, part = lane & 1;
+    const QuadCol q = quad_col(g, qr, rc0 + cs % QC, cs / QC);
+    if (q.active) {
+      double* b0 = reinterpret_cast<double*>(xs + cs) + part;  // level k at b0[k * 2NC]
+      const int st = 2 * NC;
+      const double* tI = ti + cs % QC;
+      const double* tC = tc + cs % QC;
+      const double kh2 = g.kx[q.x] * g.kx[q.x] + g.ky[q.y] * g.ky[q.y];
+      const double dz2 = g.dz * g.dz;
       if (nz == 1) {
-        double& v = part_ref(xs, 0, TC, c, part);
+        double& v = b0[0];
         if (kh2 == 0.0) v = 0.0;
-        else if (kdiv == 1) v = -v / kh2;
+        else if (KIND == 0) v = -v / kh2;
         else v = (-4 * v) / kh2;
-      } else if (kh2 == 0.0 && kdiv == 1) {
+      } else if (kh2 == 0.0 && KIND == 0) {
         // psi = 0 on the horizontal mean (src/elliptic.jl:257-270); warn on a
         // significant mean of q.
         if (part == 0 && warn) {
           double m = 0.0;
           for (int k = 0; k < nz; ++k) {
-            cplx<double> v = xs[(size_t)k * TC + c];
-            double av = hypot(v.re, v.im);
+            const double av = hypot(b0[k * st], b0[k * st + 1]);
             m = av > m ? av : m;
           }
           if (m > 1e-10) atomicMax(warn, (unsigned long long)__double_as_longlong(m));
         }
       } else {
-        const bool gauge = kh2 == 0.0;
-        // forward sweep: x'_k = (dz2 b_k - a_{k-1} x'_{k-1}) / denom_k, in
-        // place; kPF levels move between LDS and registers at a time so the
-        // LDS latency is paid once per batch, not once per level.
-        double xprev = 0.0;
-        for (int i0 = 0; i0 < nz; i0 += kPF) {
-          double dd[kPF], yy[kPF], bb[kPF], aa[kPF];
-#pragma unroll
-          for (int j = 0; j < kPF; ++j) {
-            const int i = i0 + j;
-            const bool in = i < nz;
-            dd[j] = in ? den[at(i)] : 1.0;
-            yy[j] = in ? inv[at(i)] : 1.0;
-            bb[j] = in ? part_ref(xs, i, TC, c, part) : 0.0;
-            aa[j] = (in && i > 0) ? al[i - 1] : 0.0;
-          }
-#pragma unroll
-          for (int j = 0; j < kPF; ++j) {
-            const int i = i0 + j;
-            double xv;
-            if (i == 0) xv = gauge ? 0.0 / 1.0 : quot(dz2 * bb[j], dd[j], yy[j]);
-            else xv = quot(dz2 * bb[j] - aa[j] * xprev, dd[j], yy[j]);
-            if (i < nz) xprev = xv;
-            bb[j] = xv;
-          }
-#pragma unroll
-          for (int j = 0; j < kPF; ++j)
-            if (i0 + j < nz) part_ref(xs, i0 + j, TC, c, part) = bb[j];
-        }
-        // back substitution
-        double above = xprev;
-        for (int i0 = nz - 2; i0 >= 0; i0 -= kPF) {
-          double cc[kPF], xx[kPF];
-#pragma unroll
-          for (int j = 0; j < kPF; ++j) {
-            const int i = i0 - j;
-            cc[j] = i >= 0 ? cp[at(i)] : 0.0;
-            xx[j] = i >= 0 ? part_ref(xs, i, TC, c, part) : 0.0;
-          }
-#pragma unroll
-          for (int j = 0; j < kPF; ++j) {
-            const double xi = xx[j] - cc[j] * above;
-            if (i0 - j >= 0) above = xi;
-            xx[j] = xi;
-          }
-#pragma unroll
-          for (int j = 0; j < kPF; ++j)
-            if (i0 - j >= 0) part_ref(xs, i0 - j, TC, c, part) = xx[j];
-        }
-        if (gauge) {  // remove the vertical mean (src/elliptic.jl:930-945)
+        // For the B -> A mean column the stored reciprocal of the gauge row
+        // A[1] = 0 is 0, which gives x'_0 = 0.
+        mirror_sweeps(
+            b0, st, kpad,
+            [&](int k, double b) {
+              const double y = tI[k * QC];
+              return make_double2(al[k] * y, (dz2 * b) * y);
+            },
+            [&](int k, double xp) { return make_double2(tC[k * QC], xp); });
+        if (kh2 == 0.0) {  // remove the vertical mean (src/elliptic.jl:930-945)
           double mean = 0.0;
-          for (int k = 0; k < nz; ++k) mean = mean + part_ref(xs, k, TC, c, part);
+          for (int k = 0; k < nz; ++k) mean = mean + b0[k * st];
           mean = mean / nz;
-          for (int k = 0; k < nz; ++k) {
-            double& slot = part_ref(xs, k, TC, c, part);
-            slot = slot - mean;
-          }
+          for (int k = 0; k < nz; ++k) b0[k * st] = b0[k * st] - mean;
         }
       }
     }
   }
   __syncthreads();
 
-  // phase 3: write back (psi = 0 on the mean column), and C = dA/dz
-  for (int e = threadIdx.x; e < nz * TC; e += kThreads) {
-    const int k = e / TC, c = e % TC;
-    if (c >= ncol) continue;
-    const int col = col0 + c;
-    const int x = col % g.nx, y = col / g.nx;
-    const bool zero_col = kdiv == 1 && nz > 1 && g.kx[x] == 0.0 && g.ky[y] == 0.0;
-    cplx<double> v = zero_col ? mk(0.0, 0.0) : xs[e];
-    out[(size_t)k * plane + col] = v;
-    if (Cout) {
-      cplx<double> cv = mk(0.0, 0.0);
-      if (k < nz - 1) {
-        cplx<double> up = xs[e + TC];
-        cv = mk((up.re - v.re) / dz, (up.im - v.im) / dz);
-      }
-      Cout[(size_t)k * plane + col] = cv;
+  // write back (psi = 0 on the mean column), and C = dA/dz
+  if (!sl.q.active) return;
+  const size_t plane = (size_t)g.nx * g.ny;
+  const bool zero_col = KIND == 0 && nz > 1 && sl.q.x == 0 && sl.q.y == 0;
+  for (int k = sl.kf; k < nz; k += MirrorSlot<QC>::kstep) {
+    const size_t at = (size_t)k * plane + sl.off;
+    const cplx<double> r = zero_col ? mk(0.0, 0.0) : xs[k * NC + sl.c];
+    out[at] = r;
+    if (KIND == 2) {
+      const cplx<double> up = xs[min(k + 1, kpad - 1) * NC + sl.c];
+      Cout[at] = k < nz - 1 ? mk((up.re - r.re) / g.dz, (up.im - r.im) / g.dz) : mk(0.0, 0.0);
     }
   }
 }
 
-Quarter quarter(const Geo& g) { return Quarter{g.nx / 2 + 1, g.ny / 2 + 1}; }
+template <int KIND, int QC>
+void launch_k(const Geo& g, hipStream_t st, const cplx<double>* rhs, cplx<double>* out, cplx<double>* Cout,
+              const double* inv, const double* cp, const double* al, unsigned long long* warn) {
+  const Quarter qr = quarter(g);
+  const int blocks = (qr.size() + QC - 1) / QC;
+  hipLaunchKernelGGL((k_thomas<KIND, QC>), dim3(blocks), dim3(kMirrorThreads), mirror_lds_bytes(mirror_pad(g.nz), QC),
+                     st, g, qr, rhs, out, Cout, inv, cp, al, warn);
+}
+
+template <int QC>
+void launch_qc(const Geo& g, hipStream_t st, const cplx<double>* rhs, cplx<double>* out, cplx<double>* Cout,
+               const double* inv, const double* cp, const double* al, int kdiv, unsigned long long* warn) {
+  if (kdiv == 1) launch_k<0, QC>(g, st, rhs, out, nullptr, inv, cp, al, warn);
+  else if (Cout) launch_k<2, QC>(g, st, rhs, out, Cout, inv, cp, al, warn);
+  else launch_k<1, QC>(g, st, rhs, out, nullptr, inv, cp, al, warn);
+}
+
+template <int QC>
+bool set_lds_attr() {
+  const void* fns[] = {reinterpret_cast<const void*>(k_thomas<0, QC>), reinterpret_cast<const void*>(k_thomas<1, QC>),
+                       reinterpret_cast<const void*>(k_thomas<2, QC>)};
+  for (const void* f : fns)
+    if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMirrorLdsBytes) != hipSuccess)
+      return false;
+  return true;
+}
 
 }  // namespace
 
-size_t thomas_table_size(const Geo& g) { return (size_t)(g.nx / 2 + 1) * (g.ny / 2 + 1) * g.nz; }
+size_t thomas_table_size(const Geo& g) { return (size_t)quarter(g).size() * g.nz; }
 
-void launch_thomas_factor(const Geo& g, hipStream_t st, const double* a, int kdiv, double* den, double* inv,
-                          double* cp, int* err) {
+int thomas_shifted_a_size(int nz) { return mirror_pad(nz); }
+
+bool column_depth_supported(int nz) { return nz >= 1 && mirror_qc(nz) > 0; }
+
+void launch_thomas_factor(const Geo& g, hipStream_t st, const double* a, int kdiv, double* inv, double* cp,
+                          int* err) {
   const Quarter qr = quarter(g);
-  const int rplane = qr.Wr * qr.Hr;
   const int bs = 64;
-  hipLaunchKernelGGL(k_thomas_factor, dim3((rplane + bs - 1) / bs), dim3(bs), 0, st, g, qr, a, kdiv, den, inv,
-                     cp, err);
+  hipLaunchKernelGGL(k_thomas_factor, dim3((qr.size() + bs - 1) / bs), dim3(bs), 0, st, g, qr, a, kdiv, inv, cp,
+                     err);
 }
 
-int thomas_tile_columns(int nz) {
-  int tc = 64;
-  while (tc > 1 && (size_t)tc * nz * sizeof(cplx<double>) + nz * sizeof(double) > kThomasLdsBytes) tc /= 2;
-  return tc;
-}
+bool thomas_setup() { return set_lds_attr<8>() && set_lds_attr<4>() && set_lds_attr<2>() && set_lds_attr<1>(); }
 
-bool thomas_lds_setup() {
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(k_thomas_tile),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)kThomasLdsBytes) == hipSuccess;
-}
-
-void launch_thomas_lds(const Geo& g, hipStream_t st, const cplx<double>* rhs, cplx<double>* out,
-                       cplx<double>* Cout, const double* den, const double* inv, const double* cp,
-                       const double* a, int kdiv, unsigned long long* warn) {
-  const int plane = g.nx * g.ny;
-  const int tc = thomas_tile_columns(g.nz);
-  const size_t lds = (size_t)tc * g.nz * sizeof(cplx<double>) + g.nz * sizeof(double);
-  hipLaunchKernelGGL(k_thomas_tile, dim3((plane + tc - 1) / tc), dim3(kThreads), lds, st, g, quarter(g), rhs,
-                     out, Cout, den, inv, cp, a, kdiv, tc, warn);
+void launch_thomas(const Geo& g, hipStream_t st, const cplx<double>* rhs, cplx<double>* out, cplx<double>* Cout,
+                   const double* inv, const double* cp, const double* al, int kdiv, unsigned long long* warn) {
+  switch (mirror_qc(g.nz)) {
+    case 8: launch_qc<8>(g, st, rhs, out, Cout, inv, cp, al, kdiv, warn); return;
+    case 4: launch_qc<4>(g, st, rhs, out, Cout, inv, cp, al, kdiv, warn); return;
+    case 2: launch_qc<2>(g, st, rhs, out, Cout, inv, cp, al, kdiv, warn); return;
+    case 1: launch_qc<1>(g, st, rhs, out, Cout, inv, cp, al, kdiv, warn); return;
+    default: return;  // rejected by column_depth_supported at model creation
+  }
 }
 
 }  // namespace qg
