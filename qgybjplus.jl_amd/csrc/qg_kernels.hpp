@@ -634,12 +634,16 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
 // diagnostics.jl:149-229)
 // ===========================================================================
 
-// Y-pass (inverse, half width) of u, v and the four omega-RHS factors.
+// Y-pass (inverse, half width) of the four omega-RHS factors
+// (_omega_eqn_rhs_direct!, src/diagnostics.jl:149-229): psi_z (forward
+// difference to the level above, zero at the top) times i*kx and i*ky, and
+// the two-level average psi-bar (psi itself at the top) times -i*kx*kh2 and
+// -i*ky*kh2.  psi(z) and psi(z+1) are read once; the four Hermitian inputs are
+// formed in registers before the four transforms.
 template <typename R, int NY>
 __global__ void __launch_bounds__((Tile<R, NY>::THREADS))
-k_yf(Geo g, int want_omega, const cplx<double>* __restrict__ psi, cplx<R>* __restrict__ Pu,
-     cplx<R>* __restrict__ Pv, cplx<R>* __restrict__ Pbx, cplx<R>* __restrict__ Pby,
-     cplx<R>* __restrict__ Pxx, cplx<R>* __restrict__ Pxy, const cplx<R>* __restrict__ tw) {
+k_yf_omega(Geo g, const cplx<double>* __restrict__ psi, cplx<R>* __restrict__ Pbx, cplx<R>* __restrict__ Pby,
+           cplx<R>* __restrict__ Pxx, cplx<R>* __restrict__ Pxy, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY>;
   using F = Fft<R, NY, +1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
@@ -652,70 +656,43 @@ k_yf(Geo g, int want_omega, const cplx<double>* __restrict__ psi, cplx<R>* __res
   cplx<R>* sl = lds + s * TL::LD;
   const double kxv = ok ? g.kx[x] : 0.0, kxm = ok ? g.kx[mx] : 0.0;
   const bool top = z == g.nz - 1;
+  const int zu = top ? z : z + 1;
   const double dz = g.dz;
-  cplx<double> p[F::IT][F::R0], pm[F::IT][F::R0];
-  cplx<R> v[F::IT][F::R0];
+  cplx<R> v[4][F::IT][F::R0];
 #pragma unroll
   for (int it = 0; it < F::IT; ++it)
 #pragma unroll
     for (int t = 0; t < F::R0; ++t) {
-      int y = F::elem(lane, it, t), my = mirror(y, NY);
-      bool in = ok && F::valid(lane, it);
-      p[it][t] = in ? psi[((size_t)z * NY + y) * g.nx + x] : czero<double>();
-      pm[it][t] = in ? psi[((size_t)z * NY + my) * g.nx + mx] : czero<double>();
-    }
-  const int nout = want_omega ? 6 : 2;
-  for (int o = 0; o < nout; ++o) {
-    cplx<R>* dst = o == 0 ? Pu : o == 1 ? Pv : o == 2 ? Pbx : o == 3 ? Pby : o == 4 ? Pxx : Pxy;
-#pragma unroll
-    for (int it = 0; it < F::IT; ++it)
-#pragma unroll
-      for (int t = 0; t < F::R0; ++t) {
-        int y = F::elem(lane, it, t), my = mirror(y, NY);
-        bool in = ok && F::valid(lane, it);
-        cplx<double> a, am;
-        if (o == 0) {
-          a = ik_mul(-g.ky[y], p[it][t]);
-          am = ik_mul(-g.ky[my], pm[it][t]);
-        } else if (o == 1) {
-          a = ik_mul(kxv, p[it][t]);
-          am = ik_mul(kxm, pm[it][t]);
-        } else {
-          cplx<double> pz = czero<double>(), pzm = czero<double>(), pa = p[it][t], pam = pm[it][t];
-          if (!top && in) {
-            // psi one level up: psi_z forward difference and two-level average
-            const cplx<double> up = psi[((size_t)(z + 1) * NY + y) * g.nx + x];
-            const cplx<double> upm = psi[((size_t)(z + 1) * NY + my) * g.nx + mx];
-            pz = mk<double>((up.re - p[it][t].re) / dz, (up.im - p[it][t].im) / dz);
-            pzm = mk<double>((upm.re - pm[it][t].re) / dz, (upm.im - pm[it][t].im) / dz);
-            pa = scale(up + p[it][t], 0.5);
-            pam = scale(upm + pm[it][t], 0.5);
-          }
-          const double ky = g.ky[y], kym = g.ky[my];
-          const double kh2 = kxv * kxv + ky * ky, kh2m = kxm * kxm + kym * kym;
-          if (o == 2) {
-            a = ik_mul(kxv, pz);
-            am = ik_mul(kxm, pzm);
-          } else if (o == 3) {
-            a = ik_mul(ky, pz);
-            am = ik_mul(kym, pzm);
-          } else if (o == 4) {
-            a = ik_mul(-(kxv * kh2), pa);
-            am = ik_mul(-(kxm * kh2m), pam);
-          } else {
-            a = ik_mul(-(ky * kh2), pa);
-            am = ik_mul(-(kym * kh2m), pam);
-          }
-        }
-        v[it][t] = cvt<R>(herm(a, am, g.inv_n));
+      const int y = F::elem(lane, it, t), my = mirror(y, NY);
+      const bool in = ok && F::valid(lane, it);
+      const cplx<double> p = in ? psi[((size_t)z * NY + y) * g.nx + x] : czero<double>();
+      const cplx<double> pm = in ? psi[((size_t)z * NY + my) * g.nx + mx] : czero<double>();
+      const cplx<double> up = in ? psi[((size_t)zu * NY + y) * g.nx + x] : czero<double>();
+      const cplx<double> upm = in ? psi[((size_t)zu * NY + my) * g.nx + mx] : czero<double>();
+      cplx<double> pz = czero<double>(), pzm = czero<double>(), pa = p, pam = pm;
+      if (!top) {
+        pz = mk<double>((up.re - p.re) / dz, (up.im - p.im) / dz);
+        pzm = mk<double>((upm.re - pm.re) / dz, (upm.im - pm.im) / dz);
+        pa = scale(up + p, 0.5);
+        pam = scale(upm + pm, 0.5);
       }
-    F::run(v, sl, lane, twr);
+      const double ky = g.ky[y], kym = g.ky[my];
+      const double kh2 = kxv * kxv + ky * ky, kh2m = kxm * kxm + kym * kym;
+      v[0][it][t] = cvt<R>(herm(ik_mul(kxv, pz), ik_mul(kxm, pzm), g.inv_n));
+      v[1][it][t] = cvt<R>(herm(ik_mul(ky, pz), ik_mul(kym, pzm), g.inv_n));
+      v[2][it][t] = cvt<R>(herm(ik_mul(-(kxv * kh2), pa), ik_mul(-(kxm * kh2m), pam), g.inv_n));
+      v[3][it][t] = cvt<R>(herm(ik_mul(-(ky * kh2), pa), ik_mul(-(kym * kh2m), pam), g.inv_n));
+    }
+#pragma unroll
+  for (int o = 0; o < 4; ++o) {
+    cplx<R>* dst = o == 0 ? Pbx : o == 1 ? Pby : o == 2 ? Pxx : Pxy;
+    F::run(v[o], sl, lane, twr);
 #pragma unroll
     for (int it = 0; it < F::IT; ++it)
 #pragma unroll
       for (int t = 0; t < F::R0; ++t) {
-        int y = F::elem(lane, it, t);
-        if (ok && F::valid(lane, it)) dst[((size_t)z * NY + y) * g.Wp + x] = v[it][t];
+        const int y = F::elem(lane, it, t);
+        if (ok && F::valid(lane, it)) dst[((size_t)z * NY + y) * g.Wp + x] = v[o][it][t];
       }
   }
 }

@@ -35,8 +35,7 @@ struct Ops {
                     D* rhs0, D* qout, EtdView etd, double nu, double inv_dz2, const C* twy);
   static void yb2_B(const Geo& g, hipStream_t st, int mode, int flags, const C* G3, const C* G4, const C* G5,
                     const D* A, const D* Bn, D* rhs0, D* Bout, EtdView etd, double alpha, const C* twy);
-  static void yf(const Geo& g, hipStream_t st, int want_omega, const D* psi, C* Pu, C* Pv, C* Pbx, C* Pby,
-                 C* Pxx, C* Pxy, const C* twy);
+  static void yf_omega(const Geo& g, hipStream_t st, const D* psi, C* Pbx, C* Pby, C* Pxx, C* Pxy, const C* twy);
   static void xf(const Geo& g, hipStream_t st, int want_omega, const C* Pu, const C* Pv, const C* Pbx,
                  const C* Pby, const C* Pxx, const C* Pxy, R* u, R* v, C* Rw, const C* twx);
   static void y_r2c_mask(const Geo& g, hipStream_t st, const C* Rw, D* out, const C* twy);

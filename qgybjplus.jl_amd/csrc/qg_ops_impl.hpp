@@ -113,9 +113,8 @@ void Ops<R>::yb2_B(const Geo& g, hipStream_t st, int mode, int flags, const C* G
 }
 
 template <typename R>
-void Ops<R>::yf(const Geo& g, hipStream_t st, int want_omega, const D* psi, C* Pu, C* Pv, C* Pbx, C* Pby,
-                C* Pxx, C* Pxy, const C* twy) {
-#define CALL(N) YL((k_yf<R, N>), N, g.W, g, want_omega, psi, Pu, Pv, Pbx, Pby, Pxx, Pxy, twy)
+void Ops<R>::yf_omega(const Geo& g, hipStream_t st, const D* psi, C* Pbx, C* Pby, C* Pxx, C* Pxy, const C* twy) {
+#define CALL(N) YL((k_yf_omega<R, N>), N, g.W, g, psi, Pbx, Pby, Pxx, Pxy, twy)
   QG_SWITCH(g.ny, CALL)
 #undef CALL
 }

@@ -309,9 +309,11 @@ struct Engine {
 
   // u, v and the QG omega-equation w (compute_velocities!(compute_w=true)).
   void velocities_w(const D* psi) {
-    K("yf", P * cD + Ph * 6 * cR, [&] {
-      O::yf(h.g, h.st, 1, psi, c(h.Pu), c(h.Pv), c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), twy());
+    K("yb_uv", P * cD + Ph * 2 * cR, [&] {
+      O::yb_adv(h.g, h.st, qg::OUT_U | qg::OUT_V, psi, nullptr, c(h.Pu), c(h.Pv), nullptr, nullptr, twy());
     });
+    K("yf_omega", P * cD + Ph * 4 * cR,
+      [&] { O::yf_omega(h.g, h.st, psi, c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), twy()); });
     K("xf", Ph * 7 * cR + P * 2 * rR, [&] {
       O::xf(h.g, h.st, 1, c(h.Pu), c(h.Pv), c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), r(h.u), r(h.v), c(h.Pq),
             twx());
