@@ -242,9 +242,12 @@ struct Engine {
   // algorithmic bytes per point: complex R, real R, complex double; P = all
   // points, Ph = points of a half-width (real-field) intermediate
   const double cR = 2.0 * sizeof(R), rR = sizeof(R), cD = 16.0;
-  const double P, Ph;
+  // Tq: the two per-level tables of a column solve, stored for the quarter
+  // columns |ix| <= nx/2, |iy| <= ny/2
+  const double P, Ph, Tq;
   explicit Engine(qg_handle& hh)
-      : h(hh), P((double)hh.npts), Ph((double)hh.g.W * hh.ny * hh.nz) {}
+      : h(hh), P((double)hh.npts), Ph((double)hh.g.W * hh.ny * hh.nz),
+        Tq(2.0 * 8.0 * (hh.nx / 2 + 1) * (hh.ny / 2 + 1) * hh.nz) {}
   C* c(void* p) const { return static_cast<C*>(p); }
   R* r(void* p) const { return static_cast<R*>(p); }
   const C* twx() const { return static_cast<const C*>(h.twx); }
@@ -283,7 +286,7 @@ struct Engine {
         src = h.qeff;
         t1 = true;
       }
-      K("thomas_psi", P * 2 * cD,
+      K("thomas_psi", P * 2 * cD + Tq,
         [&] { solve_psi(h, src, psi_out); });
     }
     if (need_t1 && !t1) t1_only(Bsrc);
@@ -295,7 +298,7 @@ struct Engine {
       if (Cout) qg::launch_zero(Cout, h.npts, h.st);
       return;
     }
-    K(Cout ? "thomas_AC" : "thomas_A", P * (Cout ? 3 : 2) * cD,
+    K(Cout ? "thomas_AC" : "thomas_A", P * (Cout ? 3 : 2) * cD + Tq,
       [&] { solve_A(h, Bsrc, Aout, Cout); });
   }
 
@@ -325,7 +328,7 @@ struct Engine {
                h.sing_omega[1] + 1, h.sing_omega[2]);
       throw QgError(QG_ERR_GTSV, buf);
     }
-    K("omega_gtsv", P * 2 * cD, [&] {
+    K("omega_gtsv", P * 2 * cD + Tq, [&] {
       qg::launch_omega(h.g, h.st, h.rq0, h.qeff, h.om_M, h.om_D, h.om_piv, h.d_cz, h.d_cu, h.d_fN2, h.s0, h.s1,
                        h.s2);
     });
