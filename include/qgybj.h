@@ -89,6 +89,11 @@ typedef struct qg_config {
   int linear;               /* LinearDynamics() */
   int no_dispersion;        /* NoDispersion() */
   int precision;            /* QG_PRECISION_FP64 | QG_PRECISION_MIXED */
+  /* Ensemble of models of one grid class (0 or 1: a single model).  Member m
+   * occupies stacked levels [m*nz, (m+1)*nz); N2 and a_ell then hold
+   * members*nz entries (member-major), every other setting is shared.  Each
+   * pass of the step covers all members in one launch. */
+  int members;
 } qg_config;
 
 /* Replaces build_runtime + allocate_fields (src/core/runtime.jl:34-83,
@@ -108,8 +113,16 @@ const char* qg_last_error(const qg_handle* h);
  * fft_forward!).  Real fields (u, v, w) are physical. */
 int qg_set_field(qg_handle* h, int field, const double* host, int space);
 
-/* Copies one field to host memory (spectral for q, B, psi, A, C). */
+/* Copies one field to host memory (spectral for q, B, psi, A, C).  With
+ * members > 1 both calls exchange the whole stack, a (members*nz, nx, ny)
+ * array. */
 int qg_get_field(qg_handle* h, int field, double* host);
+
+/* The same for one ensemble member: a (nz, nx, ny) array.  No reference
+ * counterpart (the reference has no ensembles); used by batched runs and the
+ * serving loop to set up and read back one run. */
+int qg_set_field_member(qg_handle* h, int member, int field, const double* host, int space);
+int qg_get_field_member(qg_handle* h, int member, int field, double* host);
 
 /* Replaces step!(model, ::ExponentialRungeKutta2) (src/core/runtime.jl:239-263)
  * -> _advance_etdrk2! (src/timestep.jl:335-415), nsteps times with step size
@@ -119,6 +132,8 @@ int qg_step(qg_handle* h, double dt, int nsteps);
 /* Replaces _check_termination_conditions! (src/simulation.jl:25-44): any
  * non-finite value in q, B or psi, and max |ifft(psi)| over the grid. */
 int qg_check(qg_handle* h, int* nonfinite, double* max_abs_psi);
+/* The check of one ensemble member. */
+int qg_check_member(qg_handle* h, int member, int* nonfinite, double* max_abs_psi);
 
 /* Model-level operators (src/core/runtime.jl:169-206). */
 int qg_invert_q_to_psi(qg_handle* h);                 /* invert_q_to_psi!(model) */
@@ -127,6 +142,11 @@ int qg_compute_velocities(qg_handle* h, int compute_w); /* compute_velocities!(m
 /* q from psi: method 0 = compute_q_from_psi! (src/initialization.jl:597-644),
  * 1 = compute_barotropic_q_from_psi! (src/initialization.jl:653-668). */
 int qg_compute_q_from_psi(qg_handle* h, int method);
+/* The same two set-up operators for one ensemble member (no reference
+ * counterpart; a run admitted into an ensemble is set up without touching the
+ * other members). */
+int qg_compute_q_from_psi_member(qg_handle* h, int member, int method);
+int qg_invert_B_to_A_member(qg_handle* h, int member);
 /* compute_qw_complex!(qw, B, ...) (src/nonlinear.jl:859-944) on the current
  * B; result copied to host (complex, (z,x,y)). */
 int qg_compute_qw(qg_handle* h, double* host_out);

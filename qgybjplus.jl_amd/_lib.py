@@ -35,6 +35,8 @@ EXPORTED = (
     "qg_step", "qg_check", "qg_invert_q_to_psi", "qg_invert_B_to_A",
     "qg_compute_velocities", "qg_compute_q_from_psi", "qg_compute_qw", "qg_fft",
     "qg_synchronize", "qg_take_mean_q_warning", "qg_profile_begin", "qg_profile_end",
+    "qg_set_field_member", "qg_get_field_member", "qg_check_member",
+    "qg_compute_q_from_psi_member", "qg_invert_B_to_A_member",
 )
 
 
@@ -52,6 +54,7 @@ class QgConfig(C.Structure):
         ("fixed_flow", C.c_int), ("wave_feedback", C.c_int), ("formulation", C.c_int),
         ("inviscid", C.c_int), ("linear", C.c_int), ("no_dispersion", C.c_int),
         ("precision", C.c_int),
+        ("members", C.c_int),
     ]
 
 
@@ -86,6 +89,11 @@ def load() -> C.CDLL:
         "qg_get_field": (C.c_int, [P, C.c_int, dp]),
         "qg_step": (C.c_int, [P, C.c_double, C.c_int]),
         "qg_check": (C.c_int, [P, C.POINTER(C.c_int), dp]),
+        "qg_set_field_member": (C.c_int, [P, C.c_int, C.c_int, dp, C.c_int]),
+        "qg_get_field_member": (C.c_int, [P, C.c_int, C.c_int, dp]),
+        "qg_check_member": (C.c_int, [P, C.c_int, C.POINTER(C.c_int), dp]),
+        "qg_compute_q_from_psi_member": (C.c_int, [P, C.c_int, C.c_int]),
+        "qg_invert_B_to_A_member": (C.c_int, [P, C.c_int]),
         "qg_invert_q_to_psi": (C.c_int, [P]),
         "qg_invert_B_to_A": (C.c_int, [P]),
         "qg_compute_velocities": (C.c_int, [P, C.c_int]),
@@ -118,7 +126,7 @@ class Handle:
 
     def __init__(self, *, nx, ny, nz, f, dz, kx, ky, N2, a_ell, flow_terms, wave_terms,
                  nu_z, fixed_flow, wave_feedback, formulation, inviscid, linear,
-                 no_dispersion, precision):
+                 no_dispersion, precision, members: int = 1):
         lib = load()
         self._lib = lib
         self._keep = []
@@ -137,14 +145,16 @@ class Handle:
             nx, ny, nz, float(f), float(dz), _dptr(kx_), _dptr(ky_), _dptr(N2_), _dptr(a_),
             len(flow_terms), _dptr(fc), _iptr(fo), len(wave_terms), _dptr(wc), _iptr(wo),
             float(nu_z), int(bool(fixed_flow)), int(bool(wave_feedback)), int(formulation),
-            int(bool(inviscid)), int(bool(linear)), int(bool(no_dispersion)), int(precision))
+            int(bool(inviscid)), int(bool(linear)), int(bool(no_dispersion)), int(precision), int(members))
         h = C.c_void_p()
         rc = lib.qg_create(C.byref(cfg), C.byref(h))
         if rc != QG_OK:
             msg = lib.qg_last_error(None).decode()
             raise _exception(rc, msg)
         self.h = h
-        self.shape = (nz, nx, ny)
+        self.members = int(members)
+        self.member_shape = (nz, nx, ny)
+        self.shape = (nz * self.members, nx, ny)
 
     def _check(self, rc: int):
         if rc != QG_OK:
@@ -178,6 +188,36 @@ class Handle:
         out = np.empty(self.shape, dtype=dt, order="F")
         self._check(self._lib.qg_get_field(self.h, fid, out.ctypes.data_as(C.POINTER(C.c_double))))
         return np.ascontiguousarray(out)
+
+    # -- one ensemble member -----------------------------------------------------
+    def set_field_member(self, member: int, name: str, values, space: int = SPACE_SPECTRAL):
+        fid = FIELD_IDS[name]
+        dt = np.complex128 if name in COMPLEX_FIELDS else np.float64
+        a = np.asfortranarray(np.asarray(values, dtype=dt))
+        if a.shape != self.member_shape:
+            raise ValueError(f"field has shape {a.shape}; expected {self.member_shape} in (z, x, y) layout")
+        self._check(self._lib.qg_set_field_member(self.h, int(member), fid,
+                                                  a.ctypes.data_as(C.POINTER(C.c_double)), space))
+
+    def get_field_member(self, member: int, name: str) -> np.ndarray:
+        fid = FIELD_IDS[name]
+        dt = np.complex128 if name in COMPLEX_FIELDS else np.float64
+        out = np.empty(self.member_shape, dtype=dt, order="F")
+        self._check(self._lib.qg_get_field_member(self.h, int(member), fid,
+                                                  out.ctypes.data_as(C.POINTER(C.c_double))))
+        return np.ascontiguousarray(out)
+
+    def check_member(self, member: int):
+        bad = C.c_int(0)
+        m = C.c_double(0.0)
+        self._check(self._lib.qg_check_member(self.h, int(member), C.byref(bad), C.byref(m)))
+        return bool(bad.value), float(m.value)
+
+    def compute_q_from_psi_member(self, member: int, barotropic: bool = False):
+        self._check(self._lib.qg_compute_q_from_psi_member(self.h, int(member), 1 if barotropic else 0))
+
+    def invert_B_to_A_member(self, member: int):
+        self._check(self._lib.qg_invert_B_to_A_member(self.h, int(member)))
 
     # -- operators --------------------------------------------------------------
     def step(self, dt: float, nsteps: int = 1):

@@ -337,15 +337,22 @@ class QGYBJModel:
                  closure=None, vertical_diffusion=None, flow="evolving", feedback="none",
                  formulation=None, ybj_plus: bool = True, inviscid=False, linear=False,
                  no_dispersion=False, architecture: GPU = GPU(), verbose: bool = False):
+        self._configure(grid=grid, coriolis=coriolis, closure=closure, vertical_diffusion=vertical_diffusion,
+                        flow=flow, feedback=feedback, formulation=formulation, ybj_plus=ybj_plus,
+                        inviscid=inviscid, linear=linear, no_dispersion=no_dispersion,
+                        architecture=architecture)
+        self.stratification = self._stratification(stratification)
+        self.N2, self.a_ell = self._profiles(self.stratification)
+        self._h = _lib.Handle(**self._handle_args(self.N2, self.a_ell, 1))
+        self.fields = _Fields(self)
+        self.finalized = False
+
+    def _configure(self, *, grid, coriolis, closure, vertical_diffusion, flow, feedback, formulation,
+                   ybj_plus, inviscid, linear, no_dispersion, architecture):
         if not isinstance(grid, RectilinearGrid):
             raise ValueError("grid must be a RectilinearGrid")
         self.grid = grid
         self.coriolis = coriolis if isinstance(coriolis, FPlane) else FPlane(1e-4 if coriolis is None else coriolis)
-        if stratification is None:
-            stratification = ConstantStratification(1e-5)
-        elif isinstance(stratification, (int, float)):
-            stratification = ConstantStratification(stratification)
-        self.stratification = stratification
         self.closure = closure if closure is not None else HorizontalHyperdiffusivity()
         vd = vertical_diffusion if vertical_diffusion is not None else VerticalDiffusivity()
         self.vertical_diffusion = vd if isinstance(vd, VerticalDiffusivity) else VerticalDiffusivity(vd)
@@ -368,17 +375,28 @@ class QGYBJModel:
         self.dispersion = _component(no_dispersion, DispersionMode, {},
                                      {True: NoDispersion, False: Dispersive}, "no_dispersion")
         self.architecture = architecture
-
-        nx, ny, nz = grid.size
-        # N² on the bottom face of each cell (src/stratification.jl:308-318)
-        self.N2 = np.array([float(self.stratification.evaluate(grid.z[k] - grid.dz / 2))
-                            for k in range(nz)])
-        if not np.all(np.isfinite(self.N2)) or not np.all(self.N2 > 0):
-            raise ValueError("N² must be finite and positive at every level")
-        f = self.coriolis.f
-        self.a_ell = (f * f) / np.maximum(self.N2, SQRT_EPS)  # src/physics.jl:2-25
         self.dealias_mask = dealias_mask(grid)
 
+    @staticmethod
+    def _stratification(stratification):
+        if stratification is None:
+            return ConstantStratification(1e-5)
+        if isinstance(stratification, (int, float)):
+            return ConstantStratification(stratification)
+        return stratification
+
+    def _profiles(self, stratification):
+        grid = self.grid
+        nz = grid.size[2]
+        # N² on the bottom face of each cell (src/stratification.jl:308-318)
+        N2 = np.array([float(stratification.evaluate(grid.z[k] - grid.dz / 2)) for k in range(nz)])
+        if not np.all(np.isfinite(N2)) or not np.all(N2 > 0):
+            raise ValueError("N² must be finite and positive at every level")
+        f = self.coriolis.f
+        return N2, (f * f) / np.maximum(N2, SQRT_EPS)  # src/physics.jl:2-25
+
+    def _handle_args(self, N2, a_ell, members):
+        nx, ny, nz = self.grid.size
         cl = self.closure
         if isinstance(cl, HorizontalHyperdiffusivity):
             flow_terms = list(zip(cl.flow.coefficients, cl.flow.orders))
@@ -390,18 +408,17 @@ class QGYBJModel:
         else:
             raise ValueError("closure must be a hyperdiffusivity component")
         form = {YBJPlus: _lib.YBJ_PLUS, PassiveWave: _lib.PASSIVE, YBJ: _lib.YBJ}[type(self.formulation)]
-        self._h = _lib.Handle(
-            nx=nx, ny=ny, nz=nz, f=f, dz=grid.dz_solver, kx=grid.kx, ky=grid.ky, N2=self.N2,
-            a_ell=self.a_ell, flow_terms=flow_terms, wave_terms=wave_terms,
+        return dict(
+            nx=nx, ny=ny, nz=nz, f=self.coriolis.f, dz=self.grid.dz_solver, kx=self.grid.kx, ky=self.grid.ky,
+            N2=N2, a_ell=a_ell, flow_terms=flow_terms, wave_terms=wave_terms,
             nu_z=self.vertical_diffusion.coefficient,
             fixed_flow=isinstance(self.flow, FixedFlow),
             wave_feedback=isinstance(self.feedback, WaveMeanFeedback),
             formulation=form, inviscid=isinstance(self.dissipation, Inviscid),
             linear=isinstance(self.dynamics, LinearDynamics),
             no_dispersion=isinstance(self.dispersion, NoDispersion),
-            precision=_lib.PRECISION_MIXED if architecture.precision == "mixed" else _lib.PRECISION_FP64)
-        self.fields = _Fields(self)
-        self.finalized = False
+            precision=_lib.PRECISION_MIXED if self.architecture.precision == "mixed" else _lib.PRECISION_FP64,
+            members=members)
 
     # model-level operators (src/core/runtime.jl:169-206)
     def invert_q_to_psi(self):
@@ -439,6 +456,91 @@ def finalize_model_(model: QGYBJModel):
     return model.finalize()
 
 
+class _MemberFields:
+    """``fields`` of one ensemble member: (z, x, y) arrays of its nz levels."""
+
+    def __init__(self, ens, member):
+        object.__setattr__(self, "_e", ens)
+        object.__setattr__(self, "_m", member)
+
+    def __getattr__(self, name):
+        if name in _lib.FIELD_IDS:
+            return self._e._h.get_field_member(self._m, name)
+        raise AttributeError(name)
+
+    def __setattr__(self, name, value):
+        if name in _lib.FIELD_IDS:
+            self._e._h.set_field_member(self._m, name, value, _lib.SPACE_SPECTRAL)
+        else:
+            raise AttributeError(name)
+
+
+class QGYBJEnsemble(QGYBJModel):
+    """Ensemble of models of one grid class on one GPU.
+
+    Every setting is shared except the stratification: ``stratifications``
+    gives one per member.  The members are stacked as extra levels of one
+    device model, so every pass of the ETD-RK2 step covers all of them in one
+    launch and ``step_`` advances all members together.  Member m is read and
+    set through ``member_fields(m)`` and ``set_member``; its results are those
+    of the same model run alone."""
+
+    def __init__(self, *, stratifications, **kwargs):
+        strat = list(stratifications)
+        if not strat:
+            raise ValueError("an ensemble needs at least one member")
+        kwargs.pop("stratification", None)
+        self._configure(**{k: kwargs.get(k, d) for k, d in (
+            ("grid", None), ("coriolis", None), ("closure", None), ("vertical_diffusion", None),
+            ("flow", "evolving"), ("feedback", "none"), ("formulation", None), ("ybj_plus", True),
+            ("inviscid", False), ("linear", False), ("no_dispersion", False), ("architecture", GPU()))})
+        self.stratifications = [self._stratification(x) for x in strat]
+        prof = [self._profiles(x) for x in self.stratifications]
+        self.members = len(prof)
+        self.N2 = np.concatenate([p[0] for p in prof])
+        self.a_ell = np.concatenate([p[1] for p in prof])
+        self._h = _lib.Handle(**self._handle_args(self.N2, self.a_ell, self.members))
+        self.fields = _Fields(self)
+        self.finalized = False
+
+    def member_fields(self, member: int):
+        if not 0 <= member < self.members:
+            raise IndexError(f"member {member} out of range")
+        return _MemberFields(self, member)
+
+    def set_member(self, member: int, *, psi=None, pv_method: str = "qg", waves=None):
+        """``set!`` restricted to one member (the others are not touched)."""
+        if self.finalized:
+            raise RuntimeError("cannot modify a finalized model")
+        h = self._h
+        if psi is not None:
+            if callable(psi):
+                h.set_field_member(member, "psi", _evaluate_on_nodes(self.grid, psi), _lib.SPACE_PHYSICAL)
+            else:
+                vals, space = _field_values_zxy(self, psi)
+                h.set_field_member(member, "psi", vals,
+                                   _lib.SPACE_PHYSICAL if space == "physical" else _lib.SPACE_SPECTRAL)
+            if pv_method in ("qg", "balanced", "barotropic", "asselin"):
+                h.compute_q_from_psi_member(member, barotropic=pv_method in ("barotropic", "asselin"))
+            elif pv_method != "none":
+                raise ValueError("pv_method must be :qg, :barotropic, or :none")
+        if waves is not None:
+            if isinstance(waves, SurfaceWave):
+                h.set_field_member(member, "B", _surface_wave_values(self.grid, waves), _lib.SPACE_PHYSICAL)
+            else:
+                vals, space = _field_values_zxy(self, waves)
+                h.set_field_member(member, "B", vals,
+                                   _lib.SPACE_PHYSICAL if space == "physical" else _lib.SPACE_SPECTRAL)
+            h.invert_B_to_A_member(member)
+        return self
+
+    def check_member(self, member: int):
+        return self._h.check_member(member)
+
+    def __repr__(self):
+        return f"QGYBJEnsemble(grid={self.grid.size}, members={self.members}, architecture={self.architecture})"
+
+
 # ---------------------------------------------------------------------------
 # set! (src/simulation.jl:230-329, 405-503)
 # ---------------------------------------------------------------------------
@@ -472,6 +574,19 @@ def _field_values_zxy(model, source) -> Tuple[np.ndarray, str]:
     return vals.astype(np.complex128), space
 
 
+def _surface_wave_values(g, wave) -> np.ndarray:
+    """Physical B of set_surface_waves! (src/initialization.jl:552-595): the
+    amplitude times the vertical profile at each cell depth."""
+    nx, ny, nz = g.size
+    phys = np.empty((nz, nx, ny), dtype=np.complex128)
+    for k in range(nz):
+        depth = max(0.0, -g.z[k] - g.dz / 2)
+        prof = math.exp(-(depth ** 2) / wave.scale ** 2) if wave.profile == "gaussian" \
+            else math.exp(-depth / wave.scale)
+        phys[k] = complex(wave.amplitude * prof)
+    return phys
+
+
 def set_(model, *, psi=None, mean_flow=None, pv_method: str = "qg", waves=None, B=None,
          verbose: bool = False):
     """``set!(model; ψ, pv_method, waves)`` (src/simulation.jl:461-503)."""
@@ -500,15 +615,7 @@ def set_(model, *, psi=None, mean_flow=None, pv_method: str = "qg", waves=None, 
         h.compute_velocities(compute_w=False)
     if wave is not None:
         if isinstance(wave, SurfaceWave):
-            g = model.grid
-            nx, ny, nz = g.size
-            phys = np.empty((nz, nx, ny), dtype=np.complex128)
-            for k in range(nz):
-                depth = max(0.0, -g.z[k] - g.dz / 2)
-                prof = math.exp(-(depth ** 2) / wave.scale ** 2) if wave.profile == "gaussian" \
-                    else math.exp(-depth / wave.scale)
-                phys[k] = complex(wave.amplitude * prof)
-            h.set_field("B", phys, _lib.SPACE_PHYSICAL)
+            h.set_field("B", _surface_wave_values(model.grid, wave), _lib.SPACE_PHYSICAL)
         elif isinstance(wave, (FieldArray, np.ndarray)):
             vals, space = _field_values_zxy(model, wave)
             h.set_field("B", vals, _lib.SPACE_PHYSICAL if space == "physical" else _lib.SPACE_SPECTRAL)

@@ -20,7 +20,13 @@ __global__ void k_q_from_psi(Geo g, const cplx<double>* __restrict__ psi, cplx<d
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= plane) return;
   const int x = col % g.nx, y = col / g.nx;
-  const int nz = g.nz;
+  const int nz = g.nzc;
+  {
+    const size_t m = blockIdx.y;  // member
+    psi += m * nz * plane;
+    q += m * nz * plane;
+    a += m * nz;
+  }
   const double kh2 = g.kx[x] * g.kx[x] + g.ky[y] * g.ky[y];
   auto at = [&](int k) { return (size_t)k * plane + col; };
   if (barotropic || nz == 1) {
@@ -124,7 +130,7 @@ void launch_q_from_psi(const Geo& g, hipStream_t st, const cplx<double>* psi, cp
                        const double* a, int barotropic) {
   const int plane = g.nx * g.ny;
   const int bs = 64;
-  hipLaunchKernelGGL(k_q_from_psi, dim3((plane + bs - 1) / bs), dim3(bs), 0, st, g, psi, q, a, barotropic);
+  hipLaunchKernelGGL(k_q_from_psi, dim3((plane + bs - 1) / bs, g.members), dim3(bs), 0, st, g, psi, q, a, barotropic);
 }
 
 template <typename T>

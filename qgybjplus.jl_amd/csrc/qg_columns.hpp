@@ -16,8 +16,9 @@ void launch_thomas_factor(const Geo& g, hipStream_t st, const double* a, int kdi
                           int* err);
 bool thomas_setup();
 // `al` is a shifted by one level (al[k] = a[k-1], al[0] = 0), zero-padded to
-// thomas_shifted_a_size(nz) entries.
-int thomas_shifted_a_size(int nz);
+// thomas_shifted_a_size(nzc) entries per member.  Per-member arrays (a, N2
+// derived) are concatenated in member order.
+int thomas_shifted_a_size(int nzc);
 // Thomas solves of all columns; C (= dA/dz) is written when Cout is set.
 void launch_thomas(const Geo& g, hipStream_t st, const cplx<double>* rhs, cplx<double>* out,
                    cplx<double>* Cout, const double* inv, const double* cp, const double* al, int kdiv,

@@ -43,8 +43,13 @@ __host__ __device__ inline cplx<To> cvt(cplx<From> a) { return mk((To)a.re, (To)
 
 // Grid description passed by value to every kernel.  Device layout of every
 // 3-D field is planar: index (z*ny + y)*stride + x, x fastest.
+// nz counts all stacked levels: an ensemble of `members` models of one grid
+// class keeps member m in levels [m*nzc, (m+1)*nzc), so every horizontal pass
+// covers all members in one launch; column operators work per member.
 struct Geo {
   int nx, ny, nz;
+  int nzc;      // levels per member (the column depth)
+  int members;  // nz == members * nzc
   int W;    // nx/2 + 1 : columns kept by half-spectrum (real-field) arrays
   int Wp;   // row stride of half-spectrum arrays (W rounded up to 16)
   const double* kx;            // [nx]

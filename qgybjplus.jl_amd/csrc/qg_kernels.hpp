@@ -443,15 +443,17 @@ struct Etd {
 };
 
 // Vertical diffusion nu_z * d2q/dz2 with Neumann ends (dissipation_q_nv!, nonlinear.jl:1010-1044).
-__device__ inline cplx<double> vdiff(const cplx<double>* __restrict__ q, int z, int nz, size_t col,
+// z is the stacked level; the column of its member has nzc levels.
+__device__ inline cplx<double> vdiff(const cplx<double>* __restrict__ q, int z, int nzc, size_t col,
                                      size_t plane, double nu, double inv_dz2) {
-  if (nz <= 1) return czero<double>();
+  if (nzc <= 1) return czero<double>();
+  const int zl = z % nzc;
   cplx<double> c = q[(size_t)z * plane + col];
   cplx<double> r;
-  if (z == 0) {
-    cplx<double> up = q[plane + col];
+  if (zl == 0) {
+    cplx<double> up = q[(size_t)(z + 1) * plane + col];
     r = up - c;
-  } else if (z == nz - 1) {
+  } else if (zl == nzc - 1) {
     cplx<double> dn = q[(size_t)(z - 1) * plane + col];
     r = dn - c;
   } else {
@@ -516,7 +518,7 @@ k_yb2_q(Geo g, int mode, int flags, const cplx<R>* __restrict__ G1, const cplx<R
         cplx<double> q0 = qn[ad];
         cplx<double> rhs = czero<double>();
         if (keep && !(flags & YQ_FIXED)) {
-          cplx<double> dq = (flags & YQ_DIFF) ? vdiff(qd, z, g.nz, col, plane, nu, inv_dz2) : czero<double>();
+          cplx<double> dq = (flags & YQ_DIFF) ? vdiff(qd, z, g.nzc, col, plane, nu, inv_dz2) : czero<double>();
           rhs = mk<double>(-n.re + dq.re, -n.im + dq.im);
         }
         cplx<double> out = czero<double>();
@@ -655,7 +657,7 @@ k_yf_omega(Geo g, const cplx<double>* __restrict__ psi, cplx<R>* __restrict__ Pb
   const int mx = mirror(x, g.nx);
   cplx<R>* sl = lds + s * TL::LD;
   const double kxv = ok ? g.kx[x] : 0.0, kxm = ok ? g.kx[mx] : 0.0;
-  const bool top = z == g.nz - 1;
+  const bool top = z % g.nzc == g.nzc - 1;  // top level of this member's column
   const int zu = top ? z : z + 1;
   const double dz = g.dz;
   cplx<R> v[4][F::IT][F::R0];

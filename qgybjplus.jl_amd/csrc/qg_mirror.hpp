@@ -83,8 +83,9 @@ struct MirrorSlot {
   }
 };
 
-// Phase 1.  xs: [kpad][NC] complex, t0/t1: [kpad][QC].  The right-hand side
-// of level k (0 <= k < nz) is multiplied by scale(k), which is 0 where the
+// Phase 1.  xs: [kpad][NC] complex, t0/t1: [kpad][QC].  rhs and the tables
+// point at the member's first level; the column has g.nzc levels.  The
+// right-hand side of level k is multiplied by scale(k), which is 0 where the
 // operator has no row.
 template <int QC, class Scale>
 __device__ inline void mirror_stage(const Geo& g, const Quarter& qr, int rc0, const MirrorSlot<QC>& sl,
@@ -93,7 +94,7 @@ __device__ inline void mirror_stage(const Geo& g, const Quarter& qr, int rc0, co
                                     int kpad, Scale scale) {
   constexpr int NC = MirrorSlot<QC>::NC, kstep = MirrorSlot<QC>::kstep;
   constexpr int TU = kMirrorLoads * QC / NC;  // table loads per pass (= 4)
-  const int nz = g.nz;
+  const int nz = g.nzc;
   const size_t plane = (size_t)g.nx * g.ny;
   const int rplane = qr.size();
   const int j = threadIdx.x % QC, kt = threadIdx.x / QC, ktstep = kMirrorThreads / QC;

@@ -42,7 +42,11 @@ __global__ void k_omega_factor(Geo g, Quarter qr, const double* __restrict__ cz,
   const int rc = blockIdx.x * blockDim.x + threadIdx.x;
   if (rc >= rplane) return;
   const int x = rc % qr.Wr, y = rc / qr.Wr;
-  const int nz = g.nz;
+  const int nz = g.nzc, m = blockIdx.y;  // member m: its own N2(z) and tables
+  cz += (size_t)m * nz;
+  M += (size_t)m * nz * rplane;
+  D += (size_t)m * nz * rplane;
+  pivoted += (size_t)m * rplane;
   auto at = [&](int k) { return (size_t)k * rplane + rc; };
   for (int k = 0; k < nz; ++k) {
     M[at(k)] = 0.0;
@@ -156,8 +160,9 @@ __device__ void gtsv_general(double* b0, int TC2, int n, double kh2, const doubl
 }
 
 // rhsw: the omega right-hand side (complex, all levels); wk: w-hat, zero on
-// the end levels and where kh = 0.  cu[k] = cz[k] for 1 <= k <= nz-3 (the
-// upper diagonal of row k-1), zero elsewhere, padded to mirror_pad(nz).
+// the end levels and where kh = 0.  blockIdx.y is the member.  cu[k] = cz[k]
+// for 1 <= k <= nzc-3 (the upper diagonal of row k-1), zero elsewhere, padded
+// to mirror_pad(nzc) per member; cz and fN2 have nzc entries per member.
 template <int QC>
 __global__ void __launch_bounds__(kMirrorThreads)
 k_omega(Geo g, Quarter qr, const cplx<double>* __restrict__ rhsw, cplx<double>* __restrict__ wk,
@@ -166,8 +171,23 @@ k_omega(Geo g, Quarter qr, const cplx<double>* __restrict__ rhsw, cplx<double>* 
         double* __restrict__ sd, double* __restrict__ sdu, double* __restrict__ sfill) {
   constexpr int NC = 4 * QC;
   extern __shared__ cplx<double> xs[];  // [kpad][NC], then tm[kpad][QC], td[kpad][QC]
-  const int nz = g.nz, kpad = mirror_pad(nz);
+  const int nz = g.nzc, kpad = mirror_pad(nz);
   const int rc0 = blockIdx.x * QC;
+  const size_t plane = (size_t)g.nx * g.ny;
+  {
+    const size_t m = blockIdx.y;
+    rhsw += m * nz * plane;
+    wk += m * nz * plane;
+    M += m * nz * qr.size();
+    D += m * nz * qr.size();
+    pivoted += m * qr.size();
+    cz += m * nz;
+    cu += m * kpad;
+    fN2 += m * nz;
+    sd += m * nz * plane;
+    sdu += m * nz * plane;
+    sfill += m * nz * plane;
+  }
   double* tm = reinterpret_cast<double*>(xs + (size_t)kpad * NC);
   double* td = tm + (size_t)kpad * QC;
   const MirrorSlot<QC> sl(g, qr, rc0);
@@ -202,7 +222,6 @@ k_omega(Geo g, Quarter qr, const cplx<double>* __restrict__ rhsw, cplx<double>* 
   __syncthreads();
 
   if (!sl.q.active) return;
-  const size_t plane = (size_t)g.nx * g.ny;
   for (int k = sl.kf; k < nz; k += MirrorSlot<QC>::kstep) wk[(size_t)k * plane + sl.off] = xs[k * NC + sl.c];
 }
 
@@ -212,8 +231,9 @@ void launch_qc(const Geo& g, hipStream_t st, const cplx<double>* rhsw, cplx<doub
                double* sd, double* sdu, double* sfill) {
   const Quarter qr = quarter(g);
   const int blocks = (qr.size() + QC - 1) / QC;
-  hipLaunchKernelGGL((k_omega<QC>), dim3(blocks), dim3(kMirrorThreads), mirror_lds_bytes(mirror_pad(g.nz), QC), st,
-                     g, qr, rhsw, wk, M, D, pivoted, cz, cu, fN2, sd, sdu, sfill);
+  hipLaunchKernelGGL((k_omega<QC>), dim3(blocks, g.members), dim3(kMirrorThreads),
+                     mirror_lds_bytes(mirror_pad(g.nzc), QC), st, g, qr, rhsw, wk, M, D, pivoted, cz, cu, fN2, sd, sdu,
+                     sfill);
 }
 
 template <int QC>
@@ -228,8 +248,8 @@ void launch_omega_factor(const Geo& g, hipStream_t st, const double* cz, double*
                          unsigned char* pivoted, int* err) {
   const Quarter qr = quarter(g);
   const int bs = 64;
-  hipLaunchKernelGGL(k_omega_factor, dim3((qr.size() + bs - 1) / bs), dim3(bs), 0, st, g, qr, cz, M, D, pivoted,
-                     err);
+  hipLaunchKernelGGL(k_omega_factor, dim3((qr.size() + bs - 1) / bs, g.members), dim3(bs), 0, st, g, qr, cz, M, D,
+                     pivoted, err);
 }
 
 bool omega_setup() { return set_lds_attr<8>() && set_lds_attr<4>() && set_lds_attr<2>() && set_lds_attr<1>(); }
@@ -237,7 +257,7 @@ bool omega_setup() { return set_lds_attr<8>() && set_lds_attr<4>() && set_lds_at
 void launch_omega(const Geo& g, hipStream_t st, const cplx<double>* rhsw, cplx<double>* wk, const double* M,
                   const double* D, const unsigned char* pivoted, const double* cz, const double* cu,
                   const double* fN2, double* sd, double* sdu, double* sfill) {
-  switch (mirror_qc(g.nz)) {
+  switch (mirror_qc(g.nzc)) {
     case 8: launch_qc<8>(g, st, rhsw, wk, M, D, pivoted, cz, cu, fN2, sd, sdu, sfill); return;
     case 4: launch_qc<4>(g, st, rhsw, wk, M, D, pivoted, cz, cu, fN2, sd, sdu, sfill); return;
     case 2: launch_qc<2>(g, st, rhsw, wk, M, D, pivoted, cz, cu, fN2, sd, sdu, sfill); return;

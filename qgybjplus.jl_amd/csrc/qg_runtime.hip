@@ -34,7 +34,8 @@ struct QgError : std::runtime_error {
 }  // namespace
 
 struct qg_handle {
-  int nx = 0, ny = 0, nz = 0, precision = 0;
+  // nz counts all stacked levels: members * nzc (see qg::Geo)
+  int nx = 0, ny = 0, nz = 0, nzc = 0, members = 1, precision = 0;
   double f = 0, dz = 1, nu_z = 0;
   bool fixed_flow = false, feedback = false, passive = false, inviscid = false, linear = false,
        no_disp = false;
@@ -467,15 +468,20 @@ void* real_field(qg_handle& h, int field) {
 }
 
 // planar double field -> device planar field in place: forward FFT.
-void fft2_inplace(qg_handle& h, D* f, int dir, unsigned long long* absmax = nullptr, D* out = nullptr) {
+// 2-D transforms of `levels` planes starting at f (all stacked levels by
+// default).
+void fft2_inplace(qg_handle& h, D* f, int dir, unsigned long long* absmax = nullptr, D* out = nullptr,
+                  int levels = -1) {
+  qg::Geo g = h.g;
+  if (levels >= 0) g.nz = levels;
   // y first for the inverse (spectral -> physical), x first for the forward
   if (dir < 0) {
-    qg::Ops<double>::x_c2c(h.g, h.st, -1, f, f, nullptr, h.tw64x);
-    qg::Ops<double>::y_c2c(h.g, h.st, -1, f, out ? out : f, h.tw64y);
+    qg::Ops<double>::x_c2c(g, h.st, -1, f, f, nullptr, h.tw64x);
+    qg::Ops<double>::y_c2c(g, h.st, -1, f, out ? out : f, h.tw64y);
   } else {
     D* o = out ? out : f;
-    qg::Ops<double>::y_c2c(h.g, h.st, +1, f, o, h.tw64y);
-    qg::Ops<double>::x_c2c(h.g, h.st, +1, o, absmax ? nullptr : o, absmax, h.tw64x);
+    qg::Ops<double>::y_c2c(g, h.st, +1, f, o, h.tw64y);
+    qg::Ops<double>::x_c2c(g, h.st, +1, o, absmax ? nullptr : o, absmax, h.tw64x);
   }
 }
 
@@ -505,6 +511,9 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
                         std::to_string(cfg->nx) + "x" + std::to_string(cfg->ny) + ")");
     if (!qg::column_depth_supported(cfg->nz))
       throw QgError(QG_ERR_UNSUPPORTED, "nz must be between 1 and 1024 for the column solvers");
+    const int members = cfg->members > 0 ? cfg->members : 1;
+    if ((long long)members * cfg->nz > (1 << 20))
+      throw QgError(QG_ERR_ARGUMENT, "members * nz must not exceed 2^20 stacked levels");
     if (cfg->formulation == QG_YBJ)
       throw QgError(QG_ERR_UNSUPPORTED, "the normal YBJ formulation is not part of this library; use YBJ+");
     if (cfg->formulation != QG_YBJ_PLUS && cfg->formulation != QG_PASSIVE)
@@ -529,7 +538,9 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
     h = new qg_handle();
     h->nx = cfg->nx;
     h->ny = cfg->ny;
-    h->nz = cfg->nz;
+    h->nzc = cfg->nz;
+    h->members = members;
+    h->nz = members * cfg->nz;
     h->precision = cfg->precision;
     h->f = cfg->f;
     h->dz = cfg->nz > 1 ? cfg->dz : 1.0;
@@ -542,8 +553,8 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
     h->no_disp = cfg->no_dispersion != 0;
     h->kx.assign(cfg->kx, cfg->kx + cfg->nx);
     h->ky.assign(cfg->ky, cfg->ky + cfg->ny);
-    h->N2.assign(cfg->N2, cfg->N2 + cfg->nz);
-    h->a.assign(cfg->a_ell, cfg->a_ell + cfg->nz);
+    h->N2.assign(cfg->N2, cfg->N2 + h->nz);  // per member, concatenated
+    h->a.assign(cfg->a_ell, cfg->a_ell + h->nz);
     if (cfg->n_flow_terms > 0) {
       h->fc.assign(cfg->flow_coefficients, cfg->flow_coefficients + cfg->n_flow_terms);
       h->fo.assign(cfg->flow_orders, cfg->flow_orders + cfg->n_flow_terms);
@@ -579,17 +590,20 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
     HIPCHECK(hipMemcpy(h->d_kx, h->kx.data(), nx * sizeof(double), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(h->d_ky, h->ky.data(), ny * sizeof(double), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(h->d_a, h->a.data(), nz * sizeof(double), hipMemcpyHostToDevice));
+    const int nzc = h->nzc, kpad = qg::thomas_shifted_a_size(nzc);
     {
-      std::vector<double> al(qg::thomas_shifted_a_size(nz), 0.0);
-      for (int k = 1; k < nz; ++k) al[k] = h->a[k - 1];
+      std::vector<double> al((size_t)members * kpad, 0.0);
+      for (int m = 0; m < members; ++m)
+        for (int k = 1; k < nzc; ++k) al[(size_t)m * kpad + k] = h->a[(size_t)m * nzc + k - 1];
       h->d_al = h->alloc<double>(al.size());
       HIPCHECK(hipMemcpy(h->d_al, al.data(), al.size() * sizeof(double), hipMemcpyHostToDevice));
     }
     HIPCHECK(hipMemcpy(h->d_cz, cz.data(), nz * sizeof(double), hipMemcpyHostToDevice));
     {
       // upper diagonal of the omega interior row k-1, by level
-      std::vector<double> cu(qg::thomas_shifted_a_size(nz), 0.0);
-      for (int k = 1; k <= nz - 3; ++k) cu[k] = cz[k];
+      std::vector<double> cu((size_t)members * kpad, 0.0);
+      for (int m = 0; m < members; ++m)
+        for (int k = 1; k <= nzc - 3; ++k) cu[(size_t)m * kpad + k] = cz[(size_t)m * nzc + k];
       h->d_cu = h->alloc<double>(cu.size());
       HIPCHECK(hipMemcpy(h->d_cu, cu.data(), cu.size() * sizeof(double), hipMemcpyHostToDevice));
     }
@@ -653,6 +667,8 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
     g.nz = nz;
     g.W = W;
     g.Wp = Wp;
+    g.nzc = h->nzc;
+    g.members = h->members;
     g.kx = h->d_kx;
     g.ky = h->d_ky;
     g.mask = h->d_mask;
@@ -678,7 +694,7 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
       std::memcpy(h->sing_A, he + 4, 3 * sizeof(int));
       h->om_M = h->alloc<double>(tsz);
       h->om_D = h->alloc<double>(tsz);
-      h->om_piv = h->alloc<unsigned char>(tsz / nz);
+      h->om_piv = h->alloc<unsigned char>(tsz / nzc);
       HIPCHECK(hipMemset(e, 0, 8 * sizeof(int)));
       qg::launch_omega_factor(g, h->st, h->d_cz, h->om_M, h->om_D, h->om_piv, e);
       HIPCHECK(hipMemcpyAsync(he, e, sizeof he, hipMemcpyDeviceToHost, h->st));
@@ -704,55 +720,125 @@ int qg_destroy(qg_handle* h) {
   return QG_OK;
 }
 
-int qg_set_field(qg_handle* h, int field, const double* host, int space) {
-  return guarded(h, [&] {
-    if (!host) throw QgError(QG_ERR_ARGUMENT, "null host buffer");
-    qg_handle& H = *h;
-    if (D* dst = complex_field(H, field)) {
-      HIPCHECK(hipMemcpyAsync(H.stg, host, H.npts * sizeof(D), hipMemcpyHostToDevice, H.st));
-      qg::launch_transpose<D>(H.stg, dst, H.nx, H.ny, H.nz, true, H.st);
-      if (space == QG_SPACE_PHYSICAL) fft2_inplace(H, dst, -1);
-      else if (space != QG_SPACE_SPECTRAL) throw QgError(QG_ERR_ARGUMENT, "space must be spectral or physical");
-    } else if (void* rdst = real_field(H, field)) {
-      double* tmp = reinterpret_cast<double*>(H.stg);
-      double* pl = reinterpret_cast<double*>(H.qeff);
-      HIPCHECK(hipMemcpyAsync(tmp, host, H.npts * sizeof(double), hipMemcpyHostToDevice, H.st));
-      if (H.precision == QG_PRECISION_MIXED) {
-        qg::launch_transpose<double>(tmp, pl, H.nx, H.ny, H.nz, true, H.st);
-        qg::launch_convert<double, float>(pl, static_cast<float*>(rdst), H.npts, H.st);
-      } else {
-        qg::launch_transpose<double>(tmp, static_cast<double*>(rdst), H.nx, H.ny, H.nz, true, H.st);
-      }
+}  // extern "C"
+
+namespace {
+
+// Field exchange for members [m0, m0 + nm): a (z, x, y) array of nm * nzc
+// levels (member-major along z).
+void set_field_range(qg_handle& H, int field, const double* host, int space, int m0, int nm) {
+  if (!host) throw QgError(QG_ERR_ARGUMENT, "null host buffer");
+  const int lv = nm * H.nzc;
+  const size_t off = (size_t)m0 * H.nzc * H.plane, n = (size_t)lv * H.plane;
+  if (D* dst = complex_field(H, field)) {
+    HIPCHECK(hipMemcpyAsync(H.stg, host, n * sizeof(D), hipMemcpyHostToDevice, H.st));
+    qg::launch_transpose<D>(H.stg, dst + off, H.nx, H.ny, lv, true, H.st);
+    if (space == QG_SPACE_PHYSICAL) fft2_inplace(H, dst + off, -1, nullptr, nullptr, lv);
+    else if (space != QG_SPACE_SPECTRAL) throw QgError(QG_ERR_ARGUMENT, "space must be spectral or physical");
+  } else if (void* rdst = real_field(H, field)) {
+    double* tmp = reinterpret_cast<double*>(H.stg);
+    double* pl = reinterpret_cast<double*>(H.qeff);
+    HIPCHECK(hipMemcpyAsync(tmp, host, n * sizeof(double), hipMemcpyHostToDevice, H.st));
+    if (H.precision == QG_PRECISION_MIXED) {
+      qg::launch_transpose<double>(tmp, pl, H.nx, H.ny, lv, true, H.st);
+      qg::launch_convert<double, float>(pl, static_cast<float*>(rdst) + off, n, H.st);
     } else {
-      throw QgError(QG_ERR_ARGUMENT, "unknown field id " + std::to_string(field));
+      qg::launch_transpose<double>(tmp, static_cast<double*>(rdst) + off, H.nx, H.ny, lv, true, H.st);
     }
-    HIPCHECK(hipStreamSynchronize(H.st));
-    H.diag_valid = false;
-    H.t1_valid = false;
-  });
+  } else {
+    throw QgError(QG_ERR_ARGUMENT, "unknown field id " + std::to_string(field));
+  }
+  HIPCHECK(hipStreamSynchronize(H.st));
+  H.diag_valid = false;
+  H.t1_valid = false;
+}
+
+void get_field_range(qg_handle& H, int field, double* host, int m0, int nm) {
+  if (!host) throw QgError(QG_ERR_ARGUMENT, "null host buffer");
+  const int lv = nm * H.nzc;
+  const size_t off = (size_t)m0 * H.nzc * H.plane, n = (size_t)lv * H.plane;
+  if (D* src = complex_field(H, field)) {
+    qg::launch_transpose<D>(src + off, H.stg, H.nx, H.ny, lv, false, H.st);
+    HIPCHECK(hipMemcpyAsync(host, H.stg, n * sizeof(D), hipMemcpyDeviceToHost, H.st));
+  } else if (void* rsrc = real_field(H, field)) {
+    double* tmp = reinterpret_cast<double*>(H.stg);
+    const double* pl = static_cast<const double*>(rsrc) + off;
+    if (H.precision == QG_PRECISION_MIXED) {
+      double* conv = reinterpret_cast<double*>(H.qeff);
+      qg::launch_convert<float, double>(static_cast<const float*>(rsrc) + off, conv, n, H.st);
+      pl = conv;
+    }
+    qg::launch_transpose<double>(pl, tmp, H.nx, H.ny, lv, false, H.st);
+    HIPCHECK(hipMemcpyAsync(host, tmp, n * sizeof(double), hipMemcpyDeviceToHost, H.st));
+  } else {
+    throw QgError(QG_ERR_ARGUMENT, "unknown field id " + std::to_string(field));
+  }
+  HIPCHECK(hipStreamSynchronize(H.st));
+}
+
+void check_range(qg_handle& H, int m0, int nm, int* nonfinite, double* max_abs_psi) {
+  const int lv = nm * H.nzc;
+  const size_t off = (size_t)m0 * H.nzc * H.plane, n = (size_t)lv * H.plane;
+  HIPCHECK(hipMemsetAsync(H.d_flag, 0, sizeof(int), H.st));
+  HIPCHECK(hipMemsetAsync(H.d_absmax, 0, sizeof(unsigned long long), H.st));
+  qg::launch_nonfinite(H.q[H.cur] + off, n, H.d_flag, H.st);
+  qg::launch_nonfinite(H.B[H.cur] + off, n, H.d_flag, H.st);
+  qg::launch_nonfinite(H.psi + off, n, H.d_flag, H.st);
+  fft2_inplace(H, H.psi + off, +1, H.d_absmax, H.stg, lv);
+  int flag = 0;
+  unsigned long long bits = 0;
+  HIPCHECK(hipMemcpyAsync(&flag, H.d_flag, sizeof(int), hipMemcpyDeviceToHost, H.st));
+  HIPCHECK(hipMemcpyAsync(&bits, H.d_absmax, sizeof(bits), hipMemcpyDeviceToHost, H.st));
+  HIPCHECK(hipStreamSynchronize(H.st));
+  double m;
+  std::memcpy(&m, &bits, sizeof m);
+  if (nonfinite) *nonfinite = flag;
+  if (max_abs_psi) *max_abs_psi = m;
+}
+
+// One member seen as a single model: its Geo and the offsets of its levels,
+// its Thomas tables and its shifted coefficient array.
+struct MemberView {
+  qg::Geo g;
+  size_t off, toff, aoff;
+  MemberView(const qg_handle& H, int m) : g(H.g) {
+    g.nz = H.nzc;
+    g.members = 1;
+    off = (size_t)m * H.nzc * H.plane;
+    toff = (size_t)m * qg::thomas_table_size(g);
+    aoff = (size_t)m * qg::thomas_shifted_a_size(H.nzc);
+  }
+};
+
+void check_member_index(const qg_handle& H, int member) {
+  if (member < 0 || member >= H.members)
+    throw QgError(QG_ERR_ARGUMENT, "member " + std::to_string(member) + " out of range [0, " +
+                                       std::to_string(H.members) + ")");
+}
+
+}  // namespace
+
+extern "C" {
+
+int qg_set_field(qg_handle* h, int field, const double* host, int space) {
+  return guarded(h, [&] { set_field_range(*h, field, host, space, 0, h->members); });
 }
 
 int qg_get_field(qg_handle* h, int field, double* host) {
+  return guarded(h, [&] { get_field_range(*h, field, host, 0, h->members); });
+}
+
+int qg_set_field_member(qg_handle* h, int member, int field, const double* host, int space) {
   return guarded(h, [&] {
-    if (!host) throw QgError(QG_ERR_ARGUMENT, "null host buffer");
-    qg_handle& H = *h;
-    if (D* src = complex_field(H, field)) {
-      qg::launch_transpose<D>(src, H.stg, H.nx, H.ny, H.nz, false, H.st);
-      HIPCHECK(hipMemcpyAsync(host, H.stg, H.npts * sizeof(D), hipMemcpyDeviceToHost, H.st));
-    } else if (void* rsrc = real_field(H, field)) {
-      double* tmp = reinterpret_cast<double*>(H.stg);
-      const double* pl = static_cast<const double*>(rsrc);
-      if (H.precision == QG_PRECISION_MIXED) {
-        double* conv = reinterpret_cast<double*>(H.qeff);
-        qg::launch_convert<float, double>(static_cast<const float*>(rsrc), conv, H.npts, H.st);
-        pl = conv;
-      }
-      qg::launch_transpose<double>(pl, tmp, H.nx, H.ny, H.nz, false, H.st);
-      HIPCHECK(hipMemcpyAsync(host, tmp, H.npts * sizeof(double), hipMemcpyDeviceToHost, H.st));
-    } else {
-      throw QgError(QG_ERR_ARGUMENT, "unknown field id " + std::to_string(field));
-    }
-    HIPCHECK(hipStreamSynchronize(H.st));
+    check_member_index(*h, member);
+    set_field_range(*h, field, host, space, member, 1);
+  });
+}
+
+int qg_get_field_member(qg_handle* h, int member, int field, double* host) {
+  return guarded(h, [&] {
+    check_member_index(*h, member);
+    get_field_range(*h, field, host, member, 1);
   });
 }
 
@@ -786,23 +872,13 @@ int qg_synchronize(qg_handle* h) {
 }
 
 int qg_check(qg_handle* h, int* nonfinite, double* max_abs_psi) {
+  return guarded(h, [&] { check_range(*h, 0, h->members, nonfinite, max_abs_psi); });
+}
+
+int qg_check_member(qg_handle* h, int member, int* nonfinite, double* max_abs_psi) {
   return guarded(h, [&] {
-    qg_handle& H = *h;
-    HIPCHECK(hipMemsetAsync(H.d_flag, 0, sizeof(int), H.st));
-    HIPCHECK(hipMemsetAsync(H.d_absmax, 0, sizeof(unsigned long long), H.st));
-    qg::launch_nonfinite(H.q[H.cur], H.npts, H.d_flag, H.st);
-    qg::launch_nonfinite(H.B[H.cur], H.npts, H.d_flag, H.st);
-    qg::launch_nonfinite(H.psi, H.npts, H.d_flag, H.st);
-    fft2_inplace(H, H.psi, +1, H.d_absmax, H.stg);
-    int flag = 0;
-    unsigned long long bits = 0;
-    HIPCHECK(hipMemcpyAsync(&flag, H.d_flag, sizeof(int), hipMemcpyDeviceToHost, H.st));
-    HIPCHECK(hipMemcpyAsync(&bits, H.d_absmax, sizeof(bits), hipMemcpyDeviceToHost, H.st));
-    HIPCHECK(hipStreamSynchronize(H.st));
-    double m;
-    std::memcpy(&m, &bits, sizeof m);
-    if (nonfinite) *nonfinite = flag;
-    if (max_abs_psi) *max_abs_psi = m;
+    check_member_index(*h, member);
+    check_range(*h, member, 1, nonfinite, max_abs_psi);
   });
 }
 
@@ -819,6 +895,31 @@ int qg_invert_B_to_A(qg_handle* h) {
     solve_A(*h, h->B[h->cur], h->A, h->C);
     h->diag_valid = false;
     check_status(*h);
+  });
+}
+
+int qg_invert_B_to_A_member(qg_handle* h, int member) {
+  return guarded(h, [&] {
+    qg_handle& H = *h;
+    check_member_index(H, member);
+    if (H.sing_A[0]) raise_singular(H.sing_A);
+    const MemberView mv(H, member);
+    qg::launch_thomas(mv.g, H.st, H.B[H.cur] + mv.off, H.A + mv.off, H.C + mv.off, H.th_inv[1] + mv.toff,
+                      H.th_cp[1] + mv.toff, H.d_al + mv.aoff, 4, nullptr);
+    H.diag_valid = false;
+    check_status(H);
+  });
+}
+
+int qg_compute_q_from_psi_member(qg_handle* h, int member, int method) {
+  return guarded(h, [&] {
+    qg_handle& H = *h;
+    check_member_index(H, member);
+    if (method != 0 && method != 1) throw QgError(QG_ERR_ARGUMENT, "pv method must be 0 (qg) or 1 (barotropic)");
+    const MemberView mv(H, member);
+    qg::launch_q_from_psi(mv.g, H.st, H.psi + mv.off, H.q[H.cur] + mv.off, H.d_a + (size_t)member * H.nzc, method);
+    H.diag_valid = false;
+    HIPCHECK(hipStreamSynchronize(H.st));
   });
 }
 

@@ -36,7 +36,10 @@ __global__ void k_thomas_factor(Geo g, Quarter qr, const double* __restrict__ a,
   const int rc = blockIdx.x * blockDim.x + threadIdx.x;
   if (rc >= rplane) return;
   const int x = rc % qr.Wr, y = rc / qr.Wr;
-  const int nz = g.nz;
+  const int nz = g.nzc, m = blockIdx.y;  // member m: its own a(z) and tables
+  a += (size_t)m * nz;
+  inv += (size_t)m * nz * rplane;
+  cp += (size_t)m * nz * rplane;
   const double kh2 = g.kx[x] * g.kx[x] + g.ky[y] * g.ky[y];
   const double dz2 = g.dz * g.dz;
   auto at = [&](int k) { return (size_t)k * rplane + rc; };
@@ -69,8 +72,9 @@ __global__ void k_thomas_factor(Geo g, Quarter qr, const double* __restrict__ a,
   }
 }
 
-// KIND 0: q -> psi; 1: B -> A; 2: B -> A and C = dA/dz.
-// al[k] = a[k-1] (al[0] = 0), zero-padded to mirror_pad(nz) entries.
+// KIND 0: q -> psi; 1: B -> A; 2: B -> A and C = dA/dz.  blockIdx.y is the
+// member.  al[k] = a[k-1] (al[0] = 0), zero-padded to mirror_pad(nzc)
+// entries per member.
 template <int KIND, int QC>
 __global__ void __launch_bounds__(kMirrorThreads)
 k_thomas(Geo g, Quarter qr, const cplx<double>* __restrict__ rhs, cplx<double>* __restrict__ out,
@@ -78,8 +82,18 @@ k_thomas(Geo g, Quarter qr, const cplx<double>* __restrict__ rhs, cplx<double>* 
          const double* __restrict__ al, unsigned long long* __restrict__ warn) {
   constexpr int NC = 4 * QC;
   extern __shared__ cplx<double> xs[];  // [kpad][NC], then ti[kpad][QC], tc[kpad][QC]
-  const int nz = g.nz, kpad = mirror_pad(nz);
+  const int nz = g.nzc, kpad = mirror_pad(nz);
   const int rc0 = blockIdx.x * QC;
+  const size_t plane = (size_t)g.nx * g.ny;
+  {
+    const size_t m = blockIdx.y;
+    rhs += m * nz * plane;
+    out += m * nz * plane;
+    if (KIND == 2) Cout += m * nz * plane;
+    inv += m * nz * qr.size();
+    cp += m * nz * qr.size();
+    al += m * kpad;
+  }
   double* ti = reinterpret_cast<double*>(xs + (size_t)kpad * NC);
   double* tc = ti + (size_t)kpad * QC;
   const MirrorSlot<QC> sl(g, qr, rc0);
@@ -136,7 +150,6 @@ k_thomas(Geo g, Quarter qr, const cplx<double>* __restrict__ rhs, cplx<double>* 
 
   // write back (psi = 0 on the mean column), and C = dA/dz
   if (!sl.q.active) return;
-  const size_t plane = (size_t)g.nx * g.ny;
   const bool zero_col = KIND == 0 && nz > 1 && sl.q.x == 0 && sl.q.y == 0;
   for (int k = sl.kf; k < nz; k += MirrorSlot<QC>::kstep) {
     const size_t at = (size_t)k * plane + sl.off;
@@ -154,8 +167,8 @@ void launch_k(const Geo& g, hipStream_t st, const cplx<double>* rhs, cplx<double
               const double* inv, const double* cp, const double* al, unsigned long long* warn) {
   const Quarter qr = quarter(g);
   const int blocks = (qr.size() + QC - 1) / QC;
-  hipLaunchKernelGGL((k_thomas<KIND, QC>), dim3(blocks), dim3(kMirrorThreads), mirror_lds_bytes(mirror_pad(g.nz), QC),
-                     st, g, qr, rhs, out, Cout, inv, cp, al, warn);
+  hipLaunchKernelGGL((k_thomas<KIND, QC>), dim3(blocks, g.members), dim3(kMirrorThreads),
+                     mirror_lds_bytes(mirror_pad(g.nzc), QC), st, g, qr, rhs, out, Cout, inv, cp, al, warn);
 }
 
 template <int QC>
@@ -180,7 +193,7 @@ bool set_lds_attr() {
 
 size_t thomas_table_size(const Geo& g) { return (size_t)quarter(g).size() * g.nz; }
 
-int thomas_shifted_a_size(int nz) { return mirror_pad(nz); }
+int thomas_shifted_a_size(int nzc) { return mirror_pad(nzc); }
 
 bool column_depth_supported(int nz) { return nz >= 1 && mirror_qc(nz) > 0; }
 
@@ -188,15 +201,15 @@ void launch_thomas_factor(const Geo& g, hipStream_t st, const double* a, int kdi
                           int* err) {
   const Quarter qr = quarter(g);
   const int bs = 64;
-  hipLaunchKernelGGL(k_thomas_factor, dim3((qr.size() + bs - 1) / bs), dim3(bs), 0, st, g, qr, a, kdiv, inv, cp,
-                     err);
+  hipLaunchKernelGGL(k_thomas_factor, dim3((qr.size() + bs - 1) / bs, g.members), dim3(bs), 0, st, g, qr, a, kdiv,
+                     inv, cp, err);
 }
 
 bool thomas_setup() { return set_lds_attr<8>() && set_lds_attr<4>() && set_lds_attr<2>() && set_lds_attr<1>(); }
 
 void launch_thomas(const Geo& g, hipStream_t st, const cplx<double>* rhs, cplx<double>* out, cplx<double>* Cout,
                    const double* inv, const double* cp, const double* al, int kdiv, unsigned long long* warn) {
-  switch (mirror_qc(g.nz)) {
+  switch (mirror_qc(g.nzc)) {
     case 8: launch_qc<8>(g, st, rhs, out, Cout, inv, cp, al, kdiv, warn); return;
     case 4: launch_qc<4>(g, st, rhs, out, Cout, inv, cp, al, kdiv, warn); return;
     case 2: launch_qc<2>(g, st, rhs, out, Cout, inv, cp, al, kdiv, warn); return;
