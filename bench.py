@@ -59,6 +59,10 @@ def measured_traffic(name):
     return None if entry is None else entry["hbm_bytes_per_launch"]
 
 
+def log(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def build_model(qg, precision):
     from qgybjplus_amd import synthetic  # noqa: F401  (registered by load_package)
     grid = qg.RectilinearGrid(size=SIZE, extent=EXTENT, centered=True)
@@ -82,6 +86,60 @@ def timed_run(qg, model, steps, warmup):
     t1 = time.perf_counter()
     prof = h.profile_end()
     return t1 - t0, prof
+
+
+ENSEMBLE_CLASSES = ((64, 32), (128, 64), (256, 64))  # (nx = ny, nz), 16 members each
+ENSEMBLE_MEMBERS = 16
+
+
+def ensemble_leg(qg, steps, warmup=2, seed=2026):
+    """The batched-ensemble workload: 48 members of uneven grids, 16 each of
+    64²x32, 128²x64 and 256²x64 over 500 km x 500 km x 4000 m, N² = N0²
+    exp(z/800) with N0² ~ U[5e-6, 5e-5], psi = the seeded k^-3 field plus 5 %
+    of a member-seeded one, surface wave 0.1 m/s over 50 m, feedback on,
+    mixed precision, dt = 20 s.  Each grid class is one device ensemble whose
+    passes cover its 16 members in one launch; the classes step one after
+    the other.  Returns throughput over all 48 members."""
+    from qgybjplus_amd import synthetic
+    rng = np.random.default_rng(seed)
+    n0 = rng.uniform(5e-6, 5e-5, size=len(ENSEMBLE_CLASSES) * ENSEMBLE_MEMBERS)
+    ens_list, points = [], 0
+    for ci, (n, nz) in enumerate(ENSEMBLE_CLASSES):
+        grid = qg.RectilinearGrid(size=(n, n, nz), extent=EXTENT, centered=True)
+        strats = [qg.AnalyticalProfile(synthetic.exponential_N2(float(n0[ci * ENSEMBLE_MEMBERS + j]), 800.0))
+                  for j in range(ENSEMBLE_MEMBERS)]
+        ens = qg.QGYBJEnsemble(stratifications=strats, grid=grid, coriolis=qg.FPlane(1e-4),
+                               feedback=qg.WaveMeanFeedback(), formulation=qg.YBJPlus(),
+                               architecture=qg.GPU("mixed"))
+        base = synthetic.k3_streamfunction(grid, seed=seed, urms=0.1, kpeak=6.0)
+        for j in range(ENSEMBLE_MEMBERS):
+            noise = synthetic.k3_streamfunction(grid, seed=seed + 1 + ci * ENSEMBLE_MEMBERS + j, urms=0.1,
+                                                kpeak=6.0)
+            ens.set_member(j, psi=qg.FieldArray(base + 0.05 * noise), waves=qg.SurfaceWave(0.1, 50.0))
+        ens_list.append(ens)
+        points += n * n * nz * ENSEMBLE_MEMBERS
+    ts = qg.ExponentialRungeKutta2(dt=DT)
+    for ens in ens_list:
+        qg.step_(ens, ts, warmup)
+    per_class = []
+    t0 = time.perf_counter()
+    for ens in ens_list:
+        c0 = time.perf_counter()
+        qg.step_(ens, ts, steps)
+        ens._h.synchronize()
+        el = time.perf_counter() - c0
+        nx, ny, nz = ens.grid.size
+        per_class.append({"grid": [nx, ny, nz], "members": ens.members, "seconds": el,
+                          "Mgridpt_steps_per_s": nx * ny * nz * ens.members * steps / el / 1e6})
+    total = time.perf_counter() - t0
+    bad = [j for ens in ens_list for j in range(ens.members) if ens.check_member(j)[0]]
+    for ens in ens_list:
+        ens.finalize()
+    return {"workload": "48-member ensemble: 16 x 64^2x32, 16 x 128^2x64, 16 x 256^2x64, mixed precision, "
+                        "N0^2 ~ U[5e-6, 5e-5], perturbed k^-3 psi, dt = 20 s",
+            "members": len(ENSEMBLE_CLASSES) * ENSEMBLE_MEMBERS, "gridpoints": points, "steps": steps,
+            "warmup": warmup, "value": points * steps / total / 1e6, "unit": "Mgridpt-steps/s",
+            "seconds": total, "per_class": per_class, "nonfinite_members": bad}
 
 
 def cpu_baseline(psi_phys, B_phys, budget_s=25.0):
@@ -115,6 +173,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=25.0)
+    ap.add_argument("--ensemble-steps", type=int, default=200,
+                    help="steps of the 48-member batched-ensemble leg (0 skips it)")
     ap.add_argument("--modes", choices=("both", "mixed"), default="both",
                     help="'mixed' times only the mixed-precision model (for profiler runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -159,11 +219,13 @@ def main():
         qg.set_(m, waves=qg.SurfaceWave(0.1, 50.0))
 
     npts = SIZE[0] * SIZE[1] * SIZE[2]
+    log("timing the mixed-precision model")
     t32, p32 = timed_run(qg, m32, args.steps, args.warmup)
     v32 = npts * args.steps / t32 / 1e6
     errs = {}
     t64 = v64 = None
     if m64 is not None:
+        log("timing the FP64 model")
         t64, _ = timed_run(qg, m64, args.steps, args.warmup)
         v64 = npts * args.steps / t64 / 1e6
         for n in ("q", "B", "psi", "A"):
@@ -210,13 +272,17 @@ def main():
                      "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_s * 1e6},
         "kernels_mixed": per_kernel,
     }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(psi, B, args.cpu_budget)
-    else:
-        out["cpu_baseline"] = None
     m32.finalize()
     if m64 is not None:
         m64.finalize()
+    if args.ensemble_steps > 0:
+        log("batched-ensemble leg")
+    out["ensemble"] = ensemble_leg(qg, args.ensemble_steps) if args.ensemble_steps > 0 else None
+    if not args.no_cpu:
+        log("CPU baseline (numpy oracle, one core)")
+        out["cpu_baseline"] = cpu_baseline(psi, B, args.cpu_budget)
+    else:
+        out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
 
 
