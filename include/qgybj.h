@@ -155,6 +155,11 @@ int qg_compute_qw(qg_handle* h, double* host_out);
  * host array (src/transforms.jl:19-23, src/parallel_mpi.jl:735-797). */
 int qg_fft(qg_handle* h, const double* host_in, double* host_out, int direction);
 
+/* Steady-state steps of qg_step (diagnostics left valid by the previous step,
+ * no per-kernel profiling) replay a hipGraph of two captured steps; enable = 0
+ * runs every step eagerly.  On by default.  No reference counterpart. */
+int qg_use_graphs(qg_handle* h, int enable);
+
 /* Waits for the handle's stream. */
 int qg_synchronize(qg_handle* h);
 

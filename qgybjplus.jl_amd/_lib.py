@@ -36,7 +36,7 @@ EXPORTED = (
     "qg_compute_velocities", "qg_compute_q_from_psi", "qg_compute_qw", "qg_fft",
     "qg_synchronize", "qg_take_mean_q_warning", "qg_profile_begin", "qg_profile_end",
     "qg_set_field_member", "qg_get_field_member", "qg_check_member",
-    "qg_compute_q_from_psi_member", "qg_invert_B_to_A_member",
+    "qg_compute_q_from_psi_member", "qg_invert_B_to_A_member", "qg_use_graphs",
 )
 
 
@@ -94,6 +94,7 @@ def load() -> C.CDLL:
         "qg_check_member": (C.c_int, [P, C.c_int, C.POINTER(C.c_int), dp]),
         "qg_compute_q_from_psi_member": (C.c_int, [P, C.c_int, C.c_int]),
         "qg_invert_B_to_A_member": (C.c_int, [P, C.c_int]),
+        "qg_use_graphs": (C.c_int, [P, C.c_int]),
         "qg_invert_q_to_psi": (C.c_int, [P]),
         "qg_invert_B_to_A": (C.c_int, [P]),
         "qg_compute_velocities": (C.c_int, [P, C.c_int]),
@@ -220,6 +221,9 @@ class Handle:
         self._check(self._lib.qg_invert_B_to_A_member(self.h, int(member)))
 
     # -- operators --------------------------------------------------------------
+    def use_graphs(self, enable: bool = True):
+        self._check(self._lib.qg_use_graphs(self.h, int(bool(enable))))
+
     def step(self, dt: float, nsteps: int = 1):
         self._check(self._lib.qg_step(self.h, float(dt), int(nsteps)))
 

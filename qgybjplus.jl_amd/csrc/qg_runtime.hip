@@ -75,6 +75,10 @@ struct qg_handle {
   unsigned long long* d_absmax = nullptr;
   int* d_flag = nullptr;
   bool diag_valid = false, t1_valid = false;
+  // hipGraphs of two steady-state steps starting from each parity of the
+  // double buffer (after two swaps the buffers are back in place)
+  bool graphs = true;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
   std::string err;
   // profiling: bit 1 = events around every kernel, bit 2 = one event per step
   struct ProfRec {
@@ -85,6 +89,7 @@ struct qg_handle {
   int prof_mode = 0;
   std::vector<ProfRec> prof;
   std::vector<hipEvent_t> step_events;
+  std::vector<int> step_weights;  // steps covered by each interval of step_events
   std::vector<hipEvent_t> pool;
   size_t pool_used = 0;
   hipEvent_t event() {
@@ -110,8 +115,16 @@ struct qg_handle {
     allocs.push_back(p);
     return p;
   }
+  void drop_graphs() {
+    for (hipGraphExec_t& g : graph)
+      if (g) {
+        (void)hipGraphExecDestroy(g);
+        g = nullptr;
+      }
+  }
   ~qg_handle() {
     if (st) (void)hipStreamSynchronize(st);
+    drop_graphs();
     for (hipEvent_t e : pool) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
     if (st) (void)hipStreamDestroy(st);
@@ -846,24 +859,71 @@ int qg_step(qg_handle* h, double dt, int nsteps) {
   return guarded(h, [&] {
     if (!(std::isfinite(dt) && dt > 0)) throw QgError(QG_ERR_ARGUMENT, "Δt must be finite and positive");
     if (nsteps < 0) throw QgError(QG_ERR_ARGUMENT, "nsteps must be non-negative");
-    build_etd(*h, dt);
-    with_engine(*h, [&](auto& e) {
-      for (int i = 0; i < nsteps; ++i) {
-        if ((h->prof_mode & 2) && h->step_events.empty()) {
-          hipEvent_t ev = h->event();
-          HIPCHECK(hipEventRecord(ev, h->st));
-          h->step_events.push_back(ev);
-        }
-        e.step();
-        if (h->prof_mode & 2) {
-          hipEvent_t ev = h->event();
-          HIPCHECK(hipEventRecord(ev, h->st));
-          h->step_events.push_back(ev);
+    qg_handle& H = *h;
+    build_etd(H, dt);  // rewrites the ETD tables in place: captured graphs stay valid
+    auto mark = [&](int weight) {
+      if (!(H.prof_mode & 2)) return;
+      if (H.step_events.empty()) {
+        hipEvent_t ev = H.event();
+        HIPCHECK(hipEventRecord(ev, H.st));
+        H.step_events.push_back(ev);
+        if (weight == 0) return;
+      }
+      hipEvent_t ev = H.event();
+      HIPCHECK(hipEventRecord(ev, H.st));
+      H.step_events.push_back(ev);
+      H.step_weights.push_back(weight);
+    };
+    with_engine(H, [&](auto& e) {
+      // Steady-state steps (diagnostics valid, no per-kernel events) replay a
+      // captured two-step graph; the others run eagerly.
+      const bool use_graphs = H.graphs && !(H.prof_mode & 1);
+      if (H.prof_mode & 2) mark(0);
+      int i = 0;
+      while (i < nsteps) {
+        if (use_graphs && H.diag_valid && nsteps - i >= 2) {
+          const int c = H.cur;
+          if (!H.graph[c]) {
+            hipGraph_t g = nullptr;
+            HIPCHECK(hipStreamBeginCapture(H.st, hipStreamCaptureModeThreadLocal));
+            try {
+              e.step();
+              e.step();
+            } catch (...) {
+              (void)hipStreamEndCapture(H.st, &g);
+              if (g) (void)hipGraphDestroy(g);
+              throw;
+            }
+            HIPCHECK(hipStreamEndCapture(H.st, &g));
+            const hipError_t ie = hipGraphInstantiate(&H.graph[c], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) {
+              H.graph[c] = nullptr;
+              throw QgError(QG_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+            }
+          }
+          HIPCHECK(hipGraphLaunch(H.graph[c], H.st));
+          i += 2;
+          mark(2);
+        } else {
+          e.step();
+          ++i;
+          mark(1);
         }
       }
     });
     HIPCHECK(hipGetLastError());
-    check_status(*h);
+    check_status(H);
+  });
+}
+
+int qg_use_graphs(qg_handle* h, int enable) {
+  return guarded(h, [&] {
+    h->graphs = enable != 0;
+    if (!h->graphs) {
+      HIPCHECK(hipStreamSynchronize(h->st));
+      h->drop_graphs();
+    }
   });
 }
 
@@ -971,6 +1031,7 @@ int qg_profile_begin(qg_handle* h, int mode) {
     HIPCHECK(hipStreamSynchronize(h->st));
     h->prof.clear();
     h->step_events.clear();
+    h->step_weights.clear();
     h->pool_used = 0;
     h->prof_mode = mode & 3;
   });
@@ -1006,17 +1067,26 @@ int qg_profile_end(qg_handle* h, char* buf, size_t len, size_t* needed) {
                agg[i].name.c_str(), agg[i].count, agg[i].ms, agg[i].bytes);
       js += tmp;
     }
+    // an interval covering w steps (a replayed two-step graph) counts as w
+    // steps of equal length; intervals of weight 0 are gaps between calls
     js += "}, \"steps_ms\": [";
-    for (size_t i = 1; i < h->step_events.size(); ++i) {
+    bool first = true;
+    for (size_t i = 1; i < h->step_events.size() && i - 1 < h->step_weights.size(); ++i) {
+      const int w = h->step_weights[i - 1];
+      if (w <= 0) continue;
       float ms = 0;
       HIPCHECK(hipEventElapsedTime(&ms, h->step_events[i - 1], h->step_events[i]));
-      snprintf(tmp, sizeof tmp, "%s%.6f", i > 1 ? ", " : "", ms);
-      js += tmp;
+      for (int k = 0; k < w; ++k) {
+        snprintf(tmp, sizeof tmp, "%s%.6f", first ? "" : ", ", ms / w);
+        js += tmp;
+        first = false;
+      }
     }
     js += "]}";
     h->prof_mode = 0;
     h->prof.clear();
     h->step_events.clear();
+    h->step_weights.clear();
     h->pool_used = 0;
     if (needed) *needed = js.size() + 1;
     if (buf && len > 0) {
