@@ -155,6 +155,26 @@ def ensemble_leg(qg, steps, warmup=2, seed=2026):
             "seconds": total, "per_class": per_class, "nonfinite_members": bad}
 
 
+def serving_leg(qg, n_requests, capacity_mpts, seed=7, utilization=0.8):
+    """The serving workload: Poisson arrivals of QG-YBJ+ runs with grids
+    uniform over {64,128,256,384}² x {32,64,128}, steps log-uniform over
+    [100, 2000], seeded k^-3 psi, wave amplitude U[0.05, 0.2], constant or
+    exponential N², mixed precision, dt = 20 s.  The arrival rate is set for
+    `utilization` of the measured single-model throughput.  Runs share
+    per-grid-class device ensembles (continuous batching); reports p50/p99
+    completion latency (arrival to result on the host) and throughput."""
+    from qgybjplus_amd import serving
+    rate = utilization * capacity_mpts * 1e6 / serving.expected_work()
+    trace = serving.poisson_trace(n_requests, rate=rate, seed=seed)
+    server = serving.Server(precision="mixed", slots=4, chunk=50)
+    _, summary = server.run(trace, realtime=True)
+    summary.update({"workload": f"{n_requests} Poisson-arrival runs, grids {{64,128,256,384}}^2 x {{32,64,128}}, "
+                                "steps log-uniform [100, 2000], mixed precision, dt = 20 s",
+                    "arrival_rate_per_s": rate, "target_utilization": utilization,
+                    "latency": "arrival to result on the host"})
+    return summary
+
+
 def cpu_baseline(psi_phys, B_phys, budget_s=25.0):
     """The numpy oracle (an op-for-op restatement of the reference's CPU step)
     on the same workload, one host core, over a bounded number of steps."""
@@ -188,6 +208,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=25.0)
     ap.add_argument("--ensemble-steps", type=int, default=200,
                     help="steps of the 48-member batched-ensemble leg (0 skips it)")
+    ap.add_argument("--serve-requests", type=int, default=32,
+                    help="requests of the serving leg (0 skips it)")
     ap.add_argument("--modes", choices=("both", "mixed"), default="both",
                     help="'mixed' times only the mixed-precision model (for profiler runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -297,6 +319,9 @@ def main():
     if args.ensemble_steps > 0:
         log("batched-ensemble leg")
     out["ensemble"] = ensemble_leg(qg, args.ensemble_steps) if args.ensemble_steps > 0 else None
+    if args.serve_requests > 0:
+        log(f"serving leg ({args.serve_requests} requests)")
+    out["serving"] = serving_leg(qg, args.serve_requests, v32) if args.serve_requests > 0 else None
     if not args.no_cpu:
         log("CPU baseline (numpy oracle, one core)")
         out["cpu_baseline"] = cpu_baseline(psi, B, args.cpu_budget)

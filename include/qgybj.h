@@ -100,6 +100,12 @@ typedef struct qg_config {
  * src/core/model.jl:79-121): allocates the model on the current device. */
 int qg_create(const qg_config* cfg, qg_handle** out);
 
+/* Device bytes qg_create would allocate for cfg (no allocation; used to admit
+ * runs by free HBM), and the device's free and total memory.  No reference
+ * counterpart. */
+int qg_memory_required(const qg_config* cfg, size_t* bytes);
+int qg_device_memory(size_t* free_bytes, size_t* total_bytes);
+
 /* Replaces finalize_model! (src/core/model.jl:124). Safe on NULL. */
 int qg_destroy(qg_handle* h);
 
@@ -147,6 +153,12 @@ int qg_compute_q_from_psi(qg_handle* h, int method);
  * other members). */
 int qg_compute_q_from_psi_member(qg_handle* h, int member, int method);
 int qg_invert_B_to_A_member(qg_handle* h, int member);
+/* Replaces one member's N2 and a_ell profiles (nz entries each, computed as
+ * for qg_config) and re-records its column eliminations, so a free ensemble
+ * slot can take a run with another stratification.  Fails with
+ * QG_ERR_SINGULAR / QG_ERR_GTSV, leaving the other members untouched, when
+ * the new profile gives a singular column.  No reference counterpart. */
+int qg_set_member_stratification(qg_handle* h, int member, const double* N2, const double* a_ell);
 /* compute_qw_complex!(qw, B, ...) (src/nonlinear.jl:859-944) on the current
  * B; result copied to host (complex, (z,x,y)). */
 int qg_compute_qw(qg_handle* h, double* host_out);
@@ -154,6 +166,15 @@ int qg_compute_qw(qg_handle* h, double* host_out);
 /* fft_forward! (direction -1) / fft_backward! (+1) of one complex (z,x,y)
  * host array (src/transforms.jl:19-23, src/parallel_mpi.jl:735-797). */
 int qg_fft(qg_handle* h, const double* host_in, double* host_out, int direction);
+
+/* Ensemble extent: qg_step advances members [0, active) only (all members
+ * after qg_create; 0 makes qg_step a no-op), and qg_move_member copies member
+ * src's state (q, B, psi, A, C, u, v, w) and coefficients over member dst, so a
+ * serving loop keeps its resident runs in the leading slots.  The moved run's
+ * next step recomputes its diagnostics, exactly as the previous step left
+ * them.  No reference counterpart. */
+int qg_set_active_members(qg_handle* h, int active);
+int qg_move_member(qg_handle* h, int src, int dst);
 
 /* Steady-state steps of qg_step (diagnostics left valid by the previous step,
  * no per-kernel profiling) replay a hipGraph of two captured steps; enable = 0

@@ -15,3 +15,4 @@ from .api import (  # noqa: F401
     ExponentialRungeKutta2, step_, Simulation, SimulationState, Clock, run_,
     check_termination_, finalize_simulation_,
 )
+from . import synthetic, serving  # noqa: F401,E402

@@ -37,6 +37,8 @@ EXPORTED = (
     "qg_synchronize", "qg_take_mean_q_warning", "qg_profile_begin", "qg_profile_end",
     "qg_set_field_member", "qg_get_field_member", "qg_check_member",
     "qg_compute_q_from_psi_member", "qg_invert_B_to_A_member", "qg_use_graphs",
+    "qg_set_member_stratification", "qg_memory_required", "qg_device_memory",
+    "qg_set_active_members", "qg_move_member",
 )
 
 
@@ -95,6 +97,11 @@ def load() -> C.CDLL:
         "qg_compute_q_from_psi_member": (C.c_int, [P, C.c_int, C.c_int]),
         "qg_invert_B_to_A_member": (C.c_int, [P, C.c_int]),
         "qg_use_graphs": (C.c_int, [P, C.c_int]),
+        "qg_set_active_members": (C.c_int, [P, C.c_int]),
+        "qg_move_member": (C.c_int, [P, C.c_int, C.c_int]),
+        "qg_set_member_stratification": (C.c_int, [P, C.c_int, dp, dp]),
+        "qg_memory_required": (C.c_int, [C.POINTER(QgConfig), C.POINTER(C.c_size_t)]),
+        "qg_device_memory": (C.c_int, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
         "qg_invert_q_to_psi": (C.c_int, [P]),
         "qg_invert_B_to_A": (C.c_int, [P]),
         "qg_compute_velocities": (C.c_int, [P, C.c_int]),
@@ -120,6 +127,26 @@ def _dptr(a: np.ndarray):
 
 def _iptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def device_memory():
+    """(free, total) bytes of the current device."""
+    lib = load()
+    f, t = C.c_size_t(0), C.c_size_t(0)
+    if lib.qg_device_memory(C.byref(f), C.byref(t)) != QG_OK:
+        raise QgError(QG_ERR_DEVICE, "hipMemGetInfo failed")
+    return int(f.value), int(t.value)
+
+
+def memory_required(*, nx, ny, nz, precision, members=1) -> int:
+    """Device bytes a model (or ensemble) of this shape allocates."""
+    lib = load()
+    cfg = QgConfig()
+    cfg.nx, cfg.ny, cfg.nz, cfg.precision, cfg.members = int(nx), int(ny), int(nz), int(precision), int(members)
+    b = C.c_size_t(0)
+    if lib.qg_memory_required(C.byref(cfg), C.byref(b)) != QG_OK:
+        raise ValueError("invalid shape")
+    return int(b.value)
 
 
 class Handle:
@@ -220,7 +247,20 @@ class Handle:
     def invert_B_to_A_member(self, member: int):
         self._check(self._lib.qg_invert_B_to_A_member(self.h, int(member)))
 
+    def set_member_stratification(self, member: int, N2, a_ell):
+        n2 = np.ascontiguousarray(np.asarray(N2, dtype=np.float64))
+        a = np.ascontiguousarray(np.asarray(a_ell, dtype=np.float64))
+        if n2.shape != (self.member_shape[0],) or a.shape != n2.shape:
+            raise ValueError("N2 and a_ell need one entry per level")
+        self._check(self._lib.qg_set_member_stratification(self.h, int(member), _dptr(n2), _dptr(a)))
+
     # -- operators --------------------------------------------------------------
+    def set_active_members(self, active: int):
+        self._check(self._lib.qg_set_active_members(self.h, int(active)))
+
+    def move_member(self, src: int, dst: int):
+        self._check(self._lib.qg_move_member(self.h, int(src), int(dst)))
+
     def use_graphs(self, enable: bool = True):
         self._check(self._lib.qg_use_graphs(self.h, int(bool(enable))))
 

@@ -537,6 +537,33 @@ class QGYBJEnsemble(QGYBJModel):
     def check_member(self, member: int):
         return self._h.check_member(member)
 
+    def set_active_members(self, active: int):
+        """Step only members [0, active) from now on."""
+        self._h.set_active_members(active)
+        return self
+
+    def move_member(self, src: int, dst: int):
+        """Copy member src (state, stratification and column eliminations)
+        over member dst."""
+        self._h.move_member(src, dst)
+        self.stratifications[dst] = self.stratifications[src]
+        nz = self.grid.size[2]
+        self.N2[dst * nz:(dst + 1) * nz] = self.N2[src * nz:(src + 1) * nz]
+        self.a_ell[dst * nz:(dst + 1) * nz] = self.a_ell[src * nz:(src + 1) * nz]
+        return self
+
+    def set_member_stratification(self, member: int, stratification):
+        """Give slot `member` another N²(z) (its column eliminations are
+        recorded again); the other members are not touched."""
+        strat = self._stratification(stratification)
+        N2, a = self._profiles(strat)
+        self._h.set_member_stratification(member, N2, a)
+        nz = self.grid.size[2]
+        self.N2[member * nz:(member + 1) * nz] = N2
+        self.a_ell[member * nz:(member + 1) * nz] = a
+        self.stratifications[member] = strat
+        return self
+
     def __repr__(self):
         return f"QGYBJEnsemble(grid={self.grid.size}, members={self.members}, architecture={self.architecture})"
 

@@ -36,6 +36,8 @@ struct QgError : std::runtime_error {
 struct qg_handle {
   // nz counts all stacked levels: members * nzc (see qg::Geo)
   int nx = 0, ny = 0, nz = 0, nzc = 0, members = 1, precision = 0;
+  // members [0, active) are stepped; g describes them (g.members == active)
+  int active = 1;
   double f = 0, dz = 1, nu_z = 0;
   bool fixed_flow = false, feedback = false, passive = false, inviscid = false, linear = false,
        no_disp = false;
@@ -71,6 +73,7 @@ struct qg_handle {
   void *twx = nullptr, *twy = nullptr;  // intermediate precision
   D *tw64x = nullptr, *tw64y = nullptr;
   int* d_err = nullptr;
+  int* d_ferr = nullptr;  // error flags of member re-factorisations
   unsigned long long* d_warn = nullptr;
   unsigned long long* d_absmax = nullptr;
   int* d_flag = nullptr;
@@ -260,8 +263,8 @@ struct Engine {
   // columns |ix| <= nx/2, |iy| <= ny/2
   const double P, Ph, Tq;
   explicit Engine(qg_handle& hh)
-      : h(hh), P((double)hh.npts), Ph((double)hh.g.W * hh.ny * hh.nz),
-        Tq(2.0 * 8.0 * (hh.nx / 2 + 1) * (hh.ny / 2 + 1) * hh.nz) {}
+      : h(hh), P((double)hh.plane * hh.g.nz), Ph((double)hh.g.W * hh.ny * hh.g.nz),
+        Tq(2.0 * 8.0 * (hh.nx / 2 + 1) * (hh.ny / 2 + 1) * hh.g.nz) {}
   C* c(void* p) const { return static_cast<C*>(p); }
   R* r(void* p) const { return static_cast<R*>(p); }
   const C* twx() const { return static_cast<const C*>(h.twx); }
@@ -308,8 +311,8 @@ struct Engine {
 
   void wave_inv(const D* Bsrc, D* Aout, D* Cout) {
     if (h.passive || h.no_disp) {
-      qg::launch_zero(Aout, h.npts, h.st);
-      if (Cout) qg::launch_zero(Cout, h.npts, h.st);
+      qg::launch_zero(Aout, h.plane * h.g.nz, h.st);
+      if (Cout) qg::launch_zero(Cout, h.plane * h.g.nz, h.st);
       return;
     }
     K(Cout ? "thomas_AC" : "thomas_A", P * (Cout ? 3 : 2) * cD + Tq,
@@ -429,7 +432,7 @@ struct Engine {
       velocities_w(h.psi);
     } else {
       velocities(h.psi);
-      qg::launch_zero_bytes(h.w, h.npts * h.rsize(), h.st);
+      qg::launch_zero_bytes(h.w, h.plane * h.g.nz * h.rsize(), h.st);
     }
   }
 };
@@ -553,6 +556,7 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
     h->ny = cfg->ny;
     h->nzc = cfg->nz;
     h->members = members;
+    h->active = members;
     h->nz = members * cfg->nz;
     h->precision = cfg->precision;
     h->f = cfg->f;
@@ -647,6 +651,7 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
     for (void** p : {&h->T1, &h->T3, &h->G3, &h->G4, &h->G5}) *p = h->ralloc(2 * h->npts);
     for (void** p : {&h->Qa, &h->Qb, &h->Pu, &h->Pv, &h->Pq, &h->Pz, &h->G1, &h->G2}) *p = h->ralloc(2 * h->nhalf);
     h->d_err = h->alloc<int>(4);
+    h->d_ferr = h->alloc<int>(8);
     h->d_warn = h->alloc<unsigned long long>(1);
     h->d_absmax = h->alloc<unsigned long long>(1);
     h->d_flag = h->alloc<int>(1);
@@ -726,6 +731,34 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
     delete h;
     return QG_ERR_DEVICE;
   }
+}
+
+int qg_memory_required(const qg_config* cfg, size_t* bytes) {
+  if (!cfg || !bytes || cfg->nx <= 0 || cfg->ny <= 0 || cfg->nz <= 0) return QG_ERR_ARGUMENT;
+  const size_t members = cfg->members > 0 ? (size_t)cfg->members : 1;
+  const size_t nx = cfg->nx, ny = cfg->ny, nz = members * cfg->nz, plane = nx * ny, npts = plane * nz;
+  const size_t nhalf = ((nx / 2 + 1 + 15) / 16 * 16) * ny * nz;
+  const size_t r = cfg->precision == QG_PRECISION_MIXED ? 4 : 8;
+  const size_t quarter = (nx / 2 + 1) * (ny / 2 + 1);
+  const size_t kpad = (size_t)qg::thomas_shifted_a_size(cfg->nz);
+  size_t b = 15 * 16 * npts       // complex FP64 fields and stage/work buffers
+             + 3 * 8 * npts       // omega scratch
+             + 3 * r * npts       // u, v, w
+             + 5 * 2 * r * npts   // full-width transform intermediates
+             + 8 * 2 * r * nhalf  // half-width transform intermediates
+             + 6 * 8 * quarter * nz + quarter * members  // Thomas and omega tables
+             + 6 * 8 * plane + plane + 4 * 16 * (nx + ny) + (3 * 8 * nz) + 2 * 8 * members * kpad;
+  b += 64 * 256;  // allocation padding
+  *bytes = b;
+  return QG_OK;
+}
+
+int qg_device_memory(size_t* free_bytes, size_t* total_bytes) {
+  size_t f = 0, t = 0;
+  if (hipMemGetInfo(&f, &t) != hipSuccess) return QG_ERR_DEVICE;
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return QG_OK;
 }
 
 int qg_destroy(qg_handle* h) {
@@ -860,6 +893,7 @@ int qg_step(qg_handle* h, double dt, int nsteps) {
     if (!(std::isfinite(dt) && dt > 0)) throw QgError(QG_ERR_ARGUMENT, "Δt must be finite and positive");
     if (nsteps < 0) throw QgError(QG_ERR_ARGUMENT, "nsteps must be non-negative");
     qg_handle& H = *h;
+    if (H.active == 0) return;
     build_etd(H, dt);  // rewrites the ETD tables in place: captured graphs stay valid
     auto mark = [&](int weight) {
       if (!(H.prof_mode & 2)) return;
@@ -914,6 +948,108 @@ int qg_step(qg_handle* h, double dt, int nsteps) {
     });
     HIPCHECK(hipGetLastError());
     check_status(H);
+  });
+}
+
+int qg_set_member_stratification(qg_handle* h, int member, const double* N2, const double* a_ell) {
+  return guarded(h, [&] {
+    qg_handle& H = *h;
+    check_member_index(H, member);
+    if (!N2 || !a_ell) throw QgError(QG_ERR_ARGUMENT, "N2 and a_ell are required");
+    const int nzc = H.nzc, kpad = qg::thomas_shifted_a_size(nzc);
+    for (int k = 0; k < nzc; ++k)
+      if (!(std::isfinite(N2[k]) && N2[k] > 0 && std::isfinite(a_ell[k])))
+        throw QgError(QG_ERR_ARGUMENT, "N² must be finite and positive at every level");
+    const size_t o = (size_t)member * nzc;
+    std::copy(N2, N2 + nzc, H.N2.begin() + o);
+    std::copy(a_ell, a_ell + nzc, H.a.begin() + o);
+    std::vector<double> cz(nzc), fN2(nzc), al(kpad, 0.0), cu(kpad, 0.0);
+    const double f2 = H.f * H.f;
+    for (int k = 0; k < nzc; ++k) {
+      cz[k] = (f2 / N2[k]) / (H.dz * H.dz);  // src/operators.jl:428
+      fN2[k] = H.f / N2[k];                  // src/operators.jl:439
+    }
+    for (int k = 1; k < nzc; ++k) al[k] = a_ell[k - 1];
+    for (int k = 1; k <= nzc - 3; ++k) cu[k] = cz[k];
+    HIPCHECK(hipMemcpyAsync(H.d_a + o, a_ell, nzc * sizeof(double), hipMemcpyHostToDevice, H.st));
+    HIPCHECK(hipMemcpyAsync(H.d_cz + o, cz.data(), nzc * sizeof(double), hipMemcpyHostToDevice, H.st));
+    HIPCHECK(hipMemcpyAsync(H.d_fN2 + o, fN2.data(), nzc * sizeof(double), hipMemcpyHostToDevice, H.st));
+    HIPCHECK(hipMemcpyAsync(H.d_al + (size_t)member * kpad, al.data(), kpad * sizeof(double),
+                            hipMemcpyHostToDevice, H.st));
+    HIPCHECK(hipMemcpyAsync(H.d_cu + (size_t)member * kpad, cu.data(), kpad * sizeof(double),
+                            hipMemcpyHostToDevice, H.st));
+    const MemberView mv(H, member);
+    const size_t quarter = qg::thomas_table_size(mv.g) / nzc;
+    HIPCHECK(hipMemsetAsync(H.d_ferr, 0, 8 * sizeof(int), H.st));
+    qg::launch_thomas_factor(mv.g, H.st, H.d_a + o, 1, H.th_inv[0] + mv.toff, H.th_cp[0] + mv.toff, H.d_ferr);
+    qg::launch_thomas_factor(mv.g, H.st, H.d_a + o, 4, H.th_inv[1] + mv.toff, H.th_cp[1] + mv.toff, H.d_ferr + 4);
+    int he[8];
+    HIPCHECK(hipMemcpyAsync(he, H.d_ferr, sizeof he, hipMemcpyDeviceToHost, H.st));
+    HIPCHECK(hipStreamSynchronize(H.st));
+    if (he[0] || he[4]) raise_singular(he[0] ? he : he + 4);
+    HIPCHECK(hipMemsetAsync(H.d_ferr, 0, 8 * sizeof(int), H.st));
+    qg::launch_omega_factor(mv.g, H.st, H.d_cz + o, H.om_M + mv.toff, H.om_D + mv.toff,
+                            H.om_piv + (size_t)member * quarter, H.d_ferr);
+    HIPCHECK(hipMemcpyAsync(he, H.d_ferr, sizeof he, hipMemcpyDeviceToHost, H.st));
+    HIPCHECK(hipStreamSynchronize(H.st));
+    if (he[0]) {
+      char buf[160];
+      snprintf(buf, sizeof buf, "LAPACK-style gtsv failed for vertical velocity at row %d of column %d",
+               he[1] + 1, he[2]);
+      throw QgError(QG_ERR_GTSV, buf);
+    }
+    H.diag_valid = false;
+    H.t1_valid = false;
+  });
+}
+
+int qg_set_active_members(qg_handle* h, int active) {
+  return guarded(h, [&] {
+    qg_handle& H = *h;
+    if (active < 0 || active > H.members)
+      throw QgError(QG_ERR_ARGUMENT, "active members must be in [0, " + std::to_string(H.members) + "]");
+    if (active == H.active) return;
+    HIPCHECK(hipStreamSynchronize(H.st));
+    H.drop_graphs();  // captured for the previous extent
+    if (active > H.active) {  // newly stepped members have no diagnostics yet
+      H.diag_valid = false;
+      H.t1_valid = false;
+    }
+    H.active = active;
+    H.g.members = active;
+    H.g.nz = active * H.nzc;
+  });
+}
+
+int qg_move_member(qg_handle* h, int src, int dst) {
+  return guarded(h, [&] {
+    qg_handle& H = *h;
+    check_member_index(H, src);
+    check_member_index(H, dst);
+    if (src == dst) return;
+    const size_t lv = (size_t)H.nzc * H.plane;
+    auto mv = [&](void* base, size_t elem, size_t count) {
+      char* b = static_cast<char*>(base);
+      HIPCHECK(hipMemcpyAsync(b + dst * count * elem, b + src * count * elem, count * elem,
+                              hipMemcpyDeviceToDevice, H.st));
+    };
+    // prognostic state and the diagnostics a caller may read back
+    for (D* f : {H.q[H.cur], H.B[H.cur], H.psi, H.A, H.C}) mv(f, sizeof(D), lv);
+    for (void* f : {H.u, H.v, H.w}) mv(f, H.rsize(), lv);
+    // the member's coefficients and recorded eliminations
+    const int kpad = qg::thomas_shifted_a_size(H.nzc);
+    const size_t tq = (size_t)(H.nx / 2 + 1) * (H.ny / 2 + 1) * H.nzc;  // table entries per member
+    for (double* f : {H.d_a, H.d_cz, H.d_fN2}) mv(f, sizeof(double), H.nzc);
+    for (double* f : {H.d_al, H.d_cu}) mv(f, sizeof(double), kpad);
+    for (double* f : {H.th_inv[0], H.th_cp[0], H.th_inv[1], H.th_cp[1], H.om_M, H.om_D}) mv(f, sizeof(double), tq);
+    mv(H.om_piv, 1, tq / H.nzc);
+    HIPCHECK(hipStreamSynchronize(H.st));
+    std::copy(H.N2.begin() + (size_t)src * H.nzc, H.N2.begin() + (size_t)(src + 1) * H.nzc,
+              H.N2.begin() + (size_t)dst * H.nzc);
+    std::copy(H.a.begin() + (size_t)src * H.nzc, H.a.begin() + (size_t)(src + 1) * H.nzc,
+              H.a.begin() + (size_t)dst * H.nzc);
+    H.diag_valid = false;
+    H.t1_valid = false;
   });
 }
 

@@ -36,17 +36,23 @@ def k3_streamfunction(grid, seed: int = 2026, urms: float = 0.1, kpeak: float = 
         return np.fft.ifft2(np.where(mask, s, 0)).real
 
     bt = one()
-    out = np.empty((nz, nx, ny))
-    bc = one() if baroclinic else None
-    for k in range(nz):
-        out[k] = bt if bc is None else bt + bc * math.cos(math.pi * (grid.z[k] + Lz) / Lz)
-    kx = grid.kx[None, :, None]
-    ky = grid.ky[None, None, :]
-    sp = np.fft.fft2(out, axes=(1, 2))
-    u = np.fft.ifft2(-1j * ky * sp, axes=(1, 2)).real
-    v = np.fft.ifft2(1j * kx * sp, axes=(1, 2)).real
-    rms = math.sqrt(float(np.mean(u * u + v * v)))
-    return out * (urms / rms)
+    bc = one() if baroclinic else np.zeros_like(bt)
+    c = np.array([math.cos(math.pi * (grid.z[k] + Lz) / Lz) if baroclinic else 0.0 for k in range(nz)])
+    out = bt[None, :, :] + c[:, None, None] * bc[None, :, :]
+    # RMS velocity of psi_bt + c(z) psi_bc from the two 2-D velocity fields:
+    # mean over z of |u_bt + c u_bc|^2 = |u_bt|^2 + 2 <c> u_bt.u_bc + <c^2> |u_bc|^2
+    kx = grid.kx[:, None]
+    ky = grid.ky[None, :]
+
+    def vel(f):
+        sp = np.fft.fft2(f)
+        return np.fft.ifft2(-1j * ky * sp).real, np.fft.ifft2(1j * kx * sp).real
+
+    ub, vb = vel(bt)
+    uc, vc = vel(bc)
+    e = (np.mean(ub * ub + vb * vb) + 2 * np.mean(c) * np.mean(ub * uc + vb * vc) +
+         np.mean(c * c) * np.mean(uc * uc + vc * vc))
+    return out * (urms / math.sqrt(float(e)))
 
 
 def exponential_N2(N0sq: float = 1e-4, scale: float = 800.0):

@@ -8,6 +8,7 @@ import re
 import subprocess
 import tempfile
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -76,3 +77,27 @@ def test_header_compiles_as_c_and_cpp():
             open(p, "w").write('#include "qgybj.h"\nint main(void) { return QG_OK; }\n')
             subprocess.run([comp, "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), p, "-o",
                             os.path.join(d, "inc_" + ext)], check=True)
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libqgybj.so not built")
+def test_memory_required_scales_with_members_and_precision(qg):
+    from importlib import import_module
+    lib_mod = import_module(qg.__name__ + "._lib")
+    one = lib_mod.memory_required(nx=64, ny=64, nz=32, precision=lib_mod.PRECISION_FP64)
+    four = lib_mod.memory_required(nx=64, ny=64, nz=32, precision=lib_mod.PRECISION_FP64, members=4)
+    mixed = lib_mod.memory_required(nx=64, ny=64, nz=32, precision=lib_mod.PRECISION_MIXED)
+    npts = 64 * 64 * 32
+    assert 300 * npts < one < 500 * npts  # ~15 complex FP64 fields plus work buffers
+    assert 3.9 * one < four < 4.1 * one
+    assert mixed < one
+
+
+def test_poisson_trace_matches_its_distribution(qg):
+    tr = qg.serving.poisson_trace(4000, rate=2.0, seed=3)
+    gaps = np.diff([0.0] + [r.arrival for r in tr])
+    assert abs(np.mean(gaps) - 0.5) < 0.03
+    assert {r.nx for r in tr} == {64, 128, 256, 384} and {r.nz for r in tr} == {32, 64, 128}
+    steps = np.array([r.steps for r in tr])
+    assert steps.min() >= 100 and steps.max() <= 2000
+    assert abs(np.median(np.log(steps)) - 0.5 * (np.log(100) + np.log(2000))) < 0.1
+    assert 0.45 < np.mean([r.stratification == "constant" for r in tr]) < 0.55

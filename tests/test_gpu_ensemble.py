@@ -138,3 +138,30 @@ def test_member_check_isolates_nonfinite_member(qg):
     assert ens.check_member(1)[0] is True
     assert ens.fields.q.shape[0] == 2 * SIZE[2]
     ens.finalize()
+
+
+def test_active_members_and_member_moves(qg):
+    """Only members [0, active) step; a member moved to another slot goes on
+    exactly as it would have in its own slot."""
+    kw = _kwargs(qg)
+    ens = qg.QGYBJEnsemble(stratifications=[_strat(qg, i) for i in range(3)], **kw)
+    for i in range(3):
+        psi, B = _initial(qg, kw["grid"], i)
+        ens.set_member(i, psi=qg.FieldArray(psi), waves=qg.FieldArray(B))
+    frozen = ens.member_fields(2).q
+    ts = qg.ExponentialRungeKutta2(dt=600.0)
+    ens.set_active_members(2)
+    qg.step_(ens, ts, 2)
+    assert np.array_equal(ens.member_fields(2).q, frozen)
+    # member 2 (still at step 0) replaces member 0; both remaining runs go on
+    ens.move_member(2, 0)
+    qg.step_(ens, ts, 3)
+    ref2, ref1 = _alone(qg, 2, 3), _alone(qg, 1, 5)
+    for n in ("q", "B", "psi", "A"):
+        a, b = getattr(ens.member_fields(0), n), ref2[n]
+        assert np.max(np.abs(a - b)) <= 1e-14 * np.max(np.abs(b)), n
+        a, b = getattr(ens.member_fields(1), n), ref1[n]
+        assert np.max(np.abs(a - b)) <= 1e-14 * np.max(np.abs(b)), n
+    ens.set_active_members(0)
+    qg.step_(ens, ts, 2)  # a no-op
+    ens.finalize()
