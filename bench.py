@@ -155,6 +155,47 @@ def ensemble_leg(qg, steps, warmup=2, seed=2026):
             "seconds": total, "per_class": per_class, "nonfinite_members": bad}
 
 
+C3_SIZE = (512, 512, 256)
+
+
+def large_leg(qg, steps, kernel_steps=20, seed=2026):
+    """The single large run: 512x512x256 over 500 km x 500 km x 4000 m, the
+    256²x128 case's stratification and seeded k^-3 psi, B = 32 random
+    horizontal modes with a 30 m surface e-folding scale, feedback on, mixed
+    precision, dt = 10 s.  Throughput over `steps` graph-replayed steps, then
+    per-kernel HIP events over `kernel_steps` eager steps for the roofline of
+    each kernel at a size well past the 256 MB infinity cache."""
+    from qgybjplus_amd import synthetic
+    grid = qg.RectilinearGrid(size=C3_SIZE, extent=EXTENT, centered=True)
+    m = qg.QGYBJModel(grid=grid, coriolis=qg.FPlane(1e-4),
+                      stratification=qg.AnalyticalProfile(synthetic.exponential_N2(1e-4, 800.0)),
+                      feedback=qg.WaveMeanFeedback(), formulation=qg.YBJPlus(), architecture=qg.GPU("mixed"))
+    qg.set_(m, psi=qg.FieldArray(synthetic.k3_streamfunction(grid, seed=seed, urms=0.1, kpeak=6.0)),
+            waves=qg.FieldArray(synthetic.wave_modes(grid, 32, seed=seed, rms=0.1, efold=30.0), space="spectral"))
+    ts = qg.ExponentialRungeKutta2(dt=10.0)
+    qg.step_(m, ts, 2)
+    m._h.synchronize()
+    t0 = time.perf_counter()
+    qg.step_(m, ts, steps)
+    m._h.synchronize()
+    el = time.perf_counter() - t0
+    m._h.profile_begin(kernels=True, steps=False)
+    qg.step_(m, ts, kernel_steps)
+    m._h.synchronize()
+    prof = m._h.profile_end()
+    bad, mx = m._h.check()
+    m.finalize()
+    npts = C3_SIZE[0] * C3_SIZE[1] * C3_SIZE[2]
+    kern = {k: {"launches": v["count"], "avg_us": 1e3 * v["ms"] / v["count"],
+                "GBps": v["bytes"] / (v["ms"] / 1e3) / 1e9, "frac_of_peak": v["bytes"] / (v["ms"] / 1e3) / PEAK_HBM}
+            for k, v in prof["kernels"].items()}
+    return {"workload": "512x512x256, 500 km x 500 km x 4000 m, N2=1e-4*exp(z/800), seeded k^-3 psi, "
+                        "B = 32 random modes x exp(-depth/30 m), feedback on, YBJ+, mixed precision, dt=10 s",
+            "steps": steps, "value": npts * steps / el / 1e6, "unit": "Mgridpt-steps/s",
+            "ms_per_step": 1e3 * el / steps, "kernel_profile_steps": kernel_steps, "kernels": kern,
+            "nonfinite": bool(bad), "max_abs_psi": mx}
+
+
 def serving_leg(qg, n_requests, capacity_mpts, seed=7, utilization=0.8):
     """The serving workload: Poisson arrivals of QG-YBJ+ runs with grids
     uniform over {64,128,256,384}² x {32,64,128}, steps log-uniform over
@@ -208,6 +249,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=25.0)
     ap.add_argument("--ensemble-steps", type=int, default=200,
                     help="steps of the 48-member batched-ensemble leg (0 skips it)")
+    ap.add_argument("--large-steps", type=int, default=500,
+                    help="steps of the 512x512x256 leg (0 skips it)")
     ap.add_argument("--serve-requests", type=int, default=32,
                     help="requests of the serving leg (0 skips it)")
     ap.add_argument("--modes", choices=("both", "mixed"), default="both",
@@ -316,6 +359,9 @@ def main():
     m32.finalize()
     if m64 is not None:
         m64.finalize()
+    if args.large_steps > 0:
+        log("512x512x256 leg")
+    out["large"] = large_leg(qg, args.large_steps) if args.large_steps > 0 else None
     if args.ensemble_steps > 0:
         log("batched-ensemble leg")
     out["ensemble"] = ensemble_leg(qg, args.ensemble_steps) if args.ensemble_steps > 0 else None

@@ -58,3 +58,30 @@ def k3_streamfunction(grid, seed: int = 2026, urms: float = 0.1, kpeak: float = 
 def exponential_N2(N0sq: float = 1e-4, scale: float = 800.0):
     """N²(z) = N0² exp(z / scale)."""
     return lambda z: N0sq * math.exp(z / scale)
+
+
+def wave_modes(grid, n_modes: int = 32, seed: int = 2026, rms: float = 0.1, efold: float = 30.0,
+               profile: str = "exponential") -> np.ndarray:
+    """Spectral B, shape (nz, nx, ny): a random superposition of `n_modes`
+    horizontal Fourier modes (distinct integer wavevectors inside the 2/3
+    dealias disk, random phases, equal amplitudes with surface RMS |B| =
+    `rms`) times exp(-depth/efold) (or exp(-(depth/efold)^2) for
+    profile="gaussian").  Values follow the unnormalised forward FFT
+    convention: a mode of physical amplitude a has spectral value nx*ny*a."""
+    nx, ny, nz = grid.size
+    rng = np.random.default_rng(seed)
+    kmax = min(nx, ny) // 3
+    modes = set()
+    while len(modes) < n_modes:
+        i, j = (int(v) for v in rng.integers(-kmax, kmax + 1, size=2))
+        if (i, j) != (0, 0) and i * i + j * j <= kmax * kmax:
+            modes.add((i, j))
+    amp = rms / math.sqrt(n_modes)
+    top = np.zeros((nx, ny), dtype=np.complex128)
+    for (i, j) in sorted(modes):
+        top[i % nx, j % ny] += nx * ny * amp * np.exp(1j * rng.uniform(0.0, 2 * math.pi))
+    out = np.empty((nz, nx, ny), dtype=np.complex128)
+    for k in range(nz):
+        depth = max(0.0, -grid.z[k] - grid.dz / 2)
+        out[k] = top * (math.exp(-depth / efold) if profile == "exponential" else math.exp(-(depth / efold) ** 2))
+    return out
