@@ -1,0 +1,57 @@
+// Instantiations of the partitioned column-solve kernel (qg_colscan.hpp) for
+// every operator and compiled launch shape.
+#include "qg_colscan.hpp"
+
+#pragma clang fp contract(off)
+
+namespace qg {
+
+namespace {
+
+template <int OP, int M, int QC>
+void launch_shape(const Geo& g, hipStream_t st, const ScanArgs& a) {
+  const Quarter qr = quarter(g);
+  const ScanShape s{M, QC, (g.nzc + M - 1) / M};
+  const int blocks = (qr.size() + QC - 1) / QC;
+  hipLaunchKernelGGL((k_colscan<OP, M, QC>), dim3(blocks, g.members), dim3(s.threads()), s.lds_bytes(), st, g, qr,
+                     a);
+}
+
+template <int OP, int I>
+void launch_idx(const Geo& g, hipStream_t st, int idx, const ScanArgs& a) {
+  if constexpr (I < kScanShapeCount) {
+    if (idx == I) launch_shape<OP, kScanShapes[I][0], kScanShapes[I][1]>(g, st, a);
+    else launch_idx<OP, I + 1>(g, st, idx, a);
+  }
+}
+
+template <int OP, int I>
+bool setup_idx() {
+  if constexpr (I < kScanShapeCount) {
+    const void* f = reinterpret_cast<const void*>(k_colscan<OP, kScanShapes[I][0], kScanShapes[I][1]>);
+    if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kScanMaxLds) != hipSuccess)
+      return false;
+    return setup_idx<OP, I + 1>();
+  }
+  return true;
+}
+
+}  // namespace
+
+template <int OP>
+void launch_colscan(const Geo& g, hipStream_t st, int idx, const ScanArgs& a) {
+  launch_idx<OP, 0>(g, st, idx, a);
+}
+
+template void launch_colscan<SCAN_PSI>(const Geo&, hipStream_t, int, const ScanArgs&);
+template void launch_colscan<SCAN_A>(const Geo&, hipStream_t, int, const ScanArgs&);
+template void launch_colscan<SCAN_AC>(const Geo&, hipStream_t, int, const ScanArgs&);
+template void launch_colscan<SCAN_OMEGA>(const Geo&, hipStream_t, int, const ScanArgs&);
+
+bool colscan_setup() {
+  static const bool ok = setup_idx<SCAN_PSI, 0>() && setup_idx<SCAN_A, 0>() && setup_idx<SCAN_AC, 0>() &&
+                         setup_idx<SCAN_OMEGA, 0>();
+  return ok;
+}
+
+}  // namespace qg

@@ -1,0 +1,380 @@
+// Partitioned column solves for the per-(kx, ky) vertical operators.
+//
+// Every column operator of the step is a tridiagonal elimination whose
+// factors depend on (kx, ky) only through kh2, so the factors are recorded
+// once per model for the quarter of wavenumbers |ix| <= nx/2, |iy| <= ny/2
+// ("quarter columns", index rc = ry * (nx/2 + 1) + rx), level-indexed
+// [k][rc].  Applying a recorded elimination to a right-hand side b is two
+// first-order linear recurrences:
+//   forward   x'_k = -G_k x'_{k-1} + H_k(b_k)     (x'_{-1} = 0)
+//   backward  x_k  = -P_k x_{k+1}  + Q_k(x'_k)    (x_{kpad} = 0)
+//
+// A tile is QC consecutive quarter columns and their mirror images (+-kx,
+// +-ky): 4*QC grid columns, 8*QC column parts (the real and imaginary parts
+// are solved separately, as the reference does).  A block splits the depth
+// into P partitions of M levels; each thread owns one column part of one
+// partition and keeps its M values in registers.  A recurrence is evaluated
+// in three steps:
+//   1. every partition runs it locally from a zero carry, and the product of
+//      its multipliers (-G or -P) over the partition;
+//   2. the partitions' end values and products go through LDS, and each
+//      thread folds those of the partitions before (after) its own into the
+//      exact carry entering its partition;
+//   3. the carry times the running product of multipliers is added to each
+//      level.
+// This is the same elimination with the same recorded factors as one serial
+// sweep, evaluated with a different association of its sums (a few ulp).
+// All threads of a block do sweep work, the column state lives in registers
+// (no column tile in LDS), and the loads of a thread are independent of its
+// sweeps, so many columns per CU are in flight at once.
+#pragma once
+
+#include <type_traits>
+
+#include "qg_common.hpp"
+
+namespace qg {
+
+struct Quarter {
+  int Wr, Hr;  // nx/2 + 1, ny/2 + 1
+  __host__ __device__ int size() const { return Wr * Hr; }
+};
+
+inline Quarter quarter(const Geo& g) { return Quarter{g.nx / 2 + 1, g.ny / 2 + 1}; }
+
+struct QuadCol {
+  int x, y;
+  bool active;
+};
+
+// Grid column of mirror image m (bit 0: -kx, bit 1: -ky) of quarter column
+// rc; inactive past the quarter or when it duplicates a lower image.
+__device__ inline QuadCol quad_col(const Geo& g, const Quarter& qr, int rc, int m) {
+  QuadCol q{0, 0, false};
+  if (rc >= qr.size()) return q;
+  const int rx = rc % qr.Wr, ry = rc / qr.Wr;
+  q.x = (m & 1) ? (g.nx - rx) % g.nx : rx;
+  q.y = (m & 2) ? (g.ny - ry) % g.ny : ry;
+  q.active = !(((m & 1) && q.x == rx) || ((m & 2) && q.y == ry));
+  return q;
+}
+
+// Column operators applied by k_colscan.
+enum : int { SCAN_PSI = 0, SCAN_A = 1, SCAN_AC = 2, SCAN_OMEGA = 3 };
+
+// Launch shape: M levels per thread, QC quarter columns per tile, P
+// partitions (P * M >= nz).  A block has P * 8 * QC threads.
+struct ScanShape {
+  int M, QC, P;
+  int threads() const { return P * 8 * QC; }
+  int kpad() const { return P * M; }
+  size_t lds_bytes() const {
+    return sizeof(double) * ((size_t)2 * kpad() * QC + 2 * (size_t)kpad() + 4 * (size_t)P * 8 * QC);
+  }
+};
+
+constexpr int kScanMaxThreads = 1024;
+// Word 3 of a raw buffer resource on the gfx9 family (gfx950): dword
+// access, no format conversion.
+constexpr int kRawBufferWord3 = 0x00020000;
+constexpr size_t kScanMaxLds = 96 * 1024;
+
+// The compiled (M, QC) pairs.
+constexpr int kScanShapes[][2] = {{8, 8}, {16, 8}, {32, 8}, {16, 4}, {32, 4}, {64, 4}};
+constexpr int kScanShapeCount = sizeof(kScanShapes) / sizeof(kScanShapes[0]);
+
+inline bool scan_shape_fits(const ScanShape& s) {
+  return s.P >= 1 && s.threads() <= kScanMaxThreads && s.lds_bytes() <= kScanMaxLds;
+}
+
+inline ScanShape scan_shape_of(int idx, int nz) {
+  ScanShape s{kScanShapes[idx][0], kScanShapes[idx][1], 0};
+  s.P = (nz + s.M - 1) / s.M;
+  return s;
+}
+
+// Default shape for depth nz: the smallest compiled M with at most 16
+// partitions among the 8-quarter-column tiles (64 column parts, one wave per
+// partition), else the deepest 4-quarter-column shape that fits.
+inline int scan_default_shape(int nz) {
+  for (int i = 0; i < kScanShapeCount; ++i) {
+    ScanShape s = scan_shape_of(i, nz);
+    if (s.QC == 8 && scan_shape_fits(s) && (s.M >= nz || s.M == 32)) return i;
+  }
+  for (int i = kScanShapeCount - 1; i >= 0; --i)
+    if (scan_shape_fits(scan_shape_of(i, nz))) return i;
+  return -1;
+}
+
+struct ScanArgs {
+  const cplx<double>* rhs;  // member-0 level-0 of the right-hand side
+  cplx<double>* out;
+  cplx<double>* Cout;       // SCAN_AC: C = dA/dz
+  const double* t0;         // Thomas: 1/pivot; omega: row multiplier
+  const double* t1;         // Thomas: c'; omega: 1/pivot
+  const double* lv0;        // Thomas: a_{k-1}; omega: upper diagonal by level
+  const double* lv1;        // omega: f/N2 by level
+  int lv0_stride;           // per-member entries of lv0
+  int lv1_stride;           // per-member entries of lv1
+  const unsigned char* pivoted;  // omega: per quarter column, solved elsewhere
+  unsigned long long* warn;      // SCAN_PSI: max |q| of the kh = 0 column
+};
+
+// Keeps the scheduler from hoisting the LDS reads of a whole unrolled sweep
+// ahead of its first levels (which would hold every level's factors in
+// registers at once): sweeps are scheduled in chunks of 8 levels.
+__device__ inline void sweep_chunk_fence(int u) {
+  if ((u & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+}
+
+template <int M, int QC>
+struct ScanBounds {
+  static constexpr int TP = 8 * QC;
+  static constexpr int MAXP = kScanMaxThreads / TP < 16 ? kScanMaxThreads / TP : 16;
+  static constexpr int MAXT = MAXP * TP;
+};
+
+// blockIdx.x: tile of QC quarter columns; blockIdx.y: member.
+template <int OP, int M, int QC>
+__global__ void __launch_bounds__((ScanBounds<M, QC>::MAXT))
+k_colscan(Geo g, Quarter qr, ScanArgs a) {
+  constexpr int TP = 8 * QC;
+  constexpr int R = (M * QC + TP - 1) / TP;  // table entries staged per thread
+  constexpr int LR = (M + TP - 1) / TP;      // level entries staged per thread
+  const int nz = g.nzc;
+  const int P = blockDim.x / TP;
+  const int kpad = P * M;
+  const int w = threadIdx.x / TP, t = threadIdx.x % TP;
+  const int part = t & 1, cs = t >> 1, j = cs % QC, mir = cs / QC;
+  const int rplane = qr.size();
+  const int rc0 = blockIdx.x * QC;
+  const size_t plane = (size_t)g.nx * g.ny;
+  const size_t mem = blockIdx.y;
+  const double* rhs = reinterpret_cast<const double*>(a.rhs + mem * nz * plane);
+  double* out = reinterpret_cast<double*>(a.out + mem * nz * plane);
+  const double* t0 = a.t0 + mem * nz * rplane;
+  const double* t1 = a.t1 + mem * nz * rplane;
+  const double* lv0 = a.lv0 + mem * a.lv0_stride;
+  const double* lv1 = (OP == SCAN_OMEGA) ? a.lv1 + mem * a.lv1_stride : nullptr;
+
+  extern __shared__ double lds[];
+  double* T0 = lds;               // [kpad][QC]
+  double* T1 = T0 + kpad * QC;    // [kpad][QC]
+  double* L0 = T1 + kpad * QC;    // [kpad]
+  double* L1 = L0 + kpad;         // [kpad]
+  double* XF = L1 + kpad;         // [2][P][TP]: forward end values, products
+  double* XB = XF + 2 * P * TP;   // [2][P][TP]: backward end values, products
+
+  const QuadCol q = quad_col(g, qr, rc0 + j, mir);
+  // This thread's column part within a level, in doubles.  With 64 column
+  // parts per partition a partition is one wave, so its first level is
+  // wave-uniform and every level's address is a scalar base plus this offset.
+  constexpr bool UNI = TP % 64 == 0;
+  const unsigned boff = 8u * (2u * (q.active ? (unsigned)(q.y * g.nx + q.x) : 0u) + (unsigned)part);
+  int k0 = w * M;
+  if constexpr (UNI) k0 = __builtin_amdgcn_readfirstlane(k0);
+  const size_t lvlb = plane * sizeof(cplx<double>);  // bytes per level
+  // Wave-uniform partitions address their levels through a buffer resource
+  // over the partition's levels inside the field: one 32-bit lane offset and
+  // a scalar offset per level; levels past nz read as zero and are not
+  // stored.  Otherwise each level's address is formed per lane, clamped.
+  const int nlev = min(M, nz - k0);
+  auto rsrc = [&](const void* field) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(static_cast<const char*>(field)) + (size_t)k0 * lvlb,
+                                             0, (int)(nlev * lvlb), kRawBufferWord3);
+  };
+  auto at_level = [&](auto* base, int k) {
+    using T = std::remove_pointer_t<decltype(base)>;
+    using Byte = std::conditional_t<std::is_const_v<T>, const char, char>;
+    return reinterpret_cast<T*>(reinterpret_cast<Byte*>(base + (size_t)k * (2 * plane)) + boff);
+  };
+
+  // Loads: the tables of the tile first, then this thread's right-hand side
+  // levels; addresses are clamped so that no load is conditional.
+  double v0[R], v1[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int e = threadIdx.x + r * blockDim.x;
+    const int k = min(e / QC, nz - 1), jj = min(rc0 + e % QC, rplane - 1);
+    v0[r] = t0[(size_t)k * rplane + jj];
+    v1[r] = t1[(size_t)k * rplane + jj];
+  }
+  double l0[LR], l1[LR];
+#pragma unroll
+  for (int r = 0; r < LR; ++r) {
+    const int e = min((int)threadIdx.x + r * (int)blockDim.x, nz - 1);
+    l0[r] = lv0[e];
+    l1[r] = (OP == SCAN_OMEGA) ? lv1[e] : 0.0;
+  }
+  double y[M];
+  if constexpr (UNI) {
+    const auto rs = rsrc(rhs);
+#pragma unroll
+    for (int u = 0; u < M; ++u)
+      y[u] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, boff, u * (int)lvlb, 0));
+  } else {
+#pragma unroll
+    for (int u = 0; u < M; ++u) y[u] = *at_level(rhs, min(k0 + u, nz - 1));
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int e = threadIdx.x + r * blockDim.x;
+    if (e < kpad * QC) {
+      const bool ok = e / QC < nz && rc0 + e % QC < rplane;
+      T0[e] = ok ? v0[r] : 0.0;
+      T1[e] = ok ? v1[r] : 0.0;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < LR; ++r) {
+    const int e = threadIdx.x + r * blockDim.x;
+    if (e < kpad) {
+      L0[e] = e < nz ? l0[r] : 0.0;
+      if (OP == SCAN_OMEGA) L1[e] = (e >= 1 && e <= nz - 2) ? l1[r] : 0.0;  // f/N2 on interior rows
+    }
+  }
+  __syncthreads();
+
+  const double dz2 = g.dz * g.dz;
+  auto G = [&](int k) -> double {
+    if constexpr (OP == SCAN_OMEGA) return T0[k * QC + j];
+    else return L0[k] * T0[k * QC + j];
+  };
+  auto H = [&](int k, double b) -> double {
+    if constexpr (OP == SCAN_OMEGA) return L1[k] * b;
+    else return (dz2 * b) * T0[k * QC + j];
+  };
+  auto Pm = [&](int k) -> double {
+    if constexpr (OP == SCAN_OMEGA) return L0[k] * T1[k * QC + j];
+    else return T1[k * QC + j];
+  };
+  auto Q = [&](int k, double xp) -> double {
+    if constexpr (OP == SCAN_OMEGA) return xp * T1[k * QC + j];
+    else return xp;
+  };
+
+  const bool mean_col = q.active && q.x == 0 && q.y == 0;
+
+  // q -> psi: psi = 0 on the horizontal mean; warn on a significant mean of
+  // q (src/elliptic.jl:257-270).  Block 0 holds that column.
+  if (OP == SCAN_PSI && a.warn && nz > 1 && blockIdx.x == 0) {
+    double m = 0.0;
+#pragma unroll
+    for (int u = 0; u < M; ++u) {
+      const double other = __shfl_xor(y[u], 1);
+      const double av = hypot(y[u], other);
+      if (k0 + u < nz) m = av > m ? av : m;
+    }
+    if (mean_col && part == 0 && m > 1e-10) atomicMax(a.warn, (unsigned long long)__double_as_longlong(m));
+  }
+
+  // One-level columns need no special case: their recorded factor is the
+  // reciprocal of the operator's only entry (0 where kh = 0) and the omega
+  // equation has no interior row.
+  double Y = 0.0;  // final value of the level above this partition
+  {
+    // forward recurrence, local
+    double xp = 0.0, phi = 1.0;
+#pragma unroll
+    for (int u = 0; u < M; ++u) {
+      const int k = k0 + u;
+      const double gk = G(k);
+      xp = fma(-gk, xp, H(k, y[u]));
+      y[u] = xp;
+      phi = phi * -gk;
+      sweep_chunk_fence(u);
+    }
+    XF[w * TP + t] = xp;
+    XF[(P + w) * TP + t] = phi;
+    __syncthreads();
+    if (w > 0) {
+      double X = 0.0;  // x' of the level below this partition
+      for (int p = 0; p < w; ++p) X = fma(XF[(P + p) * TP + t], X, XF[p * TP + t]);
+      double ph = 1.0;
+#pragma unroll
+      for (int u = 0; u < M; ++u) {
+        ph = ph * -G(k0 + u);
+        y[u] = fma(ph, X, y[u]);
+        sweep_chunk_fence(u);
+      }
+    }
+    // backward recurrence, local
+    double xn = 0.0, psi = 1.0;
+#pragma unroll
+    for (int u = M - 1; u >= 0; --u) {
+      const int k = k0 + u;
+      const double pk = Pm(k);
+      xn = fma(-pk, xn, Q(k, y[u]));
+      y[u] = xn;
+      psi = psi * -pk;
+      sweep_chunk_fence(M - 1 - u);
+    }
+    XB[w * TP + t] = xn;
+    XB[(P + w) * TP + t] = psi;
+    __syncthreads();
+    if (w < P - 1) {
+      for (int p = P - 1; p > w; --p) Y = fma(XB[(P + p) * TP + t], Y, XB[p * TP + t]);
+      double ps = 1.0;
+#pragma unroll
+      for (int u = M - 1; u >= 0; --u) {
+        ps = ps * -Pm(k0 + u);
+        y[u] = fma(ps, Y, y[u]);
+        sweep_chunk_fence(M - 1 - u);
+      }
+    }
+    // B -> A: remove the vertical mean of the kh = 0 column
+    // (src/elliptic.jl:930-945).  XF is free after the second barrier.
+    if ((OP == SCAN_A || OP == SCAN_AC) && blockIdx.x == 0) {
+      double s = 0.0;
+#pragma unroll
+      for (int u = 0; u < M; ++u)
+        if (k0 + u < nz) s = s + y[u];
+      XF[w * TP + t] = s;
+      __syncthreads();
+      if (mean_col) {
+        double tot = 0.0;
+        for (int p = 0; p < P; ++p) tot = tot + XF[p * TP + t];
+        const double mean = tot / nz;
+#pragma unroll
+        for (int u = 0; u < M; ++u) y[u] = y[u] - mean;
+        Y = Y - mean;
+      }
+    }
+  }
+
+  if (!q.active) return;
+  if (OP == SCAN_OMEGA && a.pivoted[mem * rplane + rc0 + j]) return;  // k_omega_pivoted
+  const bool zero_col = OP == SCAN_PSI && nz > 1 && mean_col;
+  double* Co = (OP == SCAN_AC) ? reinterpret_cast<double*>(a.Cout + mem * nz * plane) : nullptr;
+  using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(rsrc(out), 0, 0, 0));
+  const auto ro = rsrc(out);
+  const auto rcz = rsrc((OP == SCAN_AC) ? Co : out);
+#pragma unroll
+  for (int u = 0; u < M; ++u) {
+    const int k = k0 + u;
+    if (k < nz) {
+      const double r = zero_col ? 0.0 : y[u];
+      double c = 0.0;
+      if constexpr (OP == SCAN_AC) {
+        const double up = u + 1 < M ? y[u + 1] : Y;
+        c = k < nz - 1 ? (up - r) / g.dz : 0.0;
+      }
+      if constexpr (UNI) {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, r), ro, boff, u * (int)lvlb, 0);
+        if constexpr (OP == SCAN_AC)
+          __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, c), rcz, boff, u * (int)lvlb, 0);
+      } else {
+        *at_level(out, k) = r;
+        if constexpr (OP == SCAN_AC) *at_level(Co, k) = c;
+      }
+    }
+  }
+}
+
+// Host side: launch OP with compiled shape `idx` (see kScanShapes).
+template <int OP>
+void launch_colscan(const Geo& g, hipStream_t st, int idx, const ScanArgs& a);
+// Raise the dynamic LDS limit of every instantiation (once per process).
+bool colscan_setup();
+
+}  // namespace qg

@@ -5,8 +5,9 @@
 
 namespace qg {
 
-// Column solves run as mirror tiles (qg_mirror.hpp); depths up to 1024
-// levels are supported.
+// Column solves run as partitioned column kernels (qg_colscan.hpp); depths up
+// to 1024 levels are supported.  `shape` selects a compiled launch shape
+// (an index into kScanShapes; -1: the default for the depth).
 bool column_depth_supported(int nz);
 // Thomas elimination tables (1/pivot, c') of the quarter columns
 // |ix| <= nx/2, |iy| <= ny/2, level-indexed [k][rc] (kdiv 1: q->psi,
@@ -22,17 +23,19 @@ int thomas_shifted_a_size(int nzc);
 // Thomas solves of all columns; C (= dA/dz) is written when Cout is set.
 void launch_thomas(const Geo& g, hipStream_t st, const cplx<double>* rhs, cplx<double>* out,
                    cplx<double>* Cout, const double* inv, const double* cp, const double* al, int kdiv,
-                   unsigned long long* warn);
+                   unsigned long long* warn, int shape = -1);
 // QG omega-equation interior solves: elimination recorded once per model
-// (tables of thomas_table_size() entries, pivoted flags per quarter column),
-// then applied to every step's right-hand sides.  cu: upper diagonal by level,
-// zero-padded to thomas_shifted_a_size(nz) entries.
+// (tables of thomas_table_size() entries, pivoted flags per quarter column,
+// err[3] set when any column interchanged rows), then applied to every
+// step's right-hand sides.  cu: upper diagonal by level, zero-padded to
+// thomas_shifted_a_size(nz) entries.  Columns with interchanges are solved
+// by a second kernel when any_pivoted is set.
 void launch_omega_factor(const Geo& g, hipStream_t st, const double* cz, double* M, double* D,
                          unsigned char* pivoted, int* err);
 bool omega_setup();
 void launch_omega(const Geo& g, hipStream_t st, const cplx<double>* rhsw, cplx<double>* wk, const double* M,
-                  const double* D, const unsigned char* pivoted, const double* cz, const double* cu,
-                  const double* fN2, double* sd, double* sdu, double* sfill);
+                  const double* D, const unsigned char* pivoted, bool any_pivoted, const double* cz,
+                  const double* cu, const double* fN2, double* sd, double* sdu, double* sfill, int shape = -1);
 void launch_q_from_psi(const Geo& g, hipStream_t st, const cplx<double>* psi, cplx<double>* q,
                        const double* a, int barotropic);
 template <typename T>

@@ -5,15 +5,15 @@
 // pivoting) once for the real and once for the imaginary part of each column.
 // The elimination depends only on (kh2, f^2/N^2, dz), so k_omega_factor runs
 // it once per model for the quarter columns and records, level-indexed, the
-// multiplier applied to each row and the reciprocal of its eliminated pivot;
-// a column whose elimination interchanged rows is marked and solved by the
-// general path, a restatement of dgtsv.  Per step, k_omega is a mirror-tile
-// kernel (qg_mirror.hpp): it applies the recorded elimination to the
-// right-hand sides, b_k -= m_k b_{k-1}, and back substitutes
-// x_k = (b_k - u_k x_{k+1}) / d_k as fma(-(u_k / d_k), x_{k+1}, b_k / d_k),
-// with the quotients formed from the stored reciprocals.
+// multiplier applied to each row and the reciprocal of its eliminated pivot.
+// Per step the partitioned column kernel (qg_colscan.hpp) applies the
+// recorded elimination to the right-hand sides, b_k -= m_k b_{k-1}, and back
+// substitutes x_k = (b_k - u_k x_{k+1}) / d_k as
+// fma(-(u_k / d_k), x_{k+1}, b_k / d_k), with the quotients formed from the
+// stored reciprocals.  A column whose elimination interchanged rows is marked
+// and solved by k_omega_pivoted, a restatement of dgtsv.
 #include "qg_columns.hpp"
-#include "qg_mirror.hpp"
+#include "qg_colscan.hpp"
 
 #pragma clang fp contract(off)
 
@@ -34,7 +34,7 @@ __device__ inline void flag_error(int* err, int code, int level, int col) {
 //   D[k]: 1 / eliminated pivot of row k-1 (0 unless 1 <= k <= nz-2 and
 //         kh2 > 0),
 //   pivoted[rc]: any row interchange (dgtsv INFO > 0 is recorded in err,
-//   code 2).
+//   code 2; err[3] is set when any column interchanged rows).
 __global__ void k_omega_factor(Geo g, Quarter qr, const double* __restrict__ cz, double* __restrict__ M,
                                double* __restrict__ D, unsigned char* __restrict__ pivoted,
                                int* __restrict__ err) {
@@ -84,23 +84,55 @@ __global__ void k_omega_factor(Geo g, Quarter qr, const double* __restrict__ cz,
   if (d == 0.0) flag_error(err, 2, n - 1, y * g.nx + x);
   D[at(n)] = 1.0 / d;
   pivoted[rc] = piv ? 1 : 0;
+  if (piv) atomicOr(err + 3, 1);
 }
 
-// General dgtsv path (row interchanges) for one column; b (complex) in LDS
-// with row stride TC2 doubles, eliminated rows kept in global scratch.
-__device__ void gtsv_general(double* b0, int TC2, int n, double kh2, const double* cz, double* sd, double* sdu,
-                             double* sfill, int plane, int col) {
+// dgtsv with row interchanges for the columns k_omega_factor marked, one
+// thread per grid column (both parts).  b = (f/N2) * RHS on the interior
+// levels is formed in wk and solved in place; the eliminated rows are kept in
+// the scratch arrays sd, sdu, sfill (per grid column and level).
+__global__ void k_omega_pivoted(Geo g, Quarter qr, const cplx<double>* __restrict__ rhsw, cplx<double>* __restrict__ wk,
+                                const unsigned char* __restrict__ pivoted, const double* __restrict__ cz,
+                                const double* __restrict__ fN2, double* __restrict__ sd, double* __restrict__ sdu,
+                                double* __restrict__ sfill) {
+  const int rplane = qr.size();
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= 4 * rplane) return;
+  const int rc = id >> 2, mir = id & 3;
+  const int nz = g.nzc;
+  const size_t plane = (size_t)g.nx * g.ny;
+  const size_t mem = blockIdx.y;
+  if (!pivoted[mem * rplane + rc] || nz <= 2) return;
+  const QuadCol q = quad_col(g, qr, rc, mir);
+  if (!q.active) return;
+  rhsw += mem * nz * plane;
+  wk += mem * nz * plane;
+  cz += mem * nz;
+  fN2 += mem * nz;
+  sd += mem * nz * plane;
+  sdu += mem * nz * plane;
+  sfill += mem * nz * plane;
+  const size_t col = (size_t)q.y * g.nx + q.x;
+  const double kh2 = g.kx[q.x] * g.kx[q.x] + g.ky[q.y] * g.ky[q.y];
+  const int n = nz - 2;
   auto at = [&](int iz) { return (size_t)iz * plane + col; };
-  auto Br = [&](int iz) -> double& { return b0[(size_t)iz * TC2]; };
-  auto Bi = [&](int iz) -> double& { return b0[(size_t)iz * TC2 + 1]; };
+  // interior row iz is level iz + 1
+  auto Wv = [&](int iz) -> cplx<double>& { return wk[(size_t)(iz + 1) * plane + col]; };
+  for (int iz = 0; iz < n; ++iz) {
+    const cplx<double> r = rhsw[(size_t)(iz + 1) * plane + col];
+    const double s = fN2[iz + 1];
+    Wv(iz) = mk(s * r.re, s * r.im);
+  }
+  wk[col] = mk(0.0, 0.0);
+  wk[(size_t)(nz - 1) * plane + col] = mk(0.0, 0.0);
   double d = -2 * cz[1] - kh2;
   double du = (n > 1) ? cz[1] : 0.0;
-  double br = Br(0), bi = Bi(0);
+  double br = Wv(0).re, bi = Wv(0).im;
   for (int i = 0; i < n - 1; ++i) {
     const double dl = cz[i + 2];
     double dn = -2 * cz[i + 2] - kh2;
     double dun = (i + 1 < n - 1) ? cz[i + 2] : 0.0;
-    double bnr = Br(i + 1), bni = Bi(i + 1);
+    double bnr = Wv(i + 1).re, bni = Wv(i + 1).im;
     double fill = 0.0;
     if (fabs(d) >= fabs(dl)) {
       const double f = dl / d;
@@ -126,120 +158,34 @@ __device__ void gtsv_general(double* b0, int TC2, int n, double kh2, const doubl
     sd[at(i)] = d;
     sdu[at(i)] = du;
     sfill[at(i)] = fill;
-    Br(i) = br;
-    Bi(i) = bi;
+    Wv(i) = mk(br, bi);
     d = dn;
     du = dun;
     br = bnr;
     bi = bni;
   }
   double xr = br / d, xi = bi / d;
-  Br(n - 1) = xr;
-  Bi(n - 1) = xi;
+  Wv(n - 1) = mk(xr, xi);
   if (n > 1) {
     const int i = n - 2;
-    double x0r = (Br(i) - sdu[at(i)] * xr) / sd[at(i)];
-    double x0i = (Bi(i) - sdu[at(i)] * xi) / sd[at(i)];
-    Br(i) = x0r;
-    Bi(i) = x0i;
+    const double x0r = (Wv(i).re - sdu[at(i)] * xr) / sd[at(i)];
+    const double x0i = (Wv(i).im - sdu[at(i)] * xi) / sd[at(i)];
+    Wv(i) = mk(x0r, x0i);
     double x2r = xr, x2i = xi;
     xr = x0r;
     xi = x0i;
     for (int ii = n - 3; ii >= 0; --ii) {
       const double dd = sd[at(ii)], uu = sdu[at(ii)], ff = sfill[at(ii)];
-      double vr = (Br(ii) - uu * xr - ff * x2r) / dd;
-      double vi = (Bi(ii) - uu * xi - ff * x2i) / dd;
-      Br(ii) = vr;
-      Bi(ii) = vi;
+      const cplx<double> b = Wv(ii);
+      const double vr = (b.re - uu * xr - ff * x2r) / dd;
+      const double vi = (b.im - uu * xi - ff * x2i) / dd;
+      Wv(ii) = mk(vr, vi);
       x2r = xr;
       x2i = xi;
       xr = vr;
       xi = vi;
     }
   }
-}
-
-// rhsw: the omega right-hand side (complex, all levels); wk: w-hat, zero on
-// the end levels and where kh = 0.  blockIdx.y is the member.  cu[k] = cz[k]
-// for 1 <= k <= nzc-3 (the upper diagonal of row k-1), zero elsewhere, padded
-// to mirror_pad(nzc) per member; cz and fN2 have nzc entries per member.
-template <int QC>
-__global__ void __launch_bounds__(kMirrorThreads)
-k_omega(Geo g, Quarter qr, const cplx<double>* __restrict__ rhsw, cplx<double>* __restrict__ wk,
-        const double* __restrict__ M, const double* __restrict__ D, const unsigned char* __restrict__ pivoted,
-        const double* __restrict__ cz, const double* __restrict__ cu, const double* __restrict__ fN2,
-        double* __restrict__ sd, double* __restrict__ sdu, double* __restrict__ sfill) {
-  constexpr int NC = 4 * QC;
-  extern __shared__ cplx<double> xs[];  // [kpad][NC], then tm[kpad][QC], td[kpad][QC]
-  const int nz = g.nzc, kpad = mirror_pad(nz);
-  const int rc0 = blockIdx.x * QC;
-  const size_t plane = (size_t)g.nx * g.ny;
-  {
-    const size_t m = blockIdx.y;
-    rhsw += m * nz * plane;
-    wk += m * nz * plane;
-    M += m * nz * qr.size();
-    D += m * nz * qr.size();
-    pivoted += m * qr.size();
-    cz += m * nz;
-    cu += m * kpad;
-    fN2 += m * nz;
-    sd += m * nz * plane;
-    sdu += m * nz * plane;
-    sfill += m * nz * plane;
-  }
-  double* tm = reinterpret_cast<double*>(xs + (size_t)kpad * NC);
-  double* td = tm + (size_t)kpad * QC;
-  const MirrorSlot<QC> sl(g, qr, rc0);
-  // b = (f / N2[k]) * RHS[k] on the interior levels (src/operators.jl:439)
-  mirror_stage<QC>(g, qr, rc0, sl, rhsw, M, D, xs, tm, td, kpad,
-                   [&](int k) { return (k >= 1 && k <= nz - 2) ? fN2[k] : 0.0; });
-  __syncthreads();
-
-  const int lane = threadIdx.x;
-  if (lane < 2 * NC) {
-    const int cs = lane >> 1, part = lane & 1;
-    const int rc = rc0 + cs % QC;
-    const QuadCol q = quad_col(g, qr, rc, cs / QC);
-    if (q.active && nz > 2) {
-      double* b0 = reinterpret_cast<double*>(xs + cs) + part;  // level k at b0[k * 2NC]
-      const int st = 2 * NC;
-      if (pivoted[rc]) {
-        const double kh2 = g.kx[q.x] * g.kx[q.x] + g.ky[q.y] * g.ky[q.y];
-        if (part == 0) gtsv_general(b0 + st, st, nz - 2, kh2, cz, sd, sdu, sfill, g.nx * g.ny, q.y * g.nx + q.x);
-      } else {
-        const double* tM = tm + cs % QC;
-        const double* tD = td + cs % QC;
-        mirror_sweeps(
-            b0, st, kpad, [&](int k, double b) { return make_double2(tM[k * QC], b); },
-            [&](int k, double bp) {
-              const double r = tD[k * QC];
-              return make_double2(cu[k] * r, bp * r);
-            });
-      }
-    }
-  }
-  __syncthreads();
-
-  if (!sl.q.active) return;
-  for (int k = sl.kf; k < nz; k += MirrorSlot<QC>::kstep) wk[(size_t)k * plane + sl.off] = xs[k * NC + sl.c];
-}
-
-template <int QC>
-void launch_qc(const Geo& g, hipStream_t st, const cplx<double>* rhsw, cplx<double>* wk, const double* M,
-               const double* D, const unsigned char* pivoted, const double* cz, const double* cu, const double* fN2,
-               double* sd, double* sdu, double* sfill) {
-  const Quarter qr = quarter(g);
-  const int blocks = (qr.size() + QC - 1) / QC;
-  hipLaunchKernelGGL((k_omega<QC>), dim3(blocks, g.members), dim3(kMirrorThreads),
-                     mirror_lds_bytes(mirror_pad(g.nzc), QC), st, g, qr, rhsw, wk, M, D, pivoted, cz, cu, fN2, sd, sdu,
-                     sfill);
-}
-
-template <int QC>
-bool set_lds_attr() {
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(k_omega<QC>), hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)kMirrorLdsBytes) == hipSuccess;
 }
 
 }  // namespace
@@ -252,17 +198,28 @@ void launch_omega_factor(const Geo& g, hipStream_t st, const double* cz, double*
                      pivoted, err);
 }
 
-bool omega_setup() { return set_lds_attr<8>() && set_lds_attr<4>() && set_lds_attr<2>() && set_lds_attr<1>(); }
+bool omega_setup() { return colscan_setup(); }
 
 void launch_omega(const Geo& g, hipStream_t st, const cplx<double>* rhsw, cplx<double>* wk, const double* M,
-                  const double* D, const unsigned char* pivoted, const double* cz, const double* cu,
-                  const double* fN2, double* sd, double* sdu, double* sfill) {
-  switch (mirror_qc(g.nzc)) {
-    case 8: launch_qc<8>(g, st, rhsw, wk, M, D, pivoted, cz, cu, fN2, sd, sdu, sfill); return;
-    case 4: launch_qc<4>(g, st, rhsw, wk, M, D, pivoted, cz, cu, fN2, sd, sdu, sfill); return;
-    case 2: launch_qc<2>(g, st, rhsw, wk, M, D, pivoted, cz, cu, fN2, sd, sdu, sfill); return;
-    case 1: launch_qc<1>(g, st, rhsw, wk, M, D, pivoted, cz, cu, fN2, sd, sdu, sfill); return;
-    default: return;  // rejected by column_depth_supported at model creation
+                  const double* D, const unsigned char* pivoted, bool any_pivoted, const double* cz,
+                  const double* cu, const double* fN2, double* sd, double* sdu, double* sfill, int shape) {
+  if (shape < 0) shape = scan_default_shape(g.nzc);
+  ScanArgs a{};
+  a.rhs = rhsw;
+  a.out = wk;
+  a.t0 = M;
+  a.t1 = D;
+  a.lv0 = cu;
+  a.lv0_stride = thomas_shifted_a_size(g.nzc);
+  a.lv1 = fN2;
+  a.lv1_stride = g.nzc;
+  a.pivoted = pivoted;
+  launch_colscan<SCAN_OMEGA>(g, st, shape, a);
+  if (any_pivoted) {
+    const Quarter qr = quarter(g);
+    const int bs = 64;
+    hipLaunchKernelGGL(k_omega_pivoted, dim3((4 * qr.size() + bs - 1) / bs, g.members), dim3(bs), 0, st, g, qr,
+                       rhsw, wk, pivoted, cz, fN2, sd, sdu, sfill);
   }
 }
 

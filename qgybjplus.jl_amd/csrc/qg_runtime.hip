@@ -65,6 +65,7 @@ struct qg_handle {
   // recorded dgtsv elimination of the omega-equation columns
   double *om_M = nullptr, *om_D = nullptr;
   unsigned char* om_piv = nullptr;
+  bool om_any_piv = false;  // some column's elimination interchanged rows
   int sing_omega[3] = {0, 0, 0};
   void *u = nullptr, *v = nullptr, *w = nullptr;
   void *T1 = nullptr, *T3 = nullptr, *G3 = nullptr, *G4 = nullptr, *G5 = nullptr;
@@ -346,8 +347,8 @@ struct Engine {
       throw QgError(QG_ERR_GTSV, buf);
     }
     K("omega_gtsv", P * 2 * cD + Tq, [&] {
-      qg::launch_omega(h.g, h.st, h.rq0, h.qeff, h.om_M, h.om_D, h.om_piv, h.d_cz, h.d_cu, h.d_fN2, h.s0, h.s1,
-                       h.s2);
+      qg::launch_omega(h.g, h.st, h.rq0, h.qeff, h.om_M, h.om_D, h.om_piv, h.om_any_piv, h.d_cz, h.d_cu,
+                       h.d_fN2, h.s0, h.s1, h.s2);
     });
     K("y_herm_w", P * cD + Ph * cR, [&] { O::y_herm(h.g, h.st, h.qeff, c(h.Pz), twy()); });
     K("x_c2r_w", Ph * cR + P * rR, [&] { O::x_c2r(h.g, h.st, 1, c(h.Pz), nullptr, r(h.w), nullptr, twx()); });
@@ -718,6 +719,7 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
       HIPCHECK(hipMemcpyAsync(he, e, sizeof he, hipMemcpyDeviceToHost, h->st));
       HIPCHECK(hipStreamSynchronize(h->st));
       std::memcpy(h->sing_omega, he, 3 * sizeof(int));
+      h->om_any_piv = he[3] != 0;
     }
     HIPCHECK(hipDeviceSynchronize());
     *out = h;
@@ -992,6 +994,7 @@ int qg_set_member_stratification(qg_handle* h, int member, const double* N2, con
                             H.om_piv + (size_t)member * quarter, H.d_ferr);
     HIPCHECK(hipMemcpyAsync(he, H.d_ferr, sizeof he, hipMemcpyDeviceToHost, H.st));
     HIPCHECK(hipStreamSynchronize(H.st));
+    if (he[3]) H.om_any_piv = true;  // kept for the ensemble's lifetime
     if (he[0]) {
       char buf[160];
       snprintf(buf, sizeof buf, "LAPACK-style gtsv failed for vertical velocity at row %d of column %d",

@@ -5,13 +5,14 @@
 // The elimination of a column depends only on (kh2, a, dz), so
 // k_thomas_factor runs it once per model and records per level the pivot
 // denom_k, its reciprocal and c'_k for the quarter columns (the reference
-// recomputes them on every call).  A solve is a mirror-tile kernel: the
-// forward sweep x'_k = (dz2 b_k - a_{k-1} x'_{k-1}) / denom_k is evaluated as
+// recomputes them on every call).  A solve applies them with the partitioned
+// column kernel (qg_colscan.hpp): the forward sweep
+// x'_k = (dz2 b_k - a_{k-1} x'_{k-1}) / denom_k is evaluated as
 // fma(-(a_{k-1} inv_k), x'_{k-1}, (dz2 b_k) inv_k) and the back substitution
-// as fma(-c'_k, x_{k+1}, x'_k), one FMA per level on the serial chain; the
-// results differ from the reference's division by a few ulp.
+// as fma(-c'_k, x_{k+1}, x'_k); the results differ from the reference's
+// division by a few ulp.
 #include "qg_columns.hpp"
-#include "qg_mirror.hpp"
+#include "qg_colscan.hpp"
 
 #pragma clang fp contract(off)
 
@@ -43,7 +44,14 @@ __global__ void k_thomas_factor(Geo g, Quarter qr, const double* __restrict__ a,
   const double kh2 = g.kx[x] * g.kx[x] + g.ky[y] * g.ky[y];
   const double dz2 = g.dz * g.dz;
   auto at = [&](int k) { return (size_t)k * rplane + rc; };
-  if (nz == 1 || (kh2 == 0.0 && kdiv == 1)) {
+  if (nz == 1) {
+    // one level: psi = -q / kh2, A = -4 B / kh2 (0 where kh = 0); the solve
+    // multiplies by this reciprocal (dz = 1 for one level)
+    inv[at(0)] = kh2 == 0.0 ? 0.0 : (kdiv == 1 ? -1.0 : -4.0) / kh2;
+    cp[at(0)] = 0.0;
+    return;
+  }
+  if (kh2 == 0.0 && kdiv == 1) {  // psi = 0 on the horizontal mean, written by the solve
     for (int k = 0; k < nz; ++k) {
       inv[at(k)] = 1.0;
       cp[at(k)] = 0.0;
@@ -72,130 +80,13 @@ __global__ void k_thomas_factor(Geo g, Quarter qr, const double* __restrict__ a,
   }
 }
 
-// KIND 0: q -> psi; 1: B -> A; 2: B -> A and C = dA/dz.  blockIdx.y is the
-// member.  al[k] = a[k-1] (al[0] = 0), zero-padded to mirror_pad(nzc)
-// entries per member.
-template <int KIND, int QC>
-__global__ void __launch_bounds__(kMirrorThreads)
-k_thomas(Geo g, Quarter qr, const cplx<double>* __restrict__ rhs, cplx<double>* __restrict__ out,
-         cplx<double>* __restrict__ Cout, const double* __restrict__ inv, const double* __restrict__ cp,
-         const double* __restrict__ al, unsigned long long* __restrict__ warn) {
-  constexpr int NC = 4 * QC;
-  extern __shared__ cplx<double> xs[];  // [kpad][NC], then ti[kpad][QC], tc[kpad][QC]
-  const int nz = g.nzc, kpad = mirror_pad(nz);
-  const int rc0 = blockIdx.x * QC;
-  const size_t plane = (size_t)g.nx * g.ny;
-  {
-    const size_t m = blockIdx.y;
-    rhs += m * nz * plane;
-    out += m * nz * plane;
-    if (KIND == 2) Cout += m * nz * plane;
-    inv += m * nz * qr.size();
-    cp += m * nz * qr.size();
-    al += m * kpad;
-  }
-  double* ti = reinterpret_cast<double*>(xs + (size_t)kpad * NC);
-  double* tc = ti + (size_t)kpad * QC;
-  const MirrorSlot<QC> sl(g, qr, rc0);
-  mirror_stage<QC>(g, qr, rc0, sl, rhs, inv, cp, xs, ti, tc, kpad, [](int) { return 1.0; });
-  __syncthreads();
-
-  const int lane = threadIdx.x;
-  if (lane < 2 * NC) {
-    const int cs = lane >> 1, part = lane & 1;
-    const QuadCol q = quad_col(g, qr, rc0 + cs % QC, cs / QC);
-    if (q.active) {
-      double* b0 = reinterpret_cast<double*>(xs + cs) + part;  // level k at b0[k * 2NC]
-      const int st = 2 * NC;
-      const double* tI = ti + cs % QC;
-      const double* tC = tc + cs % QC;
-      const double kh2 = g.kx[q.x] * g.kx[q.x] + g.ky[q.y] * g.ky[q.y];
-      const double dz2 = g.dz * g.dz;
-      if (nz == 1) {
-        double& v = b0[0];
-        if (kh2 == 0.0) v = 0.0;
-        else if (KIND == 0) v = -v / kh2;
-        else v = (-4 * v) / kh2;
-      } else if (kh2 == 0.0 && KIND == 0) {
-        // psi = 0 on the horizontal mean (src/elliptic.jl:257-270); warn on a
-        // significant mean of q.
-        if (part == 0 && warn) {
-          double m = 0.0;
-          for (int k = 0; k < nz; ++k) {
-            const double av = hypot(b0[k * st], b0[k * st + 1]);
-            m = av > m ? av : m;
-          }
-          if (m > 1e-10) atomicMax(warn, (unsigned long long)__double_as_longlong(m));
-        }
-      } else {
-        // For the B -> A mean column the stored reciprocal of the gauge row
-        // A[1] = 0 is 0, which gives x'_0 = 0.
-        mirror_sweeps(
-            b0, st, kpad,
-            [&](int k, double b) {
-              const double y = tI[k * QC];
-              return make_double2(al[k] * y, (dz2 * b) * y);
-            },
-            [&](int k, double xp) { return make_double2(tC[k * QC], xp); });
-        if (kh2 == 0.0) {  // remove the vertical mean (src/elliptic.jl:930-945)
-          double mean = 0.0;
-          for (int k = 0; k < nz; ++k) mean = mean + b0[k * st];
-          mean = mean / nz;
-          for (int k = 0; k < nz; ++k) b0[k * st] = b0[k * st] - mean;
-        }
-      }
-    }
-  }
-  __syncthreads();
-
-  // write back (psi = 0 on the mean column), and C = dA/dz
-  if (!sl.q.active) return;
-  const bool zero_col = KIND == 0 && nz > 1 && sl.q.x == 0 && sl.q.y == 0;
-  for (int k = sl.kf; k < nz; k += MirrorSlot<QC>::kstep) {
-    const size_t at = (size_t)k * plane + sl.off;
-    const cplx<double> r = zero_col ? mk(0.0, 0.0) : xs[k * NC + sl.c];
-    out[at] = r;
-    if (KIND == 2) {
-      const cplx<double> up = xs[min(k + 1, kpad - 1) * NC + sl.c];
-      Cout[at] = k < nz - 1 ? mk((up.re - r.re) / g.dz, (up.im - r.im) / g.dz) : mk(0.0, 0.0);
-    }
-  }
-}
-
-template <int KIND, int QC>
-void launch_k(const Geo& g, hipStream_t st, const cplx<double>* rhs, cplx<double>* out, cplx<double>* Cout,
-              const double* inv, const double* cp, const double* al, unsigned long long* warn) {
-  const Quarter qr = quarter(g);
-  const int blocks = (qr.size() + QC - 1) / QC;
-  hipLaunchKernelGGL((k_thomas<KIND, QC>), dim3(blocks, g.members), dim3(kMirrorThreads),
-                     mirror_lds_bytes(mirror_pad(g.nzc), QC), st, g, qr, rhs, out, Cout, inv, cp, al, warn);
-}
-
-template <int QC>
-void launch_qc(const Geo& g, hipStream_t st, const cplx<double>* rhs, cplx<double>* out, cplx<double>* Cout,
-               const double* inv, const double* cp, const double* al, int kdiv, unsigned long long* warn) {
-  if (kdiv == 1) launch_k<0, QC>(g, st, rhs, out, nullptr, inv, cp, al, warn);
-  else if (Cout) launch_k<2, QC>(g, st, rhs, out, Cout, inv, cp, al, warn);
-  else launch_k<1, QC>(g, st, rhs, out, nullptr, inv, cp, al, warn);
-}
-
-template <int QC>
-bool set_lds_attr() {
-  const void* fns[] = {reinterpret_cast<const void*>(k_thomas<0, QC>), reinterpret_cast<const void*>(k_thomas<1, QC>),
-                       reinterpret_cast<const void*>(k_thomas<2, QC>)};
-  for (const void* f : fns)
-    if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMirrorLdsBytes) != hipSuccess)
-      return false;
-  return true;
-}
-
 }  // namespace
 
 size_t thomas_table_size(const Geo& g) { return (size_t)quarter(g).size() * g.nz; }
 
-int thomas_shifted_a_size(int nzc) { return mirror_pad(nzc); }
+int thomas_shifted_a_size(int nzc) { return (nzc + 15) / 16 * 16; }
 
-bool column_depth_supported(int nz) { return nz >= 1 && mirror_qc(nz) > 0; }
+bool column_depth_supported(int nz) { return nz >= 1 && nz <= 1024 && scan_default_shape(nz) >= 0; }
 
 void launch_thomas_factor(const Geo& g, hipStream_t st, const double* a, int kdiv, double* inv, double* cp,
                           int* err) {
@@ -205,16 +96,27 @@ void launch_thomas_factor(const Geo& g, hipStream_t st, const double* a, int kdi
                      inv, cp, err);
 }
 
-bool thomas_setup() { return set_lds_attr<8>() && set_lds_attr<4>() && set_lds_attr<2>() && set_lds_attr<1>(); }
+bool thomas_setup() { return colscan_setup(); }
 
 void launch_thomas(const Geo& g, hipStream_t st, const cplx<double>* rhs, cplx<double>* out, cplx<double>* Cout,
-                   const double* inv, const double* cp, const double* al, int kdiv, unsigned long long* warn) {
-  switch (mirror_qc(g.nzc)) {
-    case 8: launch_qc<8>(g, st, rhs, out, Cout, inv, cp, al, kdiv, warn); return;
-    case 4: launch_qc<4>(g, st, rhs, out, Cout, inv, cp, al, kdiv, warn); return;
-    case 2: launch_qc<2>(g, st, rhs, out, Cout, inv, cp, al, kdiv, warn); return;
-    case 1: launch_qc<1>(g, st, rhs, out, Cout, inv, cp, al, kdiv, warn); return;
-    default: return;  // rejected by column_depth_supported at model creation
+                   const double* inv, const double* cp, const double* al, int kdiv, unsigned long long* warn,
+                   int shape) {
+  if (shape < 0) shape = scan_default_shape(g.nzc);
+  ScanArgs a{};
+  a.rhs = rhs;
+  a.out = out;
+  a.Cout = Cout;
+  a.t0 = inv;
+  a.t1 = cp;
+  a.lv0 = al;
+  a.lv0_stride = thomas_shifted_a_size(g.nzc);
+  if (kdiv == 1) {
+    a.warn = warn;
+    launch_colscan<SCAN_PSI>(g, st, shape, a);
+  } else if (Cout) {
+    launch_colscan<SCAN_AC>(g, st, shape, a);
+  } else {
+    launch_colscan<SCAN_A>(g, st, shape, a);
   }
 }
 
