@@ -105,7 +105,7 @@ ENSEMBLE_CLASSES = ((64, 32), (128, 64), (256, 64))  # (nx = ny, nz), 16 members
 ENSEMBLE_MEMBERS = 16
 
 
-def ensemble_leg(qg, steps, warmup=2, seed=2026):
+def ensemble_leg(qg, steps, warmup=2, seed=2026, autotune=True):
     """The batched-ensemble workload: 48 members of uneven grids, 16 each of
     64²x32, 128²x64 and 256²x64 over 500 km x 500 km x 4000 m, N² = N0²
     exp(z/800) with N0² ~ U[5e-6, 5e-5], psi = the seeded k^-3 field plus 5 %
@@ -133,6 +133,8 @@ def ensemble_leg(qg, steps, warmup=2, seed=2026):
         points += n * n * nz * ENSEMBLE_MEMBERS
     ts = qg.ExponentialRungeKutta2(dt=DT)
     for ens in ens_list:
+        if autotune:
+            ens.autotune(DT, steps=2)
         qg.step_(ens, ts, warmup)
     per_class = []
     t0 = time.perf_counter()
@@ -158,7 +160,7 @@ def ensemble_leg(qg, steps, warmup=2, seed=2026):
 C3_SIZE = (512, 512, 256)
 
 
-def large_leg(qg, steps, kernel_steps=20, seed=2026):
+def large_leg(qg, steps, kernel_steps=20, seed=2026, autotune=True):
     """The single large run: 512x512x256 over 500 km x 500 km x 4000 m, the
     256²x128 case's stratification and seeded k^-3 psi, B = 32 random
     horizontal modes with a 30 m surface e-folding scale, feedback on, mixed
@@ -173,6 +175,7 @@ def large_leg(qg, steps, kernel_steps=20, seed=2026):
     qg.set_(m, psi=qg.FieldArray(synthetic.k3_streamfunction(grid, seed=seed, urms=0.1, kpeak=6.0)),
             waves=qg.FieldArray(synthetic.wave_modes(grid, 32, seed=seed, rms=0.1, efold=30.0), space="spectral"))
     ts = qg.ExponentialRungeKutta2(dt=10.0)
+    choice = m.autotune(10.0, steps=2) if autotune else None
     qg.step_(m, ts, 2)
     m._h.synchronize()
     t0 = time.perf_counter()
@@ -193,10 +196,11 @@ def large_leg(qg, steps, kernel_steps=20, seed=2026):
                         "B = 32 random modes x exp(-depth/30 m), feedback on, YBJ+, mixed precision, dt=10 s",
             "steps": steps, "value": npts * steps / el / 1e6, "unit": "Mgridpt-steps/s",
             "ms_per_step": 1e3 * el / steps, "kernel_profile_steps": kernel_steps, "kernels": kern,
+            "autotune": choice,
             "nonfinite": bool(bad), "max_abs_psi": mx}
 
 
-def serving_leg(qg, n_requests, capacity_mpts, seed=7, utilization=0.8):
+def serving_leg(qg, n_requests, capacity_mpts, seed=7, utilization=0.8, autotune=True):
     """The serving workload: Poisson arrivals of QG-YBJ+ runs with grids
     uniform over {64,128,256,384}² x {32,64,128}, steps log-uniform over
     [100, 2000], seeded k^-3 psi, wave amplitude U[0.05, 0.2], constant or
@@ -207,7 +211,7 @@ def serving_leg(qg, n_requests, capacity_mpts, seed=7, utilization=0.8):
     from qgybjplus_amd import serving
     rate = utilization * capacity_mpts * 1e6 / serving.expected_work()
     trace = serving.poisson_trace(n_requests, rate=rate, seed=seed)
-    server = serving.Server(precision="mixed", slots=4, chunk=50)
+    server = serving.Server(precision="mixed", slots=4, chunk=50, autotune=autotune)
     _, summary = server.run(trace, realtime=True)
     summary.update({"workload": f"{n_requests} Poisson-arrival runs, grids {{64,128,256,384}}^2 x {{32,64,128}}, "
                                 "steps log-uniform [100, 2000], mixed precision, dt = 20 s",
@@ -253,6 +257,8 @@ def main():
                     help="steps of the 512x512x256 leg (0 skips it)")
     ap.add_argument("--serve-requests", type=int, default=32,
                     help="requests of the serving leg (0 skips it)")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="run every pass at its default tile variant / launch shape")
     ap.add_argument("--modes", choices=("both", "mixed"), default="both",
                     help="'mixed' times only the mixed-precision model (for profiler runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -290,11 +296,16 @@ def main():
     for k in range(nz):
         depth = max(0.0, -grid.z[k] - grid.dz / 2)
         B[k] = 0.1 * math.exp(-(depth ** 2) / 50.0 ** 2)
-    for m in (m32, m64):
+    tuning = {}
+    for name, m in (("mixed", m32), ("fp64", m64)):
         if m is None:
             continue
         qg.set_(m, psi=qg.FieldArray(psi), pv_method="qg")
         qg.set_(m, waves=qg.SurfaceWave(0.1, 50.0))
+        if args.autotune:
+            t0 = time.perf_counter()
+            tuning[name] = {"choice": m.autotune(DT, steps=3), "seconds": time.perf_counter() - t0}
+            log(f"autotuned the {name} model in {tuning[name]['seconds']:.2f} s")
 
     npts = SIZE[0] * SIZE[1] * SIZE[2]
     log("timing the mixed-precision model")
@@ -355,19 +366,22 @@ def main():
                      "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_s * 1e6,
                      "kernel_profile_steps": kernel_steps},
         "kernels_mixed": per_kernel,
+        "autotune": tuning or None,
     }
     m32.finalize()
     if m64 is not None:
         m64.finalize()
     if args.large_steps > 0:
         log("512x512x256 leg")
-    out["large"] = large_leg(qg, args.large_steps) if args.large_steps > 0 else None
+    out["large"] = large_leg(qg, args.large_steps, autotune=args.autotune) if args.large_steps > 0 else None
     if args.ensemble_steps > 0:
         log("batched-ensemble leg")
-    out["ensemble"] = ensemble_leg(qg, args.ensemble_steps) if args.ensemble_steps > 0 else None
+    out["ensemble"] = (ensemble_leg(qg, args.ensemble_steps, autotune=args.autotune)
+                       if args.ensemble_steps > 0 else None)
     if args.serve_requests > 0:
         log(f"serving leg ({args.serve_requests} requests)")
-    out["serving"] = serving_leg(qg, args.serve_requests, v32) if args.serve_requests > 0 else None
+    out["serving"] = (serving_leg(qg, args.serve_requests, v32, autotune=args.autotune)
+                      if args.serve_requests > 0 else None)
     if not args.no_cpu:
         log("CPU baseline (numpy oracle, one core)")
         out["cpu_baseline"] = cpu_baseline(psi, B, args.cpu_budget)

@@ -184,6 +184,26 @@ int qg_use_graphs(qg_handle* h, int enable);
 /* Waits for the handle's stream. */
 int qg_synchronize(qg_handle* h);
 
+/* Autotuner (no reference counterpart: FFTW plans are fixed by PencilFFTs,
+ * src/parallel_mpi.jl:676-720, and the Thomas loops have no launch shape).
+ * The tunable passes are the 13 fused FFT passes (tile variant 0, 1 or 2:
+ * half, base or twice the values per block; every variant gives bitwise the
+ * same result) and the 4 column operators q->psi, B->A, B->A with C, omega
+ * (launch shape: partition depth and tile width; shapes differ in the
+ * association of the partition carries, a few ulp).  qg_autotune times
+ * `steps` eager steps of size dt per candidate on the model's own state,
+ * restores the state, and keeps the fastest choice per pass; the choice is
+ * cached per grid class (nx, ny, nz per member, precision) in the process and
+ * reused by later calls unless flags has QG_TUNE_FORCE.  -1 is the default
+ * choice of a pass.  choices[] has QG_TUNE_PASSES entries in the order
+ * ya_qw, xa_qw, ya2_qeff, yb_adv, x_c2r, xb_q, xb_B, yb2_q, yb2_B, yf_omega,
+ * xf, y_r2c, y_herm, thomas_psi, thomas_A, thomas_AC, omega. */
+#define QG_TUNE_PASSES 17
+#define QG_TUNE_FORCE 1
+int qg_autotune(qg_handle* h, double dt, int steps, int flags);
+int qg_get_tuning(qg_handle* h, int* choices, int n);
+int qg_set_tuning(qg_handle* h, const int* choices, int n);
+
 /* Measurement hooks (no reference counterpart: the reference has no
  * profiler, docs/src/advanced/performance.md:47-51).  mode bit 1 records HIP
  * events around every kernel launch on the handle's stream, bit 2 one event

@@ -38,7 +38,7 @@ EXPORTED = (
     "qg_set_field_member", "qg_get_field_member", "qg_check_member",
     "qg_compute_q_from_psi_member", "qg_invert_B_to_A_member", "qg_use_graphs",
     "qg_set_member_stratification", "qg_memory_required", "qg_device_memory",
-    "qg_set_active_members", "qg_move_member",
+    "qg_set_active_members", "qg_move_member", "qg_autotune", "qg_get_tuning", "qg_set_tuning",
 )
 
 
@@ -58,6 +58,13 @@ class QgConfig(C.Structure):
         ("precision", C.c_int),
         ("members", C.c_int),
     ]
+
+
+# tunable passes, in the order of qg_get_tuning / qg_set_tuning
+TUNE_PASS_NAMES = ("ya_qw", "xa_qw", "ya2_qeff", "yb_adv", "x_c2r", "xb_q", "xb_B", "yb2_q", "yb2_B",
+                   "yf_omega", "xf", "y_r2c", "y_herm", "thomas_psi", "thomas_A", "thomas_AC", "omega")
+TUNE_PASSES = len(TUNE_PASS_NAMES)  # QG_TUNE_PASSES
+TUNE_FORCE = 1                      # QG_TUNE_FORCE
 
 
 class QgError(RuntimeError):
@@ -112,6 +119,9 @@ def load() -> C.CDLL:
         "qg_take_mean_q_warning": (C.c_int, [P, dp]),
         "qg_profile_begin": (C.c_int, [P, C.c_int]),
         "qg_profile_end": (C.c_int, [P, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+        "qg_autotune": (C.c_int, [P, C.c_double, C.c_int, C.c_int]),
+        "qg_get_tuning": (C.c_int, [P, C.POINTER(C.c_int), C.c_int]),
+        "qg_set_tuning": (C.c_int, [P, C.POINTER(C.c_int), C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -314,6 +324,22 @@ class Handle:
             raise RuntimeError("profile summary larger than the buffer")
         self._check(rc)
         return json.loads(buf.value.decode())
+
+    # -- autotuner ---------------------------------------------------------------
+    def autotune(self, dt: float, steps: int = 3, force: bool = False) -> dict:
+        """Time the tile variants / launch shapes of every tunable pass on this
+        model's state and keep the fastest (cached per grid class)."""
+        self._check(self._lib.qg_autotune(self.h, float(dt), int(steps), TUNE_FORCE if force else 0))
+        return self.tuning()
+
+    def tuning(self) -> dict:
+        t = np.zeros(TUNE_PASSES, dtype=np.int32)
+        self._check(self._lib.qg_get_tuning(self.h, _iptr(t), TUNE_PASSES))
+        return dict(zip(TUNE_PASS_NAMES, (int(v) for v in t)))
+
+    def set_tuning(self, choices: dict):
+        t = np.array([int(choices.get(n, -1)) for n in TUNE_PASS_NAMES], dtype=np.int32)
+        self._check(self._lib.qg_set_tuning(self.h, _iptr(t), TUNE_PASSES))
 
     def take_mean_q_warning(self) -> float:
         m = C.c_double(0.0)

@@ -442,6 +442,20 @@ class QGYBJModel:
     def compute_qw(self) -> np.ndarray:
         return self._h.compute_qw()
 
+    def autotune(self, dt: float, steps: int = 3, force: bool = False) -> dict:
+        """Choose the FFT-pass tile variants and column-solve launch shapes by
+        timing `steps` steps of size `dt` per candidate on this model's state
+        (which is restored).  The choice is cached per grid class in the
+        process; `force` re-times.  Returns the choice per pass."""
+        return self._h.autotune(dt, steps, force)
+
+    def tuning(self) -> dict:
+        return self._h.tuning()
+
+    def set_tuning(self, choices: dict):
+        self._h.set_tuning(choices)
+        return self
+
     def finalize(self):
         if not self.finalized:
             self._h.close()

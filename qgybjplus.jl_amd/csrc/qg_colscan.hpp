@@ -93,13 +93,15 @@ inline ScanShape scan_shape_of(int idx, int nz) {
   return s;
 }
 
-// Default shape for depth nz: the smallest compiled M with at most 16
-// partitions among the 8-quarter-column tiles (64 column parts, one wave per
-// partition), else the deepest 4-quarter-column shape that fits.
+// Default shape for depth nz: among the 8-quarter-column tiles (64 column
+// parts, one wave per partition), M = 8 for nz <= 8, else the smallest of
+// M = 16, 32 that fits (at most 16 partitions); else the deepest
+// 4-quarter-column shape that fits.  Short partitions keep fewer registers
+// per thread and more columns in flight.
 inline int scan_default_shape(int nz) {
   for (int i = 0; i < kScanShapeCount; ++i) {
     ScanShape s = scan_shape_of(i, nz);
-    if (s.QC == 8 && scan_shape_fits(s) && (s.M >= nz || s.M == 32)) return i;
+    if (s.QC == 8 && scan_shape_fits(s) && (s.M >= 16 || nz <= 8)) return i;
   }
   for (int i = kScanShapeCount - 1; i >= 0; --i)
     if (scan_shape_fits(scan_shape_of(i, nz))) return i;

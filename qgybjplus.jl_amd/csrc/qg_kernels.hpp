@@ -17,14 +17,26 @@
 
 namespace qg {
 
-template <typename R, int N>
+// Tile of a pass: S sequences of length N per block, T lanes per sequence.
+// The tile variant V sets the values a block holds: half (V = 0), the base
+// of 256 FP64 / 512 FP32 values (V = 1) or twice that (V = 2), within 64
+// sequences, 512 threads (larger blocks cap a kernel at 128 VGPRs, which the
+// multi-field passes exceed) and 64 KB of LDS.  Every sequence's transform is
+// the same in every variant, so variants give bitwise identical results
+// (kTileVariants of them, qg_flags.hpp).
+
+template <typename R, int N, int V = 1>
 struct Tile {
   static constexpr int T = FftPlan<N>::T;
-  static constexpr int TARGET = sizeof(R) == 8 ? 256 : 512;
-  static constexpr int S0 = TARGET / T;
-  static constexpr int S = S0 < 1 ? 1 : (S0 > 64 ? 64 : S0);
-  static constexpr int THREADS = S * T;
   static constexpr int LD = N + N / FftPlan<N>::r0 + 1;  // padded sequence stride in LDS
+  static constexpr int BASE = sizeof(R) == 8 ? 256 : 512;
+  static constexpr int TARGET = V == 0 ? BASE / 2 : (V == 1 ? BASE : 2 * BASE);
+  static constexpr int S0 = TARGET / T;
+  static constexpr int S1 = S0 < 1 ? 1 : (S0 > 64 ? 64 : S0);
+  static constexpr int S2 = S1 * T > 512 ? (512 / T < 1 ? 1 : 512 / T) : S1;
+  static constexpr int SL = (int)(65536 / (LD * sizeof(cplx<R>)));
+  static constexpr int S = S2 > SL ? (SL < 1 ? 1 : SL) : S2;
+  static constexpr int THREADS = S * T;
   static constexpr int NN = N;
 };
 
@@ -41,11 +53,11 @@ __device__ inline cplx<double> herm(cplx<double> a, cplx<double> am, double s) {
 // ===========================================================================
 
 // Y-pass: Bm = mask*B (scaled 1/N) -> T1 = IFFTy(Bm), T3 = IFFTy(i*ky*Bm).
-template <typename R, int NY>
-__global__ void __launch_bounds__((Tile<R, NY>::THREADS))
+template <typename R, int NY, int V>
+__global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
 k_ya_qw(Geo g, const cplx<double>* __restrict__ B, cplx<R>* __restrict__ T1,
         cplx<R>* __restrict__ T3, const cplx<R>* __restrict__ tw) {
-  using TL = Tile<R, NY>;
+  using TL = Tile<R, NY, V>;
   using F = Fft<R, NY, +1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
@@ -87,11 +99,11 @@ k_ya_qw(Geo g, const cplx<double>* __restrict__ B, cplx<R>* __restrict__ T1,
 // X-pass: B = IFFTx(T1), Bx = IFFTx(i*kx*T1), By = IFFTx(T3);
 // qwr = Re(i/2 (conj(Bx)By - conj(By)Bx)) = Bx.im*By.re - Bx.re*By.im,
 // |B|^2; forward real FFTs along x -> Qa (qwr), Qb (|B|^2), half width.
-template <typename R, int NX>
-__global__ void __launch_bounds__((Tile<R, NX>::THREADS))
+template <typename R, int NX, int V>
+__global__ void __launch_bounds__((Tile<R, NX, V>::THREADS))
 k_xa_qw(Geo g, const cplx<R>* __restrict__ T1, const cplx<R>* __restrict__ T3,
         cplx<R>* __restrict__ Qa, cplx<R>* __restrict__ Qb, const cplx<R>* __restrict__ tw) {
-  using TL = Tile<R, NX>;
+  using TL = Tile<R, NX, V>;
   using FI = Fft<R, NX, +1>;
   using FF = Fft<R, NX, -1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
@@ -153,12 +165,12 @@ k_xa_qw(Geo g, const cplx<R>* __restrict__ T1, const cplx<R>* __restrict__ T3,
 
 // Y-pass (forward, half width): q_w = mask*(F(qwr) - kh2/4 F(|B|^2))/f and
 // q_eff = mask ? q - q_w : 0  (replace_q_with_wave_feedback_rhs!, timestep.jl:122-140).
-template <typename R, int NY>
-__global__ void __launch_bounds__((Tile<R, NY>::THREADS))
+template <typename R, int NY, int V>
+__global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
 k_ya2_qeff(Geo g, const cplx<R>* __restrict__ Qa, const cplx<R>* __restrict__ Qb,
            const cplx<double>* __restrict__ q, cplx<double>* __restrict__ qeff,
            cplx<double>* __restrict__ qw_out, double inv_f, const cplx<R>* __restrict__ tw) {
-  using TL = Tile<R, NY>;
+  using TL = Tile<R, NY, V>;
   using F = Fft<R, NY, -1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
@@ -211,12 +223,12 @@ k_ya2_qeff(Geo g, const cplx<R>* __restrict__ Qa, const cplx<R>* __restrict__ Qb
 // Y-pass (inverse, half width) producing the y-transformed Hermitian parts of
 // u = Re ifft(-i ky psi), v = Re ifft(i kx psi) (unmasked psi), of
 // Re ifft(mask q) and of zeta = Re ifft(mask (-kh2 psi)).
-template <typename R, int NY>
-__global__ void __launch_bounds__((Tile<R, NY>::THREADS))
+template <typename R, int NY, int V>
+__global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
 k_yb_adv(Geo g, int outs, const cplx<double>* __restrict__ psi, const cplx<double>* __restrict__ q,
          cplx<R>* __restrict__ Pu, cplx<R>* __restrict__ Pv, cplx<R>* __restrict__ Pq,
          cplx<R>* __restrict__ Pz, const cplx<R>* __restrict__ tw) {
-  using TL = Tile<R, NY>;
+  using TL = Tile<R, NY, V>;
   using F = Fft<R, NY, +1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
@@ -299,11 +311,11 @@ __device__ inline void load_herm_row(cplx<R> (&v)[F::IT][F::R0], const cplx<R>* 
 
 // X-pass: real rows from half-width intermediates (C2R), written to `outs`.
 // Used for u, v (velocities) and w.
-template <typename R, int NX>
-__global__ void __launch_bounds__((Tile<R, NX>::THREADS))
+template <typename R, int NX, int V>
+__global__ void __launch_bounds__((Tile<R, NX, V>::THREADS))
 k_x_c2r(Geo g, int nfields, const cplx<R>* __restrict__ P0, const cplx<R>* __restrict__ P1,
         R* __restrict__ o0, R* __restrict__ o1, const cplx<R>* __restrict__ tw) {
-  using TL = Tile<R, NX>;
+  using TL = Tile<R, NX, V>;
   using FI = Fft<R, NX, +1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
@@ -329,11 +341,11 @@ k_x_c2r(Geo g, int nfields, const cplx<R>* __restrict__ P0, const cplx<R>* __res
 }
 
 // X-pass for q advection: q = C2R(Pq); u*q, v*q -> forward real FFTs -> G1, G2.
-template <typename R, int NX>
-__global__ void __launch_bounds__((Tile<R, NX>::THREADS))
+template <typename R, int NX, int V>
+__global__ void __launch_bounds__((Tile<R, NX, V>::THREADS))
 k_xb_q(Geo g, const cplx<R>* __restrict__ Pq, const R* __restrict__ u, const R* __restrict__ vv,
        cplx<R>* __restrict__ G1, cplx<R>* __restrict__ G2, const cplx<R>* __restrict__ tw) {
-  using TL = Tile<R, NX>;
+  using TL = Tile<R, NX, V>;
   using FI = Fft<R, NX, +1>;
   using FF = Fft<R, NX, -1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
@@ -376,12 +388,12 @@ k_xb_q(Geo g, const cplx<R>* __restrict__ Pq, const R* __restrict__ u, const R* 
 
 // X-pass for B: B = IFFTx(T1), zeta = C2R(Pz); u*B, v*B, zeta*B -> forward
 // complex FFTs -> G3, G4, G5 (full width).
-template <typename R, int NX>
-__global__ void __launch_bounds__((Tile<R, NX>::THREADS))
+template <typename R, int NX, int V>
+__global__ void __launch_bounds__((Tile<R, NX, V>::THREADS))
 k_xb_B(Geo g, int flags, const cplx<R>* __restrict__ T1, const cplx<R>* __restrict__ Pz,
        const R* __restrict__ u, const R* __restrict__ vv, cplx<R>* __restrict__ G3,
        cplx<R>* __restrict__ G4, cplx<R>* __restrict__ G5, const cplx<R>* __restrict__ tw) {
-  using TL = Tile<R, NX>;
+  using TL = Tile<R, NX, V>;
   using FI = Fft<R, NX, +1>;
   using FF = Fft<R, NX, -1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
@@ -465,14 +477,14 @@ __device__ inline cplx<double> vdiff(const cplx<double>* __restrict__ q, int z, 
 
 // Y-pass (forward, half width): nq = mask*(i kx F(uq) + i ky F(vq)); rhs_q and
 // the ETD stage/final update of q (timestep.jl:286-295, 362-404).
-template <typename R, int NY>
-__global__ void __launch_bounds__((Tile<R, NY>::THREADS))
+template <typename R, int NY, int V>
+__global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
 k_yb2_q(Geo g, int mode, int flags, const cplx<R>* __restrict__ G1, const cplx<R>* __restrict__ G2,
         const cplx<double>* __restrict__ qd, const cplx<double>* __restrict__ qn,
         cplx<double>* __restrict__ rhs0,
         cplx<double>* __restrict__ qout, Etd etd, double nu, double inv_dz2,
         const cplx<R>* __restrict__ tw) {
-  using TL = Tile<R, NY>;
+  using TL = Tile<R, NY, V>;
   using F = Fft<R, NY, -1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
@@ -544,13 +556,13 @@ k_yb2_q(Geo g, int mode, int flags, const cplx<R>* __restrict__ G1, const cplx<R
 
 // Y-pass (forward, full width): nB = mask*(i kx F(uB) + i ky F(vB)), rB =
 // mask*F(zeta B); rhs_B = -nB + i (f/2) kh2 A - (i/2) rB; ETD update of B.
-template <typename R, int NY>
-__global__ void __launch_bounds__((Tile<R, NY>::THREADS))
+template <typename R, int NY, int V>
+__global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
 k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R>* __restrict__ G4,
         const cplx<R>* __restrict__ G5, const cplx<double>* __restrict__ A,
         const cplx<double>* __restrict__ Bn, cplx<double>* __restrict__ rhs0,
         cplx<double>* __restrict__ Bout, Etd etd, double alpha, const cplx<R>* __restrict__ tw) {
-  using TL = Tile<R, NY>;
+  using TL = Tile<R, NY, V>;
   using F = Fft<R, NY, -1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
@@ -642,11 +654,11 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
 // the two-level average psi-bar (psi itself at the top) times -i*kx*kh2 and
 // -i*ky*kh2.  psi(z) and psi(z+1) are read once; the four Hermitian inputs are
 // formed in registers before the four transforms.
-template <typename R, int NY>
-__global__ void __launch_bounds__((Tile<R, NY>::THREADS))
+template <typename R, int NY, int V>
+__global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
 k_yf_omega(Geo g, const cplx<double>* __restrict__ psi, cplx<R>* __restrict__ Pbx, cplx<R>* __restrict__ Pby,
            cplx<R>* __restrict__ Pxx, cplx<R>* __restrict__ Pxy, const cplx<R>* __restrict__ tw) {
-  using TL = Tile<R, NY>;
+  using TL = Tile<R, NY, V>;
   using F = Fft<R, NY, +1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
@@ -700,13 +712,13 @@ k_yf_omega(Geo g, const cplx<double>* __restrict__ psi, cplx<R>* __restrict__ Pb
 }
 
 // X-pass: u, v out; omega RHS 2(bx*xy - by*xx) -> forward real FFT -> Rw (half).
-template <typename R, int NX>
-__global__ void __launch_bounds__((Tile<R, NX>::THREADS))
+template <typename R, int NX, int V>
+__global__ void __launch_bounds__((Tile<R, NX, V>::THREADS))
 k_xf(Geo g, int want_omega, const cplx<R>* __restrict__ Pu, const cplx<R>* __restrict__ Pv,
      const cplx<R>* __restrict__ Pbx, const cplx<R>* __restrict__ Pby, const cplx<R>* __restrict__ Pxx,
      const cplx<R>* __restrict__ Pxy, R* __restrict__ u, R* __restrict__ vv, cplx<R>* __restrict__ Rw,
      const cplx<R>* __restrict__ tw) {
-  using TL = Tile<R, NX>;
+  using TL = Tile<R, NX, V>;
   using FI = Fft<R, NX, +1>;
   using FF = Fft<R, NX, -1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
@@ -768,11 +780,11 @@ k_xf(Geo g, int want_omega, const cplx<R>* __restrict__ Pu, const cplx<R>* __res
 
 // Y-pass (forward, half width) of a real field, masked, expanded to a full
 // Hermitian spectrum in double precision.
-template <typename R, int NY>
-__global__ void __launch_bounds__((Tile<R, NY>::THREADS))
+template <typename R, int NY, int V>
+__global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
 k_y_r2c_mask(Geo g, const cplx<R>* __restrict__ Rw, cplx<double>* __restrict__ out,
              const cplx<R>* __restrict__ tw) {
-  using TL = Tile<R, NY>;
+  using TL = Tile<R, NY, V>;
   using F = Fft<R, NY, -1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
@@ -805,11 +817,11 @@ k_y_r2c_mask(Geo g, const cplx<R>* __restrict__ Rw, cplx<double>* __restrict__ o
 }
 
 // Y-pass (inverse, half width): Hermitian part of a double spectrum.
-template <typename R, int NY>
-__global__ void __launch_bounds__((Tile<R, NY>::THREADS))
+template <typename R, int NY, int V>
+__global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
 k_y_herm(Geo g, const cplx<double>* __restrict__ in, cplx<R>* __restrict__ P,
          const cplx<R>* __restrict__ tw) {
-  using TL = Tile<R, NY>;
+  using TL = Tile<R, NY, V>;
   using F = Fft<R, NY, +1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
@@ -847,11 +859,11 @@ k_y_herm(Geo g, const cplx<double>* __restrict__ in, cplx<R>* __restrict__ P,
 // ===========================================================================
 
 // Y-pass C2C of full double fields; inverse applies 1/(nx*ny).
-template <typename R, int NY, int DIR>
-__global__ void __launch_bounds__((Tile<R, NY>::THREADS))
+template <typename R, int NY, int DIR, int V>
+__global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
 k_y_c2c(Geo g, const cplx<double>* __restrict__ in, cplx<double>* __restrict__ out,
         const cplx<R>* __restrict__ tw) {
-  using TL = Tile<R, NY>;
+  using TL = Tile<R, NY, V>;
   using F = Fft<R, NY, DIR>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
@@ -882,11 +894,11 @@ k_y_c2c(Geo g, const cplx<double>* __restrict__ in, cplx<double>* __restrict__ o
 
 // X-pass C2C of full double fields.  If `absmax` is set, also folds max|out|
 // into it (non-negative doubles compare like their bit patterns).
-template <typename R, int NX, int DIR>
-__global__ void __launch_bounds__((Tile<R, NX>::THREADS))
+template <typename R, int NX, int DIR, int V>
+__global__ void __launch_bounds__((Tile<R, NX, V>::THREADS))
 k_x_c2c(Geo g, const cplx<double>* __restrict__ in, cplx<double>* __restrict__ out,
         unsigned long long* __restrict__ absmax, const cplx<R>* __restrict__ tw) {
-  using TL = Tile<R, NX>;
+  using TL = Tile<R, NX, V>;
   using F = Fft<R, NX, DIR>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;

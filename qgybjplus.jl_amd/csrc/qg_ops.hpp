@@ -1,5 +1,6 @@
-// Host launchers of the fused FFT-pass kernels, one instantiation per
-// intermediate precision R (double: plain FP64 mode, float: mixed mode).
+// Host launchers of the fused FFT-pass kernels: OpsV<R, V> launches tile
+// variant V at intermediate precision R (double: plain FP64 mode, float:
+// mixed mode); Ops<R> picks the variant at run time.
 #pragma once
 
 #include "qg_common.hpp"
@@ -14,8 +15,8 @@ struct EtdView {
 
 bool fft_size_supported(int n);
 
-template <typename R>
-struct Ops {
+template <typename R, int V>
+struct OpsV {
   using C = cplx<R>;
   using D = cplx<double>;
   static void ya_qw(const Geo& g, hipStream_t st, const D* B, C* T1, C* T3, const C* twy);
@@ -44,6 +45,37 @@ struct Ops {
   static void y_c2c(const Geo& g, hipStream_t st, int dir, const D* in, D* out, const C* twy);
   static void x_c2c(const Geo& g, hipStream_t st, int dir, const D* in, D* out, unsigned long long* absmax,
                     const C* twx);
+};
+
+// Run-time choice of the tile variant: Ops<R>::name(v, args...) calls
+// OpsV<R, v>::name(args...) (v outside [0, 3) selects the base variant 1).
+template <typename R>
+struct Ops {
+#define QG_OPS_FWD(name)                                        \
+  template <class... A>                                         \
+  static void name(int v, A&&... a) {                           \
+    switch (v) {                                                \
+      case 0: OpsV<R, 0>::name(static_cast<A&&>(a)...); break;  \
+      case 2: OpsV<R, 2>::name(static_cast<A&&>(a)...); break;  \
+      default: OpsV<R, 1>::name(static_cast<A&&>(a)...); break; \
+    }                                                           \
+  }
+  QG_OPS_FWD(ya_qw)
+  QG_OPS_FWD(xa_qw)
+  QG_OPS_FWD(ya2_qeff)
+  QG_OPS_FWD(yb_adv)
+  QG_OPS_FWD(x_c2r)
+  QG_OPS_FWD(xb_q)
+  QG_OPS_FWD(xb_B)
+  QG_OPS_FWD(yb2_q)
+  QG_OPS_FWD(yb2_B)
+  QG_OPS_FWD(yf_omega)
+  QG_OPS_FWD(xf)
+  QG_OPS_FWD(y_r2c_mask)
+  QG_OPS_FWD(y_herm)
+  QG_OPS_FWD(y_c2c)
+  QG_OPS_FWD(x_c2c)
+#undef QG_OPS_FWD
 };
 
 }  // namespace qg

@@ -1,4 +1,5 @@
-// FP64 instantiation of the fused FFT-pass kernels (plain mode).
+// FP64 instantiation of the fused FFT-pass kernels (plain mode),
+// tile variant 1.
 #include "qg_ops_impl.hpp"
 
 namespace qg {
@@ -8,5 +9,5 @@ bool fft_size_supported(int n) {
     default: return false;
   }
 }
-template struct Ops<double>;
+template struct OpsV<double, 1>;
 }  // namespace qg

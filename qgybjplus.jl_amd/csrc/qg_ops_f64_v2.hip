@@ -1,0 +1,7 @@
+// FP64 instantiation of the fused FFT-pass kernels (plain mode),
+// tile variant 2.
+#include "qg_ops_impl.hpp"
+
+namespace qg {
+template struct OpsV<double, 2>;
+}  // namespace qg

@@ -1,5 +1,6 @@
-// Definitions of Ops<R>: runtime dispatch on the transform length to the
-// compile-time kernel instantiations.  Included once per precision.
+// Definitions of OpsV<R, V>: runtime dispatch on the transform length to the
+// compile-time kernel instantiations of tile variant V.  Included once per
+// (precision, variant) translation unit.
 #pragma once
 
 #include <stdexcept>
@@ -29,140 +30,140 @@ static inline dim3 xgrid(int rows, int S) { return dim3((rows + S - 1) / S); }
 
 #define YL(K, N, ncols, ...)                                                                 \
   do {                                                                                       \
-    using TL = Tile<R, N>;                                                                   \
+    using TL = Tile<R, N, V>;                                                                   \
     hipLaunchKernelGGL(K, dim3(((ncols) + TL::S - 1) / TL::S, g.nz), dim3(TL::THREADS), 0, st, \
                        __VA_ARGS__);                                                         \
   } while (0)
 #define XL(K, N, ...)                                                                        \
   do {                                                                                       \
-    using TL = Tile<R, N>;                                                                   \
+    using TL = Tile<R, N, V>;                                                                   \
     hipLaunchKernelGGL(K, xgrid(g.nz * g.ny, TL::S), dim3(TL::THREADS), 0, st, __VA_ARGS__); \
   } while (0)
 
-template <typename R>
-void Ops<R>::ya_qw(const Geo& g, hipStream_t st, const D* B, C* T1, C* T3, const C* twy) {
-#define CALL(N) YL((k_ya_qw<R, N>), N, g.nx, g, B, T1, T3, twy)
+template <typename R, int V>
+void OpsV<R, V>::ya_qw(const Geo& g, hipStream_t st, const D* B, C* T1, C* T3, const C* twy) {
+#define CALL(N) YL((k_ya_qw<R, N, V>), N, g.nx, g, B, T1, T3, twy)
   QG_SWITCH(g.ny, CALL)
 #undef CALL
 }
 
-template <typename R>
-void Ops<R>::xa_qw(const Geo& g, hipStream_t st, const C* T1, const C* T3, C* Qa, C* Qb, const C* twx) {
-#define CALL(N) XL((k_xa_qw<R, N>), N, g, T1, T3, Qa, Qb, twx)
+template <typename R, int V>
+void OpsV<R, V>::xa_qw(const Geo& g, hipStream_t st, const C* T1, const C* T3, C* Qa, C* Qb, const C* twx) {
+#define CALL(N) XL((k_xa_qw<R, N, V>), N, g, T1, T3, Qa, Qb, twx)
   QG_SWITCH(g.nx, CALL)
 #undef CALL
 }
 
-template <typename R>
-void Ops<R>::ya2_qeff(const Geo& g, hipStream_t st, const C* Qa, const C* Qb, const D* q, D* qeff, D* qw,
+template <typename R, int V>
+void OpsV<R, V>::ya2_qeff(const Geo& g, hipStream_t st, const C* Qa, const C* Qb, const D* q, D* qeff, D* qw,
                       double inv_f, const C* twy) {
-#define CALL(N) YL((k_ya2_qeff<R, N>), N, g.W, g, Qa, Qb, q, qeff, qw, inv_f, twy)
+#define CALL(N) YL((k_ya2_qeff<R, N, V>), N, g.W, g, Qa, Qb, q, qeff, qw, inv_f, twy)
   QG_SWITCH(g.ny, CALL)
 #undef CALL
 }
 
-template <typename R>
-void Ops<R>::yb_adv(const Geo& g, hipStream_t st, int outs, const D* psi, const D* q, C* Pu, C* Pv, C* Pq,
+template <typename R, int V>
+void OpsV<R, V>::yb_adv(const Geo& g, hipStream_t st, int outs, const D* psi, const D* q, C* Pu, C* Pv, C* Pq,
                     C* Pz, const C* twy) {
-#define CALL(N) YL((k_yb_adv<R, N>), N, g.W, g, outs, psi, q, Pu, Pv, Pq, Pz, twy)
+#define CALL(N) YL((k_yb_adv<R, N, V>), N, g.W, g, outs, psi, q, Pu, Pv, Pq, Pz, twy)
   QG_SWITCH(g.ny, CALL)
 #undef CALL
 }
 
-template <typename R>
-void Ops<R>::x_c2r(const Geo& g, hipStream_t st, int nfields, const C* P0, const C* P1, R* o0, R* o1,
+template <typename R, int V>
+void OpsV<R, V>::x_c2r(const Geo& g, hipStream_t st, int nfields, const C* P0, const C* P1, R* o0, R* o1,
                    const C* twx) {
-#define CALL(N) XL((k_x_c2r<R, N>), N, g, nfields, P0, P1, o0, o1, twx)
+#define CALL(N) XL((k_x_c2r<R, N, V>), N, g, nfields, P0, P1, o0, o1, twx)
   QG_SWITCH(g.nx, CALL)
 #undef CALL
 }
 
-template <typename R>
-void Ops<R>::xb_q(const Geo& g, hipStream_t st, const C* Pq, const R* u, const R* v, C* G1, C* G2,
+template <typename R, int V>
+void OpsV<R, V>::xb_q(const Geo& g, hipStream_t st, const C* Pq, const R* u, const R* v, C* G1, C* G2,
                   const C* twx) {
-#define CALL(N) XL((k_xb_q<R, N>), N, g, Pq, u, v, G1, G2, twx)
+#define CALL(N) XL((k_xb_q<R, N, V>), N, g, Pq, u, v, G1, G2, twx)
   QG_SWITCH(g.nx, CALL)
 #undef CALL
 }
 
-template <typename R>
-void Ops<R>::xb_B(const Geo& g, hipStream_t st, int flags, const C* T1, const C* Pz, const R* u, const R* v,
+template <typename R, int V>
+void OpsV<R, V>::xb_B(const Geo& g, hipStream_t st, int flags, const C* T1, const C* Pz, const R* u, const R* v,
                   C* G3, C* G4, C* G5, const C* twx) {
-#define CALL(N) XL((k_xb_B<R, N>), N, g, flags, T1, Pz, u, v, G3, G4, G5, twx)
+#define CALL(N) XL((k_xb_B<R, N, V>), N, g, flags, T1, Pz, u, v, G3, G4, G5, twx)
   QG_SWITCH(g.nx, CALL)
 #undef CALL
 }
 
-template <typename R>
-void Ops<R>::yb2_q(const Geo& g, hipStream_t st, int mode, int flags, const C* G1, const C* G2, const D* qd,
+template <typename R, int V>
+void OpsV<R, V>::yb2_q(const Geo& g, hipStream_t st, int mode, int flags, const C* G1, const C* G2, const D* qd,
                    const D* qn,
                    D* rhs0, D* qout, EtdView etd, double nu, double inv_dz2, const C* twy) {
   Etd e{etd.tab, etd.plane};
-#define CALL(N) YL((k_yb2_q<R, N>), N, g.W, g, mode, flags, G1, G2, qd, qn, rhs0, qout, e, nu, inv_dz2, twy)
+#define CALL(N) YL((k_yb2_q<R, N, V>), N, g.W, g, mode, flags, G1, G2, qd, qn, rhs0, qout, e, nu, inv_dz2, twy)
   QG_SWITCH(g.ny, CALL)
 #undef CALL
 }
 
-template <typename R>
-void Ops<R>::yb2_B(const Geo& g, hipStream_t st, int mode, int flags, const C* G3, const C* G4, const C* G5,
+template <typename R, int V>
+void OpsV<R, V>::yb2_B(const Geo& g, hipStream_t st, int mode, int flags, const C* G3, const C* G4, const C* G5,
                    const D* A, const D* Bn, D* rhs0, D* Bout, EtdView etd, double alpha, const C* twy) {
   Etd e{etd.tab, etd.plane};
-#define CALL(N) YL((k_yb2_B<R, N>), N, g.nx, g, mode, flags, G3, G4, G5, A, Bn, rhs0, Bout, e, alpha, twy)
+#define CALL(N) YL((k_yb2_B<R, N, V>), N, g.nx, g, mode, flags, G3, G4, G5, A, Bn, rhs0, Bout, e, alpha, twy)
   QG_SWITCH(g.ny, CALL)
 #undef CALL
 }
 
-template <typename R>
-void Ops<R>::yf_omega(const Geo& g, hipStream_t st, const D* psi, C* Pbx, C* Pby, C* Pxx, C* Pxy, const C* twy) {
-#define CALL(N) YL((k_yf_omega<R, N>), N, g.W, g, psi, Pbx, Pby, Pxx, Pxy, twy)
+template <typename R, int V>
+void OpsV<R, V>::yf_omega(const Geo& g, hipStream_t st, const D* psi, C* Pbx, C* Pby, C* Pxx, C* Pxy, const C* twy) {
+#define CALL(N) YL((k_yf_omega<R, N, V>), N, g.W, g, psi, Pbx, Pby, Pxx, Pxy, twy)
   QG_SWITCH(g.ny, CALL)
 #undef CALL
 }
 
-template <typename R>
-void Ops<R>::xf(const Geo& g, hipStream_t st, int want_omega, const C* Pu, const C* Pv, const C* Pbx,
+template <typename R, int V>
+void OpsV<R, V>::xf(const Geo& g, hipStream_t st, int want_omega, const C* Pu, const C* Pv, const C* Pbx,
                 const C* Pby, const C* Pxx, const C* Pxy, R* u, R* v, C* Rw, const C* twx) {
-#define CALL(N) XL((k_xf<R, N>), N, g, want_omega, Pu, Pv, Pbx, Pby, Pxx, Pxy, u, v, Rw, twx)
+#define CALL(N) XL((k_xf<R, N, V>), N, g, want_omega, Pu, Pv, Pbx, Pby, Pxx, Pxy, u, v, Rw, twx)
   QG_SWITCH(g.nx, CALL)
 #undef CALL
 }
 
-template <typename R>
-void Ops<R>::y_r2c_mask(const Geo& g, hipStream_t st, const C* Rw, D* out, const C* twy) {
-#define CALL(N) YL((k_y_r2c_mask<R, N>), N, g.W, g, Rw, out, twy)
+template <typename R, int V>
+void OpsV<R, V>::y_r2c_mask(const Geo& g, hipStream_t st, const C* Rw, D* out, const C* twy) {
+#define CALL(N) YL((k_y_r2c_mask<R, N, V>), N, g.W, g, Rw, out, twy)
   QG_SWITCH(g.ny, CALL)
 #undef CALL
 }
 
-template <typename R>
-void Ops<R>::y_herm(const Geo& g, hipStream_t st, const D* in, C* P, const C* twy) {
-#define CALL(N) YL((k_y_herm<R, N>), N, g.W, g, in, P, twy)
+template <typename R, int V>
+void OpsV<R, V>::y_herm(const Geo& g, hipStream_t st, const D* in, C* P, const C* twy) {
+#define CALL(N) YL((k_y_herm<R, N, V>), N, g.W, g, in, P, twy)
   QG_SWITCH(g.ny, CALL)
 #undef CALL
 }
 
-template <typename R>
-void Ops<R>::y_c2c(const Geo& g, hipStream_t st, int dir, const D* in, D* out, const C* twy) {
+template <typename R, int V>
+void OpsV<R, V>::y_c2c(const Geo& g, hipStream_t st, int dir, const D* in, D* out, const C* twy) {
   if (dir < 0) {
-#define CALL(N) YL((k_y_c2c<R, N, -1>), N, g.nx, g, in, out, twy)
+#define CALL(N) YL((k_y_c2c<R, N, -1, V>), N, g.nx, g, in, out, twy)
     QG_SWITCH(g.ny, CALL)
 #undef CALL
   } else {
-#define CALL(N) YL((k_y_c2c<R, N, +1>), N, g.nx, g, in, out, twy)
+#define CALL(N) YL((k_y_c2c<R, N, +1, V>), N, g.nx, g, in, out, twy)
     QG_SWITCH(g.ny, CALL)
 #undef CALL
   }
 }
 
-template <typename R>
-void Ops<R>::x_c2c(const Geo& g, hipStream_t st, int dir, const D* in, D* out, unsigned long long* absmax,
+template <typename R, int V>
+void OpsV<R, V>::x_c2c(const Geo& g, hipStream_t st, int dir, const D* in, D* out, unsigned long long* absmax,
                    const C* twx) {
   if (dir < 0) {
-#define CALL(N) XL((k_x_c2c<R, N, -1>), N, g, in, out, absmax, twx)
+#define CALL(N) XL((k_x_c2c<R, N, -1, V>), N, g, in, out, absmax, twx)
     QG_SWITCH(g.nx, CALL)
 #undef CALL
   } else {
-#define CALL(N) XL((k_x_c2c<R, N, +1>), N, g, in, out, absmax, twx)
+#define CALL(N) XL((k_x_c2c<R, N, +1, V>), N, g, in, out, absmax, twx)
     QG_SWITCH(g.nx, CALL)
 #undef CALL
   }

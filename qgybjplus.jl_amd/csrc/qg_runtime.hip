@@ -1,7 +1,12 @@
 // Host runtime of the QG-YBJ+ library: model handle, HBM buffers, the
 // ETD-RK2 step orchestration and the C ABI declared in include/qgybj.h.
 #include <algorithm>
+#include <array>
 #include <cmath>
+#include <limits>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
@@ -9,6 +14,7 @@
 #include <vector>
 
 #include "../../include/qgybj.h"
+#include "qg_colscan.hpp"
 #include "qg_columns.hpp"
 #include "qg_ops.hpp"
 
@@ -32,6 +38,16 @@ struct QgError : std::runtime_error {
   } while (0)
 
 }  // namespace
+
+namespace qg {
+// Tunable passes of the step: the fused FFT passes (tile variant 0..2) and
+// the column operators (launch shape index into kScanShapes).
+enum Pass : int {
+  P_YA_QW, P_XA_QW, P_YA2_QEFF, P_YB_ADV, P_X_C2R, P_XB_Q, P_XB_B, P_YB2_Q, P_YB2_B, P_YF_OMEGA, P_XF,
+  P_Y_R2C, P_Y_HERM, P_FFT_END,
+  P_THOMAS_PSI = P_FFT_END, P_THOMAS_A, P_THOMAS_AC, P_OMEGA, P_END
+};
+}  // namespace qg
 
 struct qg_handle {
   // nz counts all stacked levels: members * nzc (see qg::Geo)
@@ -87,11 +103,15 @@ struct qg_handle {
   // profiling: bit 1 = events around every kernel, bit 2 = one event per step
   struct ProfRec {
     const char* name;
+    int pass;
     double bytes;
     hipEvent_t a, b;
   };
+  // tuned choice per pass (-1: the default variant / shape)
+  int tune[qg::P_END];
   int prof_mode = 0;
   std::vector<ProfRec> prof;
+  qg_handle() { std::fill(tune, tune + qg::P_END, -1); }
   std::vector<hipEvent_t> step_events;
   std::vector<int> step_weights;  // steps covered by each interval of step_events
   std::vector<hipEvent_t> pool;
@@ -243,13 +263,15 @@ void check_status(qg_handle& h) {
 // invert_q_to_psi! (src/elliptic.jl:189-314) on device arrays.
 void solve_psi(qg_handle& h, const D* q, D* psi) {
   if (h.sing_psi[0]) raise_singular(h.sing_psi);
-  qg::launch_thomas(h.g, h.st, q, psi, nullptr, h.th_inv[0], h.th_cp[0], h.d_al, 1, h.d_warn);
+  qg::launch_thomas(h.g, h.st, q, psi, nullptr, h.th_inv[0], h.th_cp[0], h.d_al, 1, h.d_warn,
+                    h.tune[qg::P_THOMAS_PSI]);
 }
 
 // invert_B_to_A! (src/elliptic.jl:836-990) on device arrays; C optional.
 void solve_A(qg_handle& h, const D* B, D* A, D* C) {
   if (h.sing_A[0]) raise_singular(h.sing_A);
-  qg::launch_thomas(h.g, h.st, B, A, C, h.th_inv[1], h.th_cp[1], h.d_al, 4, nullptr);
+  qg::launch_thomas(h.g, h.st, B, A, C, h.th_inv[1], h.th_cp[1], h.d_al, 4, nullptr,
+                    h.tune[C ? qg::P_THOMAS_AC : qg::P_THOMAS_A]);
 }
 
 template <typename R>
@@ -273,20 +295,20 @@ struct Engine {
   qg::EtdView etd() const { return qg::EtdView{h.d_etd, h.plane}; }
 
   template <class F>
-  void K(const char* name, double bytes, F&& fn) {
+  void K(const char* name, int pass, double bytes, F&& fn) {
     if (h.prof_mode & 1) {
       hipEvent_t a = h.event(), b = h.event();
       HIPCHECK(hipEventRecord(a, h.st));
       fn();
       HIPCHECK(hipEventRecord(b, h.st));
-      h.prof.push_back({name, bytes, a, b});
+      h.prof.push_back({name, pass, bytes, a, b});
     } else {
       fn();
     }
   }
 
   void t1_only(const D* Bsrc) {
-    K("ya_T1", P * (cD + cR), [&] { O::ya_qw(h.g, h.st, Bsrc, c(h.T1), nullptr, twy()); });
+    K("ya_T1", qg::P_YA_QW, P * (cD + cR), [&] { O::ya_qw(h.tune[qg::P_YA_QW], h.g, h.st, Bsrc, c(h.T1), nullptr, twy()); });
   }
 
   // _diagnose_flow! (src/timestep.jl:192-215) minus the velocities: q_eff with
@@ -296,15 +318,15 @@ struct Engine {
     if (!h.fixed_flow) {
       const D* src = qsrc;
       if (h.feedback) {
-        K("ya_qw", P * (cD + 2 * cR), [&] { O::ya_qw(h.g, h.st, Bsrc, c(h.T1), c(h.T3), twy()); });
-        K("xa_qw", P * 2 * cR + Ph * 2 * cR,
-          [&] { O::xa_qw(h.g, h.st, c(h.T1), c(h.T3), c(h.Qa), c(h.Qb), twx()); });
-        K("ya2_qeff", Ph * 2 * cR + P * 2 * cD,
-          [&] { O::ya2_qeff(h.g, h.st, c(h.Qa), c(h.Qb), qsrc, h.qeff, nullptr, 1.0 / h.f, twy()); });
+        K("ya_qw", qg::P_YA_QW, P * (cD + 2 * cR), [&] { O::ya_qw(h.tune[qg::P_YA_QW], h.g, h.st, Bsrc, c(h.T1), c(h.T3), twy()); });
+        K("xa_qw", qg::P_XA_QW, P * 2 * cR + Ph * 2 * cR,
+          [&] { O::xa_qw(h.tune[qg::P_XA_QW], h.g, h.st, c(h.T1), c(h.T3), c(h.Qa), c(h.Qb), twx()); });
+        K("ya2_qeff", qg::P_YA2_QEFF, Ph * 2 * cR + P * 2 * cD,
+          [&] { O::ya2_qeff(h.tune[qg::P_YA2_QEFF], h.g, h.st, c(h.Qa), c(h.Qb), qsrc, h.qeff, nullptr, 1.0 / h.f, twy()); });
         src = h.qeff;
         t1 = true;
       }
-      K("thomas_psi", P * 2 * cD + Tq,
+      K("thomas_psi", qg::P_THOMAS_PSI, P * 2 * cD + Tq,
         [&] { solve_psi(h, src, psi_out); });
     }
     if (need_t1 && !t1) t1_only(Bsrc);
@@ -316,42 +338,42 @@ struct Engine {
       if (Cout) qg::launch_zero(Cout, h.plane * h.g.nz, h.st);
       return;
     }
-    K(Cout ? "thomas_AC" : "thomas_A", P * (Cout ? 3 : 2) * cD + Tq,
+    K(Cout ? "thomas_AC" : "thomas_A", Cout ? qg::P_THOMAS_AC : qg::P_THOMAS_A, P * (Cout ? 3 : 2) * cD + Tq,
       [&] { solve_A(h, Bsrc, Aout, Cout); });
   }
 
   void velocities(const D* psi) {
-    K("yb_uv", P * cD + Ph * 2 * cR, [&] {
-      O::yb_adv(h.g, h.st, qg::OUT_U | qg::OUT_V, psi, nullptr, c(h.Pu), c(h.Pv), nullptr, nullptr, twy());
+    K("yb_uv", qg::P_YB_ADV, P * cD + Ph * 2 * cR, [&] {
+      O::yb_adv(h.tune[qg::P_YB_ADV], h.g, h.st, qg::OUT_U | qg::OUT_V, psi, nullptr, c(h.Pu), c(h.Pv), nullptr, nullptr, twy());
     });
-    K("x_c2r_uv", Ph * 2 * cR + P * 2 * rR,
-      [&] { O::x_c2r(h.g, h.st, 2, c(h.Pu), c(h.Pv), r(h.u), r(h.v), twx()); });
+    K("x_c2r_uv", qg::P_X_C2R, Ph * 2 * cR + P * 2 * rR,
+      [&] { O::x_c2r(h.tune[qg::P_X_C2R], h.g, h.st, 2, c(h.Pu), c(h.Pv), r(h.u), r(h.v), twx()); });
   }
 
   // u, v and the QG omega-equation w (compute_velocities!(compute_w=true)).
   void velocities_w(const D* psi) {
-    K("yb_uv", P * cD + Ph * 2 * cR, [&] {
-      O::yb_adv(h.g, h.st, qg::OUT_U | qg::OUT_V, psi, nullptr, c(h.Pu), c(h.Pv), nullptr, nullptr, twy());
+    K("yb_uv", qg::P_YB_ADV, P * cD + Ph * 2 * cR, [&] {
+      O::yb_adv(h.tune[qg::P_YB_ADV], h.g, h.st, qg::OUT_U | qg::OUT_V, psi, nullptr, c(h.Pu), c(h.Pv), nullptr, nullptr, twy());
     });
-    K("yf_omega", P * cD + Ph * 4 * cR,
-      [&] { O::yf_omega(h.g, h.st, psi, c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), twy()); });
-    K("xf", Ph * 7 * cR + P * 2 * rR, [&] {
-      O::xf(h.g, h.st, 1, c(h.Pu), c(h.Pv), c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), r(h.u), r(h.v), c(h.Pq),
+    K("yf_omega", qg::P_YF_OMEGA, P * cD + Ph * 4 * cR,
+      [&] { O::yf_omega(h.tune[qg::P_YF_OMEGA], h.g, h.st, psi, c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), twy()); });
+    K("xf", qg::P_XF, Ph * 7 * cR + P * 2 * rR, [&] {
+      O::xf(h.tune[qg::P_XF], h.g, h.st, 1, c(h.Pu), c(h.Pv), c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), r(h.u), r(h.v), c(h.Pq),
             twx());
     });
-    K("y_r2c_omega", Ph * cR + P * cD, [&] { O::y_r2c_mask(h.g, h.st, c(h.Pq), h.rq0, twy()); });
+    K("y_r2c_omega", qg::P_Y_R2C, Ph * cR + P * cD, [&] { O::y_r2c_mask(h.tune[qg::P_Y_R2C], h.g, h.st, c(h.Pq), h.rq0, twy()); });
     if (h.sing_omega[0]) {
       char buf[160];
       snprintf(buf, sizeof buf, "LAPACK-style gtsv failed for vertical velocity at row %d of column %d",
                h.sing_omega[1] + 1, h.sing_omega[2]);
       throw QgError(QG_ERR_GTSV, buf);
     }
-    K("omega_gtsv", P * 2 * cD + Tq, [&] {
+    K("omega_gtsv", qg::P_OMEGA, P * 2 * cD + Tq, [&] {
       qg::launch_omega(h.g, h.st, h.rq0, h.qeff, h.om_M, h.om_D, h.om_piv, h.om_any_piv, h.d_cz, h.d_cu,
-                       h.d_fN2, h.s0, h.s1, h.s2);
+                       h.d_fN2, h.s0, h.s1, h.s2, h.tune[qg::P_OMEGA]);
     });
-    K("y_herm_w", P * cD + Ph * cR, [&] { O::y_herm(h.g, h.st, h.qeff, c(h.Pz), twy()); });
-    K("x_c2r_w", Ph * cR + P * rR, [&] { O::x_c2r(h.g, h.st, 1, c(h.Pz), nullptr, r(h.w), nullptr, twx()); });
+    K("y_herm_w", qg::P_Y_HERM, P * cD + Ph * cR, [&] { O::y_herm(h.tune[qg::P_Y_HERM], h.g, h.st, h.qeff, c(h.Pz), twy()); });
+    K("x_c2r_w", qg::P_X_C2R, Ph * cR + P * rR, [&] { O::x_c2r(h.tune[qg::P_X_C2R], h.g, h.st, 1, c(h.Pz), nullptr, r(h.w), nullptr, twx()); });
   }
 
   // One RHS evaluation at (qS, psiS, AS, T1, u, v) fused with the ETD update
@@ -367,26 +389,26 @@ struct Engine {
     if (refr) outs |= qg::OUT_Z;
     const int nouts = ((outs & qg::OUT_Q) ? 1 : 0) + ((outs & qg::OUT_Z) ? 1 : 0);
     if (outs)
-      K("yb_adv", P * (((outs & qg::OUT_Q) ? cD : 0) + ((outs & qg::OUT_Z) ? cD : 0)) + Ph * nouts * cR,
-        [&] { O::yb_adv(h.g, h.st, outs, psiS, qS, nullptr, nullptr, c(h.Pq), c(h.Pz), twy()); });
+      K("yb_adv", qg::P_YB_ADV, P * (((outs & qg::OUT_Q) ? cD : 0) + ((outs & qg::OUT_Z) ? cD : 0)) + Ph * nouts * cR,
+        [&] { O::yb_adv(h.tune[qg::P_YB_ADV], h.g, h.st, outs, psiS, qS, nullptr, nullptr, c(h.Pq), c(h.Pz), twy()); });
     if (adv && qdyn)
-      K("xb_q", Ph * 3 * cR + P * 2 * rR,
-        [&] { O::xb_q(h.g, h.st, c(h.Pq), r(h.u), r(h.v), c(h.G1), c(h.G2), twx()); });
+      K("xb_q", qg::P_XB_Q, Ph * 3 * cR + P * 2 * rR,
+        [&] { O::xb_q(h.tune[qg::P_XB_Q], h.g, h.st, c(h.Pq), r(h.u), r(h.v), c(h.G1), c(h.G2), twx()); });
     int xbf = (adv ? qg::XB_ADV : 0) | (refr ? qg::XB_REFR : 0);
     if (xbf)
-      K("xb_B", P * cR + (refr ? Ph * cR : 0) + (adv ? P * 2 * rR : 0) + P * cR * ((adv ? 2 : 0) + (refr ? 1 : 0)),
-        [&] { O::xb_B(h.g, h.st, xbf, c(h.T1), c(h.Pz), r(h.u), r(h.v), c(h.G3), c(h.G4), c(h.G5), twx()); });
+      K("xb_B", qg::P_XB_B, P * cR + (refr ? Ph * cR : 0) + (adv ? P * 2 * rR : 0) + P * cR * ((adv ? 2 : 0) + (refr ? 1 : 0)),
+        [&] { O::xb_B(h.tune[qg::P_XB_B], h.g, h.st, xbf, c(h.T1), c(h.Pz), r(h.u), r(h.v), c(h.G3), c(h.G4), c(h.G5), twx()); });
     const bool diff = h.nu_z != 0.0 && !h.inviscid && h.nz > 1;
     int qf = ((adv && qdyn) ? qg::YQ_ADV : 0) | (h.fixed_flow ? qg::YQ_FIXED : 0) | (diff ? qg::YQ_DIFF : 0);
     const double inv_dz2 = h.nz > 1 ? 1.0 / (h.dz * h.dz) : 0.0;
-    K(mode == qg::MODE_STAGE ? "yb2_q_stage" : "yb2_q_final",
+    K(mode == qg::MODE_STAGE ? "yb2_q_stage" : "yb2_q_final", qg::P_YB2_Q,
       ((qf & qg::YQ_ADV) ? Ph * 2 * cR : 0) + P * cD * (3 + (diff ? 1 : 0)),
-      [&] { O::yb2_q(h.g, h.st, mode, qf, c(h.G1), c(h.G2), qS, qn, h.rq0, qout, etd(), h.nu_z, inv_dz2, twy()); });
+      [&] { O::yb2_q(h.tune[qg::P_YB2_Q], h.g, h.st, mode, qf, c(h.G1), c(h.G2), qS, qn, h.rq0, qout, etd(), h.nu_z, inv_dz2, twy()); });
     const bool disp = !(h.passive || h.no_disp);
     int bf = (adv ? qg::YB_ADV : 0) | (refr ? qg::YB_REFR : 0) | (disp ? qg::YB_DISP : 0);
-    K(mode == qg::MODE_STAGE ? "yb2_B_stage" : "yb2_B_final",
+    K(mode == qg::MODE_STAGE ? "yb2_B_stage" : "yb2_B_final", qg::P_YB2_B,
       P * cR * ((adv ? 2 : 0) + (refr ? 1 : 0)) + P * cD * (3 + (disp ? 1 : 0)),
-      [&] { O::yb2_B(h.g, h.st, mode, bf, c(h.G3), c(h.G4), c(h.G5), AS, Bn, h.rB0, Bout, etd(), h.f / 2, twy()); });
+      [&] { O::yb2_B(h.tune[qg::P_YB2_B], h.g, h.st, mode, bf, c(h.G3), c(h.G4), c(h.G5), AS, Bn, h.rB0, Bout, etd(), h.f / 2, twy()); });
   }
 
   // step!(model, ExponentialRungeKutta2) -> _advance_etdrk2! (src/timestep.jl:335-415).
@@ -422,9 +444,9 @@ struct Engine {
   }
 
   void compute_qw(D* out) {
-    O::ya_qw(h.g, h.st, h.B[h.cur], c(h.T1), c(h.T3), twy());
-    O::xa_qw(h.g, h.st, c(h.T1), c(h.T3), c(h.Qa), c(h.Qb), twx());
-    O::ya2_qeff(h.g, h.st, c(h.Qa), c(h.Qb), h.q[h.cur], nullptr, out, 1.0 / h.f, twy());
+    O::ya_qw(h.tune[qg::P_YA_QW], h.g, h.st, h.B[h.cur], c(h.T1), c(h.T3), twy());
+    O::xa_qw(h.tune[qg::P_XA_QW], h.g, h.st, c(h.T1), c(h.T3), c(h.Qa), c(h.Qb), twx());
+    O::ya2_qeff(h.tune[qg::P_YA2_QEFF], h.g, h.st, c(h.Qa), c(h.Qb), h.q[h.cur], nullptr, out, 1.0 / h.f, twy());
     h.t1_valid = false;
   }
 
@@ -493,12 +515,12 @@ void fft2_inplace(qg_handle& h, D* f, int dir, unsigned long long* absmax = null
   if (levels >= 0) g.nz = levels;
   // y first for the inverse (spectral -> physical), x first for the forward
   if (dir < 0) {
-    qg::Ops<double>::x_c2c(g, h.st, -1, f, f, nullptr, h.tw64x);
-    qg::Ops<double>::y_c2c(g, h.st, -1, f, out ? out : f, h.tw64y);
+    qg::Ops<double>::x_c2c(1, g, h.st, -1, f, f, nullptr, h.tw64x);
+    qg::Ops<double>::y_c2c(1, g, h.st, -1, f, out ? out : f, h.tw64y);
   } else {
     D* o = out ? out : f;
-    qg::Ops<double>::y_c2c(g, h.st, +1, f, o, h.tw64y);
-    qg::Ops<double>::x_c2c(g, h.st, +1, o, absmax ? nullptr : o, absmax, h.tw64x);
+    qg::Ops<double>::y_c2c(1, g, h.st, +1, f, o, h.tw64y);
+    qg::Ops<double>::x_c2c(1, g, h.st, +1, o, absmax ? nullptr : o, absmax, h.tw64x);
   }
 }
 
@@ -1104,7 +1126,7 @@ int qg_invert_B_to_A_member(qg_handle* h, int member) {
     if (H.sing_A[0]) raise_singular(H.sing_A);
     const MemberView mv(H, member);
     qg::launch_thomas(mv.g, H.st, H.B[H.cur] + mv.off, H.A + mv.off, H.C + mv.off, H.th_inv[1] + mv.toff,
-                      H.th_cp[1] + mv.toff, H.d_al + mv.aoff, 4, nullptr);
+                      H.th_cp[1] + mv.toff, H.d_al + mv.aoff, 4, nullptr, H.tune[qg::P_THOMAS_AC]);
     H.diag_valid = false;
     check_status(H);
   });
@@ -1162,6 +1184,162 @@ int qg_fft(qg_handle* h, const double* host_in, double* host_out, int direction)
     qg::launch_transpose<D>(H.qeff, H.stg, H.nx, H.ny, H.nz, false, H.st);
     HIPCHECK(hipMemcpyAsync(host_out, H.stg, H.npts * sizeof(D), hipMemcpyDeviceToHost, H.st));
     HIPCHECK(hipStreamSynchronize(H.st));
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Autotuner: tile variants of the FFT passes and launch shapes of the column
+// solves, chosen per grid class (nx, ny, nzc, precision) by timing whole
+// eager steps on the model's own state with HIP events around every launch.
+// Candidate c runs every FFT pass at tile variant c (c < 3) and every column
+// operator at the c-th launch shape that fits the depth; each pass keeps the
+// candidate with its least total time.  The state is restored after every
+// candidate and at the end, so tuning leaves the model where it was.
+// ---------------------------------------------------------------------------
+namespace {
+
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int>, std::array<int, qg::P_END>> g_tune_cache;
+
+std::tuple<int, int, int, int> tune_key(const qg_handle& h) { return {h.nx, h.ny, h.nzc, h.precision}; }
+
+std::vector<int> column_candidates(int nzc) {
+  std::vector<int> c;
+  for (int i = 0; i < qg::kScanShapeCount; ++i)
+    if (qg::scan_shape_fits(qg::scan_shape_of(i, nzc))) c.push_back(i);
+  return c;
+}
+
+bool valid_choice(const qg_handle& h, int pass, int v) {
+  if (v == -1) return true;
+  if (pass < qg::P_FFT_END) return v >= 0 && v < qg::kTileVariants;
+  return v >= 0 && v < qg::kScanShapeCount && qg::scan_shape_fits(qg::scan_shape_of(v, h.nzc));
+}
+
+void autotune(qg_handle& H, double dt, int steps) {
+  if (H.prof_mode) throw QgError(QG_ERR_ARGUMENT, "autotune while a profile is being recorded");
+  HIPCHECK(hipStreamSynchronize(H.st));
+  const size_t n = (size_t)H.nzc * H.plane * H.active;
+  D* save = nullptr;
+  HIPCHECK(hipMalloc(&save, 2 * n * sizeof(D)));
+  struct Cleanup {
+    qg_handle& H;
+    D* save;
+    size_t n;
+    int cur;
+    ~Cleanup() {
+      (void)hipStreamSynchronize(H.st);
+      if (H.cur != cur) H.cur = cur;
+      (void)hipMemcpyAsync(H.q[H.cur], save, n * sizeof(D), hipMemcpyDeviceToDevice, H.st);
+      (void)hipMemcpyAsync(H.B[H.cur], save + n, n * sizeof(D), hipMemcpyDeviceToDevice, H.st);
+      (void)hipStreamSynchronize(H.st);
+      (void)hipFree(save);
+      H.prof_mode = 0;
+      H.prof.clear();
+      H.pool_used = 0;
+      H.diag_valid = false;
+      H.t1_valid = false;
+      H.drop_graphs();
+    }
+  } cleanup{H, save, n, H.cur};
+  HIPCHECK(hipMemcpyAsync(save, H.q[H.cur], n * sizeof(D), hipMemcpyDeviceToDevice, H.st));
+  HIPCHECK(hipMemcpyAsync(save + n, H.B[H.cur], n * sizeof(D), hipMemcpyDeviceToDevice, H.st));
+  build_etd(H, dt);
+  const std::vector<int> shapes = column_candidates(H.nzc);
+  const int ncfg = std::max<int>(qg::kTileVariants, (int)shapes.size());
+  double best[qg::P_END];
+  int choice[qg::P_END];
+  std::fill(best, best + qg::P_END, std::numeric_limits<double>::infinity());
+  std::fill(choice, choice + qg::P_END, -1);
+  for (int c = 0; c < ncfg; ++c) {
+    const int fv = c < qg::kTileVariants ? c : 1;
+    const int cs = c < (int)shapes.size() ? shapes[c] : qg::scan_default_shape(H.nzc);
+    for (int p = 0; p < qg::P_FFT_END; ++p) H.tune[p] = fv;
+    for (int p = qg::P_FFT_END; p < qg::P_END; ++p) H.tune[p] = cs;
+    HIPCHECK(hipStreamSynchronize(H.st));
+    HIPCHECK(hipMemcpyAsync(H.q[cleanup.cur], save, n * sizeof(D), hipMemcpyDeviceToDevice, H.st));
+    HIPCHECK(hipMemcpyAsync(H.B[cleanup.cur], save + n, n * sizeof(D), hipMemcpyDeviceToDevice, H.st));
+    H.cur = cleanup.cur;
+    H.diag_valid = false;
+    H.t1_valid = false;
+    std::vector<double> ms(qg::P_END, 0.0);
+    with_engine(H, [&](auto& e) {
+      e.step();  // warm-up; recomputes the diagnostics of the restored state
+      H.prof.clear();
+      H.pool_used = 0;
+      H.prof_mode = 1;
+      for (int i = 0; i < steps; ++i) e.step();
+      H.prof_mode = 0;
+    });
+    HIPCHECK(hipStreamSynchronize(H.st));
+    check_status(H);
+    for (auto& r : H.prof) {
+      float t = 0;
+      HIPCHECK(hipEventElapsedTime(&t, r.a, r.b));
+      ms[r.pass] += t;
+    }
+    H.prof.clear();
+    H.pool_used = 0;
+    for (int p = 0; p < qg::P_END; ++p) {
+      const bool attributable = p < qg::P_FFT_END ? c < qg::kTileVariants : c < (int)shapes.size();
+      if (!attributable || ms[p] <= 0.0) continue;
+      if (ms[p] < best[p]) {
+        best[p] = ms[p];
+        choice[p] = H.tune[p];
+      }
+    }
+  }
+  std::copy(choice, choice + qg::P_END, H.tune);
+}
+
+}  // namespace
+
+extern "C" {
+
+int qg_autotune(qg_handle* h, double dt, int steps, int flags) {
+  return guarded(h, [&] {
+    if (!(std::isfinite(dt) && dt > 0)) throw QgError(QG_ERR_ARGUMENT, "Δt must be finite and positive");
+    if (steps <= 0) steps = 3;
+    qg_handle& H = *h;
+    if (H.active == 0) return;
+    const auto key = tune_key(H);
+    if (!(flags & QG_TUNE_FORCE)) {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      auto it = g_tune_cache.find(key);
+      if (it != g_tune_cache.end()) {
+        HIPCHECK(hipStreamSynchronize(H.st));
+        std::copy(it->second.begin(), it->second.end(), H.tune);
+        H.drop_graphs();
+        return;
+      }
+    }
+    autotune(H, dt, steps);
+    std::array<int, qg::P_END> t;
+    std::copy(H.tune, H.tune + qg::P_END, t.begin());
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = t;
+  });
+}
+
+int qg_get_tuning(qg_handle* h, int* choices, int n) {
+  return guarded(h, [&] {
+    if (!choices || n < (int)qg::P_END) throw QgError(QG_ERR_ARGUMENT, "need QG_TUNE_PASSES entries");
+    std::copy(h->tune, h->tune + qg::P_END, choices);
+  });
+}
+
+int qg_set_tuning(qg_handle* h, const int* choices, int n) {
+  return guarded(h, [&] {
+    if (!choices || n < (int)qg::P_END) throw QgError(QG_ERR_ARGUMENT, "need QG_TUNE_PASSES entries");
+    for (int p = 0; p < qg::P_END; ++p)
+      if (!valid_choice(*h, p, choices[p]))
+        throw QgError(QG_ERR_ARGUMENT, "tuning choice " + std::to_string(choices[p]) + " is not valid for pass " +
+                                           std::to_string(p));
+    HIPCHECK(hipStreamSynchronize(h->st));
+    std::copy(choices, choices + qg::P_END, h->tune);
+    h->drop_graphs();  // captured with the previous kernels
   });
 }
 

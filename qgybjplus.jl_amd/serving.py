@@ -112,13 +112,15 @@ class Pool:
     runs occupy the leading slots [0, n) and only those are stepped: a run
     that finishes is replaced by the last resident one (qg_move_member)."""
 
-    def __init__(self, grid_class, slots: int, precision: str, dt: float):
+    def __init__(self, grid_class, slots: int, precision: str, dt: float, autotune: bool = False):
         nx, ny, nz = grid_class
         self.grid = qg.RectilinearGrid(size=(nx, ny, nz), extent=EXTENT, centered=True)
         self.slots = slots
         self.ens = qg.QGYBJEnsemble(stratifications=[qg.ConstantStratification(1e-5)] * slots, grid=self.grid,
                                     coriolis=qg.FPlane(1e-4), feedback=qg.WaveMeanFeedback(),
                                     formulation=qg.YBJPlus(), architecture=qg.GPU(precision))
+        if autotune:  # timed on the zero state of every slot; cached per grid class
+            self.ens.autotune(dt, steps=2)
         self.ens.set_active_members(0)
         self.ts = qg.ExponentialRungeKutta2(dt=dt)
         self.runs: List[RunRequest] = []
@@ -178,7 +180,8 @@ class Server:
 
     def __init__(self, *, precision: str = "mixed", dt: float = 20.0, slots: int = 4, chunk: int = 50,
                  reserve_bytes: int = 8 << 30, memory_budget: Optional[int] = None, keep_fields: bool = False,
-                 workers: int = 4):
+                 workers: int = 4, autotune: bool = False):
+        self.autotune = autotune  # tune each new pool's tile variants and launch shapes
         self.precision = precision
         self.dt = dt
         self.slots = slots
@@ -218,7 +221,7 @@ class Server:
                 if not self._release_idle_pool():
                     return False
             nx, ny, nz = gc
-            pool = Pool(gc, self.slots, self.precision, self.dt)
+            pool = Pool(gc, self.slots, self.precision, self.dt, autotune=self.autotune)
             self.pools[gc] = pool
             self.pool_bytes[gc] = _lib.memory_required(nx=nx, ny=ny, nz=nz, precision=self._prec_id(),
                                                        members=self.slots)

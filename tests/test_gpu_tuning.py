@@ -1,0 +1,111 @@
+"""The autotuner's search space and its effect on results: every tile
+variant of the FFT passes gives bitwise the result of the base variant, every
+column-solve launch shape agrees with the default to round-off, and
+qg_autotune leaves the model's state where it was."""
+
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+EXTENT = (500e3, 500e3, 4000.0)
+FIELDS = ("q", "B", "psi", "A", "C", "u", "v", "w")
+
+
+def _model(qg, size, precision):
+    grid = qg.RectilinearGrid(size=size, extent=EXTENT)
+    return qg.QGYBJModel(grid=grid, coriolis=qg.FPlane(1e-4), feedback=qg.WaveMeanFeedback(),
+                         stratification=qg.AnalyticalProfile(lambda z: 1e-4 * math.exp(z / 800.0)),
+                         architecture=qg.GPU(precision))
+
+
+def _init(qg, m, seed=4):
+    g = m.grid
+    nx, ny, nz = g.size
+    rng = np.random.default_rng(seed)
+    X, Y = np.meshgrid(g.x, g.y, indexing="ij")
+    psi = np.empty((nz, nx, ny), complex)
+    B = np.empty((nz, nx, ny), complex)
+    for k in range(nz):
+        psi[k] = 2e3 * np.sin(2 * math.pi * X / g.extent[0] + rng.uniform(0, 6)) * \
+            np.cos(4 * math.pi * Y / g.extent[1] + rng.uniform(0, 6))
+        B[k] = 0.1 * math.exp(g.z[k] / 200.0) * np.exp(2j * math.pi * (X / g.extent[0] + 2 * Y / g.extent[1]))
+    qg.set_(m, psi=qg.FieldArray(psi), waves=qg.FieldArray(B))
+
+
+def _run(qg, m, choices, steps=4, dt=300.0):
+    _init(qg, m)
+    m.set_tuning(choices)
+    qg.step_(m, qg.ExponentialRungeKutta2(dt=dt), steps)
+    return {n: getattr(m.fields, n) for n in FIELDS}
+
+
+def _rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / nb if nb > 0 else np.linalg.norm(a)
+
+
+FFT = ("ya_qw", "xa_qw", "ya2_qeff", "yb_adv", "x_c2r", "xb_q", "xb_B", "yb2_q", "yb2_B", "yf_omega", "xf",
+       "y_r2c", "y_herm")
+COLUMN = ("thomas_psi", "thomas_A", "thomas_AC", "omega")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fft_tile_variants_are_bitwise_identical(qg, precision):
+    print(f"case tile variants {precision}", flush=True)
+    m = _model(qg, (64, 32, 6), precision)
+    ref = _run(qg, m, {})
+    for v in (0, 1, 2):
+        out = _run(qg, m, {p: v for p in FFT})
+        for n in FIELDS:
+            assert np.array_equal(out[n], ref[n]), (precision, v, n)
+    m.finalize()
+
+
+@pytest.mark.parametrize("nz", [5, 12, 40, 128])
+def test_column_shapes_agree_with_default(qg, nz):
+    print(f"case column shapes nz={nz}", flush=True)
+    m = _model(qg, (16, 16, nz), "fp64")
+    ref = _run(qg, m, {})
+    tried = 0
+    for shape in range(6):
+        try:
+            out = _run(qg, m, {p: shape for p in COLUMN})
+        except ValueError:  # the shape does not fit this depth
+            continue
+        tried += 1
+        for n in FIELDS:
+            e = _rel(out[n], ref[n])
+            assert e <= 1e-12, (nz, shape, n, e)
+    assert tried >= 3
+    m.finalize()
+
+
+def test_autotune_restores_state_and_caches(qg):
+    print("case autotune", flush=True)
+    m = _model(qg, (64, 64, 16), "mixed")
+    _init(qg, m)
+    before = {n: m._h.get_field(n) for n in ("q", "B")}
+    choice = m.autotune(dt=300.0, steps=2, force=True)
+    assert set(choice) == set(FFT + COLUMN)
+    assert all(-1 <= choice[p] <= 2 for p in FFT)
+    for n in ("q", "B"):
+        assert np.array_equal(m._h.get_field(n), before[n]), n
+    # stepping after tuning matches an untuned model to round-off
+    u = _model(qg, (64, 64, 16), "mixed")
+    _init(qg, u)
+    ts = qg.ExponentialRungeKutta2(dt=300.0)
+    qg.step_(m, ts, 6)
+    qg.step_(u, ts, 6)
+    for n in ("q", "B", "psi", "A"):
+        assert _rel(m._h.get_field(n), u._h.get_field(n)) <= 1e-12, n
+    # a second model of the same grid class takes the cached choice
+    assert u.autotune(dt=300.0) == choice
+    with pytest.raises(ValueError):
+        u.set_tuning({"ya_qw": 7})
+    with pytest.raises(ValueError):
+        u.set_tuning({"thomas_psi": 99})
+    m.finalize()
+    u.finalize()
