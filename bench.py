@@ -134,7 +134,7 @@ def ensemble_leg(qg, steps, warmup=2, seed=2026, autotune=True):
     ts = qg.ExponentialRungeKutta2(dt=DT)
     for ens in ens_list:
         if autotune:
-            ens.autotune(DT, steps=2)
+            ens.autotune(DT, steps=4)
         qg.step_(ens, ts, warmup)
     per_class = []
     t0 = time.perf_counter()
@@ -175,7 +175,7 @@ def large_leg(qg, steps, kernel_steps=20, seed=2026, autotune=True):
     qg.set_(m, psi=qg.FieldArray(synthetic.k3_streamfunction(grid, seed=seed, urms=0.1, kpeak=6.0)),
             waves=qg.FieldArray(synthetic.wave_modes(grid, 32, seed=seed, rms=0.1, efold=30.0), space="spectral"))
     ts = qg.ExponentialRungeKutta2(dt=10.0)
-    choice = m.autotune(10.0, steps=2) if autotune else None
+    choice = m.autotune(10.0, steps=3) if autotune else None
     qg.step_(m, ts, 2)
     m._h.synchronize()
     t0 = time.perf_counter()
@@ -304,7 +304,7 @@ def main():
         qg.set_(m, waves=qg.SurfaceWave(0.1, 50.0))
         if args.autotune:
             t0 = time.perf_counter()
-            tuning[name] = {"choice": m.autotune(DT, steps=3), "seconds": time.perf_counter() - t0}
+            tuning[name] = {"choice": m.autotune(DT, steps=6), "seconds": time.perf_counter() - t0}
             log(f"autotuned the {name} model in {tuning[name]['seconds']:.2f} s")
 
     npts = SIZE[0] * SIZE[1] * SIZE[2]

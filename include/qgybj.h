@@ -153,6 +153,24 @@ int qg_compute_q_from_psi(qg_handle* h, int method);
  * other members). */
 int qg_compute_q_from_psi_member(qg_handle* h, int member, int method);
 int qg_invert_B_to_A_member(qg_handle* h, int member);
+/* Seeded random streamfunction on the device: replaces init_mpi_random_psi!
+ * (src/parallel_mpi.jl:1159-1216) as called by set_mean_flow!(method=:random)
+ * for RandomStreamfunction (src/simulation.jl:268-275, src/core/components.jl:
+ * 218-222,264-272).  Writes psi-hat of one member (member == -1: every member,
+ * each with the same seed) with the reference's counter hash over the
+ * 1-based (level, x, y) indices of one model, energy amplitude*|k|^slope, and
+ * the 2/3 mask.  q, u, v are derived by the caller's set-up operators
+ * (qg_compute_q_from_psi[_member], qg_compute_velocities).  A negative seed
+ * is an ArgumentError (the reference's UInt64(seed) throws). */
+int qg_set_random_psi(qg_handle* h, int member, double amplitude, double slope, long long seed);
+
+/* SurfaceWave on the device: replaces the field fill and fft_forward! of
+ * set_surface_waves! (src/simulation.jl:292-329).  profile[k] is the
+ * amplitude times the vertical profile of level k (nz entries for one member,
+ * members*nz for member == -1); B-hat becomes nx*ny*profile[k] on the kh = 0
+ * mode and zero elsewhere.  A follows from qg_invert_B_to_A[_member]. */
+int qg_set_surface_wave(qg_handle* h, int member, const double* profile);
+
 /* Replaces one member's N2 and a_ell profiles (nz entries each, computed as
  * for qg_config) and re-records its column eliminations, so a free ensemble
  * slot can take a run with another stratification.  Fails with
@@ -192,7 +210,8 @@ int qg_synchronize(qg_handle* h);
  * (launch shape: partition depth and tile width; shapes differ in the
  * association of the partition carries, a few ulp).  qg_autotune times
  * `steps` eager steps of size dt per candidate on the model's own state,
- * restores the state, and keeps the fastest choice per pass; the choice is
+ * restores the state, and keeps per pass the candidate with the least median
+ * launch time when it beats the default by at least 1 %; the choice is
  * cached per grid class (nx, ny, nz per member, precision) in the process and
  * reused by later calls unless flags has QG_TUNE_FORCE.  -1 is the default
  * choice of a pass.  choices[] has QG_TUNE_PASSES entries in the order

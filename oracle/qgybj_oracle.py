@@ -42,6 +42,13 @@ Pins (what checks each function; see ``tests/test_oracle.py``):
   dense brute-force solve of the interior system.
 * ``set_surface_waves``/``compute_barotropic_q``: reference KAT
   ``test/test_model_initialization.jl:25-52``.
+* ``mix64``/``hash_u01``: the published SplitMix64 output sequence for seed 0
+  (the reference's ``_mix64`` is SplitMix64's finalizer with its increment);
+  ``random_psi``: Hermitian symmetry (real inverse transform), the 2/3 mask,
+  the zero mean, the k^slope energy law of its construction, and seed
+  determinism (the reference pins determinism only,
+  ``test/test_model_initialization.jl:93-98``).  Its ``log``/``cos``/``pow``
+  are the platform's, so exact values can differ from Julia's by an ulp.
 
 Parity unpinned: the omega-equation ``w`` *values* of the reference (no
 reference test checks them; the restated gtsv is pinned against a dense solve
@@ -824,3 +831,79 @@ def relative_l2(a: np.ndarray, b: np.ndarray) -> float:
     if nb == 0.0:
         return nd
     return nd / nb
+
+
+# ---------------------------------------------------------------------------
+# Seeded random streamfunction (src/parallel_mpi.jl:1115-1216)
+# ---------------------------------------------------------------------------
+
+_M64 = (1 << 64) - 1
+
+
+def mix64(x: int) -> int:
+    """_mix64 (src/parallel_mpi.jl:1117-1122) on Python integers mod 2^64."""
+    x = (x + 0x9E3779B97F4A7C15) & _M64
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & _M64
+    return x ^ (x >> 31)
+
+
+def _mix64_np(x: np.ndarray) -> np.ndarray:
+    x = x + np.uint64(0x9E3779B97F4A7C15)
+    x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return x ^ (x >> np.uint64(31))
+
+
+def hash_u01(seed: int, vals, tag: int):
+    """_hash_u01 (src/parallel_mpi.jl:1124-1131): vals are integer arrays (or
+    ints) of the 1-based indices; returns Float64((x >> 11) + 1) * 2^-53."""
+    with np.errstate(over="ignore"):
+        x = np.full(np.broadcast(*[np.asarray(v) for v in vals]).shape, seed, dtype=np.uint64)
+        for v in vals:
+            x = _mix64_np(x ^ np.asarray(v, dtype=np.int64).astype(np.uint64))
+        x = _mix64_np(x ^ np.uint64(tag))
+    return ((x >> np.uint64(11)) + np.uint64(1)).astype(np.float64) * 2.0 ** -53
+
+
+def hash_randn(seed: int, vals, tag: int):
+    """_hash_randn (src/parallel_mpi.jl:1133-1137): Box-Muller of two hashes."""
+    u1 = hash_u01(seed, vals, tag)
+    u2 = hash_u01(seed, vals, tag + 1)
+    return np.sqrt(-2 * np.log(u1)) * np.cos(2 * math.pi * u2)
+
+
+def random_psi(nx: int, ny: int, nz: int, amplitude: float, slope: float, seed: int) -> np.ndarray:
+    """init_mpi_random_psi! (src/parallel_mpi.jl:1159-1216) with seed_offset 0:
+    psi-hat [k, i, j], energy amplitude*|k|^slope per mode, Hermitian by
+    construction (the canonical member of a conjugate pair draws), then the
+    2/3 mask (apply_dealiasing_mask!)."""
+    if seed < 0:
+        raise ValueError("seed must be non-negative")
+    base = mix64(seed & _M64)
+    ig = np.arange(1, nx + 1)[:, None] * np.ones((1, ny), dtype=np.int64)
+    jg = np.ones((nx, 1), dtype=np.int64) * np.arange(1, ny + 1)[None, :]
+    kx = np.where(ig <= nx // 2, ig - 1, ig - 1 - nx)
+    ky = np.where(jg <= ny // 2, jg - 1, jg - 1 - ny)
+    k_total = np.sqrt((kx * kx + ky * ky).astype(np.float64))
+    with np.errstate(divide="ignore"):
+        energy = amplitude * np.where(k_total > 0, k_total, 1.0) ** slope
+    ic = np.where(ig == 1, 1, nx - ig + 2)
+    jc = np.where(jg == 1, 1, ny - jg + 2)
+    selfconj = (ig == ic) & (jg == jc)
+    canonical = (ig < ic) | ((ig == ic) & (jg <= jc))
+    ican = np.where(canonical, ig, ic)
+    jcan = np.where(canonical, jg, jc)
+    mask = dealias_mask(nx, ny)
+    out = np.zeros((nz, nx, ny), dtype=np.complex128)
+    for k in range(nz):
+        kg = k + 1
+        amp_self = np.sqrt(2 * energy) * hash_randn(base, (kg, ig, jg), 0x10)
+        amp = np.sqrt(2 * energy) * hash_randn(base, (kg, ican, jcan), 0x20)
+        phase = 2 * math.pi * hash_u01(base, (kg, ican, jcan), 0x30)
+        val = amp * np.cos(phase) + 1j * (amp * np.sin(phase))
+        val = np.where(canonical, val, np.conj(val))
+        v = np.where(selfconj, amp_self + 0j, val)
+        v[(kx == 0) & (ky == 0)] = 0
+        out[k] = np.where(mask, v, 0)
+    return out

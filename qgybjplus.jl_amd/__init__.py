@@ -11,7 +11,7 @@ from .api import (  # noqa: F401
     FixedFlow, EvolvingFlow, NoFeedback, WaveMeanFeedback, NoWaveFeedback, YBJPlus, YBJ,
     PassiveWave, Dissipative, Inviscid, NonlinearDynamics, LinearDynamics, Dispersive,
     NoDispersion, FlowHyperdiffusivity, WaveHyperdiffusivity, HorizontalHyperdiffusivity,
-    VerticalDiffusivity, SurfaceWave, FieldArray, GPU, QGYBJModel, QGYBJEnsemble, finalize_model_, set_,
+    VerticalDiffusivity, SurfaceWave, RandomStreamfunction, FieldArray, GPU, QGYBJModel, QGYBJEnsemble, finalize_model_, set_,
     ExponentialRungeKutta2, step_, Simulation, SimulationState, Clock, run_,
     check_termination_, finalize_simulation_,
 )

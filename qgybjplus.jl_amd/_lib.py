@@ -39,6 +39,7 @@ EXPORTED = (
     "qg_compute_q_from_psi_member", "qg_invert_B_to_A_member", "qg_use_graphs",
     "qg_set_member_stratification", "qg_memory_required", "qg_device_memory",
     "qg_set_active_members", "qg_move_member", "qg_autotune", "qg_get_tuning", "qg_set_tuning",
+    "qg_set_random_psi", "qg_set_surface_wave",
 )
 
 
@@ -120,6 +121,8 @@ def load() -> C.CDLL:
         "qg_profile_begin": (C.c_int, [P, C.c_int]),
         "qg_profile_end": (C.c_int, [P, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
         "qg_autotune": (C.c_int, [P, C.c_double, C.c_int, C.c_int]),
+        "qg_set_random_psi": (C.c_int, [P, C.c_int, C.c_double, C.c_double, C.c_longlong]),
+        "qg_set_surface_wave": (C.c_int, [P, C.c_int, dp]),
         "qg_get_tuning": (C.c_int, [P, C.POINTER(C.c_int), C.c_int]),
         "qg_set_tuning": (C.c_int, [P, C.POINTER(C.c_int), C.c_int]),
     }
@@ -324,6 +327,14 @@ class Handle:
             raise RuntimeError("profile summary larger than the buffer")
         self._check(rc)
         return json.loads(buf.value.decode())
+
+    # -- device-side initial conditions -----------------------------------------
+    def set_random_psi(self, amplitude: float, slope: float, seed: int, member: int = -1):
+        self._check(self._lib.qg_set_random_psi(self.h, int(member), float(amplitude), float(slope), int(seed)))
+
+    def set_surface_wave(self, profile, member: int = -1):
+        p = np.ascontiguousarray(profile, dtype=np.float64)
+        self._check(self._lib.qg_set_surface_wave(self.h, int(member), _dptr(p)))
 
     # -- autotuner ---------------------------------------------------------------
     def autotune(self, dt: float, steps: int = 3, force: bool = False) -> dict:

@@ -267,6 +267,23 @@ class SurfaceWave:
 
 
 @dataclass(frozen=True)
+class RandomStreamfunction:
+    """Deterministic random streamfunction spectrum (src/core/components.jl:
+    218-222,264-272), generated on the device by the reference's counter hash
+    (init_mpi_random_psi!, src/parallel_mpi.jl:1159-1216)."""
+    amplitude: float = 1.0
+    spectral_slope: float = -3.0
+    seed: int = 0
+
+    def __post_init__(self):
+        if not (math.isfinite(self.amplitude) and math.isfinite(self.spectral_slope)):
+            raise ValueError("random streamfunction parameters must be finite")
+        if self.amplitude < 0:
+            raise ValueError("amplitude must be non-negative")
+        object.__setattr__(self, "seed", int(self.seed))
+
+
+@dataclass(frozen=True)
 class FieldArray:
     """A 3-D field for ``set_`` (src/core/components.jl:225-241)."""
     values: np.ndarray
@@ -528,7 +545,9 @@ class QGYBJEnsemble(QGYBJModel):
             raise RuntimeError("cannot modify a finalized model")
         h = self._h
         if psi is not None:
-            if callable(psi):
+            if isinstance(psi, RandomStreamfunction):
+                h.set_random_psi(psi.amplitude, psi.spectral_slope, psi.seed, member=member)
+            elif callable(psi):
                 h.set_field_member(member, "psi", _evaluate_on_nodes(self.grid, psi), _lib.SPACE_PHYSICAL)
             else:
                 vals, space = _field_values_zxy(self, psi)
@@ -540,7 +559,7 @@ class QGYBJEnsemble(QGYBJModel):
                 raise ValueError("pv_method must be :qg, :barotropic, or :none")
         if waves is not None:
             if isinstance(waves, SurfaceWave):
-                h.set_field_member(member, "B", _surface_wave_values(self.grid, waves), _lib.SPACE_PHYSICAL)
+                h.set_surface_wave(_surface_wave_profile(self.grid, waves), member=member)
             else:
                 vals, space = _field_values_zxy(self, waves)
                 h.set_field_member(member, "B", vals,
@@ -615,17 +634,18 @@ def _field_values_zxy(model, source) -> Tuple[np.ndarray, str]:
     return vals.astype(np.complex128), space
 
 
-def _surface_wave_values(g, wave) -> np.ndarray:
-    """Physical B of set_surface_waves! (src/initialization.jl:552-595): the
-    amplitude times the vertical profile at each cell depth."""
-    nx, ny, nz = g.size
-    phys = np.empty((nz, nx, ny), dtype=np.complex128)
+def _surface_wave_profile(g, wave) -> np.ndarray:
+    """The amplitude times the vertical profile of set_surface_waves!
+    (src/simulation.jl:317-324) at each level's depth max(0, -z - dz/2); the
+    horizontally uniform field and its transform are formed on the device."""
+    nz = g.size[2]
+    prof = np.empty(nz)
     for k in range(nz):
         depth = max(0.0, -g.z[k] - g.dz / 2)
-        prof = math.exp(-(depth ** 2) / wave.scale ** 2) if wave.profile == "gaussian" \
+        p = math.exp(-(depth ** 2) / wave.scale ** 2) if wave.profile == "gaussian" \
             else math.exp(-depth / wave.scale)
-        phys[k] = complex(wave.amplitude * prof)
-    return phys
+        prof[k] = wave.amplitude * p
+    return prof
 
 
 def set_(model, *, psi=None, mean_flow=None, pv_method: str = "qg", waves=None, B=None,
@@ -640,13 +660,15 @@ def set_(model, *, psi=None, mean_flow=None, pv_method: str = "qg", waves=None, 
     flow = mean_flow if mean_flow is not None else psi
     wave = waves if waves is not None else B
     if flow is not None:
-        if callable(flow):
+        if isinstance(flow, RandomStreamfunction):
+            h.set_random_psi(flow.amplitude, flow.spectral_slope, flow.seed)
+        elif callable(flow):
             h.set_field("psi", _evaluate_on_nodes(model.grid, flow), _lib.SPACE_PHYSICAL)
         elif isinstance(flow, (FieldArray, np.ndarray)):
             vals, space = _field_values_zxy(model, flow)
             h.set_field("psi", vals, _lib.SPACE_PHYSICAL if space == "physical" else _lib.SPACE_SPECTRAL)
         else:
-            raise ValueError("mean_flow/ψ must be a function, array or FieldArray")
+            raise ValueError("mean_flow/ψ must be a function, RandomStreamfunction, array or FieldArray")
         if pv_method in ("qg", "balanced"):
             h.compute_q_from_psi(barotropic=False)
         elif pv_method in ("barotropic", "asselin"):
@@ -656,7 +678,8 @@ def set_(model, *, psi=None, mean_flow=None, pv_method: str = "qg", waves=None, 
         h.compute_velocities(compute_w=False)
     if wave is not None:
         if isinstance(wave, SurfaceWave):
-            h.set_field("B", _surface_wave_values(model.grid, wave), _lib.SPACE_PHYSICAL)
+            members = getattr(model, "members", 1)
+            h.set_surface_wave(np.tile(_surface_wave_profile(model.grid, wave), members))
         elif isinstance(wave, (FieldArray, np.ndarray)):
             vals, space = _field_values_zxy(model, wave)
             h.set_field("B", vals, _lib.SPACE_PHYSICAL if space == "physical" else _lib.SPACE_SPECTRAL)

@@ -36,6 +36,12 @@ bool omega_setup();
 void launch_omega(const Geo& g, hipStream_t st, const cplx<double>* rhsw, cplx<double>* wk, const double* M,
                   const double* D, const unsigned char* pivoted, bool any_pivoted, const double* cz,
                   const double* cu, const double* fN2, double* sd, double* sdu, double* sfill, int shape = -1);
+// Initial conditions on the device (qg_init.hip): the seeded random
+// streamfunction of members [m0, m0 + nm) and the spectral surface wave
+// (profile: nzc values per member, the amplitude times the vertical profile).
+void launch_random_psi(const Geo& g, hipStream_t st, int m0, int nm, unsigned long long seed, double amplitude,
+                       double slope, cplx<double>* psi);
+void launch_surface_wave(const Geo& g, hipStream_t st, int m0, int nm, const double* profile, cplx<double>* B);
 void launch_q_from_psi(const Geo& g, hipStream_t st, const cplx<double>* psi, cplx<double>* q,
                        const double* a, int barotropic);
 template <typename T>

@@ -64,6 +64,7 @@ struct qg_handle {
   size_t plane = 0, npts = 0, nhalf = 0;
   std::vector<void*> allocs;
   // device
+  double* d_prof = nullptr;  // a vertical profile per stacked level (set-up input)
   double *d_kx = nullptr, *d_ky = nullptr, *d_a = nullptr, *d_al = nullptr, *d_cz = nullptr, *d_cu = nullptr,
          *d_fN2 = nullptr;
   unsigned char* d_mask = nullptr;
@@ -616,6 +617,7 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
     h->d_a = h->alloc<double>(nz);
     h->d_cz = h->alloc<double>(nz);
     h->d_fN2 = h->alloc<double>(nz);
+    h->d_prof = h->alloc<double>(nz);
     h->d_mask = h->alloc<unsigned char>(h->plane);
     h->d_etd = h->alloc<double>(6 * h->plane);
     std::vector<unsigned char> mask(h->plane);
@@ -771,7 +773,7 @@ int qg_memory_required(const qg_config* cfg, size_t* bytes) {
              + 5 * 2 * r * npts   // full-width transform intermediates
              + 8 * 2 * r * nhalf  // half-width transform intermediates
              + 6 * 8 * quarter * nz + quarter * members  // Thomas and omega tables
-             + 6 * 8 * plane + plane + 4 * 16 * (nx + ny) + (3 * 8 * nz) + 2 * 8 * members * kpad;
+             + 6 * 8 * plane + plane + 4 * 16 * (nx + ny) + (4 * 8 * nz) + 2 * 8 * members * kpad;
   b += 64 * 256;  // allocation padding
   *bytes = b;
   return QG_OK;
@@ -972,6 +974,59 @@ int qg_step(qg_handle* h, double dt, int nsteps) {
     });
     HIPCHECK(hipGetLastError());
     check_status(H);
+  });
+}
+
+// Members addressed by a set-up call: one member, or all (member == -1).
+static void member_span(const qg_handle& H, int member, int& m0, int& nm) {
+  if (member == -1) {
+    m0 = 0;
+    nm = H.members;
+    return;
+  }
+  check_member_index(H, member);
+  m0 = member;
+  nm = 1;
+}
+
+int qg_set_random_psi(qg_handle* h, int member, double amplitude, double slope, long long seed) {
+  return guarded(h, [&] {
+    qg_handle& H = *h;
+    int m0, nm;
+    member_span(H, member, m0, nm);
+    if (!(std::isfinite(amplitude) && std::isfinite(slope)))
+      throw QgError(QG_ERR_ARGUMENT, "random streamfunction parameters must be finite");
+    if (amplitude < 0) throw QgError(QG_ERR_ARGUMENT, "amplitude must be non-negative");
+    if (seed < 0) throw QgError(QG_ERR_ARGUMENT, "seed must be non-negative (UInt64(seed))");
+    qg::Geo g = H.g;
+    g.nz = H.nz;
+    g.members = H.members;
+    qg::launch_random_psi(g, H.st, m0, nm, (unsigned long long)seed, amplitude, slope, H.psi);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(H.st));
+    H.diag_valid = false;
+    H.t1_valid = false;
+  });
+}
+
+int qg_set_surface_wave(qg_handle* h, int member, const double* profile) {
+  return guarded(h, [&] {
+    qg_handle& H = *h;
+    int m0, nm;
+    member_span(H, member, m0, nm);
+    if (!profile) throw QgError(QG_ERR_ARGUMENT, "profile is required");
+    const size_t n = (size_t)nm * H.nzc;
+    for (size_t k = 0; k < n; ++k)
+      if (!std::isfinite(profile[k])) throw QgError(QG_ERR_ARGUMENT, "wave profile must be finite");
+    HIPCHECK(hipMemcpyAsync(H.d_prof, profile, n * sizeof(double), hipMemcpyHostToDevice, H.st));
+    qg::Geo g = H.g;
+    g.nz = H.nz;
+    g.members = H.members;
+    qg::launch_surface_wave(g, H.st, m0, nm, H.d_prof, H.B[H.cur]);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(H.st));
+    H.diag_valid = false;
+    H.t1_valid = false;
   });
 }
 
@@ -1194,9 +1249,11 @@ int qg_fft(qg_handle* h, const double* host_in, double* host_out, int direction)
 // solves, chosen per grid class (nx, ny, nzc, precision) by timing whole
 // eager steps on the model's own state with HIP events around every launch.
 // Candidate c runs every FFT pass at tile variant c (c < 3) and every column
-// operator at the c-th launch shape that fits the depth; each pass keeps the
-// candidate with its least total time.  The state is restored after every
-// candidate and at the end, so tuning leaves the model where it was.
+// operator at the c-th launch shape that fits the depth.  A pass's time for a
+// candidate is the median of its launch durations; a pass leaves its default
+// only for a candidate at least 1 % faster than the default (timing noise
+// between equal variants is about that size).  The state is restored after
+// every candidate and at the end, so tuning leaves the model where it was.
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -1249,9 +1306,10 @@ void autotune(qg_handle& H, double dt, int steps) {
   build_etd(H, dt);
   const std::vector<int> shapes = column_candidates(H.nzc);
   const int ncfg = std::max<int>(qg::kTileVariants, (int)shapes.size());
-  double best[qg::P_END];
+  double best[qg::P_END], dflt[qg::P_END];
   int choice[qg::P_END];
   std::fill(best, best + qg::P_END, std::numeric_limits<double>::infinity());
+  std::fill(dflt, dflt + qg::P_END, std::numeric_limits<double>::infinity());
   std::fill(choice, choice + qg::P_END, -1);
   for (int c = 0; c < ncfg; ++c) {
     const int fv = c < qg::kTileVariants ? c : 1;
@@ -1264,7 +1322,7 @@ void autotune(qg_handle& H, double dt, int steps) {
     H.cur = cleanup.cur;
     H.diag_valid = false;
     H.t1_valid = false;
-    std::vector<double> ms(qg::P_END, 0.0);
+    std::vector<std::vector<float>> ms(qg::P_END);
     with_engine(H, [&](auto& e) {
       e.step();  // warm-up; recomputes the diagnostics of the restored state
       H.prof.clear();
@@ -1278,19 +1336,27 @@ void autotune(qg_handle& H, double dt, int steps) {
     for (auto& r : H.prof) {
       float t = 0;
       HIPCHECK(hipEventElapsedTime(&t, r.a, r.b));
-      ms[r.pass] += t;
+      ms[r.pass].push_back(t);
     }
     H.prof.clear();
     H.pool_used = 0;
     for (int p = 0; p < qg::P_END; ++p) {
       const bool attributable = p < qg::P_FFT_END ? c < qg::kTileVariants : c < (int)shapes.size();
-      if (!attributable || ms[p] <= 0.0) continue;
-      if (ms[p] < best[p]) {
-        best[p] = ms[p];
+      if (!attributable || ms[p].empty()) continue;
+      std::vector<float>& v = ms[p];
+      std::nth_element(v.begin(), v.begin() + v.size() / 2, v.end());
+      const double med = v[v.size() / 2];
+      const int def = p < qg::P_FFT_END ? 1 : qg::scan_default_shape(H.nzc);
+      if (H.tune[p] == def) dflt[p] = med;
+      if (med < best[p]) {
+        best[p] = med;
         choice[p] = H.tune[p];
       }
     }
   }
+  for (int p = 0; p < qg::P_END; ++p)  // keep the default unless clearly beaten
+    if (choice[p] >= 0 && dflt[p] < std::numeric_limits<double>::infinity() && best[p] > 0.99 * dflt[p])
+      choice[p] = -1;
   std::copy(choice, choice + qg::P_END, H.tune);
 }
 
@@ -1301,7 +1367,7 @@ extern "C" {
 int qg_autotune(qg_handle* h, double dt, int steps, int flags) {
   return guarded(h, [&] {
     if (!(std::isfinite(dt) && dt > 0)) throw QgError(QG_ERR_ARGUMENT, "Δt must be finite and positive");
-    if (steps <= 0) steps = 3;
+    if (steps <= 0) steps = 4;
     qg_handle& H = *h;
     if (H.active == 0) return;
     const auto key = tune_key(H);

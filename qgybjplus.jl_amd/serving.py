@@ -60,9 +60,13 @@ class RunRequest:
             return qg.ConstantStratification(1e-5)
         return qg.AnalyticalProfile(synthetic.exponential_N2(1e-4, 800.0))
 
-    def initial_state(self, grid) -> Tuple[np.ndarray, "qg.SurfaceWave"]:
-        psi = synthetic.k3_streamfunction(grid, seed=self.seed, urms=0.1, kpeak=min(6.0, grid.size[0] / 8))
-        return psi, qg.SurfaceWave(self.wave_amplitude, 50.0)
+    def initial_state(self, grid) -> Tuple["qg.RandomStreamfunction", "qg.SurfaceWave"]:
+        """The reference's seeded k^-3 streamfunction (RandomStreamfunction,
+        generated on the device) scaled to an expected RMS velocity of
+        0.1 m/s, and a surface wave over a 50 m mixed layer."""
+        amp = synthetic.random_streamfunction_amplitude(grid, urms=0.1, slope=-3.0)
+        return (qg.RandomStreamfunction(amplitude=amp, spectral_slope=-3.0, seed=self.seed),
+                qg.SurfaceWave(self.wave_amplitude, 50.0))
 
 
 @dataclass
@@ -134,7 +138,7 @@ class Pool:
         slot = len(self.runs)
         self.ens.set_member_stratification(slot, req.strat())
         psi, wave = req.initial_state(self.grid)
-        self.ens.set_member(slot, psi=qg.FieldArray(psi), waves=wave)
+        self.ens.set_member(slot, psi=psi, waves=wave)
         self.runs.append(req)
         self.left.append(req.steps)
         self.admitted.append(now)
@@ -298,7 +302,7 @@ def run_alone(req: RunRequest, precision: str = "mixed", dt: float = 20.0) -> Di
     m = qg.QGYBJModel(grid=grid, coriolis=qg.FPlane(1e-4), stratification=req.strat(),
                       feedback=qg.WaveMeanFeedback(), formulation=qg.YBJPlus(), architecture=qg.GPU(precision))
     psi, wave = req.initial_state(grid)
-    qg.set_(m, psi=qg.FieldArray(psi), waves=wave)
+    qg.set_(m, psi=psi, waves=wave)
     qg.step_(m, qg.ExponentialRungeKutta2(dt=dt), req.steps)
     out = {n: getattr(m.fields, n) for n in ("q", "B", "psi")}
     m.finalize()

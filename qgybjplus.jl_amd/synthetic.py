@@ -3,8 +3,10 @@
 The streamfunction has an isotropic spectrum with energy ∝ k⁻³ beyond a peak
 at horizontal index wavenumber ``kpeak``, Gaussian random phases, the 2/3 mask
 applied, zero mean, a barotropic plus a first-baroclinic-mode part, and is
-scaled to a target RMS velocity.  This is the benchmark input only; the
-reference's own ``RandomStreamfunction`` generator is a different algorithm.
+scaled to a target RMS velocity.  This is the benchmark input of the fixed
+workloads; the serving workload draws the reference's own
+``RandomStreamfunction`` on the device, with the amplitude from
+``random_streamfunction_amplitude``.
 """
 
 from __future__ import annotations
@@ -53,6 +55,24 @@ def k3_streamfunction(grid, seed: int = 2026, urms: float = 0.1, kpeak: float = 
     e = (np.mean(ub * ub + vb * vb) + 2 * np.mean(c) * np.mean(ub * uc + vb * vc) +
          np.mean(c * c) * np.mean(uc * uc + vc * vc))
     return out * (urms / math.sqrt(float(e)))
+
+
+def random_streamfunction_amplitude(grid, urms: float = 0.1, slope: float = -3.0) -> float:
+    """Amplitude of ``RandomStreamfunction`` whose expected RMS velocity
+    (mean of u^2 + v^2 over the grid) is `urms`: each kept mode draws
+    E|psi-hat|^2 = 2 A |k|^slope (k in index units), so by Parseval
+    E mean(u^2 + v^2) = (2 A / (nx ny)^2) sum kh^2 |k|^slope."""
+    nx, ny, _ = grid.size
+    ix = np.fft.fftfreq(nx) * nx
+    iy = np.fft.fftfreq(ny) * ny
+    K = np.sqrt(ix[:, None] ** 2 + iy[None, :] ** 2)
+    kmax = min(nx, ny) // 3
+    jx = np.where(np.arange(nx) <= nx // 2, np.arange(nx), np.arange(nx) - nx)
+    jy = np.where(np.arange(ny) <= ny // 2, np.arange(ny), np.arange(ny) - ny)
+    keep = ((jx[:, None] ** 2 + jy[None, :] ** 2) <= kmax ** 2) & (K > 0)
+    kh2 = grid.kx[:, None] ** 2 + grid.ky[None, :] ** 2
+    s = float(np.sum(np.where(keep, kh2 * np.where(K > 0, K, 1.0) ** slope, 0.0)))
+    return urms * urms * float(nx * ny) ** 2 / (2.0 * s)
 
 
 def exponential_N2(N0sq: float = 1e-4, scale: float = 800.0):

@@ -317,3 +317,49 @@ def test_config1_short_run_is_finite_and_checked():
         m.step(20.0)
     bad, mx = m.check()
     assert not bad and 0 < mx < 1e10
+
+
+def test_mix64_is_splitmix64():
+    """The reference's _mix64 (src/parallel_mpi.jl:1117-1122) is SplitMix64's
+    output function: from state 0 it gives SplitMix64's published sequence."""
+    gamma = 0x9E3779B97F4A7C15
+    expect = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F, 0xF88BB8A8724C81EC]
+    got = [O.mix64((n * gamma) & ((1 << 64) - 1)) for n in range(4)]
+    assert got == expect
+    # _hash_u01 maps into (0, 1] with 53-bit resolution
+    u = O.hash_u01(O.mix64(7), (np.arange(1, 2001), 3, 5), 0x1)
+    assert np.all(u > 0) and np.all(u <= 1) and np.all(u * 2.0 ** 53 == np.round(u * 2.0 ** 53))
+    assert abs(np.mean(u) - 0.5) < 0.03
+
+
+@pytest.mark.parametrize("size", [(16, 16, 3), (24, 12, 2), (32, 32, 1)])
+def test_random_psi_structure(size):
+    """init_mpi_random_psi! (src/parallel_mpi.jl:1159-1216): Hermitian (its
+    inverse transform is real), masked, zero mean, deterministic per seed."""
+    nx, ny, nz = size
+    p = O.random_psi(nx, ny, nz, 2.0, -3.0, 11)
+    phys = np.fft.ifft2(p, axes=(1, 2))
+    assert np.max(np.abs(phys.imag)) <= 1e-15 * np.max(np.abs(phys.real))
+    assert np.all(p[:, ~O.dealias_mask(nx, ny)] == 0)
+    assert np.all(p[:, 0, 0] == 0)
+    assert np.array_equal(p, O.random_psi(nx, ny, nz, 2.0, -3.0, 11))
+    assert not np.array_equal(p, O.random_psi(nx, ny, nz, 2.0, -3.0, 12))
+    # levels draw independently
+    if nz > 1:
+        assert not np.allclose(p[0], p[1])
+    with pytest.raises(ValueError):
+        O.random_psi(nx, ny, nz, 1.0, -3.0, -1)
+
+
+def test_random_psi_energy_law():
+    """Mean |psi-hat|^2 over the modes of one shell is 2*amplitude*k^slope
+    (E[randn^2] = 1): a dropped factor 2 or a wrong exponent fails."""
+    nx = ny = 64
+    p = O.random_psi(nx, ny, 16, 3.0, -2.5, 5)
+    i = np.fft.fftfreq(nx) * nx
+    K = np.sqrt(i[:, None] ** 2 + i[None, :] ** 2)
+    for lo, hi in ((1.0, 8.0), (8.0, 14.0), (14.0, 21.0)):
+        band = (K >= lo) & (K < hi) & O.dealias_mask(nx, ny)
+        vals = np.abs(p[:, band]) ** 2 / (2 * 3.0 * K[band] ** -2.5)
+        # about half the modes are independent draws: sd of the mean < 0.04
+        assert abs(np.mean(vals) - 1.0) < 0.12, (lo, hi, np.mean(vals))
