@@ -171,6 +171,15 @@ int qg_set_random_psi(qg_handle* h, int member, double amplitude, double slope, 
  * mode and zero elsewhere.  A follows from qg_invert_B_to_A[_member]. */
 int qg_set_surface_wave(qg_handle* h, int member, const double* profile);
 
+/* Energy diagnostics: replaces _local_energy_components /
+ * _record_energy_diagnostics! (src/core/io.jl:331-419) on the device.  psi is
+ * recomputed from q (no wave feedback) and A, C from B, on scratch fields (the
+ * model's diagnostics are not touched), then out receives per member
+ * {wave_KE, wave_PE, wave_CE, mean_flow_KE, mean_flow_PE}: 5 values for one
+ * member, 5*members for member == -1.  The sums are reduced in a fixed order,
+ * so repeated calls give identical values. */
+int qg_energies(qg_handle* h, int member, double* out);
+
 /* Replaces one member's N2 and a_ell profiles (nz entries each, computed as
  * for qg_config) and re-records its column eliminations, so a free ensemble
  * slot can take a run with another stratification.  Fails with

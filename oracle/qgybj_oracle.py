@@ -49,6 +49,10 @@ Pins (what checks each function; see ``tests/test_oracle.py``):
   determinism (the reference pins determinism only,
   ``test/test_model_initialization.jl:93-98``).  Its ``log``/``cos``/``pow``
   are the platform's, so exact values can differ from Julia's by an ulp.
+* ``energy_components``: Parseval against physical-space means of the
+  velocity, shear, |grad C|^2 and |lap A|^2, and the wave KE through the
+  L+ relation LA = B + kh^2 A / 4 (reference KAT ``test/test_simulation_lifecycle.jl:108-114``),
+  which reaches it from B and A instead of C.
 
 Parity unpinned: the omega-equation ``w`` *values* of the reference (no
 reference test checks them; the restated gtsv is pinned against a dense solve
@@ -907,3 +911,48 @@ def random_psi(nx: int, ny: int, nz: int, amplitude: float, slope: float, seed: 
         v[(kx == 0) & (ky == 0)] = 0
         out[k] = np.where(mask, v, 0)
     return out
+
+
+# ---------------------------------------------------------------------------
+# Energy diagnostics (src/core/io.jl:331-419)
+# ---------------------------------------------------------------------------
+
+def energy_components(psi, A, C, grid: Grid, a: np.ndarray):
+    """_local_energy_components (src/core/io.jl:331-395) on one model, in the
+    reference's summation order (j, i outer; k inner).  Returns
+    (wave_KE, wave_PE, wave_CE, mean_flow_KE, mean_flow_PE)."""
+    nz, nx, ny = psi.shape
+    dz = abs(grid.z[1] - grid.z[0]) if nz > 1 else 1.0
+    mask = dealias_mask(nx, ny)
+    kh2 = grid.kh2
+    w = np.ones((nx, ny))
+    w[0, 0] = 0.5
+    abs2 = lambda v: v.real * v.real + v.imag * v.imag  # noqa: E731
+    mke = np.zeros((nx, ny)); mpe = np.zeros((nx, ny))
+    wke = np.zeros((nx, ny)); wpe = np.zeros((nx, ny)); wce = np.zeros((nx, ny))
+    for k in range(nz):
+        ak = a[k]
+        mke += w * kh2 * abs2(psi[k])
+        if k < nz - 1:
+            mpe += w * ak * abs2((psi[k + 1] - psi[k]) / dz)
+        if nz == 1:
+            LA = np.zeros_like(C[k])
+        elif k == 0:
+            LA = ak * C[k] / dz
+        elif k == nz - 1:
+            LA = -a[k - 1] * C[k - 1] / dz
+        else:
+            LA = (ak * C[k] - a[k - 1] * C[k - 1]) / dz
+        wke += w * abs2(LA)
+        wpe += w * (0.5 / ak) * kh2 * abs2(C[k])
+        wce += w * (1 / (8 * ak ** 2)) * kh2 ** 2 * abs2(A[k])
+    norm = 0.5 / ((nx * ny) ** 2 * nz)
+    return tuple(float(np.sum(np.where(mask, t, 0.0))) * norm for t in (wke, wpe, wce, mke, mpe))
+
+
+def record_energies(q, B, grid: Grid, a: np.ndarray):
+    """_record_energy_diagnostics! (src/core/io.jl:398-419): psi from q
+    without the wave feedback, A and C from B, then the five energies."""
+    psi = invert_q_to_psi(q, grid, a)
+    A, C = invert_B_to_A(B, grid, a)
+    return energy_components(psi, A, C, grid, a)

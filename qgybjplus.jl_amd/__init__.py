@@ -12,7 +12,8 @@ from .api import (  # noqa: F401
     PassiveWave, Dissipative, Inviscid, NonlinearDynamics, LinearDynamics, Dispersive,
     NoDispersion, FlowHyperdiffusivity, WaveHyperdiffusivity, HorizontalHyperdiffusivity,
     VerticalDiffusivity, SurfaceWave, RandomStreamfunction, FieldArray, GPU, QGYBJModel, QGYBJEnsemble, finalize_model_, set_,
-    ExponentialRungeKutta2, step_, Simulation, SimulationState, Clock, run_,
+    ExponentialRungeKutta2, step_, Simulation, SimulationState, Clock, run_, IterationInterval, TimeInterval,
+    EnergyDiagnostics, ENERGY_NAMES,
     check_termination_, finalize_simulation_,
 )
 from . import synthetic, serving  # noqa: F401,E402

@@ -39,7 +39,7 @@ EXPORTED = (
     "qg_compute_q_from_psi_member", "qg_invert_B_to_A_member", "qg_use_graphs",
     "qg_set_member_stratification", "qg_memory_required", "qg_device_memory",
     "qg_set_active_members", "qg_move_member", "qg_autotune", "qg_get_tuning", "qg_set_tuning",
-    "qg_set_random_psi", "qg_set_surface_wave",
+    "qg_set_random_psi", "qg_set_surface_wave", "qg_energies",
 )
 
 
@@ -123,6 +123,7 @@ def load() -> C.CDLL:
         "qg_autotune": (C.c_int, [P, C.c_double, C.c_int, C.c_int]),
         "qg_set_random_psi": (C.c_int, [P, C.c_int, C.c_double, C.c_double, C.c_longlong]),
         "qg_set_surface_wave": (C.c_int, [P, C.c_int, dp]),
+        "qg_energies": (C.c_int, [P, C.c_int, dp]),
         "qg_get_tuning": (C.c_int, [P, C.POINTER(C.c_int), C.c_int]),
         "qg_set_tuning": (C.c_int, [P, C.POINTER(C.c_int), C.c_int]),
     }
@@ -335,6 +336,13 @@ class Handle:
     def set_surface_wave(self, profile, member: int = -1):
         p = np.ascontiguousarray(profile, dtype=np.float64)
         self._check(self._lib.qg_set_surface_wave(self.h, int(member), _dptr(p)))
+
+    def energies(self, member: int = -1) -> np.ndarray:
+        """(members or 1, 5): wave KE, PE, CE, mean-flow KE, PE."""
+        n = self.members if member == -1 else 1
+        out = np.zeros(5 * n)
+        self._check(self._lib.qg_energies(self.h, int(member), _dptr(out)))
+        return out.reshape(n, 5)
 
     # -- autotuner ---------------------------------------------------------------
     def autotune(self, dt: float, steps: int = 3, force: bool = False) -> dict:

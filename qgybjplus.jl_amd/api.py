@@ -459,6 +459,11 @@ class QGYBJModel:
     def compute_qw(self) -> np.ndarray:
         return self._h.compute_qw()
 
+    def energies(self) -> dict:
+        """The five energies of _local_energy_components (src/core/io.jl:331-395),
+        reduced on the device from psi = L^-1 q and A, C from B."""
+        return dict(zip(ENERGY_NAMES, (float(v) for v in self._h.energies(-1)[0])))
+
     def autotune(self, dt: float, steps: int = 3, force: bool = False) -> dict:
         """Choose the FFT-pass tile variants and column-solve launch shapes by
         timing `steps` steps of size `dt` per candidate on this model's state
@@ -569,6 +574,9 @@ class QGYBJEnsemble(QGYBJModel):
 
     def check_member(self, member: int):
         return self._h.check_member(member)
+
+    def energies_member(self, member: int) -> dict:
+        return dict(zip(ENERGY_NAMES, (float(v) for v in self._h.energies(member)[0])))
 
     def set_active_members(self, active: int):
         """Step only members [0, active) from now on."""
@@ -716,11 +724,80 @@ def step_(model: QGYBJModel, timestepper: ExponentialRungeKutta2, nsteps: int = 
 
 
 class SimulationState(enum.Enum):
+    """``SimulationState`` (src/core/simulation.jl:9-16)."""
     Ready = 0
     Running = 1
-    Completed = 2
+    Stopped = 2
     Failed = 3
     Finalized = 4
+
+
+@dataclass(frozen=True)
+class IterationInterval:
+    """Every `interval` iterations (src/core/components.jl:297-305)."""
+    interval: int
+
+    def __post_init__(self):
+        if not int(self.interval) > 0:
+            raise ValueError(f"interval must be positive (got {self.interval})")
+        object.__setattr__(self, "interval", int(self.interval))
+
+
+@dataclass(frozen=True)
+class TimeInterval:
+    """Every `interval` seconds of model time (src/core/components.jl:285-296)."""
+    interval: float
+
+    def __post_init__(self):
+        v = float(self.interval)
+        if not math.isfinite(v):
+            raise ValueError(f"interval must be finite (got {self.interval})")
+        if not v > 0:
+            raise ValueError(f"interval must be positive (got {self.interval})")
+        object.__setattr__(self, "interval", v)
+
+
+ENERGY_NAMES = ("wave_KE", "wave_PE", "wave_CE", "mean_flow_KE", "mean_flow_PE")
+
+
+class EnergyDiagnostics:
+    """``EnergyDiagnosticsManager`` (src/core/io.jl:243-326,398-431) kept in
+    memory: the time series of the five energies, recorded on the device by
+    ``qg_energies`` when the schedule is due (the NetCDF writer is not part of
+    this library)."""
+
+    def __init__(self, schedule, start_time: float = 0.0):
+        if not isinstance(schedule, (IterationInterval, TimeInterval)):
+            raise ValueError("simulation diagnostics must be an IterationInterval, TimeInterval, or False")
+        self.schedule = schedule
+        self.last_iteration = None
+        self.next_time = start_time + schedule.interval if isinstance(schedule, TimeInterval) else None
+        self.series = {"time": [], **{n: [] for n in ENERGY_NAMES}}
+
+    def due(self, clock, initial: bool = False) -> bool:
+        """``_diagnostics_due`` (src/core/io.jl:299-314)."""
+        if self.last_iteration == clock.iteration:
+            return False
+        if initial:
+            return True
+        if isinstance(self.schedule, IterationInterval):
+            return clock.iteration % self.schedule.interval == 0
+        tol = 8 * np.finfo(float).eps * max(abs(clock.time), abs(self.schedule.interval), 1.0)
+        return self.next_time is not None and clock.time >= self.next_time - tol
+
+    def record(self, model, clock):
+        """``_record_energy_diagnostics!`` (src/core/io.jl:398-431)."""
+        e = model.energies()
+        self.series["time"].append(clock.time)
+        for n in ENERGY_NAMES:
+            self.series[n].append(e[n])
+        self.last_iteration = clock.iteration
+        if isinstance(self.schedule, TimeInterval):  # _advance_diagnostics_deadline!
+            if self.next_time is None:
+                self.next_time = clock.time + self.schedule.interval
+            tol = 8 * np.finfo(float).eps * max(abs(clock.time), abs(self.schedule.interval), 1.0)
+            while self.next_time <= clock.time + tol:
+                self.next_time += self.schedule.interval
 
 
 @dataclass
@@ -733,10 +810,12 @@ class Simulation:
     """``Simulation(model; Δt, stop_time, stop_iteration)`` (src/simulation.jl:204-227)."""
 
     def __init__(self, model: QGYBJModel, *, dt: float = 1.0, stop_time=None, stop_iteration=None,
-                 check_every: int = 1):
+                 check_every: int = 1, diagnostics=False):
         if model.finalized:
             raise RuntimeError("cannot create a simulation from a finalized model")
         self.model = model
+        self.diagnostics = diagnostics  # False, IterationInterval or TimeInterval
+        self.diagnostics_manager: Optional[EnergyDiagnostics] = None
         self.timestepper = ExponentialRungeKutta2(dt=dt)
         self.clock = Clock()
         self.stop_time = stop_time
@@ -770,6 +849,12 @@ def run_(simulation: Simulation, *, stop_iteration=None, stop_time=None, progres
     """``run!(simulation)`` (src/simulation.jl:570-662) without output writers."""
     if simulation.state is SimulationState.Finalized:
         raise RuntimeError("cannot run a finalized simulation")
+    if simulation.state is SimulationState.Running:
+        raise RuntimeError("simulation is already running")
+    if simulation.state is SimulationState.Failed:
+        raise RuntimeError("simulation is in a failed state")
+    if simulation.state is SimulationState.Stopped:
+        raise RuntimeError("simulation has stopped; construct a new simulation to continue")
     if stop_iteration is not None:
         simulation.stop_iteration = stop_iteration
     if stop_time is not None:
@@ -780,6 +865,12 @@ def run_(simulation: Simulation, *, stop_iteration=None, stop_time=None, progres
     model = simulation.model
     ts = simulation.timestepper
     try:
+        if simulation.diagnostics is not False and simulation.diagnostics is not None:
+            if simulation.diagnostics_manager is None:
+                simulation.diagnostics_manager = EnergyDiagnostics(simulation.diagnostics, simulation.clock.time)
+        mgr = simulation.diagnostics_manager
+        if mgr is not None and mgr.due(simulation.clock, initial=True):
+            mgr.record(model, simulation.clock)
         while True:
             it = simulation.clock.iteration
             if simulation.stop_iteration is not None and it >= simulation.stop_iteration:
@@ -791,9 +882,11 @@ def run_(simulation: Simulation, *, stop_iteration=None, stop_time=None, progres
                 check_termination_(simulation)
             simulation.clock.iteration += 1
             simulation.clock.time += ts.dt
+            if mgr is not None and mgr.due(simulation.clock):
+                mgr.record(model, simulation.clock)
             if progress:
                 print(f"iteration={simulation.clock.iteration} | time={simulation.clock.time} s", flush=True)
-        simulation.state = SimulationState.Completed
+        simulation.state = SimulationState.Stopped
     except Exception:
         simulation.state = SimulationState.Failed
         raise

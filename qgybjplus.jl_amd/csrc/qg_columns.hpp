@@ -42,6 +42,12 @@ void launch_omega(const Geo& g, hipStream_t st, const cplx<double>* rhsw, cplx<d
 void launch_random_psi(const Geo& g, hipStream_t st, int m0, int nm, unsigned long long seed, double amplitude,
                        double slope, cplx<double>* psi);
 void launch_surface_wave(const Geo& g, hipStream_t st, int m0, int nm, const double* profile, cplx<double>* B);
+// Energy diagnostics (qg_diag.hip) of members [m0, m0 + nmem): out receives
+// 5 values per member (wave KE, PE, CE, mean-flow KE, PE); partial holds
+// energy_partial_size(g, nmem) doubles.
+size_t energy_partial_size(const Geo& g, int nmem);
+void launch_energies(const Geo& g, hipStream_t st, int m0, int nmem, const cplx<double>* psi,
+                     const cplx<double>* A, const cplx<double>* C, const double* a, double* partial, double* out);
 void launch_q_from_psi(const Geo& g, hipStream_t st, const cplx<double>* psi, cplx<double>* q,
                        const double* a, int barotropic);
 template <typename T>

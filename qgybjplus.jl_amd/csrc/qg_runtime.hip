@@ -65,6 +65,7 @@ struct qg_handle {
   std::vector<void*> allocs;
   // device
   double* d_prof = nullptr;  // a vertical profile per stacked level (set-up input)
+  double *d_epart = nullptr, *d_energy = nullptr;  // energy reduction partials, results
   double *d_kx = nullptr, *d_ky = nullptr, *d_a = nullptr, *d_al = nullptr, *d_cz = nullptr, *d_cu = nullptr,
          *d_fN2 = nullptr;
   unsigned char* d_mask = nullptr;
@@ -1027,6 +1028,38 @@ int qg_set_surface_wave(qg_handle* h, int member, const double* profile) {
     HIPCHECK(hipStreamSynchronize(H.st));
     H.diag_valid = false;
     H.t1_valid = false;
+  });
+}
+
+int qg_energies(qg_handle* h, int member, double* out) {
+  return guarded(h, [&] {
+    qg_handle& H = *h;
+    int m0, nm;
+    member_span(H, member, m0, nm);
+    if (!out) throw QgError(QG_ERR_ARGUMENT, "out is required");
+    if (H.sing_psi[0]) raise_singular(H.sing_psi);
+    if (H.sing_A[0]) raise_singular(H.sing_A);
+    qg::Geo all = H.g;
+    all.nz = H.nz;
+    all.members = H.members;
+    if (!H.d_epart) {
+      H.d_epart = H.alloc<double>(qg::energy_partial_size(all, H.members));
+      H.d_energy = H.alloc<double>(5 * (size_t)H.members);
+    }
+    // _record_energy_diagnostics! (src/core/io.jl:405-419): psi from q
+    // without the wave feedback, A and C from B, on scratch fields
+    const MemberView mv(H, m0);
+    qg::Geo g = mv.g;
+    g.members = nm;
+    g.nz = nm * H.nzc;
+    qg::launch_thomas(g, H.st, H.q[H.cur] + mv.off, H.psis + mv.off, nullptr, H.th_inv[0] + mv.toff,
+                      H.th_cp[0] + mv.toff, H.d_al + mv.aoff, 1, nullptr, H.tune[qg::P_THOMAS_PSI]);
+    qg::launch_thomas(g, H.st, H.B[H.cur] + mv.off, H.As + mv.off, H.qeff + mv.off, H.th_inv[1] + mv.toff,
+                      H.th_cp[1] + mv.toff, H.d_al + mv.aoff, 4, nullptr, H.tune[qg::P_THOMAS_AC]);
+    qg::launch_energies(all, H.st, m0, nm, H.psis, H.As, H.qeff, H.d_a, H.d_epart, H.d_energy);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(out, H.d_energy, 5 * (size_t)nm * sizeof(double), hipMemcpyDeviceToHost, H.st));
+    HIPCHECK(hipStreamSynchronize(H.st));
   });
 }
 

@@ -363,3 +363,54 @@ def test_random_psi_energy_law():
         vals = np.abs(p[:, band]) ** 2 / (2 * 3.0 * K[band] ** -2.5)
         # about half the modes are independent draws: sd of the mean < 0.04
         assert abs(np.mean(vals) - 1.0) < 0.12, (lo, hi, np.mean(vals))
+
+
+def _energy_case(seed=3, size=(16, 16, 6)):
+    nx, ny, nz = size
+    g = O.make_grid(size, extent=(500e3, 500e3, 4000.0), centered=True)
+    N2 = O.stratification_profile(g, lambda z: 1e-4 * math.exp(z / 800.0))
+    a = O.a_ell_from_N2(N2, 1e-4)
+    rng = np.random.default_rng(seed)
+    mask = O.dealias_mask(nx, ny)
+
+    def field(scale):
+        f = np.fft.fft2(rng.normal(size=(nz, nx, ny)), axes=(1, 2)) * scale
+        f = np.where(mask[None], f, 0)
+        f[:, 0, 0] = 0  # zero horizontal mean: Parseval without the kh = 0 weight
+        return f
+    return g, a, field(1e3), field(0.1)
+
+
+def test_energy_components_parseval():
+    """Each energy equals the physical-space mean of its density (Parseval,
+    numpy's ifft convention), and the wave KE is reached through the L+
+    relation LA = B + kh^2 A / 4 from B and A instead of C."""
+    g, a, psi, B = _energy_case()
+    A, C = O.invert_B_to_A(B, g, a)
+    wke, wpe, wce, mke, mpe = O.energy_components(psi, A, C, g, a)
+    nz = psi.shape[0]
+    ifft = lambda f: np.fft.ifft2(f, axes=(-2, -1))  # noqa: E731
+    kx, ky = g.kx[:, None], g.ky[None, :]
+    u, v = ifft(-1j * ky * psi), ifft(1j * kx * psi)
+    assert math.isclose(mke, 0.5 * np.mean(np.abs(u) ** 2 + np.abs(v) ** 2), rel_tol=1e-12)
+    pz = ifft(np.diff(psi, axis=0) / g.dz)
+    assert math.isclose(mpe, 0.5 / nz * sum(a[k] * np.mean(np.abs(pz[k]) ** 2) for k in range(nz - 1)),
+                        rel_tol=1e-12)
+    cx, cy = ifft(1j * kx * C), ifft(1j * ky * C)
+    assert math.isclose(wpe, 0.5 / nz * sum(np.mean(0.5 / a[k] * (np.abs(cx[k]) ** 2 + np.abs(cy[k]) ** 2))
+                                            for k in range(nz)), rel_tol=1e-12)
+    lapA = ifft(-g.kh2 * A)
+    assert math.isclose(wce, 0.5 / nz * sum(np.mean(np.abs(lapA[k]) ** 2) / (8 * a[k] ** 2) for k in range(nz)),
+                        rel_tol=1e-12)
+    LA = B + g.kh2 * A / 4
+    assert math.isclose(wke, 0.5 * np.mean(np.abs(ifft(LA)) ** 2), rel_tol=1e-9)
+
+
+def test_record_energies_uses_plain_q():
+    """_record_energy_diagnostics! inverts q itself (no wave feedback)."""
+    g, a, psi, B = _energy_case(5)
+    q = O.compute_q_from_psi(psi, g, a)
+    e = O.record_energies(q, B, g, a)
+    A, C = O.invert_B_to_A(B, g, a)
+    e2 = O.energy_components(psi, A, C, g, a)
+    assert np.allclose(e, e2, rtol=1e-10, atol=0)
