@@ -8,29 +8,35 @@ namespace qg {
 
 namespace {
 
-template <int OP, int M, int QC>
+template <int OP, int M, int QC, typename T>
 void launch_shape(const Geo& g, hipStream_t st, const ScanArgs& a) {
   const Quarter qr = quarter(g);
   const ScanShape s{M, QC, (g.nzc + M - 1) / M};
   const int blocks = (qr.size() + QC - 1) / QC;
-  hipLaunchKernelGGL((k_colscan<OP, M, QC>), dim3(blocks, g.members), dim3(s.threads()), s.lds_bytes(), st, g, qr,
-                     a);
+  hipLaunchKernelGGL((k_colscan<OP, M, QC, T>), dim3(blocks, g.members), dim3(s.threads()), s.lds_bytes(), st, g,
+                     qr, a);
 }
 
 template <int OP, int I>
-void launch_idx(const Geo& g, hipStream_t st, int idx, const ScanArgs& a) {
+void launch_idx(const Geo& g, hipStream_t st, int idx, const ScanArgs& a, bool f32) {
   if constexpr (I < kScanShapeCount) {
-    if (idx == I) launch_shape<OP, kScanShapes[I][0], kScanShapes[I][1]>(g, st, a);
-    else launch_idx<OP, I + 1>(g, st, idx, a);
+    if (idx == I) {
+      if (f32) launch_shape<OP, kScanShapes[I][0], kScanShapes[I][1], float>(g, st, a);
+      else launch_shape<OP, kScanShapes[I][0], kScanShapes[I][1], double>(g, st, a);
+    } else {
+      launch_idx<OP, I + 1>(g, st, idx, a, f32);
+    }
   }
 }
 
 template <int OP, int I>
 bool setup_idx() {
   if constexpr (I < kScanShapeCount) {
-    const void* f = reinterpret_cast<const void*>(k_colscan<OP, kScanShapes[I][0], kScanShapes[I][1]>);
-    if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kScanMaxLds) != hipSuccess)
-      return false;
+    const void* f64 = reinterpret_cast<const void*>(k_colscan<OP, kScanShapes[I][0], kScanShapes[I][1], double>);
+    const void* f32 = reinterpret_cast<const void*>(k_colscan<OP, kScanShapes[I][0], kScanShapes[I][1], float>);
+    for (const void* f : {f64, f32})
+      if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kScanMaxLds) != hipSuccess)
+        return false;
     return setup_idx<OP, I + 1>();
   }
   return true;
@@ -39,14 +45,14 @@ bool setup_idx() {
 }  // namespace
 
 template <int OP>
-void launch_colscan(const Geo& g, hipStream_t st, int idx, const ScanArgs& a) {
-  launch_idx<OP, 0>(g, st, idx, a);
+void launch_colscan(const Geo& g, hipStream_t st, int idx, const ScanArgs& a, bool f32) {
+  launch_idx<OP, 0>(g, st, idx, a, f32);
 }
 
-template void launch_colscan<SCAN_PSI>(const Geo&, hipStream_t, int, const ScanArgs&);
-template void launch_colscan<SCAN_A>(const Geo&, hipStream_t, int, const ScanArgs&);
-template void launch_colscan<SCAN_AC>(const Geo&, hipStream_t, int, const ScanArgs&);
-template void launch_colscan<SCAN_OMEGA>(const Geo&, hipStream_t, int, const ScanArgs&);
+template void launch_colscan<SCAN_PSI>(const Geo&, hipStream_t, int, const ScanArgs&, bool);
+template void launch_colscan<SCAN_A>(const Geo&, hipStream_t, int, const ScanArgs&, bool);
+template void launch_colscan<SCAN_AC>(const Geo&, hipStream_t, int, const ScanArgs&, bool);
+template void launch_colscan<SCAN_OMEGA>(const Geo&, hipStream_t, int, const ScanArgs&, bool);
 
 bool colscan_setup() {
   static const bool ok = setup_idx<SCAN_PSI, 0>() && setup_idx<SCAN_A, 0>() && setup_idx<SCAN_AC, 0>() &&

@@ -108,10 +108,12 @@ inline int scan_default_shape(int nz) {
   return -1;
 }
 
+// Field pointers are cplx<T>: T = double, or float for fields stored in FP32
+// (every recurrence runs in FP64 either way).
 struct ScanArgs {
-  const cplx<double>* rhs;  // member-0 level-0 of the right-hand side
-  cplx<double>* out;
-  cplx<double>* Cout;       // SCAN_AC: C = dA/dz
+  const void* rhs;          // member-0 level-0 of the right-hand side
+  void* out;
+  void* Cout;               // SCAN_AC: C = dA/dz
   const double* t0;         // Thomas: 1/pivot; omega: row multiplier
   const double* t1;         // Thomas: c'; omega: 1/pivot
   const double* lv0;        // Thomas: a_{k-1}; omega: upper diagonal by level
@@ -136,8 +138,24 @@ struct ScanBounds {
   static constexpr int MAXT = MAXP * TP;
 };
 
+// Loads and stores of one column part of storage type T as FP64.
+template <typename T>
+__device__ inline double scan_load(__amdgpu_buffer_rsrc_t r, unsigned off, int soff) {
+  if constexpr (sizeof(T) == 8) return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
+  else return (double)__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
+}
+template <typename T>
+__device__ inline void scan_store(double v, __amdgpu_buffer_rsrc_t r, unsigned off, int soff) {
+  if constexpr (sizeof(T) == 8) {
+    using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, off, soff, 0);
+  } else {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, (float)v), r, off, soff, 0);
+  }
+}
+
 // blockIdx.x: tile of QC quarter columns; blockIdx.y: member.
-template <int OP, int M, int QC>
+template <int OP, int M, int QC, typename T>
 __global__ void __launch_bounds__((ScanBounds<M, QC>::MAXT))
 k_colscan(Geo g, Quarter qr, ScanArgs a) {
   constexpr int TP = 8 * QC;
@@ -152,8 +170,8 @@ k_colscan(Geo g, Quarter qr, ScanArgs a) {
   const int rc0 = blockIdx.x * QC;
   const size_t plane = (size_t)g.nx * g.ny;
   const size_t mem = blockIdx.y;
-  const double* rhs = reinterpret_cast<const double*>(a.rhs + mem * nz * plane);
-  double* out = reinterpret_cast<double*>(a.out + mem * nz * plane);
+  const T* rhs = static_cast<const T*>(a.rhs) + 2 * mem * nz * plane;
+  T* out = static_cast<T*>(a.out) + 2 * mem * nz * plane;
   const double* t0 = a.t0 + mem * nz * rplane;
   const double* t1 = a.t1 + mem * nz * rplane;
   const double* lv0 = a.lv0 + mem * a.lv0_stride;
@@ -172,10 +190,10 @@ k_colscan(Geo g, Quarter qr, ScanArgs a) {
   // parts per partition a partition is one wave, so its first level is
   // wave-uniform and every level's address is a scalar base plus this offset.
   constexpr bool UNI = TP % 64 == 0;
-  const unsigned boff = 8u * (2u * (q.active ? (unsigned)(q.y * g.nx + q.x) : 0u) + (unsigned)part);
+  const unsigned boff = (unsigned)sizeof(T) * (2u * (q.active ? (unsigned)(q.y * g.nx + q.x) : 0u) + (unsigned)part);
   int k0 = w * M;
   if constexpr (UNI) k0 = __builtin_amdgcn_readfirstlane(k0);
-  const size_t lvlb = plane * sizeof(cplx<double>);  // bytes per level
+  const size_t lvlb = plane * 2 * sizeof(T);  // bytes per level
   // Wave-uniform partitions address their levels through a buffer resource
   // over the partition's levels inside the field: one 32-bit lane offset and
   // a scalar offset per level; levels past nz read as zero and are not
@@ -186,9 +204,9 @@ k_colscan(Geo g, Quarter qr, ScanArgs a) {
                                              0, (int)(nlev * lvlb), kRawBufferWord3);
   };
   auto at_level = [&](auto* base, int k) {
-    using T = std::remove_pointer_t<decltype(base)>;
-    using Byte = std::conditional_t<std::is_const_v<T>, const char, char>;
-    return reinterpret_cast<T*>(reinterpret_cast<Byte*>(base + (size_t)k * (2 * plane)) + boff);
+    using U = std::remove_pointer_t<decltype(base)>;
+    using Byte = std::conditional_t<std::is_const_v<U>, const char, char>;
+    return reinterpret_cast<U*>(reinterpret_cast<Byte*>(base + (size_t)k * (2 * plane)) + boff);
   };
 
   // Loads: the tables of the tile first, then this thread's right-hand side
@@ -212,11 +230,10 @@ k_colscan(Geo g, Quarter qr, ScanArgs a) {
   if constexpr (UNI) {
     const auto rs = rsrc(rhs);
 #pragma unroll
-    for (int u = 0; u < M; ++u)
-      y[u] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, boff, u * (int)lvlb, 0));
+    for (int u = 0; u < M; ++u) y[u] = scan_load<T>(rs, boff, u * (int)lvlb);
   } else {
 #pragma unroll
-    for (int u = 0; u < M; ++u) y[u] = *at_level(rhs, min(k0 + u, nz - 1));
+    for (int u = 0; u < M; ++u) y[u] = (double)*at_level(rhs, min(k0 + u, nz - 1));
   }
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -347,8 +364,7 @@ k_colscan(Geo g, Quarter qr, ScanArgs a) {
   if (!q.active) return;
   if (OP == SCAN_OMEGA && a.pivoted[mem * rplane + rc0 + j]) return;  // k_omega_pivoted
   const bool zero_col = OP == SCAN_PSI && nz > 1 && mean_col;
-  double* Co = (OP == SCAN_AC) ? reinterpret_cast<double*>(a.Cout + mem * nz * plane) : nullptr;
-  using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(rsrc(out), 0, 0, 0));
+  T* Co = (OP == SCAN_AC) ? static_cast<T*>(a.Cout) + 2 * mem * nz * plane : nullptr;
   const auto ro = rsrc(out);
   const auto rcz = rsrc((OP == SCAN_AC) ? Co : out);
 #pragma unroll
@@ -362,20 +378,20 @@ k_colscan(Geo g, Quarter qr, ScanArgs a) {
         c = k < nz - 1 ? (up - r) / g.dz : 0.0;
       }
       if constexpr (UNI) {
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, r), ro, boff, u * (int)lvlb, 0);
-        if constexpr (OP == SCAN_AC)
-          __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, c), rcz, boff, u * (int)lvlb, 0);
+        scan_store<T>(r, ro, boff, u * (int)lvlb);
+        if constexpr (OP == SCAN_AC) scan_store<T>(c, rcz, boff, u * (int)lvlb);
       } else {
-        *at_level(out, k) = r;
-        if constexpr (OP == SCAN_AC) *at_level(Co, k) = c;
+        *at_level(out, k) = (T)r;
+        if constexpr (OP == SCAN_AC) *at_level(Co, k) = (T)c;
       }
     }
   }
 }
 
-// Host side: launch OP with compiled shape `idx` (see kScanShapes).
+// Host side: launch OP with compiled shape `idx` (see kScanShapes) on fields
+// stored in FP32 (f32) or FP64.
 template <int OP>
-void launch_colscan(const Geo& g, hipStream_t st, int idx, const ScanArgs& a);
+void launch_colscan(const Geo& g, hipStream_t st, int idx, const ScanArgs& a, bool f32);
 // Raise the dynamic LDS limit of every instantiation (once per process).
 bool colscan_setup();
 

@@ -14,21 +14,32 @@ namespace {
 
 // q = L psi (compute_q_from_psi!, initialization.jl:597-644) or the barotropic
 // q = -kh2 psi (initialization.jl:653-668).
-__global__ void k_q_from_psi(Geo g, const cplx<double>* __restrict__ psi, cplx<double>* __restrict__ q,
+template <typename TI, typename TO>
+__global__ void k_q_from_psi(Geo g, const cplx<TI>* __restrict__ psi_s, cplx<TO>* __restrict__ q_s,
                              const double* __restrict__ a, int barotropic) {
   const int plane = g.nx * g.ny;
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= plane) return;
   const int x = col % g.nx, y = col / g.nx;
   const int nz = g.nzc;
-  {
-    const size_t m = blockIdx.y;  // member
-    psi += m * nz * plane;
-    q += m * nz * plane;
-    a += m * nz;
-  }
+  const size_t m = blockIdx.y;  // member
+  psi_s += m * nz * plane;
+  q_s += m * nz * plane;
+  a += m * nz;
   const double kh2 = g.kx[x] * g.kx[x] + g.ky[y] * g.ky[y];
   auto at = [&](int k) { return (size_t)k * plane + col; };
+  struct Ld {
+    const cplx<TI>* p;
+    __device__ cplx<double> operator[](size_t i) const { return cvt<double>(p[i]); }
+  } psi{psi_s};
+  struct St {
+    cplx<TO>* p;
+    struct Ref {
+      cplx<TO>* e;
+      __device__ void operator=(cplx<double> v) { *e = cvt<TO>(v); }
+    };
+    __device__ Ref operator[](size_t i) const { return Ref{p + i}; }
+  } q{q_s};
   if (barotropic || nz == 1) {
     for (int k = 0; k < nz; ++k) {
       cplx<double> p = psi[at(k)];
@@ -97,12 +108,13 @@ __global__ void k_convert(const From* __restrict__ in, To* __restrict__ out, siz
 }
 
 // Any non-finite value among n complex entries -> flag.
-__global__ void k_nonfinite(const cplx<double>* __restrict__ a, size_t n, int* __restrict__ flag) {
+template <typename T>
+__global__ void k_nonfinite(const cplx<T>* __restrict__ a, size_t n, int* __restrict__ flag) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t stride = (size_t)gridDim.x * blockDim.x;
   bool bad = false;
   for (; i < n; i += stride) {
-    cplx<double> v = a[i];
+    cplx<T> v = a[i];
     bad |= !(isfinite(v.re) && isfinite(v.im));
   }
   if (bad) atomicOr(flag, 1);
@@ -126,11 +138,20 @@ void launch_zero_bytes(void* p, size_t bytes, hipStream_t st) {
 
 void launch_zero(cplx<double>* p, size_t n, hipStream_t st) { launch_zero_bytes(p, n * sizeof(cplx<double>), st); }
 
-void launch_q_from_psi(const Geo& g, hipStream_t st, const cplx<double>* psi, cplx<double>* q,
-                       const double* a, int barotropic) {
+void launch_q_from_psi(const Geo& g, hipStream_t st, const void* psi, void* q, bool f32, const double* a,
+                       int barotropic, bool psi_f64) {
   const int plane = g.nx * g.ny;
   const int bs = 64;
-  hipLaunchKernelGGL(k_q_from_psi, dim3((plane + bs - 1) / bs, g.members), dim3(bs), 0, st, g, psi, q, a, barotropic);
+  const dim3 grid((plane + bs - 1) / bs, g.members);
+  if (f32 && psi_f64)
+    hipLaunchKernelGGL((k_q_from_psi<double, float>), grid, dim3(bs), 0, st, g,
+                       static_cast<const cplx<double>*>(psi), static_cast<cplx<float>*>(q), a, barotropic);
+  else if (f32)
+    hipLaunchKernelGGL((k_q_from_psi<float, float>), grid, dim3(bs), 0, st, g, static_cast<const cplx<float>*>(psi),
+                       static_cast<cplx<float>*>(q), a, barotropic);
+  else
+    hipLaunchKernelGGL((k_q_from_psi<double, double>), grid, dim3(bs), 0, st, g,
+                       static_cast<const cplx<double>*>(psi), static_cast<cplx<double>*>(q), a, barotropic);
 }
 
 template <typename T>
@@ -150,8 +171,13 @@ void launch_convert(const From* in, To* out, size_t n, hipStream_t st) {
 template void launch_convert<float, double>(const float*, double*, size_t, hipStream_t);
 template void launch_convert<double, float>(const double*, float*, size_t, hipStream_t);
 
-void launch_nonfinite(const cplx<double>* a, size_t n, int* flag, hipStream_t st) {
-  hipLaunchKernelGGL(k_nonfinite, dim3(1024), dim3(256), 0, st, a, n, flag);
+void launch_nonfinite(const void* a, bool f32, size_t n, int* flag, hipStream_t st) {
+  if (f32)
+    hipLaunchKernelGGL(k_nonfinite<float>, dim3(1024), dim3(256), 0, st, static_cast<const cplx<float>*>(a), n,
+                       flag);
+  else
+    hipLaunchKernelGGL(k_nonfinite<double>, dim3(1024), dim3(256), 0, st, static_cast<const cplx<double>*>(a), n,
+                       flag);
 }
 
 }  // namespace qg

@@ -21,9 +21,11 @@ bool thomas_setup();
 // derived) are concatenated in member order.
 int thomas_shifted_a_size(int nzc);
 // Thomas solves of all columns; C (= dA/dz) is written when Cout is set.
-void launch_thomas(const Geo& g, hipStream_t st, const cplx<double>* rhs, cplx<double>* out,
-                   cplx<double>* Cout, const double* inv, const double* cp, const double* al, int kdiv,
-                   unsigned long long* warn, int shape = -1);
+// Field arguments (rhs, out, Cout, and below psi, q, B, A, C) are cplx<float>
+// when f32 is set and cplx<double> otherwise; the solves run in FP64.
+void launch_thomas(const Geo& g, hipStream_t st, const void* rhs, void* out, void* Cout, bool f32,
+                   const double* inv, const double* cp, const double* al, int kdiv, unsigned long long* warn,
+                   int shape = -1);
 // QG omega-equation interior solves: elimination recorded once per model
 // (tables of thomas_table_size() entries, pivoted flags per quarter column,
 // err[3] set when any column interchanged rows), then applied to every
@@ -33,28 +35,30 @@ void launch_thomas(const Geo& g, hipStream_t st, const cplx<double>* rhs, cplx<d
 void launch_omega_factor(const Geo& g, hipStream_t st, const double* cz, double* M, double* D,
                          unsigned char* pivoted, int* err);
 bool omega_setup();
-void launch_omega(const Geo& g, hipStream_t st, const cplx<double>* rhsw, cplx<double>* wk, const double* M,
+void launch_omega(const Geo& g, hipStream_t st, const void* rhsw, void* wk, bool f32, const double* M,
                   const double* D, const unsigned char* pivoted, bool any_pivoted, const double* cz,
                   const double* cu, const double* fN2, double* sd, double* sdu, double* sfill, int shape = -1);
 // Initial conditions on the device (qg_init.hip): the seeded random
 // streamfunction of members [m0, m0 + nm) and the spectral surface wave
 // (profile: nzc values per member, the amplitude times the vertical profile).
 void launch_random_psi(const Geo& g, hipStream_t st, int m0, int nm, unsigned long long seed, double amplitude,
-                       double slope, cplx<double>* psi);
-void launch_surface_wave(const Geo& g, hipStream_t st, int m0, int nm, const double* profile, cplx<double>* B);
+                       double slope, void* psi, bool f32);
+void launch_surface_wave(const Geo& g, hipStream_t st, int m0, int nm, const double* profile, void* B, bool f32);
 // Energy diagnostics (qg_diag.hip) of members [m0, m0 + nmem): out receives
 // 5 values per member (wave KE, PE, CE, mean-flow KE, PE); partial holds
 // energy_partial_size(g, nmem) doubles.
 size_t energy_partial_size(const Geo& g, int nmem);
-void launch_energies(const Geo& g, hipStream_t st, int m0, int nmem, const cplx<double>* psi,
-                     const cplx<double>* A, const cplx<double>* C, const double* a, double* partial, double* out);
-void launch_q_from_psi(const Geo& g, hipStream_t st, const cplx<double>* psi, cplx<double>* q,
-                       const double* a, int barotropic);
+void launch_energies(const Geo& g, hipStream_t st, int m0, int nmem, const void* psi, const void* A,
+                     const void* C, bool f32, const double* a, double* partial, double* out);
+// psi_f64: psi is cplx<double> while q is stored per f32 (set-up from a
+// psi still held in FP64).
+void launch_q_from_psi(const Geo& g, hipStream_t st, const void* psi, void* q, bool f32, const double* a,
+                       int barotropic, bool psi_f64 = false);
 template <typename T>
 void launch_transpose(const T* in, T* out, int nx, int ny, int nz, bool to_planar, hipStream_t st);
 template <typename From, typename To>
 void launch_convert(const From* in, To* out, size_t n, hipStream_t st);
-void launch_nonfinite(const cplx<double>* a, size_t n, int* flag, hipStream_t st);
+void launch_nonfinite(const void* a, bool f32, size_t n, int* flag, hipStream_t st);
 void launch_zero_bytes(void* p, size_t bytes, hipStream_t st);
 void launch_zero(cplx<double>* p, size_t n, hipStream_t st);
 

@@ -19,9 +19,10 @@ constexpr int kEnergyTerms = 5;  // wave KE, PE, CE, mean-flow KE, PE
 __device__ inline double abs2(cplx<double> v) { return v.re * v.re + v.im * v.im; }
 
 // blockIdx.y: member (m0 + blockIdx.y); partials [member][block][5].
+template <typename T>
 __global__ void __launch_bounds__(kEnergyThreads)
-k_energy_partial(Geo g, int m0, const cplx<double>* __restrict__ psi, const cplx<double>* __restrict__ A,
-                 const cplx<double>* __restrict__ C, const double* __restrict__ a, double* __restrict__ partial) {
+k_energy_partial(Geo g, int m0, const cplx<T>* __restrict__ psi, const cplx<T>* __restrict__ A,
+                 const cplx<T>* __restrict__ C, const double* __restrict__ a, double* __restrict__ partial) {
   const int plane = g.nx * g.ny;
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   const int nz = g.nzc;
@@ -39,20 +40,20 @@ k_energy_partial(Geo g, int m0, const cplx<double>* __restrict__ psi, const cplx
     auto at = [&](int k) { return (size_t)k * plane + col; };
     for (int k = 0; k < nz; ++k) {
       const double ak = a[k];
-      const cplx<double> p = psi[at(k)];
+      const cplx<double> p = cvt<double>(psi[at(k)]);
       acc[3] += w * kh2 * abs2(p);
       if (k < nz - 1) {
-        const cplx<double> pu = psi[at(k + 1)];
+        const cplx<double> pu = cvt<double>(psi[at(k + 1)]);
         const cplx<double> pz = mk((pu.re - p.re) / dz, (pu.im - p.im) / dz);
         acc[4] += w * ak * abs2(pz);
       }
-      const cplx<double> ck = C[at(k)];
+      const cplx<double> ck = cvt<double>(C[at(k)]);
       cplx<double> la = mk(0.0, 0.0);
       if (nz > 1) {
         if (k == 0) {
           la = mk((ak * ck.re) / dz, (ak * ck.im) / dz);
         } else {
-          const cplx<double> cm = C[at(k - 1)];
+          const cplx<double> cm = cvt<double>(C[at(k - 1)]);
           const double am = a[k - 1];
           la = (k == nz - 1) ? mk((-am * cm.re) / dz, (-am * cm.im) / dz)
                              : mk((ak * ck.re - am * cm.re) / dz, (ak * ck.im - am * cm.im) / dz);
@@ -60,7 +61,7 @@ k_energy_partial(Geo g, int m0, const cplx<double>* __restrict__ psi, const cplx
       }
       acc[0] += w * abs2(la);
       acc[1] += w * (0.5 / ak) * kh2 * abs2(ck);
-      acc[2] += w * (1 / (8 * (ak * ak))) * (kh2 * kh2) * abs2(A[at(k)]);
+      acc[2] += w * (1 / (8 * (ak * ak))) * (kh2 * kh2) * abs2(cvt<double>(A[at(k)]));
     }
   }
   __shared__ double red[kEnergyTerms][kEnergyThreads];
@@ -93,12 +94,18 @@ size_t energy_partial_size(const Geo& g, int nmem) {
   return (size_t)nmem * ((plane + kEnergyThreads - 1) / kEnergyThreads) * kEnergyTerms;
 }
 
-void launch_energies(const Geo& g, hipStream_t st, int m0, int nmem, const cplx<double>* psi,
-                     const cplx<double>* A, const cplx<double>* C, const double* a, double* partial, double* out) {
+void launch_energies(const Geo& g, hipStream_t st, int m0, int nmem, const void* psi, const void* A,
+                     const void* C, bool f32, const double* a, double* partial, double* out) {
   const int plane = g.nx * g.ny;
   const int nblocks = (plane + kEnergyThreads - 1) / kEnergyThreads;
-  hipLaunchKernelGGL(k_energy_partial, dim3(nblocks, nmem), dim3(kEnergyThreads), 0, st, g, m0, psi, A, C, a,
-                     partial);
+  if (f32)
+    hipLaunchKernelGGL(k_energy_partial<float>, dim3(nblocks, nmem), dim3(kEnergyThreads), 0, st, g, m0,
+                       static_cast<const cplx<float>*>(psi), static_cast<const cplx<float>*>(A),
+                       static_cast<const cplx<float>*>(C), a, partial);
+  else
+    hipLaunchKernelGGL(k_energy_partial<double>, dim3(nblocks, nmem), dim3(kEnergyThreads), 0, st, g, m0,
+                       static_cast<const cplx<double>*>(psi), static_cast<const cplx<double>*>(A),
+                       static_cast<const cplx<double>*>(C), a, partial);
   // normalization 0.5 / ((nx*ny)^2 * nz), as the reference forms it
   const double norm = 0.5 / ((double)((long long)plane * plane) * g.nzc);
   const int n = nmem * kEnergyTerms;

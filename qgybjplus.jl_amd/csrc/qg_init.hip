@@ -39,8 +39,9 @@ __device__ inline double hash_randn(unsigned long long seed, int a, int b, int c
 // [m0, m0 + gridDim.y / nzc): psi-hat with energy amplitude * |k|^slope per
 // mode, Hermitian by construction, then the 2/3 mask.  Indices are the
 // reference's 1-based global (level, x, y) of one model.
+template <typename T>
 __global__ void k_random_psi(Geo g, int m0, unsigned long long base_seed, double amplitude, double slope,
-                             cplx<double>* __restrict__ psi) {
+                             cplx<T>* __restrict__ psi) {
   const int plane = g.nx * g.ny;
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= plane) return;
@@ -68,19 +69,20 @@ __global__ void k_random_psi(Geo g, int m0, unsigned long long base_seed, double
       if (!canonical) v.im = -v.im;
     }
   }
-  psi[at] = v;
+  psi[at] = cvt<T>(v);
 }
 
 // set_surface_waves! (src/simulation.jl:292-329): a horizontally uniform
 // physical B(z) = profile[k] whose forward transform is nx*ny*profile[k] on the
 // kh = 0 mode and zero elsewhere.  profile has nzc entries per set member.
-__global__ void k_surface_wave(Geo g, int m0, const double* __restrict__ profile, cplx<double>* __restrict__ B) {
+template <typename T>
+__global__ void k_surface_wave(Geo g, int m0, const double* __restrict__ profile, cplx<T>* __restrict__ B) {
   const int plane = g.nx * g.ny;
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= plane) return;
   const int lvl = blockIdx.y;
   const size_t at = ((size_t)m0 * g.nzc + lvl) * plane + col;
-  B[at] = col == 0 ? mk((double)plane * profile[lvl], 0.0) : mk(0.0, 0.0);
+  B[at] = cvt<T>(col == 0 ? mk((double)plane * profile[lvl], 0.0) : mk(0.0, 0.0));
 }
 
 }  // namespace
@@ -94,15 +96,25 @@ unsigned long long random_psi_base_seed(unsigned long long seed) {
 }
 
 void launch_random_psi(const Geo& g, hipStream_t st, int m0, int nm, unsigned long long seed, double amplitude,
-                       double slope, cplx<double>* psi) {
+                       double slope, void* psi, bool f32) {
   const int plane = g.nx * g.ny, bs = 256;
-  hipLaunchKernelGGL(k_random_psi, dim3((plane + bs - 1) / bs, nm * g.nzc), dim3(bs), 0, st, g, m0,
-                     random_psi_base_seed(seed), amplitude, slope, psi);
+  const dim3 grid((plane + bs - 1) / bs, nm * g.nzc);
+  const unsigned long long base = random_psi_base_seed(seed);
+  if (f32)
+    hipLaunchKernelGGL(k_random_psi<float>, grid, dim3(bs), 0, st, g, m0, base, amplitude, slope,
+                       static_cast<cplx<float>*>(psi));
+  else
+    hipLaunchKernelGGL(k_random_psi<double>, grid, dim3(bs), 0, st, g, m0, base, amplitude, slope,
+                       static_cast<cplx<double>*>(psi));
 }
 
-void launch_surface_wave(const Geo& g, hipStream_t st, int m0, int nm, const double* profile, cplx<double>* B) {
+void launch_surface_wave(const Geo& g, hipStream_t st, int m0, int nm, const double* profile, void* B, bool f32) {
   const int plane = g.nx * g.ny, bs = 256;
-  hipLaunchKernelGGL(k_surface_wave, dim3((plane + bs - 1) / bs, nm * g.nzc), dim3(bs), 0, st, g, m0, profile, B);
+  const dim3 grid((plane + bs - 1) / bs, nm * g.nzc);
+  if (f32)
+    hipLaunchKernelGGL(k_surface_wave<float>, grid, dim3(bs), 0, st, g, m0, profile, static_cast<cplx<float>*>(B));
+  else
+    hipLaunchKernelGGL(k_surface_wave<double>, grid, dim3(bs), 0, st, g, m0, profile, static_cast<cplx<double>*>(B));
 }
 
 }  // namespace qg

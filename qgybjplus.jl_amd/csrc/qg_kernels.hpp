@@ -43,9 +43,10 @@ struct Tile {
 template <typename R>
 __device__ inline cplx<R> czero() { return mk<R>((R)0, (R)0); }
 
-// Hermitian part (X(k) + conj X(-k))/2, pre-scaled by `s`.
-__device__ inline cplx<double> herm(cplx<double> a, cplx<double> am, double s) {
-  return mk<double>(0.5 * s * (a.re + am.re), 0.5 * s * (a.im - am.im));
+// Hermitian part (X(k) + conj X(-k))/2, pre-scaled by `s`, at precision T.
+template <typename T>
+__device__ inline cplx<T> herm(cplx<T> a, cplx<T> am, T s) {
+  return mk<T>((T)0.5 * s * (a.re + am.re), (T)0.5 * s * (a.im - am.im));
 }
 
 // ===========================================================================
@@ -55,7 +56,7 @@ __device__ inline cplx<double> herm(cplx<double> a, cplx<double> am, double s) {
 // Y-pass: Bm = mask*B (scaled 1/N) -> T1 = IFFTy(Bm), T3 = IFFTy(i*ky*Bm).
 template <typename R, int NY, int V>
 __global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
-k_ya_qw(Geo g, const cplx<double>* __restrict__ B, cplx<R>* __restrict__ T1,
+k_ya_qw(Geo g, const cplx<R>* __restrict__ B, cplx<R>* __restrict__ T1,
         cplx<R>* __restrict__ T3, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY, V>;
   using F = Fft<R, NY, +1>;
@@ -76,7 +77,7 @@ k_ya_qw(Geo g, const cplx<double>* __restrict__ B, cplx<R>* __restrict__ T1,
       int y = F::elem(lane, it, t);
       if (ok && F::valid(lane, it)) {
         size_t a = ((size_t)z * NY + y) * g.nx + x;
-        cplx<double> b = g.mask[y * g.nx + x] ? scale(B[a], g.inv_n) : czero<double>();
+        cplx<double> b = g.mask[y * g.nx + x] ? scale(cvt<double>(B[a]), g.inv_n) : czero<double>();
         v[it][t] = cvt<R>(b);
         w[it][t] = cvt<R>(ik_mul(g.ky[y], b));
       }
@@ -168,8 +169,8 @@ k_xa_qw(Geo g, const cplx<R>* __restrict__ T1, const cplx<R>* __restrict__ T3,
 template <typename R, int NY, int V>
 __global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
 k_ya2_qeff(Geo g, const cplx<R>* __restrict__ Qa, const cplx<R>* __restrict__ Qb,
-           const cplx<double>* __restrict__ q, cplx<double>* __restrict__ qeff,
-           cplx<double>* __restrict__ qw_out, double inv_f, const cplx<R>* __restrict__ tw) {
+           const cplx<R>* __restrict__ q, cplx<R>* __restrict__ qeff,
+           cplx<R>* __restrict__ qw_out, double inv_f, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY, V>;
   using F = Fft<R, NY, -1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
@@ -204,14 +205,14 @@ k_ya2_qeff(Geo g, const cplx<R>* __restrict__ Qa, const cplx<R>* __restrict__ Qb
       cplx<double> fq = cvt<double>(v[it][t]), fm = cvt<double>(w[it][t]);
       cplx<double> qw = keep ? scale(fq - scale(fm, 0.25 * kh2), inv_f) : czero<double>();
       size_t a = ((size_t)z * NY + y) * g.nx + x;
-      if (qeff) qeff[a] = keep ? q[a] - qw : czero<double>();
-      if (qw_out) qw_out[a] = qw;
+      if (qeff) qeff[a] = cvt<R>(keep ? cvt<double>(q[a]) - qw : czero<double>());
+      if (qw_out) qw_out[a] = cvt<R>(qw);
       if (mx != x) {
         const int my = mirror(y, NY);
         size_t am = ((size_t)z * NY + my) * g.nx + mx;
         cplx<double> qwm = conj(qw);
-        if (qeff) qeff[am] = keep ? q[am] - qwm : czero<double>();
-        if (qw_out) qw_out[am] = qwm;
+        if (qeff) qeff[am] = cvt<R>(keep ? cvt<double>(q[am]) - qwm : czero<double>());
+        if (qw_out) qw_out[am] = cvt<R>(qwm);
       }
     }
 }
@@ -225,7 +226,7 @@ k_ya2_qeff(Geo g, const cplx<R>* __restrict__ Qa, const cplx<R>* __restrict__ Qb
 // Re ifft(mask q) and of zeta = Re ifft(mask (-kh2 psi)).
 template <typename R, int NY, int V>
 __global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
-k_yb_adv(Geo g, int outs, const cplx<double>* __restrict__ psi, const cplx<double>* __restrict__ q,
+k_yb_adv(Geo g, int outs, const cplx<R>* __restrict__ psi, const cplx<R>* __restrict__ q,
          cplx<R>* __restrict__ Pu, cplx<R>* __restrict__ Pv, cplx<R>* __restrict__ Pq,
          cplx<R>* __restrict__ Pz, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY, V>;
@@ -239,7 +240,9 @@ k_yb_adv(Geo g, int outs, const cplx<double>* __restrict__ psi, const cplx<doubl
   const int mx = mirror(x, g.nx);
   cplx<R>* sl = lds + s * TL::LD;
   const double kxv = ok ? g.kx[x] : 0.0, kxm = ok ? g.kx[mx] : 0.0;
-  cplx<double> p[F::IT][F::R0], pm[F::IT][F::R0];
+  // psi and its mirror are loaded once, in storage precision; every output's
+  // transform input is formed from them at the transform precision R
+  cplx<R> p[F::IT][F::R0], pm[F::IT][F::R0];
   cplx<R> v[F::IT][F::R0];
 #pragma unroll
   for (int it = 0; it < F::IT; ++it)
@@ -247,8 +250,8 @@ k_yb_adv(Geo g, int outs, const cplx<double>* __restrict__ psi, const cplx<doubl
     for (int t = 0; t < F::R0; ++t) {
       int y = F::elem(lane, it, t), my = mirror(y, NY);
       bool in = ok && F::valid(lane, it);
-      p[it][t] = in ? psi[((size_t)z * NY + y) * g.nx + x] : czero<double>();
-      pm[it][t] = in ? psi[((size_t)z * NY + my) * g.nx + mx] : czero<double>();
+      p[it][t] = in ? psi[((size_t)z * NY + y) * g.nx + x] : czero<R>();
+      pm[it][t] = in ? psi[((size_t)z * NY + my) * g.nx + mx] : czero<R>();
     }
   for (int o = 0; o < 4; ++o) {
     const int bit = 1 << o;
@@ -259,27 +262,27 @@ k_yb_adv(Geo g, int outs, const cplx<double>* __restrict__ psi, const cplx<doubl
 #pragma unroll
       for (int t = 0; t < F::R0; ++t) {
         int y = F::elem(lane, it, t), my = mirror(y, NY);
-        cplx<double> a, am;
+        cplx<R> a, am;
         if (o == 0) {
-          a = ik_mul(-g.ky[y], p[it][t]);
-          am = ik_mul(-g.ky[my], pm[it][t]);
+          a = ik_mul((R)-g.ky[y], p[it][t]);
+          am = ik_mul((R)-g.ky[my], pm[it][t]);
         } else if (o == 1) {
-          a = ik_mul(kxv, p[it][t]);
-          am = ik_mul(kxm, pm[it][t]);
+          a = ik_mul((R)kxv, p[it][t]);
+          am = ik_mul((R)kxm, pm[it][t]);
         } else if (o == 2) {
           bool in = ok && F::valid(lane, it);
           bool keep = in && g.mask[y * g.nx + x];
-          a = keep ? q[((size_t)z * NY + y) * g.nx + x] : czero<double>();
-          am = keep ? q[((size_t)z * NY + my) * g.nx + mx] : czero<double>();
+          a = keep ? q[((size_t)z * NY + y) * g.nx + x] : czero<R>();
+          am = keep ? q[((size_t)z * NY + my) * g.nx + mx] : czero<R>();
         } else {
           bool in = ok && F::valid(lane, it);
           bool keep = in && g.mask[y * g.nx + x];
           double kh2 = kxv * kxv + g.ky[y] * g.ky[y];
           double kh2m = kxm * kxm + g.ky[my] * g.ky[my];
-          a = keep ? scale(p[it][t], -kh2) : czero<double>();
-          am = keep ? scale(pm[it][t], -kh2m) : czero<double>();
+          a = keep ? scale(p[it][t], (R)-kh2) : czero<R>();
+          am = keep ? scale(pm[it][t], (R)-kh2m) : czero<R>();
         }
-        v[it][t] = cvt<R>(herm(a, am, g.inv_n));
+        v[it][t] = herm(a, am, (R)g.inv_n);
       }
     F::run(v, sl, lane, twr);
 #pragma unroll
@@ -456,20 +459,22 @@ struct Etd {
 
 // Vertical diffusion nu_z * d2q/dz2 with Neumann ends (dissipation_q_nv!, nonlinear.jl:1010-1044).
 // z is the stacked level; the column of its member has nzc levels.
-__device__ inline cplx<double> vdiff(const cplx<double>* __restrict__ q, int z, int nzc, size_t col,
+template <typename S>
+__device__ inline cplx<double> vdiff(const cplx<S>* __restrict__ q, int z, int nzc, size_t col,
                                      size_t plane, double nu, double inv_dz2) {
   if (nzc <= 1) return czero<double>();
   const int zl = z % nzc;
-  cplx<double> c = q[(size_t)z * plane + col];
+  cplx<double> c = cvt<double>(q[(size_t)z * plane + col]);
   cplx<double> r;
   if (zl == 0) {
-    cplx<double> up = q[(size_t)(z + 1) * plane + col];
+    cplx<double> up = cvt<double>(q[(size_t)(z + 1) * plane + col]);
     r = up - c;
   } else if (zl == nzc - 1) {
-    cplx<double> dn = q[(size_t)(z - 1) * plane + col];
+    cplx<double> dn = cvt<double>(q[(size_t)(z - 1) * plane + col]);
     r = dn - c;
   } else {
-    cplx<double> up = q[(size_t)(z + 1) * plane + col], dn = q[(size_t)(z - 1) * plane + col];
+    cplx<double> up = cvt<double>(q[(size_t)(z + 1) * plane + col]);
+    cplx<double> dn = cvt<double>(q[(size_t)(z - 1) * plane + col]);
     r = mk<double>(up.re - 2 * c.re + dn.re, up.im - 2 * c.im + dn.im);
   }
   return scale(scale(r, nu), inv_dz2);
@@ -480,9 +485,9 @@ __device__ inline cplx<double> vdiff(const cplx<double>* __restrict__ q, int z, 
 template <typename R, int NY, int V>
 __global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
 k_yb2_q(Geo g, int mode, int flags, const cplx<R>* __restrict__ G1, const cplx<R>* __restrict__ G2,
-        const cplx<double>* __restrict__ qd, const cplx<double>* __restrict__ qn,
-        cplx<double>* __restrict__ rhs0,
-        cplx<double>* __restrict__ qout, Etd etd, double nu, double inv_dz2,
+        const cplx<R>* __restrict__ qd, const cplx<R>* __restrict__ qn,
+        cplx<R>* __restrict__ rhs0,
+        cplx<R>* __restrict__ qout, Etd etd, double nu, double inv_dz2,
         const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY, V>;
   using F = Fft<R, NY, -1>;
@@ -527,7 +532,7 @@ k_yb2_q(Geo g, int mode, int flags, const cplx<R>* __restrict__ G1, const cplx<R
         const size_t col = (size_t)cy * g.nx + cx;
         const size_t ad = (size_t)z * plane + col;
         cplx<double> n = side ? conj(nq) : nq;
-        cplx<double> q0 = qn[ad];
+        cplx<double> q0 = cvt<double>(qn[ad]);
         cplx<double> rhs = czero<double>();
         if (keep && !(flags & YQ_FIXED)) {
           cplx<double> dq = (flags & YQ_DIFF) ? vdiff(qd, z, g.nzc, col, plane, nu, inv_dz2) : czero<double>();
@@ -543,13 +548,13 @@ k_yb2_q(Geo g, int mode, int flags, const cplx<R>* __restrict__ G1, const cplx<R
               out = scale(q0, E) + scale(rhs, h1);
             } else {
               const double h2 = etd.at(2, col);
-              cplx<double> r0 = rhs0[ad];
+              cplx<double> r0 = cvt<double>(rhs0[ad]);
               out = (scale(q0, E) + scale(r0, h1)) + scale(rhs - r0, h2);
             }
           }
         }
-        if (mode == MODE_STAGE) rhs0[ad] = rhs;
-        qout[ad] = out;
+        if (mode == MODE_STAGE) rhs0[ad] = cvt<R>(rhs);
+        qout[ad] = cvt<R>(out);
       }
     }
 }
@@ -559,9 +564,9 @@ k_yb2_q(Geo g, int mode, int flags, const cplx<R>* __restrict__ G1, const cplx<R
 template <typename R, int NY, int V>
 __global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
 k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R>* __restrict__ G4,
-        const cplx<R>* __restrict__ G5, const cplx<double>* __restrict__ A,
-        const cplx<double>* __restrict__ Bn, cplx<double>* __restrict__ rhs0,
-        cplx<double>* __restrict__ Bout, Etd etd, double alpha, const cplx<R>* __restrict__ tw) {
+        const cplx<R>* __restrict__ G5, const cplx<R>* __restrict__ A,
+        const cplx<R>* __restrict__ Bn, cplx<R>* __restrict__ rhs0,
+        cplx<R>* __restrict__ Bout, Etd etd, double alpha, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY, V>;
   using F = Fft<R, NY, -1>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
@@ -620,7 +625,7 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
         if (flags & YB_DISP) {
           const double kh2 = kxv * kxv + g.ky[y] * g.ky[y];
           const double c = alpha * kh2;
-          cplx<double> Av = A[ad];
+          cplx<double> Av = cvt<double>(A[ad]);
           t1 = mk<double>(-(c * Av.im), c * Av.re);
         }
         cplx<double> t2 = mk<double>(-(0.5 * rB.im), 0.5 * rB.re);
@@ -629,17 +634,17 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
       cplx<double> out = czero<double>();
       if (keep) {
         const double E = etd.at(3, col), h1 = etd.at(4, col);
-        cplx<double> b0 = Bn[ad];
+        cplx<double> b0 = cvt<double>(Bn[ad]);
         if (mode == MODE_STAGE) {
           out = scale(b0, E) + scale(rhs, h1);
         } else {
           const double h2 = etd.at(5, col);
-          cplx<double> r0 = rhs0[ad];
+          cplx<double> r0 = cvt<double>(rhs0[ad]);
           out = (scale(b0, E) + scale(r0, h1)) + scale(rhs - r0, h2);
         }
       }
-      if (mode == MODE_STAGE) rhs0[ad] = rhs;
-      Bout[ad] = out;
+      if (mode == MODE_STAGE) rhs0[ad] = cvt<R>(rhs);
+      Bout[ad] = cvt<R>(out);
     }
 }
 
@@ -656,7 +661,7 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
 // formed in registers before the four transforms.
 template <typename R, int NY, int V>
 __global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
-k_yf_omega(Geo g, const cplx<double>* __restrict__ psi, cplx<R>* __restrict__ Pbx, cplx<R>* __restrict__ Pby,
+k_yf_omega(Geo g, const cplx<R>* __restrict__ psi, cplx<R>* __restrict__ Pbx, cplx<R>* __restrict__ Pby,
            cplx<R>* __restrict__ Pxx, cplx<R>* __restrict__ Pxy, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY, V>;
   using F = Fft<R, NY, +1>;
@@ -679,23 +684,26 @@ k_yf_omega(Geo g, const cplx<double>* __restrict__ psi, cplx<R>* __restrict__ Pb
     for (int t = 0; t < F::R0; ++t) {
       const int y = F::elem(lane, it, t), my = mirror(y, NY);
       const bool in = ok && F::valid(lane, it);
-      const cplx<double> p = in ? psi[((size_t)z * NY + y) * g.nx + x] : czero<double>();
-      const cplx<double> pm = in ? psi[((size_t)z * NY + my) * g.nx + mx] : czero<double>();
-      const cplx<double> up = in ? psi[((size_t)zu * NY + y) * g.nx + x] : czero<double>();
-      const cplx<double> upm = in ? psi[((size_t)zu * NY + my) * g.nx + mx] : czero<double>();
-      cplx<double> pz = czero<double>(), pzm = czero<double>(), pa = p, pam = pm;
+      // the four factors are formed at the transform precision R
+      const cplx<R> p = in ? psi[((size_t)z * NY + y) * g.nx + x] : czero<R>();
+      const cplx<R> pm = in ? psi[((size_t)z * NY + my) * g.nx + mx] : czero<R>();
+      const cplx<R> up = in ? psi[((size_t)zu * NY + y) * g.nx + x] : czero<R>();
+      const cplx<R> upm = in ? psi[((size_t)zu * NY + my) * g.nx + mx] : czero<R>();
+      const R rdz = (R)dz;
+      cplx<R> pz = czero<R>(), pzm = czero<R>(), pa = p, pam = pm;
       if (!top) {
-        pz = mk<double>((up.re - p.re) / dz, (up.im - p.im) / dz);
-        pzm = mk<double>((upm.re - pm.re) / dz, (upm.im - pm.im) / dz);
-        pa = scale(up + p, 0.5);
-        pam = scale(upm + pm, 0.5);
+        pz = mk<R>((up.re - p.re) / rdz, (up.im - p.im) / rdz);
+        pzm = mk<R>((upm.re - pm.re) / rdz, (upm.im - pm.im) / rdz);
+        pa = scale(up + p, (R)0.5);
+        pam = scale(upm + pm, (R)0.5);
       }
       const double ky = g.ky[y], kym = g.ky[my];
       const double kh2 = kxv * kxv + ky * ky, kh2m = kxm * kxm + kym * kym;
-      v[0][it][t] = cvt<R>(herm(ik_mul(kxv, pz), ik_mul(kxm, pzm), g.inv_n));
-      v[1][it][t] = cvt<R>(herm(ik_mul(ky, pz), ik_mul(kym, pzm), g.inv_n));
-      v[2][it][t] = cvt<R>(herm(ik_mul(-(kxv * kh2), pa), ik_mul(-(kxm * kh2m), pam), g.inv_n));
-      v[3][it][t] = cvt<R>(herm(ik_mul(-(ky * kh2), pa), ik_mul(-(kym * kh2m), pam), g.inv_n));
+      const R rinv = (R)g.inv_n;
+      v[0][it][t] = herm(ik_mul((R)kxv, pz), ik_mul((R)kxm, pzm), rinv);
+      v[1][it][t] = herm(ik_mul((R)ky, pz), ik_mul((R)kym, pzm), rinv);
+      v[2][it][t] = herm(ik_mul((R)-(kxv * kh2), pa), ik_mul((R)-(kxm * kh2m), pam), rinv);
+      v[3][it][t] = herm(ik_mul((R)-(ky * kh2), pa), ik_mul((R)-(kym * kh2m), pam), rinv);
     }
 #pragma unroll
   for (int o = 0; o < 4; ++o) {
@@ -782,7 +790,7 @@ k_xf(Geo g, int want_omega, const cplx<R>* __restrict__ Pu, const cplx<R>* __res
 // Hermitian spectrum in double precision.
 template <typename R, int NY, int V>
 __global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
-k_y_r2c_mask(Geo g, const cplx<R>* __restrict__ Rw, cplx<double>* __restrict__ out,
+k_y_r2c_mask(Geo g, const cplx<R>* __restrict__ Rw, cplx<R>* __restrict__ out,
              const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY, V>;
   using F = Fft<R, NY, -1>;
@@ -811,15 +819,15 @@ k_y_r2c_mask(Geo g, const cplx<R>* __restrict__ Rw, cplx<double>* __restrict__ o
       if (!(ok && F::valid(lane, it))) continue;
       const bool keep = g.mask[y * g.nx + x] != 0;
       cplx<double> c = keep ? cvt<double>(v[it][t]) : czero<double>();
-      out[((size_t)z * NY + y) * g.nx + x] = c;
-      if (mx != x) out[((size_t)z * NY + mirror(y, NY)) * g.nx + mx] = conj(c);
+      out[((size_t)z * NY + y) * g.nx + x] = cvt<R>(c);
+      if (mx != x) out[((size_t)z * NY + mirror(y, NY)) * g.nx + mx] = cvt<R>(conj(c));
     }
 }
 
 // Y-pass (inverse, half width): Hermitian part of a double spectrum.
 template <typename R, int NY, int V>
 __global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
-k_y_herm(Geo g, const cplx<double>* __restrict__ in, cplx<R>* __restrict__ P,
+k_y_herm(Geo g, const cplx<R>* __restrict__ in, cplx<R>* __restrict__ P,
          const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY, V>;
   using F = Fft<R, NY, +1>;
@@ -837,12 +845,12 @@ k_y_herm(Geo g, const cplx<double>* __restrict__ in, cplx<R>* __restrict__ P,
 #pragma unroll
     for (int t = 0; t < F::R0; ++t) {
       int y = F::elem(lane, it, t);
-      cplx<double> a = czero<double>(), am = czero<double>();
+      cplx<R> a = czero<R>(), am = czero<R>();
       if (ok && F::valid(lane, it)) {
         a = in[((size_t)z * NY + y) * g.nx + x];
         am = in[((size_t)z * NY + mirror(y, NY)) * g.nx + mx];
       }
-      v[it][t] = cvt<R>(herm(a, am, g.inv_n));
+      v[it][t] = herm(a, am, (R)g.inv_n);
     }
   F::run(v, sl, lane, twr);
 #pragma unroll

@@ -91,7 +91,8 @@ __global__ void k_omega_factor(Geo g, Quarter qr, const double* __restrict__ cz,
 // thread per grid column (both parts).  b = (f/N2) * RHS on the interior
 // levels is formed in wk and solved in place; the eliminated rows are kept in
 // the scratch arrays sd, sdu, sfill (per grid column and level).
-__global__ void k_omega_pivoted(Geo g, Quarter qr, const cplx<double>* __restrict__ rhsw, cplx<double>* __restrict__ wk,
+template <typename T>
+__global__ void k_omega_pivoted(Geo g, Quarter qr, const cplx<T>* __restrict__ rhsw, cplx<T>* __restrict__ wk,
                                 const unsigned char* __restrict__ pivoted, const double* __restrict__ cz,
                                 const double* __restrict__ fN2, double* __restrict__ sd, double* __restrict__ sdu,
                                 double* __restrict__ sfill) {
@@ -116,23 +117,25 @@ __global__ void k_omega_pivoted(Geo g, Quarter qr, const cplx<double>* __restric
   const double kh2 = g.kx[q.x] * g.kx[q.x] + g.ky[q.y] * g.ky[q.y];
   const int n = nz - 2;
   auto at = [&](int iz) { return (size_t)iz * plane + col; };
-  // interior row iz is level iz + 1
-  auto Wv = [&](int iz) -> cplx<double>& { return wk[(size_t)(iz + 1) * plane + col]; };
+  // interior row iz is level iz + 1; wk holds the eliminated right-hand
+  // sides between the sweeps (in the field's storage precision)
+  auto ld = [&](int iz) { return cvt<double>(wk[(size_t)(iz + 1) * plane + col]); };
+  auto st = [&](int iz, cplx<double> v) { wk[(size_t)(iz + 1) * plane + col] = cvt<T>(v); };
   for (int iz = 0; iz < n; ++iz) {
-    const cplx<double> r = rhsw[(size_t)(iz + 1) * plane + col];
+    const cplx<double> r = cvt<double>(rhsw[(size_t)(iz + 1) * plane + col]);
     const double s = fN2[iz + 1];
-    Wv(iz) = mk(s * r.re, s * r.im);
+    st(iz, mk(s * r.re, s * r.im));
   }
-  wk[col] = mk(0.0, 0.0);
-  wk[(size_t)(nz - 1) * plane + col] = mk(0.0, 0.0);
+  wk[col] = cvt<T>(mk(0.0, 0.0));
+  wk[(size_t)(nz - 1) * plane + col] = cvt<T>(mk(0.0, 0.0));
   double d = -2 * cz[1] - kh2;
   double du = (n > 1) ? cz[1] : 0.0;
-  double br = Wv(0).re, bi = Wv(0).im;
+  double br = ld(0).re, bi = ld(0).im;
   for (int i = 0; i < n - 1; ++i) {
     const double dl = cz[i + 2];
     double dn = -2 * cz[i + 2] - kh2;
     double dun = (i + 1 < n - 1) ? cz[i + 2] : 0.0;
-    double bnr = Wv(i + 1).re, bni = Wv(i + 1).im;
+    double bnr = ld(i + 1).re, bni = ld(i + 1).im;
     double fill = 0.0;
     if (fabs(d) >= fabs(dl)) {
       const double f = dl / d;
@@ -158,28 +161,28 @@ __global__ void k_omega_pivoted(Geo g, Quarter qr, const cplx<double>* __restric
     sd[at(i)] = d;
     sdu[at(i)] = du;
     sfill[at(i)] = fill;
-    Wv(i) = mk(br, bi);
+    st(i, mk(br, bi));
     d = dn;
     du = dun;
     br = bnr;
     bi = bni;
   }
   double xr = br / d, xi = bi / d;
-  Wv(n - 1) = mk(xr, xi);
+  st(n - 1, mk(xr, xi));
   if (n > 1) {
     const int i = n - 2;
-    const double x0r = (Wv(i).re - sdu[at(i)] * xr) / sd[at(i)];
-    const double x0i = (Wv(i).im - sdu[at(i)] * xi) / sd[at(i)];
-    Wv(i) = mk(x0r, x0i);
+    const double x0r = (ld(i).re - sdu[at(i)] * xr) / sd[at(i)];
+    const double x0i = (ld(i).im - sdu[at(i)] * xi) / sd[at(i)];
+    st(i, mk(x0r, x0i));
     double x2r = xr, x2i = xi;
     xr = x0r;
     xi = x0i;
     for (int ii = n - 3; ii >= 0; --ii) {
       const double dd = sd[at(ii)], uu = sdu[at(ii)], ff = sfill[at(ii)];
-      const cplx<double> b = Wv(ii);
+      const cplx<double> b = ld(ii);
       const double vr = (b.re - uu * xr - ff * x2r) / dd;
       const double vi = (b.im - uu * xi - ff * x2i) / dd;
-      Wv(ii) = mk(vr, vi);
+      st(ii, mk(vr, vi));
       x2r = xr;
       x2i = xi;
       xr = vr;
@@ -200,7 +203,7 @@ void launch_omega_factor(const Geo& g, hipStream_t st, const double* cz, double*
 
 bool omega_setup() { return colscan_setup(); }
 
-void launch_omega(const Geo& g, hipStream_t st, const cplx<double>* rhsw, cplx<double>* wk, const double* M,
+void launch_omega(const Geo& g, hipStream_t st, const void* rhsw, void* wk, bool f32, const double* M,
                   const double* D, const unsigned char* pivoted, bool any_pivoted, const double* cz,
                   const double* cu, const double* fN2, double* sd, double* sdu, double* sfill, int shape) {
   if (shape < 0) shape = scan_default_shape(g.nzc);
@@ -214,12 +217,18 @@ void launch_omega(const Geo& g, hipStream_t st, const cplx<double>* rhsw, cplx<d
   a.lv1 = fN2;
   a.lv1_stride = g.nzc;
   a.pivoted = pivoted;
-  launch_colscan<SCAN_OMEGA>(g, st, shape, a);
+  launch_colscan<SCAN_OMEGA>(g, st, shape, a, f32);
   if (any_pivoted) {
     const Quarter qr = quarter(g);
     const int bs = 64;
-    hipLaunchKernelGGL(k_omega_pivoted, dim3((4 * qr.size() + bs - 1) / bs, g.members), dim3(bs), 0, st, g, qr,
-                       rhsw, wk, pivoted, cz, fN2, sd, sdu, sfill);
+    const dim3 grid((4 * qr.size() + bs - 1) / bs, g.members);
+    if (f32)
+      hipLaunchKernelGGL(k_omega_pivoted<float>, grid, dim3(bs), 0, st, g, qr, static_cast<const cplx<float>*>(rhsw),
+                         static_cast<cplx<float>*>(wk), pivoted, cz, fN2, sd, sdu, sfill);
+    else
+      hipLaunchKernelGGL(k_omega_pivoted<double>, grid, dim3(bs), 0, st, g, qr,
+                         static_cast<const cplx<double>*>(rhsw), static_cast<cplx<double>*>(wk), pivoted, cz, fN2,
+                         sd, sdu, sfill);
   }
 }
 

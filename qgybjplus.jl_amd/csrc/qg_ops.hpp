@@ -19,11 +19,11 @@ template <typename R, int V>
 struct OpsV {
   using C = cplx<R>;
   using D = cplx<double>;
-  static void ya_qw(const Geo& g, hipStream_t st, const D* B, C* T1, C* T3, const C* twy);
+  static void ya_qw(const Geo& g, hipStream_t st, const C* B, C* T1, C* T3, const C* twy);
   static void xa_qw(const Geo& g, hipStream_t st, const C* T1, const C* T3, C* Qa, C* Qb, const C* twx);
-  static void ya2_qeff(const Geo& g, hipStream_t st, const C* Qa, const C* Qb, const D* q, D* qeff, D* qw,
+  static void ya2_qeff(const Geo& g, hipStream_t st, const C* Qa, const C* Qb, const C* q, C* qeff, C* qw,
                        double inv_f, const C* twy);
-  static void yb_adv(const Geo& g, hipStream_t st, int outs, const D* psi, const D* q, C* Pu, C* Pv, C* Pq,
+  static void yb_adv(const Geo& g, hipStream_t st, int outs, const C* psi, const C* q, C* Pu, C* Pv, C* Pq,
                      C* Pz, const C* twy);
   static void x_c2r(const Geo& g, hipStream_t st, int nfields, const C* P0, const C* P1, R* o0, R* o1,
                     const C* twx);
@@ -31,16 +31,16 @@ struct OpsV {
                    const C* twx);
   static void xb_B(const Geo& g, hipStream_t st, int flags, const C* T1, const C* Pz, const R* u, const R* v,
                    C* G3, C* G4, C* G5, const C* twx);
-  static void yb2_q(const Geo& g, hipStream_t st, int mode, int flags, const C* G1, const C* G2, const D* qd,
-                    const D* qn,
-                    D* rhs0, D* qout, EtdView etd, double nu, double inv_dz2, const C* twy);
+  static void yb2_q(const Geo& g, hipStream_t st, int mode, int flags, const C* G1, const C* G2, const C* qd,
+                    const C* qn,
+                    C* rhs0, C* qout, EtdView etd, double nu, double inv_dz2, const C* twy);
   static void yb2_B(const Geo& g, hipStream_t st, int mode, int flags, const C* G3, const C* G4, const C* G5,
-                    const D* A, const D* Bn, D* rhs0, D* Bout, EtdView etd, double alpha, const C* twy);
-  static void yf_omega(const Geo& g, hipStream_t st, const D* psi, C* Pbx, C* Pby, C* Pxx, C* Pxy, const C* twy);
+                    const C* A, const C* Bn, C* rhs0, C* Bout, EtdView etd, double alpha, const C* twy);
+  static void yf_omega(const Geo& g, hipStream_t st, const C* psi, C* Pbx, C* Pby, C* Pxx, C* Pxy, const C* twy);
   static void xf(const Geo& g, hipStream_t st, int want_omega, const C* Pu, const C* Pv, const C* Pbx,
                  const C* Pby, const C* Pxx, const C* Pxy, R* u, R* v, C* Rw, const C* twx);
-  static void y_r2c_mask(const Geo& g, hipStream_t st, const C* Rw, D* out, const C* twy);
-  static void y_herm(const Geo& g, hipStream_t st, const D* in, C* P, const C* twy);
+  static void y_r2c_mask(const Geo& g, hipStream_t st, const C* Rw, C* out, const C* twy);
+  static void y_herm(const Geo& g, hipStream_t st, const C* in, C* P, const C* twy);
   // plain complex 2-D transforms of double fields (dir = -1 forward, +1 inverse)
   static void y_c2c(const Geo& g, hipStream_t st, int dir, const D* in, D* out, const C* twy);
   static void x_c2c(const Geo& g, hipStream_t st, int dir, const D* in, D* out, unsigned long long* absmax,

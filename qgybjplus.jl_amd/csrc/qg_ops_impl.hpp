@@ -41,7 +41,7 @@ static inline dim3 xgrid(int rows, int S) { return dim3((rows + S - 1) / S); }
   } while (0)
 
 template <typename R, int V>
-void OpsV<R, V>::ya_qw(const Geo& g, hipStream_t st, const D* B, C* T1, C* T3, const C* twy) {
+void OpsV<R, V>::ya_qw(const Geo& g, hipStream_t st, const C* B, C* T1, C* T3, const C* twy) {
 #define CALL(N) YL((k_ya_qw<R, N, V>), N, g.nx, g, B, T1, T3, twy)
   QG_SWITCH(g.ny, CALL)
 #undef CALL
@@ -55,7 +55,7 @@ void OpsV<R, V>::xa_qw(const Geo& g, hipStream_t st, const C* T1, const C* T3, C
 }
 
 template <typename R, int V>
-void OpsV<R, V>::ya2_qeff(const Geo& g, hipStream_t st, const C* Qa, const C* Qb, const D* q, D* qeff, D* qw,
+void OpsV<R, V>::ya2_qeff(const Geo& g, hipStream_t st, const C* Qa, const C* Qb, const C* q, C* qeff, C* qw,
                       double inv_f, const C* twy) {
 #define CALL(N) YL((k_ya2_qeff<R, N, V>), N, g.W, g, Qa, Qb, q, qeff, qw, inv_f, twy)
   QG_SWITCH(g.ny, CALL)
@@ -63,7 +63,7 @@ void OpsV<R, V>::ya2_qeff(const Geo& g, hipStream_t st, const C* Qa, const C* Qb
 }
 
 template <typename R, int V>
-void OpsV<R, V>::yb_adv(const Geo& g, hipStream_t st, int outs, const D* psi, const D* q, C* Pu, C* Pv, C* Pq,
+void OpsV<R, V>::yb_adv(const Geo& g, hipStream_t st, int outs, const C* psi, const C* q, C* Pu, C* Pv, C* Pq,
                     C* Pz, const C* twy) {
 #define CALL(N) YL((k_yb_adv<R, N, V>), N, g.W, g, outs, psi, q, Pu, Pv, Pq, Pz, twy)
   QG_SWITCH(g.ny, CALL)
@@ -95,9 +95,9 @@ void OpsV<R, V>::xb_B(const Geo& g, hipStream_t st, int flags, const C* T1, cons
 }
 
 template <typename R, int V>
-void OpsV<R, V>::yb2_q(const Geo& g, hipStream_t st, int mode, int flags, const C* G1, const C* G2, const D* qd,
-                   const D* qn,
-                   D* rhs0, D* qout, EtdView etd, double nu, double inv_dz2, const C* twy) {
+void OpsV<R, V>::yb2_q(const Geo& g, hipStream_t st, int mode, int flags, const C* G1, const C* G2, const C* qd,
+                   const C* qn,
+                   C* rhs0, C* qout, EtdView etd, double nu, double inv_dz2, const C* twy) {
   Etd e{etd.tab, etd.plane};
 #define CALL(N) YL((k_yb2_q<R, N, V>), N, g.W, g, mode, flags, G1, G2, qd, qn, rhs0, qout, e, nu, inv_dz2, twy)
   QG_SWITCH(g.ny, CALL)
@@ -106,7 +106,7 @@ void OpsV<R, V>::yb2_q(const Geo& g, hipStream_t st, int mode, int flags, const 
 
 template <typename R, int V>
 void OpsV<R, V>::yb2_B(const Geo& g, hipStream_t st, int mode, int flags, const C* G3, const C* G4, const C* G5,
-                   const D* A, const D* Bn, D* rhs0, D* Bout, EtdView etd, double alpha, const C* twy) {
+                   const C* A, const C* Bn, C* rhs0, C* Bout, EtdView etd, double alpha, const C* twy) {
   Etd e{etd.tab, etd.plane};
 #define CALL(N) YL((k_yb2_B<R, N, V>), N, g.nx, g, mode, flags, G3, G4, G5, A, Bn, rhs0, Bout, e, alpha, twy)
   QG_SWITCH(g.ny, CALL)
@@ -114,7 +114,7 @@ void OpsV<R, V>::yb2_B(const Geo& g, hipStream_t st, int mode, int flags, const 
 }
 
 template <typename R, int V>
-void OpsV<R, V>::yf_omega(const Geo& g, hipStream_t st, const D* psi, C* Pbx, C* Pby, C* Pxx, C* Pxy, const C* twy) {
+void OpsV<R, V>::yf_omega(const Geo& g, hipStream_t st, const C* psi, C* Pbx, C* Pby, C* Pxx, C* Pxy, const C* twy) {
 #define CALL(N) YL((k_yf_omega<R, N, V>), N, g.W, g, psi, Pbx, Pby, Pxx, Pxy, twy)
   QG_SWITCH(g.ny, CALL)
 #undef CALL
@@ -129,14 +129,14 @@ void OpsV<R, V>::xf(const Geo& g, hipStream_t st, int want_omega, const C* Pu, c
 }
 
 template <typename R, int V>
-void OpsV<R, V>::y_r2c_mask(const Geo& g, hipStream_t st, const C* Rw, D* out, const C* twy) {
+void OpsV<R, V>::y_r2c_mask(const Geo& g, hipStream_t st, const C* Rw, C* out, const C* twy) {
 #define CALL(N) YL((k_y_r2c_mask<R, N, V>), N, g.W, g, Rw, out, twy)
   QG_SWITCH(g.ny, CALL)
 #undef CALL
 }
 
 template <typename R, int V>
-void OpsV<R, V>::y_herm(const Geo& g, hipStream_t st, const D* in, C* P, const C* twy) {
+void OpsV<R, V>::y_herm(const Geo& g, hipStream_t st, const C* in, C* P, const C* twy) {
 #define CALL(N) YL((k_y_herm<R, N, V>), N, g.W, g, in, P, twy)
   QG_SWITCH(g.ny, CALL)
 #undef CALL

@@ -71,10 +71,18 @@ struct qg_handle {
   unsigned char* d_mask = nullptr;
   double* d_etd = nullptr;
   double etd_dt = std::nan("");
-  D *q[2] = {nullptr, nullptr}, *B[2] = {nullptr, nullptr};
+  // Complex fields in the mode's storage precision: cplx<double> in FP64
+  // mode, cplx<float> in mixed mode (f32()).  stg and stg2 are FP64 staging
+  // for host exchange and plain transforms (stg2 aliases qeff in FP64 mode).
+  void *q[2] = {nullptr, nullptr}, *B[2] = {nullptr, nullptr};
   int cur = 0;
-  D *psi = nullptr, *A = nullptr, *C = nullptr, *qs = nullptr, *Bs = nullptr, *psis = nullptr,
-    *As = nullptr, *rq0 = nullptr, *rB0 = nullptr, *qeff = nullptr, *stg = nullptr;
+  void *psi = nullptr, *A = nullptr, *C = nullptr, *qs = nullptr, *Bs = nullptr, *psis = nullptr,
+       *As = nullptr, *rq0 = nullptr, *rB0 = nullptr, *qeff = nullptr;
+  D *stg = nullptr, *stg2 = nullptr;
+  // mixed mode: members [psi64_m0, psi64_m0 + psi64_nm) have the FP64 psi-hat
+  // of their last set-up in stg2 (at their own offsets), so q is derived from
+  // it rather than from the FP32-rounded psi; any other use of stg2 clears it
+  int psi64_m0 = 0, psi64_nm = 0;
   double *s0 = nullptr, *s1 = nullptr, *s2 = nullptr;
   // Thomas elimination factors of the q->psi and B->A operators, and where
   // (if anywhere) their pivots fall below sqrt(eps): {code, level, column}
@@ -135,6 +143,11 @@ struct qg_handle {
     return static_cast<T*>(p);
   }
   size_t rsize() const { return precision == QG_PRECISION_MIXED ? 4 : 8; }
+  bool f32() const { return precision == QG_PRECISION_MIXED; }
+  size_t fsize() const { return 2 * rsize(); }  // bytes of one complex field value
+  // field pointer advanced by `elems` complex values
+  void* at(void* p, size_t elems) const { return p ? static_cast<char*>(p) + elems * fsize() : nullptr; }
+  void* falloc(size_t count) { return ralloc(2 * count); }
   void* ralloc(size_t count) {  // count of R scalars
     void* p = nullptr;
     HIPCHECK(hipMalloc(&p, count * rsize() + 256));
@@ -263,16 +276,16 @@ void check_status(qg_handle& h) {
 }
 
 // invert_q_to_psi! (src/elliptic.jl:189-314) on device arrays.
-void solve_psi(qg_handle& h, const D* q, D* psi) {
+void solve_psi(qg_handle& h, const void* q, void* psi) {
   if (h.sing_psi[0]) raise_singular(h.sing_psi);
-  qg::launch_thomas(h.g, h.st, q, psi, nullptr, h.th_inv[0], h.th_cp[0], h.d_al, 1, h.d_warn,
+  qg::launch_thomas(h.g, h.st, q, psi, nullptr, h.f32(), h.th_inv[0], h.th_cp[0], h.d_al, 1, h.d_warn,
                     h.tune[qg::P_THOMAS_PSI]);
 }
 
 // invert_B_to_A! (src/elliptic.jl:836-990) on device arrays; C optional.
-void solve_A(qg_handle& h, const D* B, D* A, D* C) {
+void solve_A(qg_handle& h, const void* B, void* A, void* C) {
   if (h.sing_A[0]) raise_singular(h.sing_A);
-  qg::launch_thomas(h.g, h.st, B, A, C, h.th_inv[1], h.th_cp[1], h.d_al, 4, nullptr,
+  qg::launch_thomas(h.g, h.st, B, A, C, h.f32(), h.th_inv[1], h.th_cp[1], h.d_al, 4, nullptr,
                     h.tune[C ? qg::P_THOMAS_AC : qg::P_THOMAS_A]);
 }
 
@@ -281,9 +294,10 @@ struct Engine {
   using C = cplx<R>;
   using O = qg::Ops<R>;
   qg_handle& h;
-  // algorithmic bytes per point: complex R, real R, complex double; P = all
-  // points, Ph = points of a half-width (real-field) intermediate
-  const double cR = 2.0 * sizeof(R), rR = sizeof(R), cD = 16.0;
+  // algorithmic bytes per point: complex R, real R, complex field value in
+  // storage (the fields are stored at R); P = all points, Ph = points of a
+  // half-width (real-field) intermediate
+  const double cR = 2.0 * sizeof(R), rR = sizeof(R), cS = 2.0 * sizeof(R);
   // Tq: the two per-level tables of a column solve, stored for the quarter
   // columns |ix| <= nx/2, |iy| <= ny/2
   const double P, Ph, Tq;
@@ -309,43 +323,43 @@ struct Engine {
     }
   }
 
-  void t1_only(const D* Bsrc) {
-    K("ya_T1", qg::P_YA_QW, P * (cD + cR), [&] { O::ya_qw(h.tune[qg::P_YA_QW], h.g, h.st, Bsrc, c(h.T1), nullptr, twy()); });
+  void t1_only(const C* Bsrc) {
+    K("ya_T1", qg::P_YA_QW, P * (cS + cR), [&] { O::ya_qw(h.tune[qg::P_YA_QW], h.g, h.st, Bsrc, c(h.T1), nullptr, twy()); });
   }
 
   // _diagnose_flow! (src/timestep.jl:192-215) minus the velocities: q_eff with
   // wave feedback, then q -> psi.  Leaves T1 = IFFTy(mask B) when need_t1.
-  void diag(const D* qsrc, const D* Bsrc, D* psi_out, bool need_t1) {
+  void diag(const C* qsrc, const C* Bsrc, C* psi_out, bool need_t1) {
     bool t1 = false;
     if (!h.fixed_flow) {
-      const D* src = qsrc;
+      const C* src = qsrc;
       if (h.feedback) {
-        K("ya_qw", qg::P_YA_QW, P * (cD + 2 * cR), [&] { O::ya_qw(h.tune[qg::P_YA_QW], h.g, h.st, Bsrc, c(h.T1), c(h.T3), twy()); });
+        K("ya_qw", qg::P_YA_QW, P * (cS + 2 * cR), [&] { O::ya_qw(h.tune[qg::P_YA_QW], h.g, h.st, Bsrc, c(h.T1), c(h.T3), twy()); });
         K("xa_qw", qg::P_XA_QW, P * 2 * cR + Ph * 2 * cR,
           [&] { O::xa_qw(h.tune[qg::P_XA_QW], h.g, h.st, c(h.T1), c(h.T3), c(h.Qa), c(h.Qb), twx()); });
-        K("ya2_qeff", qg::P_YA2_QEFF, Ph * 2 * cR + P * 2 * cD,
-          [&] { O::ya2_qeff(h.tune[qg::P_YA2_QEFF], h.g, h.st, c(h.Qa), c(h.Qb), qsrc, h.qeff, nullptr, 1.0 / h.f, twy()); });
-        src = h.qeff;
+        K("ya2_qeff", qg::P_YA2_QEFF, Ph * 2 * cR + P * 2 * cS,
+          [&] { O::ya2_qeff(h.tune[qg::P_YA2_QEFF], h.g, h.st, c(h.Qa), c(h.Qb), qsrc, c(h.qeff), nullptr, 1.0 / h.f, twy()); });
+        src = c(h.qeff);
         t1 = true;
       }
-      K("thomas_psi", qg::P_THOMAS_PSI, P * 2 * cD + Tq,
+      K("thomas_psi", qg::P_THOMAS_PSI, P * 2 * cS + Tq,
         [&] { solve_psi(h, src, psi_out); });
     }
     if (need_t1 && !t1) t1_only(Bsrc);
   }
 
-  void wave_inv(const D* Bsrc, D* Aout, D* Cout) {
+  void wave_inv(const C* Bsrc, C* Aout, C* Cout) {
     if (h.passive || h.no_disp) {
-      qg::launch_zero(Aout, h.plane * h.g.nz, h.st);
-      if (Cout) qg::launch_zero(Cout, h.plane * h.g.nz, h.st);
+      qg::launch_zero_bytes(Aout, h.plane * h.g.nz * h.fsize(), h.st);
+      if (Cout) qg::launch_zero_bytes(Cout, h.plane * h.g.nz * h.fsize(), h.st);
       return;
     }
-    K(Cout ? "thomas_AC" : "thomas_A", Cout ? qg::P_THOMAS_AC : qg::P_THOMAS_A, P * (Cout ? 3 : 2) * cD + Tq,
+    K(Cout ? "thomas_AC" : "thomas_A", Cout ? qg::P_THOMAS_AC : qg::P_THOMAS_A, P * (Cout ? 3 : 2) * cS + Tq,
       [&] { solve_A(h, Bsrc, Aout, Cout); });
   }
 
-  void velocities(const D* psi) {
-    K("yb_uv", qg::P_YB_ADV, P * cD + Ph * 2 * cR, [&] {
+  void velocities(const C* psi) {
+    K("yb_uv", qg::P_YB_ADV, P * cS + Ph * 2 * cR, [&] {
       O::yb_adv(h.tune[qg::P_YB_ADV], h.g, h.st, qg::OUT_U | qg::OUT_V, psi, nullptr, c(h.Pu), c(h.Pv), nullptr, nullptr, twy());
     });
     K("x_c2r_uv", qg::P_X_C2R, Ph * 2 * cR + P * 2 * rR,
@@ -353,36 +367,36 @@ struct Engine {
   }
 
   // u, v and the QG omega-equation w (compute_velocities!(compute_w=true)).
-  void velocities_w(const D* psi) {
-    K("yb_uv", qg::P_YB_ADV, P * cD + Ph * 2 * cR, [&] {
+  void velocities_w(const C* psi) {
+    K("yb_uv", qg::P_YB_ADV, P * cS + Ph * 2 * cR, [&] {
       O::yb_adv(h.tune[qg::P_YB_ADV], h.g, h.st, qg::OUT_U | qg::OUT_V, psi, nullptr, c(h.Pu), c(h.Pv), nullptr, nullptr, twy());
     });
-    K("yf_omega", qg::P_YF_OMEGA, P * cD + Ph * 4 * cR,
+    K("yf_omega", qg::P_YF_OMEGA, P * cS + Ph * 4 * cR,
       [&] { O::yf_omega(h.tune[qg::P_YF_OMEGA], h.g, h.st, psi, c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), twy()); });
     K("xf", qg::P_XF, Ph * 7 * cR + P * 2 * rR, [&] {
       O::xf(h.tune[qg::P_XF], h.g, h.st, 1, c(h.Pu), c(h.Pv), c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), r(h.u), r(h.v), c(h.Pq),
             twx());
     });
-    K("y_r2c_omega", qg::P_Y_R2C, Ph * cR + P * cD, [&] { O::y_r2c_mask(h.tune[qg::P_Y_R2C], h.g, h.st, c(h.Pq), h.rq0, twy()); });
+    K("y_r2c_omega", qg::P_Y_R2C, Ph * cR + P * cS, [&] { O::y_r2c_mask(h.tune[qg::P_Y_R2C], h.g, h.st, c(h.Pq), c(h.rq0), twy()); });
     if (h.sing_omega[0]) {
       char buf[160];
       snprintf(buf, sizeof buf, "LAPACK-style gtsv failed for vertical velocity at row %d of column %d",
                h.sing_omega[1] + 1, h.sing_omega[2]);
       throw QgError(QG_ERR_GTSV, buf);
     }
-    K("omega_gtsv", qg::P_OMEGA, P * 2 * cD + Tq, [&] {
-      qg::launch_omega(h.g, h.st, h.rq0, h.qeff, h.om_M, h.om_D, h.om_piv, h.om_any_piv, h.d_cz, h.d_cu,
+    K("omega_gtsv", qg::P_OMEGA, P * 2 * cS + Tq, [&] {
+      qg::launch_omega(h.g, h.st, c(h.rq0), c(h.qeff), h.f32(), h.om_M, h.om_D, h.om_piv, h.om_any_piv, h.d_cz, h.d_cu,
                        h.d_fN2, h.s0, h.s1, h.s2, h.tune[qg::P_OMEGA]);
     });
-    K("y_herm_w", qg::P_Y_HERM, P * cD + Ph * cR, [&] { O::y_herm(h.tune[qg::P_Y_HERM], h.g, h.st, h.qeff, c(h.Pz), twy()); });
+    K("y_herm_w", qg::P_Y_HERM, P * cS + Ph * cR, [&] { O::y_herm(h.tune[qg::P_Y_HERM], h.g, h.st, c(h.qeff), c(h.Pz), twy()); });
     K("x_c2r_w", qg::P_X_C2R, Ph * cR + P * rR, [&] { O::x_c2r(h.tune[qg::P_X_C2R], h.g, h.st, 1, c(h.Pz), nullptr, r(h.w), nullptr, twx()); });
   }
 
   // One RHS evaluation at (qS, psiS, AS, T1, u, v) fused with the ETD update
   // from the base state (qn, Bn): _compute_etdrk2_rhs! (src/timestep.jl:230-298)
   // and the stage/final loops (src/timestep.jl:362-404).
-  void rhs_update(int mode, const D* qS, const D* psiS, const D* AS, const D* qn, const D* Bn, D* qout,
-                  D* Bout) {
+  void rhs_update(int mode, const C* qS, const C* psiS, const C* AS, const C* qn, const C* Bn, C* qout,
+                  C* Bout) {
     const bool adv = !h.linear;
     const bool refr = !h.passive;
     const bool qdyn = !h.fixed_flow;
@@ -391,7 +405,7 @@ struct Engine {
     if (refr) outs |= qg::OUT_Z;
     const int nouts = ((outs & qg::OUT_Q) ? 1 : 0) + ((outs & qg::OUT_Z) ? 1 : 0);
     if (outs)
-      K("yb_adv", qg::P_YB_ADV, P * (((outs & qg::OUT_Q) ? cD : 0) + ((outs & qg::OUT_Z) ? cD : 0)) + Ph * nouts * cR,
+      K("yb_adv", qg::P_YB_ADV, P * (((outs & qg::OUT_Q) ? cS : 0) + ((outs & qg::OUT_Z) ? cS : 0)) + Ph * nouts * cR,
         [&] { O::yb_adv(h.tune[qg::P_YB_ADV], h.g, h.st, outs, psiS, qS, nullptr, nullptr, c(h.Pq), c(h.Pz), twy()); });
     if (adv && qdyn)
       K("xb_q", qg::P_XB_Q, Ph * 3 * cR + P * 2 * rR,
@@ -404,59 +418,59 @@ struct Engine {
     int qf = ((adv && qdyn) ? qg::YQ_ADV : 0) | (h.fixed_flow ? qg::YQ_FIXED : 0) | (diff ? qg::YQ_DIFF : 0);
     const double inv_dz2 = h.nz > 1 ? 1.0 / (h.dz * h.dz) : 0.0;
     K(mode == qg::MODE_STAGE ? "yb2_q_stage" : "yb2_q_final", qg::P_YB2_Q,
-      ((qf & qg::YQ_ADV) ? Ph * 2 * cR : 0) + P * cD * (3 + (diff ? 1 : 0)),
-      [&] { O::yb2_q(h.tune[qg::P_YB2_Q], h.g, h.st, mode, qf, c(h.G1), c(h.G2), qS, qn, h.rq0, qout, etd(), h.nu_z, inv_dz2, twy()); });
+      ((qf & qg::YQ_ADV) ? Ph * 2 * cR : 0) + P * cS * (3 + (diff ? 1 : 0)),
+      [&] { O::yb2_q(h.tune[qg::P_YB2_Q], h.g, h.st, mode, qf, c(h.G1), c(h.G2), qS, qn, c(h.rq0), qout, etd(), h.nu_z, inv_dz2, twy()); });
     const bool disp = !(h.passive || h.no_disp);
     int bf = (adv ? qg::YB_ADV : 0) | (refr ? qg::YB_REFR : 0) | (disp ? qg::YB_DISP : 0);
     K(mode == qg::MODE_STAGE ? "yb2_B_stage" : "yb2_B_final", qg::P_YB2_B,
-      P * cR * ((adv ? 2 : 0) + (refr ? 1 : 0)) + P * cD * (3 + (disp ? 1 : 0)),
-      [&] { O::yb2_B(h.tune[qg::P_YB2_B], h.g, h.st, mode, bf, c(h.G3), c(h.G4), c(h.G5), AS, Bn, h.rB0, Bout, etd(), h.f / 2, twy()); });
+      P * cR * ((adv ? 2 : 0) + (refr ? 1 : 0)) + P * cS * (3 + (disp ? 1 : 0)),
+      [&] { O::yb2_B(h.tune[qg::P_YB2_B], h.g, h.st, mode, bf, c(h.G3), c(h.G4), c(h.G5), AS, Bn, c(h.rB0), Bout, etd(), h.f / 2, twy()); });
   }
 
   // step!(model, ExponentialRungeKutta2) -> _advance_etdrk2! (src/timestep.jl:335-415).
   void step() {
-    D* qn = h.q[h.cur];
-    D* Bn = h.B[h.cur];
-    D* q1 = h.q[1 - h.cur];
-    D* B1 = h.B[1 - h.cur];
+    C* qn = c(h.q[h.cur]);
+    C* Bn = c(h.B[h.cur]);
+    C* q1 = c(h.q[1 - h.cur]);
+    C* B1 = c(h.B[1 - h.cur]);
     const bool need_uv = !h.linear;
     // RHS at S_n.  psi, A, u, v and T1 left by the previous finalize are the
     // values this evaluation would recompute from (q_n, B_n).
     if (!h.diag_valid) {
-      diag(qn, Bn, h.psi, true);
-      wave_inv(Bn, h.A, nullptr);
-      if (need_uv) velocities(h.psi);
+      diag(qn, Bn, c(h.psi), true);
+      wave_inv(Bn, c(h.A), nullptr);
+      if (need_uv) velocities(c(h.psi));
     } else if (!h.t1_valid) {
       t1_only(Bn);
     }
-    rhs_update(qg::MODE_STAGE, qn, h.psi, h.A, qn, Bn, h.qs, h.Bs);
+    rhs_update(qg::MODE_STAGE, qn, c(h.psi), c(h.A), qn, Bn, c(h.qs), c(h.Bs));
     // RHS at the stage
-    D* psiS = h.fixed_flow ? h.psi : h.psis;
-    diag(h.qs, h.Bs, psiS, true);
-    wave_inv(h.Bs, h.As, nullptr);
+    C* psiS = h.fixed_flow ? c(h.psi) : c(h.psis);
+    diag(c(h.qs), c(h.Bs), psiS, true);
+    wave_inv(c(h.Bs), c(h.As), nullptr);
     if (need_uv && !h.fixed_flow) velocities(psiS);
-    rhs_update(qg::MODE_FINAL, h.qs, psiS, h.As, qn, Bn, q1, B1);
+    rhs_update(qg::MODE_FINAL, c(h.qs), psiS, c(h.As), qn, Bn, q1, B1);
     // finalize (src/timestep.jl:300-332)
-    diag(q1, B1, h.psi, false);
-    wave_inv(B1, h.A, h.C);
-    velocities_w(h.psi);
+    diag(q1, B1, c(h.psi), false);
+    wave_inv(B1, c(h.A), c(h.C));
+    velocities_w(c(h.psi));
     h.cur = 1 - h.cur;
     h.diag_valid = true;
     h.t1_valid = h.feedback && !h.fixed_flow;
   }
 
-  void compute_qw(D* out) {
-    O::ya_qw(h.tune[qg::P_YA_QW], h.g, h.st, h.B[h.cur], c(h.T1), c(h.T3), twy());
+  void compute_qw(C* out) {
+    O::ya_qw(h.tune[qg::P_YA_QW], h.g, h.st, c(h.B[h.cur]), c(h.T1), c(h.T3), twy());
     O::xa_qw(h.tune[qg::P_XA_QW], h.g, h.st, c(h.T1), c(h.T3), c(h.Qa), c(h.Qb), twx());
-    O::ya2_qeff(h.tune[qg::P_YA2_QEFF], h.g, h.st, c(h.Qa), c(h.Qb), h.q[h.cur], nullptr, out, 1.0 / h.f, twy());
+    O::ya2_qeff(h.tune[qg::P_YA2_QEFF], h.g, h.st, c(h.Qa), c(h.Qb), c(h.q[h.cur]), nullptr, out, 1.0 / h.f, twy());
     h.t1_valid = false;
   }
 
   void compute_velocities(bool compute_w) {
     if (compute_w) {
-      velocities_w(h.psi);
+      velocities_w(c(h.psi));
     } else {
-      velocities(h.psi);
+      velocities(c(h.psi));
       qg::launch_zero_bytes(h.w, h.plane * h.g.nz * h.rsize(), h.st);
     }
   }
@@ -489,7 +503,7 @@ void with_engine(qg_handle& h, Fn&& fn) {
   }
 }
 
-D* complex_field(qg_handle& h, int field) {
+void* complex_field(qg_handle& h, int field) {
   switch (field) {
     case QG_FIELD_Q: return h.q[h.cur];
     case QG_FIELD_B: return h.B[h.cur];
@@ -654,20 +668,13 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
     HIPCHECK(hipMemcpy(h->d_mask, mask.data(), h->plane, hipMemcpyHostToDevice));
 
     for (int i = 0; i < 2; ++i) {
-      h->q[i] = h->alloc<D>(h->npts);
-      h->B[i] = h->alloc<D>(h->npts);
+      h->q[i] = h->falloc(h->npts);
+      h->B[i] = h->falloc(h->npts);
     }
-    h->psi = h->alloc<D>(h->npts);
-    h->A = h->alloc<D>(h->npts);
-    h->C = h->alloc<D>(h->npts);
-    h->qs = h->alloc<D>(h->npts);
-    h->Bs = h->alloc<D>(h->npts);
-    h->psis = h->alloc<D>(h->npts);
-    h->As = h->alloc<D>(h->npts);
-    h->rq0 = h->alloc<D>(h->npts);
-    h->rB0 = h->alloc<D>(h->npts);
-    h->qeff = h->alloc<D>(h->npts);
+    for (void** p : {&h->psi, &h->A, &h->C, &h->qs, &h->Bs, &h->psis, &h->As, &h->rq0, &h->rB0, &h->qeff})
+      *p = h->falloc(h->npts);
     h->stg = h->alloc<D>(h->npts);
+    h->stg2 = h->f32() ? h->alloc<D>(h->npts) : static_cast<D*>(h->qeff);
     h->s0 = h->alloc<double>(h->npts);
     h->s1 = h->alloc<double>(h->npts);
     h->s2 = h->alloc<double>(h->npts);
@@ -683,8 +690,8 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
     h->d_flag = h->alloc<int>(1);
     HIPCHECK(hipMemset(h->d_err, 0, 4 * sizeof(int)));
     HIPCHECK(hipMemset(h->d_warn, 0, sizeof(unsigned long long)));
-    for (D* p : {h->q[0], h->q[1], h->B[0], h->B[1], h->psi, h->A, h->C})
-      HIPCHECK(hipMemset(p, 0, h->npts * sizeof(D)));
+    for (void* p : {h->q[0], h->q[1], h->B[0], h->B[1], h->psi, h->A, h->C})
+      HIPCHECK(hipMemset(p, 0, h->npts * h->fsize()));
     for (void* p : {h->u, h->v, h->w}) HIPCHECK(hipMemset(p, 0, h->npts * h->rsize()));
 
     h->tw64x = h->alloc<D>(nx);
@@ -768,7 +775,8 @@ int qg_memory_required(const qg_config* cfg, size_t* bytes) {
   const size_t r = cfg->precision == QG_PRECISION_MIXED ? 4 : 8;
   const size_t quarter = (nx / 2 + 1) * (ny / 2 + 1);
   const size_t kpad = (size_t)qg::thomas_shifted_a_size(cfg->nz);
-  size_t b = 15 * 16 * npts       // complex FP64 fields and stage/work buffers
+  size_t b = 14 * 2 * r * npts    // complex fields and stage/work buffers (storage precision)
+             + (r == 4 ? 2 : 1) * 16 * npts  // FP64 staging (the second aliases qeff in FP64 mode)
              + 3 * 8 * npts       // omega scratch
              + 3 * r * npts       // u, v, w
              + 5 * 2 * r * npts   // full-width transform intermediates
@@ -797,20 +805,44 @@ int qg_destroy(qg_handle* h) {
 
 namespace {
 
+// n values of a complex field from value `off` on, as FP64: the field itself
+// in FP64 mode, else converted into the FP64 staging stg2.
+D* as_fp64(qg_handle& H, void* field, size_t off, size_t n) {
+  if (!H.f32()) return static_cast<D*>(field) + off;
+  H.psi64_nm = 0;
+  qg::launch_convert<float, double>(static_cast<const float*>(H.at(field, off)), reinterpret_cast<double*>(H.stg2),
+                                    2 * n, H.st);
+  return H.stg2;
+}
+
 // Field exchange for members [m0, m0 + nm): a (z, x, y) array of nm * nzc
 // levels (member-major along z).
 void set_field_range(qg_handle& H, int field, const double* host, int space, int m0, int nm) {
   if (!host) throw QgError(QG_ERR_ARGUMENT, "null host buffer");
   const int lv = nm * H.nzc;
   const size_t off = (size_t)m0 * H.nzc * H.plane, n = (size_t)lv * H.plane;
-  if (D* dst = complex_field(H, field)) {
+  if (void* dst = complex_field(H, field)) {
+    if (space != QG_SPACE_PHYSICAL && space != QG_SPACE_SPECTRAL)
+      throw QgError(QG_ERR_ARGUMENT, "space must be spectral or physical");
+    // FP64 planar values (transformed in FP64), then stored at the field's
+    // precision
+    D* pl = H.f32() ? H.stg2 + off : static_cast<D*>(dst) + off;
+    if (H.f32()) H.psi64_nm = 0;
     HIPCHECK(hipMemcpyAsync(H.stg, host, n * sizeof(D), hipMemcpyHostToDevice, H.st));
-    qg::launch_transpose<D>(H.stg, dst + off, H.nx, H.ny, lv, true, H.st);
-    if (space == QG_SPACE_PHYSICAL) fft2_inplace(H, dst + off, -1, nullptr, nullptr, lv);
-    else if (space != QG_SPACE_SPECTRAL) throw QgError(QG_ERR_ARGUMENT, "space must be spectral or physical");
+    qg::launch_transpose<D>(H.stg, pl, H.nx, H.ny, lv, true, H.st);
+    if (space == QG_SPACE_PHYSICAL) fft2_inplace(H, pl, -1, nullptr, nullptr, lv);
+    if (H.f32()) {
+      qg::launch_convert<double, float>(reinterpret_cast<const double*>(pl),
+                                        static_cast<float*>(H.at(dst, off)), 2 * n, H.st);
+      if (field == QG_FIELD_PSI) {
+        H.psi64_m0 = m0;
+        H.psi64_nm = nm;
+      }
+    }
   } else if (void* rdst = real_field(H, field)) {
     double* tmp = reinterpret_cast<double*>(H.stg);
-    double* pl = reinterpret_cast<double*>(H.qeff);
+    double* pl = reinterpret_cast<double*>(H.stg2);
+    H.psi64_nm = 0;
     HIPCHECK(hipMemcpyAsync(tmp, host, n * sizeof(double), hipMemcpyHostToDevice, H.st));
     if (H.precision == QG_PRECISION_MIXED) {
       qg::launch_transpose<double>(tmp, pl, H.nx, H.ny, lv, true, H.st);
@@ -830,14 +862,15 @@ void get_field_range(qg_handle& H, int field, double* host, int m0, int nm) {
   if (!host) throw QgError(QG_ERR_ARGUMENT, "null host buffer");
   const int lv = nm * H.nzc;
   const size_t off = (size_t)m0 * H.nzc * H.plane, n = (size_t)lv * H.plane;
-  if (D* src = complex_field(H, field)) {
-    qg::launch_transpose<D>(src + off, H.stg, H.nx, H.ny, lv, false, H.st);
+  if (void* src = complex_field(H, field)) {
+    qg::launch_transpose<D>(as_fp64(H, src, off, n), H.stg, H.nx, H.ny, lv, false, H.st);
     HIPCHECK(hipMemcpyAsync(host, H.stg, n * sizeof(D), hipMemcpyDeviceToHost, H.st));
   } else if (void* rsrc = real_field(H, field)) {
     double* tmp = reinterpret_cast<double*>(H.stg);
     const double* pl = static_cast<const double*>(rsrc) + off;
     if (H.precision == QG_PRECISION_MIXED) {
-      double* conv = reinterpret_cast<double*>(H.qeff);
+      double* conv = reinterpret_cast<double*>(H.stg2);
+      H.psi64_nm = 0;
       qg::launch_convert<float, double>(static_cast<const float*>(rsrc) + off, conv, n, H.st);
       pl = conv;
     }
@@ -854,10 +887,10 @@ void check_range(qg_handle& H, int m0, int nm, int* nonfinite, double* max_abs_p
   const size_t off = (size_t)m0 * H.nzc * H.plane, n = (size_t)lv * H.plane;
   HIPCHECK(hipMemsetAsync(H.d_flag, 0, sizeof(int), H.st));
   HIPCHECK(hipMemsetAsync(H.d_absmax, 0, sizeof(unsigned long long), H.st));
-  qg::launch_nonfinite(H.q[H.cur] + off, n, H.d_flag, H.st);
-  qg::launch_nonfinite(H.B[H.cur] + off, n, H.d_flag, H.st);
-  qg::launch_nonfinite(H.psi + off, n, H.d_flag, H.st);
-  fft2_inplace(H, H.psi + off, +1, H.d_absmax, H.stg, lv);
+  qg::launch_nonfinite(H.at(H.q[H.cur], off), H.f32(), n, H.d_flag, H.st);
+  qg::launch_nonfinite(H.at(H.B[H.cur], off), H.f32(), n, H.d_flag, H.st);
+  qg::launch_nonfinite(H.at(H.psi, off), H.f32(), n, H.d_flag, H.st);
+  fft2_inplace(H, as_fp64(H, H.psi, off, n), +1, H.d_absmax, H.stg, lv);
   int flag = 0;
   unsigned long long bits = 0;
   HIPCHECK(hipMemcpyAsync(&flag, H.d_flag, sizeof(int), hipMemcpyDeviceToHost, H.st));
@@ -921,6 +954,7 @@ int qg_step(qg_handle* h, double dt, int nsteps) {
     if (nsteps < 0) throw QgError(QG_ERR_ARGUMENT, "nsteps must be non-negative");
     qg_handle& H = *h;
     if (H.active == 0) return;
+    H.psi64_nm = 0;    // psi is rediagnosed from q
     build_etd(H, dt);  // rewrites the ETD tables in place: captured graphs stay valid
     auto mark = [&](int weight) {
       if (!(H.prof_mode & 2)) return;
@@ -1002,7 +1036,16 @@ int qg_set_random_psi(qg_handle* h, int member, double amplitude, double slope, 
     qg::Geo g = H.g;
     g.nz = H.nz;
     g.members = H.members;
-    qg::launch_random_psi(g, H.st, m0, nm, (unsigned long long)seed, amplitude, slope, H.psi);
+    if (H.f32()) {  // FP64 values kept in stg2 for the q of the set-up
+      const size_t off = (size_t)m0 * H.nzc * H.plane, n = (size_t)nm * H.nzc * H.plane;
+      qg::launch_random_psi(g, H.st, m0, nm, (unsigned long long)seed, amplitude, slope, H.stg2, false);
+      qg::launch_convert<double, float>(reinterpret_cast<const double*>(H.stg2 + off),
+                                        static_cast<float*>(H.at(H.psi, off)), 2 * n, H.st);
+      H.psi64_m0 = m0;
+      H.psi64_nm = nm;
+    } else {
+      qg::launch_random_psi(g, H.st, m0, nm, (unsigned long long)seed, amplitude, slope, H.psi, false);
+    }
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(H.st));
     H.diag_valid = false;
@@ -1023,7 +1066,7 @@ int qg_set_surface_wave(qg_handle* h, int member, const double* profile) {
     qg::Geo g = H.g;
     g.nz = H.nz;
     g.members = H.members;
-    qg::launch_surface_wave(g, H.st, m0, nm, H.d_prof, H.B[H.cur]);
+    qg::launch_surface_wave(g, H.st, m0, nm, H.d_prof, H.B[H.cur], H.f32());
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(H.st));
     H.diag_valid = false;
@@ -1052,11 +1095,12 @@ int qg_energies(qg_handle* h, int member, double* out) {
     qg::Geo g = mv.g;
     g.members = nm;
     g.nz = nm * H.nzc;
-    qg::launch_thomas(g, H.st, H.q[H.cur] + mv.off, H.psis + mv.off, nullptr, H.th_inv[0] + mv.toff,
+    qg::launch_thomas(g, H.st, H.at(H.q[H.cur], mv.off), H.at(H.psis, mv.off), nullptr, H.f32(), H.th_inv[0] + mv.toff,
                       H.th_cp[0] + mv.toff, H.d_al + mv.aoff, 1, nullptr, H.tune[qg::P_THOMAS_PSI]);
-    qg::launch_thomas(g, H.st, H.B[H.cur] + mv.off, H.As + mv.off, H.qeff + mv.off, H.th_inv[1] + mv.toff,
+    qg::launch_thomas(g, H.st, H.at(H.B[H.cur], mv.off), H.at(H.As, mv.off), H.at(H.qeff, mv.off), H.f32(),
+                      H.th_inv[1] + mv.toff,
                       H.th_cp[1] + mv.toff, H.d_al + mv.aoff, 4, nullptr, H.tune[qg::P_THOMAS_AC]);
-    qg::launch_energies(all, H.st, m0, nm, H.psis, H.As, H.qeff, H.d_a, H.d_epart, H.d_energy);
+    qg::launch_energies(all, H.st, m0, nm, H.psis, H.As, H.qeff, H.f32(), H.d_a, H.d_epart, H.d_energy);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(out, H.d_energy, 5 * (size_t)nm * sizeof(double), hipMemcpyDeviceToHost, H.st));
     HIPCHECK(hipStreamSynchronize(H.st));
@@ -1140,6 +1184,7 @@ int qg_move_member(qg_handle* h, int src, int dst) {
     check_member_index(H, src);
     check_member_index(H, dst);
     if (src == dst) return;
+    H.psi64_nm = 0;
     const size_t lv = (size_t)H.nzc * H.plane;
     auto mv = [&](void* base, size_t elem, size_t count) {
       char* b = static_cast<char*>(base);
@@ -1147,7 +1192,7 @@ int qg_move_member(qg_handle* h, int src, int dst) {
                               hipMemcpyDeviceToDevice, H.st));
     };
     // prognostic state and the diagnostics a caller may read back
-    for (D* f : {H.q[H.cur], H.B[H.cur], H.psi, H.A, H.C}) mv(f, sizeof(D), lv);
+    for (void* f : {H.q[H.cur], H.B[H.cur], H.psi, H.A, H.C}) mv(f, H.fsize(), lv);
     for (void* f : {H.u, H.v, H.w}) mv(f, H.rsize(), lv);
     // the member's coefficients and recorded eliminations
     const int kpad = qg::thomas_shifted_a_size(H.nzc);
@@ -1193,6 +1238,7 @@ int qg_check_member(qg_handle* h, int member, int* nonfinite, double* max_abs_ps
 
 int qg_invert_q_to_psi(qg_handle* h) {
   return guarded(h, [&] {
+    h->psi64_nm = 0;
     solve_psi(*h, h->q[h->cur], h->psi);
     h->diag_valid = false;
     check_status(*h);
@@ -1213,7 +1259,8 @@ int qg_invert_B_to_A_member(qg_handle* h, int member) {
     check_member_index(H, member);
     if (H.sing_A[0]) raise_singular(H.sing_A);
     const MemberView mv(H, member);
-    qg::launch_thomas(mv.g, H.st, H.B[H.cur] + mv.off, H.A + mv.off, H.C + mv.off, H.th_inv[1] + mv.toff,
+    qg::launch_thomas(mv.g, H.st, H.at(H.B[H.cur], mv.off), H.at(H.A, mv.off), H.at(H.C, mv.off), H.f32(),
+                      H.th_inv[1] + mv.toff,
                       H.th_cp[1] + mv.toff, H.d_al + mv.aoff, 4, nullptr, H.tune[qg::P_THOMAS_AC]);
     H.diag_valid = false;
     check_status(H);
@@ -1226,7 +1273,9 @@ int qg_compute_q_from_psi_member(qg_handle* h, int member, int method) {
     check_member_index(H, member);
     if (method != 0 && method != 1) throw QgError(QG_ERR_ARGUMENT, "pv method must be 0 (qg) or 1 (barotropic)");
     const MemberView mv(H, member);
-    qg::launch_q_from_psi(mv.g, H.st, H.psi + mv.off, H.q[H.cur] + mv.off, H.d_a + (size_t)member * H.nzc, method);
+    const bool staged = H.f32() && member >= H.psi64_m0 && member < H.psi64_m0 + H.psi64_nm;
+    qg::launch_q_from_psi(mv.g, H.st, staged ? static_cast<void*>(H.stg2 + mv.off) : H.at(H.psi, mv.off),
+                          H.at(H.q[H.cur], mv.off), H.f32(), H.d_a + (size_t)member * H.nzc, method, staged);
     H.diag_valid = false;
     HIPCHECK(hipStreamSynchronize(H.st));
   });
@@ -1243,7 +1292,9 @@ int qg_compute_velocities(qg_handle* h, int compute_w) {
 int qg_compute_q_from_psi(qg_handle* h, int method) {
   return guarded(h, [&] {
     if (method != 0 && method != 1) throw QgError(QG_ERR_ARGUMENT, "pv method must be 0 (qg) or 1 (barotropic)");
-    qg::launch_q_from_psi(h->g, h->st, h->psi, h->q[h->cur], h->d_a, method);
+    const bool staged = h->f32() && h->psi64_m0 == 0 && h->psi64_nm >= h->g.members;
+    qg::launch_q_from_psi(h->g, h->st, staged ? static_cast<void*>(h->stg2) : h->psi, h->q[h->cur], h->f32(),
+                          h->d_a, method, staged);
     h->diag_valid = false;
     HIPCHECK(hipStreamSynchronize(h->st));
   });
@@ -1254,8 +1305,8 @@ int qg_compute_qw(qg_handle* h, double* host_out) {
     if (!host_out) throw QgError(QG_ERR_ARGUMENT, "null host buffer");
     if (!(std::isfinite(h->f) && h->f != 0))
       throw QgError(QG_ERR_ARGUMENT, "wave feedback requires a finite, nonzero Coriolis frequency");
-    with_engine(*h, [&](auto& e) { e.compute_qw(h->qeff); });
-    qg::launch_transpose<D>(h->qeff, h->stg, h->nx, h->ny, h->nz, false, h->st);
+    with_engine(*h, [&](auto& e) { e.compute_qw(e.c(h->qeff)); });
+    qg::launch_transpose<D>(as_fp64(*h, h->qeff, 0, h->npts), h->stg, h->nx, h->ny, h->nz, false, h->st);
     HIPCHECK(hipMemcpyAsync(host_out, h->stg, h->npts * sizeof(D), hipMemcpyDeviceToHost, h->st));
     HIPCHECK(hipStreamSynchronize(h->st));
   });
@@ -1267,9 +1318,10 @@ int qg_fft(qg_handle* h, const double* host_in, double* host_out, int direction)
     if (direction != -1 && direction != 1) throw QgError(QG_ERR_ARGUMENT, "direction must be -1 or +1");
     qg_handle& H = *h;
     HIPCHECK(hipMemcpyAsync(H.stg, host_in, H.npts * sizeof(D), hipMemcpyHostToDevice, H.st));
-    qg::launch_transpose<D>(H.stg, H.qeff, H.nx, H.ny, H.nz, true, H.st);
-    fft2_inplace(H, H.qeff, direction);
-    qg::launch_transpose<D>(H.qeff, H.stg, H.nx, H.ny, H.nz, false, H.st);
+    H.psi64_nm = 0;
+    qg::launch_transpose<D>(H.stg, H.stg2, H.nx, H.ny, H.nz, true, H.st);
+    fft2_inplace(H, H.stg2, direction);
+    qg::launch_transpose<D>(H.stg2, H.stg, H.nx, H.ny, H.nz, false, H.st);
     HIPCHECK(hipMemcpyAsync(host_out, H.stg, H.npts * sizeof(D), hipMemcpyDeviceToHost, H.st));
     HIPCHECK(hipStreamSynchronize(H.st));
   });
@@ -1310,20 +1362,22 @@ bool valid_choice(const qg_handle& h, int pass, int v) {
 
 void autotune(qg_handle& H, double dt, int steps) {
   if (H.prof_mode) throw QgError(QG_ERR_ARGUMENT, "autotune while a profile is being recorded");
+  H.psi64_nm = 0;
   HIPCHECK(hipStreamSynchronize(H.st));
   const size_t n = (size_t)H.nzc * H.plane * H.active;
-  D* save = nullptr;
-  HIPCHECK(hipMalloc(&save, 2 * n * sizeof(D)));
+  char* save = nullptr;
+  const size_t fb = n * H.fsize();  // bytes of one saved field
+  HIPCHECK(hipMalloc(&save, 2 * fb));
   struct Cleanup {
     qg_handle& H;
-    D* save;
-    size_t n;
+    char* save;
+    size_t fb;
     int cur;
     ~Cleanup() {
       (void)hipStreamSynchronize(H.st);
       if (H.cur != cur) H.cur = cur;
-      (void)hipMemcpyAsync(H.q[H.cur], save, n * sizeof(D), hipMemcpyDeviceToDevice, H.st);
-      (void)hipMemcpyAsync(H.B[H.cur], save + n, n * sizeof(D), hipMemcpyDeviceToDevice, H.st);
+      (void)hipMemcpyAsync(H.q[H.cur], save, fb, hipMemcpyDeviceToDevice, H.st);
+      (void)hipMemcpyAsync(H.B[H.cur], save + fb, fb, hipMemcpyDeviceToDevice, H.st);
       (void)hipStreamSynchronize(H.st);
       (void)hipFree(save);
       H.prof_mode = 0;
@@ -1333,9 +1387,9 @@ void autotune(qg_handle& H, double dt, int steps) {
       H.t1_valid = false;
       H.drop_graphs();
     }
-  } cleanup{H, save, n, H.cur};
-  HIPCHECK(hipMemcpyAsync(save, H.q[H.cur], n * sizeof(D), hipMemcpyDeviceToDevice, H.st));
-  HIPCHECK(hipMemcpyAsync(save + n, H.B[H.cur], n * sizeof(D), hipMemcpyDeviceToDevice, H.st));
+  } cleanup{H, save, fb, H.cur};
+  HIPCHECK(hipMemcpyAsync(save, H.q[H.cur], fb, hipMemcpyDeviceToDevice, H.st));
+  HIPCHECK(hipMemcpyAsync(save + fb, H.B[H.cur], fb, hipMemcpyDeviceToDevice, H.st));
   build_etd(H, dt);
   const std::vector<int> shapes = column_candidates(H.nzc);
   const int ncfg = std::max<int>(qg::kTileVariants, (int)shapes.size());
@@ -1350,8 +1404,8 @@ void autotune(qg_handle& H, double dt, int steps) {
     for (int p = 0; p < qg::P_FFT_END; ++p) H.tune[p] = fv;
     for (int p = qg::P_FFT_END; p < qg::P_END; ++p) H.tune[p] = cs;
     HIPCHECK(hipStreamSynchronize(H.st));
-    HIPCHECK(hipMemcpyAsync(H.q[cleanup.cur], save, n * sizeof(D), hipMemcpyDeviceToDevice, H.st));
-    HIPCHECK(hipMemcpyAsync(H.B[cleanup.cur], save + n, n * sizeof(D), hipMemcpyDeviceToDevice, H.st));
+    HIPCHECK(hipMemcpyAsync(H.q[cleanup.cur], save, fb, hipMemcpyDeviceToDevice, H.st));
+    HIPCHECK(hipMemcpyAsync(H.B[cleanup.cur], save + fb, fb, hipMemcpyDeviceToDevice, H.st));
     H.cur = cleanup.cur;
     H.diag_valid = false;
     H.t1_valid = false;

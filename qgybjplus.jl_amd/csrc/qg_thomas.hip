@@ -98,7 +98,7 @@ void launch_thomas_factor(const Geo& g, hipStream_t st, const double* a, int kdi
 
 bool thomas_setup() { return colscan_setup(); }
 
-void launch_thomas(const Geo& g, hipStream_t st, const cplx<double>* rhs, cplx<double>* out, cplx<double>* Cout,
+void launch_thomas(const Geo& g, hipStream_t st, const void* rhs, void* out, void* Cout, bool f32,
                    const double* inv, const double* cp, const double* al, int kdiv, unsigned long long* warn,
                    int shape) {
   if (shape < 0) shape = scan_default_shape(g.nzc);
@@ -112,11 +112,11 @@ void launch_thomas(const Geo& g, hipStream_t st, const cplx<double>* rhs, cplx<d
   a.lv0_stride = thomas_shifted_a_size(g.nzc);
   if (kdiv == 1) {
     a.warn = warn;
-    launch_colscan<SCAN_PSI>(g, st, shape, a);
+    launch_colscan<SCAN_PSI>(g, st, shape, a, f32);
   } else if (Cout) {
-    launch_colscan<SCAN_AC>(g, st, shape, a);
+    launch_colscan<SCAN_AC>(g, st, shape, a, f32);
   } else {
-    launch_colscan<SCAN_A>(g, st, shape, a);
+    launch_colscan<SCAN_A>(g, st, shape, a, f32);
   }
 }
 

@@ -93,14 +93,16 @@ def test_autotune_restores_state_and_caches(qg):
     assert all(-1 <= choice[p] <= 2 for p in FFT)
     for n in ("q", "B"):
         assert np.array_equal(m._h.get_field(n), before[n]), n
-    # stepping after tuning matches an untuned model to round-off
+    # stepping after tuning matches an untuned model to round-off: the
+    # column shapes differ in the last FP64 bits, which the FP32 field
+    # storage of mixed mode rounds to at most an FP32 ulp here and there
     u = _model(qg, (64, 64, 16), "mixed")
     _init(qg, u)
     ts = qg.ExponentialRungeKutta2(dt=300.0)
     qg.step_(m, ts, 6)
     qg.step_(u, ts, 6)
     for n in ("q", "B", "psi", "A"):
-        assert _rel(m._h.get_field(n), u._h.get_field(n)) <= 1e-12, n
+        assert _rel(m._h.get_field(n), u._h.get_field(n)) <= 1e-6, n
     # a second model of the same grid class takes the cached choice
     assert u.autotune(dt=300.0) == choice
     with pytest.raises(ValueError):
