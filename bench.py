@@ -33,30 +33,57 @@ EXTENT = (500e3, 500e3, 4000.0)
 DT = 20.0
 
 
-# Kernel symbols behind the library's per-launch names (several names share a
-# symbol: the stage and final launches of one kernel, the two velocity passes).
-SYMBOL_OF = {
-    "thomas_psi": "k_thomas<0,", "thomas_A": "k_thomas<1,", "thomas_AC": "k_thomas<2,",
-    "omega_gtsv": "k_omega<", "yb2_B_stage": "k_yb2_B<", "yb2_B_final": "k_yb2_B<",
-    "yb2_q_stage": "k_yb2_q<", "yb2_q_final": "k_yb2_q<", "yb_adv": "k_yb_adv<", "yb_uv": "k_yb_adv<",
-    "xb_B": "k_xb_B<", "xb_q": "k_xb_q<", "ya_qw": "k_ya_qw<", "ya_T1": "k_ya_qw<", "xa_qw": "k_xa_qw<",
-    "ya2_qeff": "k_ya2_qeff<", "yf_omega": "k_yf_omega<", "xf": "k_xf<", "y_r2c_omega": "k_y_r2c_mask<",
-    "y_herm_w": "k_y_herm<", "x_c2r_uv": "k_x_c2r<", "x_c2r_w": "k_x_c2r<",
+# Kernel behind each of the library's per-launch names: (kernel, tuned pass,
+# launch kind).  Several names share a kernel (the two velocity passes); the
+# stage and final launches of one kernel are told apart by launch order in the
+# rocprofv3 summary.
+LAUNCH_KERNEL = {
+    "ya_T1": ("k_ya_qw", "ya_qw", None), "ya_qw": ("k_ya_qw", "ya_qw", None),
+    "xa_qw": ("k_xa_qw", "xa_qw", None), "ya2_qeff": ("k_ya2_qeff", "ya2_qeff", None),
+    "yb_uv": ("k_yb_adv", "yb_adv", None), "yb_adv": ("k_yb_adv", "yb_adv", None),
+    "x_c2r_uv": ("k_x_c2r", "x_c2r", None), "x_c2r_w": ("k_x_c2r", "x_c2r", None),
+    "xb_q": ("k_xb_q", "xb_q", None), "xb_B": ("k_xb_B", "xb_B", None),
+    "yb2_q_stage": ("k_yb2_q", "yb2_q", "stage"), "yb2_q_final": ("k_yb2_q", "yb2_q", "final"),
+    "yb2_B_stage": ("k_yb2_B", "yb2_B", "stage"), "yb2_B_final": ("k_yb2_B", "yb2_B", "final"),
+    "yf_omega": ("k_yf_omega", "yf_omega", None), "xf": ("k_xf", "xf", None),
+    "y_r2c_omega": ("k_y_r2c_mask", "y_r2c", None), "y_herm_w": ("k_y_herm", "y_herm", None),
+    "thomas_psi": ("k_colscan<0", "thomas_psi", None), "thomas_A": ("k_colscan<1", "thomas_A", None),
+    "thomas_AC": ("k_colscan<2", "thomas_AC", None), "omega_gtsv": ("k_colscan<3", "omega", None),
 }
+# Column-solve launch shapes (levels per partition, quarter columns per
+# block) in the library's shape-index order, and the default index for a
+# column depth above 8.
+SCAN_SHAPES = ((8, 8), (16, 8), (32, 8), (16, 4), (32, 4), (64, 4))
+SCAN_DEFAULT = 1
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_256x256x128_mixed.json")
 
 
-def measured_traffic(name):
+def kernel_symbol(name, choice):
+    """The demangled symbol rocprofv3 reports for launch `name` of the FP32
+    mixed-precision model at 256x256 under tuning `choice` (pass -> variant
+    or shape index, -1 = default), with its launch kind."""
+    base, tpass, kind = LAUNCH_KERNEL[name]
+    v = int((choice or {}).get(tpass, -1))
+    if base.startswith("k_colscan"):
+        m, qc = SCAN_SHAPES[v if v >= 0 else SCAN_DEFAULT]
+        return f"{base}, {m}, {qc}, float>", kind
+    return f"{base}<float, {SIZE[0]}, {v if v >= 0 else 1}>", kind
+
+
+def measured_traffic(name, choice):
     """HBM bytes per launch of `name` from the committed rocprofv3 FETCH_SIZE +
     WRITE_SIZE summary (already scaled by the copy-kernel calibration), or
     None when that summary has no entry for the kernel."""
-    sym = SYMBOL_OF.get(name)
-    if sym is None or not os.path.exists(TRAFFIC_FILE):
+    if name not in LAUNCH_KERNEL or not os.path.exists(TRAFFIC_FILE):
         return None
+    sym, kind = kernel_symbol(name, choice)
     with open(TRAFFIC_FILE) as fh:
-        kernels = json.load(fh).get("kernels", {})
-    entry = kernels.get(sym)
-    return None if entry is None else entry["hbm_bytes_per_launch"]
+        entry = json.load(fh).get("kernels", {}).get(sym)
+    if entry is None:
+        return None
+    if kind is not None and kind in entry.get("by_launch", {}):
+        entry = entry["by_launch"][kind]
+    return entry["hbm_bytes_per_launch"]
 
 
 def log(msg):
@@ -332,7 +359,7 @@ def main():
     bytes_per_launch = kern[dom]["bytes"] / kern[dom]["count"]
     achieved = bytes_per_launch / avg_s
     steps_ms = np.array(s32["steps_ms"]) if s32["steps_ms"] else np.array([t32 / args.steps * 1e3])
-    traffic = measured_traffic(dom)
+    traffic = measured_traffic(dom, tuning.get("mixed", {}).get("choice"))
 
     per_kernel = {k: {"launches": v["count"], "avg_us": 1e3 * v["ms"] / v["count"],
                       "GBps": v["bytes"] / (v["ms"] / 1e3) / 1e9 if v["ms"] > 0 else None,
