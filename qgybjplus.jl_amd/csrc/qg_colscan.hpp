@@ -154,7 +154,7 @@ __device__ inline void scan_store(double v, __amdgpu_buffer_rsrc_t r, unsigned o
   }
 }
 
-// blockIdx.x: tile of QC quarter columns; blockIdx.y: member.
+// Logical block x: tile of QC quarter columns; y: member (xcd_block()).
 template <int OP, int M, int QC, typename T>
 __global__ void __launch_bounds__((ScanBounds<M, QC>::MAXT))
 k_colscan(Geo g, Quarter qr, ScanArgs a) {
@@ -167,9 +167,10 @@ k_colscan(Geo g, Quarter qr, ScanArgs a) {
   const int w = threadIdx.x / TP, t = threadIdx.x % TP;
   const int part = t & 1, cs = t >> 1, j = cs % QC, mir = cs / QC;
   const int rplane = qr.size();
-  const int rc0 = blockIdx.x * QC;
+  const Blk blk = xcd_block();
+  const int rc0 = blk.x * QC;
   const size_t plane = (size_t)g.nx * g.ny;
-  const size_t mem = blockIdx.y;
+  const size_t mem = blk.y;
   const T* rhs = static_cast<const T*>(a.rhs) + 2 * mem * nz * plane;
   T* out = static_cast<T*>(a.out) + 2 * mem * nz * plane;
   const double* t0 = a.t0 + mem * nz * rplane;
@@ -276,7 +277,7 @@ k_colscan(Geo g, Quarter qr, ScanArgs a) {
 
   // q -> psi: psi = 0 on the horizontal mean; warn on a significant mean of
   // q (src/elliptic.jl:257-270).  Block 0 holds that column.
-  if (OP == SCAN_PSI && a.warn && nz > 1 && blockIdx.x == 0) {
+  if (OP == SCAN_PSI && a.warn && nz > 1 && blk.x == 0) {
     double m = 0.0;
 #pragma unroll
     for (int u = 0; u < M; ++u) {
@@ -343,7 +344,7 @@ k_colscan(Geo g, Quarter qr, ScanArgs a) {
     }
     // B -> A: remove the vertical mean of the kh = 0 column
     // (src/elliptic.jl:930-945).  XF is free after the second barrier.
-    if ((OP == SCAN_A || OP == SCAN_AC) && blockIdx.x == 0) {
+    if ((OP == SCAN_A || OP == SCAN_AC) && blk.x == 0) {
       double s = 0.0;
 #pragma unroll
       for (int u = 0; u < M; ++u)

@@ -61,4 +61,22 @@ struct Geo {
 
 __host__ __device__ inline int mirror(int i, int n) { return i == 0 ? 0 : n - i; }
 
+// Workgroups reach the 8 XCDs of the chip (each with its own L2) round-robin
+// by linear workgroup id.  xcd_block() renumbers a 2-D grid so that every XCD
+// takes a contiguous range of logical blocks (x fastest) in the order it
+// receives them: neighbouring tiles, whose edges share cache lines, then run
+// on one XCD close together in time and the second reads hit its L2.  The
+// renumbering is a permutation of the grid, so results do not depend on it.
+constexpr int kXcds = 8;
+struct Blk {
+  int x, y;
+};
+__device__ inline Blk xcd_block() {
+  const int gx = gridDim.x, total = gx * gridDim.y;
+  const int l = blockIdx.x + gx * blockIdx.y;
+  const int xcd = l % kXcds, loc = l / kXcds, q = total / kXcds, r = total % kXcds;
+  const int L = xcd * q + (xcd < r ? xcd : r) + loc;
+  return Blk{L % gx, L / gx};
+}
+
 }  // namespace qg

@@ -64,7 +64,8 @@ k_ya_qw(Geo g, const cplx<R>* __restrict__ B, cplx<R>* __restrict__ T1,
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
   TwReg<R, TL::NN> twr;
   twr.load(lane, tw);
-  const int x = blockIdx.x * TL::S + s, z = blockIdx.y;
+  const Blk blk = xcd_block();
+  const int x = blk.x * TL::S + s, z = blk.y;
   const bool ok = x < g.nx;
   cplx<R>* sl = lds + s * TL::LD;
   cplx<R> v[F::IT][F::R0], w[F::IT][F::R0];
@@ -177,7 +178,8 @@ k_ya2_qeff(Geo g, const cplx<R>* __restrict__ Qa, const cplx<R>* __restrict__ Qb
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
   TwReg<R, TL::NN> twr;
   twr.load(lane, tw);
-  const int x = blockIdx.x * TL::S + s, z = blockIdx.y;
+  const Blk blk = xcd_block();
+  const int x = blk.x * TL::S + s, z = blk.y;
   const bool ok = x < g.W;
   cplx<R>* sl = lds + s * TL::LD;
   cplx<R> v[F::IT][F::R0], w[F::IT][F::R0];
@@ -235,7 +237,8 @@ k_yb_adv(Geo g, int outs, const cplx<R>* __restrict__ psi, const cplx<R>* __rest
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
   TwReg<R, TL::NN> twr;
   twr.load(lane, tw);
-  const int x = blockIdx.x * TL::S + s, z = blockIdx.y;
+  const Blk blk = xcd_block();
+  const int x = blk.x * TL::S + s, z = blk.y;
   const bool ok = x < g.W;
   const int mx = mirror(x, g.nx);
   cplx<R>* sl = lds + s * TL::LD;
@@ -495,7 +498,8 @@ k_yb2_q(Geo g, int mode, int flags, const cplx<R>* __restrict__ G1, const cplx<R
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
   TwReg<R, TL::NN> twr;
   twr.load(lane, tw);
-  const int x = blockIdx.x * TL::S + s, z = blockIdx.y;
+  const Blk blk = xcd_block();
+  const int x = blk.x * TL::S + s, z = blk.y;
   const bool ok = x < g.W;
   cplx<R>* sl = lds + s * TL::LD;
   cplx<R> a[F::IT][F::R0], b[F::IT][F::R0];
@@ -573,7 +577,8 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
   TwReg<R, TL::NN> twr;
   twr.load(lane, tw);
-  const int x = blockIdx.x * TL::S + s, z = blockIdx.y;
+  const Blk blk = xcd_block();
+  const int x = blk.x * TL::S + s, z = blk.y;
   const bool ok = x < g.nx;
   cplx<R>* sl = lds + s * TL::LD;
   cplx<double> acc[F::IT][F::R0];
@@ -669,7 +674,8 @@ k_yf_omega(Geo g, const cplx<R>* __restrict__ psi, cplx<R>* __restrict__ Pbx, cp
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
   TwReg<R, TL::NN> twr;
   twr.load(lane, tw);
-  const int x = blockIdx.x * TL::S + s, z = blockIdx.y;
+  const Blk blk = xcd_block();
+  const int x = blk.x * TL::S + s, z = blk.y;
   const bool ok = x < g.W;
   const int mx = mirror(x, g.nx);
   cplx<R>* sl = lds + s * TL::LD;
@@ -798,7 +804,8 @@ k_y_r2c_mask(Geo g, const cplx<R>* __restrict__ Rw, cplx<R>* __restrict__ out,
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
   TwReg<R, TL::NN> twr;
   twr.load(lane, tw);
-  const int x = blockIdx.x * TL::S + s, z = blockIdx.y;
+  const Blk blk = xcd_block();
+  const int x = blk.x * TL::S + s, z = blk.y;
   const bool ok = x < g.W;
   cplx<R>* sl = lds + s * TL::LD;
   cplx<R> v[F::IT][F::R0];
@@ -835,7 +842,8 @@ k_y_herm(Geo g, const cplx<R>* __restrict__ in, cplx<R>* __restrict__ P,
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
   TwReg<R, TL::NN> twr;
   twr.load(lane, tw);
-  const int x = blockIdx.x * TL::S + s, z = blockIdx.y;
+  const Blk blk = xcd_block();
+  const int x = blk.x * TL::S + s, z = blk.y;
   const bool ok = x < g.W;
   const int mx = mirror(x, g.nx);
   cplx<R>* sl = lds + s * TL::LD;
@@ -877,7 +885,8 @@ k_y_c2c(Geo g, const cplx<double>* __restrict__ in, cplx<double>* __restrict__ o
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
   TwReg<R, TL::NN> twr;
   twr.load(lane, tw);
-  const int x = blockIdx.x * TL::S + s, z = blockIdx.y;
+  const Blk blk = xcd_block();
+  const int x = blk.x * TL::S + s, z = blk.y;
   const bool ok = x < g.nx;
   cplx<R>* sl = lds + s * TL::LD;
   cplx<R> v[F::IT][F::R0];
