@@ -393,12 +393,15 @@ k_xb_q(Geo g, const cplx<R>* __restrict__ Pq, const R* __restrict__ u, const R* 
 }
 
 // X-pass for B: B = IFFTx(T1), zeta = C2R(Pz); u*B, v*B, zeta*B -> forward
-// complex FFTs -> G3, G4, G5 (full width).
+// complex FFTs along x.  The x-derivative of the flux and the refraction
+// factor are per-kx multipliers, so they are applied here and the two terms
+// leave as one field: G3 = -i kx F(uB) - (i/2) F(zeta B), G4 = F(vB) (full
+// width).  k_yb2_B adds -i ky F_y(G4) after the y transform.
 template <typename R, int NX, int V>
 __global__ void __launch_bounds__((Tile<R, NX, V>::THREADS))
 k_xb_B(Geo g, int flags, const cplx<R>* __restrict__ T1, const cplx<R>* __restrict__ Pz,
        const R* __restrict__ u, const R* __restrict__ vv, cplx<R>* __restrict__ G3,
-       cplx<R>* __restrict__ G4, cplx<R>* __restrict__ G5, const cplx<R>* __restrict__ tw) {
+       cplx<R>* __restrict__ G4, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NX, V>;
   using FI = Fft<R, NX, +1>;
   using FF = Fft<R, NX, -1>;
@@ -410,7 +413,7 @@ k_xb_B(Geo g, int flags, const cplx<R>* __restrict__ T1, const cplx<R>* __restri
   const bool ok = row < g.nz * g.ny;
   cplx<R>* sl = lds + r * TL::LD;
   const size_t base = (size_t)row * NX;
-  cplx<R> b[FI::IT][FI::R0], v[FI::IT][FI::R0];
+  cplx<R> b[FI::IT][FI::R0], v[FI::IT][FI::R0], h[FI::IT][FI::R0];
   R zt[FI::IT][FI::R0];
 #pragma unroll
   for (int it = 0; it < FI::IT; ++it)
@@ -418,6 +421,7 @@ k_xb_B(Geo g, int flags, const cplx<R>* __restrict__ T1, const cplx<R>* __restri
     for (int t = 0; t < FI::R0; ++t) {
       int x = FI::elem(lane, it, t);
       b[it][t] = (ok && FI::valid(lane, it)) ? T1[base + x] : czero<R>();
+      h[it][t] = czero<R>();
     }
   FI::run(b, sl, lane, twr);
   if (flags & XB_REFR) {
@@ -428,10 +432,10 @@ k_xb_B(Geo g, int flags, const cplx<R>* __restrict__ T1, const cplx<R>* __restri
 #pragma unroll
       for (int t = 0; t < FI::R0; ++t) zt[it][t] = v[it][t].re;
   }
+  // f = 0: u*B, 1: v*B, 2: zeta*B
   for (int f = 0; f < 3; ++f) {
     if (f < 2 && !(flags & XB_ADV)) continue;
     if (f == 2 && !(flags & XB_REFR)) continue;
-    cplx<R>* dst = f == 0 ? G3 : f == 1 ? G4 : G5;
 #pragma unroll
     for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
@@ -448,9 +452,21 @@ k_xb_B(Geo g, int flags, const cplx<R>* __restrict__ T1, const cplx<R>* __restri
 #pragma unroll
       for (int t = 0; t < FI::R0; ++t) {
         int x = FI::elem(lane, it, t);
-        if (ok && FI::valid(lane, it)) dst[base + x] = v[it][t];
+        if (f == 1) {
+          if (ok && FI::valid(lane, it)) G4[base + x] = v[it][t];
+        } else {
+          const R m = f == 0 ? (R)-g.kx[x] : (R)-0.5;
+          h[it][t] = h[it][t] + ik_mul(m, v[it][t]);
+        }
       }
   }
+#pragma unroll
+  for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < FI::R0; ++t) {
+      int x = FI::elem(lane, it, t);
+      if (ok && FI::valid(lane, it)) G3[base + x] = h[it][t];
+    }
 }
 
 // ETD tables per column, layout [6][ny][nx]: Eq, hphi1q, hphi2q, EB, hphi1B, hphi2B.
@@ -563,13 +579,14 @@ k_yb2_q(Geo g, int mode, int flags, const cplx<R>* __restrict__ G1, const cplx<R
     }
 }
 
-// Y-pass (forward, full width): nB = mask*(i kx F(uB) + i ky F(vB)), rB =
-// mask*F(zeta B); rhs_B = -nB + i (f/2) kh2 A - (i/2) rB; ETD update of B.
+// Y-pass (forward, full width) of G3 = -i kx F_x(uB) - (i/2) F_x(zeta B) and
+// G4 = F_x(vB): rhs_B = mask*(F_y(G3) - i ky F_y(G4)) + i (f/2) kh2 A, i.e.
+// -nB - (i/2) rB + i (f/2) kh2 A with nB = i kx F(uB) + i ky F(vB) and
+// rB = F(zeta B); then the ETD update of B.
 template <typename R, int NY, int V>
 __global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
 k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R>* __restrict__ G4,
-        const cplx<R>* __restrict__ G5, const cplx<R>* __restrict__ A,
-        const cplx<R>* __restrict__ Bn, cplx<R>* __restrict__ rhs0,
+        const cplx<R>* __restrict__ A, const cplx<R>* __restrict__ Bn, cplx<R>* __restrict__ rhs0,
         cplx<R>* __restrict__ Bout, Etd etd, double alpha, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY, V>;
   using F = Fft<R, NY, -1>;
@@ -587,11 +604,10 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
   for (int it = 0; it < F::IT; ++it)
 #pragma unroll
     for (int t = 0; t < F::R0; ++t) acc[it][t] = czero<double>();
-  const double kxv = ok ? g.kx[x] : 0.0;
-  for (int f = 0; f < 3; ++f) {
-    if (f < 2 && !(flags & YB_ADV)) continue;
-    if (f == 2 && !(flags & YB_REFR)) continue;
-    const cplx<R>* src = f == 0 ? G3 : f == 1 ? G4 : G5;
+  for (int f = 0; f < 2; ++f) {
+    if (f == 0 && !(flags & (YB_ADV | YB_REFR))) continue;
+    if (f == 1 && !(flags & YB_ADV)) continue;
+    const cplx<R>* src = f == 0 ? G3 : G4;
 #pragma unroll
     for (int it = 0; it < F::IT; ++it)
 #pragma unroll
@@ -601,18 +617,18 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
         v[it][t] = in ? src[((size_t)z * NY + y) * g.nx + x] : czero<R>();
       }
     F::run(v, sl, lane, twr);
-    if (f == 2) break;  // v keeps F(zeta*B) for the combine below
 #pragma unroll
     for (int it = 0; it < F::IT; ++it)
 #pragma unroll
       for (int t = 0; t < F::R0; ++t) {
         int y = F::elem(lane, it, t);
         cplx<double> c = cvt<double>(v[it][t]);
-        if (f == 0) acc[it][t] = ik_mul(kxv, c);                      // i kx F(uB)
-        else acc[it][t] = acc[it][t] + ik_mul(g.ky[y], c);           // + i ky F(vB)
+        if (f == 0) acc[it][t] = c;                                    // -i kx F(uB) - (i/2) F(zeta B)
+        else acc[it][t] = acc[it][t] + ik_mul(-g.ky[y], c);           // - i ky F(vB)
       }
   }
   const size_t plane = (size_t)g.nx * NY;
+  const double kxv = ok ? g.kx[x] : 0.0;
 #pragma unroll
   for (int it = 0; it < F::IT; ++it)
 #pragma unroll
@@ -624,8 +640,6 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
       const size_t ad = (size_t)z * plane + col;
       cplx<double> rhs = czero<double>();
       if (keep) {
-        cplx<double> nB = (flags & YB_ADV) ? acc[it][t] : czero<double>();
-        cplx<double> rB = (flags & YB_REFR) ? cvt<double>(v[it][t]) : czero<double>();
         cplx<double> t1 = czero<double>();
         if (flags & YB_DISP) {
           const double kh2 = kxv * kxv + g.ky[y] * g.ky[y];
@@ -633,8 +647,7 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
           cplx<double> Av = cvt<double>(A[ad]);
           t1 = mk<double>(-(c * Av.im), c * Av.re);
         }
-        cplx<double> t2 = mk<double>(-(0.5 * rB.im), 0.5 * rB.re);
-        rhs = mk<double>(-nB.re + t1.re, -nB.im + t1.im) - t2;
+        rhs = acc[it][t] + t1;
       }
       cplx<double> out = czero<double>();
       if (keep) {

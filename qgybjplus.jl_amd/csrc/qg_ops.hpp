@@ -30,11 +30,11 @@ struct OpsV {
   static void xb_q(const Geo& g, hipStream_t st, const C* Pq, const R* u, const R* v, C* G1, C* G2,
                    const C* twx);
   static void xb_B(const Geo& g, hipStream_t st, int flags, const C* T1, const C* Pz, const R* u, const R* v,
-                   C* G3, C* G4, C* G5, const C* twx);
+                   C* G3, C* G4, const C* twx);
   static void yb2_q(const Geo& g, hipStream_t st, int mode, int flags, const C* G1, const C* G2, const C* qd,
                     const C* qn,
                     C* rhs0, C* qout, EtdView etd, double nu, double inv_dz2, const C* twy);
-  static void yb2_B(const Geo& g, hipStream_t st, int mode, int flags, const C* G3, const C* G4, const C* G5,
+  static void yb2_B(const Geo& g, hipStream_t st, int mode, int flags, const C* G3, const C* G4,
                     const C* A, const C* Bn, C* rhs0, C* Bout, EtdView etd, double alpha, const C* twy);
   static void yf_omega(const Geo& g, hipStream_t st, const C* psi, C* Pbx, C* Pby, C* Pxx, C* Pxy, const C* twy);
   static void xf(const Geo& g, hipStream_t st, int want_omega, const C* Pu, const C* Pv, const C* Pbx,

@@ -88,8 +88,8 @@ void OpsV<R, V>::xb_q(const Geo& g, hipStream_t st, const C* Pq, const R* u, con
 
 template <typename R, int V>
 void OpsV<R, V>::xb_B(const Geo& g, hipStream_t st, int flags, const C* T1, const C* Pz, const R* u, const R* v,
-                  C* G3, C* G4, C* G5, const C* twx) {
-#define CALL(N) XL((k_xb_B<R, N, V>), N, g, flags, T1, Pz, u, v, G3, G4, G5, twx)
+                  C* G3, C* G4, const C* twx) {
+#define CALL(N) XL((k_xb_B<R, N, V>), N, g, flags, T1, Pz, u, v, G3, G4, twx)
   QG_SWITCH(g.nx, CALL)
 #undef CALL
 }
@@ -105,10 +105,10 @@ void OpsV<R, V>::yb2_q(const Geo& g, hipStream_t st, int mode, int flags, const 
 }
 
 template <typename R, int V>
-void OpsV<R, V>::yb2_B(const Geo& g, hipStream_t st, int mode, int flags, const C* G3, const C* G4, const C* G5,
+void OpsV<R, V>::yb2_B(const Geo& g, hipStream_t st, int mode, int flags, const C* G3, const C* G4,
                    const C* A, const C* Bn, C* rhs0, C* Bout, EtdView etd, double alpha, const C* twy) {
   Etd e{etd.tab, etd.plane};
-#define CALL(N) YL((k_yb2_B<R, N, V>), N, g.nx, g, mode, flags, G3, G4, G5, A, Bn, rhs0, Bout, e, alpha, twy)
+#define CALL(N) YL((k_yb2_B<R, N, V>), N, g.nx, g, mode, flags, G3, G4, A, Bn, rhs0, Bout, e, alpha, twy)
   QG_SWITCH(g.ny, CALL)
 #undef CALL
 }

@@ -94,7 +94,7 @@ struct qg_handle {
   bool om_any_piv = false;  // some column's elimination interchanged rows
   int sing_omega[3] = {0, 0, 0};
   void *u = nullptr, *v = nullptr, *w = nullptr;
-  void *T1 = nullptr, *T3 = nullptr, *G3 = nullptr, *G4 = nullptr, *G5 = nullptr;
+  void *T1 = nullptr, *T3 = nullptr, *G3 = nullptr, *G4 = nullptr;
   void *Qa = nullptr, *Qb = nullptr, *Pu = nullptr, *Pv = nullptr, *Pq = nullptr, *Pz = nullptr,
        *G1 = nullptr, *G2 = nullptr;
   void *twx = nullptr, *twy = nullptr;  // intermediate precision
@@ -412,8 +412,8 @@ struct Engine {
         [&] { O::xb_q(h.tune[qg::P_XB_Q], h.g, h.st, c(h.Pq), r(h.u), r(h.v), c(h.G1), c(h.G2), twx()); });
     int xbf = (adv ? qg::XB_ADV : 0) | (refr ? qg::XB_REFR : 0);
     if (xbf)
-      K("xb_B", qg::P_XB_B, P * cR + (refr ? Ph * cR : 0) + (adv ? P * 2 * rR : 0) + P * cR * ((adv ? 2 : 0) + (refr ? 1 : 0)),
-        [&] { O::xb_B(h.tune[qg::P_XB_B], h.g, h.st, xbf, c(h.T1), c(h.Pz), r(h.u), r(h.v), c(h.G3), c(h.G4), c(h.G5), twx()); });
+      K("xb_B", qg::P_XB_B, P * cR + (refr ? Ph * cR : 0) + (adv ? P * 2 * rR : 0) + P * cR * (1 + (adv ? 1 : 0)),
+        [&] { O::xb_B(h.tune[qg::P_XB_B], h.g, h.st, xbf, c(h.T1), c(h.Pz), r(h.u), r(h.v), c(h.G3), c(h.G4), twx()); });
     const bool diff = h.nu_z != 0.0 && !h.inviscid && h.nz > 1;
     int qf = ((adv && qdyn) ? qg::YQ_ADV : 0) | (h.fixed_flow ? qg::YQ_FIXED : 0) | (diff ? qg::YQ_DIFF : 0);
     const double inv_dz2 = h.nz > 1 ? 1.0 / (h.dz * h.dz) : 0.0;
@@ -423,8 +423,8 @@ struct Engine {
     const bool disp = !(h.passive || h.no_disp);
     int bf = (adv ? qg::YB_ADV : 0) | (refr ? qg::YB_REFR : 0) | (disp ? qg::YB_DISP : 0);
     K(mode == qg::MODE_STAGE ? "yb2_B_stage" : "yb2_B_final", qg::P_YB2_B,
-      P * cR * ((adv ? 2 : 0) + (refr ? 1 : 0)) + P * cS * (3 + (disp ? 1 : 0)),
-      [&] { O::yb2_B(h.tune[qg::P_YB2_B], h.g, h.st, mode, bf, c(h.G3), c(h.G4), c(h.G5), AS, Bn, c(h.rB0), Bout, etd(), h.f / 2, twy()); });
+      P * cR * (((adv || refr) ? 1 : 0) + (adv ? 1 : 0)) + P * cS * (3 + (disp ? 1 : 0)),
+      [&] { O::yb2_B(h.tune[qg::P_YB2_B], h.g, h.st, mode, bf, c(h.G3), c(h.G4), AS, Bn, c(h.rB0), Bout, etd(), h.f / 2, twy()); });
   }
 
   // step!(model, ExponentialRungeKutta2) -> _advance_etdrk2! (src/timestep.jl:335-415).
@@ -681,7 +681,7 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
     h->u = h->ralloc(h->npts);
     h->v = h->ralloc(h->npts);
     h->w = h->ralloc(h->npts);
-    for (void** p : {&h->T1, &h->T3, &h->G3, &h->G4, &h->G5}) *p = h->ralloc(2 * h->npts);
+    for (void** p : {&h->T1, &h->T3, &h->G3, &h->G4}) *p = h->ralloc(2 * h->npts);
     for (void** p : {&h->Qa, &h->Qb, &h->Pu, &h->Pv, &h->Pq, &h->Pz, &h->G1, &h->G2}) *p = h->ralloc(2 * h->nhalf);
     h->d_err = h->alloc<int>(4);
     h->d_ferr = h->alloc<int>(8);
@@ -779,7 +779,7 @@ int qg_memory_required(const qg_config* cfg, size_t* bytes) {
              + (r == 4 ? 2 : 1) * 16 * npts  // FP64 staging (the second aliases qeff in FP64 mode)
              + 3 * 8 * npts       // omega scratch
              + 3 * r * npts       // u, v, w
-             + 5 * 2 * r * npts   // full-width transform intermediates
+             + 4 * 2 * r * npts   // full-width transform intermediates
              + 8 * 2 * r * nhalf  // half-width transform intermediates
              + 6 * 8 * quarter * nz + quarter * members  // Thomas and omega tables
              + 6 * 8 * plane + plane + 4 * 16 * (nx + ny) + (4 * 8 * nz) + 2 * 8 * members * kpad;
