@@ -546,6 +546,12 @@ k_yb2_q(Geo g, int mode, int flags, const cplx<R>* __restrict__ G1, const cplx<R
       if ((flags & YQ_ADV) && keep)
         nq = ik_mul(g.kx[x], cvt<double>(a[it][t])) + ik_mul(g.ky[y], cvt<double>(b[it][t]));
       const int my = mirror(y, NY);
+      // The ETD coefficients depend on (kx, ky) through kh2 only, so the
+      // mirror column (mx, my) has those of (x, y).
+      const bool upd = keep && !(flags & YQ_FIXED);
+      const size_t col0 = (size_t)y * g.nx + x;
+      const double E = upd ? etd.at(0, col0) : 0.0, h1 = upd ? etd.at(1, col0) : 0.0;
+      const double h2 = (upd && mode != MODE_STAGE) ? etd.at(2, col0) : 0.0;
       for (int side = 0; side < 2; ++side) {
         if (side == 1 && mx == x) break;
         const int cx = side ? mx : x, cy = side ? my : y;
@@ -563,11 +569,9 @@ k_yb2_q(Geo g, int mode, int flags, const cplx<R>* __restrict__ G1, const cplx<R
           if (flags & YQ_FIXED) {
             out = q0;
           } else {
-            const double E = etd.at(0, col), h1 = etd.at(1, col);
             if (mode == MODE_STAGE) {
               out = scale(q0, E) + scale(rhs, h1);
             } else {
-              const double h2 = etd.at(2, col);
               cplx<double> r0 = cvt<double>(rhs0[ad]);
               out = (scale(q0, E) + scale(r0, h1)) + scale(rhs - r0, h2);
             }
