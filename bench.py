@@ -53,7 +53,7 @@ LAUNCH_KERNEL = {
 # Column-solve launch shapes (levels per partition, quarter columns per
 # block) in the library's shape-index order, and the default index for a
 # column depth above 8.
-SCAN_SHAPES = ((8, 8), (16, 8), (32, 8), (16, 4), (32, 4), (64, 4))
+SCAN_SHAPES = ((8, 8), (16, 8), (32, 8), (16, 4), (32, 4), (64, 4), (16, 16), (32, 16))
 SCAN_DEFAULT = 1
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_256x256x128_mixed.json")
 

@@ -80,7 +80,7 @@ constexpr int kRawBufferWord3 = 0x00020000;
 constexpr size_t kScanMaxLds = 96 * 1024;
 
 // The compiled (M, QC) pairs.
-constexpr int kScanShapes[][2] = {{8, 8}, {16, 8}, {32, 8}, {16, 4}, {32, 4}, {64, 4}};
+constexpr int kScanShapes[][2] = {{8, 8}, {16, 8}, {32, 8}, {16, 4}, {32, 4}, {64, 4}, {16, 16}, {32, 16}};
 constexpr int kScanShapeCount = sizeof(kScanShapes) / sizeof(kScanShapes[0]);
 
 inline bool scan_shape_fits(const ScanShape& s) {
@@ -103,8 +103,10 @@ inline int scan_default_shape(int nz) {
     ScanShape s = scan_shape_of(i, nz);
     if (s.QC == 8 && scan_shape_fits(s) && (s.M >= 16 || nz <= 8)) return i;
   }
-  for (int i = kScanShapeCount - 1; i >= 0; --i)
-    if (scan_shape_fits(scan_shape_of(i, nz))) return i;
+  for (int i = kScanShapeCount - 1; i >= 0; --i) {
+    ScanShape s = scan_shape_of(i, nz);
+    if (s.QC == 4 && scan_shape_fits(s)) return i;
+  }
   return -1;
 }
 

@@ -101,3 +101,17 @@ def test_poisson_trace_matches_its_distribution(qg):
     assert steps.min() >= 100 and steps.max() <= 2000
     assert abs(np.median(np.log(steps)) - 0.5 * (np.log(100) + np.log(2000))) < 0.1
     assert 0.45 < np.mean([r.stratification == "constant" for r in tr]) < 0.55
+
+
+def test_bench_scan_shapes_match_the_compiled_table():
+    """bench.py names column-solve kernels by (levels per thread, quarter
+    columns per tile) from the library's shape table; both lists agree."""
+    src = open(os.path.join(ROOT, "qgybjplus.jl_amd", "csrc", "qg_colscan.hpp")).read()
+    table = re.search(r"kScanShapes\[\]\[2\]\s*=\s*\{(.*?)\};", src, flags=re.S).group(1)
+    compiled = tuple((int(m), int(q)) for m, q in re.findall(r"\{\s*(\d+)\s*,\s*(\d+)\s*\}", table))
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_for_test", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    assert bench.SCAN_SHAPES == compiled
+    assert compiled[bench.SCAN_DEFAULT] == (16, 8)

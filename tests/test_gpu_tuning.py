@@ -70,7 +70,7 @@ def test_column_shapes_agree_with_default(qg, nz):
     m = _model(qg, (16, 16, nz), "fp64")
     ref = _run(qg, m, {})
     tried = 0
-    for shape in range(6):
+    for shape in range(8):  # every compiled shape (kScanShapes)
         try:
             out = _run(qg, m, {p: shape for p in COLUMN})
         except ValueError:  # the shape does not fit this depth
