@@ -136,8 +136,25 @@ int qg_get_field_member(qg_handle* h, int member, int field, double* host);
 int qg_step(qg_handle* h, double dt, int nsteps);
 
 /* Replaces _check_termination_conditions! (src/simulation.jl:25-44): any
- * non-finite value in q, B or psi, and max |ifft(psi)| over the grid. */
+ * non-finite value in q, B or psi, and max |ifft(psi)| over the grid (the
+ * complex inverse transform, computed at the mode's transform precision). */
 int qg_check(qg_handle* h, int* nonfinite, double* max_abs_psi);
+
+/* Replaces the stepping loop of run!(simulation) (src/simulation.jl:621-625):
+ * up to nsteps steps of dt, each followed by _check_termination_conditions!,
+ * with the checks made on the device (one host round trip per 32 steps).
+ * Stops right after the first step whose check fails, as run! does:
+ *   *done        steps taken, the failing step included;
+ *   *failure     0 none, 1 non-finite q, B or psi ("non-finite value
+ *                detected in the model state"), 2 max |ifft(psi)| > 1e10
+ *                ("solution appears to be blowing up");
+ *   *max_abs_psi the failing step's max |ifft(psi)|, else the largest over
+ *                the steps taken.
+ * The model state after a failure is the state right after the failing step.
+ * The first call allocates five complex and two real fields of scratch: the
+ * state every chunk of steps starts from (q, B, psi, A, the y-transformed B,
+ * u, v), kept to step again up to a failing step. */
+int qg_run(qg_handle* h, double dt, int nsteps, int* done, int* failure, double* max_abs_psi);
 /* The check of one ensemble member. */
 int qg_check_member(qg_handle* h, int member, int* nonfinite, double* max_abs_psi);
 

@@ -827,21 +827,49 @@ class Simulation:
         if self.state in (SimulationState.Running, SimulationState.Failed, SimulationState.Finalized):
             raise RuntimeError(f"cannot modify a {self.state.name.lower()} simulation")
 
-    def _before_stop_time(self) -> bool:
+    def _before_stop_time(self, t=None) -> bool:
+        """``_before_stop_time`` (src/simulation.jl:15-23) at model time t
+        (default: the clock's)."""
         if self.stop_time is None:
             return True
-        t = self.clock.time
+        t = self.clock.time if t is None else t
         tol = 8 * np.finfo(float).eps * max(abs(t), abs(self.stop_time), abs(self.timestepper.dt), 1.0)
         return t < self.stop_time - tol
+
+    def _steps_to_next_event(self, mgr, progress: bool) -> int:
+        """Iterations the run! loop takes from the current clock before it
+        stops (stop iteration or stop time) or has to do something on the host
+        after a step (a due diagnostics record, a progress line): the clock is
+        advanced on a copy exactly as the loop advances it."""
+        c = Clock(self.clock.time, self.clock.iteration)
+        dt = self.timestepper.dt
+        n = 0
+        while True:
+            if self.stop_iteration is not None and c.iteration >= self.stop_iteration:
+                break
+            if not self._before_stop_time(c.time):
+                break
+            n += 1
+            c.iteration += 1
+            c.time += dt
+            if progress or (mgr is not None and mgr.due(c)) or n >= self._max_stretch:
+                break
+        return n
+
+    _max_stretch = 1 << 16  # iterations handed to the device at once
+
+
+def _raise_termination(nonfinite: bool, max_abs_psi: float):
+    if nonfinite:
+        raise FloatingPointError("non-finite value detected in the model state")
+    if not max_abs_psi <= 1e10:
+        raise FloatingPointError(f"solution appears to be blowing up (max |psi| = {max_abs_psi})")
 
 
 def check_termination_(simulation: Simulation):
     """``_check_termination_conditions!`` (src/simulation.jl:25-44)."""
     bad, m = simulation.model._h.check()
-    if bad:
-        raise FloatingPointError("non-finite value detected in the model state")
-    if not m <= 1e10:
-        raise FloatingPointError(f"solution appears to be blowing up (max |psi| = {m})")
+    _raise_termination(bad, m)
     return simulation
 
 
@@ -871,21 +899,42 @@ def run_(simulation: Simulation, *, stop_iteration=None, stop_time=None, progres
         mgr = simulation.diagnostics_manager
         if mgr is not None and mgr.due(simulation.clock, initial=True):
             mgr.record(model, simulation.clock)
-        while True:
-            it = simulation.clock.iteration
-            if simulation.stop_iteration is not None and it >= simulation.stop_iteration:
-                break
-            if not simulation._before_stop_time():
-                break
-            step_(model, ts)
-            if simulation.check_every and (it + 1) % simulation.check_every == 0:
-                check_termination_(simulation)
-            simulation.clock.iteration += 1
-            simulation.clock.time += ts.dt
-            if mgr is not None and mgr.due(simulation.clock):
-                mgr.record(model, simulation.clock)
-            if progress:
-                print(f"iteration={simulation.clock.iteration} | time={simulation.clock.time} s", flush=True)
+        if simulation.check_every == 1:
+            # every step checked, as run! does: stretches of steps between
+            # host events run on the device (qg_run), checks included
+            while True:
+                n = simulation._steps_to_next_event(mgr, progress)
+                if n == 0:
+                    break
+                if model.finalized:
+                    raise RuntimeError("cannot use operators after model finalization")
+                done, failure, m = model._h.run(ts.dt, n)
+                ts.workspace = "device"
+                for _ in range(done):
+                    simulation.clock.iteration += 1
+                    simulation.clock.time += ts.dt
+                if failure:
+                    _raise_termination(failure == 1, m)
+                if mgr is not None and mgr.due(simulation.clock):
+                    mgr.record(model, simulation.clock)
+                if progress:
+                    print(f"iteration={simulation.clock.iteration} | time={simulation.clock.time} s", flush=True)
+        else:
+            while True:
+                it = simulation.clock.iteration
+                if simulation.stop_iteration is not None and it >= simulation.stop_iteration:
+                    break
+                if not simulation._before_stop_time():
+                    break
+                step_(model, ts)
+                if simulation.check_every and (it + 1) % simulation.check_every == 0:
+                    check_termination_(simulation)
+                simulation.clock.iteration += 1
+                simulation.clock.time += ts.dt
+                if mgr is not None and mgr.due(simulation.clock):
+                    mgr.record(model, simulation.clock)
+                if progress:
+                    print(f"iteration={simulation.clock.iteration} | time={simulation.clock.time} s", flush=True)
         simulation.state = SimulationState.Stopped
     except Exception:
         simulation.state = SimulationState.Failed

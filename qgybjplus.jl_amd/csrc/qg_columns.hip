@@ -107,19 +107,6 @@ __global__ void k_convert(const From* __restrict__ in, To* __restrict__ out, siz
   if (i < n) out[i] = (To)in[i];
 }
 
-// Any non-finite value among n complex entries -> flag.
-template <typename T>
-__global__ void k_nonfinite(const cplx<T>* __restrict__ a, size_t n, int* __restrict__ flag) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  size_t stride = (size_t)gridDim.x * blockDim.x;
-  bool bad = false;
-  for (; i < n; i += stride) {
-    cplx<T> v = a[i];
-    bad |= !(isfinite(v.re) && isfinite(v.im));
-  }
-  if (bad) atomicOr(flag, 1);
-}
-
 __global__ void k_zero(unsigned int* __restrict__ p, size_t nwords) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -170,14 +157,5 @@ void launch_convert(const From* in, To* out, size_t n, hipStream_t st) {
 }
 template void launch_convert<float, double>(const float*, double*, size_t, hipStream_t);
 template void launch_convert<double, float>(const double*, float*, size_t, hipStream_t);
-
-void launch_nonfinite(const void* a, bool f32, size_t n, int* flag, hipStream_t st) {
-  if (f32)
-    hipLaunchKernelGGL(k_nonfinite<float>, dim3(1024), dim3(256), 0, st, static_cast<const cplx<float>*>(a), n,
-                       flag);
-  else
-    hipLaunchKernelGGL(k_nonfinite<double>, dim3(1024), dim3(256), 0, st, static_cast<const cplx<double>*>(a), n,
-                       flag);
-}
 
 }  // namespace qg

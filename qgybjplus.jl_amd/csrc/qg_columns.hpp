@@ -58,7 +58,6 @@ template <typename T>
 void launch_transpose(const T* in, T* out, int nx, int ny, int nz, bool to_planar, hipStream_t st);
 template <typename From, typename To>
 void launch_convert(const From* in, To* out, size_t n, hipStream_t st);
-void launch_nonfinite(const void* a, bool f32, size_t n, int* flag, hipStream_t st);
 void launch_zero_bytes(void* p, size_t bytes, hipStream_t st);
 void launch_zero(cplx<double>* p, size_t n, hipStream_t st);
 

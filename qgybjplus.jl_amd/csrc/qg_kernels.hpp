@@ -969,4 +969,105 @@ k_x_c2c(Geo g, const cplx<double>* __restrict__ in, cplx<double>* __restrict__ o
   }
 }
 
+// ===========================================================================
+// Termination check (_check_termination_conditions!, src/simulation.jl:25-44)
+// at the transform precision: any non-finite value of q, B or psi, and
+// max |ifft2(psi)| (complex inverse, 1/(nx*ny)).  Both fold into accumulators
+// (flag |= 1, absmax = max) that the caller clears, so a run of steps can
+// check every step without a host round trip.
+// ===========================================================================
+
+template <typename R>
+__device__ inline bool finite_c(cplx<R> a) {
+  return __builtin_isfinite(a.re) && __builtin_isfinite(a.im);
+}
+
+// Y-pass: out = inverse y transform of psi / (nx*ny); q, B and psi are tested
+// at the points it loads (every point of the levels it covers).
+template <typename R, int NY, int V>
+__global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
+k_y_check(Geo g, const cplx<R>* __restrict__ psi, const cplx<R>* __restrict__ q, const cplx<R>* __restrict__ B,
+          cplx<R>* __restrict__ out, int* __restrict__ flag, const cplx<R>* __restrict__ tw) {
+  using TL = Tile<R, NY, V>;
+  using F = Fft<R, NY, +1>;
+  __shared__ cplx<R> lds[TL::S * TL::LD];
+  const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
+  TwReg<R, TL::NN> twr;
+  twr.load(lane, tw);
+  const Blk blk = xcd_block();
+  const int x = blk.x * TL::S + s, z = blk.y;
+  const bool ok = x < g.nx;
+  cplx<R>* sl = lds + s * TL::LD;
+  cplx<R> v[F::IT][F::R0];
+  bool fin = true;
+#pragma unroll
+  for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < F::R0; ++t) {
+      int y = F::elem(lane, it, t);
+      const bool in = ok && F::valid(lane, it);
+      const size_t ad = ((size_t)z * NY + y) * g.nx + x;
+      cplx<R> p = czero<R>();
+      if (in) {
+        p = psi[ad];
+        fin = fin && finite_c(p) && finite_c(q[ad]) && finite_c(B[ad]);
+      }
+      v[it][t] = scale(p, (R)g.inv_n);
+    }
+  if (!fin) atomicOr(flag, 1);
+  F::run(v, sl, lane, twr);
+#pragma unroll
+  for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < F::R0; ++t) {
+      int y = F::elem(lane, it, t);
+      if (ok && F::valid(lane, it)) out[((size_t)z * NY + y) * g.nx + x] = v[it][t];
+    }
+}
+
+// X-pass: inverse x transform of the Y-pass output; folds max |value| into
+// absmax (non-negative doubles compare like their bit patterns; NaN counts
+// as +inf).  Nothing is stored.
+template <typename R, int NX, int V>
+__global__ void __launch_bounds__((Tile<R, NX, V>::THREADS))
+k_x_check(Geo g, const cplx<R>* __restrict__ in, unsigned long long* __restrict__ absmax,
+          const cplx<R>* __restrict__ tw) {
+  using TL = Tile<R, NX, V>;
+  using F = Fft<R, NX, +1>;
+  __shared__ cplx<R> lds[TL::S * TL::LD];
+  const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
+  TwReg<R, TL::NN> twr;
+  twr.load(lane, tw);
+  const int row = blockIdx.x * TL::S + r;
+  const bool ok = row < g.nz * g.ny;
+  cplx<R>* sl = lds + r * TL::LD;
+  cplx<R> v[F::IT][F::R0];
+#pragma unroll
+  for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < F::R0; ++t) {
+      int x = F::elem(lane, it, t);
+      v[it][t] = (ok && F::valid(lane, it)) ? in[(size_t)row * NX + x] : czero<R>();
+    }
+  F::run(v, sl, lane, twr);
+  double m = 0.0;
+#pragma unroll
+  for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < F::R0; ++t) {
+      if (ok && F::valid(lane, it)) {
+        const cplx<double> c = cvt<double>(v[it][t]);
+        const double a = hypot(c.re, c.im);
+        m = (a > m || a != a) ? a : m;
+      }
+    }
+  if (m != m) m = __builtin_huge_val();
+  // one atomic per wave: the wave's largest value
+  for (int o = 32; o > 0; o >>= 1) {
+    const double w = __shfl_xor(m, o);
+    m = w > m ? w : m;
+  }
+  if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(absmax, (unsigned long long)__double_as_longlong(m));
+}
+
 }  // namespace qg

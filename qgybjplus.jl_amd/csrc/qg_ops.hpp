@@ -45,6 +45,10 @@ struct OpsV {
   static void y_c2c(const Geo& g, hipStream_t st, int dir, const D* in, D* out, const C* twy);
   static void x_c2c(const Geo& g, hipStream_t st, int dir, const D* in, D* out, unsigned long long* absmax,
                     const C* twx);
+  // termination check: non-finite q, B, psi -> flag; max |ifft2(psi)| -> absmax
+  // (tmp: one full-width field of scratch)
+  static void check(const Geo& g, hipStream_t st, const C* psi, const C* q, const C* B, C* tmp, int* flag,
+                    unsigned long long* absmax, const C* twx, const C* twy);
 };
 
 // Run-time choice of the tile variant: Ops<R>::name(v, args...) calls
@@ -75,6 +79,7 @@ struct Ops {
   QG_OPS_FWD(y_herm)
   QG_OPS_FWD(y_c2c)
   QG_OPS_FWD(x_c2c)
+  QG_OPS_FWD(check)
 #undef QG_OPS_FWD
 };
 

@@ -169,4 +169,15 @@ void OpsV<R, V>::x_c2c(const Geo& g, hipStream_t st, int dir, const D* in, D* ou
   }
 }
 
+template <typename R, int V>
+void OpsV<R, V>::check(const Geo& g, hipStream_t st, const C* psi, const C* q, const C* B, C* tmp, int* flag,
+                       unsigned long long* absmax, const C* twx, const C* twy) {
+#define CALL(N) YL((k_y_check<R, N, V>), N, g.nx, g, psi, q, B, tmp, flag, twy)
+  QG_SWITCH(g.ny, CALL)
+#undef CALL
+#define CALL(N) XL((k_x_check<R, N, V>), N, g, tmp, absmax, twx)
+  QG_SWITCH(g.nx, CALL)
+#undef CALL
+}
+
 }  // namespace qg

@@ -109,6 +109,11 @@ struct qg_handle {
   // double buffer (after two swaps the buffers are back in place)
   bool graphs = true;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
+  // the same two steps, each followed by the termination check (qg_run)
+  hipGraphExec_t graph_chk[2] = {nullptr, nullptr};
+  // the state a step starts from (q, B and the diagnostics psi, A, T1, u, v
+  // it reuses) at the start of a qg_run chunk; allocated by the first qg_run
+  void* snap = nullptr;
   std::string err;
   // profiling: bit 1 = events around every kernel, bit 2 = one event per step
   struct ProfRec {
@@ -155,11 +160,12 @@ struct qg_handle {
     return p;
   }
   void drop_graphs() {
-    for (hipGraphExec_t& g : graph)
-      if (g) {
-        (void)hipGraphExecDestroy(g);
-        g = nullptr;
-      }
+    for (hipGraphExec_t* set : {graph, graph_chk})
+      for (int i = 0; i < 2; ++i)
+        if (set[i]) {
+          (void)hipGraphExecDestroy(set[i]);
+          set[i] = nullptr;
+        }
   }
   ~qg_handle() {
     if (st) (void)hipStreamSynchronize(st);
@@ -321,6 +327,22 @@ struct Engine {
     } else {
       fn();
     }
+  }
+
+  // _check_termination_conditions! (src/simulation.jl:25-44) on the levels of
+  // members [m0, m0 + nm): non-finite q, B, psi -> d_flag, max |ifft2(psi)| ->
+  // d_absmax, both accumulated (the caller clears them).  G3 is free between
+  // steps and holds the y-transformed psi.
+  void check(int m0, int nm) {
+    qg::Geo g = h.g;
+    g.nz = nm * h.nzc;
+    g.members = nm;
+    const size_t off = (size_t)m0 * h.nzc * h.plane;
+    const double pts = (double)h.plane * g.nz;
+    K("check", -1, pts * (3 * cS + 2 * cR), [&] {
+      O::check(1, g, h.st, static_cast<const C*>(h.at(h.psi, off)), static_cast<const C*>(h.at(h.q[h.cur], off)),
+               static_cast<const C*>(h.at(h.B[h.cur], off)), c(h.G3), h.d_flag, h.d_absmax, twx(), twy());
+    });
   }
 
   void t1_only(const C* Bsrc) {
@@ -541,6 +563,55 @@ void fft2_inplace(qg_handle& h, D* f, int dir, unsigned long long* absmax = null
 }
 
 }  // namespace
+
+// `nsteps` steps of the engine: steady-state pairs (diagnostics valid, no
+// per-kernel events) replay a captured two-step graph, the others run
+// eagerly.  With `checked`, every step is followed by the termination check
+// of the active members, inside the graph too.  mark(n) follows every n steps.
+template <class E, class Mark>
+void advance(qg_handle& H, E& e, int nsteps, bool checked, Mark&& mark) {
+  const bool use_graphs = H.graphs && !(H.prof_mode & 1);
+  auto one = [&] {
+    e.step();
+    if (checked) e.check(0, H.active);
+  };
+  int i = 0;
+  while (i < nsteps) {
+    if (use_graphs && H.diag_valid && nsteps - i >= 2) {
+      const int c = H.cur;
+      hipGraphExec_t& gx = checked ? H.graph_chk[c] : H.graph[c];
+      if (!gx) {
+        hipGraph_t g = nullptr;
+        HIPCHECK(hipStreamBeginCapture(H.st, hipStreamCaptureModeThreadLocal));
+        try {
+          one();
+          one();
+        } catch (...) {
+          (void)hipStreamEndCapture(H.st, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        HIPCHECK(hipStreamEndCapture(H.st, &g));
+        const hipError_t ie = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) {
+          gx = nullptr;
+          throw QgError(QG_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+        }
+      }
+      HIPCHECK(hipGraphLaunch(gx, H.st));
+      i += 2;
+      mark(2);
+    } else {
+      one();
+      ++i;
+      mark(1);
+    }
+  }
+}
+
+// Steps per qg_run chunk: one host round trip per chunk.
+constexpr int kRunChunk = 32;
 
 // ---------------------------------------------------------------------------
 // C ABI
@@ -882,22 +953,22 @@ void get_field_range(qg_handle& H, int field, double* host, int m0, int nm) {
   HIPCHECK(hipStreamSynchronize(H.st));
 }
 
-void check_range(qg_handle& H, int m0, int nm, int* nonfinite, double* max_abs_psi) {
-  const int lv = nm * H.nzc;
-  const size_t off = (size_t)m0 * H.nzc * H.plane, n = (size_t)lv * H.plane;
-  HIPCHECK(hipMemsetAsync(H.d_flag, 0, sizeof(int), H.st));
-  HIPCHECK(hipMemsetAsync(H.d_absmax, 0, sizeof(unsigned long long), H.st));
-  qg::launch_nonfinite(H.at(H.q[H.cur], off), H.f32(), n, H.d_flag, H.st);
-  qg::launch_nonfinite(H.at(H.B[H.cur], off), H.f32(), n, H.d_flag, H.st);
-  qg::launch_nonfinite(H.at(H.psi, off), H.f32(), n, H.d_flag, H.st);
-  fft2_inplace(H, as_fp64(H, H.psi, off, n), +1, H.d_absmax, H.stg, lv);
-  int flag = 0;
+// Reads (and waits for) the check accumulators.
+void read_check(qg_handle& H, int& flag, double& m) {
   unsigned long long bits = 0;
   HIPCHECK(hipMemcpyAsync(&flag, H.d_flag, sizeof(int), hipMemcpyDeviceToHost, H.st));
   HIPCHECK(hipMemcpyAsync(&bits, H.d_absmax, sizeof(bits), hipMemcpyDeviceToHost, H.st));
   HIPCHECK(hipStreamSynchronize(H.st));
-  double m;
   std::memcpy(&m, &bits, sizeof m);
+}
+
+void check_range(qg_handle& H, int m0, int nm, int* nonfinite, double* max_abs_psi) {
+  HIPCHECK(hipMemsetAsync(H.d_flag, 0, sizeof(int), H.st));
+  HIPCHECK(hipMemsetAsync(H.d_absmax, 0, sizeof(unsigned long long), H.st));
+  with_engine(H, [&](auto& e) { e.check(m0, nm); });
+  int flag = 0;
+  double m = 0.0;
+  read_check(H, flag, m);
   if (nonfinite) *nonfinite = flag;
   if (max_abs_psi) *max_abs_psi = m;
 }
@@ -970,45 +1041,96 @@ int qg_step(qg_handle* h, double dt, int nsteps) {
       H.step_weights.push_back(weight);
     };
     with_engine(H, [&](auto& e) {
-      // Steady-state steps (diagnostics valid, no per-kernel events) replay a
-      // captured two-step graph; the others run eagerly.
-      const bool use_graphs = H.graphs && !(H.prof_mode & 1);
       if (H.prof_mode & 2) mark(0);
-      int i = 0;
-      while (i < nsteps) {
-        if (use_graphs && H.diag_valid && nsteps - i >= 2) {
-          const int c = H.cur;
-          if (!H.graph[c]) {
-            hipGraph_t g = nullptr;
-            HIPCHECK(hipStreamBeginCapture(H.st, hipStreamCaptureModeThreadLocal));
-            try {
-              e.step();
-              e.step();
-            } catch (...) {
-              (void)hipStreamEndCapture(H.st, &g);
-              if (g) (void)hipGraphDestroy(g);
-              throw;
-            }
-            HIPCHECK(hipStreamEndCapture(H.st, &g));
-            const hipError_t ie = hipGraphInstantiate(&H.graph[c], g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (ie != hipSuccess) {
-              H.graph[c] = nullptr;
-              throw QgError(QG_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
-            }
-          }
-          HIPCHECK(hipGraphLaunch(H.graph[c], H.st));
-          i += 2;
-          mark(2);
-        } else {
-          e.step();
-          ++i;
-          mark(1);
-        }
-      }
+      advance(H, e, nsteps, false, mark);
     });
     HIPCHECK(hipGetLastError());
     check_status(H);
+  });
+}
+
+int qg_run(qg_handle* h, double dt, int nsteps, int* done, int* failure, double* max_abs_psi) {
+  return guarded(h, [&] {
+    if (!(std::isfinite(dt) && dt > 0)) throw QgError(QG_ERR_ARGUMENT, "Δt must be finite and positive");
+    if (nsteps < 0) throw QgError(QG_ERR_ARGUMENT, "nsteps must be non-negative");
+    qg_handle& H = *h;
+    int d = 0, fail = 0;
+    double mx = 0.0;
+    if (H.active > 0 && nsteps > 0) {
+      H.psi64_nm = 0;
+      build_etd(H, dt);
+      // everything the next step reads: q, B (current buffers), and the
+      // diagnostics a steady-state step reuses (psi, A, T1, u, v); R scalars
+      // per point of each
+      const size_t pts = (size_t)H.active * H.nzc * H.plane, r = H.rsize();
+      if (!H.snap) H.snap = H.ralloc(12 * H.npts);
+      struct Part {
+        void* field;
+        size_t per_point;  // R scalars
+      };
+      auto parts = [&](int c) {
+        return std::array<Part, 7>{Part{H.q[c], 2}, Part{H.B[c], 2}, Part{H.psi, 2}, Part{H.A, 2},
+                                   Part{H.T1, 2}, Part{H.u, 1}, Part{H.v, 1}};
+      };
+      auto copy_parts = [&](int c, bool save) {
+        char* at = static_cast<char*>(H.snap);
+        for (const Part& p : parts(c)) {
+          const size_t bytes = pts * p.per_point * r;
+          HIPCHECK(hipMemcpyAsync(save ? at : p.field, save ? p.field : at, bytes, hipMemcpyDeviceToDevice, H.st));
+          at += bytes;
+        }
+      };
+      auto clear = [&] {
+        HIPCHECK(hipMemsetAsync(H.d_flag, 0, sizeof(int), H.st));
+        HIPCHECK(hipMemsetAsync(H.d_absmax, 0, sizeof(unsigned long long), H.st));
+      };
+      auto none = [](int) {};
+      with_engine(H, [&](auto& e) {
+        while (d < nsteps && fail == 0) {
+          const int k = std::min(kRunChunk, nsteps - d);
+          const int c0 = H.cur;
+          const bool dv = H.diag_valid, tv = H.t1_valid;
+          copy_parts(c0, true);
+          clear();
+          advance(H, e, k, true, none);
+          int flag = 0;
+          double m = 0.0;
+          read_check(H, flag, m);
+          check_status(H);
+          if (flag == 0 && m <= 1e10) {
+            d += k;
+            mx = std::max(mx, m);
+            continue;
+          }
+          // A step of this chunk failed: restore the chunk's starting state
+          // and step again, one step and one check at a time, stopping right
+          // after the failing step.  The same kernels on the same state give
+          // the same values, so the state at the stop is the one the failing
+          // step produced.
+          copy_parts(c0, false);
+          H.cur = c0;
+          H.diag_valid = dv;
+          H.t1_valid = tv;
+          for (int j = 0; j < k && fail == 0; ++j) {
+            clear();
+            advance(H, e, 1, true, none);
+            read_check(H, flag, m);
+            check_status(H);
+            ++d;
+            if (flag != 0 || !(m <= 1e10)) {
+              fail = flag != 0 ? 1 : 2;
+              mx = m;
+            } else {
+              mx = std::max(mx, m);
+            }
+          }
+        }
+      });
+      HIPCHECK(hipGetLastError());
+    }
+    if (done) *done = d;
+    if (failure) *failure = fail;
+    if (max_abs_psi) *max_abs_psi = mx;
   });
 }
 
@@ -1423,7 +1545,7 @@ void autotune(qg_handle& H, double dt, int steps) {
     for (auto& r : H.prof) {
       float t = 0;
       HIPCHECK(hipEventElapsedTime(&t, r.a, r.b));
-      ms[r.pass].push_back(t);
+      if (r.pass >= 0) ms[r.pass].push_back(t);
     }
     H.prof.clear();
     H.pool_used = 0;
