@@ -128,6 +128,23 @@ def kernel_run(qg, model, steps):
     return h.profile_end()
 
 
+def run_loop_leg(qg, model, steps):
+    """run!(simulation) for `steps` iterations of `model`: the termination
+    check (non-finite q, B, psi; max |ifft psi| <= 1e10) after every step, as
+    the reference's run! makes it, here on the device."""
+    log(f"run! loop over {steps} iterations")
+    qg.run_(qg.Simulation(model, dt=DT, stop_iteration=4))  # captures the checked step graphs
+    sim = qg.Simulation(model, dt=DT, stop_iteration=steps)
+    model._h.synchronize()
+    t0 = time.perf_counter()
+    qg.run_(sim)
+    model._h.synchronize()
+    t = time.perf_counter() - t0
+    npts = SIZE[0] * SIZE[1] * SIZE[2]
+    return {"iterations": steps, "value": npts * steps / t / 1e6, "unit": "Mgridpt-steps/s",
+            "ms_per_step": 1e3 * t / steps, "check": "every step, on the device"}
+
+
 ENSEMBLE_CLASSES = ((64, 32), (128, 64), (256, 64))  # (nx = ny, nz), 16 members each
 ENSEMBLE_MEMBERS = 16
 
@@ -284,6 +301,8 @@ def main():
                     help="steps of the 512x512x256 leg (0 skips it)")
     ap.add_argument("--serve-requests", type=int, default=32,
                     help="requests of the serving leg (0 skips it)")
+    ap.add_argument("--run-steps", type=int, default=200,
+                    help="iterations of run!(simulation) (every step checked) on the mixed model")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="run every pass at its default tile variant / launch shape")
     ap.add_argument("--modes", choices=("both", "mixed"), default="both",
@@ -395,6 +414,7 @@ def main():
         "kernels_mixed": per_kernel,
         "autotune": tuning or None,
     }
+    out["run_loop"] = run_loop_leg(qg, m32, args.run_steps) if args.run_steps > 0 else None
     m32.finalize()
     if m64 is not None:
         m64.finalize()
