@@ -34,13 +34,13 @@ DT = 20.0
 
 
 # Kernel behind each of the library's per-launch names: (kernel, tuned pass,
-# launch kind).  Several names share a kernel (the two velocity passes); the
+# launch kind[, trailing template arguments]).  Several names share a kernel (the two velocity passes); the
 # stage and final launches of one kernel are told apart by launch order in the
 # rocprofv3 summary.
 LAUNCH_KERNEL = {
     "ya_T1": ("k_ya_qw", "ya_qw", None), "ya_qw": ("k_ya_qw", "ya_qw", None),
     "xa_qw": ("k_xa_qw", "xa_qw", None), "ya2_qeff": ("k_ya2_qeff", "ya2_qeff", None),
-    "yb_uv": ("k_yb_adv", "yb_adv", None), "yb_adv": ("k_yb_adv", "yb_adv", None),
+    "yb_uv": ("k_yb_adv", "yb_adv", None, 3), "yb_adv": ("k_yb_adv", "yb_adv", None, 12),
     "x_c2r_uv": ("k_x_c2r", "x_c2r", None), "x_c2r_w": ("k_x_c2r", "x_c2r", None),
     "xb_q": ("k_xb_q", "xb_q", None), "xb_B": ("k_xb_B", "xb_B", None),
     "yb2_q_stage": ("k_yb2_q", "yb2_q", "stage"), "yb2_q_final": ("k_yb2_q", "yb2_q", "final"),
@@ -62,12 +62,13 @@ def kernel_symbol(name, choice):
     """The demangled symbol rocprofv3 reports for launch `name` of the FP32
     mixed-precision model at 256x256 under tuning `choice` (pass -> variant
     or shape index, -1 = default), with its launch kind."""
-    base, tpass, kind = LAUNCH_KERNEL[name]
+    base, tpass, kind, *extra = LAUNCH_KERNEL[name]
     v = int((choice or {}).get(tpass, -1))
     if base.startswith("k_colscan"):
         m, qc = SCAN_SHAPES[v if v >= 0 else SCAN_DEFAULT]
         return f"{base}, {m}, {qc}, float>", kind
-    return f"{base}<float, {SIZE[0]}, {v if v >= 0 else 1}>", kind
+    tail = "".join(f", {e}" for e in extra)  # compile-time output set of k_yb_adv
+    return f"{base}<float, {SIZE[0]}, {v if v >= 0 else 1}{tail}>", kind
 
 
 def measured_traffic(name, choice):

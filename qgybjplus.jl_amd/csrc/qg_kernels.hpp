@@ -226,13 +226,17 @@ k_ya2_qeff(Geo g, const cplx<R>* __restrict__ Qa, const cplx<R>* __restrict__ Qb
 // Y-pass (inverse, half width) producing the y-transformed Hermitian parts of
 // u = Re ifft(-i ky psi), v = Re ifft(i kx psi) (unmasked psi), of
 // Re ifft(mask q) and of zeta = Re ifft(mask (-kh2 psi)).
-template <typename R, int NY, int V>
+// OUTS: the output set fixed at compile time (0: `outs` at run time).  Each
+// fixed set keeps only its own inputs live, which roughly halves the
+// registers of the two-output launches.
+template <typename R, int NY, int V, int OUTS>
 __global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
-k_yb_adv(Geo g, int outs, const cplx<R>* __restrict__ psi, const cplx<R>* __restrict__ q,
+k_yb_adv(Geo g, int outs_rt, const cplx<R>* __restrict__ psi, const cplx<R>* __restrict__ q,
          cplx<R>* __restrict__ Pu, cplx<R>* __restrict__ Pv, cplx<R>* __restrict__ Pq,
          cplx<R>* __restrict__ Pz, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY, V>;
   using F = Fft<R, NY, +1>;
+  const int outs = OUTS ? OUTS : outs_rt;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
   TwReg<R, TL::NN> twr;
@@ -315,6 +319,39 @@ __device__ inline void load_herm_row(cplx<R> (&v)[F::IT][F::R0], const cplx<R>* 
     }
 }
 
+// Two half-width rows of real fields' spectra packed as one complex row,
+// Z = A + i B over the full row (x >= W from the conjugate mirrors; the
+// self-conjugate points x = 0 and nx/2 take their real parts, which is what
+// the real part of a single inverse transform keeps).  One inverse transform
+// then gives a in .re and b in .im.  The pairs packed this way are fields of
+// one kind and scale (u and v; the two components of a gradient).
+template <typename R, class F>
+__device__ inline void load_herm_pair(cplx<R> (&v)[F::IT][F::R0], const cplx<R>* __restrict__ A,
+                                      const cplx<R>* __restrict__ B, size_t hb, int nx, int lane, bool ok) {
+  const int W = nx / 2 + 1;
+#pragma unroll
+  for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < F::R0; ++t) {
+      int x = F::elem(lane, it, t);
+      cplx<R> a = czero<R>(), b = czero<R>();
+      if (ok && F::valid(lane, it)) {
+        if (x < W) {
+          a = A[hb + x];
+          b = B[hb + x];
+          if (x == 0 || 2 * x == nx) {
+            a.im = (R)0;
+            b.im = (R)0;
+          }
+        } else {
+          a = conj(A[hb + (nx - x)]);
+          b = conj(B[hb + (nx - x)]);
+        }
+      }
+      v[it][t] = mk<R>(a.re - b.im, a.im + b.re);
+    }
+}
+
 // X-pass: real rows from half-width intermediates (C2R), written to `outs`.
 // Used for u, v (velocities) and w.
 template <typename R, int NX, int V>
@@ -331,19 +368,22 @@ k_x_c2r(Geo g, int nfields, const cplx<R>* __restrict__ P0, const cplx<R>* __res
   const bool ok = row < g.nz * g.ny;
   cplx<R>* sl = lds + r * TL::LD;
   cplx<R> v[FI::IT][FI::R0];
-  for (int f = 0; f < nfields; ++f) {
-    const cplx<R>* src = f == 0 ? P0 : P1;
-    R* dst = f == 0 ? o0 : o1;
-    load_herm_row<R, FI>(v, src, (size_t)row * g.Wp, NX, lane, ok);
-    FI::run(v, sl, lane, twr);
-#pragma unroll
-    for (int it = 0; it < FI::IT; ++it)
-#pragma unroll
-      for (int t = 0; t < FI::R0; ++t) {
-        int x = FI::elem(lane, it, t);
-        if (ok && FI::valid(lane, it)) dst[(size_t)row * NX + x] = v[it][t].re;
-      }
+  if (nfields == 2) {  // u and v: one packed transform
+    load_herm_pair<R, FI>(v, P0, P1, (size_t)row * g.Wp, NX, lane, ok);
+  } else {
+    load_herm_row<R, FI>(v, P0, (size_t)row * g.Wp, NX, lane, ok);
   }
+  FI::run(v, sl, lane, twr);
+#pragma unroll
+  for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < FI::R0; ++t) {
+      int x = FI::elem(lane, it, t);
+      if (ok && FI::valid(lane, it)) {
+        o0[(size_t)row * NX + x] = v[it][t].re;
+        if (nfields == 2) o1[(size_t)row * NX + x] = v[it][t].im;
+      }
+    }
 }
 
 // X-pass for q advection: q = C2R(Pq); u*q, v*q -> forward real FFTs -> G1, G2.
@@ -370,26 +410,38 @@ k_xb_q(Geo g, const cplx<R>* __restrict__ Pq, const R* __restrict__ u, const R* 
 #pragma unroll
     for (int t = 0; t < FI::R0; ++t) qv[it][t] = v[it][t].re;
   const size_t hb = (size_t)row * g.Wp;
-  for (int f = 0; f < 2; ++f) {
-    const R* vel = f == 0 ? u : vv;
-    cplx<R>* dst = f == 0 ? G1 : G2;
+  // u*q and v*q (one kind and scale) packed as Z = uq + i vq: one forward
+  // transform, then F(uq)[k] = (Z[k] + conj Z[-k])/2 and
+  // F(vq)[k] = (Z[k] - conj Z[-k])/(2i), with Z[-k] read back through LDS.
 #pragma unroll
-    for (int it = 0; it < FI::IT; ++it)
+  for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
-      for (int t = 0; t < FI::R0; ++t) {
-        int x = FI::elem(lane, it, t);
-        R c = (ok && FI::valid(lane, it)) ? vel[(size_t)row * NX + x] : (R)0;
-        v[it][t] = mk<R>(c * qv[it][t], (R)0);
+    for (int t = 0; t < FI::R0; ++t) {
+      int x = FI::elem(lane, it, t);
+      const bool in = ok && FI::valid(lane, it);
+      const R cu = in ? u[(size_t)row * NX + x] : (R)0;
+      const R cv = in ? vv[(size_t)row * NX + x] : (R)0;
+      v[it][t] = mk<R>(cu * qv[it][t], cv * qv[it][t]);
+    }
+  FF::run(v, sl, lane, twr);
+  __syncthreads();  // the last stage's LDS reads are done
+#pragma unroll
+  for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < FI::R0; ++t)
+      if (FI::valid(lane, it)) sl[FI::pad(FI::elem(lane, it, t))] = v[it][t];
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < FI::R0; ++t) {
+      int x = FI::elem(lane, it, t);
+      if (ok && FI::valid(lane, it) && x < g.W) {
+        const cplx<R> a = v[it][t], m = conj(sl[FI::pad(x == 0 ? 0 : NX - x)]);
+        G1[hb + x] = mk<R>((R)0.5 * (a.re + m.re), (R)0.5 * (a.im + m.im));
+        G2[hb + x] = mk<R>((R)0.5 * (a.im - m.im), (R)-0.5 * (a.re - m.re));
       }
-    FF::run(v, sl, lane, twr);
-#pragma unroll
-    for (int it = 0; it < FI::IT; ++it)
-#pragma unroll
-      for (int t = 0; t < FI::R0; ++t) {
-        int x = FI::elem(lane, it, t);
-        if (ok && FI::valid(lane, it) && x < g.W) dst[hb + x] = v[it][t];
-      }
-  }
+    }
 }
 
 // X-pass for B: B = IFFTx(T1), zeta = C2R(Pz); u*B, v*B, zeta*B -> forward
@@ -743,6 +795,8 @@ k_yf_omega(Geo g, const cplx<R>* __restrict__ psi, cplx<R>* __restrict__ Pbx, cp
 }
 
 // X-pass: u, v out; omega RHS 2(bx*xy - by*xx) -> forward real FFT -> Rw (half).
+// The inverse transforms go in pairs of real fields (u, v), (bx, by), (xx, xy),
+// each pair packed into one complex transform.
 template <typename R, int NX, int V>
 __global__ void __launch_bounds__((Tile<R, NX, V>::THREADS))
 k_xf(Geo g, int want_omega, const cplx<R>* __restrict__ Pu, const cplx<R>* __restrict__ Pv,
@@ -761,44 +815,38 @@ k_xf(Geo g, int want_omega, const cplx<R>* __restrict__ Pu, const cplx<R>* __res
   cplx<R>* sl = lds + r * TL::LD;
   const size_t hb = (size_t)row * g.Wp;
   cplx<R> v[FI::IT][FI::R0];
-  for (int f = 0; f < 2; ++f) {
-    load_herm_row<R, FI>(v, f == 0 ? Pu : Pv, hb, NX, lane, ok);
-    FI::run(v, sl, lane, twr);
-    R* dst = f == 0 ? u : vv;
+  // u and v: one packed transform
+  load_herm_pair<R, FI>(v, Pu, Pv, hb, NX, lane, ok);
+  FI::run(v, sl, lane, twr);
 #pragma unroll
-    for (int it = 0; it < FI::IT; ++it)
+  for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
-      for (int t = 0; t < FI::R0; ++t) {
-        int x = FI::elem(lane, it, t);
-        if (ok && FI::valid(lane, it)) dst[(size_t)row * NX + x] = v[it][t].re;
+    for (int t = 0; t < FI::R0; ++t) {
+      int x = FI::elem(lane, it, t);
+      if (ok && FI::valid(lane, it)) {
+        u[(size_t)row * NX + x] = v[it][t].re;
+        vv[(size_t)row * NX + x] = v[it][t].im;
       }
-  }
+    }
   if (!want_omega) return;  // uniform
-  R p1[FI::IT][FI::R0], tmp[FI::IT][FI::R0];
-  load_herm_row<R, FI>(v, Pbx, hb, NX, lane, ok);
+  // (bx, by) and (xx, xy): one packed transform each
+  R bx[FI::IT][FI::R0], by[FI::IT][FI::R0];
+  load_herm_pair<R, FI>(v, Pbx, Pby, hb, NX, lane, ok);
   FI::run(v, sl, lane, twr);
 #pragma unroll
   for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
-    for (int t = 0; t < FI::R0; ++t) tmp[it][t] = v[it][t].re;
-  load_herm_row<R, FI>(v, Pxy, hb, NX, lane, ok);
+    for (int t = 0; t < FI::R0; ++t) {
+      bx[it][t] = v[it][t].re;
+      by[it][t] = v[it][t].im;
+    }
+  load_herm_pair<R, FI>(v, Pxx, Pxy, hb, NX, lane, ok);
   FI::run(v, sl, lane, twr);
 #pragma unroll
   for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
-    for (int t = 0; t < FI::R0; ++t) p1[it][t] = tmp[it][t] * v[it][t].re;
-  load_herm_row<R, FI>(v, Pby, hb, NX, lane, ok);
-  FI::run(v, sl, lane, twr);
-#pragma unroll
-  for (int it = 0; it < FI::IT; ++it)
-#pragma unroll
-    for (int t = 0; t < FI::R0; ++t) tmp[it][t] = v[it][t].re;
-  load_herm_row<R, FI>(v, Pxx, hb, NX, lane, ok);
-  FI::run(v, sl, lane, twr);
-#pragma unroll
-  for (int it = 0; it < FI::IT; ++it)
-#pragma unroll
-    for (int t = 0; t < FI::R0; ++t) v[it][t] = mk<R>((R)2 * (p1[it][t] - tmp[it][t] * v[it][t].re), (R)0);
+    for (int t = 0; t < FI::R0; ++t)
+      v[it][t] = mk<R>((R)2 * (bx[it][t] * v[it][t].im - by[it][t] * v[it][t].re), (R)0);
   FF::run(v, sl, lane, twr);
 #pragma unroll
   for (int it = 0; it < FI::IT; ++it)
@@ -926,12 +974,11 @@ k_y_c2c(Geo g, const cplx<double>* __restrict__ in, cplx<double>* __restrict__ o
     }
 }
 
-// X-pass C2C of full double fields.  If `absmax` is set, also folds max|out|
-// into it (non-negative doubles compare like their bit patterns).
+// X-pass C2C of full double fields.
 template <typename R, int NX, int DIR, int V>
 __global__ void __launch_bounds__((Tile<R, NX, V>::THREADS))
 k_x_c2c(Geo g, const cplx<double>* __restrict__ in, cplx<double>* __restrict__ out,
-        unsigned long long* __restrict__ absmax, const cplx<R>* __restrict__ tw) {
+        const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NX, V>;
   using F = Fft<R, NX, DIR>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
@@ -950,23 +997,13 @@ k_x_c2c(Geo g, const cplx<double>* __restrict__ in, cplx<double>* __restrict__ o
       v[it][t] = (ok && F::valid(lane, it)) ? cvt<R>(in[(size_t)row * NX + x]) : czero<R>();
     }
   F::run(v, sl, lane, twr);
-  double m = 0.0;
 #pragma unroll
   for (int it = 0; it < F::IT; ++it)
 #pragma unroll
     for (int t = 0; t < F::R0; ++t) {
       int x = F::elem(lane, it, t);
-      if (ok && F::valid(lane, it)) {
-        cplx<double> c = cvt<double>(v[it][t]);
-        if (out) out[(size_t)row * NX + x] = c;
-        double a = hypot(c.re, c.im);
-        m = (a > m || a != a) ? a : m;
-      }
+      if (ok && F::valid(lane, it)) out[(size_t)row * NX + x] = cvt<double>(v[it][t]);
     }
-  if (absmax) {
-    if (m != m) m = __builtin_huge_val();
-    atomicMax(absmax, (unsigned long long)__double_as_longlong(m));
-  }
 }
 
 // ===========================================================================
@@ -1062,12 +1099,24 @@ k_x_check(Geo g, const cplx<R>* __restrict__ in, unsigned long long* __restrict_
       }
     }
   if (m != m) m = __builtin_huge_val();
-  // one atomic per wave: the wave's largest value
+  // the block's largest value (waves by shuffles, then LDS), folded in with
+  // at most one atomic per block, skipped when the accumulator already holds
+  // more: thousands of blocks contend for one address otherwise
   for (int o = 32; o > 0; o >>= 1) {
     const double w = __shfl_xor(m, o);
     m = w > m ? w : m;
   }
-  if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(absmax, (unsigned long long)__double_as_longlong(m));
+  constexpr int NW = (TL::THREADS + 63) / 64;
+  __shared__ double wmax[NW];
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = 0.0;
+    for (int i = 0; i < NW; ++i) b = wmax[i] > b ? wmax[i] : b;
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(b);
+    if (b > 0.0 && bits > __hip_atomic_load(absmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMax(absmax, bits);
+  }
 }
 
 }  // namespace qg

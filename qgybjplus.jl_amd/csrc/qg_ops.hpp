@@ -43,8 +43,7 @@ struct OpsV {
   static void y_herm(const Geo& g, hipStream_t st, const C* in, C* P, const C* twy);
   // plain complex 2-D transforms of double fields (dir = -1 forward, +1 inverse)
   static void y_c2c(const Geo& g, hipStream_t st, int dir, const D* in, D* out, const C* twy);
-  static void x_c2c(const Geo& g, hipStream_t st, int dir, const D* in, D* out, unsigned long long* absmax,
-                    const C* twx);
+  static void x_c2c(const Geo& g, hipStream_t st, int dir, const D* in, D* out, const C* twx);
   // termination check: non-finite q, B, psi -> flag; max |ifft2(psi)| -> absmax
   // (tmp: one full-width field of scratch)
   static void check(const Geo& g, hipStream_t st, const C* psi, const C* q, const C* B, C* tmp, int* flag,

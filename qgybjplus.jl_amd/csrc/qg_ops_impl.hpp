@@ -65,9 +65,19 @@ void OpsV<R, V>::ya2_qeff(const Geo& g, hipStream_t st, const C* Qa, const C* Qb
 template <typename R, int V>
 void OpsV<R, V>::yb_adv(const Geo& g, hipStream_t st, int outs, const C* psi, const C* q, C* Pu, C* Pv, C* Pq,
                     C* Pz, const C* twy) {
-#define CALL(N) YL((k_yb_adv<R, N, V>), N, g.W, g, outs, psi, q, Pu, Pv, Pq, Pz, twy)
-  QG_SWITCH(g.ny, CALL)
+  if (outs == (OUT_U | OUT_V)) {
+#define CALL(N) YL((k_yb_adv<R, N, V, OUT_U | OUT_V>), N, g.W, g, outs, psi, q, Pu, Pv, Pq, Pz, twy)
+    QG_SWITCH(g.ny, CALL)
 #undef CALL
+  } else if (outs == (OUT_Q | OUT_Z)) {
+#define CALL(N) YL((k_yb_adv<R, N, V, OUT_Q | OUT_Z>), N, g.W, g, outs, psi, q, Pu, Pv, Pq, Pz, twy)
+    QG_SWITCH(g.ny, CALL)
+#undef CALL
+  } else {
+#define CALL(N) YL((k_yb_adv<R, N, V, 0>), N, g.W, g, outs, psi, q, Pu, Pv, Pq, Pz, twy)
+    QG_SWITCH(g.ny, CALL)
+#undef CALL
+  }
 }
 
 template <typename R, int V>
@@ -156,14 +166,13 @@ void OpsV<R, V>::y_c2c(const Geo& g, hipStream_t st, int dir, const D* in, D* ou
 }
 
 template <typename R, int V>
-void OpsV<R, V>::x_c2c(const Geo& g, hipStream_t st, int dir, const D* in, D* out, unsigned long long* absmax,
-                   const C* twx) {
+void OpsV<R, V>::x_c2c(const Geo& g, hipStream_t st, int dir, const D* in, D* out, const C* twx) {
   if (dir < 0) {
-#define CALL(N) XL((k_x_c2c<R, N, -1, V>), N, g, in, out, absmax, twx)
+#define CALL(N) XL((k_x_c2c<R, N, -1, V>), N, g, in, out, twx)
     QG_SWITCH(g.nx, CALL)
 #undef CALL
   } else {
-#define CALL(N) XL((k_x_c2c<R, N, +1, V>), N, g, in, out, absmax, twx)
+#define CALL(N) XL((k_x_c2c<R, N, +1, V>), N, g, in, out, twx)
     QG_SWITCH(g.nx, CALL)
 #undef CALL
   }

@@ -547,18 +547,17 @@ void* real_field(qg_handle& h, int field) {
 // planar double field -> device planar field in place: forward FFT.
 // 2-D transforms of `levels` planes starting at f (all stacked levels by
 // default).
-void fft2_inplace(qg_handle& h, D* f, int dir, unsigned long long* absmax = nullptr, D* out = nullptr,
-                  int levels = -1) {
+void fft2_inplace(qg_handle& h, D* f, int dir, D* out = nullptr, int levels = -1) {
   qg::Geo g = h.g;
   if (levels >= 0) g.nz = levels;
   // y first for the inverse (spectral -> physical), x first for the forward
   if (dir < 0) {
-    qg::Ops<double>::x_c2c(1, g, h.st, -1, f, f, nullptr, h.tw64x);
+    qg::Ops<double>::x_c2c(1, g, h.st, -1, f, f, h.tw64x);
     qg::Ops<double>::y_c2c(1, g, h.st, -1, f, out ? out : f, h.tw64y);
   } else {
     D* o = out ? out : f;
     qg::Ops<double>::y_c2c(1, g, h.st, +1, f, o, h.tw64y);
-    qg::Ops<double>::x_c2c(1, g, h.st, +1, o, absmax ? nullptr : o, absmax, h.tw64x);
+    qg::Ops<double>::x_c2c(1, g, h.st, +1, o, o, h.tw64x);
   }
 }
 
@@ -901,7 +900,7 @@ void set_field_range(qg_handle& H, int field, const double* host, int space, int
     if (H.f32()) H.psi64_nm = 0;
     HIPCHECK(hipMemcpyAsync(H.stg, host, n * sizeof(D), hipMemcpyHostToDevice, H.st));
     qg::launch_transpose<D>(H.stg, pl, H.nx, H.ny, lv, true, H.st);
-    if (space == QG_SPACE_PHYSICAL) fft2_inplace(H, pl, -1, nullptr, nullptr, lv);
+    if (space == QG_SPACE_PHYSICAL) fft2_inplace(H, pl, -1, nullptr, lv);
     if (H.f32()) {
       qg::launch_convert<double, float>(reinterpret_cast<const double*>(pl),
                                         static_cast<float*>(H.at(dst, off)), 2 * n, H.st);
