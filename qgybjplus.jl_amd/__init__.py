@@ -16,4 +16,5 @@ from .api import (  # noqa: F401
     EnergyDiagnostics, ENERGY_NAMES,
     check_termination_, finalize_simulation_,
 )
-from . import synthetic, serving  # noqa: F401,E402
+from .state_io import write_state, restore_, read_prognostic  # noqa: F401,E402
+from . import synthetic, serving, state_io  # noqa: F401,E402
