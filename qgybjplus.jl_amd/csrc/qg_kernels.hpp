@@ -116,15 +116,17 @@ k_xa_qw(Geo g, const cplx<R>* __restrict__ T1, const cplx<R>* __restrict__ T3,
   const bool ok = row < g.nz * g.ny;
   cplx<R>* sl = lds + r * TL::LD;
   const size_t base = (size_t)row * NX;
-  cplx<R> v[FI::IT][FI::R0], w[FI::IT][FI::R0];
+  cplx<R> v[FI::IT][FI::R0], w[FI::IT][FI::R0], y3[FI::IT][FI::R0];
   R mag[FI::IT][FI::R0];
-  // B and Bx
+  // B and Bx; the By row is loaded with them
 #pragma unroll
   for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
     for (int t = 0; t < FI::R0; ++t) {
       int x = FI::elem(lane, it, t);
-      cplx<R> a = (ok && FI::valid(lane, it)) ? T1[base + x] : czero<R>();
+      const bool in = ok && FI::valid(lane, it);
+      cplx<R> a = in ? T1[base + x] : czero<R>();
+      y3[it][t] = in ? T3[base + x] : czero<R>();
       v[it][t] = a;
       w[it][t] = ik_mul((R)g.kx[x], a);
     }
@@ -137,10 +139,7 @@ k_xa_qw(Geo g, const cplx<R>* __restrict__ T1, const cplx<R>* __restrict__ T3,
 #pragma unroll
   for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
-    for (int t = 0; t < FI::R0; ++t) {
-      int x = FI::elem(lane, it, t);
-      v[it][t] = (ok && FI::valid(lane, it)) ? T3[base + x] : czero<R>();
-    }
+    for (int t = 0; t < FI::R0; ++t) v[it][t] = y3[it][t];
   FI::run(v, sl, lane, twr);  // v = By
 #pragma unroll
   for (int it = 0; it < FI::IT; ++it)
@@ -466,18 +465,23 @@ k_xb_B(Geo g, int flags, const cplx<R>* __restrict__ T1, const cplx<R>* __restri
   cplx<R>* sl = lds + r * TL::LD;
   const size_t base = (size_t)row * NX;
   cplx<R> b[FI::IT][FI::R0], v[FI::IT][FI::R0], h[FI::IT][FI::R0];
-  R zt[FI::IT][FI::R0];
+  R zt[FI::IT][FI::R0], ur[FI::IT][FI::R0], vr[FI::IT][FI::R0];
+  const bool adv = (flags & XB_ADV) != 0;  // uniform
+  // every input row is loaded before the first transform (u and v included)
 #pragma unroll
   for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
     for (int t = 0; t < FI::R0; ++t) {
       int x = FI::elem(lane, it, t);
-      b[it][t] = (ok && FI::valid(lane, it)) ? T1[base + x] : czero<R>();
+      const bool in = ok && FI::valid(lane, it);
+      b[it][t] = in ? T1[base + x] : czero<R>();
+      ur[it][t] = (in && adv) ? u[base + x] : (R)0;
+      vr[it][t] = (in && adv) ? vv[base + x] : (R)0;
       h[it][t] = czero<R>();
     }
+  if (flags & XB_REFR) load_herm_row<R, FI>(v, Pz, (size_t)row * g.Wp, NX, lane, ok);
   FI::run(b, sl, lane, twr);
   if (flags & XB_REFR) {
-    load_herm_row<R, FI>(v, Pz, (size_t)row * g.Wp, NX, lane, ok);
     FI::run(v, sl, lane, twr);
 #pragma unroll
     for (int it = 0; it < FI::IT; ++it)
@@ -492,10 +496,7 @@ k_xb_B(Geo g, int flags, const cplx<R>* __restrict__ T1, const cplx<R>* __restri
     for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
       for (int t = 0; t < FI::R0; ++t) {
-        int x = FI::elem(lane, it, t);
-        R c = (R)0;
-        if (f == 2) c = zt[it][t];
-        else if (ok && FI::valid(lane, it)) c = (f == 0 ? u : vv)[base + x];
+        const R c = f == 2 ? zt[it][t] : (f == 0 ? ur[it][t] : vr[it][t]);
         v[it][t] = mk<R>(c * b[it][t].re, c * b[it][t].im);
       }
     FF::run(v, sl, lane, twr);
@@ -655,32 +656,35 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
   const bool ok = x < g.nx;
   cplx<R>* sl = lds + s * TL::LD;
   cplx<double> acc[F::IT][F::R0];
-  cplx<R> v[F::IT][F::R0];
+  cplx<R> v[F::IT][F::R0], w[F::IT][F::R0];
+  const bool has3 = (flags & (YB_ADV | YB_REFR)) != 0, has4 = (flags & YB_ADV) != 0;  // uniform
+  // both inputs are loaded before either transform: twice the bytes in flight
 #pragma unroll
   for (int it = 0; it < F::IT; ++it)
 #pragma unroll
-    for (int t = 0; t < F::R0; ++t) acc[it][t] = czero<double>();
-  for (int f = 0; f < 2; ++f) {
-    if (f == 0 && !(flags & (YB_ADV | YB_REFR))) continue;
-    if (f == 1 && !(flags & YB_ADV)) continue;
-    const cplx<R>* src = f == 0 ? G3 : G4;
-#pragma unroll
-    for (int it = 0; it < F::IT; ++it)
-#pragma unroll
-      for (int t = 0; t < F::R0; ++t) {
-        int y = F::elem(lane, it, t);
-        bool in = ok && F::valid(lane, it);
-        v[it][t] = in ? src[((size_t)z * NY + y) * g.nx + x] : czero<R>();
-      }
+    for (int t = 0; t < F::R0; ++t) {
+      int y = F::elem(lane, it, t);
+      bool in = ok && F::valid(lane, it);
+      const size_t ad = ((size_t)z * NY + y) * g.nx + x;
+      v[it][t] = (in && has3) ? G3[ad] : czero<R>();
+      w[it][t] = (in && has4) ? G4[ad] : czero<R>();
+      acc[it][t] = czero<double>();
+    }
+  if (has3) {
     F::run(v, sl, lane, twr);
 #pragma unroll
     for (int it = 0; it < F::IT; ++it)
 #pragma unroll
+      for (int t = 0; t < F::R0; ++t) acc[it][t] = cvt<double>(v[it][t]);  // -i kx F(uB) - (i/2) F(zeta B)
+  }
+  if (has4) {
+    F::run(w, sl, lane, twr);
+#pragma unroll
+    for (int it = 0; it < F::IT; ++it)
+#pragma unroll
       for (int t = 0; t < F::R0; ++t) {
         int y = F::elem(lane, it, t);
-        cplx<double> c = cvt<double>(v[it][t]);
-        if (f == 0) acc[it][t] = c;                                    // -i kx F(uB) - (i/2) F(zeta B)
-        else acc[it][t] = acc[it][t] + ik_mul(-g.ky[y], c);           // - i ky F(vB)
+        acc[it][t] = acc[it][t] + ik_mul(-g.ky[y], cvt<double>(w[it][t]));  // - i ky F(vB)
       }
   }
   const size_t plane = (size_t)g.nx * NY;
@@ -814,9 +818,14 @@ k_xf(Geo g, int want_omega, const cplx<R>* __restrict__ Pu, const cplx<R>* __res
   const bool ok = row < g.nz * g.ny;
   cplx<R>* sl = lds + r * TL::LD;
   const size_t hb = (size_t)row * g.Wp;
-  cplx<R> v[FI::IT][FI::R0];
-  // u and v: one packed transform
+  cplx<R> v[FI::IT][FI::R0], gb[FI::IT][FI::R0], gx[FI::IT][FI::R0];
+  // all three packed rows are loaded before the first transform
   load_herm_pair<R, FI>(v, Pu, Pv, hb, NX, lane, ok);
+  if (want_omega) {
+    load_herm_pair<R, FI>(gb, Pbx, Pby, hb, NX, lane, ok);
+    load_herm_pair<R, FI>(gx, Pxx, Pxy, hb, NX, lane, ok);
+  }
+  // u and v: one packed transform
   FI::run(v, sl, lane, twr);
 #pragma unroll
   for (int it = 0; it < FI::IT; ++it)
@@ -830,23 +839,13 @@ k_xf(Geo g, int want_omega, const cplx<R>* __restrict__ Pu, const cplx<R>* __res
     }
   if (!want_omega) return;  // uniform
   // (bx, by) and (xx, xy): one packed transform each
-  R bx[FI::IT][FI::R0], by[FI::IT][FI::R0];
-  load_herm_pair<R, FI>(v, Pbx, Pby, hb, NX, lane, ok);
-  FI::run(v, sl, lane, twr);
-#pragma unroll
-  for (int it = 0; it < FI::IT; ++it)
-#pragma unroll
-    for (int t = 0; t < FI::R0; ++t) {
-      bx[it][t] = v[it][t].re;
-      by[it][t] = v[it][t].im;
-    }
-  load_herm_pair<R, FI>(v, Pxx, Pxy, hb, NX, lane, ok);
-  FI::run(v, sl, lane, twr);
+  FI::run(gb, sl, lane, twr);  // .re = bx, .im = by
+  FI::run(gx, sl, lane, twr);  // .re = xx, .im = xy
 #pragma unroll
   for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
     for (int t = 0; t < FI::R0; ++t)
-      v[it][t] = mk<R>((R)2 * (bx[it][t] * v[it][t].im - by[it][t] * v[it][t].re), (R)0);
+      v[it][t] = mk<R>((R)2 * (gb[it][t].re * gx[it][t].im - gb[it][t].im * gx[it][t].re), (R)0);
   FF::run(v, sl, lane, twr);
 #pragma unroll
   for (int it = 0; it < FI::IT; ++it)
