@@ -231,8 +231,10 @@ int qg_synchronize(qg_handle* h);
 /* Autotuner (no reference counterpart: FFTW plans are fixed by PencilFFTs,
  * src/parallel_mpi.jl:676-720, and the Thomas loops have no launch shape).
  * The tunable passes are the 13 fused FFT passes (tile variant 0, 1 or 2:
- * half, base or twice the values per block; every variant gives bitwise the
- * same result) and the 4 column operators q->psi, B->A, B->A with C, omega
+ * half, base or twice the values per block, bitwise the same result; 3: the
+ * alternative FFT plan where one is defined (radix 16 x 16 at N = 256) in
+ * half tiles, equal to round-off) and the 4 column operators q->psi, B->A,
+ * B->A with C, omega
  * (launch shape: partition depth and tile width; shapes differ in the
  * association of the partition carries, a few ulp).  qg_autotune times
  * `steps` eager steps of size dt per candidate on the model's own state,

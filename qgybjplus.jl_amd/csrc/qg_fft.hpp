@@ -135,7 +135,7 @@ template <int DIR> struct Dft<16, DIR> : DftCT<4, 4, DIR> {};
 // Plans: radices per stage (first == last so the register distribution of the
 // input and of the output coincide) and lanes per sequence T.
 // ---------------------------------------------------------------------------
-template <int N> struct FftPlan;
+template <int N, int PL = 0> struct FftPlan;
 template <> struct FftPlan<8>   { static constexpr int S = 1, r0 = 8, r1 = 1, T = 1; };
 template <> struct FftPlan<16>  { static constexpr int S = 2, r0 = 4, r1 = 1, T = 4; };
 template <> struct FftPlan<32>  { static constexpr int S = 3, r0 = 4, r1 = 2, T = 8; };
@@ -144,11 +144,17 @@ template <> struct FftPlan<128> { static constexpr int S = 3, r0 = 4, r1 = 8, T 
 template <> struct FftPlan<256> { static constexpr int S = 3, r0 = 8, r1 = 4, T = 32; };
 template <> struct FftPlan<384> { static constexpr int S = 3, r0 = 8, r1 = 6, T = 48; };
 template <> struct FftPlan<512> { static constexpr int S = 3, r0 = 8, r1 = 8, T = 64; };
+// Alternative plans (PL = 1): fewer lanes per sequence and more elements per
+// lane, so a 256-thread block spans twice the columns (128-byte row segments
+// of FP32 complex values in the Y-passes).  Where none is defined the
+// alternative is the base plan.
+template <int N> struct FftPlan<N, 1> : FftPlan<N, 0> {};
+template <> struct FftPlan<256, 1> { static constexpr int S = 2, r0 = 16, r1 = 1, T = 16; };
 
 // Stage twiddles of one lane, forward sign; the inverse uses the conjugates.
-template <typename R, int N>
+template <typename R, int N, int PL = 0>
 struct TwReg {
-  using P = FftPlan<N>;
+  using P = FftPlan<N, PL>;
   static constexpr int R0 = P::r0, R1 = P::r1, T = P::T;
   static constexpr int NB = N / R0;
   static constexpr int IT = (NB + T - 1) / T;
@@ -190,9 +196,9 @@ __device__ inline cplx<R> twmul(cplx<R> v, cplx<R> w) {
   return cmul(v, w);
 }
 
-template <typename R, int N, int DIR>
+template <typename R, int N, int DIR, int PL = 0>
 struct Fft {
-  using P = FftPlan<N>;
+  using P = FftPlan<N, PL>;
   static constexpr int R0 = P::r0;
   static constexpr int R1 = P::r1;
   static constexpr int T = P::T;
@@ -210,7 +216,7 @@ struct Fft {
 
   // s: this sequence's LDS (LDSZ elements).  Contains barriers: every thread
   // of the block must call it the same number of times.
-  __device__ static inline void run(cplx<R> (&v)[IT][R0], cplx<R>* s, int lane, const TwReg<R, N>& w) {
+  __device__ static inline void run(cplx<R> (&v)[IT][R0], cplx<R>* s, int lane, const TwReg<R, N, PL>& w) {
     if constexpr (P::S == 1) {
 #pragma unroll
       for (int it = 0; it < IT; ++it)

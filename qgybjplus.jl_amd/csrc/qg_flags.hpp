@@ -3,8 +3,9 @@
 
 namespace qg {
 
-// tile variants of the fused FFT passes (Tile<R, N, V>, V in [0, 3))
-constexpr int kTileVariants = 3;
+// tile variants of the fused FFT passes (Tile<R, N, V>, V in [0, 4); V = 3
+// runs the alternative FFT plan)
+constexpr int kTileVariants = 4;
 
 // outputs of the advection Y-pass
 enum : int { OUT_U = 1, OUT_V = 2, OUT_Q = 4, OUT_Z = 8 };

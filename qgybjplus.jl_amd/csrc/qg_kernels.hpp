@@ -27,10 +27,13 @@ namespace qg {
 
 template <typename R, int N, int V = 1>
 struct Tile {
-  static constexpr int T = FftPlan<N>::T;
-  static constexpr int LD = N + N / FftPlan<N>::r0 + 1;  // padded sequence stride in LDS
+  // V = 3: the alternative FFT plan (FftPlan<N, 1>) in blocks of the half
+  // tile (as V = 0).
+  static constexpr int PL = V == 3 ? 1 : 0;
+  static constexpr int T = FftPlan<N, PL>::T;
+  static constexpr int LD = N + N / FftPlan<N, PL>::r0 + 1;  // padded sequence stride in LDS
   static constexpr int BASE = sizeof(R) == 8 ? 256 : 512;
-  static constexpr int TARGET = V == 0 ? BASE / 2 : (V == 1 ? BASE : 2 * BASE);
+  static constexpr int TARGET = (V == 0 || V == 3) ? BASE / 2 : (V == 1 ? BASE : 2 * BASE);
   static constexpr int S0 = TARGET / T;
   static constexpr int S1 = S0 < 1 ? 1 : (S0 > 64 ? 64 : S0);
   static constexpr int S2 = S1 * T > 512 ? (512 / T < 1 ? 1 : 512 / T) : S1;
@@ -59,10 +62,10 @@ __global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
 k_ya_qw(Geo g, const cplx<R>* __restrict__ B, cplx<R>* __restrict__ T1,
         cplx<R>* __restrict__ T3, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY, V>;
-  using F = Fft<R, NY, +1>;
+  using F = Fft<R, NY, +1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
-  TwReg<R, TL::NN> twr;
+  TwReg<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const Blk blk = xcd_block();
   const int x = blk.x * TL::S + s, z = blk.y;
@@ -106,11 +109,11 @@ __global__ void __launch_bounds__((Tile<R, NX, V>::THREADS))
 k_xa_qw(Geo g, const cplx<R>* __restrict__ T1, const cplx<R>* __restrict__ T3,
         cplx<R>* __restrict__ Qa, cplx<R>* __restrict__ Qb, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NX, V>;
-  using FI = Fft<R, NX, +1>;
-  using FF = Fft<R, NX, -1>;
+  using FI = Fft<R, NX, +1, TL::PL>;
+  using FF = Fft<R, NX, -1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
-  TwReg<R, TL::NN> twr;
+  TwReg<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const int row = blockIdx.x * TL::S + r;
   const bool ok = row < g.nz * g.ny;
@@ -172,10 +175,10 @@ k_ya2_qeff(Geo g, const cplx<R>* __restrict__ Qa, const cplx<R>* __restrict__ Qb
            const cplx<R>* __restrict__ q, cplx<R>* __restrict__ qeff,
            cplx<R>* __restrict__ qw_out, double inv_f, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY, V>;
-  using F = Fft<R, NY, -1>;
+  using F = Fft<R, NY, -1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
-  TwReg<R, TL::NN> twr;
+  TwReg<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const Blk blk = xcd_block();
   const int x = blk.x * TL::S + s, z = blk.y;
@@ -234,11 +237,11 @@ k_yb_adv(Geo g, int outs_rt, const cplx<R>* __restrict__ psi, const cplx<R>* __r
          cplx<R>* __restrict__ Pu, cplx<R>* __restrict__ Pv, cplx<R>* __restrict__ Pq,
          cplx<R>* __restrict__ Pz, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY, V>;
-  using F = Fft<R, NY, +1>;
+  using F = Fft<R, NY, +1, TL::PL>;
   const int outs = OUTS ? OUTS : outs_rt;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
-  TwReg<R, TL::NN> twr;
+  TwReg<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const Blk blk = xcd_block();
   const int x = blk.x * TL::S + s, z = blk.y;
@@ -358,10 +361,10 @@ __global__ void __launch_bounds__((Tile<R, NX, V>::THREADS))
 k_x_c2r(Geo g, int nfields, const cplx<R>* __restrict__ P0, const cplx<R>* __restrict__ P1,
         R* __restrict__ o0, R* __restrict__ o1, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NX, V>;
-  using FI = Fft<R, NX, +1>;
+  using FI = Fft<R, NX, +1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
-  TwReg<R, TL::NN> twr;
+  TwReg<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const int row = blockIdx.x * TL::S + r;
   const bool ok = row < g.nz * g.ny;
@@ -391,11 +394,11 @@ __global__ void __launch_bounds__((Tile<R, NX, V>::THREADS))
 k_xb_q(Geo g, const cplx<R>* __restrict__ Pq, const R* __restrict__ u, const R* __restrict__ vv,
        cplx<R>* __restrict__ G1, cplx<R>* __restrict__ G2, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NX, V>;
-  using FI = Fft<R, NX, +1>;
-  using FF = Fft<R, NX, -1>;
+  using FI = Fft<R, NX, +1, TL::PL>;
+  using FF = Fft<R, NX, -1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
-  TwReg<R, TL::NN> twr;
+  TwReg<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const int row = blockIdx.x * TL::S + r;
   const bool ok = row < g.nz * g.ny;
@@ -454,11 +457,11 @@ k_xb_B(Geo g, int flags, const cplx<R>* __restrict__ T1, const cplx<R>* __restri
        const R* __restrict__ u, const R* __restrict__ vv, cplx<R>* __restrict__ G3,
        cplx<R>* __restrict__ G4, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NX, V>;
-  using FI = Fft<R, NX, +1>;
-  using FF = Fft<R, NX, -1>;
+  using FI = Fft<R, NX, +1, TL::PL>;
+  using FF = Fft<R, NX, -1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
-  TwReg<R, TL::NN> twr;
+  TwReg<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const int row = blockIdx.x * TL::S + r;
   const bool ok = row < g.nz * g.ny;
@@ -562,10 +565,10 @@ k_yb2_q(Geo g, int mode, int flags, const cplx<R>* __restrict__ G1, const cplx<R
         cplx<R>* __restrict__ qout, Etd etd, double nu, double inv_dz2,
         const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY, V>;
-  using F = Fft<R, NY, -1>;
+  using F = Fft<R, NY, -1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
-  TwReg<R, TL::NN> twr;
+  TwReg<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const Blk blk = xcd_block();
   const int x = blk.x * TL::S + s, z = blk.y;
@@ -646,10 +649,10 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
         const cplx<R>* __restrict__ A, const cplx<R>* __restrict__ Bn, cplx<R>* __restrict__ rhs0,
         cplx<R>* __restrict__ Bout, Etd etd, double alpha, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY, V>;
-  using F = Fft<R, NY, -1>;
+  using F = Fft<R, NY, -1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
-  TwReg<R, TL::NN> twr;
+  TwReg<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const Blk blk = xcd_block();
   const int x = blk.x * TL::S + s, z = blk.y;
@@ -742,10 +745,10 @@ __global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
 k_yf_omega(Geo g, const cplx<R>* __restrict__ psi, cplx<R>* __restrict__ Pbx, cplx<R>* __restrict__ Pby,
            cplx<R>* __restrict__ Pxx, cplx<R>* __restrict__ Pxy, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY, V>;
-  using F = Fft<R, NY, +1>;
+  using F = Fft<R, NY, +1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
-  TwReg<R, TL::NN> twr;
+  TwReg<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const Blk blk = xcd_block();
   const int x = blk.x * TL::S + s, z = blk.y;
@@ -808,11 +811,11 @@ k_xf(Geo g, int want_omega, const cplx<R>* __restrict__ Pu, const cplx<R>* __res
      const cplx<R>* __restrict__ Pxy, R* __restrict__ u, R* __restrict__ vv, cplx<R>* __restrict__ Rw,
      const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NX, V>;
-  using FI = Fft<R, NX, +1>;
-  using FF = Fft<R, NX, -1>;
+  using FI = Fft<R, NX, +1, TL::PL>;
+  using FF = Fft<R, NX, -1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
-  TwReg<R, TL::NN> twr;
+  TwReg<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const int row = blockIdx.x * TL::S + r;
   const bool ok = row < g.nz * g.ny;
@@ -863,10 +866,10 @@ __global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
 k_y_r2c_mask(Geo g, const cplx<R>* __restrict__ Rw, cplx<R>* __restrict__ out,
              const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY, V>;
-  using F = Fft<R, NY, -1>;
+  using F = Fft<R, NY, -1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
-  TwReg<R, TL::NN> twr;
+  TwReg<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const Blk blk = xcd_block();
   const int x = blk.x * TL::S + s, z = blk.y;
@@ -901,10 +904,10 @@ __global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
 k_y_herm(Geo g, const cplx<R>* __restrict__ in, cplx<R>* __restrict__ P,
          const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY, V>;
-  using F = Fft<R, NY, +1>;
+  using F = Fft<R, NY, +1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
-  TwReg<R, TL::NN> twr;
+  TwReg<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const Blk blk = xcd_block();
   const int x = blk.x * TL::S + s, z = blk.y;
@@ -944,10 +947,10 @@ __global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
 k_y_c2c(Geo g, const cplx<double>* __restrict__ in, cplx<double>* __restrict__ out,
         const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY, V>;
-  using F = Fft<R, NY, DIR>;
+  using F = Fft<R, NY, DIR, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
-  TwReg<R, TL::NN> twr;
+  TwReg<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const Blk blk = xcd_block();
   const int x = blk.x * TL::S + s, z = blk.y;
@@ -979,10 +982,10 @@ __global__ void __launch_bounds__((Tile<R, NX, V>::THREADS))
 k_x_c2c(Geo g, const cplx<double>* __restrict__ in, cplx<double>* __restrict__ out,
         const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NX, V>;
-  using F = Fft<R, NX, DIR>;
+  using F = Fft<R, NX, DIR, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
-  TwReg<R, TL::NN> twr;
+  TwReg<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const int row = blockIdx.x * TL::S + r;
   const bool ok = row < g.nz * g.ny;
@@ -1025,10 +1028,10 @@ __global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
 k_y_check(Geo g, const cplx<R>* __restrict__ psi, const cplx<R>* __restrict__ q, const cplx<R>* __restrict__ B,
           cplx<R>* __restrict__ out, int* __restrict__ flag, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY, V>;
-  using F = Fft<R, NY, +1>;
+  using F = Fft<R, NY, +1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
-  TwReg<R, TL::NN> twr;
+  TwReg<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const Blk blk = xcd_block();
   const int x = blk.x * TL::S + s, z = blk.y;
@@ -1069,10 +1072,10 @@ __global__ void __launch_bounds__((Tile<R, NX, V>::THREADS))
 k_x_check(Geo g, const cplx<R>* __restrict__ in, unsigned long long* __restrict__ absmax,
           const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NX, V>;
-  using F = Fft<R, NX, +1>;
+  using F = Fft<R, NX, +1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
-  TwReg<R, TL::NN> twr;
+  TwReg<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const int row = blockIdx.x * TL::S + r;
   const bool ok = row < g.nz * g.ny;

@@ -51,7 +51,7 @@ struct OpsV {
 };
 
 // Run-time choice of the tile variant: Ops<R>::name(v, args...) calls
-// OpsV<R, v>::name(args...) (v outside [0, 3) selects the base variant 1).
+// OpsV<R, v>::name(args...) (v outside [0, 4) selects the base variant 1).
 template <typename R>
 struct Ops {
 #define QG_OPS_FWD(name)                                        \
@@ -60,6 +60,7 @@ struct Ops {
     switch (v) {                                                \
       case 0: OpsV<R, 0>::name(static_cast<A&&>(a)...); break;  \
       case 2: OpsV<R, 2>::name(static_cast<A&&>(a)...); break;  \
+      case 3: OpsV<R, 3>::name(static_cast<A&&>(a)...); break;  \
       default: OpsV<R, 1>::name(static_cast<A&&>(a)...); break; \
     }                                                           \
   }
