@@ -607,33 +607,37 @@ k_yb2_q(Geo g, int mode, int flags, const cplx<R>* __restrict__ G1, const cplx<R
       const bool upd = keep && !(flags & YQ_FIXED);
       const size_t col0 = (size_t)y * g.nx + x;
       const double E = upd ? etd.at(0, col0) : 0.0, h1 = upd ? etd.at(1, col0) : 0.0;
-      const double h2 = (upd && mode != MODE_STAGE) ? etd.at(2, col0) : 0.0;
+      const double h2 = upd ? etd.at(2, col0) : 0.0;
       for (int side = 0; side < 2; ++side) {
         if (side == 1 && mx == x) break;
         const int cx = side ? mx : x, cy = side ? my : y;
         const size_t col = (size_t)cy * g.nx + cx;
         const size_t ad = (size_t)z * plane + col;
         cplx<double> n = side ? conj(nq) : nq;
-        cplx<double> q0 = cvt<double>(qn[ad]);
         cplx<double> rhs = czero<double>();
-        if (keep && !(flags & YQ_FIXED)) {
+        if (upd) {
           cplx<double> dq = (flags & YQ_DIFF) ? vdiff(qd, z, g.nzc, col, plane, nu, inv_dz2) : czero<double>();
           rhs = mk<double>(-n.re + dq.re, -n.im + dq.im);
         }
+        // ETD-RK2 (src/timestep.jl:362-404): a = E q0 + h1 r0 and
+        // q1 = a + h2 (r1 - r0) = [E q0 + (h1 - h2) r0] + h2 r1; the stage
+        // keeps the bracket for the final step, which then needs neither
+        // q0 nor r0.
         cplx<double> out = czero<double>();
         if (keep) {
           if (flags & YQ_FIXED) {
-            out = q0;
+            out = cvt<double>(qn[ad]);
+            if (mode == MODE_STAGE) rhs0[ad] = czero<R>();
+          } else if (mode == MODE_STAGE) {
+            const cplx<double> q0 = cvt<double>(qn[ad]);
+            out = scale(q0, E) + scale(rhs, h1);
+            rhs0[ad] = cvt<R>(scale(q0, E) + scale(rhs, h1 - h2));
           } else {
-            if (mode == MODE_STAGE) {
-              out = scale(q0, E) + scale(rhs, h1);
-            } else {
-              cplx<double> r0 = cvt<double>(rhs0[ad]);
-              out = (scale(q0, E) + scale(r0, h1)) + scale(rhs - r0, h2);
-            }
+            out = cvt<double>(rhs0[ad]) + scale(rhs, h2);
           }
+        } else if (mode == MODE_STAGE) {
+          rhs0[ad] = czero<R>();
         }
-        if (mode == MODE_STAGE) rhs0[ad] = cvt<R>(rhs);
         qout[ad] = cvt<R>(out);
       }
     }
@@ -712,19 +716,22 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
         }
         rhs = acc[it][t] + t1;
       }
+      // ETD-RK2 as for q: the stage keeps E B0 + (h1 - h2) r0, the final
+      // step adds h2 r1 to it
       cplx<double> out = czero<double>();
       if (keep) {
-        const double E = etd.at(3, col), h1 = etd.at(4, col);
-        cplx<double> b0 = cvt<double>(Bn[ad]);
+        const double h2 = etd.at(5, col);
         if (mode == MODE_STAGE) {
+          const double E = etd.at(3, col), h1 = etd.at(4, col);
+          const cplx<double> b0 = cvt<double>(Bn[ad]);
           out = scale(b0, E) + scale(rhs, h1);
+          rhs0[ad] = cvt<R>(scale(b0, E) + scale(rhs, h1 - h2));
         } else {
-          const double h2 = etd.at(5, col);
-          cplx<double> r0 = cvt<double>(rhs0[ad]);
-          out = (scale(b0, E) + scale(r0, h1)) + scale(rhs - r0, h2);
+          out = cvt<double>(rhs0[ad]) + scale(rhs, h2);
         }
+      } else if (mode == MODE_STAGE) {
+        rhs0[ad] = czero<R>();
       }
-      if (mode == MODE_STAGE) rhs0[ad] = cvt<R>(rhs);
       Bout[ad] = cvt<R>(out);
     }
 }

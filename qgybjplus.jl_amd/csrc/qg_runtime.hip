@@ -440,12 +440,15 @@ struct Engine {
     int qf = ((adv && qdyn) ? qg::YQ_ADV : 0) | (h.fixed_flow ? qg::YQ_FIXED : 0) | (diff ? qg::YQ_DIFF : 0);
     const double inv_dz2 = h.nz > 1 ? 1.0 / (h.dz * h.dz) : 0.0;
     K(mode == qg::MODE_STAGE ? "yb2_q_stage" : "yb2_q_final", qg::P_YB2_Q,
-      ((qf & qg::YQ_ADV) ? Ph * 2 * cR : 0) + P * cS * (3 + (diff ? 1 : 0)),
+      // stage: q_n, the kept bracket and q_S; final: the bracket and q_{n+1}
+      ((qf & qg::YQ_ADV) ? Ph * 2 * cR : 0) +
+          P * cS * ((mode == qg::MODE_STAGE || h.fixed_flow ? 3 : 2) + (diff ? 1 : 0)),
       [&] { O::yb2_q(h.tune[qg::P_YB2_Q], h.g, h.st, mode, qf, c(h.G1), c(h.G2), qS, qn, c(h.rq0), qout, etd(), h.nu_z, inv_dz2, twy()); });
     const bool disp = !(h.passive || h.no_disp);
     int bf = (adv ? qg::YB_ADV : 0) | (refr ? qg::YB_REFR : 0) | (disp ? qg::YB_DISP : 0);
     K(mode == qg::MODE_STAGE ? "yb2_B_stage" : "yb2_B_final", qg::P_YB2_B,
-      P * cR * (((adv || refr) ? 1 : 0) + (adv ? 1 : 0)) + P * cS * (3 + (disp ? 1 : 0)),
+      P * cR * (((adv || refr) ? 1 : 0) + (adv ? 1 : 0)) +
+          P * cS * ((mode == qg::MODE_STAGE ? 3 : 2) + (disp ? 1 : 0)),
       [&] { O::yb2_B(h.tune[qg::P_YB2_B], h.g, h.st, mode, bf, c(h.G3), c(h.G4), AS, Bn, c(h.rB0), Bout, etd(), h.f / 2, twy()); });
   }
 
