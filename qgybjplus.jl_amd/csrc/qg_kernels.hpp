@@ -646,12 +646,14 @@ k_yb2_q(Geo g, int mode, int flags, const cplx<R>* __restrict__ G1, const cplx<R
 // Y-pass (forward, full width) of G3 = -i kx F_x(uB) - (i/2) F_x(zeta B) and
 // G4 = F_x(vB): rhs_B = mask*(F_y(G3) - i ky F_y(G4)) + i (f/2) kh2 A, i.e.
 // -nB - (i/2) rB + i (f/2) kh2 A with nB = i kx F(uB) + i ky F(vB) and
-// rB = F(zeta B); then the ETD update of B.
+// rB = F(zeta B); then the ETD update of B and, with T1, the inverse y
+// transforms of the new B that the next pass over B needs.
 template <typename R, int NY, int V>
 __global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
 k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R>* __restrict__ G4,
         const cplx<R>* __restrict__ A, const cplx<R>* __restrict__ Bn, cplx<R>* __restrict__ rhs0,
-        cplx<R>* __restrict__ Bout, Etd etd, double alpha, const cplx<R>* __restrict__ tw) {
+        cplx<R>* __restrict__ Bout, cplx<R>* __restrict__ T1, cplx<R>* __restrict__ T3, Etd etd, double alpha,
+        const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY, V>;
   using F = Fft<R, NY, -1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
@@ -701,6 +703,7 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
 #pragma unroll
     for (int t = 0; t < F::R0; ++t) {
       int y = F::elem(lane, it, t);
+      v[it][t] = czero<R>();
       if (!(ok && F::valid(lane, it))) continue;
       const bool keep = g.mask[y * g.nx + x] != 0;
       const size_t col = (size_t)y * g.nx + x;
@@ -732,8 +735,40 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
       } else if (mode == MODE_STAGE) {
         rhs0[ad] = czero<R>();
       }
-      Bout[ad] = cvt<R>(out);
+      v[it][t] = cvt<R>(out);
+      Bout[ad] = v[it][t];
     }
+  // The next pass over these columns would be the inverse y transform of the
+  // new B (ya_qw: T1 = IFFTy(mask B / (nx ny)), T3 = IFFTy(i ky mask B /
+  // (nx ny))); it is made here from the stored values, which the column
+  // already holds.
+  if (T1) {  // uniform
+    using FI = Fft<R, NY, +1, TL::PL>;
+#pragma unroll
+    for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < F::R0; ++t) {
+        int y = F::elem(lane, it, t);
+        const bool in = ok && F::valid(lane, it);
+        const cplx<double> bm =
+            (in && g.mask[y * g.nx + x]) ? scale(cvt<double>(v[it][t]), g.inv_n) : czero<double>();
+        v[it][t] = cvt<R>(bm);
+        w[it][t] = cvt<R>(ik_mul(g.ky[y], bm));
+      }
+    FI::run(v, sl, lane, twr);
+    if (T3) FI::run(w, sl, lane, twr);  // uniform
+#pragma unroll
+    for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < F::R0; ++t) {
+        int y = F::elem(lane, it, t);
+        if (ok && F::valid(lane, it)) {
+          const size_t ad = ((size_t)z * NY + y) * g.nx + x;
+          T1[ad] = v[it][t];
+          if (T3) T3[ad] = w[it][t];
+        }
+      }
+  }
 }
 
 // ===========================================================================

@@ -35,7 +35,8 @@ struct OpsV {
                     const C* qn,
                     C* rhs0, C* qout, EtdView etd, double nu, double inv_dz2, const C* twy);
   static void yb2_B(const Geo& g, hipStream_t st, int mode, int flags, const C* G3, const C* G4,
-                    const C* A, const C* Bn, C* rhs0, C* Bout, EtdView etd, double alpha, const C* twy);
+                    const C* A, const C* Bn, C* rhs0, C* Bout, C* T1, C* T3, EtdView etd, double alpha,
+                    const C* twy);
   static void yf_omega(const Geo& g, hipStream_t st, const C* psi, C* Pbx, C* Pby, C* Pxx, C* Pxy, const C* twy);
   static void xf(const Geo& g, hipStream_t st, int want_omega, const C* Pu, const C* Pv, const C* Pbx,
                  const C* Pby, const C* Pxx, const C* Pxy, R* u, R* v, C* Rw, const C* twx);

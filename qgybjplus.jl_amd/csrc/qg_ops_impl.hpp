@@ -116,9 +116,10 @@ void OpsV<R, V>::yb2_q(const Geo& g, hipStream_t st, int mode, int flags, const 
 
 template <typename R, int V>
 void OpsV<R, V>::yb2_B(const Geo& g, hipStream_t st, int mode, int flags, const C* G3, const C* G4,
-                   const C* A, const C* Bn, C* rhs0, C* Bout, EtdView etd, double alpha, const C* twy) {
+                   const C* A, const C* Bn, C* rhs0, C* Bout, C* T1, C* T3, EtdView etd, double alpha,
+                   const C* twy) {
   Etd e{etd.tab, etd.plane};
-#define CALL(N) YL((k_yb2_B<R, N, V>), N, g.nx, g, mode, flags, G3, G4, A, Bn, rhs0, Bout, e, alpha, twy)
+#define CALL(N) YL((k_yb2_B<R, N, V>), N, g.nx, g, mode, flags, G3, G4, A, Bn, rhs0, Bout, T1, T3, e, alpha, twy)
   QG_SWITCH(g.ny, CALL)
 #undef CALL
 }

@@ -351,12 +351,15 @@ struct Engine {
 
   // _diagnose_flow! (src/timestep.jl:192-215) minus the velocities: q_eff with
   // wave feedback, then q -> psi.  Leaves T1 = IFFTy(mask B) when need_t1.
-  void diag(const C* qsrc, const C* Bsrc, C* psi_out, bool need_t1) {
-    bool t1 = false;
+  // t_ready: T1 (and T3 when q_w is needed) already hold the inverse y
+  // transforms of Bsrc, made by the yb2_B pass that produced Bsrc.
+  void diag(const C* qsrc, const C* Bsrc, C* psi_out, bool need_t1, bool t_ready = false) {
+    bool t1 = t_ready;
     if (!h.fixed_flow) {
       const C* src = qsrc;
       if (h.feedback) {
-        K("ya_qw", qg::P_YA_QW, P * (cS + 2 * cR), [&] { O::ya_qw(h.tune[qg::P_YA_QW], h.g, h.st, Bsrc, c(h.T1), c(h.T3), twy()); });
+        if (!t_ready)
+          K("ya_qw", qg::P_YA_QW, P * (cS + 2 * cR), [&] { O::ya_qw(h.tune[qg::P_YA_QW], h.g, h.st, Bsrc, c(h.T1), c(h.T3), twy()); });
         K("xa_qw", qg::P_XA_QW, P * 2 * cR + Ph * 2 * cR,
           [&] { O::xa_qw(h.tune[qg::P_XA_QW], h.g, h.st, c(h.T1), c(h.T3), c(h.Qa), c(h.Qb), twx()); });
         K("ya2_qeff", qg::P_YA2_QEFF, Ph * 2 * cR + P * 2 * cS,
@@ -446,10 +449,13 @@ struct Engine {
       [&] { O::yb2_q(h.tune[qg::P_YB2_Q], h.g, h.st, mode, qf, c(h.G1), c(h.G2), qS, qn, c(h.rq0), qout, etd(), h.nu_z, inv_dz2, twy()); });
     const bool disp = !(h.passive || h.no_disp);
     int bf = (adv ? qg::YB_ADV : 0) | (refr ? qg::YB_REFR : 0) | (disp ? qg::YB_DISP : 0);
+    // the new B's inverse y transforms for the next diagnosis: T1 always
+    // (xb_B), T3 when q_w is computed
+    C* T3 = (h.feedback && !h.fixed_flow) ? c(h.T3) : nullptr;
     K(mode == qg::MODE_STAGE ? "yb2_B_stage" : "yb2_B_final", qg::P_YB2_B,
-      P * cR * (((adv || refr) ? 1 : 0) + (adv ? 1 : 0)) +
+      P * cR * (((adv || refr) ? 1 : 0) + (adv ? 1 : 0) + 1 + (T3 ? 1 : 0)) +
           P * cS * ((mode == qg::MODE_STAGE ? 3 : 2) + (disp ? 1 : 0)),
-      [&] { O::yb2_B(h.tune[qg::P_YB2_B], h.g, h.st, mode, bf, c(h.G3), c(h.G4), AS, Bn, c(h.rB0), Bout, etd(), h.f / 2, twy()); });
+      [&] { O::yb2_B(h.tune[qg::P_YB2_B], h.g, h.st, mode, bf, c(h.G3), c(h.G4), AS, Bn, c(h.rB0), Bout, c(h.T1), T3, etd(), h.f / 2, twy()); });
   }
 
   // step!(model, ExponentialRungeKutta2) -> _advance_etdrk2! (src/timestep.jl:335-415).
@@ -469,19 +475,19 @@ struct Engine {
       t1_only(Bn);
     }
     rhs_update(qg::MODE_STAGE, qn, c(h.psi), c(h.A), qn, Bn, c(h.qs), c(h.Bs));
-    // RHS at the stage
+    // RHS at the stage (yb2_B left T1, T3 of B_S)
     C* psiS = h.fixed_flow ? c(h.psi) : c(h.psis);
-    diag(c(h.qs), c(h.Bs), psiS, true);
+    diag(c(h.qs), c(h.Bs), psiS, true, true);
     wave_inv(c(h.Bs), c(h.As), nullptr);
     if (need_uv && !h.fixed_flow) velocities(psiS);
     rhs_update(qg::MODE_FINAL, c(h.qs), psiS, c(h.As), qn, Bn, q1, B1);
-    // finalize (src/timestep.jl:300-332)
-    diag(q1, B1, c(h.psi), false);
+    // finalize (src/timestep.jl:300-332); yb2_B left T1, T3 of B_{n+1}
+    diag(q1, B1, c(h.psi), false, true);
     wave_inv(B1, c(h.A), c(h.C));
     velocities_w(c(h.psi));
     h.cur = 1 - h.cur;
     h.diag_valid = true;
-    h.t1_valid = h.feedback && !h.fixed_flow;
+    h.t1_valid = true;
   }
 
   void compute_qw(C* out) {
