@@ -155,6 +155,10 @@ int qg_check(qg_handle* h, int* nonfinite, double* max_abs_psi);
  * state every chunk of steps starts from (q, B, psi, A, the y-transformed B,
  * u, v), kept to step again up to a failing step. */
 int qg_run(qg_handle* h, double dt, int nsteps, int* done, int* failure, double* max_abs_psi);
+/* Replaces _progress_maxima (src/simulation.jl:46-81): the maximum physical
+ * wave velocity |ifft2(B + kh2 A / 4)| and the maximum flow speed
+ * hypot(u, v) over the grid (all members), at the transform precision. */
+int qg_progress_maxima(qg_handle* h, double* wave_speed, double* flow_speed);
 /* The check of one ensemble member. */
 int qg_check_member(qg_handle* h, int member, int* nonfinite, double* max_abs_psi);
 

@@ -16,6 +16,7 @@ wave profile) are evaluated on the host, as in the reference.
 from __future__ import annotations
 
 import enum
+import logging
 import math
 import time
 from dataclasses import dataclass, field
@@ -459,6 +460,12 @@ class QGYBJModel:
     def compute_qw(self) -> np.ndarray:
         return self._h.compute_qw()
 
+    def progress_maxima(self) -> dict:
+        """``_progress_maxima`` (src/simulation.jl:46-81) on the device: the
+        largest physical wave velocity |LA| and flow speed hypot(u, v)."""
+        w, f = self._h.progress_maxima()
+        return {"wave_speed": w, "flow_speed": f}
+
     def energies(self) -> dict:
         """The five energies of _local_energy_components (src/core/io.jl:331-395),
         reduced on the device from psi = L^-1 q and A, C from B."""
@@ -810,10 +817,15 @@ class Simulation:
     """``Simulation(model; Δt, stop_time, stop_iteration)`` (src/simulation.jl:204-227)."""
 
     def __init__(self, model: QGYBJModel, *, dt: float = 1.0, stop_time=None, stop_iteration=None,
-                 check_every: int = 1, diagnostics=False):
+                 check_every: int = 1, diagnostics=False, diagnostics_interval: int = 10, verbose: bool = True):
         if model.finalized:
             raise RuntimeError("cannot create a simulation from a finalized model")
+        if not int(diagnostics_interval) > 0:
+            raise ValueError("diagnostics_interval must be positive")
         self.model = model
+        # run options (SimulationRunOptions, src/core/simulation.jl:19-33)
+        self.diagnostics_interval = int(diagnostics_interval)
+        self.verbose = bool(verbose)
         self.diagnostics = diagnostics  # False, IterationInterval or TimeInterval
         self.diagnostics_manager: Optional[EnergyDiagnostics] = None
         self.timestepper = ExponentialRungeKutta2(dt=dt)
@@ -836,11 +848,12 @@ class Simulation:
         tol = 8 * np.finfo(float).eps * max(abs(t), abs(self.stop_time), abs(self.timestepper.dt), 1.0)
         return t < self.stop_time - tol
 
-    def _steps_to_next_event(self, mgr, progress: bool) -> int:
+    def _steps_to_next_event(self, mgr, report: int) -> int:
         """Iterations the run! loop takes from the current clock before it
         stops (stop iteration or stop time) or has to do something on the host
-        after a step (a due diagnostics record, a progress line): the clock is
-        advanced on a copy exactly as the loop advances it."""
+        after a step (a due diagnostics record, or a progress report every
+        `report` iterations; 0: none): the clock is advanced on a copy exactly
+        as the loop advances it."""
         c = Clock(self.clock.time, self.clock.iteration)
         dt = self.timestepper.dt
         n = 0
@@ -852,7 +865,8 @@ class Simulation:
             n += 1
             c.iteration += 1
             c.time += dt
-            if progress or (mgr is not None and mgr.due(c)) or n >= self._max_stretch:
+            if (report and c.iteration % int(report) == 0) or (mgr is not None and mgr.due(c)) \
+                    or n >= self._max_stretch:
                 break
         return n
 
@@ -873,8 +887,28 @@ def check_termination_(simulation: Simulation):
     return simulation
 
 
-def run_(simulation: Simulation, *, stop_iteration=None, stop_time=None, progress: bool = False):
-    """``run!(simulation)`` (src/simulation.jl:570-662) without output writers."""
+_log = logging.getLogger("qgybjplus_amd")
+
+
+def _report_progress(simulation: Simulation, detailed: bool):
+    """The progress report of run! (src/simulation.jl:629-641, 83-97): with
+    progress = true the line with the wave and flow speed maxima, else an
+    informational log record."""
+    c = simulation.clock
+    if detailed:
+        m = simulation.model.progress_maxima()
+        print(f"iteration={c.iteration} | time={c.time} s | max_wave_speed={m['wave_speed']:.3e} m/s | "
+              f"max_flow_speed={m['flow_speed']:.3e} m/s", flush=True)
+    else:
+        _log.info("Simulation progress iteration=%d time=%s", c.iteration, c.time)
+
+
+def run_(simulation: Simulation, *, stop_iteration=None, stop_time=None, progress: Optional[bool] = None,
+         diagnostics_interval: Optional[int] = None, verbose: Optional[bool] = None):
+    """``run!(simulation)`` (src/simulation.jl:570-662) without output writers.
+    Every diagnostics_interval iterations a verbose run reports progress:
+    progress = True prints the reference's line with the speed maxima, else an
+    INFO record goes to the ``qgybjplus_amd`` logger."""
     if simulation.state is SimulationState.Finalized:
         raise RuntimeError("cannot run a finalized simulation")
     if simulation.state is SimulationState.Running:
@@ -889,6 +923,17 @@ def run_(simulation: Simulation, *, stop_iteration=None, stop_time=None, progres
         simulation.stop_time = stop_time
     if simulation.stop_iteration is None and simulation.stop_time is None:
         raise ValueError("set stop_iteration or stop_time before run!")
+    if verbose is not None:
+        simulation.verbose = bool(verbose)
+    if progress is not None:
+        simulation.verbose = bool(progress)
+    detailed = progress is True
+    if diagnostics_interval is not None:
+        if not int(diagnostics_interval) > 0:
+            raise ValueError("diagnostics_interval must be positive")
+        simulation.diagnostics_interval = int(diagnostics_interval)
+    reporting = simulation.verbose and (detailed or _log.isEnabledFor(logging.INFO))
+    report = simulation.diagnostics_interval if reporting else 0
     simulation.state = SimulationState.Running
     model = simulation.model
     ts = simulation.timestepper
@@ -903,7 +948,7 @@ def run_(simulation: Simulation, *, stop_iteration=None, stop_time=None, progres
             # every step checked, as run! does: stretches of steps between
             # host events run on the device (qg_run), checks included
             while True:
-                n = simulation._steps_to_next_event(mgr, progress)
+                n = simulation._steps_to_next_event(mgr, report)
                 if n == 0:
                     break
                 if model.finalized:
@@ -917,8 +962,8 @@ def run_(simulation: Simulation, *, stop_iteration=None, stop_time=None, progres
                     _raise_termination(failure == 1, m)
                 if mgr is not None and mgr.due(simulation.clock):
                     mgr.record(model, simulation.clock)
-                if progress:
-                    print(f"iteration={simulation.clock.iteration} | time={simulation.clock.time} s", flush=True)
+                if report and simulation.clock.iteration % report == 0:
+                    _report_progress(simulation, detailed)
         else:
             while True:
                 it = simulation.clock.iteration
@@ -933,8 +978,8 @@ def run_(simulation: Simulation, *, stop_iteration=None, stop_time=None, progres
                 simulation.clock.time += ts.dt
                 if mgr is not None and mgr.due(simulation.clock):
                     mgr.record(model, simulation.clock)
-                if progress:
-                    print(f"iteration={simulation.clock.iteration} | time={simulation.clock.time} s", flush=True)
+                if report and simulation.clock.iteration % report == 0:
+                    _report_progress(simulation, detailed)
         simulation.state = SimulationState.Stopped
     except Exception:
         simulation.state = SimulationState.Failed

@@ -48,6 +48,9 @@ void launch_surface_wave(const Geo& g, hipStream_t st, int m0, int nm, const dou
 // 5 values per member (wave KE, PE, CE, mean-flow KE, PE); partial holds
 // energy_partial_size(g, nmem) doubles.
 size_t energy_partial_size(const Geo& g, int nmem);
+// max hypot(u, v) over n points of two real fields (FP32 or FP64), folded into
+// *out (the bit pattern of a non-negative double; the caller clears it).
+void launch_max_hypot(const void* u, const void* v, bool f32, size_t n, unsigned long long* out, hipStream_t st);
 void launch_energies(const Geo& g, hipStream_t st, int m0, int nmem, const void* psi, const void* A,
                      const void* C, bool f32, const double* a, double* partial, double* out);
 // psi_f64: psi is cplx<double> while q is stored per f32 (set-up from a

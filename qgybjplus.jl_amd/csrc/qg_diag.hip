@@ -87,7 +87,44 @@ __global__ void k_energy_final(int nblocks, int nmem, double norm, const double*
   out[id] = s * norm;
 }
 
+// max hypot(u, v) over n points (_progress_maxima's flow speed,
+// src/simulation.jl:71-76): grid-stride, one atomic per block, skipped when
+// the accumulator already holds more (non-negative doubles compare like their
+// bit patterns; NaN counts as +inf).
+template <typename T>
+__global__ void __launch_bounds__(kEnergyThreads)
+k_max_hypot(const T* __restrict__ u, const T* __restrict__ v, size_t n, unsigned long long* __restrict__ out) {
+  double m = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double h = hypot((double)u[i], (double)v[i]);
+    m = (h > m || h != h) ? h : m;
+  }
+  if (m != m) m = __builtin_huge_val();
+  __shared__ double red[kEnergyThreads];
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = kEnergyThreads / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x + s] > red[threadIdx.x] ? red[threadIdx.x + s]
+                                                                                       : red[threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && red[0] > 0.0) {
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(red[0]);
+    if (bits > __hip_atomic_load(out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(out, bits);
+  }
+}
+
 }  // namespace
+
+void launch_max_hypot(const void* u, const void* v, bool f32, size_t n, unsigned long long* out, hipStream_t st) {
+  const int blocks = 1024;
+  if (f32)
+    hipLaunchKernelGGL(k_max_hypot<float>, dim3(blocks), dim3(kEnergyThreads), 0, st, static_cast<const float*>(u),
+                       static_cast<const float*>(v), n, out);
+  else
+    hipLaunchKernelGGL(k_max_hypot<double>, dim3(blocks), dim3(kEnergyThreads), 0, st, static_cast<const double*>(u),
+                       static_cast<const double*>(v), n, out);
+}
 
 size_t energy_partial_size(const Geo& g, int nmem) {
   const int plane = g.nx * g.ny;

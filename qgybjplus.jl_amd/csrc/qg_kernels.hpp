@@ -1106,6 +1106,52 @@ k_y_check(Geo g, const cplx<R>* __restrict__ psi, const cplx<R>* __restrict__ q,
     }
 }
 
+// Progress maxima (_progress_maxima, src/simulation.jl:46-81), Y-pass of the
+// wave velocity LA: the spectrum B + kh2 A / 4 (YBJ+; B itself otherwise),
+// formed in FP64 from the stored fields, / (nx*ny), inverse y transform into
+// `out`.  k_x_check then folds max |LA| over the grid.
+template <typename R, int NY, int V>
+__global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
+k_y_la(Geo g, const cplx<R>* __restrict__ B, const cplx<R>* __restrict__ A, int ybj_plus,
+       cplx<R>* __restrict__ out, const cplx<R>* __restrict__ tw) {
+  using TL = Tile<R, NY, V>;
+  using F = Fft<R, NY, +1, TL::PL>;
+  __shared__ cplx<R> lds[TL::S * TL::LD];
+  const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
+  TwReg<R, TL::NN, TL::PL> twr;
+  twr.load(lane, tw);
+  const Blk blk = xcd_block();
+  const int x = blk.x * TL::S + s, z = blk.y;
+  const bool ok = x < g.nx;
+  cplx<R>* sl = lds + s * TL::LD;
+  cplx<R> v[F::IT][F::R0];
+  const double kxv = ok ? g.kx[x] : 0.0;
+#pragma unroll
+  for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < F::R0; ++t) {
+      int y = F::elem(lane, it, t);
+      cplx<double> la = czero<double>();
+      if (ok && F::valid(lane, it)) {
+        const size_t ad = ((size_t)z * NY + y) * g.nx + x;
+        la = cvt<double>(B[ad]);
+        if (ybj_plus) {
+          const double kh2 = kxv * kxv + g.ky[y] * g.ky[y];
+          la = la + scale(cvt<double>(A[ad]), 0.25 * kh2);
+        }
+      }
+      v[it][t] = cvt<R>(scale(la, g.inv_n));
+    }
+  F::run(v, sl, lane, twr);
+#pragma unroll
+  for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < F::R0; ++t) {
+      int y = F::elem(lane, it, t);
+      if (ok && F::valid(lane, it)) out[((size_t)z * NY + y) * g.nx + x] = v[it][t];
+    }
+}
+
 // X-pass: inverse x transform of the Y-pass output; folds max |value| into
 // absmax (non-negative doubles compare like their bit patterns; NaN counts
 // as +inf).  Nothing is stored.

@@ -49,6 +49,10 @@ struct OpsV {
   // (tmp: one full-width field of scratch)
   static void check(const Geo& g, hipStream_t st, const C* psi, const C* q, const C* B, C* tmp, int* flag,
                     unsigned long long* absmax, const C* twx, const C* twy);
+  // progress maxima: max |ifft2(B + kh2 A / 4)| -> absmax (B alone unless
+  // ybj_plus; tmp: one full-width field of scratch)
+  static void la_max(const Geo& g, hipStream_t st, const C* B, const C* A, int ybj_plus, C* tmp,
+                     unsigned long long* absmax, const C* twx, const C* twy);
 };
 
 // Run-time choice of the tile variant: Ops<R>::name(v, args...) calls
@@ -81,6 +85,7 @@ struct Ops {
   QG_OPS_FWD(y_c2c)
   QG_OPS_FWD(x_c2c)
   QG_OPS_FWD(check)
+  QG_OPS_FWD(la_max)
 #undef QG_OPS_FWD
 };
 

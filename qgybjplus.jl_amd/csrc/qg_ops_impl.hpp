@@ -190,4 +190,15 @@ void OpsV<R, V>::check(const Geo& g, hipStream_t st, const C* psi, const C* q, c
 #undef CALL
 }
 
+template <typename R, int V>
+void OpsV<R, V>::la_max(const Geo& g, hipStream_t st, const C* B, const C* A, int ybj_plus, C* tmp,
+                        unsigned long long* absmax, const C* twx, const C* twy) {
+#define CALL(N) YL((k_y_la<R, N, V>), N, g.nx, g, B, A, ybj_plus, tmp, twy)
+  QG_SWITCH(g.ny, CALL)
+#undef CALL
+#define CALL(N) XL((k_x_check<R, N, V>), N, g, tmp, absmax, twx)
+  QG_SWITCH(g.nx, CALL)
+#undef CALL
+}
+
 }  // namespace qg

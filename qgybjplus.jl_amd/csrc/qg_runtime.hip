@@ -103,6 +103,7 @@ struct qg_handle {
   int* d_ferr = nullptr;  // error flags of member re-factorisations
   unsigned long long* d_warn = nullptr;
   unsigned long long* d_absmax = nullptr;
+  unsigned long long* d_pmax = nullptr;  // progress maxima: wave speed, flow speed
   int* d_flag = nullptr;
   bool diag_valid = false, t1_valid = false;
   // hipGraphs of two steady-state steps starting from each parity of the
@@ -343,6 +344,17 @@ struct Engine {
       O::check(1, g, h.st, static_cast<const C*>(h.at(h.psi, off)), static_cast<const C*>(h.at(h.q[h.cur], off)),
                static_cast<const C*>(h.at(h.B[h.cur], off)), c(h.G3), h.d_flag, h.d_absmax, twx(), twy());
     });
+  }
+
+  // _progress_maxima (src/simulation.jl:46-81): max |ifft2(LA)| with LA =
+  // B + kh2 A / 4 (YBJ+) into d_pmax[0], max hypot(u, v) into d_pmax[1].
+  // The formulations built here are YBJ+ and passive (normal YBJ is
+  // rejected), so LA always carries kh2 A / 4; A is zero when passive or
+  // without dispersion, as the reference fills it.
+  void progress_maxima() {
+    O::la_max(1, h.g, h.st, static_cast<const C*>(h.B[h.cur]), static_cast<const C*>(h.A), 1, c(h.G3), h.d_pmax,
+              twx(), twy());
+    qg::launch_max_hypot(h.u, h.v, h.f32(), (size_t)h.plane * h.g.nz, h.d_pmax + 1, h.st);
   }
 
   void t1_only(const C* Bsrc) {
@@ -766,6 +778,7 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
     h->d_ferr = h->alloc<int>(8);
     h->d_warn = h->alloc<unsigned long long>(1);
     h->d_absmax = h->alloc<unsigned long long>(1);
+    h->d_pmax = h->alloc<unsigned long long>(2);
     h->d_flag = h->alloc<int>(1);
     HIPCHECK(hipMemset(h->d_err, 0, 4 * sizeof(int)));
     HIPCHECK(hipMemset(h->d_warn, 0, sizeof(unsigned long long)));
@@ -1363,6 +1376,21 @@ int qg_check_member(qg_handle* h, int member, int* nonfinite, double* max_abs_ps
   return guarded(h, [&] {
     check_member_index(*h, member);
     check_range(*h, member, 1, nonfinite, max_abs_psi);
+  });
+}
+
+int qg_progress_maxima(qg_handle* h, double* wave_speed, double* flow_speed) {
+  return guarded(h, [&] {
+    qg_handle& H = *h;
+    HIPCHECK(hipMemsetAsync(H.d_pmax, 0, 2 * sizeof(unsigned long long), H.st));
+    with_engine(H, [&](auto& e) { e.progress_maxima(); });
+    unsigned long long bits[2] = {0, 0};
+    HIPCHECK(hipMemcpyAsync(bits, H.d_pmax, sizeof(bits), hipMemcpyDeviceToHost, H.st));
+    HIPCHECK(hipStreamSynchronize(H.st));
+    double m[2];
+    std::memcpy(m, bits, sizeof m);
+    if (wave_speed) *wave_speed = m[0];
+    if (flow_speed) *flow_speed = m[1];
   });
 }
 

@@ -112,3 +112,34 @@ def test_run_loop_stops_at_time_and_records_diagnostics(qg):
         assert np.array_equal(getattr(b.fields, f), getattr(a.fields, f)), f
     a.finalize()
     b.finalize()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-6)])
+def test_progress_maxima_match_the_fields(qg, precision, tol):
+    """_progress_maxima (src/simulation.jl:46-81): max |ifft2(B + kh2 A / 4)|
+    and max hypot(u, v), against numpy on the downloaded fields."""
+    print(f"case progress maxima {precision}", flush=True)
+    a, _ = _pair(qg, precision, urms=0.5)
+    qg.step_(a, qg.ExponentialRungeKutta2(dt=500.0), 3)
+    m = a.progress_maxima()
+    kh2 = a.grid.kh2[None]
+    la = np.fft.ifft2(a.fields.B + 0.25 * kh2 * a.fields.A, axes=(1, 2))
+    wave = float(np.max(np.abs(la)))
+    flow = float(np.max(np.hypot(a.fields.u, a.fields.v)))
+    assert abs(m["wave_speed"] - wave) <= tol * wave, (m, wave)
+    assert abs(m["flow_speed"] - flow) <= tol * flow, (m, flow)
+    a.finalize()
+
+
+def test_progress_lines_do_not_change_the_run(qg, capsys):
+    print("case run! progress lines", flush=True)
+    a, b = _pair(qg, "fp64", urms=0.5)
+    qg.run_(qg.Simulation(a, dt=500.0, stop_iteration=23))
+    qg.run_(qg.Simulation(b, dt=500.0, stop_iteration=23), progress=True, diagnostics_interval=5)
+    lines = [ln for ln in capsys.readouterr().out.splitlines() if ln.startswith("iteration=")]
+    assert [int(ln.split("|")[0].split("=")[1]) for ln in lines] == [5, 10, 15, 20]
+    assert "max_wave_speed=" in lines[0] and "max_flow_speed=" in lines[0]
+    for f in FIELDS:
+        assert np.array_equal(getattr(b.fields, f), getattr(a.fields, f)), f
+    a.finalize()
+    b.finalize()
