@@ -42,11 +42,12 @@ LAUNCH_KERNEL = {
     "ya_T1": ("k_ya_qw", "ya_qw", None), "ya_qw": ("k_ya_qw", "ya_qw", None),
     "xa_qw": ("k_xa_qw", "xa_qw", None), "ya2_qeff": ("k_ya2_qeff", "ya2_qeff", None),
     "yb_uv": ("k_yb_adv", "yb_adv", None, 3), "yb_adv": ("k_yb_adv", "yb_adv", None, 12),
+    "yb_adv_uv": ("k_yb_adv", "yb_adv", None, 15),
     "x_c2r_uv": ("k_x_c2r", "x_c2r", None), "x_c2r_w": ("k_x_c2r", "x_c2r", None),
-    "xb_q": ("k_xb_q", "xb_q", None), "xb_B": ("k_xb_B", "xb_B", None),
+    "xb_q": ("k_xb_q", "xb_q", None, 0), "xb_q_uv": ("k_xb_q", "xb_q", None, 1), "xb_B": ("k_xb_B", "xb_B", None),
     "yb2_q_stage": ("k_yb2_q", "yb2_q", "stage"), "yb2_q_final": ("k_yb2_q", "yb2_q", "final"),
     "yb2_B_stage": ("k_yb2_B", "yb2_B", "stage"), "yb2_B_final": ("k_yb2_B", "yb2_B", "final"),
-    "yf_omega": ("k_yf_omega", "yf_omega", None), "xf": ("k_xf", "xf", None),
+    "yf_omega": ("k_yf_omega", "yf_omega", None, 1), "xf": ("k_xf", "xf", None),
     "y_r2c_omega": ("k_y_r2c_mask", "y_r2c", None), "y_herm_w": ("k_y_herm", "y_herm", None),
     "thomas_psi": ("k_colscan<0", "thomas_psi", None), "thomas_A": ("k_colscan<1", "thomas_A", None),
     "thomas_AC": ("k_colscan<2", "thomas_AC", None), "omega_gtsv": ("k_colscan<3", "omega", None),
@@ -68,7 +69,7 @@ def kernel_symbol(name, choice):
     if base.startswith("k_colscan"):
         m, qc = SCAN_SHAPES[v if v >= 0 else SCAN_DEFAULT]
         return f"{base}, {m}, {qc}, float>", kind
-    tail = "".join(f", {e}" for e in extra)  # compile-time output set of k_yb_adv
+    tail = "".join(f", {e}" for e in extra)  # compile-time output set / input mode
     return f"{base}<float, {SIZE[0]}, {v if v >= 0 else 1}{tail}>", kind
 
 

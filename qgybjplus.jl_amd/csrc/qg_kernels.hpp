@@ -389,10 +389,14 @@ k_x_c2r(Geo g, int nfields, const cplx<R>* __restrict__ P0, const cplx<R>* __res
 }
 
 // X-pass for q advection: q = C2R(Pq); u*q, v*q -> forward real FFTs -> G1, G2.
-template <typename R, int NX, int V>
+// UVP = 0: u and v are read as physical rows.  UVP = 1: they are made here
+// from their y-transformed half rows Pu, Pv (one packed inverse transform, as
+// k_x_c2r does) and also written out to u and v for the B pass that follows.
+template <typename R, int NX, int V, int UVP>
 __global__ void __launch_bounds__((Tile<R, NX, V>::THREADS))
-k_xb_q(Geo g, const cplx<R>* __restrict__ Pq, const R* __restrict__ u, const R* __restrict__ vv,
-       cplx<R>* __restrict__ G1, cplx<R>* __restrict__ G2, const cplx<R>* __restrict__ tw) {
+k_xb_q(Geo g, const cplx<R>* __restrict__ Pq, const cplx<R>* __restrict__ Pu, const cplx<R>* __restrict__ Pv,
+       R* __restrict__ u, R* __restrict__ vv, cplx<R>* __restrict__ G1, cplx<R>* __restrict__ G2,
+       const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NX, V>;
   using FI = Fft<R, NX, +1, TL::PL>;
   using FF = Fft<R, NX, -1, TL::PL>;
@@ -405,26 +409,47 @@ k_xb_q(Geo g, const cplx<R>* __restrict__ Pq, const R* __restrict__ u, const R* 
   cplx<R>* sl = lds + r * TL::LD;
   cplx<R> v[FI::IT][FI::R0];
   R qv[FI::IT][FI::R0];
-  load_herm_row<R, FI>(v, Pq, (size_t)row * g.Wp, NX, lane, ok);
-  FI::run(v, sl, lane, twr);
-#pragma unroll
-  for (int it = 0; it < FI::IT; ++it)
-#pragma unroll
-    for (int t = 0; t < FI::R0; ++t) qv[it][t] = v[it][t].re;
   const size_t hb = (size_t)row * g.Wp;
-  // u*q and v*q (one kind and scale) packed as Z = uq + i vq: one forward
-  // transform, then F(uq)[k] = (Z[k] + conj Z[-k])/2 and
-  // F(vq)[k] = (Z[k] - conj Z[-k])/(2i), with Z[-k] read back through LDS.
+  if constexpr (UVP) {
+    // both inputs are loaded before the first transform
+    cplx<R> w[FI::IT][FI::R0];
+    load_herm_pair<R, FI>(w, Pu, Pv, hb, NX, lane, ok);
+    load_herm_row<R, FI>(v, Pq, hb, NX, lane, ok);
+    FI::run(w, sl, lane, twr);
+    FI::run(v, sl, lane, twr);
 #pragma unroll
-  for (int it = 0; it < FI::IT; ++it)
+    for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
-    for (int t = 0; t < FI::R0; ++t) {
-      int x = FI::elem(lane, it, t);
-      const bool in = ok && FI::valid(lane, it);
-      const R cu = in ? u[(size_t)row * NX + x] : (R)0;
-      const R cv = in ? vv[(size_t)row * NX + x] : (R)0;
-      v[it][t] = mk<R>(cu * qv[it][t], cv * qv[it][t]);
-    }
+      for (int t = 0; t < FI::R0; ++t) {
+        int x = FI::elem(lane, it, t);
+        if (ok && FI::valid(lane, it)) {
+          u[(size_t)row * NX + x] = w[it][t].re;
+          vv[(size_t)row * NX + x] = w[it][t].im;
+        }
+        qv[it][t] = v[it][t].re;
+        v[it][t] = mk<R>(w[it][t].re * qv[it][t], w[it][t].im * qv[it][t]);
+      }
+  } else {
+    load_herm_row<R, FI>(v, Pq, hb, NX, lane, ok);
+    FI::run(v, sl, lane, twr);
+#pragma unroll
+    for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < FI::R0; ++t) qv[it][t] = v[it][t].re;
+    // u*q and v*q (one kind and scale) packed as Z = uq + i vq: one forward
+    // transform, then F(uq)[k] = (Z[k] + conj Z[-k])/2 and
+    // F(vq)[k] = (Z[k] - conj Z[-k])/(2i), with Z[-k] read back through LDS.
+#pragma unroll
+    for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < FI::R0; ++t) {
+        int x = FI::elem(lane, it, t);
+        const bool in = ok && FI::valid(lane, it);
+        const R cu = in ? u[(size_t)row * NX + x] : (R)0;
+        const R cv = in ? vv[(size_t)row * NX + x] : (R)0;
+        v[it][t] = mk<R>(cu * qv[it][t], cv * qv[it][t]);
+      }
+  }
   FF::run(v, sl, lane, twr);
   __syncthreads();  // the last stage's LDS reads are done
 #pragma unroll
@@ -782,10 +807,13 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
 // the two-level average psi-bar (psi itself at the top) times -i*kx*kh2 and
 // -i*ky*kh2.  psi(z) and psi(z+1) are read once; the four Hermitian inputs are
 // formed in registers before the four transforms.
-template <typename R, int NY, int V>
+// UV = 1: the same pass also makes the y-transformed half rows of u and v
+// (what k_yb_adv writes for OUT_U | OUT_V) from the psi it holds.
+template <typename R, int NY, int V, int UV>
 __global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
 k_yf_omega(Geo g, const cplx<R>* __restrict__ psi, cplx<R>* __restrict__ Pbx, cplx<R>* __restrict__ Pby,
-           cplx<R>* __restrict__ Pxx, cplx<R>* __restrict__ Pxy, const cplx<R>* __restrict__ tw) {
+           cplx<R>* __restrict__ Pxx, cplx<R>* __restrict__ Pxy, cplx<R>* __restrict__ Pu,
+           cplx<R>* __restrict__ Pv, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY, V>;
   using F = Fft<R, NY, +1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
@@ -801,7 +829,8 @@ k_yf_omega(Geo g, const cplx<R>* __restrict__ psi, cplx<R>* __restrict__ Pbx, cp
   const bool top = z % g.nzc == g.nzc - 1;  // top level of this member's column
   const int zu = top ? z : z + 1;
   const double dz = g.dz;
-  cplx<R> v[4][F::IT][F::R0];
+  constexpr int NO = UV ? 6 : 4;
+  cplx<R> v[NO][F::IT][F::R0];
 #pragma unroll
   for (int it = 0; it < F::IT; ++it)
 #pragma unroll
@@ -811,6 +840,10 @@ k_yf_omega(Geo g, const cplx<R>* __restrict__ psi, cplx<R>* __restrict__ Pbx, cp
       // the four factors are formed at the transform precision R
       const cplx<R> p = in ? psi[((size_t)z * NY + y) * g.nx + x] : czero<R>();
       const cplx<R> pm = in ? psi[((size_t)z * NY + my) * g.nx + mx] : czero<R>();
+      if constexpr (UV) {
+        v[4][it][t] = herm(ik_mul((R)-g.ky[y], p), ik_mul((R)-g.ky[my], pm), (R)g.inv_n);
+        v[5][it][t] = herm(ik_mul((R)kxv, p), ik_mul((R)kxm, pm), (R)g.inv_n);
+      }
       const cplx<R> up = in ? psi[((size_t)zu * NY + y) * g.nx + x] : czero<R>();
       const cplx<R> upm = in ? psi[((size_t)zu * NY + my) * g.nx + mx] : czero<R>();
       const R rdz = (R)dz;
@@ -830,8 +863,8 @@ k_yf_omega(Geo g, const cplx<R>* __restrict__ psi, cplx<R>* __restrict__ Pbx, cp
       v[3][it][t] = herm(ik_mul((R)-(ky * kh2), pa), ik_mul((R)-(kym * kh2m), pam), rinv);
     }
 #pragma unroll
-  for (int o = 0; o < 4; ++o) {
-    cplx<R>* dst = o == 0 ? Pbx : o == 1 ? Pby : o == 2 ? Pxx : Pxy;
+  for (int o = 0; o < NO; ++o) {
+    cplx<R>* dst = o == 0 ? Pbx : o == 1 ? Pby : o == 2 ? Pxx : o == 3 ? Pxy : o == 4 ? Pu : Pv;
     F::run(v[o], sl, lane, twr);
 #pragma unroll
     for (int it = 0; it < F::IT; ++it)

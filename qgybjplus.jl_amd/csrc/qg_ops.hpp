@@ -27,8 +27,9 @@ struct OpsV {
                      C* Pz, const C* twy);
   static void x_c2r(const Geo& g, hipStream_t st, int nfields, const C* P0, const C* P1, R* o0, R* o1,
                     const C* twx);
-  static void xb_q(const Geo& g, hipStream_t st, const C* Pq, const R* u, const R* v, C* G1, C* G2,
-                   const C* twx);
+  // Pu == nullptr: u, v are read; otherwise they are made from Pu, Pv and written
+  static void xb_q(const Geo& g, hipStream_t st, const C* Pq, const C* Pu, const C* Pv, R* u, R* v, C* G1,
+                   C* G2, const C* twx);
   static void xb_B(const Geo& g, hipStream_t st, int flags, const C* T1, const C* Pz, const R* u, const R* v,
                    C* G3, C* G4, const C* twx);
   static void yb2_q(const Geo& g, hipStream_t st, int mode, int flags, const C* G1, const C* G2, const C* qd,
@@ -37,7 +38,9 @@ struct OpsV {
   static void yb2_B(const Geo& g, hipStream_t st, int mode, int flags, const C* G3, const C* G4,
                     const C* A, const C* Bn, C* rhs0, C* Bout, C* T1, C* T3, EtdView etd, double alpha,
                     const C* twy);
-  static void yf_omega(const Geo& g, hipStream_t st, const C* psi, C* Pbx, C* Pby, C* Pxx, C* Pxy, const C* twy);
+  // Pu != nullptr: also the y-transformed half rows of u and v (as yb_adv)
+  static void yf_omega(const Geo& g, hipStream_t st, const C* psi, C* Pbx, C* Pby, C* Pxx, C* Pxy, C* Pu,
+                       C* Pv, const C* twy);
   static void xf(const Geo& g, hipStream_t st, int want_omega, const C* Pu, const C* Pv, const C* Pbx,
                  const C* Pby, const C* Pxx, const C* Pxy, R* u, R* v, C* Rw, const C* twx);
   static void y_r2c_mask(const Geo& g, hipStream_t st, const C* Rw, C* out, const C* twy);

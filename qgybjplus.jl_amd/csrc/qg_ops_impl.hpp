@@ -73,6 +73,10 @@ void OpsV<R, V>::yb_adv(const Geo& g, hipStream_t st, int outs, const C* psi, co
 #define CALL(N) YL((k_yb_adv<R, N, V, OUT_Q | OUT_Z>), N, g.W, g, outs, psi, q, Pu, Pv, Pq, Pz, twy)
     QG_SWITCH(g.ny, CALL)
 #undef CALL
+  } else if (outs == (OUT_U | OUT_V | OUT_Q | OUT_Z)) {
+#define CALL(N) YL((k_yb_adv<R, N, V, OUT_U | OUT_V | OUT_Q | OUT_Z>), N, g.W, g, outs, psi, q, Pu, Pv, Pq, Pz, twy)
+    QG_SWITCH(g.ny, CALL)
+#undef CALL
   } else {
 #define CALL(N) YL((k_yb_adv<R, N, V, 0>), N, g.W, g, outs, psi, q, Pu, Pv, Pq, Pz, twy)
     QG_SWITCH(g.ny, CALL)
@@ -89,11 +93,17 @@ void OpsV<R, V>::x_c2r(const Geo& g, hipStream_t st, int nfields, const C* P0, c
 }
 
 template <typename R, int V>
-void OpsV<R, V>::xb_q(const Geo& g, hipStream_t st, const C* Pq, const R* u, const R* v, C* G1, C* G2,
-                  const C* twx) {
-#define CALL(N) XL((k_xb_q<R, N, V>), N, g, Pq, u, v, G1, G2, twx)
-  QG_SWITCH(g.nx, CALL)
+void OpsV<R, V>::xb_q(const Geo& g, hipStream_t st, const C* Pq, const C* Pu, const C* Pv, R* u, R* v, C* G1,
+                  C* G2, const C* twx) {
+  if (Pu) {
+#define CALL(N) XL((k_xb_q<R, N, V, 1>), N, g, Pq, Pu, Pv, u, v, G1, G2, twx)
+    QG_SWITCH(g.nx, CALL)
 #undef CALL
+  } else {
+#define CALL(N) XL((k_xb_q<R, N, V, 0>), N, g, Pq, Pu, Pv, u, v, G1, G2, twx)
+    QG_SWITCH(g.nx, CALL)
+#undef CALL
+  }
 }
 
 template <typename R, int V>
@@ -125,10 +135,17 @@ void OpsV<R, V>::yb2_B(const Geo& g, hipStream_t st, int mode, int flags, const 
 }
 
 template <typename R, int V>
-void OpsV<R, V>::yf_omega(const Geo& g, hipStream_t st, const C* psi, C* Pbx, C* Pby, C* Pxx, C* Pxy, const C* twy) {
-#define CALL(N) YL((k_yf_omega<R, N, V>), N, g.W, g, psi, Pbx, Pby, Pxx, Pxy, twy)
-  QG_SWITCH(g.ny, CALL)
+void OpsV<R, V>::yf_omega(const Geo& g, hipStream_t st, const C* psi, C* Pbx, C* Pby, C* Pxx, C* Pxy, C* Pu,
+                      C* Pv, const C* twy) {
+  if (Pu) {
+#define CALL(N) YL((k_yf_omega<R, N, V, 1>), N, g.W, g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, twy)
+    QG_SWITCH(g.ny, CALL)
 #undef CALL
+  } else {
+#define CALL(N) YL((k_yf_omega<R, N, V, 0>), N, g.W, g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, twy)
+    QG_SWITCH(g.ny, CALL)
+#undef CALL
+  }
 }
 
 template <typename R, int V>

@@ -404,12 +404,13 @@ struct Engine {
   }
 
   // u, v and the QG omega-equation w (compute_velocities!(compute_w=true)).
+  // One Y-pass over psi makes the half rows of u and v with the four of the
+  // omega right-hand side.
   void velocities_w(const C* psi) {
-    K("yb_uv", qg::P_YB_ADV, P * cS + Ph * 2 * cR, [&] {
-      O::yb_adv(h.tune[qg::P_YB_ADV], h.g, h.st, qg::OUT_U | qg::OUT_V, psi, nullptr, c(h.Pu), c(h.Pv), nullptr, nullptr, twy());
+    K("yf_omega", qg::P_YF_OMEGA, P * cS + Ph * 6 * cR, [&] {
+      O::yf_omega(h.tune[qg::P_YF_OMEGA], h.g, h.st, psi, c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), c(h.Pu), c(h.Pv),
+                  twy());
     });
-    K("yf_omega", qg::P_YF_OMEGA, P * cS + Ph * 4 * cR,
-      [&] { O::yf_omega(h.tune[qg::P_YF_OMEGA], h.g, h.st, psi, c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), twy()); });
     K("xf", qg::P_XF, Ph * 7 * cR + P * 2 * rR, [&] {
       O::xf(h.tune[qg::P_XF], h.g, h.st, 1, c(h.Pu), c(h.Pv), c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), r(h.u), r(h.v), c(h.Pq),
             twx());
@@ -432,21 +433,34 @@ struct Engine {
   // One RHS evaluation at (qS, psiS, AS, T1, u, v) fused with the ETD update
   // from the base state (qn, Bn): _compute_etdrk2_rhs! (src/timestep.jl:230-298)
   // and the stage/final loops (src/timestep.jl:362-404).
+  // uv: u and v are not yet made from psiS; the Y-pass over psiS and q_S
+  // makes their half rows too, and the q-advection X-pass turns them into u
+  // and v (written for the B pass).  Needs the advection of q.
   void rhs_update(int mode, const C* qS, const C* psiS, const C* AS, const C* qn, const C* Bn, C* qout,
-                  C* Bout) {
+                  C* Bout, bool uv = false) {
     const bool adv = !h.linear;
     const bool refr = !h.passive;
     const bool qdyn = !h.fixed_flow;
     int outs = 0;
     if (adv && qdyn) outs |= qg::OUT_Q;
     if (refr) outs |= qg::OUT_Z;
-    const int nouts = ((outs & qg::OUT_Q) ? 1 : 0) + ((outs & qg::OUT_Z) ? 1 : 0);
+    if (uv) {
+      if (!(adv && qdyn)) throw QgError(QG_ERR_ARGUMENT, "u, v from the RHS passes need the advection of q");
+      outs |= qg::OUT_U | qg::OUT_V;
+    }
+    const int nouts = ((outs & qg::OUT_Q) ? 1 : 0) + ((outs & qg::OUT_Z) ? 1 : 0) + (uv ? 2 : 0);
     if (outs)
-      K("yb_adv", qg::P_YB_ADV, P * (((outs & qg::OUT_Q) ? cS : 0) + ((outs & qg::OUT_Z) ? cS : 0)) + Ph * nouts * cR,
-        [&] { O::yb_adv(h.tune[qg::P_YB_ADV], h.g, h.st, outs, psiS, qS, nullptr, nullptr, c(h.Pq), c(h.Pz), twy()); });
+      K(uv ? "yb_adv_uv" : "yb_adv", qg::P_YB_ADV,
+        // reads q for OUT_Q and psi for the others
+        P * (((outs & qg::OUT_Q) ? cS : 0) + ((outs & ~qg::OUT_Q) ? cS : 0)) + Ph * nouts * cR, [&] {
+          O::yb_adv(h.tune[qg::P_YB_ADV], h.g, h.st, outs, psiS, qS, uv ? c(h.Pu) : nullptr, uv ? c(h.Pv) : nullptr,
+                    c(h.Pq), c(h.Pz), twy());
+        });
     if (adv && qdyn)
-      K("xb_q", qg::P_XB_Q, Ph * 3 * cR + P * 2 * rR,
-        [&] { O::xb_q(h.tune[qg::P_XB_Q], h.g, h.st, c(h.Pq), r(h.u), r(h.v), c(h.G1), c(h.G2), twx()); });
+      K(uv ? "xb_q_uv" : "xb_q", qg::P_XB_Q, Ph * (uv ? 3 : 1) * cR + Ph * 2 * cR + P * 2 * rR, [&] {
+        O::xb_q(h.tune[qg::P_XB_Q], h.g, h.st, c(h.Pq), uv ? c(h.Pu) : nullptr, uv ? c(h.Pv) : nullptr, r(h.u),
+                r(h.v), c(h.G1), c(h.G2), twx());
+      });
     int xbf = (adv ? qg::XB_ADV : 0) | (refr ? qg::XB_REFR : 0);
     if (xbf)
       K("xb_B", qg::P_XB_B, P * cR + (refr ? Ph * cR : 0) + (adv ? P * 2 * rR : 0) + P * cR * (1 + (adv ? 1 : 0)),
@@ -491,8 +505,8 @@ struct Engine {
     C* psiS = h.fixed_flow ? c(h.psi) : c(h.psis);
     diag(c(h.qs), c(h.Bs), psiS, true, true);
     wave_inv(c(h.Bs), c(h.As), nullptr);
-    if (need_uv && !h.fixed_flow) velocities(psiS);
-    rhs_update(qg::MODE_FINAL, c(h.qs), psiS, c(h.As), qn, Bn, q1, B1);
+    // the stage's u, v come out of the RHS passes (they are needed nowhere else)
+    rhs_update(qg::MODE_FINAL, c(h.qs), psiS, c(h.As), qn, Bn, q1, B1, need_uv && !h.fixed_flow);
     // finalize (src/timestep.jl:300-332); yb2_B left T1, T3 of B_{n+1}
     diag(q1, B1, c(h.psi), false, true);
     wave_inv(B1, c(h.A), c(h.C));
