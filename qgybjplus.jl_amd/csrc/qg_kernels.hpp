@@ -550,6 +550,130 @@ k_xb_B(Geo g, int flags, const cplx<R>* __restrict__ T1, const cplx<R>* __restri
     }
 }
 
+// The two advection X-passes of one RHS in one launch (q advected, B
+// advected and refracted): what k_xb_q and k_xb_B compute, from one read of
+// each input row.  UVP = 1: u and v come from their y-transformed half rows
+// (one packed inverse transform) and are not written out.
+template <typename R, int NX, int V, int UVP>
+__global__ void __launch_bounds__((Tile<R, NX, V>::THREADS))
+k_xb_all(Geo g, const cplx<R>* __restrict__ Pq, const cplx<R>* __restrict__ Pz, const cplx<R>* __restrict__ T1,
+         const cplx<R>* __restrict__ Pu, const cplx<R>* __restrict__ Pv, const R* __restrict__ u,
+         const R* __restrict__ vv, cplx<R>* __restrict__ G1, cplx<R>* __restrict__ G2, cplx<R>* __restrict__ G3,
+         cplx<R>* __restrict__ G4, const cplx<R>* __restrict__ tw) {
+  using TL = Tile<R, NX, V>;
+  using FI = Fft<R, NX, +1, TL::PL>;
+  using FF = Fft<R, NX, -1, TL::PL>;
+  __shared__ cplx<R> lds[TL::S * TL::LD];
+  const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
+  TwReg<R, TL::NN, TL::PL> twr;
+  twr.load(lane, tw);
+  const int row = blockIdx.x * TL::S + r;
+  const bool ok = row < g.nz * g.ny;
+  cplx<R>* sl = lds + r * TL::LD;
+  const size_t base = (size_t)row * NX, hb = (size_t)row * g.Wp;
+  cplx<R> b[FI::IT][FI::R0], v[FI::IT][FI::R0], zz[FI::IT][FI::R0];
+  R ur[FI::IT][FI::R0], vr[FI::IT][FI::R0];
+  // every input row is loaded before the first transform
+  if constexpr (UVP) {
+    cplx<R> w[FI::IT][FI::R0];
+    load_herm_pair<R, FI>(w, Pu, Pv, hb, NX, lane, ok);
+#pragma unroll
+    for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < FI::R0; ++t) {
+        int x = FI::elem(lane, it, t);
+        b[it][t] = (ok && FI::valid(lane, it)) ? T1[base + x] : czero<R>();
+      }
+    load_herm_row<R, FI>(v, Pq, hb, NX, lane, ok);
+    load_herm_row<R, FI>(zz, Pz, hb, NX, lane, ok);
+    FI::run(w, sl, lane, twr);
+#pragma unroll
+    for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < FI::R0; ++t) {
+        ur[it][t] = w[it][t].re;
+        vr[it][t] = w[it][t].im;
+      }
+  } else {
+#pragma unroll
+    for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < FI::R0; ++t) {
+        int x = FI::elem(lane, it, t);
+        const bool in = ok && FI::valid(lane, it);
+        b[it][t] = in ? T1[base + x] : czero<R>();
+        ur[it][t] = in ? u[base + x] : (R)0;
+        vr[it][t] = in ? vv[base + x] : (R)0;
+      }
+    load_herm_row<R, FI>(v, Pq, hb, NX, lane, ok);
+    load_herm_row<R, FI>(zz, Pz, hb, NX, lane, ok);
+  }
+  // q: u*q + i v*q, one forward transform, unpacked through LDS (as k_xb_q)
+  FI::run(v, sl, lane, twr);
+#pragma unroll
+  for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < FI::R0; ++t) {
+      const R qv = v[it][t].re;
+      v[it][t] = mk<R>(ur[it][t] * qv, vr[it][t] * qv);
+    }
+  FF::run(v, sl, lane, twr);
+  __syncthreads();  // the last stage's LDS reads are done
+#pragma unroll
+  for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < FI::R0; ++t)
+      if (FI::valid(lane, it)) sl[FI::pad(FI::elem(lane, it, t))] = v[it][t];
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < FI::R0; ++t) {
+      int x = FI::elem(lane, it, t);
+      if (ok && FI::valid(lane, it) && x < g.W) {
+        const cplx<R> a = v[it][t], m = conj(sl[FI::pad(x == 0 ? 0 : NX - x)]);
+        G1[hb + x] = mk<R>((R)0.5 * (a.re + m.re), (R)0.5 * (a.im + m.im));
+        G2[hb + x] = mk<R>((R)0.5 * (a.im - m.im), (R)-0.5 * (a.re - m.re));
+      }
+    }
+  // B: u*B, v*B, zeta*B (as k_xb_B); FI::run starts with a barrier, so the
+  // LDS reads above are done before it writes
+  FI::run(zz, sl, lane, twr);
+  FI::run(b, sl, lane, twr);
+  cplx<R> h[FI::IT][FI::R0];
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+#pragma unroll
+    for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < FI::R0; ++t) {
+        const R c = f == 2 ? zz[it][t].re : (f == 0 ? ur[it][t] : vr[it][t]);
+        v[it][t] = mk<R>(c * b[it][t].re, c * b[it][t].im);
+      }
+    FF::run(v, sl, lane, twr);
+#pragma unroll
+    for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < FI::R0; ++t) {
+        int x = FI::elem(lane, it, t);
+        if (f == 1) {
+          if (ok && FI::valid(lane, it)) G4[base + x] = v[it][t];
+        } else if (f == 0) {
+          h[it][t] = ik_mul((R)-g.kx[x], v[it][t]);
+        } else {
+          h[it][t] = h[it][t] + ik_mul((R)-0.5, v[it][t]);
+        }
+      }
+  }
+#pragma unroll
+  for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < FI::R0; ++t) {
+      int x = FI::elem(lane, it, t);
+      if (ok && FI::valid(lane, it)) G3[base + x] = h[it][t];
+    }
+}
+
 // ETD tables per column, layout [6][ny][nx]: Eq, hphi1q, hphi2q, EB, hphi1B, hphi2B.
 struct Etd {
   const double* tab;

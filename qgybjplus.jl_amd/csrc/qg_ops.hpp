@@ -32,6 +32,10 @@ struct OpsV {
                    C* G2, const C* twx);
   static void xb_B(const Geo& g, hipStream_t st, int flags, const C* T1, const C* Pz, const R* u, const R* v,
                    C* G3, C* G4, const C* twx);
+  // xb_q and xb_B (advection and refraction both on) in one pass; Pu != nullptr:
+  // u, v from Pu, Pv
+  static void xb_all(const Geo& g, hipStream_t st, const C* Pq, const C* Pz, const C* T1, const C* Pu,
+                     const C* Pv, const R* u, const R* v, C* G1, C* G2, C* G3, C* G4, const C* twx);
   static void yb2_q(const Geo& g, hipStream_t st, int mode, int flags, const C* G1, const C* G2, const C* qd,
                     const C* qn,
                     C* rhs0, C* qout, EtdView etd, double nu, double inv_dz2, const C* twy);
@@ -79,6 +83,7 @@ struct Ops {
   QG_OPS_FWD(x_c2r)
   QG_OPS_FWD(xb_q)
   QG_OPS_FWD(xb_B)
+  QG_OPS_FWD(xb_all)
   QG_OPS_FWD(yb2_q)
   QG_OPS_FWD(yb2_B)
   QG_OPS_FWD(yf_omega)

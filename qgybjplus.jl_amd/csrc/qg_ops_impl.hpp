@@ -115,6 +115,20 @@ void OpsV<R, V>::xb_B(const Geo& g, hipStream_t st, int flags, const C* T1, cons
 }
 
 template <typename R, int V>
+void OpsV<R, V>::xb_all(const Geo& g, hipStream_t st, const C* Pq, const C* Pz, const C* T1, const C* Pu,
+                        const C* Pv, const R* u, const R* v, C* G1, C* G2, C* G3, C* G4, const C* twx) {
+  if (Pu) {
+#define CALL(N) XL((k_xb_all<R, N, V, 1>), N, g, Pq, Pz, T1, Pu, Pv, u, v, G1, G2, G3, G4, twx)
+    QG_SWITCH(g.nx, CALL)
+#undef CALL
+  } else {
+#define CALL(N) XL((k_xb_all<R, N, V, 0>), N, g, Pq, Pz, T1, Pu, Pv, u, v, G1, G2, G3, G4, twx)
+    QG_SWITCH(g.nx, CALL)
+#undef CALL
+  }
+}
+
+template <typename R, int V>
 void OpsV<R, V>::yb2_q(const Geo& g, hipStream_t st, int mode, int flags, const C* G1, const C* G2, const C* qd,
                    const C* qn,
                    C* rhs0, C* qout, EtdView etd, double nu, double inv_dz2, const C* twy) {

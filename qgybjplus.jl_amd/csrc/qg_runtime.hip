@@ -456,13 +456,22 @@ struct Engine {
           O::yb_adv(h.tune[qg::P_YB_ADV], h.g, h.st, outs, psiS, qS, uv ? c(h.Pu) : nullptr, uv ? c(h.Pv) : nullptr,
                     c(h.Pq), c(h.Pz), twy());
         });
-    if (adv && qdyn)
+    if (adv && qdyn && refr) {
+      // both X-passes in one launch: Pq, Pz, T1 and (u, v | Pu, Pv) in;
+      // G1, G2 (half width), G3, G4 out
+      K(uv ? "xb_all_uv" : "xb_all", qg::P_XB_B,
+        Ph * 4 * cR + P * cR + (uv ? Ph * 2 * cR : P * 2 * rR) + P * 2 * cR, [&] {
+          O::xb_all(h.tune[qg::P_XB_B], h.g, h.st, c(h.Pq), c(h.Pz), c(h.T1), uv ? c(h.Pu) : nullptr,
+                    uv ? c(h.Pv) : nullptr, r(h.u), r(h.v), c(h.G1), c(h.G2), c(h.G3), c(h.G4), twx());
+        });
+    } else if (adv && qdyn) {
       K(uv ? "xb_q_uv" : "xb_q", qg::P_XB_Q, Ph * (uv ? 3 : 1) * cR + Ph * 2 * cR + P * 2 * rR, [&] {
         O::xb_q(h.tune[qg::P_XB_Q], h.g, h.st, c(h.Pq), uv ? c(h.Pu) : nullptr, uv ? c(h.Pv) : nullptr, r(h.u),
                 r(h.v), c(h.G1), c(h.G2), twx());
       });
+    }
     int xbf = (adv ? qg::XB_ADV : 0) | (refr ? qg::XB_REFR : 0);
-    if (xbf)
+    if (xbf && !(adv && qdyn && refr))
       K("xb_B", qg::P_XB_B, P * cR + (refr ? Ph * cR : 0) + (adv ? P * 2 * rR : 0) + P * cR * (1 + (adv ? 1 : 0)),
         [&] { O::xb_B(h.tune[qg::P_XB_B], h.g, h.st, xbf, c(h.T1), c(h.Pz), r(h.u), r(h.v), c(h.G3), c(h.G4), twx()); });
     const bool diff = h.nu_z != 0.0 && !h.inviscid && h.nz > 1;
