@@ -52,6 +52,28 @@ __device__ inline cplx<T> herm(cplx<T> a, cplx<T> am, T s) {
   return mk<T>((T)0.5 * s * (a.re + am.re), (T)0.5 * s * (a.im - am.im));
 }
 
+// Bounds-guarded loads, issued unconditionally: a lane outside the field
+// (a partial tile, a row past the last, an unused register slot) reads
+// element 0, and its value is never stored.  The compiler turns a load that
+// feeds only a per-lane select into a branch around the load, and the
+// arithmetic it sinks into that branch waits for the load, so a pass's
+// input loads would no longer all be in flight together.
+template <typename T>
+__device__ inline T ldu(const T* __restrict__ p, size_t i, bool in) {
+  return p[in ? i : 0];
+}
+
+// v where k holds, else +0, as a bit mask: a select on a loaded value would
+// again become a branch around the load.
+__device__ inline float keep_bits(float v, bool k) { return __uint_as_float(__float_as_uint(v) & (k ? ~0u : 0u)); }
+__device__ inline double keep_bits(double v, bool k) {
+  return __longlong_as_double(__double_as_longlong(v) & (k ? ~0ll : 0ll));
+}
+template <typename R>
+__device__ inline cplx<R> keep_if(cplx<R> v, bool k) {
+  return mk<R>(keep_bits(v.re, k), keep_bits(v.im, k));
+}
+
 // ===========================================================================
 // Wave feedback q_w  (compute_qw_complex!, nonlinear.jl:859-944)
 // ===========================================================================
@@ -76,15 +98,13 @@ k_ya_qw(Geo g, const cplx<R>* __restrict__ B, cplx<R>* __restrict__ T1,
   for (int it = 0; it < F::IT; ++it)
 #pragma unroll
     for (int t = 0; t < F::R0; ++t) {
-      v[it][t] = czero<R>();
-      w[it][t] = czero<R>();
       int y = F::elem(lane, it, t);
-      if (ok && F::valid(lane, it)) {
-        size_t a = ((size_t)z * NY + y) * g.nx + x;
-        cplx<double> b = g.mask[y * g.nx + x] ? scale(cvt<double>(B[a]), g.inv_n) : czero<double>();
-        v[it][t] = cvt<R>(b);
-        w[it][t] = cvt<R>(ik_mul(g.ky[y], b));
-      }
+      const bool in = ok && F::valid(lane, it);
+      const size_t a = ((size_t)z * NY + y) * g.nx + x;
+      const bool keep = in && g.mask[in ? y * g.nx + x : 0];
+      const cplx<double> b = scale(cvt<double>(keep_if(ldu(B, a, in), keep)), g.inv_n);
+      v[it][t] = cvt<R>(b);
+      w[it][t] = cvt<R>(ik_mul(g.ky[y], b));
     }
   F::run(v, sl, lane, twr);
   if (T3) F::run(w, sl, lane, twr);  // uniform: T3 == nullptr computes T1 only
@@ -128,8 +148,8 @@ k_xa_qw(Geo g, const cplx<R>* __restrict__ T1, const cplx<R>* __restrict__ T3,
     for (int t = 0; t < FI::R0; ++t) {
       int x = FI::elem(lane, it, t);
       const bool in = ok && FI::valid(lane, it);
-      cplx<R> a = in ? T1[base + x] : czero<R>();
-      y3[it][t] = in ? T3[base + x] : czero<R>();
+      cplx<R> a = ldu(T1, base + x, in);
+      y3[it][t] = ldu(T3, base + x, in);
       v[it][t] = a;
       w[it][t] = ik_mul((R)g.kx[x], a);
     }
@@ -192,8 +212,8 @@ k_ya2_qeff(Geo g, const cplx<R>* __restrict__ Qa, const cplx<R>* __restrict__ Qb
       int y = F::elem(lane, it, t);
       bool in = ok && F::valid(lane, it);
       size_t a = ((size_t)z * NY + y) * g.Wp + x;
-      v[it][t] = in ? Qa[a] : czero<R>();
-      w[it][t] = in ? Qb[a] : czero<R>();
+      v[it][t] = ldu(Qa, a, in);
+      w[it][t] = ldu(Qb, a, in);
     }
   F::run(v, sl, lane, twr);
   F::run(w, sl, lane, twr);
@@ -259,8 +279,8 @@ k_yb_adv(Geo g, int outs_rt, const cplx<R>* __restrict__ psi, const cplx<R>* __r
     for (int t = 0; t < F::R0; ++t) {
       int y = F::elem(lane, it, t), my = mirror(y, NY);
       bool in = ok && F::valid(lane, it);
-      p[it][t] = in ? psi[((size_t)z * NY + y) * g.nx + x] : czero<R>();
-      pm[it][t] = in ? psi[((size_t)z * NY + my) * g.nx + mx] : czero<R>();
+      p[it][t] = ldu(psi, ((size_t)z * NY + y) * g.nx + x, in);
+      pm[it][t] = ldu(psi, ((size_t)z * NY + my) * g.nx + mx, in);
     }
   for (int o = 0; o < 4; ++o) {
     const int bit = 1 << o;
@@ -280,9 +300,9 @@ k_yb_adv(Geo g, int outs_rt, const cplx<R>* __restrict__ psi, const cplx<R>* __r
           am = ik_mul((R)kxm, pm[it][t]);
         } else if (o == 2) {
           bool in = ok && F::valid(lane, it);
-          bool keep = in && g.mask[y * g.nx + x];
-          a = keep ? q[((size_t)z * NY + y) * g.nx + x] : czero<R>();
-          am = keep ? q[((size_t)z * NY + my) * g.nx + mx] : czero<R>();
+          bool keep = in && g.mask[in ? y * g.nx + x : 0];
+          a = keep_if(ldu(q, ((size_t)z * NY + y) * g.nx + x, in), keep);
+          am = keep_if(ldu(q, ((size_t)z * NY + my) * g.nx + mx, in), keep);
         } else {
           bool in = ok && F::valid(lane, it);
           bool keep = in && g.mask[y * g.nx + x];
@@ -314,10 +334,10 @@ __device__ inline void load_herm_row(cplx<R> (&v)[F::IT][F::R0], const cplx<R>* 
   for (int it = 0; it < F::IT; ++it)
 #pragma unroll
     for (int t = 0; t < F::R0; ++t) {
-      int x = F::elem(lane, it, t);
-      cplx<R> a = czero<R>();
-      if (ok && F::valid(lane, it)) a = x < W ? src[hb + x] : conj(src[hb + (nx - x)]);
-      v[it][t] = a;
+      const int x = F::elem(lane, it, t);
+      const bool in = ok && F::valid(lane, it), lo = x < W;
+      const cplx<R> a = src[in ? hb + (lo ? x : nx - x) : 0];
+      v[it][t] = lo ? a : conj(a);
     }
 }
 
@@ -335,21 +355,13 @@ __device__ inline void load_herm_pair(cplx<R> (&v)[F::IT][F::R0], const cplx<R>*
   for (int it = 0; it < F::IT; ++it)
 #pragma unroll
     for (int t = 0; t < F::R0; ++t) {
-      int x = F::elem(lane, it, t);
-      cplx<R> a = czero<R>(), b = czero<R>();
-      if (ok && F::valid(lane, it)) {
-        if (x < W) {
-          a = A[hb + x];
-          b = B[hb + x];
-          if (x == 0 || 2 * x == nx) {
-            a.im = (R)0;
-            b.im = (R)0;
-          }
-        } else {
-          a = conj(A[hb + (nx - x)]);
-          b = conj(B[hb + (nx - x)]);
-        }
-      }
+      const int x = F::elem(lane, it, t);
+      const bool in = ok && F::valid(lane, it), lo = x < W, self = x == 0 || 2 * x == nx;
+      const size_t i = in ? hb + (lo ? x : nx - x) : 0;
+      cplx<R> a = A[i], b = B[i];
+      // mirrored points take the conjugate; the self-conjugate ones their real part
+      a.im = lo ? (self ? (R)0 : a.im) : -a.im;
+      b.im = lo ? (self ? (R)0 : b.im) : -b.im;
       v[it][t] = mk<R>(a.re - b.im, a.im + b.re);
     }
 }
@@ -445,8 +457,8 @@ k_xb_q(Geo g, const cplx<R>* __restrict__ Pq, const cplx<R>* __restrict__ Pu, co
       for (int t = 0; t < FI::R0; ++t) {
         int x = FI::elem(lane, it, t);
         const bool in = ok && FI::valid(lane, it);
-        const R cu = in ? u[(size_t)row * NX + x] : (R)0;
-        const R cv = in ? vv[(size_t)row * NX + x] : (R)0;
+        const R cu = ldu(u, (size_t)row * NX + x, in);
+        const R cv = ldu(vv, (size_t)row * NX + x, in);
         v[it][t] = mk<R>(cu * qv[it][t], cv * qv[it][t]);
       }
   }
@@ -502,9 +514,9 @@ k_xb_B(Geo g, int flags, const cplx<R>* __restrict__ T1, const cplx<R>* __restri
     for (int t = 0; t < FI::R0; ++t) {
       int x = FI::elem(lane, it, t);
       const bool in = ok && FI::valid(lane, it);
-      b[it][t] = in ? T1[base + x] : czero<R>();
-      ur[it][t] = (in && adv) ? u[base + x] : (R)0;
-      vr[it][t] = (in && adv) ? vv[base + x] : (R)0;
+      b[it][t] = ldu(T1, base + x, in);
+      ur[it][t] = ldu(u, base + x, in && adv);
+      vr[it][t] = ldu(vv, base + x, in && adv);
       h[it][t] = czero<R>();
     }
   if (flags & XB_REFR) load_herm_row<R, FI>(v, Pz, (size_t)row * g.Wp, NX, lane, ok);
@@ -582,7 +594,7 @@ k_xb_all(Geo g, const cplx<R>* __restrict__ Pq, const cplx<R>* __restrict__ Pz, 
 #pragma unroll
       for (int t = 0; t < FI::R0; ++t) {
         int x = FI::elem(lane, it, t);
-        b[it][t] = (ok && FI::valid(lane, it)) ? T1[base + x] : czero<R>();
+        b[it][t] = ldu(T1, base + x, ok && FI::valid(lane, it));
       }
     load_herm_row<R, FI>(v, Pq, hb, NX, lane, ok);
     load_herm_row<R, FI>(zz, Pz, hb, NX, lane, ok);
@@ -601,9 +613,9 @@ k_xb_all(Geo g, const cplx<R>* __restrict__ Pq, const cplx<R>* __restrict__ Pz, 
       for (int t = 0; t < FI::R0; ++t) {
         int x = FI::elem(lane, it, t);
         const bool in = ok && FI::valid(lane, it);
-        b[it][t] = in ? T1[base + x] : czero<R>();
-        ur[it][t] = in ? u[base + x] : (R)0;
-        vr[it][t] = in ? vv[base + x] : (R)0;
+        b[it][t] = ldu(T1, base + x, in);
+        ur[it][t] = ldu(u, base + x, in);
+        vr[it][t] = ldu(vv, base + x, in);
       }
     load_herm_row<R, FI>(v, Pq, hb, NX, lane, ok);
     load_herm_row<R, FI>(zz, Pz, hb, NX, lane, ok);
@@ -732,8 +744,8 @@ k_yb2_q(Geo g, int mode, int flags, const cplx<R>* __restrict__ G1, const cplx<R
         int y = F::elem(lane, it, t);
         bool in = ok && F::valid(lane, it);
         size_t ad = ((size_t)z * NY + y) * g.Wp + x;
-        a[it][t] = in ? G1[ad] : czero<R>();
-        b[it][t] = in ? G2[ad] : czero<R>();
+        a[it][t] = ldu(G1, ad, in);
+        b[it][t] = ldu(G2, ad, in);
       }
     F::run(a, sl, lane, twr);
     F::run(b, sl, lane, twr);
@@ -824,8 +836,8 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
       int y = F::elem(lane, it, t);
       bool in = ok && F::valid(lane, it);
       const size_t ad = ((size_t)z * NY + y) * g.nx + x;
-      v[it][t] = (in && has3) ? G3[ad] : czero<R>();
-      w[it][t] = (in && has4) ? G4[ad] : czero<R>();
+      v[it][t] = ldu(G3, ad, in && has3);
+      w[it][t] = ldu(G4, ad, in && has4);
       acc[it][t] = czero<double>();
     }
   if (has3) {
@@ -962,22 +974,21 @@ k_yf_omega(Geo g, const cplx<R>* __restrict__ psi, cplx<R>* __restrict__ Pbx, cp
       const int y = F::elem(lane, it, t), my = mirror(y, NY);
       const bool in = ok && F::valid(lane, it);
       // the four factors are formed at the transform precision R
-      const cplx<R> p = in ? psi[((size_t)z * NY + y) * g.nx + x] : czero<R>();
-      const cplx<R> pm = in ? psi[((size_t)z * NY + my) * g.nx + mx] : czero<R>();
+      const cplx<R> p = ldu(psi, ((size_t)z * NY + y) * g.nx + x, in);
+      const cplx<R> pm = ldu(psi, ((size_t)z * NY + my) * g.nx + mx, in);
       if constexpr (UV) {
         v[4][it][t] = herm(ik_mul((R)-g.ky[y], p), ik_mul((R)-g.ky[my], pm), (R)g.inv_n);
         v[5][it][t] = herm(ik_mul((R)kxv, p), ik_mul((R)kxm, pm), (R)g.inv_n);
       }
-      const cplx<R> up = in ? psi[((size_t)zu * NY + y) * g.nx + x] : czero<R>();
-      const cplx<R> upm = in ? psi[((size_t)zu * NY + my) * g.nx + mx] : czero<R>();
+      const cplx<R> up = ldu(psi, ((size_t)zu * NY + y) * g.nx + x, in);
+      const cplx<R> upm = ldu(psi, ((size_t)zu * NY + my) * g.nx + mx, in);
       const R rdz = (R)dz;
-      cplx<R> pz = czero<R>(), pzm = czero<R>(), pa = p, pam = pm;
-      if (!top) {
-        pz = mk<R>((up.re - p.re) / rdz, (up.im - p.im) / rdz);
-        pzm = mk<R>((upm.re - pm.re) / rdz, (upm.im - pm.im) / rdz);
-        pa = scale(up + p, (R)0.5);
-        pam = scale(upm + pm, (R)0.5);
-      }
+      // at the top level zu = z: the difference is exactly 0 and the average
+      // psi itself, with no branch for the loads of level zu to sink into
+      const cplx<R> pz = mk<R>((up.re - p.re) / rdz, (up.im - p.im) / rdz);
+      const cplx<R> pzm = mk<R>((upm.re - pm.re) / rdz, (upm.im - pm.im) / rdz);
+      const cplx<R> pa = scale(up + p, (R)0.5);
+      const cplx<R> pam = scale(upm + pm, (R)0.5);
       const double ky = g.ky[y], kym = g.ky[my];
       const double kh2 = kxv * kxv + ky * ky, kh2m = kxm * kxm + kym * kym;
       const R rinv = (R)g.inv_n;
@@ -1080,7 +1091,7 @@ k_y_r2c_mask(Geo g, const cplx<R>* __restrict__ Rw, cplx<R>* __restrict__ out,
 #pragma unroll
     for (int t = 0; t < F::R0; ++t) {
       int y = F::elem(lane, it, t);
-      v[it][t] = (ok && F::valid(lane, it)) ? Rw[((size_t)z * NY + y) * g.Wp + x] : czero<R>();
+      v[it][t] = ldu(Rw, ((size_t)z * NY + y) * g.Wp + x, ok && F::valid(lane, it));
     }
   F::run(v, sl, lane, twr);
   const int mx = mirror(x, g.nx);
@@ -1331,7 +1342,7 @@ k_x_check(Geo g, const cplx<R>* __restrict__ in, unsigned long long* __restrict_
 #pragma unroll
     for (int t = 0; t < F::R0; ++t) {
       int x = F::elem(lane, it, t);
-      v[it][t] = (ok && F::valid(lane, it)) ? in[(size_t)row * NX + x] : czero<R>();
+      v[it][t] = ldu(in, (size_t)row * NX + x, ok && F::valid(lane, it));
     }
   F::run(v, sl, lane, twr);
   double m = 0.0;
