@@ -212,8 +212,10 @@ k_ya2_qeff(Geo g, const cplx<R>* __restrict__ Qa, const cplx<R>* __restrict__ Qb
       int y = F::elem(lane, it, t);
       bool in = ok && F::valid(lane, it);
       size_t a = ((size_t)z * NY + y) * g.Wp + x;
-      v[it][t] = ldu(Qa, a, in);
-      w[it][t] = ldu(Qb, a, in);
+      // guarded by `in` (no branch forms here, and the clamped form of the
+      // other passes costs this one a wave per SIMD in registers)
+      v[it][t] = in ? Qa[a] : czero<R>();
+      w[it][t] = in ? Qb[a] : czero<R>();
     }
   F::run(v, sl, lane, twr);
   F::run(w, sl, lane, twr);
@@ -279,8 +281,9 @@ k_yb_adv(Geo g, int outs_rt, const cplx<R>* __restrict__ psi, const cplx<R>* __r
     for (int t = 0; t < F::R0; ++t) {
       int y = F::elem(lane, it, t), my = mirror(y, NY);
       bool in = ok && F::valid(lane, it);
-      p[it][t] = ldu(psi, ((size_t)z * NY + y) * g.nx + x, in);
-      pm[it][t] = ldu(psi, ((size_t)z * NY + my) * g.nx + mx, in);
+      // guarded by `in`: the clamped form costs this pass a wave per SIMD
+      p[it][t] = in ? psi[((size_t)z * NY + y) * g.nx + x] : czero<R>();
+      pm[it][t] = in ? psi[((size_t)z * NY + my) * g.nx + mx] : czero<R>();
     }
   for (int o = 0; o < 4; ++o) {
     const int bit = 1 << o;
@@ -299,10 +302,13 @@ k_yb_adv(Geo g, int outs_rt, const cplx<R>* __restrict__ psi, const cplx<R>* __r
           a = ik_mul((R)kxv, p[it][t]);
           am = ik_mul((R)kxm, pm[it][t]);
         } else if (o == 2) {
+          // loaded here, under the mask, rather than with psi: as unconditional
+          // loads they would be hoisted to the kernel's start and cost a wave
+          // per SIMD in registers
           bool in = ok && F::valid(lane, it);
-          bool keep = in && g.mask[in ? y * g.nx + x : 0];
-          a = keep_if(ldu(q, ((size_t)z * NY + y) * g.nx + x, in), keep);
-          am = keep_if(ldu(q, ((size_t)z * NY + my) * g.nx + mx, in), keep);
+          bool keep = in && g.mask[y * g.nx + x];
+          a = keep ? q[((size_t)z * NY + y) * g.nx + x] : czero<R>();
+          am = keep ? q[((size_t)z * NY + my) * g.nx + mx] : czero<R>();
         } else {
           bool in = ok && F::valid(lane, it);
           bool keep = in && g.mask[y * g.nx + x];

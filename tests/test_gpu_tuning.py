@@ -53,9 +53,12 @@ COLUMN = ("thomas_psi", "thomas_A", "thomas_AC", "omega")
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_fft_tile_variants_are_bitwise_identical(qg, precision):
-    print(f"case tile variants {precision}", flush=True)
-    m = _model(qg, (64, 32, 6), precision)
+@pytest.mark.parametrize("size", [(64, 32, 6), (16, 256, 4)])
+def test_fft_tile_variants_are_bitwise_identical(qg, precision, size):
+    """(16, 256, 4): 256-point columns, where the half tiles of the B update
+    copy their base field into LDS while the transforms run."""
+    print(f"case tile variants {precision} {size}", flush=True)
+    m = _model(qg, size, precision)
     ref = _run(qg, m, {})
     for v in (0, 1, 2):
         out = _run(qg, m, {p: v for p in FFT})
