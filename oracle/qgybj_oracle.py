@@ -54,10 +54,17 @@ Pins (what checks each function; see ``tests/test_oracle.py``):
   L+ relation LA = B + kh^2 A / 4 (reference KAT ``test/test_simulation_lifecycle.jl:108-114``),
   which reaches it from B and A instead of C.
 
-Parity unpinned: the omega-equation ``w`` *values* of the reference (no
-reference test checks them; the restated gtsv is pinned against a dense solve
-only), and true reference trajectories of the full nonlinear step (the Julia
-reference cannot run in this environment).
+* ``omega_eqn_rhs``: the closed form 2 J(psi_z, lap psi_bar) = a_k cos x sin y
+  of psi = Z(z) sin x + cos(y)/2 under the reference's forward difference and
+  two-level average (a_k = (Z_{k+1} - Z_k)/dz, zero at the top).
+* ``solve_omega``: the discrete sine modes sin(m pi k/(nz-1)) of the interior
+  second difference, whose eigenvalues 2 cos(theta) - 2 give w in closed
+  form for constant N2, and a dense solve for varying N2.
+
+Parity unpinned: no reference test checks the omega ``w`` values themselves
+(the closed forms above pin the equation as ``src/operators.jl:365-493``
+writes it), and true reference trajectories of the full nonlinear step (the
+Julia reference cannot run in this environment).
 """
 
 from __future__ import annotations
@@ -582,9 +589,16 @@ def omega_eqn_rhs(psi, grid: Grid, L):
 
 def compute_vertical_velocity(psi, grid: Grid, f: float, N2: np.ndarray, L):
     """``_compute_vertical_velocity_direct!``, ``src/operators.jl:365-493``."""
+    return fft_backward(solve_omega(omega_eqn_rhs(psi, grid, L), grid, f, N2)).real
+
+
+def solve_omega(rhsk, grid: Grid, f: float, N2: np.ndarray):
+    """The interior solve of ``_compute_vertical_velocity_direct!``
+    (``src/operators.jl:365-493``): per horizontal mode with kh2 > 0,
+    (f^2/N2_k)(w_{k+1} - 2 w_k + w_{k-1})/dz^2 - kh2 w_k = (f/N2_k) rhs_k on
+    the levels 1 .. nz-2, w = 0 on the first and last level; returns w-hat."""
     nz = grid.nz
-    rhsk = omega_eqn_rhs(psi, grid, L)
-    wk = np.zeros_like(psi)
+    wk = np.zeros_like(rhsk)
     kh2 = grid.kh2
     pos = kh2 > 0
     n = nz - 2
@@ -609,7 +623,7 @@ def compute_vertical_velocity(psi, grid: Grid, f: float, N2: np.ndarray, L):
         sr = gtsv(dl, d, du, rhs.real)
         si = gtsv(dl, d, du, rhs.imag)
         wk[1: nz - 1, ii, jj] = sr + 1j * si
-    return fft_backward(wk).real
+    return wk
 
 
 def compute_velocities(psi, grid: Grid, f: float, N2: np.ndarray, L, compute_w: bool):

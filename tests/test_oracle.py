@@ -200,6 +200,59 @@ def test_refraction_single_mode_closed_form():
     assert np.allclose(r, np.broadcast_to(-4 * np.sin(2 * X) * np.exp(3j * Y), r.shape), atol=1e-13)
 
 
+def test_omega_rhs_closed_form():
+    """psi = Z1(z) sin x + Z2(z) cos(2y)/2.  With a the forward difference of
+    Z over dz (zero at the top level) and b its two-level average (Z itself
+    at the top): psi_z = a1 sin x + a2 cos(2y)/2 and lap psi_bar =
+    -b1 sin x - 2 b2 cos 2y, so 2 J(psi_z, lap psi_bar) =
+    2 (4 a1 b2 - a2 b1) cos x sin 2y.  A flipped or swapped Jacobian term, or
+    the average replaced by the level's own psi, fails this (the two
+    horizontal scales differ, so J(psi_z, lap psi_z) does not vanish)."""
+    n, nz = 16, 6
+    g = O.make_grid((n, n, nz), extent=(2 * math.pi, 2 * math.pi, 1.0))
+    L = O.dealias_mask(n, n)
+    X, Y = np.meshgrid(g.x, g.y, indexing="ij")
+    Z1 = np.cos(math.pi * (g.z + 1.0))
+    Z2 = np.sin(2 * math.pi * g.z) + 0.3
+    psi = O.fft_forward(Z1[:, None, None] * np.sin(X)[None] + 0.5 * Z2[:, None, None] * np.cos(2 * Y)[None])
+    got = O.fft_backward(O.omega_eqn_rhs(psi, g, L)).real
+
+    def diff_avg(Z):
+        a, b = np.zeros(nz), Z.copy()
+        a[: nz - 1] = (Z[1:] - Z[:-1]) / g.dz_solver
+        b[: nz - 1] = 0.5 * (Z[1:] + Z[:-1])
+        return a, b
+
+    (a1, b1), (a2, b2) = diff_avg(Z1), diff_avg(Z2)
+    want = 2 * (4 * a1 * b2 - a2 * b1)[:, None, None] * (np.cos(X) * np.sin(2 * Y))[None]
+    assert np.abs(a1 * a2).max() > 0.1  # the average matters
+    assert np.allclose(got, want, atol=1e-11)
+
+
+@pytest.mark.parametrize("m", [1, 2, 3])
+def test_solve_omega_discrete_sine_modes(m):
+    """Constant N2, rhs = R sin(theta k) cos(x) on the interior levels with
+    theta = m pi/(nz-1): the second difference maps sin(theta k) to
+    (2 cos theta - 2) sin(theta k) with w = 0 at both ends, so
+    w_k = (f/N2) R sin(theta k) cos(x) / ((f^2/N2)(2 cos theta - 2)/dz^2 - kh2),
+    kh2 = 1 for cos(x)."""
+    n, nz = 8, 10
+    g = O.make_grid((n, n, nz), extent=(2 * math.pi, 2 * math.pi, 900.0))
+    X, _ = np.meshgrid(g.x, g.y, indexing="ij")
+    f, N2c, R = 1e-4, 2.5e-5, 3e-11
+    theta = m * math.pi / (nz - 1)
+    k = np.arange(nz)
+    prof = R * np.sin(theta * k)
+    prof[0] = prof[-1] = 0.0
+    rhsk = O.fft_forward(prof[:, None, None] * np.cos(X)[None])
+    w = O.fft_backward(O.solve_omega(rhsk, g, f, np.full(nz, N2c))).real
+    dz = g.dz_solver
+    lam = (f * f / N2c) * (2 * math.cos(theta) - 2) / (dz * dz) - 1.0
+    want = (f / N2c) * prof[:, None, None] * np.cos(X)[None] / lam
+    want[0] = want[-1] = 0.0
+    assert np.allclose(w, want, rtol=1e-10, atol=1e-12 * np.abs(want).max())
+
+
 def test_omega_rhs_single_mode_and_w_dense_solve():
     """psi = cos(z-mode) * sin(kx): J(psi_z, lap psi_bar) vanishes for one
     horizontal mode, so the omega RHS is zero; a two-mode psi gives a
