@@ -865,6 +865,26 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
   }
   const size_t plane = (size_t)g.nx * NY;
   const double kxv = ok ? g.kx[x] : 0.0;
+  // FP32: the update's field inputs (A, and B_n or the kept bracket) are
+  // loaded together, from clamped indices, before the update; loaded under
+  // each point's mask they would be one round trip per point.  FP64 keeps
+  // the late loads: grouped, they cost it a wave per SIMD in registers.
+  constexpr bool GROUP = sizeof(R) == 4;
+  const cplx<R>* __restrict__ base = mode == MODE_STAGE ? Bn : rhs0;
+  cplx<R> av[F::IT][F::R0], bv[F::IT][F::R0];
+  if constexpr (GROUP) {
+    const bool disp = (flags & YB_DISP) != 0;
+#pragma unroll
+    for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < F::R0; ++t) {
+        const int y = F::elem(lane, it, t);
+        const bool in = ok && F::valid(lane, it);
+        const size_t ad = ((size_t)z * NY + y) * g.nx + x;
+        av[it][t] = ldu(A, ad, in && disp);
+        bv[it][t] = ldu(base, ad, in);
+      }
+  }
 #pragma unroll
   for (int it = 0; it < F::IT; ++it)
 #pragma unroll
@@ -881,7 +901,7 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
         if (flags & YB_DISP) {
           const double kh2 = kxv * kxv + g.ky[y] * g.ky[y];
           const double c = alpha * kh2;
-          cplx<double> Av = cvt<double>(A[ad]);
+          cplx<double> Av = cvt<double>(GROUP ? av[it][t] : A[ad]);
           t1 = mk<double>(-(c * Av.im), c * Av.re);
         }
         rhs = acc[it][t] + t1;
@@ -893,11 +913,11 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
         const double h2 = etd.at(5, col);
         if (mode == MODE_STAGE) {
           const double E = etd.at(3, col), h1 = etd.at(4, col);
-          const cplx<double> b0 = cvt<double>(Bn[ad]);
+          const cplx<double> b0 = cvt<double>(GROUP ? bv[it][t] : base[ad]);
           out = scale(b0, E) + scale(rhs, h1);
           rhs0[ad] = cvt<R>(scale(b0, E) + scale(rhs, h1 - h2));
         } else {
-          out = cvt<double>(rhs0[ad]) + scale(rhs, h2);
+          out = cvt<double>(GROUP ? bv[it][t] : base[ad]) + scale(rhs, h2);
         }
       } else if (mode == MODE_STAGE) {
         rhs0[ad] = czero<R>();
