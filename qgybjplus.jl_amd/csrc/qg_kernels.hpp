@@ -865,11 +865,12 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
   }
   const size_t plane = (size_t)g.nx * NY;
   const double kxv = ok ? g.kx[x] : 0.0;
-  // FP32: the update's field inputs (A, and B_n or the kept bracket) are
-  // loaded together, from clamped indices, before the update; loaded under
-  // each point's mask they would be one round trip per point.  FP64 keeps
-  // the late loads: grouped, they cost it a wave per SIMD in registers.
-  constexpr bool GROUP = sizeof(R) == 4;
+  // FP32 up to 256-point columns: the update's field inputs (A, and B_n or
+  // the kept bracket) are loaded together, from clamped indices, before the
+  // update; loaded under each point's mask they would be one round trip per
+  // point.  FP64 and longer columns keep the late loads: grouped, they cost
+  // those a wave per SIMD in registers.
+  constexpr bool GROUP = sizeof(R) == 4 && NY <= 256;
   const cplx<R>* __restrict__ base = mode == MODE_STAGE ? Bn : rhs0;
   cplx<R> av[F::IT][F::R0], bv[F::IT][F::R0];
   if constexpr (GROUP) {
