@@ -104,6 +104,26 @@ def build_model(qg, precision):
     return grid, model
 
 
+def workload_inputs(grid):
+    """The workload's initial fields: physical psi (the seeded k^-3 field) and
+    the physical surface wave 0.1 m/s exp(-(depth/50 m)^2) that
+    SurfaceWave(0.1, 50.0) writes."""
+    from qgybjplus_amd import synthetic
+    psi = synthetic.k3_streamfunction(grid, seed=2026, urms=0.1, kpeak=6.0)
+    nx, ny, nz = grid.size
+    B = np.empty((nz, nx, ny), dtype=np.complex128)
+    for k in range(nz):
+        depth = max(0.0, -grid.z[k] - grid.dz / 2)
+        B[k] = 0.1 * math.exp(-(depth ** 2) / 50.0 ** 2)
+    return psi, B
+
+
+def set_workload(qg, model, psi):
+    """set!(model, psi = psi, pv_method = :qg) and the surface wave."""
+    qg.set_(model, psi=qg.FieldArray(psi), pv_method="qg")
+    qg.set_(model, waves=qg.SurfaceWave(0.1, 50.0))
+
+
 def timed_run(qg, model, steps, warmup):
     """The timed region: `steps` steps as the library runs them (steady-state
     pairs replay a captured hipGraph), with one HIP event per step interval."""
@@ -362,18 +382,12 @@ def main():
 
     grid, m32 = build_model(qg, "mixed")
     m64 = build_model(qg, "fp64")[1] if args.modes == "both" else None
-    psi = synthetic.k3_streamfunction(grid, seed=2026, urms=0.1, kpeak=6.0)
-    nz = SIZE[2]
-    B = np.empty((nz, SIZE[0], SIZE[1]), dtype=np.complex128)
-    for k in range(nz):
-        depth = max(0.0, -grid.z[k] - grid.dz / 2)
-        B[k] = 0.1 * math.exp(-(depth ** 2) / 50.0 ** 2)
+    psi, B = workload_inputs(grid)
     tuning = {}
     for name, m in (("mixed", m32), ("fp64", m64)):
         if m is None:
             continue
-        qg.set_(m, psi=qg.FieldArray(psi), pv_method="qg")
-        qg.set_(m, waves=qg.SurfaceWave(0.1, 50.0))
+        set_workload(qg, m, psi)
         if args.autotune:
             t0 = time.perf_counter()
             tuning[name] = {"choice": m.autotune(DT, steps=6), "seconds": time.perf_counter() - t0}

@@ -1283,8 +1283,6 @@ int qg_set_member_stratification(qg_handle* h, int member, const double* N2, con
       if (!(std::isfinite(N2[k]) && N2[k] > 0 && std::isfinite(a_ell[k])))
         throw QgError(QG_ERR_ARGUMENT, "N² must be finite and positive at every level");
     const size_t o = (size_t)member * nzc;
-    std::copy(N2, N2 + nzc, H.N2.begin() + o);
-    std::copy(a_ell, a_ell + nzc, H.a.begin() + o);
     std::vector<double> cz(nzc), fN2(nzc), al(kpad, 0.0), cu(kpad, 0.0);
     const double f2 = H.f * H.f;
     for (int k = 0; k < nzc; ++k) {
@@ -1293,33 +1291,67 @@ int qg_set_member_stratification(qg_handle* h, int member, const double* N2, con
     }
     for (int k = 1; k < nzc; ++k) al[k] = a_ell[k - 1];
     for (int k = 1; k <= nzc - 3; ++k) cu[k] = cz[k];
-    HIPCHECK(hipMemcpyAsync(H.d_a + o, a_ell, nzc * sizeof(double), hipMemcpyHostToDevice, H.st));
-    HIPCHECK(hipMemcpyAsync(H.d_cz + o, cz.data(), nzc * sizeof(double), hipMemcpyHostToDevice, H.st));
-    HIPCHECK(hipMemcpyAsync(H.d_fN2 + o, fN2.data(), nzc * sizeof(double), hipMemcpyHostToDevice, H.st));
-    HIPCHECK(hipMemcpyAsync(H.d_al + (size_t)member * kpad, al.data(), kpad * sizeof(double),
-                            hipMemcpyHostToDevice, H.st));
-    HIPCHECK(hipMemcpyAsync(H.d_cu + (size_t)member * kpad, cu.data(), kpad * sizeof(double),
-                            hipMemcpyHostToDevice, H.st));
     const MemberView mv(H, member);
-    const size_t quarter = qg::thomas_table_size(mv.g) / nzc;
+    const size_t tq = qg::thomas_table_size(mv.g);  // table entries of one member
+    const size_t quarter = tq / nzc;
+    // The eliminations of the new profile are recorded into scratch tables
+    // first: the member keeps its profile and tables (host and device alike)
+    // unless every one of them succeeds.
+    struct Scratch {
+      hipStream_t st;
+      void* p = nullptr;
+      ~Scratch() {
+        if (p) {
+          (void)hipStreamSynchronize(st);
+          (void)hipFree(p);
+        }
+      }
+    } sc{H.st};
+    const size_t ntab = 6 * tq + 2 * (size_t)nzc;
+    HIPCHECK(hipMalloc(&sc.p, ntab * sizeof(double) + quarter + 256));
+    double* s_a = static_cast<double*>(sc.p);
+    double* s_cz = s_a + nzc;
+    double* s_tab = s_cz + nzc;  // th_inv[0], th_cp[0], th_inv[1], th_cp[1], om_M, om_D
+    unsigned char* s_piv = reinterpret_cast<unsigned char*>(s_a + ntab);
+    HIPCHECK(hipMemcpyAsync(s_a, a_ell, nzc * sizeof(double), hipMemcpyHostToDevice, H.st));
+    HIPCHECK(hipMemcpyAsync(s_cz, cz.data(), nzc * sizeof(double), hipMemcpyHostToDevice, H.st));
     HIPCHECK(hipMemsetAsync(H.d_ferr, 0, 8 * sizeof(int), H.st));
-    qg::launch_thomas_factor(mv.g, H.st, H.d_a + o, 1, H.th_inv[0] + mv.toff, H.th_cp[0] + mv.toff, H.d_ferr);
-    qg::launch_thomas_factor(mv.g, H.st, H.d_a + o, 4, H.th_inv[1] + mv.toff, H.th_cp[1] + mv.toff, H.d_ferr + 4);
+    qg::launch_thomas_factor(mv.g, H.st, s_a, 1, s_tab, s_tab + tq, H.d_ferr);
+    qg::launch_thomas_factor(mv.g, H.st, s_a, 4, s_tab + 2 * tq, s_tab + 3 * tq, H.d_ferr + 4);
     int he[8];
     HIPCHECK(hipMemcpyAsync(he, H.d_ferr, sizeof he, hipMemcpyDeviceToHost, H.st));
     HIPCHECK(hipStreamSynchronize(H.st));
     if (he[0] || he[4]) raise_singular(he[0] ? he : he + 4);
     HIPCHECK(hipMemsetAsync(H.d_ferr, 0, 8 * sizeof(int), H.st));
-    qg::launch_omega_factor(mv.g, H.st, H.d_cz + o, H.om_M + mv.toff, H.om_D + mv.toff,
-                            H.om_piv + (size_t)member * quarter, H.d_ferr);
+    qg::launch_omega_factor(mv.g, H.st, s_cz, s_tab + 4 * tq, s_tab + 5 * tq, s_piv, H.d_ferr);
     HIPCHECK(hipMemcpyAsync(he, H.d_ferr, sizeof he, hipMemcpyDeviceToHost, H.st));
     HIPCHECK(hipStreamSynchronize(H.st));
-    if (he[3]) H.om_any_piv = true;  // kept for the ensemble's lifetime
     if (he[0]) {
       char buf[160];
       snprintf(buf, sizeof buf, "LAPACK-style gtsv failed for vertical velocity at row %d of column %d",
                he[1] + 1, he[2]);
       throw QgError(QG_ERR_GTSV, buf);
+    }
+    // commit: coefficients, tables and the host copies of the profile
+    auto d2d = [&](void* dst, const void* src, size_t bytes) {
+      HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, H.st));
+    };
+    double* tabs[6] = {H.th_inv[0], H.th_cp[0], H.th_inv[1], H.th_cp[1], H.om_M, H.om_D};
+    for (int t = 0; t < 6; ++t) d2d(tabs[t] + mv.toff, s_tab + t * tq, tq * sizeof(double));
+    d2d(H.om_piv + (size_t)member * quarter, s_piv, quarter);
+    d2d(H.d_a + o, s_a, nzc * sizeof(double));
+    d2d(H.d_cz + o, s_cz, nzc * sizeof(double));
+    HIPCHECK(hipMemcpyAsync(H.d_fN2 + o, fN2.data(), nzc * sizeof(double), hipMemcpyHostToDevice, H.st));
+    HIPCHECK(hipMemcpyAsync(H.d_al + mv.aoff, al.data(), kpad * sizeof(double), hipMemcpyHostToDevice, H.st));
+    HIPCHECK(hipMemcpyAsync(H.d_cu + mv.aoff, cu.data(), kpad * sizeof(double), hipMemcpyHostToDevice, H.st));
+    HIPCHECK(hipStreamSynchronize(H.st));
+    std::copy(N2, N2 + nzc, H.N2.begin() + o);
+    std::copy(a_ell, a_ell + nzc, H.a.begin() + o);
+    if (he[3] && !H.om_any_piv) {
+      // the step graphs were captured without the pivoted-column omega
+      // kernel; it now has columns to solve (kept for the ensemble's lifetime)
+      H.drop_graphs();
+      H.om_any_piv = true;
     }
     H.diag_valid = false;
     H.t1_valid = false;
@@ -1545,20 +1577,37 @@ void autotune(qg_handle& H, double dt, int steps) {
   if (H.prof_mode) throw QgError(QG_ERR_ARGUMENT, "autotune while a profile is being recorded");
   H.psi64_nm = 0;
   HIPCHECK(hipStreamSynchronize(H.st));
+  // Every field a caller can read is saved and put back afterwards: the
+  // prognostic q, B (the candidates step from them) and the diagnostics psi,
+  // A, C, u, v, w of the state as the caller left it.
   const size_t n = (size_t)H.nzc * H.plane * H.active;
+  const size_t fb = n * H.fsize(), rb = n * H.rsize();  // bytes of one complex / real field
+  struct Saved {
+    void* field;
+    size_t off, bytes;
+  };
+  std::vector<Saved> kept;
+  size_t total = 0;
+  for (void* f : {H.q[H.cur], H.B[H.cur], H.psi, H.A, H.C}) {
+    kept.push_back({f, total, fb});
+    total += fb;
+  }
+  for (void* f : {H.u, H.v, H.w}) {
+    kept.push_back({f, total, rb});
+    total += rb;
+  }
   char* save = nullptr;
-  const size_t fb = n * H.fsize();  // bytes of one saved field
-  HIPCHECK(hipMalloc(&save, 2 * fb));
+  HIPCHECK(hipMalloc(&save, total));
   struct Cleanup {
     qg_handle& H;
     char* save;
-    size_t fb;
+    const std::vector<Saved>& kept;
     int cur;
     ~Cleanup() {
       (void)hipStreamSynchronize(H.st);
-      if (H.cur != cur) H.cur = cur;
-      (void)hipMemcpyAsync(H.q[H.cur], save, fb, hipMemcpyDeviceToDevice, H.st);
-      (void)hipMemcpyAsync(H.B[H.cur], save + fb, fb, hipMemcpyDeviceToDevice, H.st);
+      if (H.cur != cur) H.cur = cur;  // kept[0], kept[1] are the buffers of parity cur
+      for (const Saved& k : kept)
+        (void)hipMemcpyAsync(k.field, save + k.off, k.bytes, hipMemcpyDeviceToDevice, H.st);
       (void)hipStreamSynchronize(H.st);
       (void)hipFree(save);
       H.prof_mode = 0;
@@ -1568,9 +1617,9 @@ void autotune(qg_handle& H, double dt, int steps) {
       H.t1_valid = false;
       H.drop_graphs();
     }
-  } cleanup{H, save, fb, H.cur};
-  HIPCHECK(hipMemcpyAsync(save, H.q[H.cur], fb, hipMemcpyDeviceToDevice, H.st));
-  HIPCHECK(hipMemcpyAsync(save + fb, H.B[H.cur], fb, hipMemcpyDeviceToDevice, H.st));
+  } cleanup{H, save, kept, H.cur};
+  for (const Saved& k : kept)
+    HIPCHECK(hipMemcpyAsync(save + k.off, k.field, k.bytes, hipMemcpyDeviceToDevice, H.st));
   build_etd(H, dt);
   const std::vector<int> shapes = column_candidates(H.nzc);
   const int ncfg = std::max<int>(qg::kTileVariants, (int)shapes.size());
@@ -1585,9 +1634,9 @@ void autotune(qg_handle& H, double dt, int steps) {
     for (int p = 0; p < qg::P_FFT_END; ++p) H.tune[p] = fv;
     for (int p = qg::P_FFT_END; p < qg::P_END; ++p) H.tune[p] = cs;
     HIPCHECK(hipStreamSynchronize(H.st));
-    HIPCHECK(hipMemcpyAsync(H.q[cleanup.cur], save, fb, hipMemcpyDeviceToDevice, H.st));
-    HIPCHECK(hipMemcpyAsync(H.B[cleanup.cur], save + fb, fb, hipMemcpyDeviceToDevice, H.st));
     H.cur = cleanup.cur;
+    HIPCHECK(hipMemcpyAsync(H.q[H.cur], save + kept[0].off, fb, hipMemcpyDeviceToDevice, H.st));
+    HIPCHECK(hipMemcpyAsync(H.B[H.cur], save + kept[1].off, fb, hipMemcpyDeviceToDevice, H.st));
     H.diag_valid = false;
     H.t1_valid = false;
     std::vector<std::vector<float>> ms(qg::P_END);

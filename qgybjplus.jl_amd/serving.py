@@ -295,12 +295,16 @@ class Server:
         return sorted(results, key=lambda r: r.rid), summary
 
 
-def run_alone(req: RunRequest, precision: str = "mixed", dt: float = 20.0) -> Dict[str, np.ndarray]:
+def run_alone(req: RunRequest, precision: str = "mixed", dt: float = 20.0,
+              autotune: bool = False) -> Dict[str, np.ndarray]:
     """The same run on a model of its own: the reference result for the
-    alone-versus-in-the-mix check."""
+    alone-versus-in-the-mix check.  With autotune the model takes the tuning
+    of its grid class (cached in the process by the pool that tuned it)."""
     grid = qg.RectilinearGrid(size=(req.nx, req.nx, req.nz), extent=EXTENT, centered=True)
     m = qg.QGYBJModel(grid=grid, coriolis=qg.FPlane(1e-4), stratification=req.strat(),
                       feedback=qg.WaveMeanFeedback(), formulation=qg.YBJPlus(), architecture=qg.GPU(precision))
+    if autotune:
+        m.autotune(dt, steps=2)
     psi, wave = req.initial_state(grid)
     qg.set_(m, psi=psi, waves=wave)
     qg.step_(m, qg.ExponentialRungeKutta2(dt=dt), req.steps)

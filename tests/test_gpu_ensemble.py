@@ -165,3 +165,61 @@ def test_active_members_and_member_moves(qg):
     ens.set_active_members(0)
     qg.step_(ens, ts, 2)  # a no-op
     ens.finalize()
+
+
+def test_stratification_change_under_captured_graphs(qg):
+    """Graphs of the steady-state step are captured while no member's omega
+    elimination interchanges rows; then member 1 is given the profile whose
+    elimination does (and a fresh start).  The graphs must be re-captured with
+    the pivoted-column kernel: member 1 then equals the same run alone, w
+    included, and member 0 goes on as its own lone run."""
+    kw = _kwargs(qg)
+    ens = qg.QGYBJEnsemble(stratifications=[_strat(qg, 0), _strat(qg, 1)], **kw)
+    for i in range(2):
+        psi, B = _initial(qg, kw["grid"], i)
+        ens.set_member(i, psi=qg.FieldArray(psi), waves=qg.FieldArray(B))
+    ts = qg.ExponentialRungeKutta2(dt=600.0)
+    qg.step_(ens, ts, 5)  # one eager step, then graph-replayed pairs
+    ens.set_member_stratification(1, _strat(qg, 2))
+    psi2, B2 = _initial(qg, kw["grid"], 2)
+    ens.set_member(1, psi=qg.FieldArray(psi2), waves=qg.FieldArray(B2))
+    qg.step_(ens, ts, 5)
+    ref2, ref0 = _alone(qg, 2, 5), _alone(qg, 0, 10)
+    for n in ("q", "B", "psi", "A", "C", "u", "v", "w"):
+        a, b = getattr(ens.member_fields(1), n), ref2[n]
+        assert np.max(np.abs(a - b)) <= 1e-14 * np.max(np.abs(b)), ("member 1", n)
+        a, b = getattr(ens.member_fields(0), n), ref0[n]
+        assert np.max(np.abs(a - b)) <= 1e-14 * np.max(np.abs(b)), ("member 0", n)
+    ens.finalize()
+
+
+def test_failed_stratification_change_keeps_the_member(qg):
+    """A profile whose Thomas elimination is singular is refused with the
+    reference's error, and the member keeps its previous profile and tables:
+    its next steps equal those of a lone run that never saw the refusal.  On
+    a 2e7 m box the longest wave has kh^2 dz^2 ~ 1.1e-8: with a = f^2/N^2 =
+    1e-12 the first pivots -(a + kh^2 dz^2 [/4]) fall below sqrt(eps) ~
+    1.5e-8, while with a = 1e-3 the smallest pivot of either operator is
+    3.3e-8."""
+    grid = qg.RectilinearGrid(size=SIZE, extent=(2e7, 2e7, 4000.0))
+    kw = dict(grid=grid, coriolis=qg.FPlane(1e-4), feedback=qg.WaveMeanFeedback(), architecture=qg.GPU("fp64"))
+    strat = qg.ConstantStratification(1e-5)
+    ens = qg.QGYBJEnsemble(stratifications=[strat, strat], **kw)
+    psi, B = _initial(qg, grid, 1)
+    for i in range(2):
+        ens.set_member(i, psi=qg.FieldArray(psi), waves=qg.FieldArray(B))
+    with pytest.raises(qg.QgError) as ei:
+        ens.set_member_stratification(1, qg.ConstantStratification(1e4))
+    assert ei.value.code == 3 and str(ei.value).startswith("Singular matrix in Thomas solver")
+    assert np.array_equal(ens.N2[SIZE[2]:], ens.N2[:SIZE[2]])
+    qg.step_(ens, qg.ExponentialRungeKutta2(dt=600.0), 3)
+    m = qg.QGYBJModel(stratification=strat, **kw)
+    qg.set_(m, psi=qg.FieldArray(psi), waves=qg.FieldArray(B))
+    qg.step_(m, qg.ExponentialRungeKutta2(dt=600.0), 3)
+    for n in ("q", "B", "psi", "A", "w"):
+        ref = getattr(m.fields, n)
+        for i in range(2):
+            got = getattr(ens.member_fields(i), n)
+            assert np.max(np.abs(got - ref)) <= 1e-14 * np.max(np.abs(ref)), (i, n)
+    m.finalize()
+    ens.finalize()

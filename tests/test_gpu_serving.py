@@ -25,6 +25,21 @@ def test_runs_in_the_mix_equal_runs_alone(qg):
             assert np.max(np.abs(a - b)) <= 1e-14 * np.max(np.abs(b)), (req.rid, n)
 
 
+def test_runs_in_the_mix_equal_runs_alone_tuned(qg):
+    """As above with the autotuner on, as the benchmark's serving leg runs:
+    each pool tunes its grid class once, and a lone run of the same class
+    takes the cached choice, so the results agree bit for bit again."""
+    trace = _trace(qg, 4)
+    server = qg.serving.Server(precision="mixed", slots=2, chunk=16, keep_fields=True, autotune=True)
+    results, summary = server.run(trace, realtime=False)
+    assert summary["requests"] == len(trace) and not summary["nonfinite_runs"]
+    for req, res in zip(sorted(trace, key=lambda r: r.rid), results):
+        alone = qg.serving.run_alone(req, precision="mixed", autotune=True)
+        for n in ("q", "B", "psi"):
+            a, b = res.fields[n], alone[n]
+            assert np.max(np.abs(a - b)) <= 1e-14 * np.max(np.abs(b)), (req.rid, n)
+
+
 def test_admission_waits_for_memory(qg):
     """A budget that holds one pool at a time: the second grid class is
     admitted only after the first pool has drained and been released."""

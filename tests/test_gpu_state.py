@@ -43,6 +43,19 @@ def test_write_restore_continue(qg, tmp_path, precision):
         assert np.array_equal(getattr(b.fields, n), getattr(a.fields, n)), n
     for n in ("u", "v"):
         assert np.array_equal(getattr(b.fields, n), getattr(a.fields, n)), n
+    # restore! rebuilds psi = L^-1 q, u, v from psi and A, C from B
+    # (src/core/io.jl:583-587): the oracle's values from the file's q-hat, B-hat
+    q_file, B_file = qg.read_prognostic(path, c.size)
+    N2 = np.full(c.size[2], O.constant_N2(c.N2))
+    a_ell = O.a_ell_from_N2(N2, c.f)
+    o_psi = O.invert_q_to_psi(q_file, grid, a_ell)
+    o_u, o_v, _ = O.compute_velocities(o_psi, grid, c.f, N2, O.dealias_mask(grid.nx, grid.ny), False)
+    o_A, o_C = O.invert_B_to_A(B_file, grid, a_ell)
+    tol = 1e-11 if precision == "fp64" else 1e-5
+    for n, ref in (("psi", o_psi), ("u", o_u), ("v", o_v), ("A", o_A), ("C", o_C)):
+        e = O.relative_l2(getattr(b.fields, n), ref)
+        print(f"  restored {n} vs oracle: {e:.2e}", flush=True)
+        assert e <= tol, (n, e)
     ts = qg.ExponentialRungeKutta2(dt=600.0)
     qg.step_(a, ts, 5)
     qg.step_(b, ts, 5)

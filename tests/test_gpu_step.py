@@ -194,3 +194,25 @@ def test_mixed_precision_error_is_stated(qg):
         assert e < 1e-4, (n, e)
     m64.finalize()
     m32.finalize()
+
+
+def test_radix3_grid_step_matches_oracle(qg):
+    """384x384x8: the radix-3 transform plan (8*6*8) inside the whole step,
+    2 steps against the oracle (the serving workload draws this size)."""
+    c = Case(size=(384, 384, 8), extent=(500e3, 500e3, 4000.0), N2_func=EXP_N2, feedback=True)
+    print("case step 384x384x8", flush=True)
+    m, o = gpu_model(qg, c), oracle_model(c)
+    psi = k3_streamfunction(o.grid, seed=21, urms=0.1, kpeak=6.0)
+    X, Y = np.meshgrid(o.grid.x, o.grid.y, indexing="ij")
+    B = np.empty(psi.shape, complex)
+    for k in range(o.grid.nz):
+        B[k] = 0.1 * math.exp(o.grid.z[k] / 300.0) * np.exp(1j * 2 * math.pi * (3 * X / o.grid.Lx + Y / o.grid.Ly))
+    set_both(m, o, psi, B, qg)
+    qg.step_(m, qg.ExponentialRungeKutta2(dt=60.0), 2)
+    for _ in range(2):
+        o.step(60.0)
+    for n in FIELDS + ("C", "u", "v", "w"):
+        e = rel(getattr(m.fields, n), getattr(o.fields, n))
+        print(f"  384 {n}: {e:.2e}", flush=True)
+        assert e <= (1e-9 if n == "w" else 1e-11), (n, e)
+    m.finalize()

@@ -103,11 +103,12 @@ def test_autotune_restores_state_and_caches(qg):
     print("case autotune", flush=True)
     m = _model(qg, (64, 64, 16), "mixed")
     _init(qg, m)
-    before = {n: m._h.get_field(n) for n in ("q", "B")}
+    every = ("q", "B", "psi", "A", "C", "u", "v", "w")
+    before = {n: m._h.get_field(n) for n in every}
     choice = m.autotune(dt=300.0, steps=2, force=True)
     assert set(choice) == set(FFT + COLUMN)
     assert all(-1 <= choice[p] <= 3 for p in FFT)
-    for n in ("q", "B"):
+    for n in every:  # the diagnostics too, not only the prognostic state
         assert np.array_equal(m._h.get_field(n), before[n]), n
     # stepping after tuning matches an untuned model to round-off: the
     # column shapes differ in the last FP64 bits, which the FP32 field
