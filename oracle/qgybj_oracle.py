@@ -61,6 +61,22 @@ Pins (what checks each function; see ``tests/test_oracle.py``):
   second difference, whose eigenvalues 2 cos(theta) - 2 give w in closed
   form for constant N2, and a dense solve for varying N2.
 
+Signs of the step's assembly (``tests/test_oracle_pins.py``):
+
+* ``OracleModel._rhs``: rhs_B = -div(u B) + i (f/2) kh^2 A - (i/2) zeta B
+  term by term on three distinct Fourier modes; rhs_q = -J(psi, q) on a
+  two-mode barotropic flow; + nu_z d_zz q on Neumann cosine eigenmodes (end
+  rows included).
+* ``OracleModel.step`` (q): one step of pure vertical diffusion multiplies
+  an eigenmode by 1 + h mu + (h mu)^2 / 2.
+* ``OracleModel._diagnose_flow``: q = 0 with the closed-form q_w gives
+  psi = L^-1(-q_w) and restores q exactly
+  (``test/test_wave_feedback.jl:91-96`` run through the function).
+* ``compute_velocities``: u = -psi_y, v = psi_x on psi = sin 2x + sin 3y.
+* Each of these pins, and the older ones above, is checked against copies of
+  this module with one planted one-token mistake (a flipped sign, a dropped
+  term, a wrong end row); every mutant fails at least one of them.
+
 Parity unpinned: no reference test checks the omega ``w`` values themselves
 (the closed forms above pin the equation as ``src/operators.jl:365-493``
 writes it), and true reference trajectories of the full nonlinear step (the
