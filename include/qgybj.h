@@ -50,7 +50,8 @@ extern "C" {
 
 /* precision modes */
 #define QG_PRECISION_FP64 0  /* every transform, product and solve in FP64 */
-#define QG_PRECISION_MIXED 1 /* FP32 transforms and products; FP64 state, solves and ETD coefficients */
+#define QG_PRECISION_MIXED 1 /* FP32 storage of every field (state included), FP32 transforms and
+                                products; FP64 column-solve recurrences and ETD coefficients */
 
 /* space of a field handed to qg_set_field */
 #define QG_SPACE_SPECTRAL 0

@@ -279,14 +279,35 @@ def etd_coefficients(x, h):
 # Transforms  (src/transforms.jl:19-23, src/parallel_mpi.jl:735-797)
 # ----------------------------------------------------------------------------
 
+# Threads of the 2-D transforms: 1 (numpy.fft, the reference's single-threaded
+# FFTW configuration, src/parallel_mpi.jl:50) unless set_fft_workers raises it
+# (scipy.fft's pocketfft with a thread pool over the z-planes).
+_FFT_WORKERS = 1
+
+
+def set_fft_workers(n: int) -> int:
+    """Set the transform thread count; returns the previous one."""
+    global _FFT_WORKERS
+    prev, _FFT_WORKERS = _FFT_WORKERS, max(1, int(n))
+    return prev
+
+
 def fft_forward(a: np.ndarray) -> np.ndarray:
     """Per-z-plane 2-D C2C forward DFT, unnormalised."""
-    return np.fft.fft2(np.asarray(a, dtype=np.complex128), axes=(1, 2))
+    a = np.asarray(a, dtype=np.complex128)
+    if _FFT_WORKERS > 1:
+        import scipy.fft
+        return scipy.fft.fft2(a, axes=(1, 2), workers=_FFT_WORKERS)
+    return np.fft.fft2(a, axes=(1, 2))
 
 
 def fft_backward(a: np.ndarray) -> np.ndarray:
     """Per-z-plane 2-D inverse DFT, normalised by 1/(nx*ny)."""
-    return np.fft.ifft2(np.asarray(a, dtype=np.complex128), axes=(1, 2))
+    a = np.asarray(a, dtype=np.complex128)
+    if _FFT_WORKERS > 1:
+        import scipy.fft
+        return scipy.fft.ifft2(a, axes=(1, 2), workers=_FFT_WORKERS)
+    return np.fft.ifft2(a, axes=(1, 2))
 
 
 # ----------------------------------------------------------------------------
