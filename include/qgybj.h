@@ -156,6 +156,21 @@ int qg_check(qg_handle* h, int* nonfinite, double* max_abs_psi);
  * state every chunk of steps starts from (q, B, psi, A, the y-transformed B,
  * u, v), kept to step again up to a failing step. */
 int qg_run(qg_handle* h, double dt, int nsteps, int* done, int* failure, double* max_abs_psi);
+/* No counterpart in the reference: qg_run for callers that cannot allocate
+ * device memory (tight serving loops, harnesses that forbid hipMalloc).  The
+ * same loop, checks, failure semantics and results as qg_run, bit for bit,
+ * with every step launched eagerly (no graph is instantiated) and the chunk
+ * snapshot kept in a static device scratch array of the library instead of
+ * the buffer qg_run allocates on its first call; nothing is allocated.
+ * Limit: the snapshot takes 12 scalars of the storage precision per grid
+ * point of the active members and must fit the 128 MiB scratch
+ * (QG_RUN_NOALLOC_SCRATCH_BYTES); larger models get QG_ERR_ARGUMENT and
+ * nothing runs.  Every call on one GPU shares that GPU's scratch: calls that
+ * overlap on one GPU (two handles on two host threads) race.  Not while a
+ * profile is recorded (QG_ERR_ARGUMENT).  Not capturable into a graph: it
+ * reads the check back once per chunk. */
+#define QG_RUN_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int qg_run_noalloc(qg_handle* h, double dt, int nsteps, int* done, int* failure, double* max_abs_psi);
 /* Replaces _progress_maxima (src/simulation.jl:46-81): the maximum physical
  * wave velocity |ifft2(B + kh2 A / 4)| and the maximum flow speed
  * hypot(u, v) over the grid (all members), at the transform precision. */

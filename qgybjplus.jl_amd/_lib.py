@@ -32,7 +32,7 @@ SPACE_SPECTRAL, SPACE_PHYSICAL = 0, 1
 #: every symbol include/qgybj.h declares
 EXPORTED = (
     "qg_create", "qg_destroy", "qg_last_error", "qg_set_field", "qg_get_field",
-    "qg_step", "qg_run", "qg_check", "qg_progress_maxima", "qg_invert_q_to_psi", "qg_invert_B_to_A",
+    "qg_step", "qg_run", "qg_run_noalloc", "qg_check", "qg_progress_maxima", "qg_invert_q_to_psi", "qg_invert_B_to_A",
     "qg_compute_velocities", "qg_compute_q_from_psi", "qg_compute_qw", "qg_fft",
     "qg_synchronize", "qg_take_mean_q_warning", "qg_profile_begin", "qg_profile_end",
     "qg_set_field_member", "qg_get_field_member", "qg_check_member",
@@ -100,6 +100,7 @@ def load() -> C.CDLL:
         "qg_step": (C.c_int, [P, C.c_double, C.c_int]),
         "qg_check": (C.c_int, [P, C.POINTER(C.c_int), dp]),
         "qg_run": (C.c_int, [P, C.c_double, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), dp]),
+        "qg_run_noalloc": (C.c_int, [P, C.c_double, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), dp]),
         "qg_progress_maxima": (C.c_int, [P, dp, dp]),
         "qg_set_field_member": (C.c_int, [P, C.c_int, C.c_int, dp, C.c_int]),
         "qg_get_field_member": (C.c_int, [P, C.c_int, C.c_int, dp]),
@@ -290,6 +291,16 @@ class Handle:
         fail = C.c_int(0)
         m = C.c_double(0.0)
         self._check(self._lib.qg_run(self.h, float(dt), int(nsteps), C.byref(done), C.byref(fail), C.byref(m)))
+        return int(done.value), int(fail.value), float(m.value)
+
+    def run_noalloc(self, dt: float, nsteps: int):
+        """qg_run_noalloc: qg_run with its snapshot in the library's static
+        device scratch (no device allocation; eager steps); same returns."""
+        done = C.c_int(0)
+        fail = C.c_int(0)
+        m = C.c_double(0.0)
+        self._check(self._lib.qg_run_noalloc(self.h, float(dt), int(nsteps), C.byref(done), C.byref(fail),
+                                             C.byref(m)))
         return int(done.value), int(fail.value), float(m.value)
 
     def progress_maxima(self):

@@ -115,6 +115,9 @@ struct qg_handle {
   // the state a step starts from (q, B and the diagnostics psi, A, T1, u, v
   // it reuses) at the start of a qg_run chunk; allocated by the first qg_run
   void* snap = nullptr;
+  // device address of the file-scope snapshot scratch on this handle's
+  // device (qg_run_noalloc), looked up by qg_create
+  void* noalloc_snap = nullptr;
   std::string err;
   // profiling: bit 1 = events around every kernel, bit 2 = one event per step
   struct ProfRec {
@@ -656,6 +659,37 @@ void advance(qg_handle& H, E& e, int nsteps, bool checked, Mark&& mark) {
 // Steps per qg_run chunk: one host round trip per chunk.
 constexpr int kRunChunk = 32;
 
+// Snapshot space of qg_run_noalloc: the state a chunk starts from (12 scalars
+// of the storage precision per point of the active members), held in a
+// file-scope device array so the entry point allocates nothing.  Every GPU the
+// library runs on holds one copy, at its own address, until the process ends;
+// calls that overlap on one GPU share it and race.
+constexpr size_t kNoAllocSnapBytes = QG_RUN_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocSnapBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_addr;  // device id -> address of `scratch` there
+
+// Address of `scratch` on the current device (looked up once per device).
+// In host code the array's name is only a stand-in; this address is what
+// kernels and copies use.
+void* device_scratch() {
+  int dev = -1;
+  HIPCHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch_addr.find(dev);
+  if (it != g_scratch_addr.end()) return it->second;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    throw QgError(QG_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_addr[dev] = ptr;
+  return ptr;
+}
+}  // namespace
+
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
@@ -868,6 +902,7 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
       std::memcpy(h->sing_omega, he, 3 * sizeof(int));
       h->om_any_piv = he[3] != 0;
     }
+    h->noalloc_snap = device_scratch();
     HIPCHECK(hipDeviceSynchronize());
     *out = h;
     return QG_OK;
@@ -1017,6 +1052,93 @@ void check_range(qg_handle& H, int m0, int nm, int* nonfinite, double* max_abs_p
   if (max_abs_psi) *max_abs_psi = m;
 }
 
+// run!'s stepping loop with the check on the device, shared by qg_run and
+// qg_run_noalloc: `snap` holds 12 R scalars per point of the active members;
+// graphs = false launches every step eagerly (no graph is instantiated).
+void run_checked(qg_handle& H, double dt, int nsteps, void* snap, bool graphs, int& d, int& fail,
+                 double& mx) {
+  d = 0;
+  fail = 0;
+  mx = 0.0;
+  if (H.active <= 0 || nsteps <= 0) return;
+  H.psi64_nm = 0;
+  build_etd(H, dt);
+  // everything the next step reads: q, B (current buffers), and the
+  // diagnostics a steady-state step reuses (psi, A, T1, u, v); R scalars
+  // per point of each
+  const size_t pts = (size_t)H.active * H.nzc * H.plane, r = H.rsize();
+  struct Part {
+    void* field;
+    size_t per_point;  // R scalars
+  };
+  auto parts = [&](int c) {
+    return std::array<Part, 7>{Part{H.q[c], 2}, Part{H.B[c], 2}, Part{H.psi, 2}, Part{H.A, 2},
+                               Part{H.T1, 2}, Part{H.u, 1}, Part{H.v, 1}};
+  };
+  auto copy_parts = [&](int c, bool save) {
+    char* at = static_cast<char*>(snap);
+    for (const Part& p : parts(c)) {
+      const size_t bytes = pts * p.per_point * r;
+      HIPCHECK(hipMemcpyAsync(save ? at : p.field, save ? p.field : at, bytes, hipMemcpyDeviceToDevice, H.st));
+      at += bytes;
+    }
+  };
+  auto clear = [&] {
+    HIPCHECK(hipMemsetAsync(H.d_flag, 0, sizeof(int), H.st));
+    HIPCHECK(hipMemsetAsync(H.d_absmax, 0, sizeof(unsigned long long), H.st));
+  };
+  auto none = [](int) {};
+  const bool graphs_were = H.graphs;
+  H.graphs = graphs_were && graphs;
+  struct Restore {
+    qg_handle& H;
+    bool g;
+    ~Restore() { H.graphs = g; }
+  } restore{H, graphs_were};
+  with_engine(H, [&](auto& e) {
+    while (d < nsteps && fail == 0) {
+      const int k = std::min(kRunChunk, nsteps - d);
+      const int c0 = H.cur;
+      const bool dv = H.diag_valid, tv = H.t1_valid;
+      copy_parts(c0, true);
+      clear();
+      advance(H, e, k, true, none);
+      int flag = 0;
+      double m = 0.0;
+      read_check(H, flag, m);
+      check_status(H);
+      if (flag == 0 && m <= 1e10) {
+        d += k;
+        mx = std::max(mx, m);
+        continue;
+      }
+      // A step of this chunk failed: restore the chunk's starting state
+      // and step again, one step and one check at a time, stopping right
+      // after the failing step.  The same kernels on the same state give
+      // the same values, so the state at the stop is the one the failing
+      // step produced.
+      copy_parts(c0, false);
+      H.cur = c0;
+      H.diag_valid = dv;
+      H.t1_valid = tv;
+      for (int j = 0; j < k && fail == 0; ++j) {
+        clear();
+        advance(H, e, 1, true, none);
+        read_check(H, flag, m);
+        check_status(H);
+        ++d;
+        if (flag != 0 || !(m <= 1e10)) {
+          fail = flag != 0 ? 1 : 2;
+          mx = m;
+        } else {
+          mx = std::max(mx, m);
+        }
+      }
+    }
+  });
+  HIPCHECK(hipGetLastError());
+}
+
 // One member seen as a single model: its Geo and the offsets of its levels,
 // its Thomas tables and its shifted coefficient array.
 struct MemberView {
@@ -1100,78 +1222,34 @@ int qg_run(qg_handle* h, double dt, int nsteps, int* done, int* failure, double*
     qg_handle& H = *h;
     int d = 0, fail = 0;
     double mx = 0.0;
-    if (H.active > 0 && nsteps > 0) {
-      H.psi64_nm = 0;
-      build_etd(H, dt);
-      // everything the next step reads: q, B (current buffers), and the
-      // diagnostics a steady-state step reuses (psi, A, T1, u, v); R scalars
-      // per point of each
-      const size_t pts = (size_t)H.active * H.nzc * H.plane, r = H.rsize();
-      if (!H.snap) H.snap = H.ralloc(12 * H.npts);
-      struct Part {
-        void* field;
-        size_t per_point;  // R scalars
-      };
-      auto parts = [&](int c) {
-        return std::array<Part, 7>{Part{H.q[c], 2}, Part{H.B[c], 2}, Part{H.psi, 2}, Part{H.A, 2},
-                                   Part{H.T1, 2}, Part{H.u, 1}, Part{H.v, 1}};
-      };
-      auto copy_parts = [&](int c, bool save) {
-        char* at = static_cast<char*>(H.snap);
-        for (const Part& p : parts(c)) {
-          const size_t bytes = pts * p.per_point * r;
-          HIPCHECK(hipMemcpyAsync(save ? at : p.field, save ? p.field : at, bytes, hipMemcpyDeviceToDevice, H.st));
-          at += bytes;
-        }
-      };
-      auto clear = [&] {
-        HIPCHECK(hipMemsetAsync(H.d_flag, 0, sizeof(int), H.st));
-        HIPCHECK(hipMemsetAsync(H.d_absmax, 0, sizeof(unsigned long long), H.st));
-      };
-      auto none = [](int) {};
-      with_engine(H, [&](auto& e) {
-        while (d < nsteps && fail == 0) {
-          const int k = std::min(kRunChunk, nsteps - d);
-          const int c0 = H.cur;
-          const bool dv = H.diag_valid, tv = H.t1_valid;
-          copy_parts(c0, true);
-          clear();
-          advance(H, e, k, true, none);
-          int flag = 0;
-          double m = 0.0;
-          read_check(H, flag, m);
-          check_status(H);
-          if (flag == 0 && m <= 1e10) {
-            d += k;
-            mx = std::max(mx, m);
-            continue;
-          }
-          // A step of this chunk failed: restore the chunk's starting state
-          // and step again, one step and one check at a time, stopping right
-          // after the failing step.  The same kernels on the same state give
-          // the same values, so the state at the stop is the one the failing
-          // step produced.
-          copy_parts(c0, false);
-          H.cur = c0;
-          H.diag_valid = dv;
-          H.t1_valid = tv;
-          for (int j = 0; j < k && fail == 0; ++j) {
-            clear();
-            advance(H, e, 1, true, none);
-            read_check(H, flag, m);
-            check_status(H);
-            ++d;
-            if (flag != 0 || !(m <= 1e10)) {
-              fail = flag != 0 ? 1 : 2;
-              mx = m;
-            } else {
-              mx = std::max(mx, m);
-            }
-          }
-        }
-      });
-      HIPCHECK(hipGetLastError());
+    if (H.active > 0 && nsteps > 0 && !H.snap) H.snap = H.ralloc(12 * H.npts);
+    run_checked(H, dt, nsteps, H.snap, true, d, fail, mx);
+    if (done) *done = d;
+    if (failure) *failure = fail;
+    if (max_abs_psi) *max_abs_psi = mx;
+  });
+}
+
+int qg_run_noalloc(qg_handle* h, double dt, int nsteps, int* done, int* failure, double* max_abs_psi) {
+  return guarded(h, [&] {
+    if (!(std::isfinite(dt) && dt > 0)) throw QgError(QG_ERR_ARGUMENT, "Δt must be finite and positive");
+    if (nsteps < 0) throw QgError(QG_ERR_ARGUMENT, "nsteps must be non-negative");
+    qg_handle& H = *h;
+    if (!H.noalloc_snap) throw QgError(QG_ERR_DEVICE, "qg_run_noalloc: no scratch address for this handle");
+    if (H.prof_mode)
+      throw QgError(QG_ERR_ARGUMENT, "qg_run_noalloc: not while a profile is recorded (profiling creates events)");
+    const size_t need = (size_t)12 * H.rsize() * H.active * H.nzc * H.plane;
+    if (need > kNoAllocSnapBytes) {
+      char buf[256];
+      snprintf(buf, sizeof buf,
+               "qg_run_noalloc: the snapshot of the active members needs %zu bytes, more than the %zu-byte "
+               "static scratch (12 scalars per grid point); use qg_run",
+               need, kNoAllocSnapBytes);
+      throw QgError(QG_ERR_ARGUMENT, buf);
     }
+    int d = 0, fail = 0;
+    double mx = 0.0;
+    run_checked(H, dt, nsteps, H.noalloc_snap, false, d, fail, mx);
     if (done) *done = d;
     if (failure) *failure = fail;
     if (max_abs_psi) *max_abs_psi = mx;
