@@ -21,27 +21,41 @@
 
 namespace qg {
 
-// cos(2*pi*m/48) for m in [0, 12]; every radix used here divides 48.
-constexpr double kCos48[13] = {
-    1.0, 0.9914448613738104, 0.9659258262890683, 0.9238795325112867,
-    0.8660254037844386, 0.7933533402912352, 0.7071067811865476,
-    0.6087614290087207, 0.5, 0.3826834323650898, 0.25881904510252074,
-    0.1305261922200516, 0.0};
+// cos(2*pi*m/240) for m in [0, 60], correctly rounded; every radix used here
+// (2, 3, 4, 5, 6, 8, 10, 12, 16) divides 240.
+constexpr double kCos240[61] = {
+    1.0, 0.9996573249755573, 0.9986295347545738, 0.996917333733128,
+    0.9945218953682733, 0.9914448613738104, 0.9876883405951378, 0.9832549075639546,
+    0.9781476007338057, 0.9723699203976766, 0.9659258262890683, 0.958819734868193,
+    0.9510565162951535, 0.9426414910921784, 0.9335804264972017, 0.9238795325112867,
+    0.9135454576426009, 0.9025852843498606, 0.8910065241883679, 0.8788171126619654,
+    0.8660254037844386, 0.8526401643540922, 0.838670567945424, 0.8241261886220157,
+    0.8090169943749475, 0.7933533402912352, 0.7771459614569709, 0.7604059656000309,
+    0.7431448254773942, 0.7253743710122876, 0.7071067811865476, 0.688354575693754,
+    0.6691306063588582, 0.6494480483301837, 0.6293203910498375, 0.6087614290087207,
+    0.5877852522924731, 0.5664062369248328, 0.5446390350150271, 0.5224985647159489,
+    0.5, 0.4771587602596084, 0.4539904997395468, 0.43051109680829514,
+    0.4067366430758002, 0.3826834323650898, 0.35836794954530027, 0.3338068592337709,
+    0.30901699437494745, 0.2840153447039226, 0.25881904510252074, 0.23344536385590542,
+    0.20791169081775934, 0.18223552549214744, 0.15643446504023087, 0.1305261922200516,
+    0.10452846326765347, 0.07845909572784494, 0.052335956242943835, 0.026176948307873153,
+    0.0};
 
-__host__ __device__ constexpr double cos48(int m) {
-  m = ((m % 48) + 48) % 48;
-  if (m <= 12) return kCos48[m];
-  if (m <= 24) return -kCos48[24 - m];
-  if (m <= 36) return -kCos48[m - 24];
-  return kCos48[48 - m];
+__host__ __device__ constexpr double cos240(int m) {
+  m = ((m % 240) + 240) % 240;
+  if (m <= 60) return kCos240[m];
+  if (m <= 120) return -kCos240[120 - m];
+  if (m <= 180) return -kCos240[m - 120];
+  return kCos240[240 - m];
 }
-__host__ __device__ constexpr double sin48(int m) { return cos48(m - 12); }
+__host__ __device__ constexpr double sin240(int m) { return cos240(m - 60); }
 
-// exp(DIR * 2*pi*i * m / N) for N dividing 48.
+// exp(DIR * 2*pi*i * m / N) for N dividing 240.
 template <typename R, int N, int DIR>
 __device__ inline cplx<R> unit_root(int m) {
-  constexpr int f = 48 / N;
-  return mk<R>((R)cos48(m * f), (R)(DIR * sin48(m * f)));
+  static_assert(240 % N == 0, "radix must divide 240");
+  constexpr int f = 240 / N;
+  return mk<R>((R)cos240(m * f), (R)(DIR * sin240(m * f)));
 }
 
 // ---------------------------------------------------------------------------
@@ -79,6 +93,32 @@ struct Dft<3, DIR> {
     v[0] = a + s;
     v[1] = m + r;
     v[2] = m - r;
+  }
+};
+
+// 5-point DFT: with s1 = a1 + a4, d1 = a1 - a4, s2 = a2 + a3, d2 = a2 - a3,
+// X1,4 = a0 + c1 s1 + c2 s2 +- DIR i (n1 d1 + n2 d2) and
+// X2,3 = a0 + c2 s1 + c1 s2 +- DIR i (n2 d1 - n1 d2), where c1 = cos(2 pi/5),
+// c2 = cos(4 pi/5), n1 = sin(2 pi/5), n2 = sin(4 pi/5).
+template <int DIR>
+struct Dft<5, DIR> {
+  template <typename R>
+  __device__ static inline void run(cplx<R>* v) {
+    const R c1 = (R)0.30901699437494745, c2 = (R)-0.8090169943749475;
+    const R n1 = (R)0.9510565162951535, n2 = (R)0.5877852522924731;
+    const cplx<R> a0 = v[0];
+    const cplx<R> s1 = v[1] + v[4], d1 = v[1] - v[4], s2 = v[2] + v[3], d2 = v[2] - v[3];
+    const cplx<R> m1 = mk<R>(a0.re + c1 * s1.re + c2 * s2.re, a0.im + c1 * s1.im + c2 * s2.im);
+    const cplx<R> m2 = mk<R>(a0.re + c2 * s1.re + c1 * s2.re, a0.im + c2 * s1.im + c1 * s2.im);
+    const cplx<R> e1 = mk<R>(n1 * d1.re + n2 * d2.re, n1 * d1.im + n2 * d2.im);
+    const cplx<R> e2 = mk<R>(n2 * d1.re - n1 * d2.re, n2 * d1.im - n1 * d2.im);
+    const cplx<R> r1 = mk<R>(-(R)DIR * e1.im, (R)DIR * e1.re);  // DIR*i*e1
+    const cplx<R> r2 = mk<R>(-(R)DIR * e2.im, (R)DIR * e2.re);  // DIR*i*e2
+    v[0] = a0 + s1 + s2;
+    v[1] = m1 + r1;
+    v[4] = m1 - r1;
+    v[2] = m2 + r2;
+    v[3] = m2 - r2;
   }
 };
 
@@ -127,6 +167,7 @@ struct DftCT {
 };
 
 template <int DIR> struct Dft<6, DIR> : DftCT<2, 3, DIR> {};
+template <int DIR> struct Dft<10, DIR> : DftCT<2, 5, DIR> {};
 template <int DIR> struct Dft<8, DIR> : DftCT<2, 4, DIR> {};
 template <int DIR> struct Dft<12, DIR> : DftCT<4, 3, DIR> {};
 template <int DIR> struct Dft<16, DIR> : DftCT<4, 4, DIR> {};
@@ -136,14 +177,28 @@ template <int DIR> struct Dft<16, DIR> : DftCT<4, 4, DIR> {};
 // input and of the output coincide) and lanes per sequence T.
 // ---------------------------------------------------------------------------
 template <int N, int PL = 0> struct FftPlan;
-template <> struct FftPlan<8>   { static constexpr int S = 1, r0 = 8, r1 = 1, T = 1; };
-template <> struct FftPlan<16>  { static constexpr int S = 2, r0 = 4, r1 = 1, T = 4; };
-template <> struct FftPlan<32>  { static constexpr int S = 3, r0 = 4, r1 = 2, T = 8; };
-template <> struct FftPlan<64>  { static constexpr int S = 2, r0 = 8, r1 = 1, T = 8; };
-template <> struct FftPlan<128> { static constexpr int S = 3, r0 = 4, r1 = 8, T = 16; };
-template <> struct FftPlan<256> { static constexpr int S = 3, r0 = 8, r1 = 4, T = 32; };
-template <> struct FftPlan<384> { static constexpr int S = 3, r0 = 8, r1 = 6, T = 48; };
-template <> struct FftPlan<512> { static constexpr int S = 3, r0 = 8, r1 = 8, T = 64; };
+template <> struct FftPlan<8>    { static constexpr int S = 1, r0 = 8, r1 = 1, T = 1; };
+template <> struct FftPlan<10>   { static constexpr int S = 1, r0 = 10, r1 = 1, T = 1; };
+template <> struct FftPlan<12>   { static constexpr int S = 1, r0 = 12, r1 = 1, T = 1; };
+template <> struct FftPlan<16>   { static constexpr int S = 2, r0 = 4, r1 = 1, T = 4; };
+template <> struct FftPlan<20>   { static constexpr int S = 3, r0 = 2, r1 = 5, T = 5; };
+template <> struct FftPlan<24>   { static constexpr int S = 3, r0 = 2, r1 = 6, T = 3; };
+template <> struct FftPlan<32>   { static constexpr int S = 3, r0 = 4, r1 = 2, T = 8; };
+template <> struct FftPlan<40>   { static constexpr int S = 3, r0 = 2, r1 = 10, T = 5; };
+template <> struct FftPlan<48>   { static constexpr int S = 3, r0 = 4, r1 = 3, T = 6; };
+template <> struct FftPlan<64>   { static constexpr int S = 2, r0 = 8, r1 = 1, T = 8; };
+template <> struct FftPlan<80>   { static constexpr int S = 3, r0 = 4, r1 = 5, T = 10; };
+template <> struct FftPlan<96>   { static constexpr int S = 3, r0 = 4, r1 = 6, T = 12; };
+template <> struct FftPlan<128>  { static constexpr int S = 3, r0 = 4, r1 = 8, T = 16; };
+template <> struct FftPlan<160>  { static constexpr int S = 3, r0 = 4, r1 = 10, T = 20; };
+template <> struct FftPlan<192>  { static constexpr int S = 3, r0 = 8, r1 = 3, T = 24; };
+template <> struct FftPlan<256>  { static constexpr int S = 3, r0 = 8, r1 = 4, T = 32; };
+template <> struct FftPlan<320>  { static constexpr int S = 3, r0 = 8, r1 = 5, T = 40; };
+template <> struct FftPlan<384>  { static constexpr int S = 3, r0 = 8, r1 = 6, T = 48; };
+template <> struct FftPlan<512>  { static constexpr int S = 3, r0 = 8, r1 = 8, T = 64; };
+template <> struct FftPlan<640>  { static constexpr int S = 3, r0 = 8, r1 = 10, T = 80; };
+template <> struct FftPlan<768>  { static constexpr int S = 3, r0 = 8, r1 = 12, T = 96; };
+template <> struct FftPlan<1024> { static constexpr int S = 3, r0 = 16, r1 = 4, T = 64; };
 // Alternative plans (PL = 1): fewer lanes per sequence and more elements per
 // lane, so a 256-thread block spans twice the columns (128-byte row segments
 // of FP32 complex values in the Y-passes).  Where none is defined the

@@ -14,6 +14,8 @@ struct EtdView {
 };
 
 bool fft_size_supported(int n);
+// lengths compiled for every tile variant (the others run the base variant)
+bool fft_variants_supported(int n);
 
 template <typename R, int V>
 struct OpsV {

@@ -1,6 +1,7 @@
 // FP32 instantiation of the fused FFT-pass kernels (mixed mode: FP32
-// transforms and products, FP64 state, column solves and ETD coefficients),
-// tile variant 1.
+// storage, transforms and products; FP64 column solves and ETD
+// coefficients), tile variant 1.
+#define QG_ALL_LENGTHS 1
 #include "qg_ops_impl.hpp"
 
 namespace qg {

@@ -11,6 +11,31 @@
 
 namespace qg {
 
+// The transform lengths compiled.  Every tile variant has the lengths whose
+// variants the autotuner explores (8, 16, 32, 64, 128, 256, 384, 512); the
+// base variant (the translation units that define QG_ALL_LENGTHS) also has the
+// other even products of the radices 2, 3, 4, 5, 6, 8, 10, 12, 16 with a plan
+// in qg_fft.hpp, which run at the base variant only (fft_size_supported,
+// fft_variants_supported).
+#ifdef QG_ALL_LENGTHS
+#define QG_MORE_LENGTHS(CALL)                                                 \
+    case 10: CALL(10); break;                                                 \
+    case 12: CALL(12); break;                                                 \
+    case 20: CALL(20); break;                                                 \
+    case 24: CALL(24); break;                                                 \
+    case 40: CALL(40); break;                                                 \
+    case 48: CALL(48); break;                                                 \
+    case 80: CALL(80); break;                                                 \
+    case 96: CALL(96); break;                                                 \
+    case 160: CALL(160); break;                                               \
+    case 192: CALL(192); break;                                               \
+    case 320: CALL(320); break;                                               \
+    case 640: CALL(640); break;                                               \
+    case 768: CALL(768); break;                                               \
+    case 1024: CALL(1024); break;
+#else
+#define QG_MORE_LENGTHS(CALL)
+#endif
 #define QG_SWITCH(n, CALL)                                                    \
   switch (n) {                                                                \
     case 8: CALL(8); break;                                                   \
@@ -21,7 +46,9 @@ namespace qg {
     case 256: CALL(256); break;                                               \
     case 384: CALL(384); break;                                               \
     case 512: CALL(512); break;                                               \
-    default: throw std::runtime_error("unsupported transform length " + std::to_string(n)); \
+    QG_MORE_LENGTHS(CALL)                                                     \
+    default: throw std::runtime_error("transform length " + std::to_string(n) + \
+                                      " not compiled for this tile variant"); \
   }
 
 template <int N>

@@ -710,7 +710,8 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
       throw QgError(QG_ERR_ARGUMENT, "grid dimensions must be positive");
     if (!qg::fft_size_supported(cfg->nx) || !qg::fft_size_supported(cfg->ny))
       throw QgError(QG_ERR_UNSUPPORTED,
-                    "horizontal sizes must be one of 8,16,32,64,128,256,384,512 (got " +
+                    "horizontal sizes must be one of 8,10,12,16,20,24,32,40,48,64,80,96,128,160,192,256,320,"
+                    "384,512,640,768,1024 (got " +
                         std::to_string(cfg->nx) + "x" + std::to_string(cfg->ny) + ")");
     if (!qg::column_depth_supported(cfg->nz))
       throw QgError(QG_ERR_UNSUPPORTED, "nz must be between 1 and 1024 for the column solvers");
@@ -1645,9 +1646,15 @@ std::vector<int> column_candidates(int nzc) {
   return c;
 }
 
+// FFT tile variants other than the base are compiled for the lengths of
+// fft_variants_supported only.
+bool fft_variants_ok(const qg_handle& h) {
+  return qg::fft_variants_supported(h.nx) && qg::fft_variants_supported(h.ny);
+}
+
 bool valid_choice(const qg_handle& h, int pass, int v) {
   if (v == -1) return true;
-  if (pass < qg::P_FFT_END) return v >= 0 && v < qg::kTileVariants;
+  if (pass < qg::P_FFT_END) return v >= 0 && v < qg::kTileVariants && (v == 1 || fft_variants_ok(h));
   return v >= 0 && v < qg::kScanShapeCount && qg::scan_shape_fits(qg::scan_shape_of(v, h.nzc));
 }
 
@@ -1707,7 +1714,7 @@ void autotune(qg_handle& H, double dt, int steps) {
   std::fill(dflt, dflt + qg::P_END, std::numeric_limits<double>::infinity());
   std::fill(choice, choice + qg::P_END, -1);
   for (int c = 0; c < ncfg; ++c) {
-    const int fv = c < qg::kTileVariants ? c : 1;
+    const int fv = (c < qg::kTileVariants && fft_variants_ok(H)) ? c : 1;
     const int cs = c < (int)shapes.size() ? shapes[c] : qg::scan_default_shape(H.nzc);
     for (int p = 0; p < qg::P_FFT_END; ++p) H.tune[p] = fv;
     for (int p = qg::P_FFT_END; p < qg::P_END; ++p) H.tune[p] = cs;
@@ -1736,7 +1743,8 @@ void autotune(qg_handle& H, double dt, int steps) {
     H.prof.clear();
     H.pool_used = 0;
     for (int p = 0; p < qg::P_END; ++p) {
-      const bool attributable = p < qg::P_FFT_END ? c < qg::kTileVariants : c < (int)shapes.size();
+      const bool attributable =
+          p < qg::P_FFT_END ? (c < qg::kTileVariants && fft_variants_ok(H)) : c < (int)shapes.size();
       if (!attributable || ms[p].empty()) continue;
       std::vector<float>& v = ms[p];
       std::nth_element(v.begin(), v.begin() + v.size() / 2, v.end());

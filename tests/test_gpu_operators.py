@@ -15,7 +15,11 @@ EXP_N2 = lambda z: 1e-4 * math.exp(z / 800.0)  # noqa: E731
 
 
 @pytest.mark.parametrize("size", [(8, 8, 4), (32, 16, 3), (64, 64, 5), (128, 128, 2),
-                                  (256, 256, 2), (384, 384, 1), (512, 512, 1)])
+                                  (256, 256, 2), (384, 384, 1), (512, 512, 1),
+                                  # the lengths compiled for the base tile variant only,
+                                  # radix-5 plans among them
+                                  (10, 12, 3), (20, 24, 2), (40, 48, 2), (80, 96, 2), (160, 192, 1),
+                                  (320, 640, 1), (768, 1024, 1)])
 def test_fft_matches_dft_and_round_trips(qg, size):
     """KAT test/test_mpi_extension.jl:62-90 (round trip < 1e-12) and the
     transform contract of src/transforms.jl:19-23 against numpy's DFT."""

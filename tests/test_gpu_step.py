@@ -196,11 +196,14 @@ def test_mixed_precision_error_is_stated(qg):
     m32.finalize()
 
 
-def test_radix3_grid_step_matches_oracle(qg):
-    """384x384x8: the radix-3 transform plan (8*6*8) inside the whole step,
-    2 steps against the oracle (the serving workload draws this size)."""
-    c = Case(size=(384, 384, 8), extent=(500e3, 500e3, 4000.0), N2_func=EXP_N2, feedback=True)
-    print("case step 384x384x8", flush=True)
+@pytest.mark.parametrize("size", [(384, 384, 8), (96, 96, 12), (160, 80, 6), (1024, 40, 3)])
+def test_other_plans_step_matches_oracle(qg, size):
+    """The whole step on grids whose transforms use the radix-3 plans (384 =
+    8*6*8, the serving workload's largest grid; 96 = 4*6*4) and the radix-5
+    ones (160 = 4*10*4, 80 = 4*5*4, 40 = 2*10*2) and 1024 = 16*4*16, 2 steps
+    against the oracle."""
+    c = Case(size=size, extent=(500e3, 500e3, 4000.0), N2_func=EXP_N2, feedback=True)
+    print(f"case step {size}", flush=True)
     m, o = gpu_model(qg, c), oracle_model(c)
     psi = k3_streamfunction(o.grid, seed=21, urms=0.1, kpeak=6.0)
     X, Y = np.meshgrid(o.grid.x, o.grid.y, indexing="ij")
@@ -213,6 +216,6 @@ def test_radix3_grid_step_matches_oracle(qg):
         o.step(60.0)
     for n in FIELDS + ("C", "u", "v", "w"):
         e = rel(getattr(m.fields, n), getattr(o.fields, n))
-        print(f"  384 {n}: {e:.2e}", flush=True)
+        print(f"  {size} {n}: {e:.2e}", flush=True)
         assert e <= (1e-9 if n == "w" else 1e-11), (n, e)
     m.finalize()

@@ -128,3 +128,25 @@ def test_autotune_restores_state_and_caches(qg):
         u.set_tuning({"thomas_psi": 99})
     m.finalize()
     u.finalize()
+
+
+def test_autotune_on_base_only_lengths(qg):
+    """96 x 80 has no tile variants compiled: the tuner keeps every FFT pass
+    at the base variant, still tunes the column shapes, refuses other FFT
+    variants, and the tuned model steps as the untuned one to round-off."""
+    print("case autotune 96x80", flush=True)
+    m = _model(qg, (96, 80, 16), "mixed")
+    _init(qg, m)
+    choice = m.autotune(dt=300.0, steps=2, force=True)
+    assert all(choice[p] == -1 for p in FFT), choice
+    with pytest.raises(ValueError):
+        m.set_tuning({"ya_qw": 3})
+    u = _model(qg, (96, 80, 16), "mixed")
+    _init(qg, u)
+    ts = qg.ExponentialRungeKutta2(dt=300.0)
+    qg.step_(m, ts, 4)
+    qg.step_(u, ts, 4)
+    for n in ("q", "B", "psi", "A"):
+        assert _rel(m._h.get_field(n), u._h.get_field(n)) <= 1e-6, n
+    m.finalize()
+    u.finalize()
