@@ -49,7 +49,7 @@ LAUNCH_KERNEL = {
     "xb_all": ("k_xb_all", "xb_B", None, 0), "xb_all_uv": ("k_xb_all", "xb_B", None, 1),
     "yb2_q_stage": ("k_yb2_q", "yb2_q", "stage"), "yb2_q_final": ("k_yb2_q", "yb2_q", "final"),
     "yb2_B_stage": ("k_yb2_B", "yb2_B", "stage"), "yb2_B_final": ("k_yb2_B", "yb2_B", "final"),
-    "yf_omega": ("k_yf_omega", "yf_omega", None, 1), "xf": ("k_xf", "xf", None),
+    "yf_omega": ("k_yf_omega", "yf_omega", None, 1, 0), "xf": ("k_xf", "xf", None, 1),
     "y_r2c_omega": ("k_y_r2c_mask", "y_r2c", None), "y_herm_w": ("k_y_herm", "y_herm", None),
     "thomas_psi": ("k_colscan<0", "thomas_psi", None), "thomas_A": ("k_colscan<1", "thomas_A", None),
     "thomas_AC": ("k_colscan<2", "thomas_AC", None), "omega_gtsv": ("k_colscan<3", "omega", None),

@@ -12,9 +12,8 @@
 // x[i + t*N/r], multiplies by w_{rP}^{t*(i mod P)}, applies an r-point DFT and
 // writes y[(i - i mod P)*r + i mod P + t*P].
 //
-// A lane's butterflies are the same in every transform it runs, so its stage
-// twiddles are loaded once per kernel (TwReg) and reused by every forward and
-// inverse transform of that kernel.
+// The stage twiddles are staged once per block in LDS (TwTab) and read by
+// every forward and inverse transform of the kernel.
 #pragma once
 
 #include "qg_common.hpp"
@@ -206,43 +205,84 @@ template <> struct FftPlan<1024> { static constexpr int S = 3, r0 = 16, r1 = 4, 
 template <int N> struct FftPlan<N, 1> : FftPlan<N, 0> {};
 template <> struct FftPlan<256, 1> { static constexpr int S = 2, r0 = 16, r1 = 1, T = 16; };
 
-// Stage twiddles of one lane, forward sign; the inverse uses the conjugates.
-template <typename R, int N, int PL = 0>
-struct TwReg {
+// Stage twiddles, forward sign (the inverse uses the conjugates).  By
+// default they are staged once per block in LDS from the length-N table, in
+// the layout the stages read: the middle stage's w^(t k step) as [t-1][k],
+// k < R0, and the last stage's w^(t i) as [t-1][i], i < N/R0, so the lanes of
+// a stage read consecutive words (no bank conflicts) and no lane keeps
+// twiddles in registers.  Every thread of the block calls load() at the start
+// of the kernel; the first barrier of the first transform orders the fill
+// before any read.  REG = true keeps each lane's own stage twiddles in
+// registers instead (the passes that measure faster so).
+template <typename R, int N, int PL = 0, bool REG = false>
+struct TwTab {
   using P = FftPlan<N, PL>;
   static constexpr int R0 = P::r0, R1 = P::r1, T = P::T;
   static constexpr int NB = N / R0;
+  static constexpr int MID = (P::S == 3) ? (R1 - 1) * R0 : 0;
+  static constexpr int LAST = (P::S >= 2) ? (R0 - 1) * NB : 0;
+  static constexpr int SIZE = MID + LAST;
+  static constexpr int STEP = (P::S == 3) ? N / (R1 * R0) : 1;
+  // register form: per lane, [it][t-1] of the stages it runs
   static constexpr int IT = (NB + T - 1) / T;
   static constexpr int NB1 = N / R1;
   static constexpr int IT1 = (NB1 + T - 1) / T;
-  static constexpr int NMID = (P::S == 3) ? IT1 * (R1 - 1) : 1;
-  static constexpr int NLAST = (P::S >= 2) ? IT * (R0 - 1) : 1;
-  cplx<R> mid[NMID];
-  cplx<R> last[NLAST];
+  static constexpr int NMID = (REG && P::S == 3) ? IT1 * (R1 - 1) : 1;
+  static constexpr int NLAST = (REG && P::S >= 2) ? IT * (R0 - 1) : 1;
+  const cplx<R>* tab;
+  cplx<R> rmid[NMID];
+  cplx<R> rlast[NLAST];
 
   // tw: length-N table, tw[m] = exp(-2*pi*i*m/N)
   __device__ inline void load(int lane, const cplx<R>* __restrict__ tw) {
-    if constexpr (P::S == 3) {
-      constexpr int step = N / (R1 * R0);
+    if constexpr (REG) {
+      if constexpr (P::S == 3) {
 #pragma unroll
-      for (int it = 0; it < IT1; ++it) {
-        const int i = lane + it * T;
-        const int k = i % R0;
+        for (int it = 0; it < IT1; ++it) {
+          const int i = lane + it * T;
 #pragma unroll
-        for (int t = 1; t < R1; ++t)
-          mid[it * (R1 - 1) + t - 1] = (NB1 % T == 0 || i < NB1) ? tw[t * k * step] : mk<R>((R)1, (R)0);
+          for (int t = 1; t < R1; ++t)
+            rmid[it * (R1 - 1) + t - 1] = (NB1 % T == 0 || i < NB1) ? tw[t * (i % R0) * STEP] : mk<R>((R)1, (R)0);
+        }
       }
-    }
-    if constexpr (P::S >= 2) {
+      if constexpr (P::S >= 2) {
 #pragma unroll
-      for (int it = 0; it < IT; ++it) {
-        const int i = lane + it * T;  // span NB, k = i, step = 1
+        for (int it = 0; it < IT; ++it) {
+          const int i = lane + it * T;
 #pragma unroll
-        for (int t = 1; t < R0; ++t)
-          last[it * (R0 - 1) + t - 1] = (NB % T == 0 || i < NB) ? tw[t * i] : mk<R>((R)1, (R)0);
+          for (int t = 1; t < R0; ++t)
+            rlast[it * (R0 - 1) + t - 1] = (NB % T == 0 || i < NB) ? tw[t * i] : mk<R>((R)1, (R)0);
+        }
       }
+      tab = nullptr;
+    } else {
+      __shared__ cplx<R> s[SIZE > 0 ? SIZE : 1];
+      for (int e = threadIdx.x; e < SIZE; e += blockDim.x) {
+        int m;
+        if (e < MID) {
+          m = (e / R0 + 1) * (e % R0) * STEP;
+        } else {
+          const int f = e - MID;
+          m = (f / NB + 1) * (f % NB);
+        }
+        s[e] = tw[m];
+      }
+      tab = s;
     }
   }
+  // middle stage: w^(t k step), 1 <= t < R1, k = i mod R0 of the lane's
+  // butterfly i = lane + it T
+  __device__ inline cplx<R> mid(int it, int t, int k) const {
+    if constexpr (REG) return rmid[it * (R1 - 1) + t - 1];
+    else return tab[(t - 1) * R0 + k];
+  }
+  // last stage: w^(t i), 1 <= t < R0, i = lane + it T < N/R0
+  __device__ inline cplx<R> last(int it, int t, int i) const {
+    if constexpr (REG) return rlast[it * (R0 - 1) + t - 1];
+    else return tab[MID + (t - 1) * NB + i];
+  }
+  // LDS bytes of the staged table
+  static constexpr int bytes() { return (SIZE > 0 ? SIZE : 1) * (int)sizeof(cplx<R>); }
 };
 
 template <typename R, int DIR>
@@ -271,7 +311,8 @@ struct Fft {
 
   // s: this sequence's LDS (LDSZ elements).  Contains barriers: every thread
   // of the block must call it the same number of times.
-  __device__ static inline void run(cplx<R> (&v)[IT][R0], cplx<R>* s, int lane, const TwReg<R, N, PL>& w) {
+  template <class TW>
+  __device__ static inline void run(cplx<R> (&v)[IT][R0], cplx<R>* s, int lane, const TW& w) {
     if constexpr (P::S == 1) {
 #pragma unroll
       for (int it = 0; it < IT; ++it)
@@ -293,12 +334,15 @@ struct Fft {
         constexpr int NB1 = N / R1;
         constexpr int IT1 = (NB1 + T - 1) / T;
         cplx<R> u[IT1][R1];
+        cplx<R> wm[IT1][R1];  // the stage's twiddles, read with the data (before the barrier)
 #pragma unroll
         for (int it = 0; it < IT1; ++it) {
           int i = lane + it * T;
           if (NB1 % T == 0 || i < NB1) {
 #pragma unroll
             for (int t = 0; t < R1; ++t) u[it][t] = s[pad(i + t * NB1)];
+#pragma unroll
+            for (int t = 1; t < R1; ++t) wm[it][t] = w.mid(it, t, i % R0);
           }
         }
         __syncthreads();
@@ -308,7 +352,7 @@ struct Fft {
           if (NB1 % T == 0 || i < NB1) {
             int k = i % R0;
 #pragma unroll
-            for (int t = 1; t < R1; ++t) u[it][t] = twmul<R, DIR>(u[it][t], w.mid[it * (R1 - 1) + t - 1]);
+            for (int t = 1; t < R1; ++t) u[it][t] = twmul<R, DIR>(u[it][t], wm[it][t]);
             Dft<R1, DIR>::run(u[it]);
             int j = (i - k) * R1 + k;
 #pragma unroll
@@ -325,7 +369,7 @@ struct Fft {
 #pragma unroll
           for (int t = 0; t < R0; ++t) v[it][t] = s[pad(i + t * NB)];
 #pragma unroll
-          for (int t = 1; t < R0; ++t) v[it][t] = twmul<R, DIR>(v[it][t], w.last[it * (R0 - 1) + t - 1]);
+          for (int t = 1; t < R0; ++t) v[it][t] = twmul<R, DIR>(v[it][t], w.last(it, t, i));
           Dft<R0, DIR>::run(v[it]);
         }
       }

@@ -37,7 +37,8 @@ struct Tile {
   static constexpr int S0 = TARGET / T;
   static constexpr int S1 = S0 < 1 ? 1 : (S0 > 64 ? 64 : S0);
   static constexpr int S2 = S1 * T > 512 ? (512 / T < 1 ? 1 : 512 / T) : S1;
-  static constexpr int SL = (int)(65536 / (LD * sizeof(cplx<R>)));
+  // sequences that fit 64 KB of LDS beside the staged twiddle table
+  static constexpr int SL = (int)((65536 - TwTab<R, N, PL>::bytes()) / (LD * sizeof(cplx<R>)));
   static constexpr int S = S2 > SL ? (SL < 1 ? 1 : SL) : S2;
   static constexpr int THREADS = S * T;
   static constexpr int NN = N;
@@ -74,6 +75,51 @@ __device__ inline cplx<R> keep_if(cplx<R> v, bool k) {
   return mk<R>(keep_bits(v.re, k), keep_bits(v.im, k));
 }
 
+// p with its value hidden from the optimizer: loads through the result are
+// neither merged with earlier loads of the same addresses nor hoisted above
+// the point where it is taken (the empty asm produces no instruction).
+template <typename T>
+__device__ inline const T* opaque(const T* p) {
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
+template <typename R>
+__device__ inline bool finite_c(cplx<R> a) {
+  return __builtin_isfinite(a.re) && __builtin_isfinite(a.im);
+}
+
+// Anti-Hermitian part (X(k) - conj X(-k))/2, pre-scaled by `s`.
+template <typename T>
+__device__ inline cplx<T> aherm(cplx<T> a, cplx<T> am, T s) {
+  return mk<T>((T)0.5 * s * (a.re - am.re), (T)0.5 * s * (a.im + am.im));
+}
+
+// Folds a block's largest value into absmax (the bit pattern of a
+// non-negative double; NaN counts as +inf): waves by shuffles, then LDS, and
+// at most one atomic per block, skipped when the accumulator already holds
+// more (thousands of blocks contend for one address otherwise).  Every
+// thread of the block calls it.
+template <int THREADS>
+__device__ inline void fold_block_max(double m, unsigned long long* __restrict__ absmax) {
+  if (m != m) m = __builtin_huge_val();
+  for (int o = 32; o > 0; o >>= 1) {
+    const double w = __shfl_xor(m, o);
+    m = w > m ? w : m;
+  }
+  constexpr int NW = (THREADS + 63) / 64;
+  __shared__ double wmax[NW];
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = 0.0;
+    for (int i = 0; i < NW; ++i) b = wmax[i] > b ? wmax[i] : b;
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(b);
+    if (b > 0.0 && bits > __hip_atomic_load(absmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMax(absmax, bits);
+  }
+}
+
 // ===========================================================================
 // Wave feedback q_w  (compute_qw_complex!, nonlinear.jl:859-944)
 // ===========================================================================
@@ -87,7 +133,7 @@ k_ya_qw(Geo g, const cplx<R>* __restrict__ B, cplx<R>* __restrict__ T1,
   using F = Fft<R, NY, +1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
-  TwReg<R, TL::NN, TL::PL> twr;
+  TwTab<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const Blk blk = xcd_block();
   const int x = blk.x * TL::S + s, z = blk.y;
@@ -133,7 +179,7 @@ k_xa_qw(Geo g, const cplx<R>* __restrict__ T1, const cplx<R>* __restrict__ T3,
   using FF = Fft<R, NX, -1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
-  TwReg<R, TL::NN, TL::PL> twr;
+  TwTab<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const int row = blockIdx.x * TL::S + r;
   const bool ok = row < g.nz * g.ny;
@@ -198,7 +244,7 @@ k_ya2_qeff(Geo g, const cplx<R>* __restrict__ Qa, const cplx<R>* __restrict__ Qb
   using F = Fft<R, NY, -1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
-  TwReg<R, TL::NN, TL::PL> twr;
+  TwTab<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const Blk blk = xcd_block();
   const int x = blk.x * TL::S + s, z = blk.y;
@@ -263,7 +309,7 @@ k_yb_adv(Geo g, int outs_rt, const cplx<R>* __restrict__ psi, const cplx<R>* __r
   const int outs = OUTS ? OUTS : outs_rt;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
-  TwReg<R, TL::NN, TL::PL> twr;
+  TwTab<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const Blk blk = xcd_block();
   const int x = blk.x * TL::S + s, z = blk.y;
@@ -295,12 +341,20 @@ k_yb_adv(Geo g, int outs_rt, const cplx<R>* __restrict__ psi, const cplx<R>* __r
       for (int t = 0; t < F::R0; ++t) {
         int y = F::elem(lane, it, t), my = mirror(y, NY);
         cplx<R> a, am;
+        // u and v as k_yf_omega forms them (the same values whichever pass
+        // makes them): the multiplier times the Hermitian part of psi, or
+        // its anti-Hermitian part where the multiplier's wavenumber is its
+        // own mirror (ky at the row ny/2, kx at the column nx/2)
         if (o == 0) {
-          a = ik_mul((R)-g.ky[y], p[it][t]);
-          am = ik_mul((R)-g.ky[my], pm[it][t]);
+          const cplx<R> f = y == NY / 2 ? aherm(p[it][t], pm[it][t], (R)g.inv_n)
+                                        : herm(p[it][t], pm[it][t], (R)g.inv_n);
+          v[it][t] = ik_mul((R)-g.ky[y], f);
+          continue;
         } else if (o == 1) {
-          a = ik_mul((R)kxv, p[it][t]);
-          am = ik_mul((R)kxm, pm[it][t]);
+          const cplx<R> f = (mx == x && x != 0) ? aherm(p[it][t], pm[it][t], (R)g.inv_n)
+                                                : herm(p[it][t], pm[it][t], (R)g.inv_n);
+          v[it][t] = ik_mul((R)kxv, f);
+          continue;
         } else if (o == 2) {
           // loaded here, under the mask, rather than with psi: as unconditional
           // loads they would be hoisted to the kernel's start and cost a wave
@@ -382,7 +436,7 @@ k_x_c2r(Geo g, int nfields, const cplx<R>* __restrict__ P0, const cplx<R>* __res
   using FI = Fft<R, NX, +1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
-  TwReg<R, TL::NN, TL::PL> twr;
+  TwTab<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const int row = blockIdx.x * TL::S + r;
   const bool ok = row < g.nz * g.ny;
@@ -420,7 +474,7 @@ k_xb_q(Geo g, const cplx<R>* __restrict__ Pq, const cplx<R>* __restrict__ Pu, co
   using FF = Fft<R, NX, -1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
-  TwReg<R, TL::NN, TL::PL> twr;
+  TwTab<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const int row = blockIdx.x * TL::S + r;
   const bool ok = row < g.nz * g.ny;
@@ -504,7 +558,7 @@ k_xb_B(Geo g, int flags, const cplx<R>* __restrict__ T1, const cplx<R>* __restri
   using FF = Fft<R, NX, -1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
-  TwReg<R, TL::NN, TL::PL> twr;
+  TwTab<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const int row = blockIdx.x * TL::S + r;
   const bool ok = row < g.nz * g.ny;
@@ -583,7 +637,7 @@ k_xb_all(Geo g, const cplx<R>* __restrict__ Pq, const cplx<R>* __restrict__ Pz, 
   using FF = Fft<R, NX, -1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
-  TwReg<R, TL::NN, TL::PL> twr;
+  TwTab<R, TL::NN, TL::PL, sizeof(R) == 8> twr;  // FP64: register twiddles (measured faster)
   twr.load(lane, tw);
   const int row = blockIdx.x * TL::S + r;
   const bool ok = row < g.nz * g.ny;
@@ -729,13 +783,13 @@ __global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
 k_yb2_q(Geo g, int mode, int flags, const cplx<R>* __restrict__ G1, const cplx<R>* __restrict__ G2,
         const cplx<R>* __restrict__ qd, const cplx<R>* __restrict__ qn,
         cplx<R>* __restrict__ rhs0,
-        cplx<R>* __restrict__ qout, Etd etd, double nu, double inv_dz2,
+        cplx<R>* __restrict__ qout, Etd etd, double nu, double inv_dz2, int* __restrict__ flag,
         const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY, V>;
   using F = Fft<R, NY, -1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
-  TwReg<R, TL::NN, TL::PL> twr;
+  TwTab<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const Blk blk = xcd_block();
   const int x = blk.x * TL::S + s, z = blk.y;
@@ -758,6 +812,7 @@ k_yb2_q(Geo g, int mode, int flags, const cplx<R>* __restrict__ G1, const cplx<R
   }
   const int mx = mirror(x, g.nx);
   const size_t plane = (size_t)g.nx * NY;
+  bool fin = true;
 #pragma unroll
   for (int it = 0; it < F::IT; ++it)
 #pragma unroll
@@ -805,9 +860,14 @@ k_yb2_q(Geo g, int mode, int flags, const cplx<R>* __restrict__ G1, const cplx<R
         } else if (mode == MODE_STAGE) {
           rhs0[ad] = czero<R>();
         }
-        qout[ad] = cvt<R>(out);
+        const cplx<R> qs = cvt<R>(out);
+        qout[ad] = qs;
+        fin = fin && finite_c(qs);
       }
     }
+  // flag: the termination check's non-finite test of the new q (every point
+  // is written above, the mirror (-x, -y) included)
+  if (flag && !fin) atomicOr(flag, 1);
 }
 
 // Y-pass (forward, full width) of G3 = -i kx F_x(uB) - (i/2) F_x(zeta B) and
@@ -820,12 +880,12 @@ __global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
 k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R>* __restrict__ G4,
         const cplx<R>* __restrict__ A, const cplx<R>* __restrict__ Bn, cplx<R>* __restrict__ rhs0,
         cplx<R>* __restrict__ Bout, cplx<R>* __restrict__ T1, cplx<R>* __restrict__ T3, Etd etd, double alpha,
-        const cplx<R>* __restrict__ tw) {
+        int* __restrict__ flag, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY, V>;
   using F = Fft<R, NY, -1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
-  TwReg<R, TL::NN, TL::PL> twr;
+  TwTab<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const Blk blk = xcd_block();
   const int x = blk.x * TL::S + s, z = blk.y;
@@ -886,6 +946,7 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
         bv[it][t] = ldu(base, ad, in);
       }
   }
+  bool fin = true;
 #pragma unroll
   for (int it = 0; it < F::IT; ++it)
 #pragma unroll
@@ -925,7 +986,10 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
       }
       v[it][t] = cvt<R>(out);
       Bout[ad] = v[it][t];
+      fin = fin && finite_c(v[it][t]);
     }
+  // flag: the termination check's non-finite test of the new B
+  if (flag && !fin) atomicOr(flag, 1);
   // The next pass over these columns would be the inverse y transform of the
   // new B (ya_qw: T1 = IFFTy(mask B / (nx ny)), T3 = IFFTy(i ky mask B /
   // (nx ny))); it is made here from the stored values, which the column
@@ -968,91 +1032,174 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
 // (_omega_eqn_rhs_direct!, src/diagnostics.jl:149-229): psi_z (forward
 // difference to the level above, zero at the top) times i*kx and i*ky, and
 // the two-level average psi-bar (psi itself at the top) times -i*kx*kh2 and
-// -i*ky*kh2.  psi(z) and psi(z+1) are read once; the four Hermitian inputs are
-// formed in registers before the four transforms.
+// -i*ky*kh2.
 // UV = 1: the same pass also makes the y-transformed half rows of u and v
 // (what k_yb_adv writes for OUT_U | OUT_V) from the psi it holds.
-template <typename R, int NY, int V, int UV>
-__global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
+// CHK = 1: also the y-transformed half rows of psi itself, Ppsi (the
+// termination check's max |ifft2(psi)|, folded in by the X-pass), and psi is
+// tested for non-finite values at every point (flag).
+//
+// Every output is the Hermitian part of a multiplier times psi, psi_z or
+// psi-bar at (x, y) and its mirror (-x, -y).  Each multiplier is odd in the
+// one wavenumber it carries (kx or ky; kh2 is even), so the output is the
+// multiplier times the Hermitian part of the field, except on the column or
+// row that is its own mirror (kx or ky at the Nyquist index), where it is the
+// multiplier times the anti-Hermitian part.  The pass keeps only the two
+// parts' sources, psi(z) and psi(z+1) (the difference and the average follow
+// by linearity), as Hermitian parts, plus their anti-Hermitian parts at the
+// row ny/2 (register slot it = 0, t = R0/2 of lane 0), and forms, transforms
+// and stores one output at a time.  The kx outputs of the Nyquist column
+// (x = nx/2) come from one extra tile column of blocks (blk.x == the number
+// of tiles), whose sources are the anti-Hermitian parts of that column.
+template <typename R, int NY, int V, int UV, int CHK>
+__global__ void __launch_bounds__((Tile<R, NY, V>::THREADS), (sizeof(R) == 4 && V != 3 ? 4 : 1))
 k_yf_omega(Geo g, const cplx<R>* __restrict__ psi, cplx<R>* __restrict__ Pbx, cplx<R>* __restrict__ Pby,
            cplx<R>* __restrict__ Pxx, cplx<R>* __restrict__ Pxy, cplx<R>* __restrict__ Pu,
-           cplx<R>* __restrict__ Pv, const cplx<R>* __restrict__ tw) {
+           cplx<R>* __restrict__ Pv, cplx<R>* __restrict__ Ppsi, int* __restrict__ flag,
+           const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY, V>;
   using F = Fft<R, NY, +1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
-  TwReg<R, TL::NN, TL::PL> twr;
+  TwTab<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const Blk blk = xcd_block();
-  const int x = blk.x * TL::S + s, z = blk.y;
-  const bool ok = x < g.W;
+  const int ntile = (g.W + TL::S - 1) / TL::S;
+  const bool nyq = blk.x == ntile;  // the Nyquist column's kx outputs (uniform)
+  const int x = nyq ? g.nx / 2 : blk.x * TL::S + s, z = blk.y;
+  const bool ok = nyq ? s == 0 : x < g.W;
   const int mx = mirror(x, g.nx);
+  const bool xnyq = mx == x && x != 0;  // the Nyquist column: its own mirror, kx != 0
   cplx<R>* sl = lds + s * TL::LD;
-  const double kxv = ok ? g.kx[x] : 0.0, kxm = ok ? g.kx[mx] : 0.0;
+  const double kxv = ok ? g.kx[x] : 0.0;
   const bool top = z % g.nzc == g.nzc - 1;  // top level of this member's column
   const int zu = top ? z : z + 1;
-  const double dz = g.dz;
-  constexpr int NO = UV ? 6 : 4;
-  cplx<R> v[NO][F::IT][F::R0];
+  const R rdz = (R)g.dz, rinv = (R)g.inv_n;
+  constexpr int TN = F::R0 / 2;  // register slot of the row ny/2 (it = 0, lane 0)
+  // psi(z), psi(zu) / (nx ny): Hermitian parts (anti-Hermitian in the
+  // Nyquist blocks), and the anti-Hermitian parts at the row ny/2
+  cplx<R> h0[F::IT][F::R0], h1[F::IT][F::R0];
+  cplx<R> a0 = czero<R>(), a1 = czero<R>();
+  bool fin = true;
 #pragma unroll
   for (int it = 0; it < F::IT; ++it)
 #pragma unroll
     for (int t = 0; t < F::R0; ++t) {
       const int y = F::elem(lane, it, t), my = mirror(y, NY);
       const bool in = ok && F::valid(lane, it);
-      // the four factors are formed at the transform precision R
       const cplx<R> p = ldu(psi, ((size_t)z * NY + y) * g.nx + x, in);
       const cplx<R> pm = ldu(psi, ((size_t)z * NY + my) * g.nx + mx, in);
-      if constexpr (UV) {
-        v[4][it][t] = herm(ik_mul((R)-g.ky[y], p), ik_mul((R)-g.ky[my], pm), (R)g.inv_n);
-        v[5][it][t] = herm(ik_mul((R)kxv, p), ik_mul((R)kxm, pm), (R)g.inv_n);
-      }
       const cplx<R> up = ldu(psi, ((size_t)zu * NY + y) * g.nx + x, in);
       const cplx<R> upm = ldu(psi, ((size_t)zu * NY + my) * g.nx + mx, in);
-      const R rdz = (R)dz;
-      // at the top level zu = z: the difference is exactly 0 and the average
-      // psi itself, with no branch for the loads of level zu to sink into
-      const cplx<R> pz = mk<R>((up.re - p.re) / rdz, (up.im - p.im) / rdz);
-      const cplx<R> pzm = mk<R>((upm.re - pm.re) / rdz, (upm.im - pm.im) / rdz);
-      const cplx<R> pa = scale(up + p, (R)0.5);
-      const cplx<R> pam = scale(upm + pm, (R)0.5);
-      const double ky = g.ky[y], kym = g.ky[my];
-      const double kh2 = kxv * kxv + ky * ky, kh2m = kxm * kxm + kym * kym;
-      const R rinv = (R)g.inv_n;
-      v[0][it][t] = herm(ik_mul((R)kxv, pz), ik_mul((R)kxm, pzm), rinv);
-      v[1][it][t] = herm(ik_mul((R)ky, pz), ik_mul((R)kym, pzm), rinv);
-      v[2][it][t] = herm(ik_mul((R)-(kxv * kh2), pa), ik_mul((R)-(kxm * kh2m), pam), rinv);
-      v[3][it][t] = herm(ik_mul((R)-(ky * kh2), pa), ik_mul((R)-(kym * kh2m), pam), rinv);
+      fin = fin && (!in || (finite_c(p) && finite_c(pm)));
+      h0[it][t] = nyq ? aherm(p, pm, rinv) : herm(p, pm, rinv);
+      h1[it][t] = nyq ? aherm(up, upm, rinv) : herm(up, upm, rinv);
+      if (it == 0 && t == TN) {
+        a0 = aherm(p, pm, rinv);
+        a1 = aherm(up, upm, rinv);
+      }
     }
+  if (CHK && !fin) atomicOr(flag, 1);
+  // psi_z = (psi(zu) - psi(z)) / dz (exactly 0 at the top, where zu = z) and
+  // psi-bar = (psi(zu) + psi(z)) / 2 (psi itself at the top)
+  auto dzf = [&](cplx<R> u1, cplx<R> u0) { return mk<R>((u1.re - u0.re) / rdz, (u1.im - u0.im) / rdz); };
+  auto avg = [&](cplx<R> u1, cplx<R> u0) { return scale(u1 + u0, (R)0.5); };
+  const double* __restrict__ gky = g.ky;
+  // outputs: 0 i kx psi_z, 1 i ky psi_z, 2 -i kx kh2 psi-bar, 3 -i ky kh2 psi-bar,
+  // 4 -i ky psi (u), 5 i kx psi (v), 6 psi
+  cplx<R> v[F::IT][F::R0];
 #pragma unroll
-  for (int o = 0; o < NO; ++o) {
-    cplx<R>* dst = o == 0 ? Pbx : o == 1 ? Pby : o == 2 ? Pxx : o == 3 ? Pxy : o == 4 ? Pu : Pv;
-    F::run(v[o], sl, lane, twr);
+  for (int o = 0; o < 7; ++o) {
+    const bool kxo = o == 0 || o == 2 || o == 5;  // multiplier odd in kx
+    if ((o == 4 || o == 5) && !UV) continue;
+    if (o == 6 && !CHK) continue;
+    if (nyq && !kxo) continue;  // uniform
+    // ky and kh2 loaded again for every output (kept, they would hold 32
+    // registers through every transform)
+    const double* __restrict__ ky_o = opaque(gky);
 #pragma unroll
     for (int it = 0; it < F::IT; ++it)
 #pragma unroll
       for (int t = 0; t < F::R0; ++t) {
         const int y = F::elem(lane, it, t);
-        if (ok && F::valid(lane, it)) dst[((size_t)z * NY + y) * g.Wp + x] = v[o][it][t];
+        const double ky = ky_o[y];
+        const double kh2 = kxv * kxv + ky * ky;
+        cplx<R> f1 = h1[it][t], f0 = h0[it][t];
+        if (!kxo && o != 6 && it == 0 && t == TN && lane == 0) {  // ky at the row ny/2
+          f1 = a1;
+          f0 = a0;
+        }
+        cplx<R> r;
+        switch (o) {
+          case 0: r = ik_mul((R)kxv, dzf(f1, f0)); break;
+          case 1: r = ik_mul((R)ky, dzf(f1, f0)); break;
+          case 2: r = ik_mul((R)(-(kxv * kh2)), avg(f1, f0)); break;
+          case 3: r = ik_mul((R)(-(ky * kh2)), avg(f1, f0)); break;
+          case 4: r = ik_mul((R)-ky, f0); break;
+          case 5: r = ik_mul((R)kxv, f0); break;
+          default: r = f0; break;
+        }
+        v[it][t] = r;
+      }
+    cplx<R>* dst = o == 0 ? Pbx : o == 1 ? Pby : o == 2 ? Pxx : o == 3 ? Pxy : o == 4 ? Pu : o == 5 ? Pv : Ppsi;
+    F::run(v, sl, lane, twr);
+    // the Nyquist column's kx outputs are stored by the Nyquist blocks
+    const bool st = ok && !(kxo && xnyq && !nyq);
+#pragma unroll
+    for (int it = 0; it < F::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < F::R0; ++t) {
+        const int y = F::elem(lane, it, t);
+        if (st && F::valid(lane, it)) dst[((size_t)z * NY + y) * g.Wp + x] = v[it][t];
       }
   }
+}
+
+// The termination check's max |ifft2(psi)| over one row: the inverse x
+// transform of the row's half spectrum (psi's Hermitian part, y-transformed
+// by k_yf_omega or k_y_check: ifft2 of a real field's spectrum), folded into
+// absmax.  Shared by k_xf and k_x_psi_max so that the fused and the stand-alone
+// check compute the same values.  Every thread of the block calls it.
+template <typename R, int NX, int THREADS, class FI, class TW>
+__device__ inline void psi_row_max(cplx<R> (&v)[FI::IT][FI::R0], const cplx<R>* __restrict__ Ppsi, size_t hb,
+                                   bool ok, int lane, cplx<R>* sl, const TW& twr,
+                                   unsigned long long* __restrict__ absmax) {
+  load_herm_row<R, FI>(v, Ppsi, hb, NX, lane, ok);
+  FI::run(v, sl, lane, twr);
+  double m = 0.0;
+#pragma unroll
+  for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+    for (int t = 0; t < FI::R0; ++t) {
+      if (ok && FI::valid(lane, it)) {
+        const double a = fabs((double)v[it][t].re);
+        m = (a > m || a != a) ? a : m;
+      }
+    }
+  fold_block_max<THREADS>(m, absmax);
 }
 
 // X-pass: u, v out; omega RHS 2(bx*xy - by*xx) -> forward real FFT -> Rw (half).
 // The inverse transforms go in pairs of real fields (u, v), (bx, by), (xx, xy),
 // each pair packed into one complex transform.
-template <typename R, int NX, int V>
+// CHK = 1 and Ppsi set: also the inverse x transform of psi's half rows (from
+// k_yf_omega), whose largest |value| is folded into absmax (psi_row_max).
+// FP64 runs the plain pass as the CHK = 0 instantiation (the check's code
+// path cost it a fifth of its speed); FP32 runs CHK = 1 either way (the
+// form that measured fastest).
+template <typename R, int NX, int V, int CHK>
 __global__ void __launch_bounds__((Tile<R, NX, V>::THREADS))
 k_xf(Geo g, int want_omega, const cplx<R>* __restrict__ Pu, const cplx<R>* __restrict__ Pv,
      const cplx<R>* __restrict__ Pbx, const cplx<R>* __restrict__ Pby, const cplx<R>* __restrict__ Pxx,
      const cplx<R>* __restrict__ Pxy, R* __restrict__ u, R* __restrict__ vv, cplx<R>* __restrict__ Rw,
+     const cplx<R>* __restrict__ Ppsi, unsigned long long* __restrict__ absmax,
      const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NX, V>;
   using FI = Fft<R, NX, +1, TL::PL>;
   using FF = Fft<R, NX, -1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
-  TwReg<R, TL::NN, TL::PL> twr;
+  TwTab<R, TL::NN, TL::PL, sizeof(R) == 8> twr;  // FP64: register twiddles (measured faster)
   twr.load(lane, tw);
   const int row = blockIdx.x * TL::S + r;
   const bool ok = row < g.nz * g.ny;
@@ -1077,23 +1224,27 @@ k_xf(Geo g, int want_omega, const cplx<R>* __restrict__ Pu, const cplx<R>* __res
         vv[(size_t)row * NX + x] = v[it][t].im;
       }
     }
-  if (!want_omega) return;  // uniform
-  // (bx, by) and (xx, xy): one packed transform each
-  FI::run(gb, sl, lane, twr);  // .re = bx, .im = by
-  FI::run(gx, sl, lane, twr);  // .re = xx, .im = xy
+  if (want_omega) {  // uniform
+    // (bx, by) and (xx, xy): one packed transform each
+    FI::run(gb, sl, lane, twr);  // .re = bx, .im = by
+    FI::run(gx, sl, lane, twr);  // .re = xx, .im = xy
 #pragma unroll
-  for (int it = 0; it < FI::IT; ++it)
+    for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
-    for (int t = 0; t < FI::R0; ++t)
-      v[it][t] = mk<R>((R)2 * (gb[it][t].re * gx[it][t].im - gb[it][t].im * gx[it][t].re), (R)0);
-  FF::run(v, sl, lane, twr);
+      for (int t = 0; t < FI::R0; ++t)
+        v[it][t] = mk<R>((R)2 * (gb[it][t].re * gx[it][t].im - gb[it][t].im * gx[it][t].re), (R)0);
+    FF::run(v, sl, lane, twr);
 #pragma unroll
-  for (int it = 0; it < FI::IT; ++it)
+    for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
-    for (int t = 0; t < FI::R0; ++t) {
-      int x = FI::elem(lane, it, t);
-      if (ok && FI::valid(lane, it) && x < g.W) Rw[hb + x] = v[it][t];
-    }
+      for (int t = 0; t < FI::R0; ++t) {
+        int x = FI::elem(lane, it, t);
+        if (ok && FI::valid(lane, it) && x < g.W) Rw[hb + x] = v[it][t];
+      }
+  }
+  if constexpr (CHK) {
+    if (Ppsi) psi_row_max<R, NX, TL::THREADS, FI>(v, Ppsi, hb, ok, lane, sl, twr, absmax);  // uniform
+  }
 }
 
 // Y-pass (forward, half width) of a real field, masked, expanded to a full
@@ -1106,7 +1257,7 @@ k_y_r2c_mask(Geo g, const cplx<R>* __restrict__ Rw, cplx<R>* __restrict__ out,
   using F = Fft<R, NY, -1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
-  TwReg<R, TL::NN, TL::PL> twr;
+  TwTab<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const Blk blk = xcd_block();
   const int x = blk.x * TL::S + s, z = blk.y;
@@ -1144,7 +1295,7 @@ k_y_herm(Geo g, const cplx<R>* __restrict__ in, cplx<R>* __restrict__ P,
   using F = Fft<R, NY, +1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
-  TwReg<R, TL::NN, TL::PL> twr;
+  TwTab<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const Blk blk = xcd_block();
   const int x = blk.x * TL::S + s, z = blk.y;
@@ -1187,7 +1338,7 @@ k_y_c2c(Geo g, const cplx<double>* __restrict__ in, cplx<double>* __restrict__ o
   using F = Fft<R, NY, DIR, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
-  TwReg<R, TL::NN, TL::PL> twr;
+  TwTab<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const Blk blk = xcd_block();
   const int x = blk.x * TL::S + s, z = blk.y;
@@ -1222,7 +1373,7 @@ k_x_c2c(Geo g, const cplx<double>* __restrict__ in, cplx<double>* __restrict__ o
   using F = Fft<R, NX, DIR, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
-  TwReg<R, TL::NN, TL::PL> twr;
+  TwTab<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const int row = blockIdx.x * TL::S + r;
   const bool ok = row < g.nz * g.ny;
@@ -1253,42 +1404,40 @@ k_x_c2c(Geo g, const cplx<double>* __restrict__ in, cplx<double>* __restrict__ o
 // check every step without a host round trip.
 // ===========================================================================
 
-template <typename R>
-__device__ inline bool finite_c(cplx<R> a) {
-  return __builtin_isfinite(a.re) && __builtin_isfinite(a.im);
-}
-
-// Y-pass: out = inverse y transform of psi / (nx*ny); q, B and psi are tested
-// at the points it loads (every point of the levels it covers).
+// Termination check (_check_termination_conditions!, src/simulation.jl:25-44)
+// on its own, Y-pass (half width): P = inverse y transform of psi's Hermitian
+// part / (nx*ny), formed as k_yf_omega forms its psi output; q, B and psi are
+// tested for non-finite values at the points it loads, (x, y) and the mirror
+// (-x, -y), which together are every point of the levels it covers.
 template <typename R, int NY, int V>
 __global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
 k_y_check(Geo g, const cplx<R>* __restrict__ psi, const cplx<R>* __restrict__ q, const cplx<R>* __restrict__ B,
-          cplx<R>* __restrict__ out, int* __restrict__ flag, const cplx<R>* __restrict__ tw) {
+          cplx<R>* __restrict__ P, int* __restrict__ flag, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY, V>;
   using F = Fft<R, NY, +1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
-  TwReg<R, TL::NN, TL::PL> twr;
+  TwTab<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const Blk blk = xcd_block();
   const int x = blk.x * TL::S + s, z = blk.y;
-  const bool ok = x < g.nx;
+  const bool ok = x < g.W;
+  const int mx = mirror(x, g.nx);
   cplx<R>* sl = lds + s * TL::LD;
+  const R rinv = (R)g.inv_n;
   cplx<R> v[F::IT][F::R0];
   bool fin = true;
 #pragma unroll
   for (int it = 0; it < F::IT; ++it)
 #pragma unroll
     for (int t = 0; t < F::R0; ++t) {
-      int y = F::elem(lane, it, t);
+      const int y = F::elem(lane, it, t), my = mirror(y, NY);
       const bool in = ok && F::valid(lane, it);
-      const size_t ad = ((size_t)z * NY + y) * g.nx + x;
-      cplx<R> p = czero<R>();
-      if (in) {
-        p = psi[ad];
-        fin = fin && finite_c(p) && finite_c(q[ad]) && finite_c(B[ad]);
-      }
-      v[it][t] = scale(p, (R)g.inv_n);
+      const size_t ad = ((size_t)z * NY + y) * g.nx + x, adm = ((size_t)z * NY + my) * g.nx + mx;
+      const cplx<R> p = ldu(psi, ad, in), pm = ldu(psi, adm, in);
+      fin = fin && (!in || (finite_c(p) && finite_c(pm) && finite_c(q[ad]) && finite_c(q[adm]) &&
+                            finite_c(B[ad]) && finite_c(B[adm])));
+      v[it][t] = herm(p, pm, rinv);
     }
   if (!fin) atomicOr(flag, 1);
   F::run(v, sl, lane, twr);
@@ -1296,9 +1445,28 @@ k_y_check(Geo g, const cplx<R>* __restrict__ psi, const cplx<R>* __restrict__ q,
   for (int it = 0; it < F::IT; ++it)
 #pragma unroll
     for (int t = 0; t < F::R0; ++t) {
-      int y = F::elem(lane, it, t);
-      if (ok && F::valid(lane, it)) out[((size_t)z * NY + y) * g.nx + x] = v[it][t];
+      const int y = F::elem(lane, it, t);
+      if (ok && F::valid(lane, it)) P[((size_t)z * NY + y) * g.Wp + x] = v[it][t];
     }
+}
+
+// X-pass of the stand-alone check: max |ifft2(psi)| from k_y_check's half
+// rows into absmax (nothing is stored).
+template <typename R, int NX, int V>
+__global__ void __launch_bounds__((Tile<R, NX, V>::THREADS))
+k_x_psi_max(Geo g, const cplx<R>* __restrict__ P, unsigned long long* __restrict__ absmax,
+            const cplx<R>* __restrict__ tw) {
+  using TL = Tile<R, NX, V>;
+  using FI = Fft<R, NX, +1, TL::PL>;
+  __shared__ cplx<R> lds[TL::S * TL::LD];
+  const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
+  TwTab<R, TL::NN, TL::PL> twr;
+  twr.load(lane, tw);
+  const int row = blockIdx.x * TL::S + r;
+  const bool ok = row < g.nz * g.ny;
+  cplx<R>* sl = lds + r * TL::LD;
+  cplx<R> v[FI::IT][FI::R0];
+  psi_row_max<R, NX, TL::THREADS, FI>(v, P, (size_t)row * g.Wp, ok, lane, sl, twr, absmax);
 }
 
 // Progress maxima (_progress_maxima, src/simulation.jl:46-81), Y-pass of the
@@ -1313,7 +1481,7 @@ k_y_la(Geo g, const cplx<R>* __restrict__ B, const cplx<R>* __restrict__ A, int 
   using F = Fft<R, NY, +1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
-  TwReg<R, TL::NN, TL::PL> twr;
+  TwTab<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const Blk blk = xcd_block();
   const int x = blk.x * TL::S + s, z = blk.y;
@@ -1358,7 +1526,7 @@ k_x_check(Geo g, const cplx<R>* __restrict__ in, unsigned long long* __restrict_
   using F = Fft<R, NX, +1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
-  TwReg<R, TL::NN, TL::PL> twr;
+  TwTab<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const int row = blockIdx.x * TL::S + r;
   const bool ok = row < g.nz * g.ny;
@@ -1383,25 +1551,7 @@ k_x_check(Geo g, const cplx<R>* __restrict__ in, unsigned long long* __restrict_
         m = (a > m || a != a) ? a : m;
       }
     }
-  if (m != m) m = __builtin_huge_val();
-  // the block's largest value (waves by shuffles, then LDS), folded in with
-  // at most one atomic per block, skipped when the accumulator already holds
-  // more: thousands of blocks contend for one address otherwise
-  for (int o = 32; o > 0; o >>= 1) {
-    const double w = __shfl_xor(m, o);
-    m = w > m ? w : m;
-  }
-  constexpr int NW = (TL::THREADS + 63) / 64;
-  __shared__ double wmax[NW];
-  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double b = 0.0;
-    for (int i = 0; i < NW; ++i) b = wmax[i] > b ? wmax[i] : b;
-    const unsigned long long bits = (unsigned long long)__double_as_longlong(b);
-    if (b > 0.0 && bits > __hip_atomic_load(absmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-      atomicMax(absmax, bits);
-  }
+  fold_block_max<TL::THREADS>(m, absmax);
 }
 
 }  // namespace qg

@@ -38,26 +38,34 @@ struct OpsV {
   // u, v from Pu, Pv
   static void xb_all(const Geo& g, hipStream_t st, const C* Pq, const C* Pz, const C* T1, const C* Pu,
                      const C* Pv, const R* u, const R* v, C* G1, C* G2, C* G3, C* G4, const C* twx);
+  // flag != nullptr: the new q / B are tested for non-finite values (the
+  // termination check fused into the step)
   static void yb2_q(const Geo& g, hipStream_t st, int mode, int flags, const C* G1, const C* G2, const C* qd,
                     const C* qn,
-                    C* rhs0, C* qout, EtdView etd, double nu, double inv_dz2, const C* twy);
+                    C* rhs0, C* qout, EtdView etd, double nu, double inv_dz2, int* flag, const C* twy);
   static void yb2_B(const Geo& g, hipStream_t st, int mode, int flags, const C* G3, const C* G4,
                     const C* A, const C* Bn, C* rhs0, C* Bout, C* T1, C* T3, EtdView etd, double alpha,
-                    const C* twy);
-  // Pu != nullptr: also the y-transformed half rows of u and v (as yb_adv)
+                    int* flag, const C* twy);
+  // Pu != nullptr: also the y-transformed half rows of u and v (as yb_adv);
+  // Ppsi != nullptr: also those of psi, and psi's non-finite test into flag
   static void yf_omega(const Geo& g, hipStream_t st, const C* psi, C* Pbx, C* Pby, C* Pxx, C* Pxy, C* Pu,
-                       C* Pv, const C* twy);
+                       C* Pv, C* Ppsi, int* flag, const C* twy);
+  // Ppsi != nullptr: max |ifft2(psi)| from yf_omega's psi half rows into absmax
   static void xf(const Geo& g, hipStream_t st, int want_omega, const C* Pu, const C* Pv, const C* Pbx,
-                 const C* Pby, const C* Pxx, const C* Pxy, R* u, R* v, C* Rw, const C* twx);
+                 const C* Pby, const C* Pxx, const C* Pxy, R* u, R* v, C* Rw, const C* Ppsi,
+                 unsigned long long* absmax, const C* twx);
   static void y_r2c_mask(const Geo& g, hipStream_t st, const C* Rw, C* out, const C* twy);
   static void y_herm(const Geo& g, hipStream_t st, const C* in, C* P, const C* twy);
   // plain complex 2-D transforms of double fields (dir = -1 forward, +1 inverse)
   static void y_c2c(const Geo& g, hipStream_t st, int dir, const D* in, D* out, const C* twy);
   static void x_c2c(const Geo& g, hipStream_t st, int dir, const D* in, D* out, const C* twx);
-  // termination check: non-finite q, B, psi -> flag; max |ifft2(psi)| -> absmax
-  // (tmp: one full-width field of scratch)
-  static void check(const Geo& g, hipStream_t st, const C* psi, const C* q, const C* B, C* tmp, int* flag,
-                    unsigned long long* absmax, const C* twx, const C* twy);
+  // termination check on its own, the Y-pass and the X-pass (run at the tile
+  // variants of yf_omega and xf, whose fused check they equal): non-finite
+  // q, B, psi -> flag; max |ifft2(psi)| -> absmax (P: one half-width field
+  // of scratch)
+  static void check_y(const Geo& g, hipStream_t st, const C* psi, const C* q, const C* B, C* P, int* flag,
+                      const C* twy);
+  static void check_x(const Geo& g, hipStream_t st, const C* P, unsigned long long* absmax, const C* twx);
   // progress maxima: max |ifft2(B + kh2 A / 4)| -> absmax (B alone unless
   // ybj_plus; tmp: one full-width field of scratch)
   static void la_max(const Geo& g, hipStream_t st, const C* B, const C* A, int ybj_plus, C* tmp,
@@ -94,7 +102,8 @@ struct Ops {
   QG_OPS_FWD(y_herm)
   QG_OPS_FWD(y_c2c)
   QG_OPS_FWD(x_c2c)
-  QG_OPS_FWD(check)
+  QG_OPS_FWD(check_y)
+  QG_OPS_FWD(check_x)
   QG_OPS_FWD(la_max)
 #undef QG_OPS_FWD
 };

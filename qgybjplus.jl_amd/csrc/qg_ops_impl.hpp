@@ -158,9 +158,9 @@ void OpsV<R, V>::xb_all(const Geo& g, hipStream_t st, const C* Pq, const C* Pz, 
 template <typename R, int V>
 void OpsV<R, V>::yb2_q(const Geo& g, hipStream_t st, int mode, int flags, const C* G1, const C* G2, const C* qd,
                    const C* qn,
-                   C* rhs0, C* qout, EtdView etd, double nu, double inv_dz2, const C* twy) {
+                   C* rhs0, C* qout, EtdView etd, double nu, double inv_dz2, int* flag, const C* twy) {
   Etd e{etd.tab, etd.plane};
-#define CALL(N) YL((k_yb2_q<R, N, V>), N, g.W, g, mode, flags, G1, G2, qd, qn, rhs0, qout, e, nu, inv_dz2, twy)
+#define CALL(N) YL((k_yb2_q<R, N, V>), N, g.W, g, mode, flags, G1, G2, qd, qn, rhs0, qout, e, nu, inv_dz2, flag, twy)
   QG_SWITCH(g.ny, CALL)
 #undef CALL
 }
@@ -168,22 +168,28 @@ void OpsV<R, V>::yb2_q(const Geo& g, hipStream_t st, int mode, int flags, const 
 template <typename R, int V>
 void OpsV<R, V>::yb2_B(const Geo& g, hipStream_t st, int mode, int flags, const C* G3, const C* G4,
                    const C* A, const C* Bn, C* rhs0, C* Bout, C* T1, C* T3, EtdView etd, double alpha,
-                   const C* twy) {
+                   int* flag, const C* twy) {
   Etd e{etd.tab, etd.plane};
-#define CALL(N) YL((k_yb2_B<R, N, V>), N, g.nx, g, mode, flags, G3, G4, A, Bn, rhs0, Bout, T1, T3, e, alpha, twy)
+#define CALL(N) YL((k_yb2_B<R, N, V>), N, g.nx, g, mode, flags, G3, G4, A, Bn, rhs0, Bout, T1, T3, e, alpha, flag, twy)
   QG_SWITCH(g.ny, CALL)
 #undef CALL
 }
 
 template <typename R, int V>
 void OpsV<R, V>::yf_omega(const Geo& g, hipStream_t st, const C* psi, C* Pbx, C* Pby, C* Pxx, C* Pxy, C* Pu,
-                      C* Pv, const C* twy) {
-  if (Pu) {
-#define CALL(N) YL((k_yf_omega<R, N, V, 1>), N, g.W, g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, twy)
+                      C* Pv, C* Ppsi, int* flag, const C* twy) {
+  // one tile column more than the half width: the Nyquist column's kx outputs
+  if (Pu && Ppsi) {
+#define CALL(N) YL((k_yf_omega<R, N, V, 1, 1>), N, (g.W + Tile<R, N, V>::S), g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, Ppsi, flag, twy)
+    QG_SWITCH(g.ny, CALL)
+#undef CALL
+  } else if (Pu) {
+#define CALL(N) YL((k_yf_omega<R, N, V, 1, 0>), N, (g.W + Tile<R, N, V>::S), g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, Ppsi, flag, twy)
     QG_SWITCH(g.ny, CALL)
 #undef CALL
   } else {
-#define CALL(N) YL((k_yf_omega<R, N, V, 0>), N, g.W, g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, twy)
+    if (Ppsi) throw std::runtime_error("yf_omega: the check needs the velocity outputs");
+#define CALL(N) YL((k_yf_omega<R, N, V, 0, 0>), N, (g.W + Tile<R, N, V>::S), g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, Ppsi, flag, twy)
     QG_SWITCH(g.ny, CALL)
 #undef CALL
   }
@@ -191,10 +197,17 @@ void OpsV<R, V>::yf_omega(const Geo& g, hipStream_t st, const C* psi, C* Pbx, C*
 
 template <typename R, int V>
 void OpsV<R, V>::xf(const Geo& g, hipStream_t st, int want_omega, const C* Pu, const C* Pv, const C* Pbx,
-                const C* Pby, const C* Pxx, const C* Pxy, R* u, R* v, C* Rw, const C* twx) {
-#define CALL(N) XL((k_xf<R, N, V>), N, g, want_omega, Pu, Pv, Pbx, Pby, Pxx, Pxy, u, v, Rw, twx)
-  QG_SWITCH(g.nx, CALL)
+                const C* Pby, const C* Pxx, const C* Pxy, R* u, R* v, C* Rw, const C* Ppsi,
+                unsigned long long* absmax, const C* twx) {
+  if (Ppsi || sizeof(R) == 4) {
+#define CALL(N) XL((k_xf<R, N, V, 1>), N, g, want_omega, Pu, Pv, Pbx, Pby, Pxx, Pxy, u, v, Rw, Ppsi, absmax, twx)
+    QG_SWITCH(g.nx, CALL)
 #undef CALL
+  } else {
+#define CALL(N) XL((k_xf<R, N, V, 0>), N, g, want_omega, Pu, Pv, Pbx, Pby, Pxx, Pxy, u, v, Rw, Ppsi, absmax, twx)
+    QG_SWITCH(g.nx, CALL)
+#undef CALL
+  }
 }
 
 template <typename R, int V>
@@ -238,12 +251,16 @@ void OpsV<R, V>::x_c2c(const Geo& g, hipStream_t st, int dir, const D* in, D* ou
 }
 
 template <typename R, int V>
-void OpsV<R, V>::check(const Geo& g, hipStream_t st, const C* psi, const C* q, const C* B, C* tmp, int* flag,
-                       unsigned long long* absmax, const C* twx, const C* twy) {
-#define CALL(N) YL((k_y_check<R, N, V>), N, g.nx, g, psi, q, B, tmp, flag, twy)
+void OpsV<R, V>::check_y(const Geo& g, hipStream_t st, const C* psi, const C* q, const C* B, C* P, int* flag,
+                         const C* twy) {
+#define CALL(N) YL((k_y_check<R, N, V>), N, g.W, g, psi, q, B, P, flag, twy)
   QG_SWITCH(g.ny, CALL)
 #undef CALL
-#define CALL(N) XL((k_x_check<R, N, V>), N, g, tmp, absmax, twx)
+}
+
+template <typename R, int V>
+void OpsV<R, V>::check_x(const Geo& g, hipStream_t st, const C* P, unsigned long long* absmax, const C* twx) {
+#define CALL(N) XL((k_x_psi_max<R, N, V>), N, g, P, absmax, twx)
   QG_SWITCH(g.nx, CALL)
 #undef CALL
 }

@@ -337,16 +337,22 @@ struct Engine {
   // members [m0, m0 + nm): non-finite q, B, psi -> d_flag, max |ifft2(psi)| ->
   // d_absmax, both accumulated (the caller clears them).  G3 is free between
   // steps and holds the y-transformed psi.
+  // The passes are those of the check the step fuses (yf_omega's psi output,
+  // xf's max), at the same tile variants, so both give the same values.  Pz
+  // is free between steps and holds psi's y-transformed half rows.
   void check(int m0, int nm) {
     qg::Geo g = h.g;
     g.nz = nm * h.nzc;
     g.members = nm;
     const size_t off = (size_t)m0 * h.nzc * h.plane;
-    const double pts = (double)h.plane * g.nz;
-    K("check", -1, pts * (3 * cS + 2 * cR), [&] {
-      O::check(1, g, h.st, static_cast<const C*>(h.at(h.psi, off)), static_cast<const C*>(h.at(h.q[h.cur], off)),
-               static_cast<const C*>(h.at(h.B[h.cur], off)), c(h.G3), h.d_flag, h.d_absmax, twx(), twy());
+    C* P = c(h.Pz) + (size_t)m0 * h.nzc * h.ny * h.g.Wp;
+    const double pts = (double)h.plane * g.nz, hpts = (double)h.g.W * h.ny * g.nz;
+    K("check_y", -1, pts * 3 * cS + hpts * cR, [&] {
+      O::check_y(h.tune[qg::P_YF_OMEGA], g, h.st, static_cast<const C*>(h.at(h.psi, off)),
+                 static_cast<const C*>(h.at(h.q[h.cur], off)), static_cast<const C*>(h.at(h.B[h.cur], off)), P,
+                 h.d_flag, twy());
     });
+    K("check_x", -1, hpts * cR, [&] { O::check_x(h.tune[qg::P_XF], g, h.st, P, h.d_absmax, twx()); });
   }
 
   // _progress_maxima (src/simulation.jl:46-81): max |ifft2(LA)| with LA =
@@ -408,15 +414,18 @@ struct Engine {
 
   // u, v and the QG omega-equation w (compute_velocities!(compute_w=true)).
   // One Y-pass over psi makes the half rows of u and v with the four of the
-  // omega right-hand side.
-  void velocities_w(const C* psi) {
-    K("yf_omega", qg::P_YF_OMEGA, P * cS + Ph * 6 * cR, [&] {
+  // omega right-hand side.  checked: the same passes also make the
+  // termination check of psi (non-finite values; max |ifft2(psi)| from psi's
+  // own half rows, in Pz until y_herm_w reuses it).
+  void velocities_w(const C* psi, bool checked = false) {
+    C* Ppsi = checked ? c(h.Pz) : nullptr;
+    K("yf_omega", qg::P_YF_OMEGA, P * cS + Ph * (checked ? 7 : 6) * cR, [&] {
       O::yf_omega(h.tune[qg::P_YF_OMEGA], h.g, h.st, psi, c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), c(h.Pu), c(h.Pv),
-                  twy());
+                  Ppsi, h.d_flag, twy());
     });
-    K("xf", qg::P_XF, Ph * 7 * cR + P * 2 * rR, [&] {
+    K("xf", qg::P_XF, Ph * (checked ? 8 : 7) * cR + P * 2 * rR, [&] {
       O::xf(h.tune[qg::P_XF], h.g, h.st, 1, c(h.Pu), c(h.Pv), c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), r(h.u), r(h.v), c(h.Pq),
-            twx());
+            Ppsi, h.d_absmax, twx());
     });
     K("y_r2c_omega", qg::P_Y_R2C, Ph * cR + P * cS, [&] { O::y_r2c_mask(h.tune[qg::P_Y_R2C], h.g, h.st, c(h.Pq), c(h.rq0), twy()); });
     if (h.sing_omega[0]) {
@@ -439,8 +448,10 @@ struct Engine {
   // uv: u and v are not yet made from psiS; the Y-pass over psiS and q_S
   // makes their half rows too, and the q-advection X-pass turns them into u
   // and v (written for the B pass).  Needs the advection of q.
+  // flag: the new q and B are tested for non-finite values (the step's fused
+  // termination check; final update only).
   void rhs_update(int mode, const C* qS, const C* psiS, const C* AS, const C* qn, const C* Bn, C* qout,
-                  C* Bout, bool uv = false) {
+                  C* Bout, bool uv = false, int* flag = nullptr) {
     const bool adv = !h.linear;
     const bool refr = !h.passive;
     const bool qdyn = !h.fixed_flow;
@@ -484,7 +495,7 @@ struct Engine {
       // stage: q_n, the kept bracket and q_S; final: the bracket and q_{n+1}
       ((qf & qg::YQ_ADV) ? Ph * 2 * cR : 0) +
           P * cS * ((mode == qg::MODE_STAGE || h.fixed_flow ? 3 : 2) + (diff ? 1 : 0)),
-      [&] { O::yb2_q(h.tune[qg::P_YB2_Q], h.g, h.st, mode, qf, c(h.G1), c(h.G2), qS, qn, c(h.rq0), qout, etd(), h.nu_z, inv_dz2, twy()); });
+      [&] { O::yb2_q(h.tune[qg::P_YB2_Q], h.g, h.st, mode, qf, c(h.G1), c(h.G2), qS, qn, c(h.rq0), qout, etd(), h.nu_z, inv_dz2, flag, twy()); });
     const bool disp = !(h.passive || h.no_disp);
     int bf = (adv ? qg::YB_ADV : 0) | (refr ? qg::YB_REFR : 0) | (disp ? qg::YB_DISP : 0);
     // the new B's inverse y transforms for the next diagnosis: T1 always
@@ -493,11 +504,16 @@ struct Engine {
     K(mode == qg::MODE_STAGE ? "yb2_B_stage" : "yb2_B_final", qg::P_YB2_B,
       P * cR * (((adv || refr) ? 1 : 0) + (adv ? 1 : 0) + 1 + (T3 ? 1 : 0)) +
           P * cS * ((mode == qg::MODE_STAGE ? 3 : 2) + (disp ? 1 : 0)),
-      [&] { O::yb2_B(h.tune[qg::P_YB2_B], h.g, h.st, mode, bf, c(h.G3), c(h.G4), AS, Bn, c(h.rB0), Bout, c(h.T1), T3, etd(), h.f / 2, twy()); });
+      [&] { O::yb2_B(h.tune[qg::P_YB2_B], h.g, h.st, mode, bf, c(h.G3), c(h.G4), AS, Bn, c(h.rB0), Bout, c(h.T1), T3, etd(), h.f / 2, flag, twy()); });
   }
 
   // step!(model, ExponentialRungeKutta2) -> _advance_etdrk2! (src/timestep.jl:335-415).
-  void step() {
+  // checked: the step's last passes also make _check_termination_conditions!
+  // (src/simulation.jl:25-44) of the state they leave, accumulated into
+  // d_flag and d_absmax: non-finite q and B where the final update writes
+  // them, non-finite psi and max |ifft2(psi)| in the finalize's velocity and
+  // omega passes, which read psi anyway.  The values are those of check().
+  void step(bool checked = false) {
     C* qn = c(h.q[h.cur]);
     C* Bn = c(h.B[h.cur]);
     C* q1 = c(h.q[1 - h.cur]);
@@ -518,11 +534,12 @@ struct Engine {
     diag(c(h.qs), c(h.Bs), psiS, true, true);
     wave_inv(c(h.Bs), c(h.As), nullptr);
     // the stage's u, v come out of the RHS passes (they are needed nowhere else)
-    rhs_update(qg::MODE_FINAL, c(h.qs), psiS, c(h.As), qn, Bn, q1, B1, need_uv && !h.fixed_flow);
+    rhs_update(qg::MODE_FINAL, c(h.qs), psiS, c(h.As), qn, Bn, q1, B1, need_uv && !h.fixed_flow,
+               checked ? h.d_flag : nullptr);
     // finalize (src/timestep.jl:300-332); yb2_B left T1, T3 of B_{n+1}
     diag(q1, B1, c(h.psi), false, true);
     wave_inv(B1, c(h.A), c(h.C));
-    velocities_w(c(h.psi));
+    velocities_w(c(h.psi), checked);
     h.cur = 1 - h.cur;
     h.diag_valid = true;
     h.t1_valid = true;
@@ -617,10 +634,7 @@ void fft2_inplace(qg_handle& h, D* f, int dir, D* out = nullptr, int levels = -1
 template <class E, class Mark>
 void advance(qg_handle& H, E& e, int nsteps, bool checked, Mark&& mark) {
   const bool use_graphs = H.graphs && !(H.prof_mode & 1);
-  auto one = [&] {
-    e.step();
-    if (checked) e.check(0, H.active);
-  };
+  auto one = [&] { e.step(checked); };  // the check is fused into the step's last passes
   int i = 0;
   while (i < nsteps) {
     if (use_graphs && H.diag_valid && nsteps - i >= 2) {

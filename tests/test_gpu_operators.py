@@ -141,10 +141,12 @@ def test_velocities_and_omega_w(qg):
 
 
 def test_check_reports_max_psi_and_nonfinite(qg):
-    """_check_termination_conditions! (src/simulation.jl:25-44)."""
+    """_check_termination_conditions! (src/simulation.jl:25-44) on the
+    spectrum of a real streamfunction (Hermitian, as every psi the model
+    holds): max |ifft2(psi)| and the non-finite test."""
     c = Case(size=(16, 16, 4))
     m = gpu_model(qg, c)
-    psi = random_spectral((4, 16, 16), 9, scale=10.0)
+    psi = random_spectral((4, 16, 16), 9, scale=10.0, hermitian=True)
     m.fields.psi = psi
     bad, mx = m._h.check()
     assert not bad
