@@ -891,7 +891,6 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
   const int x = blk.x * TL::S + s, z = blk.y;
   const bool ok = x < g.nx;
   cplx<R>* sl = lds + s * TL::LD;
-  cplx<double> acc[F::IT][F::R0];
   cplx<R> v[F::IT][F::R0], w[F::IT][F::R0];
   const bool has3 = (flags & (YB_ADV | YB_REFR)) != 0, has4 = (flags & YB_ADV) != 0;  // uniform
   // both inputs are loaded before either transform: twice the bytes in flight
@@ -904,25 +903,11 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
       const size_t ad = ((size_t)z * NY + y) * g.nx + x;
       v[it][t] = ldu(G3, ad, in && has3);
       w[it][t] = ldu(G4, ad, in && has4);
-      acc[it][t] = czero<double>();
     }
-  if (has3) {
-    F::run(v, sl, lane, twr);
-#pragma unroll
-    for (int it = 0; it < F::IT; ++it)
-#pragma unroll
-      for (int t = 0; t < F::R0; ++t) acc[it][t] = cvt<double>(v[it][t]);  // -i kx F(uB) - (i/2) F(zeta B)
-  }
-  if (has4) {
-    F::run(w, sl, lane, twr);
-#pragma unroll
-    for (int it = 0; it < F::IT; ++it)
-#pragma unroll
-      for (int t = 0; t < F::R0; ++t) {
-        int y = F::elem(lane, it, t);
-        acc[it][t] = acc[it][t] + ik_mul(-g.ky[y], cvt<double>(w[it][t]));  // - i ky F(vB)
-      }
-  }
+  // F(G3) = -i kx F(uB) - (i/2) F(zeta B) and F(G4) = F(vB) stay at the
+  // transform precision; their FP64 sum is formed per point below
+  if (has3) F::run(v, sl, lane, twr);
+  if (has4) F::run(w, sl, lane, twr);
   const size_t plane = (size_t)g.nx * NY;
   const double kxv = ok ? g.kx[x] : 0.0;
   // FP32 up to 256-point columns: the update's field inputs (A, and B_n or
@@ -952,6 +937,7 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
 #pragma unroll
     for (int t = 0; t < F::R0; ++t) {
       int y = F::elem(lane, it, t);
+      const cplx<R> g3 = v[it][t];
       v[it][t] = czero<R>();
       if (!(ok && F::valid(lane, it))) continue;
       const bool keep = g.mask[y * g.nx + x] != 0;
@@ -959,6 +945,8 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
       const size_t ad = (size_t)z * plane + col;
       cplx<double> rhs = czero<double>();
       if (keep) {
+        cplx<double> acc = has3 ? cvt<double>(g3) : czero<double>();
+        if (has4) acc = acc + ik_mul(-g.ky[y], cvt<double>(w[it][t]));  // - i ky F(vB)
         cplx<double> t1 = czero<double>();
         if (flags & YB_DISP) {
           const double kh2 = kxv * kxv + g.ky[y] * g.ky[y];
@@ -966,7 +954,7 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
           cplx<double> Av = cvt<double>(GROUP ? av[it][t] : A[ad]);
           t1 = mk<double>(-(c * Av.im), c * Av.re);
         }
-        rhs = acc[it][t] + t1;
+        rhs = acc + t1;
       }
       // ETD-RK2 as for q: the stage keeps E B0 + (h1 - h2) r0, the final
       // step adds h2 r1 to it
