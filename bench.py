@@ -47,7 +47,7 @@ LAUNCH_KERNEL = {
     "x_c2r_uv": ("k_x_c2r", "x_c2r", None), "x_c2r_w": ("k_x_c2r", "x_c2r", None),
     "xb_q": ("k_xb_q", "xb_q", None, 0), "xb_q_uv": ("k_xb_q", "xb_q", None, 1), "xb_B": ("k_xb_B", "xb_B", None),
     "xb_all": ("k_xb_all", "xb_B", None, 0), "xb_all_uv": ("k_xb_all", "xb_B", None, 1),
-    "yb2_q_stage": ("k_yb2_q", "yb2_q", "stage"), "yb2_q_final": ("k_yb2_q", "yb2_q", "final"),
+    "yb2_q_stage": ("k_yb2_q", "yb2_q", "stage", 1), "yb2_q_final": ("k_yb2_q", "yb2_q", "final", 1),
     "yb2_B_stage": ("k_yb2_B", "yb2_B", "stage"), "yb2_B_final": ("k_yb2_B", "yb2_B", "final"),
     "yf_omega": ("k_yf_omega", "yf_omega", None, 1, 0), "xf": ("k_xf", "xf", None, 1),
     "y_r2c_omega": ("k_y_r2c_mask", "y_r2c", None), "y_herm_w": ("k_y_herm", "y_herm", None),

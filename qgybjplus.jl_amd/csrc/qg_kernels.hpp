@@ -778,13 +778,16 @@ __device__ inline cplx<double> vdiff(const cplx<S>* __restrict__ q, int z, int n
 
 // Y-pass (forward, half width): nq = mask*(i kx F(uq) + i ky F(vq)); rhs_q and
 // the ETD stage/final update of q (timestep.jl:286-295, 362-404).
-template <typename R, int NY, int V>
+// FL >= 0 fixes the flag word at compile time (the launcher passes the same
+// value), which removes the per-point branches on it; FL = -1 reads `flags`.
+template <typename R, int NY, int V, int FL = -1>
 __global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
-k_yb2_q(Geo g, int mode, int flags, const cplx<R>* __restrict__ G1, const cplx<R>* __restrict__ G2,
+k_yb2_q(Geo g, int mode, int flags_rt, const cplx<R>* __restrict__ G1, const cplx<R>* __restrict__ G2,
         const cplx<R>* __restrict__ qd, const cplx<R>* __restrict__ qn,
         cplx<R>* __restrict__ rhs0,
         cplx<R>* __restrict__ qout, Etd etd, double nu, double inv_dz2, int* __restrict__ flag,
         const cplx<R>* __restrict__ tw) {
+  const int flags = FL >= 0 ? FL : flags_rt;
   using TL = Tile<R, NY, V>;
   using F = Fft<R, NY, -1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];

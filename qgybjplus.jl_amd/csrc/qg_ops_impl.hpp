@@ -160,9 +160,15 @@ void OpsV<R, V>::yb2_q(const Geo& g, hipStream_t st, int mode, int flags, const 
                    const C* qn,
                    C* rhs0, C* qout, EtdView etd, double nu, double inv_dz2, int* flag, const C* twy) {
   Etd e{etd.tab, etd.plane};
-#define CALL(N) YL((k_yb2_q<R, N, V>), N, g.W, g, mode, flags, G1, G2, qd, qn, rhs0, qout, e, nu, inv_dz2, flag, twy)
-  QG_SWITCH(g.ny, CALL)
+  if (flags == YQ_ADV) {  // the evolving flow without vertical diffusion
+#define CALL(N) YL((k_yb2_q<R, N, V, YQ_ADV>), N, g.W, g, mode, flags, G1, G2, qd, qn, rhs0, qout, e, nu, inv_dz2, flag, twy)
+    QG_SWITCH(g.ny, CALL)
 #undef CALL
+  } else {
+#define CALL(N) YL((k_yb2_q<R, N, V>), N, g.W, g, mode, flags, G1, G2, qd, qn, rhs0, qout, e, nu, inv_dz2, flag, twy)
+    QG_SWITCH(g.ny, CALL)
+#undef CALL
+  }
 }
 
 template <typename R, int V>
