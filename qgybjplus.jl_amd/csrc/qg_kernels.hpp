@@ -746,11 +746,28 @@ k_xb_all(Geo g, const cplx<R>* __restrict__ Pq, const cplx<R>* __restrict__ Pz, 
     }
 }
 
-// ETD tables per column, layout [6][ny][nx]: Eq, hphi1q, hphi2q, EB, hphi1B, hphi2B.
+// ETD coefficient tables Eq, hphi1q, hphi2q, EB, hphi1B, hphi2B, one after
+// the other.  The coefficients are functions of kx^2 + ky^2, so when the
+// wavenumbers are sign-symmetric (kx[nx - x]^2 == kx[x]^2, likewise ky, as
+// FFT-ordered wavenumbers are) each table covers only the quarter
+// 0 <= x <= nx/2, 0 <= y <= ny/2 ([ny/2 + 1][nx/2 + 1]), and a column reads
+// the entry of (min(x, nx - x), min(y, ny - y)): a quarter of the footprint,
+// which the 128 levels of a field then find in L2.  Otherwise the tables are
+// [ny][nx].
 struct Etd {
   const double* tab;
-  size_t plane;  // nx*ny
-  __device__ inline double at(int which, int col) const { return tab[which * plane + col]; }
+  size_t plane;  // entries of one table
+  int stride;    // row length of a table
+  int nx, ny;
+  bool quarter;
+  __device__ inline int col(int x, int y) const {
+    if (quarter) {
+      x = x <= nx / 2 ? x : nx - x;
+      y = y <= ny / 2 ? y : ny - y;
+    }
+    return y * stride + x;
+  }
+  __device__ inline double at(int which, int c) const { return tab[which * plane + c]; }
 };
 
 // Vertical diffusion nu_z * d2q/dz2 with Neumann ends (dissipation_q_nv!, nonlinear.jl:1010-1044).
@@ -830,9 +847,9 @@ k_yb2_q(Geo g, int mode, int flags_rt, const cplx<R>* __restrict__ G1, const cpl
       // The ETD coefficients depend on (kx, ky) through kh2 only, so the
       // mirror column (mx, my) has those of (x, y).
       const bool upd = keep && !(flags & YQ_FIXED);
-      const size_t col0 = (size_t)y * g.nx + x;
-      const double E = upd ? etd.at(0, col0) : 0.0, h1 = upd ? etd.at(1, col0) : 0.0;
-      const double h2 = upd ? etd.at(2, col0) : 0.0;
+      const int ce = etd.col(x, y);
+      const double E = upd ? etd.at(0, ce) : 0.0, h1 = upd ? etd.at(1, ce) : 0.0;
+      const double h2 = upd ? etd.at(2, ce) : 0.0;
       for (int side = 0; side < 2; ++side) {
         if (side == 1 && mx == x) break;
         const int cx = side ? mx : x, cy = side ? my : y;
@@ -963,9 +980,10 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
       // step adds h2 r1 to it
       cplx<double> out = czero<double>();
       if (keep) {
-        const double h2 = etd.at(5, col);
+        const int ce = etd.col(x, y);
+        const double h2 = etd.at(5, ce);
         if (mode == MODE_STAGE) {
-          const double E = etd.at(3, col), h1 = etd.at(4, col);
+          const double E = etd.at(3, ce), h1 = etd.at(4, ce);
           const cplx<double> b0 = cvt<double>(GROUP ? bv[it][t] : base[ad]);
           out = scale(b0, E) + scale(rhs, h1);
           rhs0[ad] = cvt<R>(scale(b0, E) + scale(rhs, h1 - h2));

@@ -8,9 +8,13 @@
 
 namespace qg {
 
+// ETD coefficient tables (see Etd in qg_kernels.hpp for the layout).
 struct EtdView {
   const double* tab;
-  size_t plane;
+  size_t plane;    // entries of one coefficient's table
+  int stride;      // table row length
+  int nx, ny;      // grid columns and rows
+  bool quarter;    // tables over (|kx|, |ky|) only
 };
 
 bool fft_size_supported(int n);

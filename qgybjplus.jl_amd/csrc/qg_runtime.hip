@@ -71,6 +71,11 @@ struct qg_handle {
   unsigned char* d_mask = nullptr;
   double* d_etd = nullptr;
   double etd_dt = std::nan("");
+  // ETD table layout (qg::Etd): quarter tables when the wavenumbers are
+  // sign-symmetric, fixed at creation so that captured graphs stay valid
+  bool etd_quarter = false;
+  int etd_stride = 0;
+  size_t etd_plane = 0;
   // Complex fields in the mode's storage precision: cplx<double> in FP64
   // mode, cplx<float> in mixed mode (f32()).  stg and stg2 are FP64 staging
   // for host exchange and plain transforms (stg2 aliases qeff in FP64 mode).
@@ -219,12 +224,29 @@ void etd_coefficients(double x, double h, double& E, double& h1, double& h2) {
   h2 = h * (em1 + x) / (x * x);
 }
 
+// The ETD tables' layout: a quarter of the plane when kx^2 and ky^2 are
+// symmetric under x -> nx - x, y -> ny - y bit for bit (then every column's
+// coefficients equal those of its quarter-plane representative), else the
+// whole plane.
+void etd_layout(qg_handle& h) {
+  auto symmetric = [](const std::vector<double>& k) {
+    const int n = (int)k.size();
+    for (int i = 1; i < n; ++i)
+      if (k[i] * k[i] != k[n - i] * k[n - i]) return false;
+    return true;
+  };
+  h.etd_quarter = symmetric(h.kx) && symmetric(h.ky);
+  h.etd_stride = h.etd_quarter ? h.nx / 2 + 1 : h.nx;
+  h.etd_plane = (size_t)h.etd_stride * (h.etd_quarter ? h.ny / 2 + 1 : h.ny);
+}
+
 void build_etd(qg_handle& h, double dt) {
   if (h.etd_dt == dt) return;
-  std::vector<double> tab(6 * h.plane);
-  for (int y = 0; y < h.ny; ++y)
-    for (int x = 0; x < h.nx; ++x) {
-      size_t col = (size_t)y * h.nx + x;
+  const int tx = h.etd_quarter ? h.nx / 2 + 1 : h.nx, ty = h.etd_quarter ? h.ny / 2 + 1 : h.ny;
+  std::vector<double> tab(6 * h.etd_plane);
+  for (int y = 0; y < ty; ++y)
+    for (int x = 0; x < tx; ++x) {
+      size_t col = (size_t)y * h.etd_stride + x;
       double kh2 = h.kx[x] * h.kx[x] + h.ky[y] * h.ky[y];
       double lq = 0.0, lB = 0.0;
       if (!h.inviscid) {
@@ -233,13 +255,13 @@ void build_etd(qg_handle& h, double dt) {
       }
       double e, a1, a2;
       etd_coefficients(lq, dt, e, a1, a2);
-      tab[0 * h.plane + col] = e;
-      tab[1 * h.plane + col] = a1;
-      tab[2 * h.plane + col] = a2;
+      tab[0 * h.etd_plane + col] = e;
+      tab[1 * h.etd_plane + col] = a1;
+      tab[2 * h.etd_plane + col] = a2;
       etd_coefficients(lB, dt, e, a1, a2);
-      tab[3 * h.plane + col] = e;
-      tab[4 * h.plane + col] = a1;
-      tab[5 * h.plane + col] = a2;
+      tab[3 * h.etd_plane + col] = e;
+      tab[4 * h.etd_plane + col] = a1;
+      tab[5 * h.etd_plane + col] = a2;
     }
   HIPCHECK(hipMemcpyAsync(h.d_etd, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, h.st));
   HIPCHECK(hipStreamSynchronize(h.st));
@@ -318,7 +340,7 @@ struct Engine {
   R* r(void* p) const { return static_cast<R*>(p); }
   const C* twx() const { return static_cast<const C*>(h.twx); }
   const C* twy() const { return static_cast<const C*>(h.twy); }
-  qg::EtdView etd() const { return qg::EtdView{h.d_etd, h.plane}; }
+  qg::EtdView etd() const { return qg::EtdView{h.d_etd, h.etd_plane, h.etd_stride, h.nx, h.ny, h.etd_quarter}; }
 
   template <class F>
   void K(const char* name, int pass, double bytes, F&& fn) {
@@ -797,7 +819,8 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
     h->d_fN2 = h->alloc<double>(nz);
     h->d_prof = h->alloc<double>(nz);
     h->d_mask = h->alloc<unsigned char>(h->plane);
-    h->d_etd = h->alloc<double>(6 * h->plane);
+    etd_layout(*h);
+    h->d_etd = h->alloc<double>(6 * h->etd_plane);
     std::vector<unsigned char> mask(h->plane);
     for (int y = 0; y < ny; ++y)
       for (int x = 0; x < nx; ++x) mask[(size_t)y * nx + x] = is_dealiased(x, y, nx, ny) ? 1 : 0;
