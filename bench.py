@@ -174,6 +174,48 @@ ENSEMBLE_CLASSES = ((64, 32), (128, 64), (256, 64))  # (nx = ny, nz), 16 members
 ENSEMBLE_MEMBERS = 16
 
 
+def ensemble_n0(seed=2026):
+    """N0² of the 48 members, class by class."""
+    rng = np.random.default_rng(seed)
+    return rng.uniform(5e-6, 5e-5, size=len(ENSEMBLE_CLASSES) * ENSEMBLE_MEMBERS)
+
+
+def ensemble_member_psi(grid, ci, j, seed=2026):
+    """Member j of grid class ci: the class's seeded k^-3 field plus 5 % of a
+    member-seeded one."""
+    from qgybjplus_amd import synthetic
+    base = synthetic.k3_streamfunction(grid, seed=seed, urms=0.1, kpeak=6.0)
+    noise = synthetic.k3_streamfunction(grid, seed=seed + 1 + ci * ENSEMBLE_MEMBERS + j, urms=0.1, kpeak=6.0)
+    return base + 0.05 * noise
+
+
+def ensemble_model_kwargs(qg, ci):
+    """Grid and physics of class ci, for an ensemble or a member run alone."""
+    n, nz = ENSEMBLE_CLASSES[ci]
+    grid = qg.RectilinearGrid(size=(n, n, nz), extent=EXTENT, centered=True)
+    return dict(grid=grid, coriolis=qg.FPlane(1e-4), feedback=qg.WaveMeanFeedback(), formulation=qg.YBJPlus(),
+                architecture=qg.GPU("mixed"))
+
+
+def build_ensembles(qg, seed=2026):
+    """The three device ensembles of the batched-ensemble workload with their
+    initial states set; returns them and the total grid points."""
+    from qgybjplus_amd import synthetic
+    n0 = ensemble_n0(seed)
+    ens_list, points = [], 0
+    for ci, (n, nz) in enumerate(ENSEMBLE_CLASSES):
+        kw = ensemble_model_kwargs(qg, ci)
+        strats = [qg.AnalyticalProfile(synthetic.exponential_N2(float(n0[ci * ENSEMBLE_MEMBERS + j]), 800.0))
+                  for j in range(ENSEMBLE_MEMBERS)]
+        ens = qg.QGYBJEnsemble(stratifications=strats, **kw)
+        for j in range(ENSEMBLE_MEMBERS):
+            ens.set_member(j, psi=qg.FieldArray(ensemble_member_psi(kw["grid"], ci, j, seed)),
+                           waves=qg.SurfaceWave(0.1, 50.0))
+        ens_list.append(ens)
+        points += n * n * nz * ENSEMBLE_MEMBERS
+    return ens_list, points
+
+
 def ensemble_leg(qg, steps, warmup=2, seed=2026, autotune=True):
     """The batched-ensemble workload: 48 members of uneven grids, 16 each of
     64²x32, 128²x64 and 256²x64 over 500 km x 500 km x 4000 m, N² = N0²
@@ -184,24 +226,7 @@ def ensemble_leg(qg, steps, warmup=2, seed=2026, autotune=True):
     after the other (per class) and then stepping concurrently, each on its
     own HIP stream from its own host thread, as the serving loop runs its
     pools; `value` is the concurrent throughput over all 48 members."""
-    from qgybjplus_amd import synthetic
-    rng = np.random.default_rng(seed)
-    n0 = rng.uniform(5e-6, 5e-5, size=len(ENSEMBLE_CLASSES) * ENSEMBLE_MEMBERS)
-    ens_list, points = [], 0
-    for ci, (n, nz) in enumerate(ENSEMBLE_CLASSES):
-        grid = qg.RectilinearGrid(size=(n, n, nz), extent=EXTENT, centered=True)
-        strats = [qg.AnalyticalProfile(synthetic.exponential_N2(float(n0[ci * ENSEMBLE_MEMBERS + j]), 800.0))
-                  for j in range(ENSEMBLE_MEMBERS)]
-        ens = qg.QGYBJEnsemble(stratifications=strats, grid=grid, coriolis=qg.FPlane(1e-4),
-                               feedback=qg.WaveMeanFeedback(), formulation=qg.YBJPlus(),
-                               architecture=qg.GPU("mixed"))
-        base = synthetic.k3_streamfunction(grid, seed=seed, urms=0.1, kpeak=6.0)
-        for j in range(ENSEMBLE_MEMBERS):
-            noise = synthetic.k3_streamfunction(grid, seed=seed + 1 + ci * ENSEMBLE_MEMBERS + j, urms=0.1,
-                                                kpeak=6.0)
-            ens.set_member(j, psi=qg.FieldArray(base + 0.05 * noise), waves=qg.SurfaceWave(0.1, 50.0))
-        ens_list.append(ens)
-        points += n * n * nz * ENSEMBLE_MEMBERS
+    ens_list, points = build_ensembles(qg, seed)
     ts = qg.ExponentialRungeKutta2(dt=DT)
     for ens in ens_list:
         if autotune:

@@ -4,7 +4,7 @@ C ABI) and the CPU oracle model with identical inputs."""
 from __future__ import annotations
 
 import math
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 from typing import Callable, Optional, Tuple
 
 import numpy as np
@@ -31,8 +31,13 @@ class Case:
     precision: str = "fp64"
 
 
-def gpu_model(qg, c: Case):
+def gpu_model(qg, c: Case, kx=None):
+    """kx: wavenumbers to use in place of the grid's (the same array goes to
+    oracle_model)."""
     grid = qg.RectilinearGrid(size=c.size, extent=c.extent, centered=c.centered)
+    if kx is not None:
+        grid.kx = np.array(kx, dtype=float)
+        grid.kh2 = grid.kx[:, None] ** 2 + grid.ky[None, :] ** 2
     strat = qg.AnalyticalProfile(c.N2_func) if c.N2_func else qg.ConstantStratification(c.N2)
     if c.closure.kind == "horizontal":
         closure = qg.HorizontalHyperdiffusivity(
@@ -56,8 +61,10 @@ def gpu_model(qg, c: Case):
         architecture=qg.GPU(c.precision))
 
 
-def oracle_model(c: Case) -> O.OracleModel:
+def oracle_model(c: Case, kx=None) -> O.OracleModel:
     g = O.make_grid(c.size, extent=c.extent, centered=c.centered)
+    if kx is not None:
+        g = replace(g, kx=np.array(kx, dtype=float))
     if c.N2_func:
         N2 = O.stratification_profile(g, c.N2_func)
     else:

@@ -219,3 +219,34 @@ def test_other_plans_step_matches_oracle(qg, size):
         print(f"  {size} {n}: {e:.2e}", flush=True)
         assert e <= (1e-9 if n == "w" else 1e-11), (n, e)
     m.finalize()
+
+
+def test_asymmetric_wavenumbers_use_whole_plane_etd_tables(qg):
+    """The ETD tables cover only the quarter plane (|kx|, |ky|) when kx^2 and
+    ky^2 are symmetric bit for bit, as FFT-ordered wavenumbers are.  Here one
+    kx is off by a few ulp, so the tables cover the whole plane; with a
+    hyperdiffusion strong enough that every coefficient differs from its
+    undamped value, 3 steps match the oracle given the same kx."""
+    c = Case(size=(32, 32, 12), extent=(500e3, 500e3, 4000.0), N2_func=EXP_N2, feedback=True,
+             closure=O.Closure(flow_coefficients=(5e3,), flow_orders=(2,), wave_coefficients=(5e3,),
+                               wave_orders=(2,)))
+    kx = O.make_grid(c.size, extent=c.extent, centered=True).kx.copy()
+    kx[-1] = kx[-1] * (1 + 2.0 ** -40)
+    assert kx[-1] ** 2 != kx[1] ** 2
+    print("case step with asymmetric kx (whole-plane ETD tables)", flush=True)
+    m, o = gpu_model(qg, c, kx=kx), oracle_model(c, kx=kx)
+    psi = k3_streamfunction(o.grid, seed=5, urms=0.1, kpeak=4.0)
+    X, Y = np.meshgrid(o.grid.x, o.grid.y, indexing="ij")
+    B = np.empty(psi.shape, complex)
+    for k in range(o.grid.nz):
+        B[k] = 0.1 * math.exp(o.grid.z[k] / 200.0) * np.exp(1j * 2 * math.pi * (5 * X / o.grid.Lx + 2 * Y / o.grid.Ly))
+    set_both(m, o, psi, B, qg)
+    ts = qg.ExponentialRungeKutta2(dt=600.0)
+    qg.step_(m, ts, 3)
+    for _ in range(3):
+        o.step(600.0)
+    for n in FIELDS + ("C", "u", "v", "w"):
+        e = rel(getattr(m.fields, n), getattr(o.fields, n))
+        print(f"  asymmetric kx {n}: {e:.2e}", flush=True)
+        assert e <= (1e-9 if n == "w" else 1e-11), (n, e)
+    m.finalize()
