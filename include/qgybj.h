@@ -149,8 +149,13 @@ int qg_check(qg_handle* h, int* nonfinite, double* max_abs_psi);
  *   *failure     0 none, 1 non-finite q, B or psi ("non-finite value
  *                detected in the model state"), 2 max |ifft(psi)| > 1e10
  *                ("solution appears to be blowing up");
- *   *max_abs_psi the failing step's max |ifft(psi)|, else the largest over
- *                the steps taken.
+ *   *max_abs_psi the failing step's max |ifft(psi)|, else that of the state
+ *                after the last step (the value qg_check gives).
+ * While the steps stay far from the limit the maximum is bounded rather than
+ * computed (sum of |psi| over each level's spectrum / (nx ny) under
+ * 1e10 * (1 - 1e-4)); a stretch of 32 steps that does not meet the bound is
+ * stepped again with the exact check, so stops and failures are those of the
+ * exact check after every step.
  * The model state after a failure is the state right after the failing step.
  * The first call allocates five complex and two real fields of scratch: the
  * state every chunk of steps starts from (q, B, psi, A, the y-transformed B,

@@ -120,6 +120,22 @@ __device__ inline void fold_block_max(double m, unsigned long long* __restrict__
   }
 }
 
+// Adds the block's sum of one value per thread to *acc (one atomic per
+// block).  Every thread of the block calls it.
+template <int THREADS>
+__device__ inline void fold_block_sum(double v, double* __restrict__ acc) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  constexpr int NW = (THREADS + 63) / 64;
+  __shared__ double wsum[NW];
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = 0.0;
+    for (int i = 0; i < NW; ++i) b += wsum[i];
+    atomicAdd(acc, b);
+  }
+}
+
 // ===========================================================================
 // Wave feedback q_w  (compute_qw_complex!, nonlinear.jl:859-944)
 // ===========================================================================
@@ -1047,6 +1063,12 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
 // CHK = 1: also the y-transformed half rows of psi itself, Ppsi (the
 // termination check's max |ifft2(psi)|, folded in by the X-pass), and psi is
 // tested for non-finite values at every point (flag).
+// CHK = 2: psi's non-finite test as for CHK = 1, and instead of Ppsi a bound
+// of the check's maximum: lsum[z] += the block's share of
+// sum over the level's whole spectrum of |h.re| + |h.im|, h = psi's Hermitian
+// part / (nx ny), which bounds |Re ifft2(psi)| at every point of the level
+// (the X-pass folds the levels' sums into absmax).  The half-width columns
+// stand for their mirrors too (x = 0 and x = nx/2 are their own).
 //
 // Every output is the Hermitian part of a multiplier times psi, psi_z or
 // psi-bar at (x, y) and its mirror (-x, -y).  Each multiplier is odd in the
@@ -1065,7 +1087,7 @@ __global__ void __launch_bounds__((Tile<R, NY, V>::THREADS), (sizeof(R) == 4 && 
 k_yf_omega(Geo g, const cplx<R>* __restrict__ psi, cplx<R>* __restrict__ Pbx, cplx<R>* __restrict__ Pby,
            cplx<R>* __restrict__ Pxx, cplx<R>* __restrict__ Pxy, cplx<R>* __restrict__ Pu,
            cplx<R>* __restrict__ Pv, cplx<R>* __restrict__ Ppsi, int* __restrict__ flag,
-           const cplx<R>* __restrict__ tw) {
+           double* __restrict__ lsum, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY, V>;
   using F = Fft<R, NY, +1, TL::PL>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
@@ -1090,6 +1112,8 @@ k_yf_omega(Geo g, const cplx<R>* __restrict__ psi, cplx<R>* __restrict__ Pbx, cp
   cplx<R> h0[F::IT][F::R0], h1[F::IT][F::R0];
   cplx<R> a0 = czero<R>(), a1 = czero<R>();
   bool fin = true;
+  double bsum = 0.0;  // CHK = 2: this thread's share of the level's bound
+  const double wx = (x == 0 || xnyq) ? 1.0 : 2.0;
 #pragma unroll
   for (int it = 0; it < F::IT; ++it)
 #pragma unroll
@@ -1103,12 +1127,15 @@ k_yf_omega(Geo g, const cplx<R>* __restrict__ psi, cplx<R>* __restrict__ Pbx, cp
       fin = fin && (!in || (finite_c(p) && finite_c(pm)));
       h0[it][t] = nyq ? aherm(p, pm, rinv) : herm(p, pm, rinv);
       h1[it][t] = nyq ? aherm(up, upm, rinv) : herm(up, upm, rinv);
+      if (CHK == 2 && in) bsum += wx * ((double)fabs(h0[it][t].re) + (double)fabs(h0[it][t].im));
       if (it == 0 && t == TN) {
         a0 = aherm(p, pm, rinv);
         a1 = aherm(up, upm, rinv);
       }
     }
   if (CHK && !fin) atomicOr(flag, 1);
+  // the regular blocks cover every column once (the Nyquist blocks repeat x = nx/2)
+  if (CHK == 2 && !nyq) fold_block_sum<TL::THREADS>(bsum, lsum + z);  // uniform
   // psi_z = (psi(zu) - psi(z)) / dz (exactly 0 at the top, where zu = z) and
   // psi-bar = (psi(zu) + psi(z)) / 2 (psi itself at the top)
   auto dzf = [&](cplx<R> u1, cplx<R> u0) { return mk<R>((u1.re - u0.re) / rdz, (u1.im - u0.im) / rdz); };
@@ -1121,7 +1148,7 @@ k_yf_omega(Geo g, const cplx<R>* __restrict__ psi, cplx<R>* __restrict__ Pbx, cp
   for (int o = 0; o < 7; ++o) {
     const bool kxo = o == 0 || o == 2 || o == 5;  // multiplier odd in kx
     if ((o == 4 || o == 5) && !UV) continue;
-    if (o == 6 && !CHK) continue;
+    if (o == 6 && CHK != 1) continue;
     if (nyq && !kxo) continue;  // uniform
     // ky and kh2 loaded again for every output (kept, they would hold 32
     // registers through every transform)
@@ -1193,6 +1220,8 @@ __device__ inline void psi_row_max(cplx<R> (&v)[FI::IT][FI::R0], const cplx<R>* 
 // each pair packed into one complex transform.
 // CHK = 1 and Ppsi set: also the inverse x transform of psi's half rows (from
 // k_yf_omega), whose largest |value| is folded into absmax (psi_row_max).
+// lsum set (k_yf_omega's CHK = 2 sums over nlev levels): the first block
+// folds their largest into absmax and clears them for the next step.
 // FP64 runs the plain pass as the CHK = 0 instantiation (the check's code
 // path cost it a fifth of its speed); FP32 runs CHK = 1 either way (the
 // form that measured fastest).
@@ -1202,10 +1231,20 @@ k_xf(Geo g, int want_omega, const cplx<R>* __restrict__ Pu, const cplx<R>* __res
      const cplx<R>* __restrict__ Pbx, const cplx<R>* __restrict__ Pby, const cplx<R>* __restrict__ Pxx,
      const cplx<R>* __restrict__ Pxy, R* __restrict__ u, R* __restrict__ vv, cplx<R>* __restrict__ Rw,
      const cplx<R>* __restrict__ Ppsi, unsigned long long* __restrict__ absmax,
-     const cplx<R>* __restrict__ tw) {
+     double* __restrict__ lsum, int nlev, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NX, V>;
   using FI = Fft<R, NX, +1, TL::PL>;
   using FF = Fft<R, NX, -1, TL::PL>;
+  if (lsum && blockIdx.x == 0) {  // uniform
+    double m = 0.0;
+    for (int i = threadIdx.x; i < nlev; i += TL::THREADS) {
+      const double a = lsum[i];
+      m = (a > m || a != a) ? a : m;
+      lsum[i] = 0.0;
+    }
+    fold_block_max<TL::THREADS>(m, absmax);
+    __syncthreads();  // its block array is free again (psi_row_max uses it too)
+  }
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
   TwTab<R, TL::NN, TL::PL, sizeof(R) == 8> twr;  // FP64: register twiddles (measured faster)

@@ -51,13 +51,17 @@ struct OpsV {
                     const C* A, const C* Bn, C* rhs0, C* Bout, C* T1, C* T3, EtdView etd, double alpha,
                     int* flag, const C* twy);
   // Pu != nullptr: also the y-transformed half rows of u and v (as yb_adv);
-  // Ppsi != nullptr: also those of psi, and psi's non-finite test into flag
+  // Ppsi != nullptr: also those of psi, and psi's non-finite test into flag;
+  // lsum != nullptr (instead of Ppsi): psi's non-finite test and the per-level
+  // bound of the check's max |ifft2(psi)| added into lsum[level]
   static void yf_omega(const Geo& g, hipStream_t st, const C* psi, C* Pbx, C* Pby, C* Pxx, C* Pxy, C* Pu,
-                       C* Pv, C* Ppsi, int* flag, const C* twy);
-  // Ppsi != nullptr: max |ifft2(psi)| from yf_omega's psi half rows into absmax
+                       C* Pv, C* Ppsi, int* flag, double* lsum, const C* twy);
+  // Ppsi != nullptr: max |ifft2(psi)| from yf_omega's psi half rows into absmax;
+  // lsum != nullptr: the largest of yf_omega's level bounds into absmax, and
+  // lsum cleared
   static void xf(const Geo& g, hipStream_t st, int want_omega, const C* Pu, const C* Pv, const C* Pbx,
                  const C* Pby, const C* Pxx, const C* Pxy, R* u, R* v, C* Rw, const C* Ppsi,
-                 unsigned long long* absmax, const C* twx);
+                 unsigned long long* absmax, double* lsum, const C* twx);
   static void y_r2c_mask(const Geo& g, hipStream_t st, const C* Rw, C* out, const C* twy);
   static void y_herm(const Geo& g, hipStream_t st, const C* in, C* P, const C* twy);
   // plain complex 2-D transforms of double fields (dir = -1 forward, +1 inverse)

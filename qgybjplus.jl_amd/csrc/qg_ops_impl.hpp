@@ -183,19 +183,24 @@ void OpsV<R, V>::yb2_B(const Geo& g, hipStream_t st, int mode, int flags, const 
 
 template <typename R, int V>
 void OpsV<R, V>::yf_omega(const Geo& g, hipStream_t st, const C* psi, C* Pbx, C* Pby, C* Pxx, C* Pxy, C* Pu,
-                      C* Pv, C* Ppsi, int* flag, const C* twy) {
+                      C* Pv, C* Ppsi, int* flag, double* lsum, const C* twy) {
   // one tile column more than the half width: the Nyquist column's kx outputs
-  if (Pu && Ppsi) {
-#define CALL(N) YL((k_yf_omega<R, N, V, 1, 1>), N, (g.W + Tile<R, N, V>::S), g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, Ppsi, flag, twy)
+  if ((Ppsi || lsum) && !Pu) throw std::runtime_error("yf_omega: the check needs the velocity outputs");
+  if (Ppsi && lsum) throw std::runtime_error("yf_omega: the exact check or its bound, not both");
+  if (Pu && lsum) {
+#define CALL(N) YL((k_yf_omega<R, N, V, 1, 2>), N, (g.W + Tile<R, N, V>::S), g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, Ppsi, flag, lsum, twy)
+    QG_SWITCH(g.ny, CALL)
+#undef CALL
+  } else if (Pu && Ppsi) {
+#define CALL(N) YL((k_yf_omega<R, N, V, 1, 1>), N, (g.W + Tile<R, N, V>::S), g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, Ppsi, flag, lsum, twy)
     QG_SWITCH(g.ny, CALL)
 #undef CALL
   } else if (Pu) {
-#define CALL(N) YL((k_yf_omega<R, N, V, 1, 0>), N, (g.W + Tile<R, N, V>::S), g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, Ppsi, flag, twy)
+#define CALL(N) YL((k_yf_omega<R, N, V, 1, 0>), N, (g.W + Tile<R, N, V>::S), g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, Ppsi, flag, lsum, twy)
     QG_SWITCH(g.ny, CALL)
 #undef CALL
   } else {
-    if (Ppsi) throw std::runtime_error("yf_omega: the check needs the velocity outputs");
-#define CALL(N) YL((k_yf_omega<R, N, V, 0, 0>), N, (g.W + Tile<R, N, V>::S), g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, Ppsi, flag, twy)
+#define CALL(N) YL((k_yf_omega<R, N, V, 0, 0>), N, (g.W + Tile<R, N, V>::S), g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, Ppsi, flag, lsum, twy)
     QG_SWITCH(g.ny, CALL)
 #undef CALL
   }
@@ -204,13 +209,13 @@ void OpsV<R, V>::yf_omega(const Geo& g, hipStream_t st, const C* psi, C* Pbx, C*
 template <typename R, int V>
 void OpsV<R, V>::xf(const Geo& g, hipStream_t st, int want_omega, const C* Pu, const C* Pv, const C* Pbx,
                 const C* Pby, const C* Pxx, const C* Pxy, R* u, R* v, C* Rw, const C* Ppsi,
-                unsigned long long* absmax, const C* twx) {
+                unsigned long long* absmax, double* lsum, const C* twx) {
   if (Ppsi || sizeof(R) == 4) {
-#define CALL(N) XL((k_xf<R, N, V, 1>), N, g, want_omega, Pu, Pv, Pbx, Pby, Pxx, Pxy, u, v, Rw, Ppsi, absmax, twx)
+#define CALL(N) XL((k_xf<R, N, V, 1>), N, g, want_omega, Pu, Pv, Pbx, Pby, Pxx, Pxy, u, v, Rw, Ppsi, absmax, lsum, g.nz, twx)
     QG_SWITCH(g.nx, CALL)
 #undef CALL
   } else {
-#define CALL(N) XL((k_xf<R, N, V, 0>), N, g, want_omega, Pu, Pv, Pbx, Pby, Pxx, Pxy, u, v, Rw, Ppsi, absmax, twx)
+#define CALL(N) XL((k_xf<R, N, V, 0>), N, g, want_omega, Pu, Pv, Pbx, Pby, Pxx, Pxy, u, v, Rw, Ppsi, absmax, lsum, g.nz, twx)
     QG_SWITCH(g.nx, CALL)
 #undef CALL
   }

@@ -110,13 +110,18 @@ struct qg_handle {
   unsigned long long* d_absmax = nullptr;
   unsigned long long* d_pmax = nullptr;  // progress maxima: wave speed, flow speed
   int* d_flag = nullptr;
+  // per stacked level: the bound of the termination check's max |ifft2(psi)|
+  // that a bound-checked step accumulates (yf_omega) and clears (xf)
+  double* d_lsum = nullptr;
   bool diag_valid = false, t1_valid = false;
   // hipGraphs of two steady-state steps starting from each parity of the
   // double buffer (after two swaps the buffers are back in place)
   bool graphs = true;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
-  // the same two steps, each followed by the termination check (qg_run)
+  // the same two steps, each followed by the termination check (qg_run's
+  // replay after a failing chunk), or by its bound (qg_run's chunks)
   hipGraphExec_t graph_chk[2] = {nullptr, nullptr};
+  hipGraphExec_t graph_bnd[2] = {nullptr, nullptr};
   // the state a step starts from (q, B and the diagnostics psi, A, T1, u, v
   // it reuses) at the start of a qg_run chunk; allocated by the first qg_run
   void* snap = nullptr;
@@ -169,7 +174,7 @@ struct qg_handle {
     return p;
   }
   void drop_graphs() {
-    for (hipGraphExec_t* set : {graph, graph_chk})
+    for (hipGraphExec_t* set : {graph, graph_chk, graph_bnd})
       for (int i = 0; i < 2; ++i)
         if (set[i]) {
           (void)hipGraphExecDestroy(set[i]);
@@ -436,18 +441,21 @@ struct Engine {
 
   // u, v and the QG omega-equation w (compute_velocities!(compute_w=true)).
   // One Y-pass over psi makes the half rows of u and v with the four of the
-  // omega right-hand side.  checked: the same passes also make the
+  // omega right-hand side.  chk = 1: the same passes also make the
   // termination check of psi (non-finite values; max |ifft2(psi)| from psi's
-  // own half rows, in Pz until y_herm_w reuses it).
-  void velocities_w(const C* psi, bool checked = false) {
-    C* Ppsi = checked ? c(h.Pz) : nullptr;
-    K("yf_omega", qg::P_YF_OMEGA, P * cS + Ph * (checked ? 7 : 6) * cR, [&] {
+  // own half rows, in Pz until y_herm_w reuses it); chk = 2: the non-finite
+  // test and, in place of the maximum, its bound (the largest over the levels
+  // of sum |psi's Hermitian part| / (nx ny)) into d_absmax.
+  void velocities_w(const C* psi, int chk = 0) {
+    C* Ppsi = chk == 1 ? c(h.Pz) : nullptr;
+    double* lsum = chk == 2 ? h.d_lsum : nullptr;
+    K("yf_omega", qg::P_YF_OMEGA, P * cS + Ph * (chk == 1 ? 7 : 6) * cR, [&] {
       O::yf_omega(h.tune[qg::P_YF_OMEGA], h.g, h.st, psi, c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), c(h.Pu), c(h.Pv),
-                  Ppsi, h.d_flag, twy());
+                  Ppsi, h.d_flag, lsum, twy());
     });
-    K("xf", qg::P_XF, Ph * (checked ? 8 : 7) * cR + P * 2 * rR, [&] {
+    K("xf", qg::P_XF, Ph * (chk == 1 ? 8 : 7) * cR + P * 2 * rR, [&] {
       O::xf(h.tune[qg::P_XF], h.g, h.st, 1, c(h.Pu), c(h.Pv), c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), r(h.u), r(h.v), c(h.Pq),
-            Ppsi, h.d_absmax, twx());
+            Ppsi, h.d_absmax, lsum, twx());
     });
     K("y_r2c_omega", qg::P_Y_R2C, Ph * cR + P * cS, [&] { O::y_r2c_mask(h.tune[qg::P_Y_R2C], h.g, h.st, c(h.Pq), c(h.rq0), twy()); });
     if (h.sing_omega[0]) {
@@ -530,12 +538,14 @@ struct Engine {
   }
 
   // step!(model, ExponentialRungeKutta2) -> _advance_etdrk2! (src/timestep.jl:335-415).
-  // checked: the step's last passes also make _check_termination_conditions!
+  // chk = 1: the step's last passes also make _check_termination_conditions!
   // (src/simulation.jl:25-44) of the state they leave, accumulated into
   // d_flag and d_absmax: non-finite q and B where the final update writes
   // them, non-finite psi and max |ifft2(psi)| in the finalize's velocity and
   // omega passes, which read psi anyway.  The values are those of check().
-  void step(bool checked = false) {
+  // chk = 2: the same with an upper bound of max |ifft2(psi)| in d_absmax
+  // (velocities_w).
+  void step(int chk = 0) {
     C* qn = c(h.q[h.cur]);
     C* Bn = c(h.B[h.cur]);
     C* q1 = c(h.q[1 - h.cur]);
@@ -557,11 +567,11 @@ struct Engine {
     wave_inv(c(h.Bs), c(h.As), nullptr);
     // the stage's u, v come out of the RHS passes (they are needed nowhere else)
     rhs_update(qg::MODE_FINAL, c(h.qs), psiS, c(h.As), qn, Bn, q1, B1, need_uv && !h.fixed_flow,
-               checked ? h.d_flag : nullptr);
+               chk ? h.d_flag : nullptr);
     // finalize (src/timestep.jl:300-332); yb2_B left T1, T3 of B_{n+1}
     diag(q1, B1, c(h.psi), false, true);
     wave_inv(B1, c(h.A), c(h.C));
-    velocities_w(c(h.psi), checked);
+    velocities_w(c(h.psi), chk);
     h.cur = 1 - h.cur;
     h.diag_valid = true;
     h.t1_valid = true;
@@ -651,17 +661,18 @@ void fft2_inplace(qg_handle& h, D* f, int dir, D* out = nullptr, int levels = -1
 
 // `nsteps` steps of the engine: steady-state pairs (diagnostics valid, no
 // per-kernel events) replay a captured two-step graph, the others run
-// eagerly.  With `checked`, every step is followed by the termination check
-// of the active members, inside the graph too.  mark(n) follows every n steps.
+// eagerly.  With chk = 1 every step is followed by the termination check of
+// the active members, inside the graph too; chk = 2 by its bound (Engine::step).
+// mark(n) follows every n steps.
 template <class E, class Mark>
-void advance(qg_handle& H, E& e, int nsteps, bool checked, Mark&& mark) {
+void advance(qg_handle& H, E& e, int nsteps, int chk, Mark&& mark) {
   const bool use_graphs = H.graphs && !(H.prof_mode & 1);
-  auto one = [&] { e.step(checked); };  // the check is fused into the step's last passes
+  auto one = [&] { e.step(chk); };  // the check is fused into the step's last passes
   int i = 0;
   while (i < nsteps) {
     if (use_graphs && H.diag_valid && nsteps - i >= 2) {
       const int c = H.cur;
-      hipGraphExec_t& gx = checked ? H.graph_chk[c] : H.graph[c];
+      hipGraphExec_t& gx = chk == 2 ? H.graph_bnd[c] : chk == 1 ? H.graph_chk[c] : H.graph[c];
       if (!gx) {
         hipGraph_t g = nullptr;
         HIPCHECK(hipStreamBeginCapture(H.st, hipStreamCaptureModeThreadLocal));
@@ -875,6 +886,8 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
     h->d_absmax = h->alloc<unsigned long long>(1);
     h->d_pmax = h->alloc<unsigned long long>(2);
     h->d_flag = h->alloc<int>(1);
+    h->d_lsum = h->alloc<double>(h->nz);
+    HIPCHECK(hipMemset(h->d_lsum, 0, h->nz * sizeof(double)));
     HIPCHECK(hipMemset(h->d_err, 0, 4 * sizeof(int)));
     HIPCHECK(hipMemset(h->d_warn, 0, sizeof(unsigned long long)));
     for (void* p : {h->q[0], h->q[1], h->B[0], h->B[1], h->psi, h->A, h->C})
@@ -1090,9 +1103,21 @@ void check_range(qg_handle& H, int m0, int nm, int* nonfinite, double* max_abs_p
   if (max_abs_psi) *max_abs_psi = m;
 }
 
+// The bound a chunk's steps must stay under to pass without the exact check:
+// 1e10 less a margin that covers the transform's rounding of the exact
+// maximum (relative 1e-6 at FP32).
+constexpr double kBoundPass = 1e10 * (1.0 - 1e-4);
+
 // run!'s stepping loop with the check on the device, shared by qg_run and
 // qg_run_noalloc: `snap` holds 12 R scalars per point of the active members;
 // graphs = false launches every step eagerly (no graph is instantiated).
+// A chunk's steps are checked against the bound of max |ifft2(psi)| (the
+// non-finite tests in full): when every step's bound is under kBoundPass
+// the exact maximum is too, and the chunk passes.  Otherwise the chunk is
+// stepped again from its snapshot with the exact check after every step, up
+// to the first failing step, so the stop, the failure kind and the state at
+// the stop are those of the exact check.  After a run without failure the
+// state it leaves is checked once more, exactly, for max_abs_psi.
 void run_checked(qg_handle& H, double dt, int nsteps, void* snap, bool graphs, int& d, int& fail,
                  double& mx) {
   d = 0;
@@ -1124,6 +1149,7 @@ void run_checked(qg_handle& H, double dt, int nsteps, void* snap, bool graphs, i
   auto clear = [&] {
     HIPCHECK(hipMemsetAsync(H.d_flag, 0, sizeof(int), H.st));
     HIPCHECK(hipMemsetAsync(H.d_absmax, 0, sizeof(unsigned long long), H.st));
+    HIPCHECK(hipMemsetAsync(H.d_lsum, 0, H.nz * sizeof(double), H.st));
   };
   auto none = [](int) {};
   const bool graphs_were = H.graphs;
@@ -1140,38 +1166,41 @@ void run_checked(qg_handle& H, double dt, int nsteps, void* snap, bool graphs, i
       const bool dv = H.diag_valid, tv = H.t1_valid;
       copy_parts(c0, true);
       clear();
-      advance(H, e, k, true, none);
+      advance(H, e, k, 2, none);
       int flag = 0;
       double m = 0.0;
       read_check(H, flag, m);
       check_status(H);
-      if (flag == 0 && m <= 1e10) {
+      if (flag == 0 && m <= kBoundPass) {
         d += k;
-        mx = std::max(mx, m);
         continue;
       }
-      // A step of this chunk failed: restore the chunk's starting state
-      // and step again, one step and one check at a time, stopping right
-      // after the failing step.  The same kernels on the same state give
-      // the same values, so the state at the stop is the one the failing
-      // step produced.
+      // A step of this chunk failed or its bound is too high: restore the
+      // chunk's starting state and step again, one step and one exact check
+      // at a time, stopping right after a failing step.  The same kernels on
+      // the same state give the same values, so the state at the stop is the
+      // one the failing step produced.
       copy_parts(c0, false);
       H.cur = c0;
       H.diag_valid = dv;
       H.t1_valid = tv;
       for (int j = 0; j < k && fail == 0; ++j) {
         clear();
-        advance(H, e, 1, true, none);
+        advance(H, e, 1, 1, none);
         read_check(H, flag, m);
         check_status(H);
         ++d;
         if (flag != 0 || !(m <= 1e10)) {
           fail = flag != 0 ? 1 : 2;
           mx = m;
-        } else {
-          mx = std::max(mx, m);
         }
       }
+    }
+    if (fail == 0 && d > 0) {  // the exact maximum of the state left
+      clear();
+      e.check(0, H.active);
+      int flag = 0;
+      read_check(H, flag, mx);
     }
   });
   HIPCHECK(hipGetLastError());
@@ -1246,7 +1275,7 @@ int qg_step(qg_handle* h, double dt, int nsteps) {
     };
     with_engine(H, [&](auto& e) {
       if (H.prof_mode & 2) mark(0);
-      advance(H, e, nsteps, false, mark);
+      advance(H, e, nsteps, 0, mark);
     });
     HIPCHECK(hipGetLastError());
     check_status(H);

@@ -1,8 +1,9 @@
 """run!'s stepping loop with the termination check made on the device
 (qg_run; src/simulation.jl:621-625 and _check_termination_conditions!,
 src/simulation.jl:25-44).  Against plain stepping with a host check after
-every step: the same fields bit for bit, the same maxima, and a stop right
-after the first failing step with the reference's failure kind."""
+every step: the same fields bit for bit, the same maximum (the failing
+step's, else the last step's), and a stop right after the first failing
+step with the reference's failure kind."""
 
 import math
 
@@ -34,16 +35,16 @@ def _pair(qg, precision, urms, seed=5):
 
 
 def _stepwise(qg, m, dt, n):
-    """step! then the check, one step at a time: (steps, failure, max)."""
+    """step! then the check, one step at a time: (steps, failure, max |ifft
+    psi| of the failing step, else of the last)."""
     ts = qg.ExponentialRungeKutta2(dt=dt)
-    mx = 0.0
+    v = 0.0
     for i in range(1, n + 1):
         qg.step_(m, ts, 1)
         bad, v = m._h.check()
         if bad or not v <= 1e10:
             return i, (1 if bad else 2), v
-        mx = max(mx, v)
-    return n, 0, mx
+    return n, 0, v
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
@@ -55,6 +56,30 @@ def test_run_equals_stepping_with_checks(qg, precision):
     got = b._h.run(500.0, n)
     assert ref[:2] == (n, 0) and got[:2] == (n, 0)
     assert got[2] == ref[2] > 0
+    for f in FIELDS:
+        assert np.array_equal(getattr(b.fields, f), getattr(a.fields, f)), f
+    a.finalize()
+    b.finalize()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_run_replays_chunks_whose_bound_is_too_high(qg, precision):
+    """qg_run passes a stretch of steps on a bound of max |ifft(psi)| (the
+    largest over the levels of sum |psi| over the spectrum / (nx ny)).
+    Here max |ifft(psi)| is about 5e9 while the bound is about 2.4e10: every
+    stretch is stepped again with the exact check, nothing fails, and the
+    result is that of step-then-check, bit for bit."""
+    print(f"case run with the bound above the limit {precision}", flush=True)
+    a, b = _pair(qg, precision, urms=1e5)
+    nx, ny, _ = a.grid.size
+    bound = float(np.max(np.sum(np.abs(a.fields.psi), axis=(1, 2)))) / (nx * ny)
+    exact = a._h.check()[1]
+    assert bound > 1.5e10 and 1e9 < exact < 0.9e10, (bound, exact)
+    n = 40  # one chunk of 32 steps and a tail
+    ref = _stepwise(qg, a, 1e-3, n)
+    got = b._h.run(1e-3, n)
+    assert ref[:2] == (n, 0) and got[:2] == (n, 0)
+    assert got[2] == ref[2] > 1e9
     for f in FIELDS:
         assert np.array_equal(getattr(b.fields, f), getattr(a.fields, f)), f
     a.finalize()
