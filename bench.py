@@ -391,6 +391,28 @@ def per_kernel_table(kern):
             for k, v in kern.items()}
 
 
+def fp64_only(qg, args):
+    """The FP64 model alone (its profiler run): autotuned, `steps` timed
+    steps, then per-kernel events over min(steps, 200) eager steps."""
+    grid, m = build_model(qg, "fp64")
+    psi, _ = workload_inputs(grid)
+    set_workload(qg, m, psi)
+    choice = m.autotune(DT, steps=6) if args.autotune else None
+    log("timing the FP64 model")
+    t, _ = timed_run(qg, m, args.steps, args.warmup)
+    kernel_steps = min(args.steps, 200)
+    prof = kernel_run(qg, m, kernel_steps)
+    m.finalize()
+    npts = SIZE[0] * SIZE[1] * SIZE[2]
+    return {"metric": METRIC, "value": npts * args.steps / t / 1e6, "unit": "Mgridpt-steps/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * t / args.steps,
+            "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "the 256x256x128 workload, FP64 model only", "grid": list(SIZE), "dt": DT,
+                       "precision": "fp64"},
+            "kernels_fp64": per_kernel_table(prof["kernels"]), "kernel_profile_steps": kernel_steps,
+            "autotune": {"fp64": {"choice": choice}} if choice is not None else None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -408,8 +430,8 @@ def main():
                     help="iterations of run!(simulation) (every step checked) on the mixed model")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="run every pass at its default tile variant / launch shape")
-    ap.add_argument("--modes", choices=("both", "mixed"), default="both",
-                    help="'mixed' times only the mixed-precision model (for profiler runs)")
+    ap.add_argument("--modes", choices=("both", "mixed", "fp64"), default="both",
+                    help="'mixed' or 'fp64' times only that model (for profiler runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run one device copy of MIB MiB (the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -435,7 +457,10 @@ def main():
 
     from __graft_entry__ import load_package
     qg = load_package()
-    from qgybjplus_amd import synthetic
+
+    if args.modes == "fp64":
+        print(json.dumps(fp64_only(qg, args)), flush=True)
+        return
 
     grid, m32 = build_model(qg, "mixed")
     m64 = build_model(qg, "fp64")[1] if args.modes == "both" else None
