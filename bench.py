@@ -382,6 +382,64 @@ def cpu_baseline(psi_phys, B_phys, budget_s=25.0, workers=1):
 ERROR_STEPS = 1000  # the steps the mixed-precision error bar is stated after
 
 
+def hbm_ceiling(nbytes, nbuf=16, reps=64):
+    """The streaming rate this GPU reaches on arrays of one field's size:
+    back-to-back device-to-device copies (the runtime's copy kernel) of
+    `nbytes` among `nbuf` buffers, each copy reading the buffer written
+    nbuf/2 copies earlier (nbuf/2 * nbytes past it, well beyond the 256 MB
+    infinity cache) and writing another, as the step's kernels read arrays
+    written several launches before.  Through the HIP runtime directly; bytes
+    read plus bytes written per second."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+    hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+    hip.hipEventSynchronize.argtypes = [C.c_void_p]
+    hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+    hip.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def ok(err, what):
+        if err != 0:
+            raise RuntimeError(f"{what}: hip error {err}")
+
+    bufs = []
+    try:
+        for _ in range(nbuf):
+            p = C.c_void_p()
+            ok(hip.hipMalloc(C.byref(p), nbytes), "hipMalloc")
+            bufs.append(p)
+            ok(hip.hipMemset(p, 0, nbytes), "hipMemset")
+        ev = [C.c_void_p(), C.c_void_p()]
+        for e in ev:
+            ok(hip.hipEventCreate(C.byref(e)), "hipEventCreate")
+
+        def copies(n):
+            for i in range(n):
+                k = i % nbuf
+                ok(hip.hipMemcpyAsync(bufs[(k + nbuf // 2) % nbuf], bufs[k], nbytes, 3, None), "hipMemcpyAsync")
+
+        copies(nbuf)
+        ok(hip.hipEventRecord(ev[0], None), "hipEventRecord")
+        copies(reps)
+        ok(hip.hipEventRecord(ev[1], None), "hipEventRecord")
+        ok(hip.hipEventSynchronize(ev[1]), "hipEventSynchronize")
+        ms = C.c_float()
+        ok(hip.hipEventElapsedTime(C.byref(ms), ev[0], ev[1]), "hipEventElapsedTime")
+        for e in ev:
+            hip.hipEventDestroy(e)
+    finally:
+        for p in bufs:
+            hip.hipFree(p)
+    rate = 2.0 * nbytes * reps / (ms.value / 1e3)
+    return {"copy_bytes": nbytes, "GBps": rate / 1e9, "frac_of_peak": rate / PEAK_HBM,
+            "method": f"{reps} device-to-device copies of {nbytes >> 20} MiB among {nbuf} buffers, each reading "
+                      f"the buffer written {nbuf // 2} copies before (runtime copy kernel, HIP events)"}
+
+
 def per_kernel_table(kern):
     tot = sum(x["ms"] for x in kern.values())
     return {k: {"launches": v["count"], "avg_us": 1e3 * v["ms"] / v["count"],
@@ -554,6 +612,9 @@ def main():
         "autotune": tuning or None,
     }
     out["run_loop"] = run_loop_leg(qg, m32, args.run_steps) if args.run_steps > 0 else None
+    # the streaming rate at one complex FP32 field's size: what "% of 8 TB/s"
+    # can reach on this GPU at this array size
+    out["hbm_ceiling"] = hbm_ceiling(npts * 8)
     m32.finalize()
     if m64 is not None:
         m64.finalize()
