@@ -250,6 +250,18 @@ int qg_move_member(qg_handle* h, int src, int dst);
  * runs every step eagerly.  On by default.  No reference counterpart. */
 int qg_use_graphs(qg_handle* h, int enable);
 
+/* Dealiased-disk pruning of the step.  The 2/3 mask (src/physics.jl:28-43)
+ * keeps the disk ix^2 + iy^2 <= kmax^2 of the spectral plane, and after one
+ * masked update every spectral field of the state is zero outside it.  A
+ * pruned step then reads and writes spectral data inside the disk only, runs
+ * its y transforms over the columns |kx| <= kmax only, and solves the columns
+ * inside the disk only; its results are those of the unpruned step (the work
+ * left out is transforms of zeros and values the mask discards).  The first
+ * step after any set-up or operator call runs unpruned.  Models whose flow is
+ * fixed (psi is never re-diagnosed from q) always step unpruned.  On by
+ * default; enable = 0 steps every step unpruned.  No reference counterpart. */
+int qg_set_pruning(qg_handle* h, int enable);
+
 /* Waits for the handle's stream. */
 int qg_synchronize(qg_handle* h);
 

@@ -36,7 +36,7 @@ EXPORTED = (
     "qg_compute_velocities", "qg_compute_q_from_psi", "qg_compute_qw", "qg_fft",
     "qg_synchronize", "qg_take_mean_q_warning", "qg_profile_begin", "qg_profile_end",
     "qg_set_field_member", "qg_get_field_member", "qg_check_member",
-    "qg_compute_q_from_psi_member", "qg_invert_B_to_A_member", "qg_use_graphs",
+    "qg_compute_q_from_psi_member", "qg_invert_B_to_A_member", "qg_use_graphs", "qg_set_pruning",
     "qg_set_member_stratification", "qg_memory_required", "qg_device_memory",
     "qg_set_active_members", "qg_move_member", "qg_autotune", "qg_get_tuning", "qg_set_tuning",
     "qg_set_random_psi", "qg_set_surface_wave", "qg_energies",
@@ -108,6 +108,7 @@ def load() -> C.CDLL:
         "qg_compute_q_from_psi_member": (C.c_int, [P, C.c_int, C.c_int]),
         "qg_invert_B_to_A_member": (C.c_int, [P, C.c_int]),
         "qg_use_graphs": (C.c_int, [P, C.c_int]),
+        "qg_set_pruning": (C.c_int, [P, C.c_int]),
         "qg_set_active_members": (C.c_int, [P, C.c_int]),
         "qg_move_member": (C.c_int, [P, C.c_int, C.c_int]),
         "qg_set_member_stratification": (C.c_int, [P, C.c_int, dp, dp]),
@@ -280,6 +281,10 @@ class Handle:
 
     def use_graphs(self, enable: bool = True):
         self._check(self._lib.qg_use_graphs(self.h, int(bool(enable))))
+
+    def set_pruning(self, enable: bool = True):
+        """qg_set_pruning: steps pruned to the dealiased disk (default) or not."""
+        self._check(self._lib.qg_set_pruning(self.h, int(bool(enable))))
 
     def step(self, dt: float, nsteps: int = 1):
         self._check(self._lib.qg_step(self.h, float(dt), int(nsteps)))

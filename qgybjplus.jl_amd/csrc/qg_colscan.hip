@@ -12,7 +12,7 @@ template <int OP, int M, int QC, typename T>
 void launch_shape(const Geo& g, hipStream_t st, const ScanArgs& a) {
   const Quarter qr = quarter(g);
   const ScanShape s{M, QC, (g.nzc + M - 1) / M};
-  const int blocks = (qr.size() + QC - 1) / QC;
+  const int blocks = scan_tiles(g, QC);
   hipLaunchKernelGGL((k_colscan<OP, M, QC, T>), dim3(blocks, g.members), dim3(s.threads()), s.lds_bytes(), st, g,
                      qr, a);
 }

@@ -37,10 +37,11 @@ namespace qg {
 
 struct Quarter {
   int Wr, Hr;  // nx/2 + 1, ny/2 + 1
+  int kd2;     // pruned (Geo::kd2 >= 0): only the quarter columns rx^2 + ry^2 <= kd2 are solved
   __host__ __device__ int size() const { return Wr * Hr; }
 };
 
-inline Quarter quarter(const Geo& g) { return Quarter{g.nx / 2 + 1, g.ny / 2 + 1}; }
+inline Quarter quarter(const Geo& g) { return Quarter{g.nx / 2 + 1, g.ny / 2 + 1, g.kd2}; }
 
 struct QuadCol {
   int x, y;
@@ -48,11 +49,14 @@ struct QuadCol {
 };
 
 // Grid column of mirror image m (bit 0: -kx, bit 1: -ky) of quarter column
-// rc; inactive past the quarter or when it duplicates a lower image.
-__device__ inline QuadCol quad_col(const Geo& g, const Quarter& qr, int rc, int m) {
+// rc; inactive past the quarter, outside the disk of a pruned solve, off the
+// quarter row `row` of its tile (row >= 0), or when it duplicates a lower
+// image.
+__device__ inline QuadCol quad_col(const Geo& g, const Quarter& qr, int rc, int m, int row = -1) {
   QuadCol q{0, 0, false};
   if (rc >= qr.size()) return q;
   const int rx = rc % qr.Wr, ry = rc / qr.Wr;
+  if ((row >= 0 && ry != row) || (qr.kd2 >= 0 && rx * rx + ry * ry > qr.kd2)) return q;
   q.x = (m & 1) ? (g.nx - rx) % g.nx : rx;
   q.y = (m & 2) ? (g.ny - ry) % g.ny : ry;
   q.active = !(((m & 1) && q.x == rx) || ((m & 2) && q.y == ry));
@@ -156,6 +160,14 @@ __device__ inline void scan_store(double v, __amdgpu_buffer_rsrc_t r, unsigned o
   }
 }
 
+// Tiles of a pruned solve: QC consecutive quarter columns of one quarter row
+// ry <= kmax (tiles per row: scan_row_tiles), so that the disk's columns of a
+// row are covered by its first tiles; tiles wholly outside the disk exit.
+__host__ __device__ inline int scan_row_tiles(const Geo& g, int QC) { return (g.Kx + QC - 1) / QC; }
+inline int scan_tiles(const Geo& g, int QC) {
+  return g.pruned() ? g.Kx * scan_row_tiles(g, QC) : (quarter(g).size() + QC - 1) / QC;
+}
+
 // Logical block x: tile of QC quarter columns; y: member (xcd_block()).
 template <int OP, int M, int QC, typename T>
 __global__ void __launch_bounds__((ScanBounds<M, QC>::MAXT))
@@ -170,7 +182,13 @@ k_colscan(Geo g, Quarter qr, ScanArgs a) {
   const int part = t & 1, cs = t >> 1, j = cs % QC, mir = cs / QC;
   const int rplane = qr.size();
   const Blk blk = xcd_block();
-  const int rc0 = blk.x * QC;
+  int rc0 = blk.x * QC, row = -1;
+  if (g.pruned()) {  // uniform
+    const int tpr = scan_row_tiles(g, QC), ry = blk.x / tpr, rx0 = (blk.x % tpr) * QC;
+    if (rx0 * rx0 + ry * ry > g.kd2) return;  // the whole tile is outside the disk
+    rc0 = ry * qr.Wr + rx0;
+    row = ry;
+  }
   const size_t plane = (size_t)g.nx * g.ny;
   const size_t mem = blk.y;
   const T* rhs = static_cast<const T*>(a.rhs) + 2 * mem * nz * plane;
@@ -188,7 +206,7 @@ k_colscan(Geo g, Quarter qr, ScanArgs a) {
   double* XF = L1 + kpad;         // [2][P][TP]: forward end values, products
   double* XB = XF + 2 * P * TP;   // [2][P][TP]: backward end values, products
 
-  const QuadCol q = quad_col(g, qr, rc0 + j, mir);
+  const QuadCol q = quad_col(g, qr, rc0 + j, mir, row);
   // This thread's column part within a level, in doubles.  With 64 column
   // parts per partition a partition is one wave, so its first level is
   // wave-uniform and every level's address is a scalar base plus this offset.

@@ -113,7 +113,26 @@ __global__ void k_zero(unsigned int* __restrict__ p, size_t nwords) {
   for (; i < nwords; i += stride) p[i] = 0u;
 }
 
+// Zeroes the points of a spectral field outside the dealiased disk of g (all
+// g.nz levels): each thread owns one (x, y) of the plane.
+template <typename T>
+__global__ void k_zero_outside_disk(Geo g, cplx<T>* __restrict__ f) {
+  const int plane = g.nx * g.ny;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= plane || g.in_disk(i % g.nx, i / g.nx)) return;
+  for (int z = blockIdx.y; z < g.nz; z += gridDim.y) f[(size_t)z * plane + i] = mk<T>((T)0, (T)0);
+}
+
 }  // namespace
+
+void launch_zero_outside_disk(const Geo& g, hipStream_t st, void* field, bool f32) {
+  const int plane = g.nx * g.ny, bs = 256;
+  const dim3 grid((plane + bs - 1) / bs, std::min(g.nz, 1024));
+  if (f32)
+    hipLaunchKernelGGL(k_zero_outside_disk<float>, grid, dim3(bs), 0, st, g, static_cast<cplx<float>*>(field));
+  else
+    hipLaunchKernelGGL(k_zero_outside_disk<double>, grid, dim3(bs), 0, st, g, static_cast<cplx<double>*>(field));
+}
 
 // Zero fill by a kernel (not hipMemsetAsync) so the step stays capturable.
 void launch_zero_bytes(void* p, size_t bytes, hipStream_t st) {

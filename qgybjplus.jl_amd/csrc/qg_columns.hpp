@@ -62,6 +62,9 @@ void launch_transpose(const T* in, T* out, int nx, int ny, int nz, bool to_plana
 template <typename From, typename To>
 void launch_convert(const From* in, To* out, size_t n, hipStream_t st);
 void launch_zero_bytes(void* p, size_t bytes, hipStream_t st);
+// Zeroes a spectral field (cplx<float> when f32, else cplx<double>) outside
+// the dealiased disk of a pruned geometry, on all its g.nz levels.
+void launch_zero_outside_disk(const Geo& g, hipStream_t st, void* field, bool f32);
 void launch_zero(cplx<double>* p, size_t n, hipStream_t st);
 
 }  // namespace qg

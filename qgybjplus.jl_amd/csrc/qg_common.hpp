@@ -46,20 +46,70 @@ __host__ __device__ inline cplx<To> cvt(cplx<From> a) { return mk((To)a.re, (To)
 // nz counts all stacked levels: an ensemble of `members` models of one grid
 // class keeps member m in levels [m*nzc, (m+1)*nzc), so every horizontal pass
 // covers all members in one launch; column operators work per member.
+//
+// Dealiased-disk pruning (kd2 >= 0).  The 2/3 mask keeps the disk
+// ix^2 + iy^2 <= kmax^2 (ix, iy the signed Fourier indices), and a state that
+// has been through one masked update is exactly zero outside it: q, B, psi,
+// A, C and every right-hand side.  A pruned pass then touches spectral data
+// only inside the disk (loads outside it read as zero, stores outside it are
+// skipped and the arrays keep their zeros there), runs its y transforms only
+// over the columns |kx| <= kmax, and carries only those columns in its
+// x-transformed intermediates.  The x transforms still run over whole rows.
+// Every value the pruned passes compute equals the unpruned one: the work
+// left out is transforms of zeros and products that the mask discards.
+// Unpruned (kd2 < 0): Kx = W, every column and point is touched.
 struct Geo {
   int nx, ny, nz;
   int nzc;      // levels per member (the column depth)
   int members;  // nz == members * nzc
   int W;    // nx/2 + 1 : columns kept by half-spectrum (real-field) arrays
   int Wp;   // row stride of half-spectrum arrays (W rounded up to 16)
+  int Kx;   // columns [0, Kx) of half-spectrum arrays carried (kmax + 1 when pruned, else W)
+  int kd2;  // pruned: kmax^2; unpruned: -1
   const double* kx;            // [nx]
   const double* ky;            // [ny]
   const unsigned char* mask;   // [ny][nx] radial 2/3 mask, 1 = keep
   double inv_n;                // 1/(nx*ny)
   double dz;                   // vertical spacing used by the column operators
+
+  __host__ __device__ bool pruned() const { return kd2 >= 0; }
+  // column x carried by a pruned pass (|kx| < Kx); every column otherwise
+  __host__ __device__ bool col_kept(int x) const {
+    const int ix = x <= nx - x ? x : nx - x;
+    return ix < Kx;
+  }
+  // point (x, y) touched by a pruned pass: inside the disk; every point
+  // otherwise
+  __host__ __device__ bool in_disk(int x, int y) const {
+    if (kd2 < 0) return true;
+    const int ix = x <= nx - x ? x : nx - x, iy = y <= ny - y ? y : ny - y;
+    return ix * ix + iy * iy <= kd2;
+  }
 };
 
 __host__ __device__ inline int mirror(int i, int n) { return i == 0 ? 0 : n - i; }
+
+// Tiles of S adjacent columns over the full width: every column, or, when
+// pruned, the columns |kx| <= kmax as tiles over [0, S*nlo) and
+// [nx - S*nhi, nx) (the two ranges carried, each rounded up to whole tiles)
+// when those do not cover the row.  fw_tiles is the number of tiles, fw_x the
+// column of slot s of tile b.
+__host__ __device__ inline void fw_split(const Geo& g, int S, int& nlo, int& nhi) {
+  nlo = (g.Kx + S - 1) / S;
+  nhi = (g.Kx - 1 + S - 1) / S;
+  if (!g.pruned() || (nlo + nhi) * S >= g.nx) nlo = nhi = -1;
+}
+__host__ __device__ inline int fw_tiles(const Geo& g, int S) {
+  int lo, hi;
+  fw_split(g, S, lo, hi);
+  return lo < 0 ? (g.nx + S - 1) / S : lo + hi;
+}
+__host__ __device__ inline int fw_x(const Geo& g, int S, int b, int s) {
+  int lo, hi;
+  fw_split(g, S, lo, hi);
+  if (lo < 0 || b < lo) return b * S + s;
+  return g.nx - (lo + hi - b) * S + s;
+}
 
 // Workgroups reach the 8 XCDs of the chip (each with its own L2) round-robin
 // by linear workgroup id.  xcd_block() renumbers a 2-D grid so that every XCD

@@ -152,8 +152,8 @@ k_ya_qw(Geo g, const cplx<R>* __restrict__ B, cplx<R>* __restrict__ T1,
   TwTab<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const Blk blk = xcd_block();
-  const int x = blk.x * TL::S + s, z = blk.y;
-  const bool ok = x < g.nx;
+  const int x = fw_x(g, TL::S, blk.x, s), z = blk.y;
+  const bool ok = x < g.nx && g.col_kept(x);
   cplx<R>* sl = lds + s * TL::LD;
   cplx<R> v[F::IT][F::R0], w[F::IT][F::R0];
 #pragma unroll
@@ -164,7 +164,7 @@ k_ya_qw(Geo g, const cplx<R>* __restrict__ B, cplx<R>* __restrict__ T1,
       const bool in = ok && F::valid(lane, it);
       const size_t a = ((size_t)z * NY + y) * g.nx + x;
       const bool keep = in && g.mask[in ? y * g.nx + x : 0];
-      const cplx<double> b = scale(cvt<double>(keep_if(ldu(B, a, in), keep)), g.inv_n);
+      const cplx<double> b = scale(cvt<double>(keep_if(ldu(B, a, in && g.in_disk(x, y)), keep)), g.inv_n);
       v[it][t] = cvt<R>(b);
       w[it][t] = cvt<R>(ik_mul(g.ky[y], b));
     }
@@ -209,9 +209,10 @@ k_xa_qw(Geo g, const cplx<R>* __restrict__ T1, const cplx<R>* __restrict__ T3,
 #pragma unroll
     for (int t = 0; t < FI::R0; ++t) {
       int x = FI::elem(lane, it, t);
-      const bool in = ok && FI::valid(lane, it);
-      cplx<R> a = ldu(T1, base + x, in);
-      y3[it][t] = ldu(T3, base + x, in);
+      // columns a pruned pass does not carry are zero
+      const bool in = ok && FI::valid(lane, it), kc = g.col_kept(x);
+      cplx<R> a = keep_if(ldu(T1, base + x, in && kc), kc);
+      y3[it][t] = keep_if(ldu(T3, base + x, in && kc), kc);
       v[it][t] = a;
       w[it][t] = ik_mul((R)g.kx[x], a);
     }
@@ -242,7 +243,7 @@ k_xa_qw(Geo g, const cplx<R>* __restrict__ T1, const cplx<R>* __restrict__ T3,
 #pragma unroll
     for (int t = 0; t < FI::R0; ++t) {
       int x = FI::elem(lane, it, t);
-      if (ok && FI::valid(lane, it) && x < g.W) {
+      if (ok && FI::valid(lane, it) && x < g.Kx) {
         Qa[hb + x] = v[it][t];
         Qb[hb + x] = w[it][t];
       }
@@ -264,7 +265,7 @@ k_ya2_qeff(Geo g, const cplx<R>* __restrict__ Qa, const cplx<R>* __restrict__ Qb
   twr.load(lane, tw);
   const Blk blk = xcd_block();
   const int x = blk.x * TL::S + s, z = blk.y;
-  const bool ok = x < g.W;
+  const bool ok = x < g.Kx;
   cplx<R>* sl = lds + s * TL::LD;
   cplx<R> v[F::IT][F::R0], w[F::IT][F::R0];
 #pragma unroll
@@ -289,6 +290,7 @@ k_ya2_qeff(Geo g, const cplx<R>* __restrict__ Qa, const cplx<R>* __restrict__ Qb
       int y = F::elem(lane, it, t);
       if (!(ok && F::valid(lane, it))) continue;
       const bool keep = g.mask[y * g.nx + x] != 0;
+      if (g.pruned() && !keep) continue;  // zero there already
       const double kh2 = g.kx[x] * g.kx[x] + g.ky[y] * g.ky[y];
       cplx<double> fq = cvt<double>(v[it][t]), fm = cvt<double>(w[it][t]);
       cplx<double> qw = keep ? scale(fq - scale(fm, 0.25 * kh2), inv_f) : czero<double>();
@@ -329,7 +331,7 @@ k_yb_adv(Geo g, int outs_rt, const cplx<R>* __restrict__ psi, const cplx<R>* __r
   twr.load(lane, tw);
   const Blk blk = xcd_block();
   const int x = blk.x * TL::S + s, z = blk.y;
-  const bool ok = x < g.W;
+  const bool ok = x < g.Kx;
   const int mx = mirror(x, g.nx);
   cplx<R>* sl = lds + s * TL::LD;
   const double kxv = ok ? g.kx[x] : 0.0, kxm = ok ? g.kx[mx] : 0.0;
@@ -342,7 +344,8 @@ k_yb_adv(Geo g, int outs_rt, const cplx<R>* __restrict__ psi, const cplx<R>* __r
 #pragma unroll
     for (int t = 0; t < F::R0; ++t) {
       int y = F::elem(lane, it, t), my = mirror(y, NY);
-      bool in = ok && F::valid(lane, it);
+      // (x, y) and its mirror are in the disk together
+      bool in = ok && F::valid(lane, it) && g.in_disk(x, y);
       // guarded by `in`: the clamped form costs this pass a wave per SIMD
       p[it][t] = in ? psi[((size_t)z * NY + y) * g.nx + x] : czero<R>();
       pm[it][t] = in ? psi[((size_t)z * NY + my) * g.nx + mx] : czero<R>();
@@ -401,18 +404,21 @@ k_yb_adv(Geo g, int outs_rt, const cplx<R>* __restrict__ psi, const cplx<R>* __r
 }
 
 // Load one half-width row as a full Hermitian row (element x >= W is
-// conj(element nx-x)).
+// conj(element nx-x)); columns from Kx on (those a pruned pass does not carry)
+// are zero.
 template <typename R, class F>
 __device__ inline void load_herm_row(cplx<R> (&v)[F::IT][F::R0], const cplx<R>* __restrict__ src,
-                                     size_t hb, int nx, int lane, bool ok) {
+                                     size_t hb, int nx, int Kx, int lane, bool ok) {
   const int W = nx / 2 + 1;
 #pragma unroll
   for (int it = 0; it < F::IT; ++it)
 #pragma unroll
     for (int t = 0; t < F::R0; ++t) {
       const int x = F::elem(lane, it, t);
-      const bool in = ok && F::valid(lane, it), lo = x < W;
-      const cplx<R> a = src[in ? hb + (lo ? x : nx - x) : 0];
+      const bool lo = x < W;
+      const int c = lo ? x : nx - x;
+      const bool k = c < Kx, in = ok && F::valid(lane, it) && k;
+      const cplx<R> a = keep_if(src[in ? hb + c : 0], k);
       v[it][t] = lo ? a : conj(a);
     }
 }
@@ -425,16 +431,19 @@ __device__ inline void load_herm_row(cplx<R> (&v)[F::IT][F::R0], const cplx<R>* 
 // one kind and scale (u and v; the two components of a gradient).
 template <typename R, class F>
 __device__ inline void load_herm_pair(cplx<R> (&v)[F::IT][F::R0], const cplx<R>* __restrict__ A,
-                                      const cplx<R>* __restrict__ B, size_t hb, int nx, int lane, bool ok) {
+                                      const cplx<R>* __restrict__ B, size_t hb, int nx, int Kx, int lane,
+                                      bool ok) {
   const int W = nx / 2 + 1;
 #pragma unroll
   for (int it = 0; it < F::IT; ++it)
 #pragma unroll
     for (int t = 0; t < F::R0; ++t) {
       const int x = F::elem(lane, it, t);
-      const bool in = ok && F::valid(lane, it), lo = x < W, self = x == 0 || 2 * x == nx;
-      const size_t i = in ? hb + (lo ? x : nx - x) : 0;
-      cplx<R> a = A[i], b = B[i];
+      const bool lo = x < W, self = x == 0 || 2 * x == nx;
+      const int c = lo ? x : nx - x;
+      const bool k = c < Kx, in = ok && F::valid(lane, it) && k;
+      const size_t i = in ? hb + c : 0;
+      cplx<R> a = keep_if(A[i], k), b = keep_if(B[i], k);
       // mirrored points take the conjugate; the self-conjugate ones their real part
       a.im = lo ? (self ? (R)0 : a.im) : -a.im;
       b.im = lo ? (self ? (R)0 : b.im) : -b.im;
@@ -459,9 +468,9 @@ k_x_c2r(Geo g, int nfields, const cplx<R>* __restrict__ P0, const cplx<R>* __res
   cplx<R>* sl = lds + r * TL::LD;
   cplx<R> v[FI::IT][FI::R0];
   if (nfields == 2) {  // u and v: one packed transform
-    load_herm_pair<R, FI>(v, P0, P1, (size_t)row * g.Wp, NX, lane, ok);
+    load_herm_pair<R, FI>(v, P0, P1, (size_t)row * g.Wp, NX, g.Kx, lane, ok);
   } else {
-    load_herm_row<R, FI>(v, P0, (size_t)row * g.Wp, NX, lane, ok);
+    load_herm_row<R, FI>(v, P0, (size_t)row * g.Wp, NX, g.Kx, lane, ok);
   }
   FI::run(v, sl, lane, twr);
 #pragma unroll
@@ -501,8 +510,8 @@ k_xb_q(Geo g, const cplx<R>* __restrict__ Pq, const cplx<R>* __restrict__ Pu, co
   if constexpr (UVP) {
     // both inputs are loaded before the first transform
     cplx<R> w[FI::IT][FI::R0];
-    load_herm_pair<R, FI>(w, Pu, Pv, hb, NX, lane, ok);
-    load_herm_row<R, FI>(v, Pq, hb, NX, lane, ok);
+    load_herm_pair<R, FI>(w, Pu, Pv, hb, NX, g.Kx, lane, ok);
+    load_herm_row<R, FI>(v, Pq, hb, NX, g.Kx, lane, ok);
     FI::run(w, sl, lane, twr);
     FI::run(v, sl, lane, twr);
 #pragma unroll
@@ -518,7 +527,7 @@ k_xb_q(Geo g, const cplx<R>* __restrict__ Pq, const cplx<R>* __restrict__ Pu, co
         v[it][t] = mk<R>(w[it][t].re * qv[it][t], w[it][t].im * qv[it][t]);
       }
   } else {
-    load_herm_row<R, FI>(v, Pq, hb, NX, lane, ok);
+    load_herm_row<R, FI>(v, Pq, hb, NX, g.Kx, lane, ok);
     FI::run(v, sl, lane, twr);
 #pragma unroll
     for (int it = 0; it < FI::IT; ++it)
@@ -551,7 +560,7 @@ k_xb_q(Geo g, const cplx<R>* __restrict__ Pq, const cplx<R>* __restrict__ Pu, co
 #pragma unroll
     for (int t = 0; t < FI::R0; ++t) {
       int x = FI::elem(lane, it, t);
-      if (ok && FI::valid(lane, it) && x < g.W) {
+      if (ok && FI::valid(lane, it) && x < g.Kx) {
         const cplx<R> a = v[it][t], m = conj(sl[FI::pad(x == 0 ? 0 : NX - x)]);
         G1[hb + x] = mk<R>((R)0.5 * (a.re + m.re), (R)0.5 * (a.im + m.im));
         G2[hb + x] = mk<R>((R)0.5 * (a.im - m.im), (R)-0.5 * (a.re - m.re));
@@ -589,13 +598,13 @@ k_xb_B(Geo g, int flags, const cplx<R>* __restrict__ T1, const cplx<R>* __restri
 #pragma unroll
     for (int t = 0; t < FI::R0; ++t) {
       int x = FI::elem(lane, it, t);
-      const bool in = ok && FI::valid(lane, it);
-      b[it][t] = ldu(T1, base + x, in);
+      const bool in = ok && FI::valid(lane, it), kc = g.col_kept(x);
+      b[it][t] = keep_if(ldu(T1, base + x, in && kc), kc);  // columns not carried are zero
       ur[it][t] = ldu(u, base + x, in && adv);
       vr[it][t] = ldu(vv, base + x, in && adv);
       h[it][t] = czero<R>();
     }
-  if (flags & XB_REFR) load_herm_row<R, FI>(v, Pz, (size_t)row * g.Wp, NX, lane, ok);
+  if (flags & XB_REFR) load_herm_row<R, FI>(v, Pz, (size_t)row * g.Wp, NX, g.Kx, lane, ok);
   FI::run(b, sl, lane, twr);
   if (flags & XB_REFR) {
     FI::run(v, sl, lane, twr);
@@ -622,7 +631,7 @@ k_xb_B(Geo g, int flags, const cplx<R>* __restrict__ T1, const cplx<R>* __restri
       for (int t = 0; t < FI::R0; ++t) {
         int x = FI::elem(lane, it, t);
         if (f == 1) {
-          if (ok && FI::valid(lane, it)) G4[base + x] = v[it][t];
+          if (ok && FI::valid(lane, it) && g.col_kept(x)) G4[base + x] = v[it][t];
         } else {
           const R m = f == 0 ? (R)-g.kx[x] : (R)-0.5;
           h[it][t] = h[it][t] + ik_mul(m, v[it][t]);
@@ -634,7 +643,7 @@ k_xb_B(Geo g, int flags, const cplx<R>* __restrict__ T1, const cplx<R>* __restri
 #pragma unroll
     for (int t = 0; t < FI::R0; ++t) {
       int x = FI::elem(lane, it, t);
-      if (ok && FI::valid(lane, it)) G3[base + x] = h[it][t];
+      if (ok && FI::valid(lane, it) && g.col_kept(x)) G3[base + x] = h[it][t];
     }
 }
 
@@ -664,16 +673,17 @@ k_xb_all(Geo g, const cplx<R>* __restrict__ Pq, const cplx<R>* __restrict__ Pz, 
   // every input row is loaded before the first transform
   if constexpr (UVP) {
     cplx<R> w[FI::IT][FI::R0];
-    load_herm_pair<R, FI>(w, Pu, Pv, hb, NX, lane, ok);
+    load_herm_pair<R, FI>(w, Pu, Pv, hb, NX, g.Kx, lane, ok);
 #pragma unroll
     for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
       for (int t = 0; t < FI::R0; ++t) {
         int x = FI::elem(lane, it, t);
-        b[it][t] = ldu(T1, base + x, ok && FI::valid(lane, it));
+        const bool kc = g.col_kept(x);
+        b[it][t] = keep_if(ldu(T1, base + x, ok && FI::valid(lane, it) && kc), kc);
       }
-    load_herm_row<R, FI>(v, Pq, hb, NX, lane, ok);
-    load_herm_row<R, FI>(zz, Pz, hb, NX, lane, ok);
+    load_herm_row<R, FI>(v, Pq, hb, NX, g.Kx, lane, ok);
+    load_herm_row<R, FI>(zz, Pz, hb, NX, g.Kx, lane, ok);
     FI::run(w, sl, lane, twr);
 #pragma unroll
     for (int it = 0; it < FI::IT; ++it)
@@ -688,13 +698,13 @@ k_xb_all(Geo g, const cplx<R>* __restrict__ Pq, const cplx<R>* __restrict__ Pz, 
 #pragma unroll
       for (int t = 0; t < FI::R0; ++t) {
         int x = FI::elem(lane, it, t);
-        const bool in = ok && FI::valid(lane, it);
-        b[it][t] = ldu(T1, base + x, in);
+        const bool in = ok && FI::valid(lane, it), kc = g.col_kept(x);
+        b[it][t] = keep_if(ldu(T1, base + x, in && kc), kc);
         ur[it][t] = ldu(u, base + x, in);
         vr[it][t] = ldu(vv, base + x, in);
       }
-    load_herm_row<R, FI>(v, Pq, hb, NX, lane, ok);
-    load_herm_row<R, FI>(zz, Pz, hb, NX, lane, ok);
+    load_herm_row<R, FI>(v, Pq, hb, NX, g.Kx, lane, ok);
+    load_herm_row<R, FI>(zz, Pz, hb, NX, g.Kx, lane, ok);
   }
   // q: u*q + i v*q, one forward transform, unpacked through LDS (as k_xb_q)
   FI::run(v, sl, lane, twr);
@@ -718,7 +728,7 @@ k_xb_all(Geo g, const cplx<R>* __restrict__ Pq, const cplx<R>* __restrict__ Pz, 
 #pragma unroll
     for (int t = 0; t < FI::R0; ++t) {
       int x = FI::elem(lane, it, t);
-      if (ok && FI::valid(lane, it) && x < g.W) {
+      if (ok && FI::valid(lane, it) && x < g.Kx) {
         const cplx<R> a = v[it][t], m = conj(sl[FI::pad(x == 0 ? 0 : NX - x)]);
         G1[hb + x] = mk<R>((R)0.5 * (a.re + m.re), (R)0.5 * (a.im + m.im));
         G2[hb + x] = mk<R>((R)0.5 * (a.im - m.im), (R)-0.5 * (a.re - m.re));
@@ -745,7 +755,7 @@ k_xb_all(Geo g, const cplx<R>* __restrict__ Pq, const cplx<R>* __restrict__ Pz, 
       for (int t = 0; t < FI::R0; ++t) {
         int x = FI::elem(lane, it, t);
         if (f == 1) {
-          if (ok && FI::valid(lane, it)) G4[base + x] = v[it][t];
+          if (ok && FI::valid(lane, it) && g.col_kept(x)) G4[base + x] = v[it][t];
         } else if (f == 0) {
           h[it][t] = ik_mul((R)-g.kx[x], v[it][t]);
         } else {
@@ -758,7 +768,7 @@ k_xb_all(Geo g, const cplx<R>* __restrict__ Pq, const cplx<R>* __restrict__ Pz, 
 #pragma unroll
     for (int t = 0; t < FI::R0; ++t) {
       int x = FI::elem(lane, it, t);
-      if (ok && FI::valid(lane, it)) G3[base + x] = h[it][t];
+      if (ok && FI::valid(lane, it) && g.col_kept(x)) G3[base + x] = h[it][t];
     }
 }
 
@@ -829,7 +839,7 @@ k_yb2_q(Geo g, int mode, int flags_rt, const cplx<R>* __restrict__ G1, const cpl
   twr.load(lane, tw);
   const Blk blk = xcd_block();
   const int x = blk.x * TL::S + s, z = blk.y;
-  const bool ok = x < g.W;
+  const bool ok = x < g.Kx;
   cplx<R>* sl = lds + s * TL::LD;
   cplx<R> a[F::IT][F::R0], b[F::IT][F::R0];
   if (flags & YQ_ADV) {  // uniform
@@ -856,6 +866,7 @@ k_yb2_q(Geo g, int mode, int flags_rt, const cplx<R>* __restrict__ G1, const cpl
       int y = F::elem(lane, it, t);
       if (!(ok && F::valid(lane, it))) continue;
       const bool keep = g.mask[y * g.nx + x] != 0;
+      if (g.pruned() && !keep) continue;  // q, the bracket: zero there already
       cplx<double> nq = czero<double>();
       if ((flags & YQ_ADV) && keep)
         nq = ik_mul(g.kx[x], cvt<double>(a[it][t])) + ik_mul(g.ky[y], cvt<double>(b[it][t]));
@@ -924,8 +935,8 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
   TwTab<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const Blk blk = xcd_block();
-  const int x = blk.x * TL::S + s, z = blk.y;
-  const bool ok = x < g.nx;
+  const int x = fw_x(g, TL::S, blk.x, s), z = blk.y;
+  const bool ok = x < g.nx && g.col_kept(x);
   cplx<R>* sl = lds + s * TL::LD;
   cplx<R> v[F::IT][F::R0], w[F::IT][F::R0];
   const bool has3 = (flags & (YB_ADV | YB_REFR)) != 0, has4 = (flags & YB_ADV) != 0;  // uniform
@@ -961,7 +972,7 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
 #pragma unroll
       for (int t = 0; t < F::R0; ++t) {
         const int y = F::elem(lane, it, t);
-        const bool in = ok && F::valid(lane, it);
+        const bool in = ok && F::valid(lane, it) && g.in_disk(x, y);
         const size_t ad = ((size_t)z * NY + y) * g.nx + x;
         av[it][t] = ldu(A, ad, in && disp);
         bv[it][t] = ldu(base, ad, in);
@@ -977,6 +988,7 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
       v[it][t] = czero<R>();
       if (!(ok && F::valid(lane, it))) continue;
       const bool keep = g.mask[y * g.nx + x] != 0;
+      if (g.pruned() && !keep) continue;  // B, the bracket: zero there already
       const size_t col = (size_t)y * g.nx + x;
       const size_t ad = (size_t)z * plane + col;
       cplx<double> rhs = czero<double>();
@@ -1095,10 +1107,11 @@ k_yf_omega(Geo g, const cplx<R>* __restrict__ psi, cplx<R>* __restrict__ Pbx, cp
   TwTab<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
   const Blk blk = xcd_block();
-  const int ntile = (g.W + TL::S - 1) / TL::S;
+  // (pruned passes carry no Nyquist column and launch no Nyquist blocks)
+  const int ntile = (g.Kx + TL::S - 1) / TL::S;
   const bool nyq = blk.x == ntile;  // the Nyquist column's kx outputs (uniform)
   const int x = nyq ? g.nx / 2 : blk.x * TL::S + s, z = blk.y;
-  const bool ok = nyq ? s == 0 : x < g.W;
+  const bool ok = nyq ? s == 0 : x < g.Kx;
   const int mx = mirror(x, g.nx);
   const bool xnyq = mx == x && x != 0;  // the Nyquist column: its own mirror, kx != 0
   cplx<R>* sl = lds + s * TL::LD;
@@ -1120,10 +1133,12 @@ k_yf_omega(Geo g, const cplx<R>* __restrict__ psi, cplx<R>* __restrict__ Pbx, cp
     for (int t = 0; t < F::R0; ++t) {
       const int y = F::elem(lane, it, t), my = mirror(y, NY);
       const bool in = ok && F::valid(lane, it);
-      const cplx<R> p = ldu(psi, ((size_t)z * NY + y) * g.nx + x, in);
-      const cplx<R> pm = ldu(psi, ((size_t)z * NY + my) * g.nx + mx, in);
-      const cplx<R> up = ldu(psi, ((size_t)zu * NY + y) * g.nx + x, in);
-      const cplx<R> upm = ldu(psi, ((size_t)zu * NY + my) * g.nx + mx, in);
+      // outside the disk (pruned) psi is zero: not loaded
+      const bool d = g.in_disk(x, y), ld = in && d;
+      const cplx<R> p = keep_if(ldu(psi, ((size_t)z * NY + y) * g.nx + x, ld), d);
+      const cplx<R> pm = keep_if(ldu(psi, ((size_t)z * NY + my) * g.nx + mx, ld), d);
+      const cplx<R> up = keep_if(ldu(psi, ((size_t)zu * NY + y) * g.nx + x, ld), d);
+      const cplx<R> upm = keep_if(ldu(psi, ((size_t)zu * NY + my) * g.nx + mx, ld), d);
       fin = fin && (!in || (finite_c(p) && finite_c(pm)));
       h0[it][t] = nyq ? aherm(p, pm, rinv) : herm(p, pm, rinv);
       h1[it][t] = nyq ? aherm(up, upm, rinv) : herm(up, upm, rinv);
@@ -1198,9 +1213,9 @@ k_yf_omega(Geo g, const cplx<R>* __restrict__ psi, cplx<R>* __restrict__ Pbx, cp
 // check compute the same values.  Every thread of the block calls it.
 template <typename R, int NX, int THREADS, class FI, class TW>
 __device__ inline void psi_row_max(cplx<R> (&v)[FI::IT][FI::R0], const cplx<R>* __restrict__ Ppsi, size_t hb,
-                                   bool ok, int lane, cplx<R>* sl, const TW& twr,
+                                   int Kx, bool ok, int lane, cplx<R>* sl, const TW& twr,
                                    unsigned long long* __restrict__ absmax) {
-  load_herm_row<R, FI>(v, Ppsi, hb, NX, lane, ok);
+  load_herm_row<R, FI>(v, Ppsi, hb, NX, Kx, lane, ok);
   FI::run(v, sl, lane, twr);
   double m = 0.0;
 #pragma unroll
@@ -1255,10 +1270,10 @@ k_xf(Geo g, int want_omega, const cplx<R>* __restrict__ Pu, const cplx<R>* __res
   const size_t hb = (size_t)row * g.Wp;
   cplx<R> v[FI::IT][FI::R0], gb[FI::IT][FI::R0], gx[FI::IT][FI::R0];
   // all three packed rows are loaded before the first transform
-  load_herm_pair<R, FI>(v, Pu, Pv, hb, NX, lane, ok);
+  load_herm_pair<R, FI>(v, Pu, Pv, hb, NX, g.Kx, lane, ok);
   if (want_omega) {
-    load_herm_pair<R, FI>(gb, Pbx, Pby, hb, NX, lane, ok);
-    load_herm_pair<R, FI>(gx, Pxx, Pxy, hb, NX, lane, ok);
+    load_herm_pair<R, FI>(gb, Pbx, Pby, hb, NX, g.Kx, lane, ok);
+    load_herm_pair<R, FI>(gx, Pxx, Pxy, hb, NX, g.Kx, lane, ok);
   }
   // u and v: one packed transform
   FI::run(v, sl, lane, twr);
@@ -1287,11 +1302,11 @@ k_xf(Geo g, int want_omega, const cplx<R>* __restrict__ Pu, const cplx<R>* __res
 #pragma unroll
       for (int t = 0; t < FI::R0; ++t) {
         int x = FI::elem(lane, it, t);
-        if (ok && FI::valid(lane, it) && x < g.W) Rw[hb + x] = v[it][t];
+        if (ok && FI::valid(lane, it) && x < g.Kx) Rw[hb + x] = v[it][t];
       }
   }
   if constexpr (CHK) {
-    if (Ppsi) psi_row_max<R, NX, TL::THREADS, FI>(v, Ppsi, hb, ok, lane, sl, twr, absmax);  // uniform
+    if (Ppsi) psi_row_max<R, NX, TL::THREADS, FI>(v, Ppsi, hb, g.Kx, ok, lane, sl, twr, absmax);  // uniform
   }
 }
 
@@ -1309,7 +1324,7 @@ k_y_r2c_mask(Geo g, const cplx<R>* __restrict__ Rw, cplx<R>* __restrict__ out,
   twr.load(lane, tw);
   const Blk blk = xcd_block();
   const int x = blk.x * TL::S + s, z = blk.y;
-  const bool ok = x < g.W;
+  const bool ok = x < g.Kx;
   cplx<R>* sl = lds + s * TL::LD;
   cplx<R> v[F::IT][F::R0];
 #pragma unroll
@@ -1328,6 +1343,7 @@ k_y_r2c_mask(Geo g, const cplx<R>* __restrict__ Rw, cplx<R>* __restrict__ out,
       int y = F::elem(lane, it, t);
       if (!(ok && F::valid(lane, it))) continue;
       const bool keep = g.mask[y * g.nx + x] != 0;
+      if (g.pruned() && !keep) continue;  // pruned readers do not look there
       cplx<double> c = keep ? cvt<double>(v[it][t]) : czero<double>();
       out[((size_t)z * NY + y) * g.nx + x] = cvt<R>(c);
       if (mx != x) out[((size_t)z * NY + mirror(y, NY)) * g.nx + mx] = cvt<R>(conj(c));
@@ -1347,7 +1363,7 @@ k_y_herm(Geo g, const cplx<R>* __restrict__ in, cplx<R>* __restrict__ P,
   twr.load(lane, tw);
   const Blk blk = xcd_block();
   const int x = blk.x * TL::S + s, z = blk.y;
-  const bool ok = x < g.W;
+  const bool ok = x < g.Kx;
   const int mx = mirror(x, g.nx);
   cplx<R>* sl = lds + s * TL::LD;
   cplx<R> v[F::IT][F::R0];
@@ -1357,7 +1373,7 @@ k_y_herm(Geo g, const cplx<R>* __restrict__ in, cplx<R>* __restrict__ P,
     for (int t = 0; t < F::R0; ++t) {
       int y = F::elem(lane, it, t);
       cplx<R> a = czero<R>(), am = czero<R>();
-      if (ok && F::valid(lane, it)) {
+      if (ok && F::valid(lane, it) && g.in_disk(x, y)) {
         a = in[((size_t)z * NY + y) * g.nx + x];
         am = in[((size_t)z * NY + mirror(y, NY)) * g.nx + mx];
       }
@@ -1514,7 +1530,7 @@ k_x_psi_max(Geo g, const cplx<R>* __restrict__ P, unsigned long long* __restrict
   const bool ok = row < g.nz * g.ny;
   cplx<R>* sl = lds + r * TL::LD;
   cplx<R> v[FI::IT][FI::R0];
-  psi_row_max<R, NX, TL::THREADS, FI>(v, P, (size_t)row * g.Wp, ok, lane, sl, twr, absmax);
+  psi_row_max<R, NX, TL::THREADS, FI>(v, P, (size_t)row * g.Wp, g.Kx, ok, lane, sl, twr, absmax);
 }
 
 // Progress maxima (_progress_maxima, src/simulation.jl:46-81), Y-pass of the

@@ -61,6 +61,12 @@ static inline dim3 xgrid(int rows, int S) { return dim3((rows + S - 1) / S); }
     hipLaunchKernelGGL(K, dim3(((ncols) + TL::S - 1) / TL::S, g.nz), dim3(TL::THREADS), 0, st, \
                        __VA_ARGS__);                                                         \
   } while (0)
+// full-width Y-pass over the columns the pass carries (fw_tiles)
+#define YLF(K, N, ...)                                                                       \
+  do {                                                                                       \
+    using TL = Tile<R, N, V>;                                                                   \
+    hipLaunchKernelGGL(K, dim3(fw_tiles(g, TL::S), g.nz), dim3(TL::THREADS), 0, st, __VA_ARGS__); \
+  } while (0)
 #define XL(K, N, ...)                                                                        \
   do {                                                                                       \
     using TL = Tile<R, N, V>;                                                                   \
@@ -69,7 +75,7 @@ static inline dim3 xgrid(int rows, int S) { return dim3((rows + S - 1) / S); }
 
 template <typename R, int V>
 void OpsV<R, V>::ya_qw(const Geo& g, hipStream_t st, const C* B, C* T1, C* T3, const C* twy) {
-#define CALL(N) YL((k_ya_qw<R, N, V>), N, g.nx, g, B, T1, T3, twy)
+#define CALL(N) YLF((k_ya_qw<R, N, V>), N, g, B, T1, T3, twy)
   QG_SWITCH(g.ny, CALL)
 #undef CALL
 }
@@ -84,7 +90,7 @@ void OpsV<R, V>::xa_qw(const Geo& g, hipStream_t st, const C* T1, const C* T3, C
 template <typename R, int V>
 void OpsV<R, V>::ya2_qeff(const Geo& g, hipStream_t st, const C* Qa, const C* Qb, const C* q, C* qeff, C* qw,
                       double inv_f, const C* twy) {
-#define CALL(N) YL((k_ya2_qeff<R, N, V>), N, g.W, g, Qa, Qb, q, qeff, qw, inv_f, twy)
+#define CALL(N) YL((k_ya2_qeff<R, N, V>), N, g.Kx, g, Qa, Qb, q, qeff, qw, inv_f, twy)
   QG_SWITCH(g.ny, CALL)
 #undef CALL
 }
@@ -93,19 +99,19 @@ template <typename R, int V>
 void OpsV<R, V>::yb_adv(const Geo& g, hipStream_t st, int outs, const C* psi, const C* q, C* Pu, C* Pv, C* Pq,
                     C* Pz, const C* twy) {
   if (outs == (OUT_U | OUT_V)) {
-#define CALL(N) YL((k_yb_adv<R, N, V, OUT_U | OUT_V>), N, g.W, g, outs, psi, q, Pu, Pv, Pq, Pz, twy)
+#define CALL(N) YL((k_yb_adv<R, N, V, OUT_U | OUT_V>), N, g.Kx, g, outs, psi, q, Pu, Pv, Pq, Pz, twy)
     QG_SWITCH(g.ny, CALL)
 #undef CALL
   } else if (outs == (OUT_Q | OUT_Z)) {
-#define CALL(N) YL((k_yb_adv<R, N, V, OUT_Q | OUT_Z>), N, g.W, g, outs, psi, q, Pu, Pv, Pq, Pz, twy)
+#define CALL(N) YL((k_yb_adv<R, N, V, OUT_Q | OUT_Z>), N, g.Kx, g, outs, psi, q, Pu, Pv, Pq, Pz, twy)
     QG_SWITCH(g.ny, CALL)
 #undef CALL
   } else if (outs == (OUT_U | OUT_V | OUT_Q | OUT_Z)) {
-#define CALL(N) YL((k_yb_adv<R, N, V, OUT_U | OUT_V | OUT_Q | OUT_Z>), N, g.W, g, outs, psi, q, Pu, Pv, Pq, Pz, twy)
+#define CALL(N) YL((k_yb_adv<R, N, V, OUT_U | OUT_V | OUT_Q | OUT_Z>), N, g.Kx, g, outs, psi, q, Pu, Pv, Pq, Pz, twy)
     QG_SWITCH(g.ny, CALL)
 #undef CALL
   } else {
-#define CALL(N) YL((k_yb_adv<R, N, V, 0>), N, g.W, g, outs, psi, q, Pu, Pv, Pq, Pz, twy)
+#define CALL(N) YL((k_yb_adv<R, N, V, 0>), N, g.Kx, g, outs, psi, q, Pu, Pv, Pq, Pz, twy)
     QG_SWITCH(g.ny, CALL)
 #undef CALL
   }
@@ -161,11 +167,11 @@ void OpsV<R, V>::yb2_q(const Geo& g, hipStream_t st, int mode, int flags, const 
                    C* rhs0, C* qout, EtdView etd, double nu, double inv_dz2, int* flag, const C* twy) {
   Etd e{etd.tab, etd.plane, etd.stride, etd.nx, etd.ny, etd.quarter};
   if (flags == YQ_ADV) {  // the evolving flow without vertical diffusion
-#define CALL(N) YL((k_yb2_q<R, N, V, YQ_ADV>), N, g.W, g, mode, flags, G1, G2, qd, qn, rhs0, qout, e, nu, inv_dz2, flag, twy)
+#define CALL(N) YL((k_yb2_q<R, N, V, YQ_ADV>), N, g.Kx, g, mode, flags, G1, G2, qd, qn, rhs0, qout, e, nu, inv_dz2, flag, twy)
     QG_SWITCH(g.ny, CALL)
 #undef CALL
   } else {
-#define CALL(N) YL((k_yb2_q<R, N, V>), N, g.W, g, mode, flags, G1, G2, qd, qn, rhs0, qout, e, nu, inv_dz2, flag, twy)
+#define CALL(N) YL((k_yb2_q<R, N, V>), N, g.Kx, g, mode, flags, G1, G2, qd, qn, rhs0, qout, e, nu, inv_dz2, flag, twy)
     QG_SWITCH(g.ny, CALL)
 #undef CALL
   }
@@ -176,7 +182,7 @@ void OpsV<R, V>::yb2_B(const Geo& g, hipStream_t st, int mode, int flags, const 
                    const C* A, const C* Bn, C* rhs0, C* Bout, C* T1, C* T3, EtdView etd, double alpha,
                    int* flag, const C* twy) {
   Etd e{etd.tab, etd.plane, etd.stride, etd.nx, etd.ny, etd.quarter};
-#define CALL(N) YL((k_yb2_B<R, N, V>), N, g.nx, g, mode, flags, G3, G4, A, Bn, rhs0, Bout, T1, T3, e, alpha, flag, twy)
+#define CALL(N) YLF((k_yb2_B<R, N, V>), N, g, mode, flags, G3, G4, A, Bn, rhs0, Bout, T1, T3, e, alpha, flag, twy)
   QG_SWITCH(g.ny, CALL)
 #undef CALL
 }
@@ -185,22 +191,23 @@ template <typename R, int V>
 void OpsV<R, V>::yf_omega(const Geo& g, hipStream_t st, const C* psi, C* Pbx, C* Pby, C* Pxx, C* Pxy, C* Pu,
                       C* Pv, C* Ppsi, int* flag, double* lsum, const C* twy) {
   // one tile column more than the half width: the Nyquist column's kx outputs
+  // (none when pruned: the column is not carried)
   if ((Ppsi || lsum) && !Pu) throw std::runtime_error("yf_omega: the check needs the velocity outputs");
   if (Ppsi && lsum) throw std::runtime_error("yf_omega: the exact check or its bound, not both");
   if (Pu && lsum) {
-#define CALL(N) YL((k_yf_omega<R, N, V, 1, 2>), N, (g.W + Tile<R, N, V>::S), g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, Ppsi, flag, lsum, twy)
+#define CALL(N) YL((k_yf_omega<R, N, V, 1, 2>), N, (g.Kx + (g.pruned() ? 0 : Tile<R, N, V>::S)), g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, Ppsi, flag, lsum, twy)
     QG_SWITCH(g.ny, CALL)
 #undef CALL
   } else if (Pu && Ppsi) {
-#define CALL(N) YL((k_yf_omega<R, N, V, 1, 1>), N, (g.W + Tile<R, N, V>::S), g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, Ppsi, flag, lsum, twy)
+#define CALL(N) YL((k_yf_omega<R, N, V, 1, 1>), N, (g.Kx + (g.pruned() ? 0 : Tile<R, N, V>::S)), g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, Ppsi, flag, lsum, twy)
     QG_SWITCH(g.ny, CALL)
 #undef CALL
   } else if (Pu) {
-#define CALL(N) YL((k_yf_omega<R, N, V, 1, 0>), N, (g.W + Tile<R, N, V>::S), g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, Ppsi, flag, lsum, twy)
+#define CALL(N) YL((k_yf_omega<R, N, V, 1, 0>), N, (g.Kx + (g.pruned() ? 0 : Tile<R, N, V>::S)), g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, Ppsi, flag, lsum, twy)
     QG_SWITCH(g.ny, CALL)
 #undef CALL
   } else {
-#define CALL(N) YL((k_yf_omega<R, N, V, 0, 0>), N, (g.W + Tile<R, N, V>::S), g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, Ppsi, flag, lsum, twy)
+#define CALL(N) YL((k_yf_omega<R, N, V, 0, 0>), N, (g.Kx + (g.pruned() ? 0 : Tile<R, N, V>::S)), g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, Ppsi, flag, lsum, twy)
     QG_SWITCH(g.ny, CALL)
 #undef CALL
   }
@@ -223,14 +230,14 @@ void OpsV<R, V>::xf(const Geo& g, hipStream_t st, int want_omega, const C* Pu, c
 
 template <typename R, int V>
 void OpsV<R, V>::y_r2c_mask(const Geo& g, hipStream_t st, const C* Rw, C* out, const C* twy) {
-#define CALL(N) YL((k_y_r2c_mask<R, N, V>), N, g.W, g, Rw, out, twy)
+#define CALL(N) YL((k_y_r2c_mask<R, N, V>), N, g.Kx, g, Rw, out, twy)
   QG_SWITCH(g.ny, CALL)
 #undef CALL
 }
 
 template <typename R, int V>
 void OpsV<R, V>::y_herm(const Geo& g, hipStream_t st, const C* in, C* P, const C* twy) {
-#define CALL(N) YL((k_y_herm<R, N, V>), N, g.W, g, in, P, twy)
+#define CALL(N) YL((k_y_herm<R, N, V>), N, g.Kx, g, in, P, twy)
   QG_SWITCH(g.ny, CALL)
 #undef CALL
 }

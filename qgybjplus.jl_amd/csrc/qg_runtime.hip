@@ -60,7 +60,17 @@ struct qg_handle {
   std::vector<double> kx, ky, N2, a, fc, wc;
   std::vector<int> fo, wo;
   hipStream_t st = nullptr;
-  qg::Geo g{};
+  // g: every pass unpruned; gp: the same grid pruned to the dealiased disk
+  // (qg::Geo).  A step runs pruned when the model can be (prunable: q and psi
+  // are stepped, so every spectral field it reads has been through the mask)
+  // and `clean` holds: q, B of both parities, psi, A and C are zero outside
+  // the disk.  Any set-up or operator call clears `clean`; the next step then
+  // runs unpruned and zeroes the outside of the buffers it leaves behind.
+  qg::Geo g{}, gp{};
+  bool prunable = false, prune_enabled = true, clean = false;
+  // points a pass touches (the algorithmic bytes of profiles): spectral
+  // points inside the disk per plane, for gp
+  size_t disk_plane = 0, disk_quarter = 0;
   size_t plane = 0, npts = 0, nhalf = 0;
   std::vector<void*> allocs;
   // device
@@ -288,6 +298,7 @@ void check_status(qg_handle& h) {
     HIPCHECK(hipMemsetAsync(h.d_err, 0, 3 * sizeof(int), h.st));
     HIPCHECK(hipStreamSynchronize(h.st));
     h.diag_valid = false;
+    h.clean = false;
     h.t1_valid = false;
     char buf[256];
     if (e[0] == 1) {
@@ -312,17 +323,18 @@ void check_status(qg_handle& h) {
   throw QgError(QG_ERR_SINGULAR, buf);
 }
 
-// invert_q_to_psi! (src/elliptic.jl:189-314) on device arrays.
-void solve_psi(qg_handle& h, const void* q, void* psi) {
+// invert_q_to_psi! (src/elliptic.jl:189-314) on device arrays (the columns of
+// geometry g: all, or the disk's when pruned).
+void solve_psi(qg_handle& h, const qg::Geo& g, const void* q, void* psi) {
   if (h.sing_psi[0]) raise_singular(h.sing_psi);
-  qg::launch_thomas(h.g, h.st, q, psi, nullptr, h.f32(), h.th_inv[0], h.th_cp[0], h.d_al, 1, h.d_warn,
+  qg::launch_thomas(g, h.st, q, psi, nullptr, h.f32(), h.th_inv[0], h.th_cp[0], h.d_al, 1, h.d_warn,
                     h.tune[qg::P_THOMAS_PSI]);
 }
 
 // invert_B_to_A! (src/elliptic.jl:836-990) on device arrays; C optional.
-void solve_A(qg_handle& h, const void* B, void* A, void* C) {
+void solve_A(qg_handle& h, const qg::Geo& g, const void* B, void* A, void* C) {
   if (h.sing_A[0]) raise_singular(h.sing_A);
-  qg::launch_thomas(h.g, h.st, B, A, C, h.f32(), h.th_inv[1], h.th_cp[1], h.d_al, 4, nullptr,
+  qg::launch_thomas(g, h.st, B, A, C, h.f32(), h.th_inv[1], h.th_cp[1], h.d_al, 4, nullptr,
                     h.tune[C ? qg::P_THOMAS_AC : qg::P_THOMAS_A]);
 }
 
@@ -332,15 +344,32 @@ struct Engine {
   using O = qg::Ops<R>;
   qg_handle& h;
   // algorithmic bytes per point: complex R, real R, complex field value in
-  // storage (the fields are stored at R); P = all points, Ph = points of a
-  // half-width (real-field) intermediate
+  // storage (the fields are stored at R)
   const double cR = 2.0 * sizeof(R), rR = sizeof(R), cS = 2.0 * sizeof(R);
-  // Tq: the two per-level tables of a column solve, stored for the quarter
-  // columns |ix| <= nx/2, |iy| <= ny/2
-  const double P, Ph, Tq;
-  explicit Engine(qg_handle& hh)
-      : h(hh), P((double)hh.plane * hh.g.nz), Ph((double)hh.g.W * hh.ny * hh.g.nz),
-        Tq(2.0 * 8.0 * (hh.nx / 2 + 1) * (hh.ny / 2 + 1) * hh.g.nz) {}
+  // The geometry the passes run on (h.g, or h.gp for a pruned step) and the
+  // points one field of each kind has there: P physical (every point), Ps
+  // spectral (the disk's points when pruned), Ph half-width and Pf full-width
+  // x-transformed intermediates (the columns carried), and Tq the two
+  // per-level tables of a column solve (8 bytes per quarter column solved).
+  const qg::Geo* G = nullptr;
+  double P = 0, Ps = 0, Ph = 0, Pf = 0, Tq = 0;
+  explicit Engine(qg_handle& hh) : h(hh) { use(hh.g); }
+  void use(const qg::Geo& g) {
+    G = &g;
+    const double lv = g.nz;
+    P = (double)h.plane * lv;
+    if (g.pruned()) {
+      Ps = (double)h.disk_plane * lv;
+      Ph = (double)g.Kx * h.ny * lv;
+      Pf = (double)(2 * g.Kx - 1) * h.ny * lv;
+      Tq = 2.0 * 8.0 * (double)h.disk_quarter * lv;
+    } else {
+      Ps = P;
+      Ph = (double)g.W * h.ny * lv;
+      Pf = P;
+      Tq = 2.0 * 8.0 * (h.nx / 2 + 1) * (h.ny / 2 + 1) * lv;
+    }
+  }
   C* c(void* p) const { return static_cast<C*>(p); }
   R* r(void* p) const { return static_cast<R*>(p); }
   const C* twx() const { return static_cast<const C*>(h.twx); }
@@ -394,7 +423,7 @@ struct Engine {
   }
 
   void t1_only(const C* Bsrc) {
-    K("ya_T1", qg::P_YA_QW, P * (cS + cR), [&] { O::ya_qw(h.tune[qg::P_YA_QW], h.g, h.st, Bsrc, c(h.T1), nullptr, twy()); });
+    K("ya_T1", qg::P_YA_QW, Ps * cS + Pf * cR, [&] { O::ya_qw(h.tune[qg::P_YA_QW], *G, h.st, Bsrc, c(h.T1), nullptr, twy()); });
   }
 
   // _diagnose_flow! (src/timestep.jl:192-215) minus the velocities: q_eff with
@@ -407,36 +436,36 @@ struct Engine {
       const C* src = qsrc;
       if (h.feedback) {
         if (!t_ready)
-          K("ya_qw", qg::P_YA_QW, P * (cS + 2 * cR), [&] { O::ya_qw(h.tune[qg::P_YA_QW], h.g, h.st, Bsrc, c(h.T1), c(h.T3), twy()); });
-        K("xa_qw", qg::P_XA_QW, P * 2 * cR + Ph * 2 * cR,
-          [&] { O::xa_qw(h.tune[qg::P_XA_QW], h.g, h.st, c(h.T1), c(h.T3), c(h.Qa), c(h.Qb), twx()); });
-        K("ya2_qeff", qg::P_YA2_QEFF, Ph * 2 * cR + P * 2 * cS,
-          [&] { O::ya2_qeff(h.tune[qg::P_YA2_QEFF], h.g, h.st, c(h.Qa), c(h.Qb), qsrc, c(h.qeff), nullptr, 1.0 / h.f, twy()); });
+          K("ya_qw", qg::P_YA_QW, Ps * cS + Pf * 2 * cR, [&] { O::ya_qw(h.tune[qg::P_YA_QW], *G, h.st, Bsrc, c(h.T1), c(h.T3), twy()); });
+        K("xa_qw", qg::P_XA_QW, Pf * 2 * cR + Ph * 2 * cR,
+          [&] { O::xa_qw(h.tune[qg::P_XA_QW], *G, h.st, c(h.T1), c(h.T3), c(h.Qa), c(h.Qb), twx()); });
+        K("ya2_qeff", qg::P_YA2_QEFF, Ph * 2 * cR + Ps * 2 * cS,
+          [&] { O::ya2_qeff(h.tune[qg::P_YA2_QEFF], *G, h.st, c(h.Qa), c(h.Qb), qsrc, c(h.qeff), nullptr, 1.0 / h.f, twy()); });
         src = c(h.qeff);
         t1 = true;
       }
-      K("thomas_psi", qg::P_THOMAS_PSI, P * 2 * cS + Tq,
-        [&] { solve_psi(h, src, psi_out); });
+      K("thomas_psi", qg::P_THOMAS_PSI, Ps * 2 * cS + Tq,
+        [&] { solve_psi(h, *G, src, psi_out); });
     }
     if (need_t1 && !t1) t1_only(Bsrc);
   }
 
   void wave_inv(const C* Bsrc, C* Aout, C* Cout) {
     if (h.passive || h.no_disp) {
-      qg::launch_zero_bytes(Aout, h.plane * h.g.nz * h.fsize(), h.st);
-      if (Cout) qg::launch_zero_bytes(Cout, h.plane * h.g.nz * h.fsize(), h.st);
+      qg::launch_zero_bytes(Aout, h.plane * G->nz * h.fsize(), h.st);
+      if (Cout) qg::launch_zero_bytes(Cout, h.plane * G->nz * h.fsize(), h.st);
       return;
     }
-    K(Cout ? "thomas_AC" : "thomas_A", Cout ? qg::P_THOMAS_AC : qg::P_THOMAS_A, P * (Cout ? 3 : 2) * cS + Tq,
-      [&] { solve_A(h, Bsrc, Aout, Cout); });
+    K(Cout ? "thomas_AC" : "thomas_A", Cout ? qg::P_THOMAS_AC : qg::P_THOMAS_A, Ps * (Cout ? 3 : 2) * cS + Tq,
+      [&] { solve_A(h, *G, Bsrc, Aout, Cout); });
   }
 
   void velocities(const C* psi) {
-    K("yb_uv", qg::P_YB_ADV, P * cS + Ph * 2 * cR, [&] {
-      O::yb_adv(h.tune[qg::P_YB_ADV], h.g, h.st, qg::OUT_U | qg::OUT_V, psi, nullptr, c(h.Pu), c(h.Pv), nullptr, nullptr, twy());
+    K("yb_uv", qg::P_YB_ADV, Ps * cS + Ph * 2 * cR, [&] {
+      O::yb_adv(h.tune[qg::P_YB_ADV], *G, h.st, qg::OUT_U | qg::OUT_V, psi, nullptr, c(h.Pu), c(h.Pv), nullptr, nullptr, twy());
     });
     K("x_c2r_uv", qg::P_X_C2R, Ph * 2 * cR + P * 2 * rR,
-      [&] { O::x_c2r(h.tune[qg::P_X_C2R], h.g, h.st, 2, c(h.Pu), c(h.Pv), r(h.u), r(h.v), twx()); });
+      [&] { O::x_c2r(h.tune[qg::P_X_C2R], *G, h.st, 2, c(h.Pu), c(h.Pv), r(h.u), r(h.v), twx()); });
   }
 
   // u, v and the QG omega-equation w (compute_velocities!(compute_w=true)).
@@ -449,27 +478,27 @@ struct Engine {
   void velocities_w(const C* psi, int chk = 0) {
     C* Ppsi = chk == 1 ? c(h.Pz) : nullptr;
     double* lsum = chk == 2 ? h.d_lsum : nullptr;
-    K("yf_omega", qg::P_YF_OMEGA, P * cS + Ph * (chk == 1 ? 7 : 6) * cR, [&] {
-      O::yf_omega(h.tune[qg::P_YF_OMEGA], h.g, h.st, psi, c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), c(h.Pu), c(h.Pv),
+    K("yf_omega", qg::P_YF_OMEGA, Ps * cS + Ph * (chk == 1 ? 7 : 6) * cR, [&] {
+      O::yf_omega(h.tune[qg::P_YF_OMEGA], *G, h.st, psi, c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), c(h.Pu), c(h.Pv),
                   Ppsi, h.d_flag, lsum, twy());
     });
     K("xf", qg::P_XF, Ph * (chk == 1 ? 8 : 7) * cR + P * 2 * rR, [&] {
-      O::xf(h.tune[qg::P_XF], h.g, h.st, 1, c(h.Pu), c(h.Pv), c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), r(h.u), r(h.v), c(h.Pq),
+      O::xf(h.tune[qg::P_XF], *G, h.st, 1, c(h.Pu), c(h.Pv), c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), r(h.u), r(h.v), c(h.Pq),
             Ppsi, h.d_absmax, lsum, twx());
     });
-    K("y_r2c_omega", qg::P_Y_R2C, Ph * cR + P * cS, [&] { O::y_r2c_mask(h.tune[qg::P_Y_R2C], h.g, h.st, c(h.Pq), c(h.rq0), twy()); });
+    K("y_r2c_omega", qg::P_Y_R2C, Ph * cR + Ps * cS, [&] { O::y_r2c_mask(h.tune[qg::P_Y_R2C], *G, h.st, c(h.Pq), c(h.rq0), twy()); });
     if (h.sing_omega[0]) {
       char buf[160];
       snprintf(buf, sizeof buf, "LAPACK-style gtsv failed for vertical velocity at row %d of column %d",
                h.sing_omega[1] + 1, h.sing_omega[2]);
       throw QgError(QG_ERR_GTSV, buf);
     }
-    K("omega_gtsv", qg::P_OMEGA, P * 2 * cS + Tq, [&] {
-      qg::launch_omega(h.g, h.st, c(h.rq0), c(h.qeff), h.f32(), h.om_M, h.om_D, h.om_piv, h.om_any_piv, h.d_cz, h.d_cu,
+    K("omega_gtsv", qg::P_OMEGA, Ps * 2 * cS + Tq, [&] {
+      qg::launch_omega(*G, h.st, c(h.rq0), c(h.qeff), h.f32(), h.om_M, h.om_D, h.om_piv, h.om_any_piv, h.d_cz, h.d_cu,
                        h.d_fN2, h.s0, h.s1, h.s2, h.tune[qg::P_OMEGA]);
     });
-    K("y_herm_w", qg::P_Y_HERM, P * cS + Ph * cR, [&] { O::y_herm(h.tune[qg::P_Y_HERM], h.g, h.st, c(h.qeff), c(h.Pz), twy()); });
-    K("x_c2r_w", qg::P_X_C2R, Ph * cR + P * rR, [&] { O::x_c2r(h.tune[qg::P_X_C2R], h.g, h.st, 1, c(h.Pz), nullptr, r(h.w), nullptr, twx()); });
+    K("y_herm_w", qg::P_Y_HERM, Ps * cS + Ph * cR, [&] { O::y_herm(h.tune[qg::P_Y_HERM], *G, h.st, c(h.qeff), c(h.Pz), twy()); });
+    K("x_c2r_w", qg::P_X_C2R, Ph * cR + P * rR, [&] { O::x_c2r(h.tune[qg::P_X_C2R], *G, h.st, 1, c(h.Pz), nullptr, r(h.w), nullptr, twx()); });
   }
 
   // One RHS evaluation at (qS, psiS, AS, T1, u, v) fused with the ETD update
@@ -496,45 +525,45 @@ struct Engine {
     if (outs)
       K(uv ? "yb_adv_uv" : "yb_adv", qg::P_YB_ADV,
         // reads q for OUT_Q and psi for the others
-        P * (((outs & qg::OUT_Q) ? cS : 0) + ((outs & ~qg::OUT_Q) ? cS : 0)) + Ph * nouts * cR, [&] {
-          O::yb_adv(h.tune[qg::P_YB_ADV], h.g, h.st, outs, psiS, qS, uv ? c(h.Pu) : nullptr, uv ? c(h.Pv) : nullptr,
+        Ps * (((outs & qg::OUT_Q) ? cS : 0) + ((outs & ~qg::OUT_Q) ? cS : 0)) + Ph * nouts * cR, [&] {
+          O::yb_adv(h.tune[qg::P_YB_ADV], *G, h.st, outs, psiS, qS, uv ? c(h.Pu) : nullptr, uv ? c(h.Pv) : nullptr,
                     c(h.Pq), c(h.Pz), twy());
         });
     if (adv && qdyn && refr) {
       // both X-passes in one launch: Pq, Pz, T1 and (u, v | Pu, Pv) in;
       // G1, G2 (half width), G3, G4 out
       K(uv ? "xb_all_uv" : "xb_all", qg::P_XB_B,
-        Ph * 4 * cR + P * cR + (uv ? Ph * 2 * cR : P * 2 * rR) + P * 2 * cR, [&] {
-          O::xb_all(h.tune[qg::P_XB_B], h.g, h.st, c(h.Pq), c(h.Pz), c(h.T1), uv ? c(h.Pu) : nullptr,
+        Ph * 4 * cR + Pf * cR + (uv ? Ph * 2 * cR : P * 2 * rR) + Pf * 2 * cR, [&] {
+          O::xb_all(h.tune[qg::P_XB_B], *G, h.st, c(h.Pq), c(h.Pz), c(h.T1), uv ? c(h.Pu) : nullptr,
                     uv ? c(h.Pv) : nullptr, r(h.u), r(h.v), c(h.G1), c(h.G2), c(h.G3), c(h.G4), twx());
         });
     } else if (adv && qdyn) {
       K(uv ? "xb_q_uv" : "xb_q", qg::P_XB_Q, Ph * (uv ? 3 : 1) * cR + Ph * 2 * cR + P * 2 * rR, [&] {
-        O::xb_q(h.tune[qg::P_XB_Q], h.g, h.st, c(h.Pq), uv ? c(h.Pu) : nullptr, uv ? c(h.Pv) : nullptr, r(h.u),
+        O::xb_q(h.tune[qg::P_XB_Q], *G, h.st, c(h.Pq), uv ? c(h.Pu) : nullptr, uv ? c(h.Pv) : nullptr, r(h.u),
                 r(h.v), c(h.G1), c(h.G2), twx());
       });
     }
     int xbf = (adv ? qg::XB_ADV : 0) | (refr ? qg::XB_REFR : 0);
     if (xbf && !(adv && qdyn && refr))
-      K("xb_B", qg::P_XB_B, P * cR + (refr ? Ph * cR : 0) + (adv ? P * 2 * rR : 0) + P * cR * (1 + (adv ? 1 : 0)),
-        [&] { O::xb_B(h.tune[qg::P_XB_B], h.g, h.st, xbf, c(h.T1), c(h.Pz), r(h.u), r(h.v), c(h.G3), c(h.G4), twx()); });
+      K("xb_B", qg::P_XB_B, Pf * cR + (refr ? Ph * cR : 0) + (adv ? P * 2 * rR : 0) + Pf * cR * (1 + (adv ? 1 : 0)),
+        [&] { O::xb_B(h.tune[qg::P_XB_B], *G, h.st, xbf, c(h.T1), c(h.Pz), r(h.u), r(h.v), c(h.G3), c(h.G4), twx()); });
     const bool diff = h.nu_z != 0.0 && !h.inviscid && h.nz > 1;
     int qf = ((adv && qdyn) ? qg::YQ_ADV : 0) | (h.fixed_flow ? qg::YQ_FIXED : 0) | (diff ? qg::YQ_DIFF : 0);
     const double inv_dz2 = h.nz > 1 ? 1.0 / (h.dz * h.dz) : 0.0;
     K(mode == qg::MODE_STAGE ? "yb2_q_stage" : "yb2_q_final", qg::P_YB2_Q,
       // stage: q_n, the kept bracket and q_S; final: the bracket and q_{n+1}
       ((qf & qg::YQ_ADV) ? Ph * 2 * cR : 0) +
-          P * cS * ((mode == qg::MODE_STAGE || h.fixed_flow ? 3 : 2) + (diff ? 1 : 0)),
-      [&] { O::yb2_q(h.tune[qg::P_YB2_Q], h.g, h.st, mode, qf, c(h.G1), c(h.G2), qS, qn, c(h.rq0), qout, etd(), h.nu_z, inv_dz2, flag, twy()); });
+          Ps * cS * ((mode == qg::MODE_STAGE || h.fixed_flow ? 3 : 2) + (diff ? 1 : 0)),
+      [&] { O::yb2_q(h.tune[qg::P_YB2_Q], *G, h.st, mode, qf, c(h.G1), c(h.G2), qS, qn, c(h.rq0), qout, etd(), h.nu_z, inv_dz2, flag, twy()); });
     const bool disp = !(h.passive || h.no_disp);
     int bf = (adv ? qg::YB_ADV : 0) | (refr ? qg::YB_REFR : 0) | (disp ? qg::YB_DISP : 0);
     // the new B's inverse y transforms for the next diagnosis: T1 always
     // (xb_B), T3 when q_w is computed
     C* T3 = (h.feedback && !h.fixed_flow) ? c(h.T3) : nullptr;
     K(mode == qg::MODE_STAGE ? "yb2_B_stage" : "yb2_B_final", qg::P_YB2_B,
-      P * cR * (((adv || refr) ? 1 : 0) + (adv ? 1 : 0) + 1 + (T3 ? 1 : 0)) +
-          P * cS * ((mode == qg::MODE_STAGE ? 3 : 2) + (disp ? 1 : 0)),
-      [&] { O::yb2_B(h.tune[qg::P_YB2_B], h.g, h.st, mode, bf, c(h.G3), c(h.G4), AS, Bn, c(h.rB0), Bout, c(h.T1), T3, etd(), h.f / 2, flag, twy()); });
+      Pf * cR * (((adv || refr) ? 1 : 0) + (adv ? 1 : 0) + 1 + (T3 ? 1 : 0)) +
+          Ps * cS * ((mode == qg::MODE_STAGE ? 3 : 2) + (disp ? 1 : 0)),
+      [&] { O::yb2_B(h.tune[qg::P_YB2_B], *G, h.st, mode, bf, c(h.G3), c(h.G4), AS, Bn, c(h.rB0), Bout, c(h.T1), T3, etd(), h.f / 2, flag, twy()); });
   }
 
   // step!(model, ExponentialRungeKutta2) -> _advance_etdrk2! (src/timestep.jl:335-415).
@@ -546,6 +575,8 @@ struct Engine {
   // chk = 2: the same with an upper bound of max |ifft2(psi)| in d_absmax
   // (velocities_w).
   void step(int chk = 0) {
+    const bool can = h.prunable && h.prune_enabled, prune = can && h.clean;
+    use(prune ? h.gp : h.g);
     C* qn = c(h.q[h.cur]);
     C* Bn = c(h.B[h.cur]);
     C* q1 = c(h.q[1 - h.cur]);
@@ -572,9 +603,18 @@ struct Engine {
     diag(q1, B1, c(h.psi), false, true);
     wave_inv(B1, c(h.A), c(h.C));
     velocities_w(c(h.psi), chk);
+    if (can && !prune) {
+      // The step's outputs (q, B, psi, A, C) are masked and written at every
+      // point; the buffers it read from are the next step's outputs, which a
+      // pruned step writes inside the disk only: zero their outside.
+      qg::launch_zero_outside_disk(h.gp, h.st, qn, h.f32());
+      qg::launch_zero_outside_disk(h.gp, h.st, Bn, h.f32());
+      h.clean = true;
+    }
     h.cur = 1 - h.cur;
     h.diag_valid = true;
     h.t1_valid = true;
+    use(h.g);
   }
 
   void compute_qw(C* out) {
@@ -670,7 +710,8 @@ void advance(qg_handle& H, E& e, int nsteps, int chk, Mark&& mark) {
   auto one = [&] { e.step(chk); };  // the check is fused into the step's last passes
   int i = 0;
   while (i < nsteps) {
-    if (use_graphs && H.diag_valid && nsteps - i >= 2) {
+    // an unclean state's step runs unpruned and eagerly (it makes the state clean)
+    if (use_graphs && H.diag_valid && (H.clean || !(H.prunable && H.prune_enabled)) && nsteps - i >= 2) {
       const int c = H.cur;
       hipGraphExec_t& gx = chk == 2 ? H.graph_bnd[c] : chk == 1 ? H.graph_chk[c] : H.graph[c];
       if (!gx) {
@@ -925,6 +966,19 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
     g.mask = h->d_mask;
     g.inv_n = 1.0 / ((double)nx * (double)ny);
     g.dz = h->dz;
+    g.Kx = W;
+    g.kd2 = -1;
+    {
+      const int kmax = std::min(nx, ny) / 3;  // is_dealiased (src/physics.jl:36-43)
+      h->gp = g;
+      h->gp.Kx = kmax + 1;
+      h->gp.kd2 = kmax * kmax;
+      h->prunable = !h->fixed_flow && kmax + 1 < W;
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) h->disk_plane += mask[(size_t)y * nx + x];
+      for (int y = 0; y <= ny / 2; ++y)
+        for (int x = 0; x <= nx / 2; ++x) h->disk_quarter += (x * x + y * y <= kmax * kmax) ? 1 : 0;
+    }
     // Thomas factors, computed once: they depend only on (kh2, a, dz)
     if (!qg::thomas_setup() || !qg::omega_setup())
       throw QgError(QG_ERR_DEVICE, "cannot reserve the LDS of the column solvers");
@@ -1056,6 +1110,7 @@ void set_field_range(qg_handle& H, int field, const double* host, int space, int
   }
   HIPCHECK(hipStreamSynchronize(H.st));
   H.diag_valid = false;
+  H.clean = false;
   H.t1_valid = false;
 }
 
@@ -1163,7 +1218,7 @@ void run_checked(qg_handle& H, double dt, int nsteps, void* snap, bool graphs, i
     while (d < nsteps && fail == 0) {
       const int k = std::min(kRunChunk, nsteps - d);
       const int c0 = H.cur;
-      const bool dv = H.diag_valid, tv = H.t1_valid;
+      const bool dv = H.diag_valid, tv = H.t1_valid, cl = H.clean;
       copy_parts(c0, true);
       clear();
       advance(H, e, k, 2, none);
@@ -1184,6 +1239,7 @@ void run_checked(qg_handle& H, double dt, int nsteps, void* snap, bool graphs, i
       H.cur = c0;
       H.diag_valid = dv;
       H.t1_valid = tv;
+      H.clean = cl;
       for (int j = 0; j < k && fail == 0; ++j) {
         clear();
         advance(H, e, 1, 1, none);
@@ -1360,6 +1416,7 @@ int qg_set_random_psi(qg_handle* h, int member, double amplitude, double slope, 
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(H.st));
     H.diag_valid = false;
+    H.clean = false;
     H.t1_valid = false;
   });
 }
@@ -1381,6 +1438,7 @@ int qg_set_surface_wave(qg_handle* h, int member, const double* profile) {
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(H.st));
     H.diag_valid = false;
+    H.clean = false;
     H.t1_valid = false;
   });
 }
@@ -1499,6 +1557,7 @@ int qg_set_member_stratification(qg_handle* h, int member, const double* N2, con
       H.om_any_piv = true;
     }
     H.diag_valid = false;
+    H.clean = false;
     H.t1_valid = false;
   });
 }
@@ -1513,11 +1572,12 @@ int qg_set_active_members(qg_handle* h, int active) {
     H.drop_graphs();  // captured for the previous extent
     if (active > H.active) {  // newly stepped members have no diagnostics yet
       H.diag_valid = false;
+      H.clean = false;
       H.t1_valid = false;
     }
     H.active = active;
-    H.g.members = active;
-    H.g.nz = active * H.nzc;
+    H.g.members = H.gp.members = active;
+    H.g.nz = H.gp.nz = active * H.nzc;
   });
 }
 
@@ -1550,6 +1610,7 @@ int qg_move_member(qg_handle* h, int src, int dst) {
     std::copy(H.a.begin() + (size_t)src * H.nzc, H.a.begin() + (size_t)(src + 1) * H.nzc,
               H.a.begin() + (size_t)dst * H.nzc);
     H.diag_valid = false;
+    H.clean = false;
     H.t1_valid = false;
   });
 }
@@ -1561,6 +1622,16 @@ int qg_use_graphs(qg_handle* h, int enable) {
       HIPCHECK(hipStreamSynchronize(h->st));
       h->drop_graphs();
     }
+  });
+}
+
+int qg_set_pruning(qg_handle* h, int enable) {
+  return guarded(h, [&] {
+    if (h->prune_enabled == (enable != 0)) return;
+    HIPCHECK(hipStreamSynchronize(h->st));
+    h->drop_graphs();  // captured with the other geometry
+    h->prune_enabled = enable != 0;
+    h->clean = false;  // the outside of the disk is zeroed again by the next step
   });
 }
 
@@ -1597,16 +1668,18 @@ int qg_progress_maxima(qg_handle* h, double* wave_speed, double* flow_speed) {
 int qg_invert_q_to_psi(qg_handle* h) {
   return guarded(h, [&] {
     h->psi64_nm = 0;
-    solve_psi(*h, h->q[h->cur], h->psi);
+    solve_psi(*h, h->g, h->q[h->cur], h->psi);
     h->diag_valid = false;
+    h->clean = false;
     check_status(*h);
   });
 }
 
 int qg_invert_B_to_A(qg_handle* h) {
   return guarded(h, [&] {
-    solve_A(*h, h->B[h->cur], h->A, h->C);
+    solve_A(*h, h->g, h->B[h->cur], h->A, h->C);
     h->diag_valid = false;
+    h->clean = false;
     check_status(*h);
   });
 }
@@ -1621,6 +1694,7 @@ int qg_invert_B_to_A_member(qg_handle* h, int member) {
                       H.th_inv[1] + mv.toff,
                       H.th_cp[1] + mv.toff, H.d_al + mv.aoff, 4, nullptr, H.tune[qg::P_THOMAS_AC]);
     H.diag_valid = false;
+    H.clean = false;
     check_status(H);
   });
 }
@@ -1635,6 +1709,7 @@ int qg_compute_q_from_psi_member(qg_handle* h, int member, int method) {
     qg::launch_q_from_psi(mv.g, H.st, staged ? static_cast<void*>(H.stg2 + mv.off) : H.at(H.psi, mv.off),
                           H.at(H.q[H.cur], mv.off), H.f32(), H.d_a + (size_t)member * H.nzc, method, staged);
     H.diag_valid = false;
+    H.clean = false;
     HIPCHECK(hipStreamSynchronize(H.st));
   });
 }
@@ -1643,6 +1718,7 @@ int qg_compute_velocities(qg_handle* h, int compute_w) {
   return guarded(h, [&] {
     with_engine(*h, [&](auto& e) { e.compute_velocities(compute_w != 0); });
     h->diag_valid = false;
+    h->clean = false;
     check_status(*h);
   });
 }
@@ -1654,6 +1730,7 @@ int qg_compute_q_from_psi(qg_handle* h, int method) {
     qg::launch_q_from_psi(h->g, h->st, staged ? static_cast<void*>(h->stg2) : h->psi, h->q[h->cur], h->f32(),
                           h->d_a, method, staged);
     h->diag_valid = false;
+    h->clean = false;
     HIPCHECK(hipStreamSynchronize(h->st));
   });
 }
@@ -1765,6 +1842,7 @@ void autotune(qg_handle& H, double dt, int steps) {
       H.prof.clear();
       H.pool_used = 0;
       H.diag_valid = false;
+      H.clean = false;
       H.t1_valid = false;
       H.drop_graphs();
     }
@@ -1789,6 +1867,7 @@ void autotune(qg_handle& H, double dt, int steps) {
     HIPCHECK(hipMemcpyAsync(H.q[H.cur], save + kept[0].off, fb, hipMemcpyDeviceToDevice, H.st));
     HIPCHECK(hipMemcpyAsync(H.B[H.cur], save + kept[1].off, fb, hipMemcpyDeviceToDevice, H.st));
     H.diag_valid = false;
+    H.clean = false;
     H.t1_valid = false;
     std::vector<std::vector<float>> ms(qg::P_END);
     with_engine(H, [&](auto& e) {
