@@ -66,6 +66,7 @@ struct Geo {
   int Wp;   // row stride of half-spectrum arrays (W rounded up to 16)
   int Kx;   // columns [0, Kx) of half-spectrum arrays carried (kmax + 1 when pruned, else W)
   int kd2;  // pruned: kmax^2; unpruned: -1
+  int km2;  // kmax^2 of the 2/3 mask, kmax = min(nx, ny) / 3 (src/physics.jl:36-43)
   const double* kx;            // [nx]
   const double* ky;            // [ny]
   const unsigned char* mask;   // [ny][nx] radial 2/3 mask, 1 = keep
@@ -78,13 +79,15 @@ struct Geo {
     const int ix = x <= nx - x ? x : nx - x;
     return ix < Kx;
   }
+  // the 2/3 mask at (x, y): ix^2 + iy^2 <= kmax^2 with the signed Fourier
+  // indices ix, iy (what the mask array holds, computed instead of loaded)
+  __host__ __device__ bool keep(int x, int y) const {
+    const int ix = x <= nx - x ? x : nx - x, iy = y <= ny - y ? y : ny - y;
+    return ix * ix + iy * iy <= km2;
+  }
   // point (x, y) touched by a pruned pass: inside the disk; every point
   // otherwise
-  __host__ __device__ bool in_disk(int x, int y) const {
-    if (kd2 < 0) return true;
-    const int ix = x <= nx - x ? x : nx - x, iy = y <= ny - y ? y : ny - y;
-    return ix * ix + iy * iy <= kd2;
-  }
+  __host__ __device__ bool in_disk(int x, int y) const { return kd2 < 0 || keep(x, y); }
 };
 
 __host__ __device__ inline int mirror(int i, int n) { return i == 0 ? 0 : n - i; }

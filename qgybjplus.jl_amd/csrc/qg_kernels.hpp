@@ -37,8 +37,10 @@ struct Tile {
   static constexpr int S0 = TARGET / T;
   static constexpr int S1 = S0 < 1 ? 1 : (S0 > 64 ? 64 : S0);
   static constexpr int S2 = S1 * T > 512 ? (512 / T < 1 ? 1 : 512 / T) : S1;
-  // sequences that fit 64 KB of LDS beside the staged twiddle table
-  static constexpr int SL = (int)((65536 - TwTab<R, N, PL>::bytes()) / (LD * sizeof(cplx<R>)));
+  // sequences that fit 64 KB of LDS beside the staged twiddle table and a
+  // staged wavenumber table (stage_k)
+  static constexpr int SL = (int)((65536 - TwTab<R, N, PL>::bytes() - N * (int)sizeof(double)) /
+                                  (LD * sizeof(cplx<R>)));
   static constexpr int S = S2 > SL ? (SL < 1 ? 1 : SL) : S2;
   static constexpr int THREADS = S * T;
   static constexpr int NN = N;
@@ -82,6 +84,25 @@ template <typename T>
 __device__ inline const T* opaque(const T* p) {
   asm volatile("" : "+s"(p));
   return p;
+}
+// The same for a pointer into LDS, kept in the LDS address space so that
+// reads through it stay LDS reads.
+using lds_cdouble = __attribute__((address_space(3))) const double;
+__device__ inline lds_cdouble* opaque_lds(const double* p) {
+  lds_cdouble* q = (lds_cdouble*)p;
+  asm volatile("" : "+s"(q));
+  return q;
+}
+
+// A wavenumber table of length N (ky for the Y-passes) copied into LDS by the
+// block, so the epilogues' per-point reads wait for LDS rather than for a
+// global load each.  Every thread calls it, and it ends with a barrier.
+template <int N>
+__device__ inline const double* stage_k(const double* __restrict__ k) {
+  __shared__ double s[N];
+  for (int i = threadIdx.x; i < N; i += blockDim.x) s[i] = k[i];
+  __syncthreads();
+  return s;
 }
 
 template <typename R>
@@ -155,6 +176,7 @@ k_ya_qw(Geo g, const cplx<R>* __restrict__ B, cplx<R>* __restrict__ T1,
   const int x = fw_x(g, TL::S, blk.x, s), z = blk.y;
   const bool ok = x < g.nx && g.col_kept(x);
   cplx<R>* sl = lds + s * TL::LD;
+  const double* __restrict__ ky = stage_k<NY>(g.ky);
   cplx<R> v[F::IT][F::R0], w[F::IT][F::R0];
 #pragma unroll
   for (int it = 0; it < F::IT; ++it)
@@ -163,10 +185,10 @@ k_ya_qw(Geo g, const cplx<R>* __restrict__ B, cplx<R>* __restrict__ T1,
       int y = F::elem(lane, it, t);
       const bool in = ok && F::valid(lane, it);
       const size_t a = ((size_t)z * NY + y) * g.nx + x;
-      const bool keep = in && g.mask[in ? y * g.nx + x : 0];
+      const bool keep = in && g.keep(x, y);
       const cplx<double> b = scale(cvt<double>(keep_if(ldu(B, a, in && g.in_disk(x, y)), keep)), g.inv_n);
       v[it][t] = cvt<R>(b);
-      w[it][t] = cvt<R>(ik_mul(g.ky[y], b));
+      w[it][t] = cvt<R>(ik_mul(ky[y], b));
     }
   F::run(v, sl, lane, twr);
   if (T3) F::run(w, sl, lane, twr);  // uniform: T3 == nullptr computes T1 only
@@ -267,6 +289,7 @@ k_ya2_qeff(Geo g, const cplx<R>* __restrict__ Qa, const cplx<R>* __restrict__ Qb
   const int x = blk.x * TL::S + s, z = blk.y;
   const bool ok = x < g.Kx;
   cplx<R>* sl = lds + s * TL::LD;
+  const double* __restrict__ ky = stage_k<NY>(g.ky);
   cplx<R> v[F::IT][F::R0], w[F::IT][F::R0];
 #pragma unroll
   for (int it = 0; it < F::IT; ++it)
@@ -289,9 +312,9 @@ k_ya2_qeff(Geo g, const cplx<R>* __restrict__ Qa, const cplx<R>* __restrict__ Qb
     for (int t = 0; t < F::R0; ++t) {
       int y = F::elem(lane, it, t);
       if (!(ok && F::valid(lane, it))) continue;
-      const bool keep = g.mask[y * g.nx + x] != 0;
+      const bool keep = g.keep(x, y);
       if (g.pruned() && !keep) continue;  // zero there already
-      const double kh2 = g.kx[x] * g.kx[x] + g.ky[y] * g.ky[y];
+      const double kh2 = g.kx[x] * g.kx[x] + ky[y] * ky[y];
       cplx<double> fq = cvt<double>(v[it][t]), fm = cvt<double>(w[it][t]);
       cplx<double> qw = keep ? scale(fq - scale(fm, 0.25 * kh2), inv_f) : czero<double>();
       size_t a = ((size_t)z * NY + y) * g.nx + x;
@@ -335,6 +358,7 @@ k_yb_adv(Geo g, int outs_rt, const cplx<R>* __restrict__ psi, const cplx<R>* __r
   const int mx = mirror(x, g.nx);
   cplx<R>* sl = lds + s * TL::LD;
   const double kxv = ok ? g.kx[x] : 0.0, kxm = ok ? g.kx[mx] : 0.0;
+  const double* __restrict__ ky = stage_k<NY>(g.ky);
   // psi and its mirror are loaded once, in storage precision; every output's
   // transform input is formed from them at the transform precision R
   cplx<R> p[F::IT][F::R0], pm[F::IT][F::R0];
@@ -367,7 +391,7 @@ k_yb_adv(Geo g, int outs_rt, const cplx<R>* __restrict__ psi, const cplx<R>* __r
         if (o == 0) {
           const cplx<R> f = y == NY / 2 ? aherm(p[it][t], pm[it][t], (R)g.inv_n)
                                         : herm(p[it][t], pm[it][t], (R)g.inv_n);
-          v[it][t] = ik_mul((R)-g.ky[y], f);
+          v[it][t] = ik_mul((R)-ky[y], f);
           continue;
         } else if (o == 1) {
           const cplx<R> f = (mx == x && x != 0) ? aherm(p[it][t], pm[it][t], (R)g.inv_n)
@@ -379,14 +403,14 @@ k_yb_adv(Geo g, int outs_rt, const cplx<R>* __restrict__ psi, const cplx<R>* __r
           // loads they would be hoisted to the kernel's start and cost a wave
           // per SIMD in registers
           bool in = ok && F::valid(lane, it);
-          bool keep = in && g.mask[y * g.nx + x];
+          bool keep = in && g.keep(x, y);
           a = keep ? q[((size_t)z * NY + y) * g.nx + x] : czero<R>();
           am = keep ? q[((size_t)z * NY + my) * g.nx + mx] : czero<R>();
         } else {
           bool in = ok && F::valid(lane, it);
-          bool keep = in && g.mask[y * g.nx + x];
-          double kh2 = kxv * kxv + g.ky[y] * g.ky[y];
-          double kh2m = kxm * kxm + g.ky[my] * g.ky[my];
+          bool keep = in && g.keep(x, y);
+          double kh2 = kxv * kxv + ky[y] * ky[y];
+          double kh2m = kxm * kxm + ky[my] * ky[my];
           a = keep ? scale(p[it][t], (R)-kh2) : czero<R>();
           am = keep ? scale(pm[it][t], (R)-kh2m) : czero<R>();
         }
@@ -841,6 +865,7 @@ k_yb2_q(Geo g, int mode, int flags_rt, const cplx<R>* __restrict__ G1, const cpl
   const int x = blk.x * TL::S + s, z = blk.y;
   const bool ok = x < g.Kx;
   cplx<R>* sl = lds + s * TL::LD;
+  const double* __restrict__ ky = stage_k<NY>(g.ky);
   cplx<R> a[F::IT][F::R0], b[F::IT][F::R0];
   if (flags & YQ_ADV) {  // uniform
 #pragma unroll
@@ -865,11 +890,11 @@ k_yb2_q(Geo g, int mode, int flags_rt, const cplx<R>* __restrict__ G1, const cpl
     for (int t = 0; t < F::R0; ++t) {
       int y = F::elem(lane, it, t);
       if (!(ok && F::valid(lane, it))) continue;
-      const bool keep = g.mask[y * g.nx + x] != 0;
+      const bool keep = g.keep(x, y);
       if (g.pruned() && !keep) continue;  // q, the bracket: zero there already
       cplx<double> nq = czero<double>();
       if ((flags & YQ_ADV) && keep)
-        nq = ik_mul(g.kx[x], cvt<double>(a[it][t])) + ik_mul(g.ky[y], cvt<double>(b[it][t]));
+        nq = ik_mul(g.kx[x], cvt<double>(a[it][t])) + ik_mul(ky[y], cvt<double>(b[it][t]));
       const int my = mirror(y, NY);
       // The ETD coefficients depend on (kx, ky) through kh2 only, so the
       // mirror column (mx, my) has those of (x, y).
@@ -938,6 +963,7 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
   const int x = fw_x(g, TL::S, blk.x, s), z = blk.y;
   const bool ok = x < g.nx && g.col_kept(x);
   cplx<R>* sl = lds + s * TL::LD;
+  const double* __restrict__ ky = stage_k<NY>(g.ky);
   cplx<R> v[F::IT][F::R0], w[F::IT][F::R0];
   const bool has3 = (flags & (YB_ADV | YB_REFR)) != 0, has4 = (flags & YB_ADV) != 0;  // uniform
   // both inputs are loaded before either transform: twice the bytes in flight
@@ -987,17 +1013,17 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
       const cplx<R> g3 = v[it][t];
       v[it][t] = czero<R>();
       if (!(ok && F::valid(lane, it))) continue;
-      const bool keep = g.mask[y * g.nx + x] != 0;
+      const bool keep = g.keep(x, y);
       if (g.pruned() && !keep) continue;  // B, the bracket: zero there already
       const size_t col = (size_t)y * g.nx + x;
       const size_t ad = (size_t)z * plane + col;
       cplx<double> rhs = czero<double>();
       if (keep) {
         cplx<double> acc = has3 ? cvt<double>(g3) : czero<double>();
-        if (has4) acc = acc + ik_mul(-g.ky[y], cvt<double>(w[it][t]));  // - i ky F(vB)
+        if (has4) acc = acc + ik_mul(-ky[y], cvt<double>(w[it][t]));  // - i ky F(vB)
         cplx<double> t1 = czero<double>();
         if (flags & YB_DISP) {
-          const double kh2 = kxv * kxv + g.ky[y] * g.ky[y];
+          const double kh2 = kxv * kxv + ky[y] * ky[y];
           const double c = alpha * kh2;
           cplx<double> Av = cvt<double>(GROUP ? av[it][t] : A[ad]);
           t1 = mk<double>(-(c * Av.im), c * Av.re);
@@ -1040,9 +1066,9 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
         int y = F::elem(lane, it, t);
         const bool in = ok && F::valid(lane, it);
         const cplx<double> bm =
-            (in && g.mask[y * g.nx + x]) ? scale(cvt<double>(v[it][t]), g.inv_n) : czero<double>();
+            (in && g.keep(x, y)) ? scale(cvt<double>(v[it][t]), g.inv_n) : czero<double>();
         v[it][t] = cvt<R>(bm);
-        w[it][t] = cvt<R>(ik_mul(g.ky[y], bm));
+        w[it][t] = cvt<R>(ik_mul(ky[y], bm));
       }
     FI::run(v, sl, lane, twr);
     if (T3) FI::run(w, sl, lane, twr);  // uniform
@@ -1155,7 +1181,7 @@ k_yf_omega(Geo g, const cplx<R>* __restrict__ psi, cplx<R>* __restrict__ Pbx, cp
   // psi-bar = (psi(zu) + psi(z)) / 2 (psi itself at the top)
   auto dzf = [&](cplx<R> u1, cplx<R> u0) { return mk<R>((u1.re - u0.re) / rdz, (u1.im - u0.im) / rdz); };
   auto avg = [&](cplx<R> u1, cplx<R> u0) { return scale(u1 + u0, (R)0.5); };
-  const double* __restrict__ gky = g.ky;
+  const double* __restrict__ gky = stage_k<NY>(g.ky);
   // outputs: 0 i kx psi_z, 1 i ky psi_z, 2 -i kx kh2 psi-bar, 3 -i ky kh2 psi-bar,
   // 4 -i ky psi (u), 5 i kx psi (v), 6 psi
   cplx<R> v[F::IT][F::R0];
@@ -1167,7 +1193,7 @@ k_yf_omega(Geo g, const cplx<R>* __restrict__ psi, cplx<R>* __restrict__ Pbx, cp
     if (nyq && !kxo) continue;  // uniform
     // ky and kh2 loaded again for every output (kept, they would hold 32
     // registers through every transform)
-    const double* __restrict__ ky_o = opaque(gky);
+    lds_cdouble* __restrict__ ky_o = opaque_lds(gky);
 #pragma unroll
     for (int it = 0; it < F::IT; ++it)
 #pragma unroll
@@ -1342,7 +1368,7 @@ k_y_r2c_mask(Geo g, const cplx<R>* __restrict__ Rw, cplx<R>* __restrict__ out,
     for (int t = 0; t < F::R0; ++t) {
       int y = F::elem(lane, it, t);
       if (!(ok && F::valid(lane, it))) continue;
-      const bool keep = g.mask[y * g.nx + x] != 0;
+      const bool keep = g.keep(x, y);
       if (g.pruned() && !keep) continue;  // pruned readers do not look there
       cplx<double> c = keep ? cvt<double>(v[it][t]) : czero<double>();
       out[((size_t)z * NY + y) * g.nx + x] = cvt<R>(c);

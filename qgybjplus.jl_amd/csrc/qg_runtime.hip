@@ -968,8 +968,9 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
     g.dz = h->dz;
     g.Kx = W;
     g.kd2 = -1;
+    g.km2 = (std::min(nx, ny) / 3) * (std::min(nx, ny) / 3);  // is_dealiased (src/physics.jl:36-43)
     {
-      const int kmax = std::min(nx, ny) / 3;  // is_dealiased (src/physics.jl:36-43)
+      const int kmax = std::min(nx, ny) / 3;
       h->gp = g;
       h->gp.Kx = kmax + 1;
       h->gp.kd2 = kmax * kmax;
