@@ -223,6 +223,7 @@ k_xa_qw(Geo g, const cplx<R>* __restrict__ T1, const cplx<R>* __restrict__ T3,
   const bool ok = row < g.nz * g.ny;
   cplx<R>* sl = lds + r * TL::LD;
   const size_t base = (size_t)row * NX;
+  const double* __restrict__ kx = stage_k<NX>(g.kx);
   cplx<R> v[FI::IT][FI::R0], w[FI::IT][FI::R0], y3[FI::IT][FI::R0];
   R mag[FI::IT][FI::R0];
   // B and Bx; the By row is loaded with them
@@ -236,7 +237,7 @@ k_xa_qw(Geo g, const cplx<R>* __restrict__ T1, const cplx<R>* __restrict__ T3,
       cplx<R> a = keep_if(ldu(T1, base + x, in && kc), kc);
       y3[it][t] = keep_if(ldu(T3, base + x, in && kc), kc);
       v[it][t] = a;
-      w[it][t] = ik_mul((R)g.kx[x], a);
+      w[it][t] = ik_mul((R)kx[x], a);
     }
   FI::run(v, sl, lane, twr);
 #pragma unroll
@@ -290,6 +291,7 @@ k_ya2_qeff(Geo g, const cplx<R>* __restrict__ Qa, const cplx<R>* __restrict__ Qb
   const bool ok = x < g.Kx;
   cplx<R>* sl = lds + s * TL::LD;
   const double* __restrict__ ky = stage_k<NY>(g.ky);
+  const double kxv = ok ? g.kx[x] : 0.0;
   cplx<R> v[F::IT][F::R0], w[F::IT][F::R0];
 #pragma unroll
   for (int it = 0; it < F::IT; ++it)
@@ -314,7 +316,7 @@ k_ya2_qeff(Geo g, const cplx<R>* __restrict__ Qa, const cplx<R>* __restrict__ Qb
       if (!(ok && F::valid(lane, it))) continue;
       const bool keep = g.keep(x, y);
       if (g.pruned() && !keep) continue;  // zero there already
-      const double kh2 = g.kx[x] * g.kx[x] + ky[y] * ky[y];
+      const double kh2 = kxv * kxv + ky[y] * ky[y];
       cplx<double> fq = cvt<double>(v[it][t]), fm = cvt<double>(w[it][t]);
       cplx<double> qw = keep ? scale(fq - scale(fm, 0.25 * kh2), inv_f) : czero<double>();
       size_t a = ((size_t)z * NY + y) * g.nx + x;
@@ -613,6 +615,7 @@ k_xb_B(Geo g, int flags, const cplx<R>* __restrict__ T1, const cplx<R>* __restri
   const bool ok = row < g.nz * g.ny;
   cplx<R>* sl = lds + r * TL::LD;
   const size_t base = (size_t)row * NX;
+  const double* __restrict__ kx = stage_k<NX>(g.kx);
   cplx<R> b[FI::IT][FI::R0], v[FI::IT][FI::R0], h[FI::IT][FI::R0];
   R zt[FI::IT][FI::R0], ur[FI::IT][FI::R0], vr[FI::IT][FI::R0];
   const bool adv = (flags & XB_ADV) != 0;  // uniform
@@ -657,7 +660,7 @@ k_xb_B(Geo g, int flags, const cplx<R>* __restrict__ T1, const cplx<R>* __restri
         if (f == 1) {
           if (ok && FI::valid(lane, it) && g.col_kept(x)) G4[base + x] = v[it][t];
         } else {
-          const R m = f == 0 ? (R)-g.kx[x] : (R)-0.5;
+          const R m = f == 0 ? (R)-kx[x] : (R)-0.5;
           h[it][t] = h[it][t] + ik_mul(m, v[it][t]);
         }
       }
@@ -692,8 +695,9 @@ k_xb_all(Geo g, const cplx<R>* __restrict__ Pq, const cplx<R>* __restrict__ Pz, 
   const bool ok = row < g.nz * g.ny;
   cplx<R>* sl = lds + r * TL::LD;
   const size_t base = (size_t)row * NX, hb = (size_t)row * g.Wp;
-  cplx<R> b[FI::IT][FI::R0], v[FI::IT][FI::R0], zz[FI::IT][FI::R0];
-  R ur[FI::IT][FI::R0], vr[FI::IT][FI::R0];
+  const double* __restrict__ kx = stage_k<NX>(g.kx);
+  cplx<R> b[FI::IT][FI::R0], v[FI::IT][FI::R0];
+  R ur[FI::IT][FI::R0], vr[FI::IT][FI::R0], zr[FI::IT][FI::R0];
   // every input row is loaded before the first transform
   if constexpr (UVP) {
     cplx<R> w[FI::IT][FI::R0];
@@ -706,8 +710,7 @@ k_xb_all(Geo g, const cplx<R>* __restrict__ Pq, const cplx<R>* __restrict__ Pz, 
         const bool kc = g.col_kept(x);
         b[it][t] = keep_if(ldu(T1, base + x, ok && FI::valid(lane, it) && kc), kc);
       }
-    load_herm_row<R, FI>(v, Pq, hb, NX, g.Kx, lane, ok);
-    load_herm_row<R, FI>(zz, Pz, hb, NX, g.Kx, lane, ok);
+    load_herm_pair<R, FI>(v, Pq, Pz, hb, NX, g.Kx, lane, ok);
     FI::run(w, sl, lane, twr);
 #pragma unroll
     for (int it = 0; it < FI::IT; ++it)
@@ -727,16 +730,18 @@ k_xb_all(Geo g, const cplx<R>* __restrict__ Pq, const cplx<R>* __restrict__ Pz, 
         ur[it][t] = ldu(u, base + x, in);
         vr[it][t] = ldu(vv, base + x, in);
       }
-    load_herm_row<R, FI>(v, Pq, hb, NX, g.Kx, lane, ok);
-    load_herm_row<R, FI>(zz, Pz, hb, NX, g.Kx, lane, ok);
+    load_herm_pair<R, FI>(v, Pq, Pz, hb, NX, g.Kx, lane, ok);
   }
-  // q: u*q + i v*q, one forward transform, unpacked through LDS (as k_xb_q)
+  // q and zeta (both real, of one kind: q = zeta plus the stretching term) in
+  // one packed inverse transform: q in .re, zeta in .im.  Then u*q + i v*q,
+  // one forward transform, unpacked through LDS (as k_xb_q).
   FI::run(v, sl, lane, twr);
 #pragma unroll
   for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
     for (int t = 0; t < FI::R0; ++t) {
       const R qv = v[it][t].re;
+      zr[it][t] = v[it][t].im;
       v[it][t] = mk<R>(ur[it][t] * qv, vr[it][t] * qv);
     }
   FF::run(v, sl, lane, twr);
@@ -760,7 +765,6 @@ k_xb_all(Geo g, const cplx<R>* __restrict__ Pq, const cplx<R>* __restrict__ Pz, 
     }
   // B: u*B, v*B, zeta*B (as k_xb_B); FI::run starts with a barrier, so the
   // LDS reads above are done before it writes
-  FI::run(zz, sl, lane, twr);
   FI::run(b, sl, lane, twr);
   cplx<R> h[FI::IT][FI::R0];
 #pragma unroll
@@ -769,7 +773,7 @@ k_xb_all(Geo g, const cplx<R>* __restrict__ Pq, const cplx<R>* __restrict__ Pz, 
     for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
       for (int t = 0; t < FI::R0; ++t) {
-        const R c = f == 2 ? zz[it][t].re : (f == 0 ? ur[it][t] : vr[it][t]);
+        const R c = f == 2 ? zr[it][t] : (f == 0 ? ur[it][t] : vr[it][t]);
         v[it][t] = mk<R>(c * b[it][t].re, c * b[it][t].im);
       }
     FF::run(v, sl, lane, twr);
@@ -781,7 +785,7 @@ k_xb_all(Geo g, const cplx<R>* __restrict__ Pq, const cplx<R>* __restrict__ Pz, 
         if (f == 1) {
           if (ok && FI::valid(lane, it) && g.col_kept(x)) G4[base + x] = v[it][t];
         } else if (f == 0) {
-          h[it][t] = ik_mul((R)-g.kx[x], v[it][t]);
+          h[it][t] = ik_mul((R)-kx[x], v[it][t]);
         } else {
           h[it][t] = h[it][t] + ik_mul((R)-0.5, v[it][t]);
         }
@@ -866,6 +870,7 @@ k_yb2_q(Geo g, int mode, int flags_rt, const cplx<R>* __restrict__ G1, const cpl
   const bool ok = x < g.Kx;
   cplx<R>* sl = lds + s * TL::LD;
   const double* __restrict__ ky = stage_k<NY>(g.ky);
+  const double kxv = ok ? g.kx[x] : 0.0;
   cplx<R> a[F::IT][F::R0], b[F::IT][F::R0];
   if (flags & YQ_ADV) {  // uniform
 #pragma unroll
@@ -894,7 +899,7 @@ k_yb2_q(Geo g, int mode, int flags_rt, const cplx<R>* __restrict__ G1, const cpl
       if (g.pruned() && !keep) continue;  // q, the bracket: zero there already
       cplx<double> nq = czero<double>();
       if ((flags & YQ_ADV) && keep)
-        nq = ik_mul(g.kx[x], cvt<double>(a[it][t])) + ik_mul(ky[y], cvt<double>(b[it][t]));
+        nq = ik_mul(kxv, cvt<double>(a[it][t])) + ik_mul(ky[y], cvt<double>(b[it][t]));
       const int my = mirror(y, NY);
       // The ETD coefficients depend on (kx, ky) through kh2 only, so the
       // mirror column (mx, my) has those of (x, y).
