@@ -155,6 +155,19 @@ struct qg_handle {
   std::vector<int> step_weights;  // steps covered by each interval of step_events
   std::vector<hipEvent_t> pool;
   size_t pool_used = 0;
+  // side stream of the step's independent passes (default priority, as the
+  // main stream) and the events that order the two; reused in turn (a wait
+  // refers to the record issued just before it)
+  hipStream_t st2 = nullptr;
+  bool concurrent = true;
+  hipEvent_t sync_ev[8] = {};
+  int sync_next = 0;
+  hipEvent_t last_sync = nullptr;
+  hipEvent_t sync_event() {
+    last_sync = sync_ev[sync_next];
+    sync_next = (sync_next + 1) % 8;
+    return last_sync;
+  }
   hipEvent_t event() {
     if (pool_used == pool.size()) {
       hipEvent_t e;
@@ -193,9 +206,13 @@ struct qg_handle {
   }
   ~qg_handle() {
     if (st) (void)hipStreamSynchronize(st);
+    if (st2) (void)hipStreamSynchronize(st2);
     drop_graphs();
     for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : sync_ev)
+      if (e) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
+    if (st2) (void)hipStreamDestroy(st2);
     if (st) (void)hipStreamDestroy(st);
   }
 };
@@ -325,16 +342,16 @@ void check_status(qg_handle& h) {
 
 // invert_q_to_psi! (src/elliptic.jl:189-314) on device arrays (the columns of
 // geometry g: all, or the disk's when pruned).
-void solve_psi(qg_handle& h, const qg::Geo& g, const void* q, void* psi) {
+void solve_psi(qg_handle& h, const qg::Geo& g, hipStream_t st, const void* q, void* psi) {
   if (h.sing_psi[0]) raise_singular(h.sing_psi);
-  qg::launch_thomas(g, h.st, q, psi, nullptr, h.f32(), h.th_inv[0], h.th_cp[0], h.d_al, 1, h.d_warn,
+  qg::launch_thomas(g, st, q, psi, nullptr, h.f32(), h.th_inv[0], h.th_cp[0], h.d_al, 1, h.d_warn,
                     h.tune[qg::P_THOMAS_PSI]);
 }
 
 // invert_B_to_A! (src/elliptic.jl:836-990) on device arrays; C optional.
-void solve_A(qg_handle& h, const qg::Geo& g, const void* B, void* A, void* C) {
+void solve_A(qg_handle& h, const qg::Geo& g, hipStream_t st, const void* B, void* A, void* C) {
   if (h.sing_A[0]) raise_singular(h.sing_A);
-  qg::launch_thomas(g, h.st, B, A, C, h.f32(), h.th_inv[1], h.th_cp[1], h.d_al, 4, nullptr,
+  qg::launch_thomas(g, st, B, A, C, h.f32(), h.th_inv[1], h.th_cp[1], h.d_al, 4, nullptr,
                     h.tune[C ? qg::P_THOMAS_AC : qg::P_THOMAS_A]);
 }
 
@@ -353,7 +370,37 @@ struct Engine {
   // per-level tables of a column solve (8 bytes per quarter column solved).
   const qg::Geo* G = nullptr;
   double P = 0, Ps = 0, Ph = 0, Pf = 0, Tq = 0;
-  explicit Engine(qg_handle& hh) : h(hh) { use(hh.g); }
+  // The stream launches go to: the handle's stream, or its side stream for
+  // passes that run beside the main chain (side()).
+  hipStream_t S;
+  bool joined = true;
+  explicit Engine(qg_handle& hh) : h(hh), S(hh.st) { use(hh.g); }
+
+  // Passes independent of the main chain run on the side stream, ordered
+  // after everything issued so far on the main stream; join() orders the main
+  // stream after them.  Under hipGraph capture the two streams become parallel
+  // branches of the graph.  With per-launch profiling (or the side stream
+  // switched off) everything runs on the main stream in issue order.
+  bool concurrent() const { return h.concurrent && h.st2 && !(h.prof_mode & 1); }
+  template <class F>
+  void side(F&& fn) {
+    if (!concurrent()) {
+      fn();
+      return;
+    }
+    HIPCHECK(hipEventRecord(h.sync_event(), h.st));
+    HIPCHECK(hipStreamWaitEvent(h.st2, h.last_sync, 0));
+    S = h.st2;
+    fn();
+    S = h.st;
+    joined = false;
+  }
+  void join() {
+    if (joined) return;
+    HIPCHECK(hipEventRecord(h.sync_event(), h.st2));
+    HIPCHECK(hipStreamWaitEvent(h.st, h.last_sync, 0));
+    joined = true;
+  }
   void use(const qg::Geo& g) {
     G = &g;
     const double lv = g.nz;
@@ -380,9 +427,9 @@ struct Engine {
   void K(const char* name, int pass, double bytes, F&& fn) {
     if (h.prof_mode & 1) {
       hipEvent_t a = h.event(), b = h.event();
-      HIPCHECK(hipEventRecord(a, h.st));
+      HIPCHECK(hipEventRecord(a, S));
       fn();
-      HIPCHECK(hipEventRecord(b, h.st));
+      HIPCHECK(hipEventRecord(b, S));
       h.prof.push_back({name, pass, bytes, a, b});
     } else {
       fn();
@@ -404,11 +451,11 @@ struct Engine {
     C* P = c(h.Pz) + (size_t)m0 * h.nzc * h.ny * h.g.Wp;
     const double pts = (double)h.plane * g.nz, hpts = (double)h.g.W * h.ny * g.nz;
     K("check_y", -1, pts * 3 * cS + hpts * cR, [&] {
-      O::check_y(h.tune[qg::P_YF_OMEGA], g, h.st, static_cast<const C*>(h.at(h.psi, off)),
+      O::check_y(h.tune[qg::P_YF_OMEGA], g, S, static_cast<const C*>(h.at(h.psi, off)),
                  static_cast<const C*>(h.at(h.q[h.cur], off)), static_cast<const C*>(h.at(h.B[h.cur], off)), P,
                  h.d_flag, twy());
     });
-    K("check_x", -1, hpts * cR, [&] { O::check_x(h.tune[qg::P_XF], g, h.st, P, h.d_absmax, twx()); });
+    K("check_x", -1, hpts * cR, [&] { O::check_x(h.tune[qg::P_XF], g, S, P, h.d_absmax, twx()); });
   }
 
   // _progress_maxima (src/simulation.jl:46-81): max |ifft2(LA)| with LA =
@@ -417,13 +464,13 @@ struct Engine {
   // rejected), so LA always carries kh2 A / 4; A is zero when passive or
   // without dispersion, as the reference fills it.
   void progress_maxima() {
-    O::la_max(1, h.g, h.st, static_cast<const C*>(h.B[h.cur]), static_cast<const C*>(h.A), 1, c(h.G3), h.d_pmax,
+    O::la_max(1, h.g, S, static_cast<const C*>(h.B[h.cur]), static_cast<const C*>(h.A), 1, c(h.G3), h.d_pmax,
               twx(), twy());
-    qg::launch_max_hypot(h.u, h.v, h.f32(), (size_t)h.plane * h.g.nz, h.d_pmax + 1, h.st);
+    qg::launch_max_hypot(h.u, h.v, h.f32(), (size_t)h.plane * h.g.nz, h.d_pmax + 1, S);
   }
 
   void t1_only(const C* Bsrc) {
-    K("ya_T1", qg::P_YA_QW, Ps * cS + Pf * cR, [&] { O::ya_qw(h.tune[qg::P_YA_QW], *G, h.st, Bsrc, c(h.T1), nullptr, twy()); });
+    K("ya_T1", qg::P_YA_QW, Ps * cS + Pf * cR, [&] { O::ya_qw(h.tune[qg::P_YA_QW], *G, S, Bsrc, c(h.T1), nullptr, twy()); });
   }
 
   // _diagnose_flow! (src/timestep.jl:192-215) minus the velocities: q_eff with
@@ -436,36 +483,36 @@ struct Engine {
       const C* src = qsrc;
       if (h.feedback) {
         if (!t_ready)
-          K("ya_qw", qg::P_YA_QW, Ps * cS + Pf * 2 * cR, [&] { O::ya_qw(h.tune[qg::P_YA_QW], *G, h.st, Bsrc, c(h.T1), c(h.T3), twy()); });
+          K("ya_qw", qg::P_YA_QW, Ps * cS + Pf * 2 * cR, [&] { O::ya_qw(h.tune[qg::P_YA_QW], *G, S, Bsrc, c(h.T1), c(h.T3), twy()); });
         K("xa_qw", qg::P_XA_QW, Pf * 2 * cR + Ph * 2 * cR,
-          [&] { O::xa_qw(h.tune[qg::P_XA_QW], *G, h.st, c(h.T1), c(h.T3), c(h.Qa), c(h.Qb), twx()); });
+          [&] { O::xa_qw(h.tune[qg::P_XA_QW], *G, S, c(h.T1), c(h.T3), c(h.Qa), c(h.Qb), twx()); });
         K("ya2_qeff", qg::P_YA2_QEFF, Ph * 2 * cR + Ps * 2 * cS,
-          [&] { O::ya2_qeff(h.tune[qg::P_YA2_QEFF], *G, h.st, c(h.Qa), c(h.Qb), qsrc, c(h.qeff), nullptr, 1.0 / h.f, twy()); });
+          [&] { O::ya2_qeff(h.tune[qg::P_YA2_QEFF], *G, S, c(h.Qa), c(h.Qb), qsrc, c(h.qeff), nullptr, 1.0 / h.f, twy()); });
         src = c(h.qeff);
         t1 = true;
       }
       K("thomas_psi", qg::P_THOMAS_PSI, Ps * 2 * cS + Tq,
-        [&] { solve_psi(h, *G, src, psi_out); });
+        [&] { solve_psi(h, *G, S, src, psi_out); });
     }
     if (need_t1 && !t1) t1_only(Bsrc);
   }
 
   void wave_inv(const C* Bsrc, C* Aout, C* Cout) {
     if (h.passive || h.no_disp) {
-      qg::launch_zero_bytes(Aout, h.plane * G->nz * h.fsize(), h.st);
-      if (Cout) qg::launch_zero_bytes(Cout, h.plane * G->nz * h.fsize(), h.st);
+      qg::launch_zero_bytes(Aout, h.plane * G->nz * h.fsize(), S);
+      if (Cout) qg::launch_zero_bytes(Cout, h.plane * G->nz * h.fsize(), S);
       return;
     }
     K(Cout ? "thomas_AC" : "thomas_A", Cout ? qg::P_THOMAS_AC : qg::P_THOMAS_A, Ps * (Cout ? 3 : 2) * cS + Tq,
-      [&] { solve_A(h, *G, Bsrc, Aout, Cout); });
+      [&] { solve_A(h, *G, S, Bsrc, Aout, Cout); });
   }
 
   void velocities(const C* psi) {
     K("yb_uv", qg::P_YB_ADV, Ps * cS + Ph * 2 * cR, [&] {
-      O::yb_adv(h.tune[qg::P_YB_ADV], *G, h.st, qg::OUT_U | qg::OUT_V, psi, nullptr, c(h.Pu), c(h.Pv), nullptr, nullptr, twy());
+      O::yb_adv(h.tune[qg::P_YB_ADV], *G, S, qg::OUT_U | qg::OUT_V, psi, nullptr, c(h.Pu), c(h.Pv), nullptr, nullptr, twy());
     });
     K("x_c2r_uv", qg::P_X_C2R, Ph * 2 * cR + P * 2 * rR,
-      [&] { O::x_c2r(h.tune[qg::P_X_C2R], *G, h.st, 2, c(h.Pu), c(h.Pv), r(h.u), r(h.v), twx()); });
+      [&] { O::x_c2r(h.tune[qg::P_X_C2R], *G, S, 2, c(h.Pu), c(h.Pv), r(h.u), r(h.v), twx()); });
   }
 
   // u, v and the QG omega-equation w (compute_velocities!(compute_w=true)).
@@ -479,14 +526,14 @@ struct Engine {
     C* Ppsi = chk == 1 ? c(h.Pz) : nullptr;
     double* lsum = chk == 2 ? h.d_lsum : nullptr;
     K("yf_omega", qg::P_YF_OMEGA, Ps * cS + Ph * (chk == 1 ? 7 : 6) * cR, [&] {
-      O::yf_omega(h.tune[qg::P_YF_OMEGA], *G, h.st, psi, c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), c(h.Pu), c(h.Pv),
+      O::yf_omega(h.tune[qg::P_YF_OMEGA], *G, S, psi, c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), c(h.Pu), c(h.Pv),
                   Ppsi, h.d_flag, lsum, twy());
     });
     K("xf", qg::P_XF, Ph * (chk == 1 ? 8 : 7) * cR + P * 2 * rR, [&] {
-      O::xf(h.tune[qg::P_XF], *G, h.st, 1, c(h.Pu), c(h.Pv), c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), r(h.u), r(h.v), c(h.Pq),
+      O::xf(h.tune[qg::P_XF], *G, S, 1, c(h.Pu), c(h.Pv), c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), r(h.u), r(h.v), c(h.Pq),
             Ppsi, h.d_absmax, lsum, twx());
     });
-    K("y_r2c_omega", qg::P_Y_R2C, Ph * cR + Ps * cS, [&] { O::y_r2c_mask(h.tune[qg::P_Y_R2C], *G, h.st, c(h.Pq), c(h.rq0), twy()); });
+    K("y_r2c_omega", qg::P_Y_R2C, Ph * cR + Ps * cS, [&] { O::y_r2c_mask(h.tune[qg::P_Y_R2C], *G, S, c(h.Pq), c(h.rq0), twy()); });
     if (h.sing_omega[0]) {
       char buf[160];
       snprintf(buf, sizeof buf, "LAPACK-style gtsv failed for vertical velocity at row %d of column %d",
@@ -494,11 +541,11 @@ struct Engine {
       throw QgError(QG_ERR_GTSV, buf);
     }
     K("omega_gtsv", qg::P_OMEGA, Ps * 2 * cS + Tq, [&] {
-      qg::launch_omega(*G, h.st, c(h.rq0), c(h.qeff), h.f32(), h.om_M, h.om_D, h.om_piv, h.om_any_piv, h.d_cz, h.d_cu,
+      qg::launch_omega(*G, S, c(h.rq0), c(h.qeff), h.f32(), h.om_M, h.om_D, h.om_piv, h.om_any_piv, h.d_cz, h.d_cu,
                        h.d_fN2, h.s0, h.s1, h.s2, h.tune[qg::P_OMEGA]);
     });
-    K("y_herm_w", qg::P_Y_HERM, Ps * cS + Ph * cR, [&] { O::y_herm(h.tune[qg::P_Y_HERM], *G, h.st, c(h.qeff), c(h.Pz), twy()); });
-    K("x_c2r_w", qg::P_X_C2R, Ph * cR + P * rR, [&] { O::x_c2r(h.tune[qg::P_X_C2R], *G, h.st, 1, c(h.Pz), nullptr, r(h.w), nullptr, twx()); });
+    K("y_herm_w", qg::P_Y_HERM, Ps * cS + Ph * cR, [&] { O::y_herm(h.tune[qg::P_Y_HERM], *G, S, c(h.qeff), c(h.Pz), twy()); });
+    K("x_c2r_w", qg::P_X_C2R, Ph * cR + P * rR, [&] { O::x_c2r(h.tune[qg::P_X_C2R], *G, S, 1, c(h.Pz), nullptr, r(h.w), nullptr, twx()); });
   }
 
   // One RHS evaluation at (qS, psiS, AS, T1, u, v) fused with the ETD update
@@ -526,7 +573,7 @@ struct Engine {
       K(uv ? "yb_adv_uv" : "yb_adv", qg::P_YB_ADV,
         // reads q for OUT_Q and psi for the others
         Ps * (((outs & qg::OUT_Q) ? cS : 0) + ((outs & ~qg::OUT_Q) ? cS : 0)) + Ph * nouts * cR, [&] {
-          O::yb_adv(h.tune[qg::P_YB_ADV], *G, h.st, outs, psiS, qS, uv ? c(h.Pu) : nullptr, uv ? c(h.Pv) : nullptr,
+          O::yb_adv(h.tune[qg::P_YB_ADV], *G, S, outs, psiS, qS, uv ? c(h.Pu) : nullptr, uv ? c(h.Pv) : nullptr,
                     c(h.Pq), c(h.Pz), twy());
         });
     if (adv && qdyn && refr) {
@@ -534,27 +581,30 @@ struct Engine {
       // G1, G2 (half width), G3, G4 out
       K(uv ? "xb_all_uv" : "xb_all", qg::P_XB_B,
         Ph * 4 * cR + Pf * cR + (uv ? Ph * 2 * cR : P * 2 * rR) + Pf * 2 * cR, [&] {
-          O::xb_all(h.tune[qg::P_XB_B], *G, h.st, c(h.Pq), c(h.Pz), c(h.T1), uv ? c(h.Pu) : nullptr,
+          O::xb_all(h.tune[qg::P_XB_B], *G, S, c(h.Pq), c(h.Pz), c(h.T1), uv ? c(h.Pu) : nullptr,
                     uv ? c(h.Pv) : nullptr, r(h.u), r(h.v), c(h.G1), c(h.G2), c(h.G3), c(h.G4), twx());
         });
     } else if (adv && qdyn) {
       K(uv ? "xb_q_uv" : "xb_q", qg::P_XB_Q, Ph * (uv ? 3 : 1) * cR + Ph * 2 * cR + P * 2 * rR, [&] {
-        O::xb_q(h.tune[qg::P_XB_Q], *G, h.st, c(h.Pq), uv ? c(h.Pu) : nullptr, uv ? c(h.Pv) : nullptr, r(h.u),
+        O::xb_q(h.tune[qg::P_XB_Q], *G, S, c(h.Pq), uv ? c(h.Pu) : nullptr, uv ? c(h.Pv) : nullptr, r(h.u),
                 r(h.v), c(h.G1), c(h.G2), twx());
       });
     }
     int xbf = (adv ? qg::XB_ADV : 0) | (refr ? qg::XB_REFR : 0);
     if (xbf && !(adv && qdyn && refr))
       K("xb_B", qg::P_XB_B, Pf * cR + (refr ? Ph * cR : 0) + (adv ? P * 2 * rR : 0) + Pf * cR * (1 + (adv ? 1 : 0)),
-        [&] { O::xb_B(h.tune[qg::P_XB_B], *G, h.st, xbf, c(h.T1), c(h.Pz), r(h.u), r(h.v), c(h.G3), c(h.G4), twx()); });
+        [&] { O::xb_B(h.tune[qg::P_XB_B], *G, S, xbf, c(h.T1), c(h.Pz), r(h.u), r(h.v), c(h.G3), c(h.G4), twx()); });
     const bool diff = h.nu_z != 0.0 && !h.inviscid && h.nz > 1;
     int qf = ((adv && qdyn) ? qg::YQ_ADV : 0) | (h.fixed_flow ? qg::YQ_FIXED : 0) | (diff ? qg::YQ_DIFF : 0);
     const double inv_dz2 = h.nz > 1 ? 1.0 / (h.dz * h.dz) : 0.0;
-    K(mode == qg::MODE_STAGE ? "yb2_q_stage" : "yb2_q_final", qg::P_YB2_Q,
-      // stage: q_n, the kept bracket and q_S; final: the bracket and q_{n+1}
-      ((qf & qg::YQ_ADV) ? Ph * 2 * cR : 0) +
-          Ps * cS * ((mode == qg::MODE_STAGE || h.fixed_flow ? 3 : 2) + (diff ? 1 : 0)),
-      [&] { O::yb2_q(h.tune[qg::P_YB2_Q], *G, h.st, mode, qf, c(h.G1), c(h.G2), qS, qn, c(h.rq0), qout, etd(), h.nu_z, inv_dz2, flag, twy()); });
+    // the q update beside the B update (disjoint inputs and outputs)
+    side([&] {
+      K(mode == qg::MODE_STAGE ? "yb2_q_stage" : "yb2_q_final", qg::P_YB2_Q,
+        // stage: q_n, the kept bracket and q_S; final: the bracket and q_{n+1}
+        ((qf & qg::YQ_ADV) ? Ph * 2 * cR : 0) +
+            Ps * cS * ((mode == qg::MODE_STAGE || h.fixed_flow ? 3 : 2) + (diff ? 1 : 0)),
+        [&] { O::yb2_q(h.tune[qg::P_YB2_Q], *G, S, mode, qf, c(h.G1), c(h.G2), qS, qn, c(h.rq0), qout, etd(), h.nu_z, inv_dz2, flag, twy()); });
+    });
     const bool disp = !(h.passive || h.no_disp);
     int bf = (adv ? qg::YB_ADV : 0) | (refr ? qg::YB_REFR : 0) | (disp ? qg::YB_DISP : 0);
     // the new B's inverse y transforms for the next diagnosis: T1 always
@@ -563,7 +613,8 @@ struct Engine {
     K(mode == qg::MODE_STAGE ? "yb2_B_stage" : "yb2_B_final", qg::P_YB2_B,
       Pf * cR * (((adv || refr) ? 1 : 0) + (adv ? 1 : 0) + 1 + (T3 ? 1 : 0)) +
           Ps * cS * ((mode == qg::MODE_STAGE ? 3 : 2) + (disp ? 1 : 0)),
-      [&] { O::yb2_B(h.tune[qg::P_YB2_B], *G, h.st, mode, bf, c(h.G3), c(h.G4), AS, Bn, c(h.rB0), Bout, c(h.T1), T3, etd(), h.f / 2, flag, twy()); });
+      [&] { O::yb2_B(h.tune[qg::P_YB2_B], *G, S, mode, bf, c(h.G3), c(h.G4), AS, Bn, c(h.rB0), Bout, c(h.T1), T3, etd(), h.f / 2, flag, twy()); });
+    join();
   }
 
   // step!(model, ExponentialRungeKutta2) -> _advance_etdrk2! (src/timestep.jl:335-415).
@@ -594,21 +645,26 @@ struct Engine {
     rhs_update(qg::MODE_STAGE, qn, c(h.psi), c(h.A), qn, Bn, c(h.qs), c(h.Bs));
     // RHS at the stage (yb2_B left T1, T3 of B_S)
     C* psiS = h.fixed_flow ? c(h.psi) : c(h.psis);
+    // A_S = L+^-1 B_S beside the q_w -> q_eff -> psi_S chain (independent)
+    side([&] { wave_inv(c(h.Bs), c(h.As), nullptr); });
     diag(c(h.qs), c(h.Bs), psiS, true, true);
-    wave_inv(c(h.Bs), c(h.As), nullptr);
+    join();
     // the stage's u, v come out of the RHS passes (they are needed nowhere else)
     rhs_update(qg::MODE_FINAL, c(h.qs), psiS, c(h.As), qn, Bn, q1, B1, need_uv && !h.fixed_flow,
                chk ? h.d_flag : nullptr);
-    // finalize (src/timestep.jl:300-332); yb2_B left T1, T3 of B_{n+1}
+    // finalize (src/timestep.jl:300-332); yb2_B left T1, T3 of B_{n+1}.  A
+    // and C beside the diagnosis of psi and the velocity and omega passes,
+    // which read neither.
+    side([&] { wave_inv(B1, c(h.A), c(h.C)); });
     diag(q1, B1, c(h.psi), false, true);
-    wave_inv(B1, c(h.A), c(h.C));
     velocities_w(c(h.psi), chk);
+    join();
     if (can && !prune) {
       // The step's outputs (q, B, psi, A, C) are masked and written at every
       // point; the buffers it read from are the next step's outputs, which a
       // pruned step writes inside the disk only: zero their outside.
-      qg::launch_zero_outside_disk(h.gp, h.st, qn, h.f32());
-      qg::launch_zero_outside_disk(h.gp, h.st, Bn, h.f32());
+      qg::launch_zero_outside_disk(h.gp, S, qn, h.f32());
+      qg::launch_zero_outside_disk(h.gp, S, Bn, h.f32());
       h.clean = true;
     }
     h.cur = 1 - h.cur;
@@ -618,9 +674,9 @@ struct Engine {
   }
 
   void compute_qw(C* out) {
-    O::ya_qw(h.tune[qg::P_YA_QW], h.g, h.st, c(h.B[h.cur]), c(h.T1), c(h.T3), twy());
-    O::xa_qw(h.tune[qg::P_XA_QW], h.g, h.st, c(h.T1), c(h.T3), c(h.Qa), c(h.Qb), twx());
-    O::ya2_qeff(h.tune[qg::P_YA2_QEFF], h.g, h.st, c(h.Qa), c(h.Qb), c(h.q[h.cur]), nullptr, out, 1.0 / h.f, twy());
+    O::ya_qw(h.tune[qg::P_YA_QW], h.g, S, c(h.B[h.cur]), c(h.T1), c(h.T3), twy());
+    O::xa_qw(h.tune[qg::P_XA_QW], h.g, S, c(h.T1), c(h.T3), c(h.Qa), c(h.Qb), twx());
+    O::ya2_qeff(h.tune[qg::P_YA2_QEFF], h.g, S, c(h.Qa), c(h.Qb), c(h.q[h.cur]), nullptr, out, 1.0 / h.f, twy());
     h.t1_valid = false;
   }
 
@@ -629,7 +685,7 @@ struct Engine {
       velocities_w(c(h.psi));
     } else {
       velocities(c(h.psi));
-      qg::launch_zero_bytes(h.w, h.plane * h.g.nz * h.rsize(), h.st);
+      qg::launch_zero_bytes(h.w, h.plane * h.g.nz * h.rsize(), S);
     }
   }
 };
@@ -857,6 +913,8 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
       h->wo.assign(cfg->wave_orders, cfg->wave_orders + cfg->n_wave_terms);
     }
     HIPCHECK(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
+    HIPCHECK(hipStreamCreateWithFlags(&h->st2, hipStreamNonBlocking));
+    for (hipEvent_t& e : h->sync_ev) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     const int nx = h->nx, ny = h->ny, nz = h->nz;
     h->plane = (size_t)nx * ny;
     h->npts = h->plane * nz;
@@ -1669,7 +1727,7 @@ int qg_progress_maxima(qg_handle* h, double* wave_speed, double* flow_speed) {
 int qg_invert_q_to_psi(qg_handle* h) {
   return guarded(h, [&] {
     h->psi64_nm = 0;
-    solve_psi(*h, h->g, h->q[h->cur], h->psi);
+    solve_psi(*h, h->g, h->st, h->q[h->cur], h->psi);
     h->diag_valid = false;
     h->clean = false;
     check_status(*h);
@@ -1678,7 +1736,7 @@ int qg_invert_q_to_psi(qg_handle* h) {
 
 int qg_invert_B_to_A(qg_handle* h) {
   return guarded(h, [&] {
-    solve_A(*h, h->g, h->B[h->cur], h->A, h->C);
+    solve_A(*h, h->g, h->st, h->B[h->cur], h->A, h->C);
     h->diag_valid = false;
     h->clean = false;
     check_status(*h);
