@@ -160,6 +160,12 @@ struct qg_handle {
   // refers to the record issued just before it)
   hipStream_t st2 = nullptr;
   bool concurrent = true;
+  // stream of the finalize's w passes, which run beside the next step (their
+  // buffers Pw, Pw2, Wr, Ww are theirs alone); w_pending: issued there and
+  // not yet ordered before the main stream
+  hipStream_t st3 = nullptr;
+  bool w_pending = false;
+  void *Pw = nullptr, *Pw2 = nullptr, *Wr = nullptr, *Ww = nullptr;
   hipEvent_t sync_ev[8] = {};
   int sync_next = 0;
   hipEvent_t last_sync = nullptr;
@@ -207,12 +213,14 @@ struct qg_handle {
   ~qg_handle() {
     if (st) (void)hipStreamSynchronize(st);
     if (st2) (void)hipStreamSynchronize(st2);
+    if (st3) (void)hipStreamSynchronize(st3);
     drop_graphs();
     for (hipEvent_t e : pool) (void)hipEventDestroy(e);
     for (hipEvent_t e : sync_ev)
       if (e) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
     if (st2) (void)hipStreamDestroy(st2);
+    if (st3) (void)hipStreamDestroy(st3);
     if (st) (void)hipStreamDestroy(st);
   }
 };
@@ -383,23 +391,31 @@ struct Engine {
   // switched off) everything runs on the main stream in issue order.
   bool concurrent() const { return h.concurrent && h.st2 && !(h.prof_mode & 1); }
   template <class F>
-  void side(F&& fn) {
+  void side(F&& fn, hipStream_t to = nullptr) {
+    if (!to) to = h.st2;
     if (!concurrent()) {
       fn();
       return;
     }
     HIPCHECK(hipEventRecord(h.sync_event(), h.st));
-    HIPCHECK(hipStreamWaitEvent(h.st2, h.last_sync, 0));
-    S = h.st2;
+    HIPCHECK(hipStreamWaitEvent(to, h.last_sync, 0));
+    S = to;
     fn();
     S = h.st;
-    joined = false;
+    if (to == h.st2) joined = false;
   }
   void join() {
     if (joined) return;
     HIPCHECK(hipEventRecord(h.sync_event(), h.st2));
     HIPCHECK(hipStreamWaitEvent(h.st, h.last_sync, 0));
     joined = true;
+  }
+  // orders the main stream after the w passes left running by a step
+  void join_w() {
+    if (!h.w_pending) return;
+    HIPCHECK(hipEventRecord(h.sync_event(), h.st3));
+    HIPCHECK(hipStreamWaitEvent(h.st, h.last_sync, 0));
+    h.w_pending = false;
   }
   void use(const qg::Geo& g) {
     G = &g;
@@ -522,30 +538,43 @@ struct Engine {
   // own half rows, in Pz until y_herm_w reuses it); chk = 2: the non-finite
   // test and, in place of the maximum, its bound (the largest over the levels
   // of sum |psi's Hermitian part| / (nx ny)) into d_absmax.
-  void velocities_w(const C* psi, int chk = 0) {
+  // defer: the w passes run on their own stream, beside whatever the main
+  // stream does next (the next step), until join_w().
+  void velocities_w(const C* psi, int chk = 0, bool defer = false) {
     C* Ppsi = chk == 1 ? c(h.Pz) : nullptr;
     double* lsum = chk == 2 ? h.d_lsum : nullptr;
+    join_w();  // the previous step's w passes read Pw, which xf rewrites
     K("yf_omega", qg::P_YF_OMEGA, Ps * cS + Ph * (chk == 1 ? 7 : 6) * cR, [&] {
       O::yf_omega(h.tune[qg::P_YF_OMEGA], *G, S, psi, c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), c(h.Pu), c(h.Pv),
                   Ppsi, h.d_flag, lsum, twy());
     });
     K("xf", qg::P_XF, Ph * (chk == 1 ? 8 : 7) * cR + P * 2 * rR, [&] {
-      O::xf(h.tune[qg::P_XF], *G, S, 1, c(h.Pu), c(h.Pv), c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), r(h.u), r(h.v), c(h.Pq),
+      O::xf(h.tune[qg::P_XF], *G, S, 1, c(h.Pu), c(h.Pv), c(h.Qa), c(h.Qb), c(h.G1), c(h.G2), r(h.u), r(h.v), c(h.Pw),
             Ppsi, h.d_absmax, lsum, twx());
     });
-    K("y_r2c_omega", qg::P_Y_R2C, Ph * cR + Ps * cS, [&] { O::y_r2c_mask(h.tune[qg::P_Y_R2C], *G, S, c(h.Pq), c(h.rq0), twy()); });
     if (h.sing_omega[0]) {
       char buf[160];
       snprintf(buf, sizeof buf, "LAPACK-style gtsv failed for vertical velocity at row %d of column %d",
                h.sing_omega[1] + 1, h.sing_omega[2]);
       throw QgError(QG_ERR_GTSV, buf);
     }
-    K("omega_gtsv", qg::P_OMEGA, Ps * 2 * cS + Tq, [&] {
-      qg::launch_omega(*G, S, c(h.rq0), c(h.qeff), h.f32(), h.om_M, h.om_D, h.om_piv, h.om_any_piv, h.d_cz, h.d_cu,
-                       h.d_fN2, h.s0, h.s1, h.s2, h.tune[qg::P_OMEGA]);
-    });
-    K("y_herm_w", qg::P_Y_HERM, Ps * cS + Ph * cR, [&] { O::y_herm(h.tune[qg::P_Y_HERM], *G, S, c(h.qeff), c(h.Pz), twy()); });
-    K("x_c2r_w", qg::P_X_C2R, Ph * cR + P * rR, [&] { O::x_c2r(h.tune[qg::P_X_C2R], *G, S, 1, c(h.Pz), nullptr, r(h.w), nullptr, twx()); });
+    // w from the omega right-hand side: buffers of its own (Pw, Wr, Ww, Pw2,
+    // the pivoted solve's scratch), so nothing the next step does touches them
+    auto w_passes = [&] {
+      K("y_r2c_omega", qg::P_Y_R2C, Ph * cR + Ps * cS, [&] { O::y_r2c_mask(h.tune[qg::P_Y_R2C], *G, S, c(h.Pw), c(h.Wr), twy()); });
+      K("omega_gtsv", qg::P_OMEGA, Ps * 2 * cS + Tq, [&] {
+        qg::launch_omega(*G, S, c(h.Wr), c(h.Ww), h.f32(), h.om_M, h.om_D, h.om_piv, h.om_any_piv, h.d_cz, h.d_cu,
+                         h.d_fN2, h.s0, h.s1, h.s2, h.tune[qg::P_OMEGA]);
+      });
+      K("y_herm_w", qg::P_Y_HERM, Ps * cS + Ph * cR, [&] { O::y_herm(h.tune[qg::P_Y_HERM], *G, S, c(h.Ww), c(h.Pw2), twy()); });
+      K("x_c2r_w", qg::P_X_C2R, Ph * cR + P * rR, [&] { O::x_c2r(h.tune[qg::P_X_C2R], *G, S, 1, c(h.Pw2), nullptr, r(h.w), nullptr, twx()); });
+    };
+    if (defer && concurrent()) {
+      side(w_passes, h.st3);
+      h.w_pending = true;
+    } else {
+      w_passes();
+    }
   }
 
   // One RHS evaluation at (qS, psiS, AS, T1, u, v) fused with the ETD update
@@ -657,7 +686,7 @@ struct Engine {
     // which read neither.
     side([&] { wave_inv(B1, c(h.A), c(h.C)); });
     diag(q1, B1, c(h.psi), false, true);
-    velocities_w(c(h.psi), chk);
+    velocities_w(c(h.psi), chk, true);
     join();
     if (can && !prune) {
       // The step's outputs (q, B, psi, A, C) are masked and written at every
@@ -770,13 +799,18 @@ void advance(qg_handle& H, E& e, int nsteps, int chk, Mark&& mark) {
     if (use_graphs && H.diag_valid && (H.clean || !(H.prunable && H.prune_enabled)) && nsteps - i >= 2) {
       const int c = H.cur;
       hipGraphExec_t& gx = chk == 2 ? H.graph_bnd[c] : chk == 1 ? H.graph_chk[c] : H.graph[c];
+      // w passes left running by an eager step finish before the graph's
+      // steps reuse their buffers
+      e.join_w();
       if (!gx) {
         hipGraph_t g = nullptr;
         HIPCHECK(hipStreamBeginCapture(H.st, hipStreamCaptureModeThreadLocal));
         try {
           one();
           one();
+          e.join_w();  // every branch rejoins the captured stream
         } catch (...) {
+          H.w_pending = false;
           (void)hipStreamEndCapture(H.st, &g);
           if (g) (void)hipGraphDestroy(g);
           throw;
@@ -798,6 +832,7 @@ void advance(qg_handle& H, E& e, int nsteps, int chk, Mark&& mark) {
       mark(1);
     }
   }
+  e.join_w();  // the caller may read w or synchronize the main stream
 }
 
 // Steps per qg_run chunk: one host round trip per chunk.
@@ -914,6 +949,7 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
     }
     HIPCHECK(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
     HIPCHECK(hipStreamCreateWithFlags(&h->st2, hipStreamNonBlocking));
+    HIPCHECK(hipStreamCreateWithFlags(&h->st3, hipStreamNonBlocking));
     for (hipEvent_t& e : h->sync_ev) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     const int nx = h->nx, ny = h->ny, nz = h->nz;
     h->plane = (size_t)nx * ny;
@@ -978,7 +1014,10 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
     h->v = h->ralloc(h->npts);
     h->w = h->ralloc(h->npts);
     for (void** p : {&h->T1, &h->T3, &h->G3, &h->G4}) *p = h->ralloc(2 * h->npts);
-    for (void** p : {&h->Qa, &h->Qb, &h->Pu, &h->Pv, &h->Pq, &h->Pz, &h->G1, &h->G2}) *p = h->ralloc(2 * h->nhalf);
+    for (void** p : {&h->Qa, &h->Qb, &h->Pu, &h->Pv, &h->Pq, &h->Pz, &h->G1, &h->G2, &h->Pw, &h->Pw2})
+      *p = h->ralloc(2 * h->nhalf);
+    h->Wr = h->falloc(h->npts);
+    h->Ww = h->falloc(h->npts);
     h->d_err = h->alloc<int>(4);
     h->d_ferr = h->alloc<int>(8);
     h->d_warn = h->alloc<unsigned long long>(1);
@@ -1089,12 +1128,12 @@ int qg_memory_required(const qg_config* cfg, size_t* bytes) {
   const size_t r = cfg->precision == QG_PRECISION_MIXED ? 4 : 8;
   const size_t quarter = (nx / 2 + 1) * (ny / 2 + 1);
   const size_t kpad = (size_t)qg::thomas_shifted_a_size(cfg->nz);
-  size_t b = 14 * 2 * r * npts    // complex fields and stage/work buffers (storage precision)
+  size_t b = 16 * 2 * r * npts    // complex fields and stage/work buffers (storage precision)
              + (r == 4 ? 2 : 1) * 16 * npts  // FP64 staging (the second aliases qeff in FP64 mode)
              + 3 * 8 * npts       // omega scratch
              + 3 * r * npts       // u, v, w
              + 4 * 2 * r * npts   // full-width transform intermediates
-             + 8 * 2 * r * nhalf  // half-width transform intermediates
+             + 10 * 2 * r * nhalf  // half-width transform intermediates
              + 6 * 8 * quarter * nz + quarter * members  // Thomas and omega tables
              + 6 * 8 * plane + plane + 4 * 16 * (nx + ny) + (4 * 8 * nz) + 2 * 8 * members * kpad;
   b += 64 * 256;  // allocation padding

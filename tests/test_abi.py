@@ -87,7 +87,7 @@ def test_memory_required_scales_with_members_and_precision(qg):
     four = lib_mod.memory_required(nx=64, ny=64, nz=32, precision=lib_mod.PRECISION_FP64, members=4)
     mixed = lib_mod.memory_required(nx=64, ny=64, nz=32, precision=lib_mod.PRECISION_MIXED)
     npts = 64 * 64 * 32
-    assert 300 * npts < one < 500 * npts  # ~15 complex FP64 fields plus work buffers
+    assert 300 * npts < one < 600 * npts  # ~18 complex FP64 fields plus work buffers
     assert 3.9 * one < four < 4.1 * one
     assert mixed < one
 
