@@ -76,7 +76,7 @@ typedef struct qg_config {
   const double* ky;         /* [ny] */
   const double* N2;         /* [nz] runtime N^2 profile (OperatorCoefficients.N²) */
   const double* a_ell;      /* [nz] f^2/max(N^2, sqrt(eps)) */
-  int n_flow_terms;
+  int n_flow_terms;          /* at most 8 per closure (QG_ERR_UNSUPPORTED beyond) */
   const double* flow_coefficients;
   const int* flow_orders;
   int n_wave_terms;

@@ -92,6 +92,18 @@ struct Geo {
 
 __host__ __device__ inline int mirror(int i, int n) { return i == 0 ? 0 : n - i; }
 
+// The ETD-RK2 step's parameters in device memory: dt and the closure terms
+// (coefficient, order / 2) of the flow (index 0, for q) and the waves (1, for
+// B), none when inviscid.  Rewritten in place when dt changes, so kernels in
+// a captured graph read the current values.
+constexpr int kEtdTerms = 8;
+struct EtdParams {
+  double dt;
+  int n[2];
+  int half[2][kEtdTerms];
+  double coef[2][kEtdTerms];
+};
+
 // Tiles of S adjacent columns over the full width: every column, or, when
 // pruned, the columns |kx| <= kmax as tiles over [0, S*nlo) and
 // [nx - S*nhi, nx) (the two ranges carried, each rounded up to whole tiles)

@@ -800,28 +800,45 @@ k_xb_all(Geo g, const cplx<R>* __restrict__ Pq, const cplx<R>* __restrict__ Pz, 
     }
 }
 
-// ETD coefficient tables Eq, hphi1q, hphi2q, EB, hphi1B, hphi2B, one after
-// the other.  The coefficients are functions of kx^2 + ky^2, so when the
-// wavenumbers are sign-symmetric (kx[nx - x]^2 == kx[x]^2, likewise ky, as
-// FFT-ordered wavenumbers are) each table covers only the quarter
-// 0 <= x <= nx/2, 0 <= y <= ny/2 ([ny/2 + 1][nx/2 + 1]), and a column reads
-// the entry of (min(x, nx - x), min(y, ny - y)): a quarter of the footprint,
-// which the 128 levels of a field then find in L2.  Otherwise the tables are
-// [ny][nx].
+// ETD-RK2 coefficients of one point, computed where they are used from its
+// kh2 = kx^2 + ky^2 and the model's parameter block (qg_flags.hpp: dt and the
+// closure terms, in device memory, rewritten in place when dt changes):
+// lambda*dt = dt * sum_i c_i kh2^(o_i/2) (int_factor, src/nonlinear.jl:91-97,
+// 1160-1183; zero when inviscid, no terms), then E = exp(-x), h phi_1 and
+// h phi_2 with their series below |x| = 1e-6 (_etd_coefficients,
+// src/timestep.jl:177-190).  `stage`: E and h1 are wanted too (the final
+// update needs h2 only).  Computing them costs a few FP64 operations per
+// point; loading them from per-point tables was a dependent round trip each.
 struct Etd {
-  const double* tab;
-  size_t plane;  // entries of one table
-  int stride;    // row length of a table
-  int nx, ny;
-  bool quarter;
-  __device__ inline int col(int x, int y) const {
-    if (quarter) {
-      x = x <= nx / 2 ? x : nx - x;
-      y = y <= ny / 2 ? y : ny - y;
+  const EtdParams* __restrict__ p;
+  __device__ inline void coefs(int which, double kh2, bool stage, double& E, double& h1, double& h2) const {
+    const double h = p->dt;
+    const int n = p->n[which];
+    double total = 0.0;
+    for (int i = 0; i < n; ++i) {
+      double pw = 1.0;
+      for (int j = 0; j < p->half[which][i]; ++j) pw *= kh2;
+      total = total + p->coef[which][i] * pw;
     }
-    return y * stride + x;
+    const double x = h * total;
+    E = 1.0;
+    h1 = h;
+    if (fabs(x) < 1e-6) {
+      const double x2 = x * x;
+      if (stage) {
+        E = exp(-x);
+        h1 = h * (1 - x / 2 + x2 / 6 - x2 * x / 24 + x2 * x2 / 120);
+      }
+      h2 = h * (1.0 / 2 - x / 6 + x2 / 24 - x2 * x / 120 + x2 * x2 / 720);
+    } else {
+      const double em1 = expm1(-x);
+      if (stage) {
+        E = exp(-x);
+        h1 = h * (-em1) / x;
+      }
+      h2 = h * (em1 + x) / (x * x);
+    }
   }
-  __device__ inline double at(int which, int c) const { return tab[which * plane + c]; }
 };
 
 // Vertical diffusion nu_z * d2q/dz2 with Neumann ends (dissipation_q_nv!, nonlinear.jl:1010-1044).
@@ -904,9 +921,8 @@ k_yb2_q(Geo g, int mode, int flags_rt, const cplx<R>* __restrict__ G1, const cpl
       // The ETD coefficients depend on (kx, ky) through kh2 only, so the
       // mirror column (mx, my) has those of (x, y).
       const bool upd = keep && !(flags & YQ_FIXED);
-      const int ce = etd.col(x, y);
-      const double E = upd ? etd.at(0, ce) : 0.0, h1 = upd ? etd.at(1, ce) : 0.0;
-      const double h2 = upd ? etd.at(2, ce) : 0.0;
+      double E = 0.0, h1 = 0.0, h2 = 0.0;
+      if (upd) etd.coefs(0, kxv * kxv + ky[y] * ky[y], mode == MODE_STAGE, E, h1, h2);
       for (int side = 0; side < 2; ++side) {
         if (side == 1 && mx == x) break;
         const int cx = side ? mx : x, cy = side ? my : y;
@@ -1039,10 +1055,9 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
       // step adds h2 r1 to it
       cplx<double> out = czero<double>();
       if (keep) {
-        const int ce = etd.col(x, y);
-        const double h2 = etd.at(5, ce);
+        double E, h1, h2;
+        etd.coefs(1, kxv * kxv + ky[y] * ky[y], mode == MODE_STAGE, E, h1, h2);
         if (mode == MODE_STAGE) {
-          const double E = etd.at(3, ce), h1 = etd.at(4, ce);
           const cplx<double> b0 = cvt<double>(GROUP ? bv[it][t] : base[ad]);
           out = scale(b0, E) + scale(rhs, h1);
           rhs0[ad] = cvt<R>(scale(b0, E) + scale(rhs, h1 - h2));

@@ -8,13 +8,10 @@
 
 namespace qg {
 
-// ETD coefficient tables (see Etd in qg_kernels.hpp for the layout).
+// The ETD-RK2 parameter block the update passes compute their coefficients
+// from (Etd in qg_kernels.hpp).
 struct EtdView {
-  const double* tab;
-  size_t plane;    // entries of one coefficient's table
-  int stride;      // table row length
-  int nx, ny;      // grid columns and rows
-  bool quarter;    // tables over (|kx|, |ky|) only
+  const EtdParams* p;
 };
 
 bool fft_size_supported(int n);

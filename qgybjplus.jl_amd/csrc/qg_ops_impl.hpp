@@ -165,7 +165,7 @@ template <typename R, int V>
 void OpsV<R, V>::yb2_q(const Geo& g, hipStream_t st, int mode, int flags, const C* G1, const C* G2, const C* qd,
                    const C* qn,
                    C* rhs0, C* qout, EtdView etd, double nu, double inv_dz2, int* flag, const C* twy) {
-  Etd e{etd.tab, etd.plane, etd.stride, etd.nx, etd.ny, etd.quarter};
+  Etd e{etd.p};
   if (flags == YQ_ADV) {  // the evolving flow without vertical diffusion
 #define CALL(N) YL((k_yb2_q<R, N, V, YQ_ADV>), N, g.Kx, g, mode, flags, G1, G2, qd, qn, rhs0, qout, e, nu, inv_dz2, flag, twy)
     QG_SWITCH(g.ny, CALL)
@@ -181,7 +181,7 @@ template <typename R, int V>
 void OpsV<R, V>::yb2_B(const Geo& g, hipStream_t st, int mode, int flags, const C* G3, const C* G4,
                    const C* A, const C* Bn, C* rhs0, C* Bout, C* T1, C* T3, EtdView etd, double alpha,
                    int* flag, const C* twy) {
-  Etd e{etd.tab, etd.plane, etd.stride, etd.nx, etd.ny, etd.quarter};
+  Etd e{etd.p};
 #define CALL(N) YLF((k_yb2_B<R, N, V>), N, g, mode, flags, G3, G4, A, Bn, rhs0, Bout, T1, T3, e, alpha, flag, twy)
   QG_SWITCH(g.ny, CALL)
 #undef CALL

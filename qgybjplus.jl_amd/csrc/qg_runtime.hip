@@ -79,13 +79,8 @@ struct qg_handle {
   double *d_kx = nullptr, *d_ky = nullptr, *d_a = nullptr, *d_al = nullptr, *d_cz = nullptr, *d_cu = nullptr,
          *d_fN2 = nullptr;
   unsigned char* d_mask = nullptr;
-  double* d_etd = nullptr;
+  qg::EtdParams* d_etd = nullptr;  // dt and closure terms of the update passes
   double etd_dt = std::nan("");
-  // ETD table layout (qg::Etd): quarter tables when the wavenumbers are
-  // sign-symmetric, fixed at creation so that captured graphs stay valid
-  bool etd_quarter = false;
-  int etd_stride = 0;
-  size_t etd_plane = 0;
   // Complex fields in the mode's storage precision: cplx<double> in FP64
   // mode, cplx<float> in mixed mode (f32()).  stg and stg2 are FP64 staging
   // for host exchange and plain transforms (stg2 aliases qeff in FP64 mode).
@@ -240,70 +235,24 @@ std::vector<cplx<R>> twiddles(int n) {
   return t;
 }
 
-// int_factor (src/nonlinear.jl:91-97, 1160-1183) and _etd_coefficients
-// (src/timestep.jl:177-190), per (kx, ky) column.
-double component_lambda(double kh2, double dt, const std::vector<double>& c, const std::vector<int>& o) {
-  double total = 0.0;
-  for (size_t i = 0; i < c.size(); ++i) {
-    double p = (double)powl((long double)kh2, (long double)(o[i] / 2));
-    total = total + c[i] * p;
-  }
-  return dt * total;
-}
-
-void etd_coefficients(double x, double h, double& E, double& h1, double& h2) {
-  E = std::exp(-x);
-  if (std::fabs(x) < 1e-6) {
-    double x2 = x * x;
-    h1 = h * (1 - x / 2 + x2 / 6 - x2 * x / 24 + x2 * x2 / 120);
-    h2 = h * (1.0 / 2 - x / 6 + x2 / 24 - x2 * x / 120 + x2 * x2 / 720);
-    return;
-  }
-  double em1 = std::expm1(-x);
-  h1 = h * (-em1) / x;
-  h2 = h * (em1 + x) / (x * x);
-}
-
-// The ETD tables' layout: a quarter of the plane when kx^2 and ky^2 are
-// symmetric under x -> nx - x, y -> ny - y bit for bit (then every column's
-// coefficients equal those of its quarter-plane representative), else the
-// whole plane.
-void etd_layout(qg_handle& h) {
-  auto symmetric = [](const std::vector<double>& k) {
-    const int n = (int)k.size();
-    for (int i = 1; i < n; ++i)
-      if (k[i] * k[i] != k[n - i] * k[n - i]) return false;
-    return true;
-  };
-  h.etd_quarter = symmetric(h.kx) && symmetric(h.ky);
-  h.etd_stride = h.etd_quarter ? h.nx / 2 + 1 : h.nx;
-  h.etd_plane = (size_t)h.etd_stride * (h.etd_quarter ? h.ny / 2 + 1 : h.ny);
-}
-
+// The parameters the update passes compute int_factor (src/nonlinear.jl:
+// 91-97, 1160-1183) and _etd_coefficients (src/timestep.jl:177-190) from, per
+// point (qg::Etd): dt and the closure terms, written in place (graphs
+// captured earlier read the new values).
 void build_etd(qg_handle& h, double dt) {
   if (h.etd_dt == dt) return;
-  const int tx = h.etd_quarter ? h.nx / 2 + 1 : h.nx, ty = h.etd_quarter ? h.ny / 2 + 1 : h.ny;
-  std::vector<double> tab(6 * h.etd_plane);
-  for (int y = 0; y < ty; ++y)
-    for (int x = 0; x < tx; ++x) {
-      size_t col = (size_t)y * h.etd_stride + x;
-      double kh2 = h.kx[x] * h.kx[x] + h.ky[y] * h.ky[y];
-      double lq = 0.0, lB = 0.0;
-      if (!h.inviscid) {
-        lq = h.fc.empty() ? 0.0 : component_lambda(kh2, dt, h.fc, h.fo);
-        lB = h.wc.empty() ? 0.0 : component_lambda(kh2, dt, h.wc, h.wo);
-      }
-      double e, a1, a2;
-      etd_coefficients(lq, dt, e, a1, a2);
-      tab[0 * h.etd_plane + col] = e;
-      tab[1 * h.etd_plane + col] = a1;
-      tab[2 * h.etd_plane + col] = a2;
-      etd_coefficients(lB, dt, e, a1, a2);
-      tab[3 * h.etd_plane + col] = e;
-      tab[4 * h.etd_plane + col] = a1;
-      tab[5 * h.etd_plane + col] = a2;
+  qg::EtdParams P{};
+  P.dt = dt;
+  const std::vector<double>* c[2] = {&h.fc, &h.wc};
+  const std::vector<int>* o[2] = {&h.fo, &h.wo};
+  for (int w = 0; w < 2; ++w) {
+    P.n[w] = h.inviscid ? 0 : (int)c[w]->size();
+    for (int i = 0; i < P.n[w]; ++i) {
+      P.coef[w][i] = (*c[w])[i];
+      P.half[w][i] = (*o[w])[i] / 2;
     }
-  HIPCHECK(hipMemcpyAsync(h.d_etd, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, h.st));
+  }
+  HIPCHECK(hipMemcpyAsync(h.d_etd, &P, sizeof P, hipMemcpyHostToDevice, h.st));
   HIPCHECK(hipStreamSynchronize(h.st));
   h.etd_dt = dt;
 }
@@ -437,7 +386,7 @@ struct Engine {
   R* r(void* p) const { return static_cast<R*>(p); }
   const C* twx() const { return static_cast<const C*>(h.twx); }
   const C* twy() const { return static_cast<const C*>(h.twy); }
-  qg::EtdView etd() const { return qg::EtdView{h.d_etd, h.etd_plane, h.etd_stride, h.nx, h.ny, h.etd_quarter}; }
+  qg::EtdView etd() const { return qg::EtdView{h.d_etd}; }
 
   template <class F>
   void K(const char* name, int pass, double bytes, F&& fn) {
@@ -911,6 +860,9 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
     const bool feedback = cfg->wave_feedback && !cfg->fixed_flow;
     if (feedback && !(std::isfinite(cfg->f) && cfg->f != 0))
       throw QgError(QG_ERR_ARGUMENT, "wave feedback requires a finite, nonzero Coriolis frequency");
+    if (cfg->n_flow_terms > qg::kEtdTerms || cfg->n_wave_terms > qg::kEtdTerms)
+      throw QgError(QG_ERR_UNSUPPORTED, "at most " + std::to_string(qg::kEtdTerms) +
+                                            " hyperdiffusivity terms per closure are supported");
     for (int i = 0; i < cfg->n_flow_terms; ++i)
       if (cfg->flow_orders[i] <= 0 || cfg->flow_orders[i] % 2)
         throw QgError(QG_ERR_ARGUMENT, "flow hyperdiffusivity orders must be positive and even");
@@ -965,8 +917,7 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
     h->d_fN2 = h->alloc<double>(nz);
     h->d_prof = h->alloc<double>(nz);
     h->d_mask = h->alloc<unsigned char>(h->plane);
-    etd_layout(*h);
-    h->d_etd = h->alloc<double>(6 * h->etd_plane);
+    h->d_etd = h->alloc<qg::EtdParams>(1);
     std::vector<unsigned char> mask(h->plane);
     for (int y = 0; y < ny; ++y)
       for (int x = 0; x < nx; ++x) mask[(size_t)y * nx + x] = is_dealiased(x, y, nx, ny) ? 1 : 0;
@@ -1135,7 +1086,7 @@ int qg_memory_required(const qg_config* cfg, size_t* bytes) {
              + 4 * 2 * r * npts   // full-width transform intermediates
              + 10 * 2 * r * nhalf  // half-width transform intermediates
              + 6 * 8 * quarter * nz + quarter * members  // Thomas and omega tables
-             + 6 * 8 * plane + plane + 4 * 16 * (nx + ny) + (4 * 8 * nz) + 2 * 8 * members * kpad;
+             + plane + 4 * 16 * (nx + ny) + (4 * 8 * nz) + 2 * 8 * members * kpad + sizeof(qg::EtdParams);
   b += 64 * 256;  // allocation padding
   *bytes = b;
   return QG_OK;
