@@ -50,7 +50,7 @@ def test_graph_replays_match_eager_steps(qg, precision):
         qg.set_(m, psi=qg.FieldArray(psi), waves=qg.FieldArray(B))
     ts = qg.ExponentialRungeKutta2(dt=600.0)
     # pairs replay the graph, odd counts end with an eager step, a new dt
-    # rewrites the ETD tables in place, set_ invalidates the diagnostics
+    # rewrites the ETD parameter block in place, set_ invalidates the diagnostics
     plan = [2, 2, 3, 4, 1, ("dt", 300.0), 4, ("set", 5), 6, 2]
     for item in plan:
         if isinstance(item, tuple) and item[0] == "dt":
