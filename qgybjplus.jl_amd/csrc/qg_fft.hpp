@@ -176,7 +176,45 @@ template <int DIR> struct Dft<16, DIR> : DftCT<4, 4, DIR> {};
 // input and of the output coincide) and lanes per sequence T.
 // ---------------------------------------------------------------------------
 template <int N, int PL = 0> struct FftPlan;
+template <> struct FftPlan<1>    { static constexpr int S = 1, r0 = 1, r1 = 1, T = 1; };
+template <> struct FftPlan<2>    { static constexpr int S = 1, r0 = 2, r1 = 1, T = 1; };
+template <> struct FftPlan<3>    { static constexpr int S = 1, r0 = 3, r1 = 1, T = 1; };
+template <> struct FftPlan<4>    { static constexpr int S = 1, r0 = 4, r1 = 1, T = 1; };
+template <> struct FftPlan<5>    { static constexpr int S = 1, r0 = 5, r1 = 1, T = 1; };
+template <> struct FftPlan<6>    { static constexpr int S = 1, r0 = 6, r1 = 1, T = 1; };
 template <> struct FftPlan<8>    { static constexpr int S = 1, r0 = 8, r1 = 1, T = 1; };
+template <> struct FftPlan<9>    { static constexpr int S = 2, r0 = 3, r1 = 1, T = 3; };
+template <> struct FftPlan<18>   { static constexpr int S = 3, r0 = 3, r1 = 2, T = 6; };
+template <> struct FftPlan<25>   { static constexpr int S = 2, r0 = 5, r1 = 1, T = 5; };
+template <> struct FftPlan<27>   { static constexpr int S = 3, r0 = 3, r1 = 3, T = 9; };
+template <> struct FftPlan<36>   { static constexpr int S = 2, r0 = 6, r1 = 1, T = 6; };
+template <> struct FftPlan<45>   { static constexpr int S = 3, r0 = 3, r1 = 5, T = 15; };
+template <> struct FftPlan<50>   { static constexpr int S = 3, r0 = 5, r1 = 2, T = 10; };
+template <> struct FftPlan<54>   { static constexpr int S = 3, r0 = 3, r1 = 6, T = 18; };
+template <> struct FftPlan<72>   { static constexpr int S = 3, r0 = 6, r1 = 2, T = 12; };
+template <> struct FftPlan<75>   { static constexpr int S = 3, r0 = 5, r1 = 3, T = 15; };
+template <> struct FftPlan<90>   { static constexpr int S = 3, r0 = 3, r1 = 10, T = 30; };
+template <> struct FftPlan<100>  { static constexpr int S = 2, r0 = 10, r1 = 1, T = 10; };
+template <> struct FftPlan<108>  { static constexpr int S = 3, r0 = 6, r1 = 3, T = 18; };
+template <> struct FftPlan<125>  { static constexpr int S = 3, r0 = 5, r1 = 5, T = 25; };
+template <> struct FftPlan<144>  { static constexpr int S = 2, r0 = 12, r1 = 1, T = 12; };
+template <> struct FftPlan<150>  { static constexpr int S = 3, r0 = 5, r1 = 6, T = 30; };
+template <> struct FftPlan<180>  { static constexpr int S = 3, r0 = 6, r1 = 5, T = 30; };
+template <> struct FftPlan<200>  { static constexpr int S = 3, r0 = 10, r1 = 2, T = 20; };
+template <> struct FftPlan<216>  { static constexpr int S = 3, r0 = 6, r1 = 6, T = 36; };
+template <> struct FftPlan<250>  { static constexpr int S = 3, r0 = 5, r1 = 10, T = 50; };
+template <> struct FftPlan<288>  { static constexpr int S = 3, r0 = 12, r1 = 2, T = 24; };
+template <> struct FftPlan<300>  { static constexpr int S = 3, r0 = 10, r1 = 3, T = 30; };
+template <> struct FftPlan<360>  { static constexpr int S = 3, r0 = 6, r1 = 10, T = 60; };
+template <> struct FftPlan<400>  { static constexpr int S = 3, r0 = 10, r1 = 4, T = 40; };
+template <> struct FftPlan<432>  { static constexpr int S = 3, r0 = 12, r1 = 3, T = 36; };
+template <> struct FftPlan<500>  { static constexpr int S = 3, r0 = 10, r1 = 5, T = 50; };
+template <> struct FftPlan<576>  { static constexpr int S = 3, r0 = 12, r1 = 4, T = 48; };
+template <> struct FftPlan<600>  { static constexpr int S = 3, r0 = 10, r1 = 6, T = 60; };
+template <> struct FftPlan<720>  { static constexpr int S = 3, r0 = 12, r1 = 5, T = 60; };
+template <> struct FftPlan<800>  { static constexpr int S = 3, r0 = 10, r1 = 8, T = 80; };
+template <> struct FftPlan<864>  { static constexpr int S = 3, r0 = 12, r1 = 6, T = 72; };
+template <> struct FftPlan<1000> { static constexpr int S = 3, r0 = 10, r1 = 10, T = 100; };
 template <> struct FftPlan<10>   { static constexpr int S = 1, r0 = 10, r1 = 1, T = 1; };
 template <> struct FftPlan<12>   { static constexpr int S = 1, r0 = 12, r1 = 1, T = 1; };
 template <> struct FftPlan<16>   { static constexpr int S = 2, r0 = 4, r1 = 1, T = 4; };

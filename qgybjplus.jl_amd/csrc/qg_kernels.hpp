@@ -391,7 +391,7 @@ k_yb_adv(Geo g, int outs_rt, const cplx<R>* __restrict__ psi, const cplx<R>* __r
         // its anti-Hermitian part where the multiplier's wavenumber is its
         // own mirror (ky at the row ny/2, kx at the column nx/2)
         if (o == 0) {
-          const cplx<R> f = y == NY / 2 ? aherm(p[it][t], pm[it][t], (R)g.inv_n)
+          const cplx<R> f = 2 * y == NY ? aherm(p[it][t], pm[it][t], (R)g.inv_n)
                                         : herm(p[it][t], pm[it][t], (R)g.inv_n);
           v[it][t] = ik_mul((R)-ky[y], f);
           continue;
@@ -1136,7 +1136,7 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
 // multiplier times the anti-Hermitian part.  The pass keeps only the two
 // parts' sources, psi(z) and psi(z+1) (the difference and the average follow
 // by linearity), as Hermitian parts, plus their anti-Hermitian parts at the
-// row ny/2 (register slot it = 0, t = R0/2 of lane 0), and forms, transforms
+// row ny/2 (one register slot of one lane, for even ny), and forms, transforms
 // and stores one output at a time.  The kx outputs of the Nyquist column
 // (x = nx/2) come from one extra tile column of blocks (blk.x == the number
 // of tiles), whose sources are the anti-Hermitian parts of that column.
@@ -1165,7 +1165,6 @@ k_yf_omega(Geo g, const cplx<R>* __restrict__ psi, cplx<R>* __restrict__ Pbx, cp
   const bool top = z % g.nzc == g.nzc - 1;  // top level of this member's column
   const int zu = top ? z : z + 1;
   const R rdz = (R)g.dz, rinv = (R)g.inv_n;
-  constexpr int TN = F::R0 / 2;  // register slot of the row ny/2 (it = 0, lane 0)
   // psi(z), psi(zu) / (nx ny): Hermitian parts (anti-Hermitian in the
   // Nyquist blocks), and the anti-Hermitian parts at the row ny/2
   cplx<R> h0[F::IT][F::R0], h1[F::IT][F::R0];
@@ -1189,7 +1188,7 @@ k_yf_omega(Geo g, const cplx<R>* __restrict__ psi, cplx<R>* __restrict__ Pbx, cp
       h0[it][t] = nyq ? aherm(p, pm, rinv) : herm(p, pm, rinv);
       h1[it][t] = nyq ? aherm(up, upm, rinv) : herm(up, upm, rinv);
       if (CHK == 2 && in) bsum += wx * ((double)fabs(h0[it][t].re) + (double)fabs(h0[it][t].im));
-      if (it == 0 && t == TN) {
+      if (2 * y == NY) {  // the row ny/2 (even ny), held by one lane
         a0 = aherm(p, pm, rinv);
         a1 = aherm(up, upm, rinv);
       }
@@ -1222,7 +1221,7 @@ k_yf_omega(Geo g, const cplx<R>* __restrict__ psi, cplx<R>* __restrict__ Pbx, cp
         const double ky = ky_o[y];
         const double kh2 = kxv * kxv + ky * ky;
         cplx<R> f1 = h1[it][t], f0 = h0[it][t];
-        if (!kxo && o != 6 && it == 0 && t == TN && lane == 0) {  // ky at the row ny/2
+        if (!kxo && o != 6 && 2 * y == NY) {  // ky at the row ny/2
           f1 = a1;
           f0 = a0;
         }

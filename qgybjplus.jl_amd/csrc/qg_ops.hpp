@@ -3,6 +3,8 @@
 // mixed mode); Ops<R> picks the variant at run time.
 #pragma once
 
+#include <string>
+
 #include "qg_common.hpp"
 #include "qg_flags.hpp"
 
@@ -15,6 +17,8 @@ struct EtdView {
 };
 
 bool fft_size_supported(int n);
+// the supported lengths, comma-separated (for error messages)
+std::string fft_sizes_text();
 // lengths compiled for every tile variant (the others run the base variant)
 bool fft_variants_supported(int n);
 

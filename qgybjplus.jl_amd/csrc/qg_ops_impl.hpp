@@ -32,7 +32,45 @@ namespace qg {
     case 320: CALL(320); break;                                               \
     case 640: CALL(640); break;                                               \
     case 768: CALL(768); break;                                               \
-    case 1024: CALL(1024); break;
+    case 1024: CALL(1024); break;                                             \
+    case 1: CALL(1); break;                                                   \
+    case 2: CALL(2); break;                                                   \
+    case 3: CALL(3); break;                                                   \
+    case 4: CALL(4); break;                                                   \
+    case 5: CALL(5); break;                                                   \
+    case 6: CALL(6); break;                                                   \
+    case 9: CALL(9); break;                                                   \
+    case 18: CALL(18); break;                                                 \
+    case 25: CALL(25); break;                                                 \
+    case 27: CALL(27); break;                                                 \
+    case 36: CALL(36); break;                                                 \
+    case 45: CALL(45); break;                                                 \
+    case 50: CALL(50); break;                                                 \
+    case 54: CALL(54); break;                                                 \
+    case 72: CALL(72); break;                                                 \
+    case 75: CALL(75); break;                                                 \
+    case 90: CALL(90); break;                                                 \
+    case 100: CALL(100); break;                                               \
+    case 108: CALL(108); break;                                               \
+    case 125: CALL(125); break;                                               \
+    case 144: CALL(144); break;                                               \
+    case 150: CALL(150); break;                                               \
+    case 180: CALL(180); break;                                               \
+    case 200: CALL(200); break;                                               \
+    case 216: CALL(216); break;                                               \
+    case 250: CALL(250); break;                                               \
+    case 288: CALL(288); break;                                               \
+    case 300: CALL(300); break;                                               \
+    case 360: CALL(360); break;                                               \
+    case 400: CALL(400); break;                                               \
+    case 432: CALL(432); break;                                               \
+    case 500: CALL(500); break;                                               \
+    case 576: CALL(576); break;                                               \
+    case 600: CALL(600); break;                                               \
+    case 720: CALL(720); break;                                               \
+    case 800: CALL(800); break;                                               \
+    case 864: CALL(864); break;                                               \
+    case 1000: CALL(1000); break;
 #else
 #define QG_MORE_LENGTHS(CALL)
 #endif
@@ -191,23 +229,23 @@ template <typename R, int V>
 void OpsV<R, V>::yf_omega(const Geo& g, hipStream_t st, const C* psi, C* Pbx, C* Pby, C* Pxx, C* Pxy, C* Pu,
                       C* Pv, C* Ppsi, int* flag, double* lsum, const C* twy) {
   // one tile column more than the half width: the Nyquist column's kx outputs
-  // (none when pruned: the column is not carried)
+  // (none when pruned, the column is not carried, or for odd nx, which has none)
   if ((Ppsi || lsum) && !Pu) throw std::runtime_error("yf_omega: the check needs the velocity outputs");
   if (Ppsi && lsum) throw std::runtime_error("yf_omega: the exact check or its bound, not both");
   if (Pu && lsum) {
-#define CALL(N) YL((k_yf_omega<R, N, V, 1, 2>), N, (g.Kx + (g.pruned() ? 0 : Tile<R, N, V>::S)), g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, Ppsi, flag, lsum, twy)
+#define CALL(N) YL((k_yf_omega<R, N, V, 1, 2>), N, (g.Kx + (g.pruned() || (g.nx & 1) ? 0 : Tile<R, N, V>::S)), g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, Ppsi, flag, lsum, twy)
     QG_SWITCH(g.ny, CALL)
 #undef CALL
   } else if (Pu && Ppsi) {
-#define CALL(N) YL((k_yf_omega<R, N, V, 1, 1>), N, (g.Kx + (g.pruned() ? 0 : Tile<R, N, V>::S)), g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, Ppsi, flag, lsum, twy)
+#define CALL(N) YL((k_yf_omega<R, N, V, 1, 1>), N, (g.Kx + (g.pruned() || (g.nx & 1) ? 0 : Tile<R, N, V>::S)), g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, Ppsi, flag, lsum, twy)
     QG_SWITCH(g.ny, CALL)
 #undef CALL
   } else if (Pu) {
-#define CALL(N) YL((k_yf_omega<R, N, V, 1, 0>), N, (g.Kx + (g.pruned() ? 0 : Tile<R, N, V>::S)), g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, Ppsi, flag, lsum, twy)
+#define CALL(N) YL((k_yf_omega<R, N, V, 1, 0>), N, (g.Kx + (g.pruned() || (g.nx & 1) ? 0 : Tile<R, N, V>::S)), g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, Ppsi, flag, lsum, twy)
     QG_SWITCH(g.ny, CALL)
 #undef CALL
   } else {
-#define CALL(N) YL((k_yf_omega<R, N, V, 0, 0>), N, (g.Kx + (g.pruned() ? 0 : Tile<R, N, V>::S)), g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, Ppsi, flag, lsum, twy)
+#define CALL(N) YL((k_yf_omega<R, N, V, 0, 0>), N, (g.Kx + (g.pruned() || (g.nx & 1) ? 0 : Tile<R, N, V>::S)), g, psi, Pbx, Pby, Pxx, Pxy, Pu, Pv, Ppsi, flag, lsum, twy)
     QG_SWITCH(g.ny, CALL)
 #undef CALL
   }

@@ -838,8 +838,7 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
       throw QgError(QG_ERR_ARGUMENT, "grid dimensions must be positive");
     if (!qg::fft_size_supported(cfg->nx) || !qg::fft_size_supported(cfg->ny))
       throw QgError(QG_ERR_UNSUPPORTED,
-                    "horizontal sizes must be one of 8,10,12,16,20,24,32,40,48,64,80,96,128,160,192,256,320,"
-                    "384,512,640,768,1024 (got " +
+                    "horizontal sizes must be one of " + qg::fft_sizes_text() + " (got " +
                         std::to_string(cfg->nx) + "x" + std::to_string(cfg->ny) + ")");
     if (!qg::column_depth_supported(cfg->nz))
       throw QgError(QG_ERR_UNSUPPORTED, "nz must be between 1 and 1024 for the column solvers");

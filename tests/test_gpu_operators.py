@@ -19,7 +19,13 @@ EXP_N2 = lambda z: 1e-4 * math.exp(z / 800.0)  # noqa: E731
                                   # the lengths compiled for the base tile variant only,
                                   # radix-5 plans among them
                                   (10, 12, 3), (20, 24, 2), (40, 48, 2), (80, 96, 2), (160, 192, 1),
-                                  (320, 640, 1), (768, 1024, 1)])
+                                  (320, 640, 1), (768, 1024, 1),
+                                  # single-DFT lengths 1-6 (the reference's own test grids),
+                                  # odd lengths and the other 2-3-5 plans
+                                  (1, 2, 2), (3, 4, 2), (5, 6, 2), (9, 18, 2), (25, 27, 2), (36, 45, 2),
+                                  (50, 54, 2), (72, 75, 2), (90, 100, 2), (108, 125, 2), (144, 150, 1),
+                                  (180, 200, 1), (216, 250, 1), (288, 300, 1), (360, 400, 1), (432, 500, 1),
+                                  (576, 600, 1), (720, 800, 1), (864, 1000, 1)])
 def test_fft_matches_dft_and_round_trips(qg, size):
     """KAT test/test_mpi_extension.jl:62-90 (round trip < 1e-12) and the
     transform contract of src/transforms.jl:19-23 against numpy's DFT."""

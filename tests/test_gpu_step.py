@@ -221,6 +221,42 @@ def test_other_plans_step_matches_oracle(qg, size):
     m.finalize()
 
 
+@pytest.mark.parametrize("size", [(4, 4, 2), (8, 6, 4), (2, 2, 2), (3, 1, 2), (64, 48, 24), (5, 9, 3),
+                                  (25, 27, 4), (36, 50, 6), (45, 54, 3), (75, 72, 4), (100, 90, 3),
+                                  (125, 150, 2), (200, 216, 2)])
+def test_more_sizes_step_matches_oracle(qg, size):
+    """Grids of the reference's own tests ((4, 4, 2), (8, 6, 4), (2, 2, 2),
+    (3, 1, 2), (64, 48, 24): test/test_core_components.jl and others), odd
+    lengths and the other radix-2-3-5 plans: 3 steps against the oracle."""
+    c = Case(size=size, extent=(500e3, 500e3, 4000.0), N2_func=EXP_N2, feedback=True)
+    print(f"case step {size}", flush=True)
+    m, o = gpu_model(qg, c), oracle_model(c)
+    nx, ny, nz = size
+    rng = np.random.default_rng(sum(size))
+    # random smooth-ish fields with content at every wavenumber (inside and
+    # outside the 2/3 disk)
+    psi = 1e3 * rng.normal(size=(nz, nx, ny))
+    X, Y = np.meshgrid(o.grid.x, o.grid.y, indexing="ij")
+    B = np.empty(psi.shape, complex)
+    for k in range(nz):
+        B[k] = 0.1 * math.exp(o.grid.z[k] / 300.0) * np.exp(1j * 2 * math.pi * (X / o.grid.Lx + Y / o.grid.Ly))
+        B[k] += 0.01 * (rng.normal(size=(nx, ny)) + 1j * rng.normal(size=(nx, ny)))
+    set_both(m, o, psi, B, qg)
+    qg.step_(m, qg.ExponentialRungeKutta2(dt=60.0), 3)
+    for _ in range(3):
+        o.step(60.0)
+    for n in FIELDS + ("C", "u", "v", "w"):
+        a, b = getattr(m.fields, n), getattr(o.fields, n)
+        if n == "w":  # where w is round-off (a single wavenumber shell), against the flow's scale
+            d = float(np.linalg.norm(np.ravel(a - b)))
+            e = d / max(float(np.linalg.norm(np.ravel(b))), 1e-6 * float(np.linalg.norm(o.fields.u)), 1e-300)
+        else:
+            e = rel(a, b)
+        print(f"  {size} {n}: {e:.2e}", flush=True)
+        assert e <= (1e-9 if n == "w" else 1e-11), (n, e)
+    m.finalize()
+
+
 def test_asymmetric_wavenumbers_step_matches_oracle(qg):
     """Wavenumbers need not be sign-symmetric: here one kx is off by a few
     ulp, so kx^2 differs from its mirror's.  With a hyperdiffusion strong
