@@ -21,7 +21,7 @@
 namespace qg {
 
 // cos(2*pi*m/240) for m in [0, 60], correctly rounded; every radix used here
-// (2, 3, 4, 5, 6, 8, 10, 12, 16) divides 240.
+// (2, 3, 4, 5, 6, 8, 10, 12, 15, 16, 30) divides 240.
 constexpr double kCos240[61] = {
     1.0, 0.9996573249755573, 0.9986295347545738, 0.996917333733128,
     0.9945218953682733, 0.9914448613738104, 0.9876883405951378, 0.9832549075639546,
@@ -170,6 +170,8 @@ template <int DIR> struct Dft<10, DIR> : DftCT<2, 5, DIR> {};
 template <int DIR> struct Dft<8, DIR> : DftCT<2, 4, DIR> {};
 template <int DIR> struct Dft<12, DIR> : DftCT<4, 3, DIR> {};
 template <int DIR> struct Dft<16, DIR> : DftCT<4, 4, DIR> {};
+template <int DIR> struct Dft<15, DIR> : DftCT<3, 5, DIR> {};
+template <int DIR> struct Dft<30, DIR> : DftCT<2, 15, DIR> {};
 
 // ---------------------------------------------------------------------------
 // Plans: radices per stage (first == last so the register distribution of the
@@ -215,6 +217,20 @@ template <> struct FftPlan<720>  { static constexpr int S = 3, r0 = 12, r1 = 5, 
 template <> struct FftPlan<800>  { static constexpr int S = 3, r0 = 10, r1 = 8, T = 80; };
 template <> struct FftPlan<864>  { static constexpr int S = 3, r0 = 12, r1 = 6, T = 72; };
 template <> struct FftPlan<1000> { static constexpr int S = 3, r0 = 10, r1 = 10, T = 100; };
+// lengths whose plan needs a radix of 15 or 30 (a composite in-register DFT)
+template <> struct FftPlan<15>   { static constexpr int S = 1, r0 = 15, r1 = 1, T = 1; };
+template <> struct FftPlan<30>   { static constexpr int S = 1, r0 = 30, r1 = 1, T = 1; };
+template <> struct FftPlan<60>   { static constexpr int S = 3, r0 = 2, r1 = 15, T = 30; };
+template <> struct FftPlan<120>  { static constexpr int S = 3, r0 = 2, r1 = 30, T = 60; };
+template <> struct FftPlan<135>  { static constexpr int S = 3, r0 = 3, r1 = 15, T = 45; };
+template <> struct FftPlan<225>  { static constexpr int S = 2, r0 = 15, r1 = 1, T = 15; };
+template <> struct FftPlan<240>  { static constexpr int S = 3, r0 = 4, r1 = 15, T = 60; };
+template <> struct FftPlan<375>  { static constexpr int S = 3, r0 = 5, r1 = 15, T = 75; };
+template <> struct FftPlan<480>  { static constexpr int S = 3, r0 = 4, r1 = 30, T = 60; };
+template <> struct FftPlan<540>  { static constexpr int S = 3, r0 = 6, r1 = 15, T = 90; };
+template <> struct FftPlan<750>  { static constexpr int S = 3, r0 = 5, r1 = 30, T = 75; };
+template <> struct FftPlan<900>  { static constexpr int S = 2, r0 = 30, r1 = 1, T = 30; };
+template <> struct FftPlan<960>  { static constexpr int S = 3, r0 = 8, r1 = 15, T = 120; };
 template <> struct FftPlan<10>   { static constexpr int S = 1, r0 = 10, r1 = 1, T = 1; };
 template <> struct FftPlan<12>   { static constexpr int S = 1, r0 = 12, r1 = 1, T = 1; };
 template <> struct FftPlan<16>   { static constexpr int S = 2, r0 = 4, r1 = 1, T = 4; };

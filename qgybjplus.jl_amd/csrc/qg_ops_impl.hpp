@@ -9,6 +9,14 @@
 #include "qg_kernels.hpp"
 #include "qg_ops.hpp"
 
+// A translation unit that defines QG_PART = k defines only part k of the
+// members below (explicit instantiation of OpsV then instantiates those), so
+// the base variant's many lengths compile in parallel units.
+#ifndef QG_PART
+#define QG_PART -1
+#endif
+#define QG_PART_ON(k) (QG_PART < 0 || QG_PART == (k))
+
 namespace qg {
 
 // The transform lengths compiled.  Every tile variant has the lengths whose
@@ -70,7 +78,20 @@ namespace qg {
     case 720: CALL(720); break;                                               \
     case 800: CALL(800); break;                                               \
     case 864: CALL(864); break;                                               \
-    case 1000: CALL(1000); break;
+    case 1000: CALL(1000); break;                                             \
+    case 15: CALL(15); break;                                                 \
+    case 30: CALL(30); break;                                                 \
+    case 60: CALL(60); break;                                                 \
+    case 120: CALL(120); break;                                               \
+    case 135: CALL(135); break;                                               \
+    case 225: CALL(225); break;                                               \
+    case 240: CALL(240); break;                                               \
+    case 375: CALL(375); break;                                               \
+    case 480: CALL(480); break;                                               \
+    case 540: CALL(540); break;                                               \
+    case 750: CALL(750); break;                                               \
+    case 900: CALL(900); break;                                               \
+    case 960: CALL(960); break;
 #else
 #define QG_MORE_LENGTHS(CALL)
 #endif
@@ -111,20 +132,25 @@ static inline dim3 xgrid(int rows, int S) { return dim3((rows + S - 1) / S); }
     hipLaunchKernelGGL(K, xgrid(g.nz * g.ny, TL::S), dim3(TL::THREADS), 0, st, __VA_ARGS__); \
   } while (0)
 
+#if QG_PART_ON(0)
 template <typename R, int V>
 void OpsV<R, V>::ya_qw(const Geo& g, hipStream_t st, const C* B, C* T1, C* T3, const C* twy) {
 #define CALL(N) YLF((k_ya_qw<R, N, V>), N, g, B, T1, T3, twy)
   QG_SWITCH(g.ny, CALL)
 #undef CALL
 }
+#endif
 
+#if QG_PART_ON(2)
 template <typename R, int V>
 void OpsV<R, V>::xa_qw(const Geo& g, hipStream_t st, const C* T1, const C* T3, C* Qa, C* Qb, const C* twx) {
 #define CALL(N) XL((k_xa_qw<R, N, V>), N, g, T1, T3, Qa, Qb, twx)
   QG_SWITCH(g.nx, CALL)
 #undef CALL
 }
+#endif
 
+#if QG_PART_ON(0)
 template <typename R, int V>
 void OpsV<R, V>::ya2_qeff(const Geo& g, hipStream_t st, const C* Qa, const C* Qb, const C* q, C* qeff, C* qw,
                       double inv_f, const C* twy) {
@@ -132,7 +158,9 @@ void OpsV<R, V>::ya2_qeff(const Geo& g, hipStream_t st, const C* Qa, const C* Qb
   QG_SWITCH(g.ny, CALL)
 #undef CALL
 }
+#endif
 
+#if QG_PART_ON(0)
 template <typename R, int V>
 void OpsV<R, V>::yb_adv(const Geo& g, hipStream_t st, int outs, const C* psi, const C* q, C* Pu, C* Pv, C* Pq,
                     C* Pz, const C* twy) {
@@ -154,7 +182,9 @@ void OpsV<R, V>::yb_adv(const Geo& g, hipStream_t st, int outs, const C* psi, co
 #undef CALL
   }
 }
+#endif
 
+#if QG_PART_ON(2)
 template <typename R, int V>
 void OpsV<R, V>::x_c2r(const Geo& g, hipStream_t st, int nfields, const C* P0, const C* P1, R* o0, R* o1,
                    const C* twx) {
@@ -162,7 +192,9 @@ void OpsV<R, V>::x_c2r(const Geo& g, hipStream_t st, int nfields, const C* P0, c
   QG_SWITCH(g.nx, CALL)
 #undef CALL
 }
+#endif
 
+#if QG_PART_ON(2)
 template <typename R, int V>
 void OpsV<R, V>::xb_q(const Geo& g, hipStream_t st, const C* Pq, const C* Pu, const C* Pv, R* u, R* v, C* G1,
                   C* G2, const C* twx) {
@@ -176,7 +208,9 @@ void OpsV<R, V>::xb_q(const Geo& g, hipStream_t st, const C* Pq, const C* Pu, co
 #undef CALL
   }
 }
+#endif
 
+#if QG_PART_ON(2)
 template <typename R, int V>
 void OpsV<R, V>::xb_B(const Geo& g, hipStream_t st, int flags, const C* T1, const C* Pz, const R* u, const R* v,
                   C* G3, C* G4, const C* twx) {
@@ -184,7 +218,9 @@ void OpsV<R, V>::xb_B(const Geo& g, hipStream_t st, int flags, const C* T1, cons
   QG_SWITCH(g.nx, CALL)
 #undef CALL
 }
+#endif
 
+#if QG_PART_ON(2)
 template <typename R, int V>
 void OpsV<R, V>::xb_all(const Geo& g, hipStream_t st, const C* Pq, const C* Pz, const C* T1, const C* Pu,
                         const C* Pv, const R* u, const R* v, C* G1, C* G2, C* G3, C* G4, const C* twx) {
@@ -198,7 +234,9 @@ void OpsV<R, V>::xb_all(const Geo& g, hipStream_t st, const C* Pq, const C* Pz, 
 #undef CALL
   }
 }
+#endif
 
+#if QG_PART_ON(3)
 template <typename R, int V>
 void OpsV<R, V>::yb2_q(const Geo& g, hipStream_t st, int mode, int flags, const C* G1, const C* G2, const C* qd,
                    const C* qn,
@@ -214,7 +252,9 @@ void OpsV<R, V>::yb2_q(const Geo& g, hipStream_t st, int mode, int flags, const 
 #undef CALL
   }
 }
+#endif
 
+#if QG_PART_ON(3)
 template <typename R, int V>
 void OpsV<R, V>::yb2_B(const Geo& g, hipStream_t st, int mode, int flags, const C* G3, const C* G4,
                    const C* A, const C* Bn, C* rhs0, C* Bout, C* T1, C* T3, EtdView etd, double alpha,
@@ -224,7 +264,9 @@ void OpsV<R, V>::yb2_B(const Geo& g, hipStream_t st, int mode, int flags, const 
   QG_SWITCH(g.ny, CALL)
 #undef CALL
 }
+#endif
 
+#if QG_PART_ON(1)
 template <typename R, int V>
 void OpsV<R, V>::yf_omega(const Geo& g, hipStream_t st, const C* psi, C* Pbx, C* Pby, C* Pxx, C* Pxy, C* Pu,
                       C* Pv, C* Ppsi, int* flag, double* lsum, const C* twy) {
@@ -250,7 +292,9 @@ void OpsV<R, V>::yf_omega(const Geo& g, hipStream_t st, const C* psi, C* Pbx, C*
 #undef CALL
   }
 }
+#endif
 
+#if QG_PART_ON(1)
 template <typename R, int V>
 void OpsV<R, V>::xf(const Geo& g, hipStream_t st, int want_omega, const C* Pu, const C* Pv, const C* Pbx,
                 const C* Pby, const C* Pxx, const C* Pxy, R* u, R* v, C* Rw, const C* Ppsi,
@@ -265,21 +309,27 @@ void OpsV<R, V>::xf(const Geo& g, hipStream_t st, int want_omega, const C* Pu, c
 #undef CALL
   }
 }
+#endif
 
+#if QG_PART_ON(0)
 template <typename R, int V>
 void OpsV<R, V>::y_r2c_mask(const Geo& g, hipStream_t st, const C* Rw, C* out, const C* twy) {
 #define CALL(N) YL((k_y_r2c_mask<R, N, V>), N, g.Kx, g, Rw, out, twy)
   QG_SWITCH(g.ny, CALL)
 #undef CALL
 }
+#endif
 
+#if QG_PART_ON(0)
 template <typename R, int V>
 void OpsV<R, V>::y_herm(const Geo& g, hipStream_t st, const C* in, C* P, const C* twy) {
 #define CALL(N) YL((k_y_herm<R, N, V>), N, g.Kx, g, in, P, twy)
   QG_SWITCH(g.ny, CALL)
 #undef CALL
 }
+#endif
 
+#if QG_PART_ON(3)
 template <typename R, int V>
 void OpsV<R, V>::y_c2c(const Geo& g, hipStream_t st, int dir, const D* in, D* out, const C* twy) {
   if (dir < 0) {
@@ -292,7 +342,9 @@ void OpsV<R, V>::y_c2c(const Geo& g, hipStream_t st, int dir, const D* in, D* ou
 #undef CALL
   }
 }
+#endif
 
+#if QG_PART_ON(3)
 template <typename R, int V>
 void OpsV<R, V>::x_c2c(const Geo& g, hipStream_t st, int dir, const D* in, D* out, const C* twx) {
   if (dir < 0) {
@@ -305,7 +357,9 @@ void OpsV<R, V>::x_c2c(const Geo& g, hipStream_t st, int dir, const D* in, D* ou
 #undef CALL
   }
 }
+#endif
 
+#if QG_PART_ON(1)
 template <typename R, int V>
 void OpsV<R, V>::check_y(const Geo& g, hipStream_t st, const C* psi, const C* q, const C* B, C* P, int* flag,
                          const C* twy) {
@@ -313,14 +367,18 @@ void OpsV<R, V>::check_y(const Geo& g, hipStream_t st, const C* psi, const C* q,
   QG_SWITCH(g.ny, CALL)
 #undef CALL
 }
+#endif
 
+#if QG_PART_ON(1)
 template <typename R, int V>
 void OpsV<R, V>::check_x(const Geo& g, hipStream_t st, const C* P, unsigned long long* absmax, const C* twx) {
 #define CALL(N) XL((k_x_psi_max<R, N, V>), N, g, P, absmax, twx)
   QG_SWITCH(g.nx, CALL)
 #undef CALL
 }
+#endif
 
+#if QG_PART_ON(3)
 template <typename R, int V>
 void OpsV<R, V>::la_max(const Geo& g, hipStream_t st, const C* B, const C* A, int ybj_plus, C* tmp,
                         unsigned long long* absmax, const C* twx, const C* twy) {
@@ -331,5 +389,6 @@ void OpsV<R, V>::la_max(const Geo& g, hipStream_t st, const C* B, const C* A, in
   QG_SWITCH(g.nx, CALL)
 #undef CALL
 }
+#endif
 
 }  // namespace qg

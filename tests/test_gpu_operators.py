@@ -25,7 +25,10 @@ EXP_N2 = lambda z: 1e-4 * math.exp(z / 800.0)  # noqa: E731
                                   (1, 2, 2), (3, 4, 2), (5, 6, 2), (9, 18, 2), (25, 27, 2), (36, 45, 2),
                                   (50, 54, 2), (72, 75, 2), (90, 100, 2), (108, 125, 2), (144, 150, 1),
                                   (180, 200, 1), (216, 250, 1), (288, 300, 1), (360, 400, 1), (432, 500, 1),
-                                  (576, 600, 1), (720, 800, 1), (864, 1000, 1)])
+                                  (576, 600, 1), (720, 800, 1), (864, 1000, 1),
+                                  # plans with a radix-15 or radix-30 stage
+                                  (15, 30, 2), (60, 120, 2), (135, 225, 1), (240, 375, 1),
+                                  (480, 540, 1), (750, 900, 1), (960, 15, 1)])
 def test_fft_matches_dft_and_round_trips(qg, size):
     """KAT test/test_mpi_extension.jl:62-90 (round trip < 1e-12) and the
     transform contract of src/transforms.jl:19-23 against numpy's DFT."""
