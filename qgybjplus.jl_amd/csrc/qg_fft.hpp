@@ -20,8 +20,8 @@
 
 namespace qg {
 
-// cos(2*pi*m/240) for m in [0, 60], correctly rounded; every radix used here
-// (2, 3, 4, 5, 6, 8, 10, 12, 15, 16, 30) divides 240.
+// cos(2*pi*m/240) for m in [0, 60], correctly rounded; the radices 2, 3, 4,
+// 5, 6, 8, 10, 12, 15, 16 and 30 divide 240.
 constexpr double kCos240[61] = {
     1.0, 0.9996573249755573, 0.9986295347545738, 0.996917333733128,
     0.9945218953682733, 0.9914448613738104, 0.9876883405951378, 0.9832549075639546,
@@ -49,12 +49,88 @@ __host__ __device__ constexpr double cos240(int m) {
 }
 __host__ __device__ constexpr double sin240(int m) { return cos240(m - 60); }
 
-// exp(DIR * 2*pi*i * m / N) for N dividing 240.
+// cos(2*pi*m/900) for m in [0, 225], correctly rounded (evaluated to 50
+// digits); the roots of the radices 9 and 25, which do not divide 240.
+constexpr double kCos900[226] = {
+    1.0, 0.9999756307053947, 0.9999025240093042, 0.9997806834748455,
+    0.9996101150403544, 0.9993908270190958, 0.9991228300988584, 0.9988061373414341,
+    0.998440764181981, 0.9980267284282716, 0.9975640502598242, 0.9970527522269202,
+    0.9964928592495044, 0.9958843986159703, 0.9952273999818312, 0.9945218953682733,
+    0.9937679191605964, 0.9929655081065369, 0.9921147013144779, 0.9912155402515417,
+    0.9902680687415704, 0.9892723329629883, 0.9882283814465528, 0.9871362650729879,
+    0.985996037070505, 0.984807753012208, 0.9835714708133859, 0.9822872507286887,
+    0.9809551553491915, 0.9795752495993441, 0.9781476007338057, 0.9766722783341679,
+    0.9751493543055633, 0.9735789028731603, 0.9719610005785463, 0.9702957262759965,
+    0.9685831611286311, 0.9668233886044594, 0.9650164944723114, 0.9631625667976581,
+    0.9612616959383189, 0.9593139745400575, 0.9573194975320672, 0.9552783621223436,
+    0.953190667792947, 0.9510565162951535, 0.9488760116444965, 0.9466492601156964,
+    0.9443763702374811, 0.9420574527872967, 0.9396926207859084, 0.9372819894918915,
+    0.9348256763960144, 0.9323238012155122, 0.9297764858882515, 0.9271838545667874,
+    0.9245460336123131, 0.9218631515885005, 0.9191353392552344, 0.9163627295622396,
+    0.9135454576426009, 0.9106836608061771, 0.9077774785329087, 0.9048270524660196,
+    0.9018325264051138, 0.898794046299167, 0.8957117602394129, 0.8925858184521255,
+    0.8894163732912975, 0.8862035792312147, 0.882947592858927, 0.8796485728666165,
+    0.8763066800438636, 0.8729220772698096, 0.869494929505219, 0.8660254037844386,
+    0.8625136692072575, 0.8589598969306644, 0.8553642601605066, 0.8517269341430477,
+    0.848048096156426, 0.8443279255020151, 0.8405666034956842, 0.8367643134589616,
+    0.8329212407100994, 0.8290375725550417, 0.8251134982782952, 0.821149209133704,
+    0.8171448983351285, 0.8131007610470277, 0.8090169943749475, 0.8048937973559142,
+    0.8007313709487335, 0.7965299180241963, 0.7922896433551907, 0.7880107536067219,
+    0.7836934573258398, 0.7793379649314741, 0.7749444887041796, 0.7705132427757893,
+    0.766044443118978, 0.7615383075367368, 0.7569950556517564, 0.7524149088957244,
+    0.7477980904985319, 0.7431448254773942, 0.7384553406258838, 0.7337298645028764,
+    0.7289686274214116, 0.7241718614374675, 0.7193398003386512, 0.7144726796328034,
+    0.7095707365365209, 0.7046342099635947, 0.6996633405133654, 0.6946583704589973,
+    0.6896195437356698, 0.6845471059286887, 0.6794413042615165, 0.6743023875837234,
+    0.6691306063588582, 0.6639262126522416, 0.6586894601186805, 0.6534206039901055,
+    0.6481199010631309, 0.6427876096865394, 0.6374239897486897, 0.6320293026648509,
+    0.6266038113644605, 0.6211477802783104, 0.6156614753256583, 0.6101451639012676,
+    0.6045991148623748, 0.5990235985155858, 0.5934188866037015, 0.5877852522924731,
+    0.5821229701572894, 0.5764323161697933, 0.5707135676844317, 0.564967003424938,
+    0.5591929034707468, 0.5533915492433441, 0.5475632234925503, 0.5417082102827399,
+    0.5358267949789967, 0.5299192642332049, 0.5239859059700791, 0.5180270093731302,
+    0.5120428648705715, 0.5060337641211639, 0.5, 0.49394186658423095,
+    0.4878596591387327, 0.48175367410171527, 0.47562420907027525, 0.46947156278589075,
+    0.4632960351198617, 0.4570979270586942, 0.45087754068943076, 0.4446351791849275,
+    0.4383711467890774, 0.4320857488019823, 0.42577929156507266, 0.4194520824461771,
+    0.41310442982454176, 0.4067366430758002, 0.40034903255689497, 0.3939419095909511,
+    0.387515586452103, 0.3810703763502741, 0.374606593415912, 0.368124552684678,
+    0.3616245700820923, 0.35510696240813705, 0.34857204732181524, 0.3420201433256687,
+    0.33545156975025503, 0.32886664673858323, 0.3222656952305111, 0.31564903694710245,
+    0.30901699437494745, 0.3023698907504445, 0.2957080500440467, 0.2890317969444716,
+    0.2823414568428765, 0.27563735581699916, 0.2689198206152657, 0.2621891786408647,
+    0.25544575793579055, 0.2486898871648548, 0.24192189559966773, 0.23514211310259003,
+    0.22835087011065575, 0.2215484976194673, 0.2147353271670632, 0.20791169081775934,
+    0.20107792114596468, 0.19423435121997198, 0.18738131458572463, 0.18051914525055998,
+    0.17364817766693036, 0.16676874671610228, 0.15988118769183488, 0.15298583628403806,
+    0.14608302856241162, 0.13917310096006544, 0.13225639025712244, 0.12533323356430426,
+    0.11840396830650095, 0.11146893220632548, 0.10452846326765347, 0.09758289975914947,
+    0.09063258019778016, 0.0836778433323155, 0.07671902812681865, 0.0697564737441253,
+    0.06279051952931337, 0.0558215049931638, 0.048849769795613264, 0.04187565372919963,
+    0.03489949670250097, 0.02792163872356888, 0.02094241988335696, 0.013962180339145272,
+    0.0069812602979615525, 0.0,
+};
+
+__host__ __device__ constexpr double cos900(int m) {
+  m = ((m % 900) + 900) % 900;
+  if (m <= 225) return kCos900[m];
+  if (m <= 450) return -kCos900[450 - m];
+  if (m <= 675) return -kCos900[m - 450];
+  return kCos900[900 - m];
+}
+__host__ __device__ constexpr double sin900(int m) { return cos900(m - 225); }
+
+// exp(DIR * 2*pi*i * m / N) for N dividing 240 or 900.
 template <typename R, int N, int DIR>
 __device__ inline cplx<R> unit_root(int m) {
-  static_assert(240 % N == 0, "radix must divide 240");
-  constexpr int f = 240 / N;
-  return mk<R>((R)cos240(m * f), (R)(DIR * sin240(m * f)));
+  if constexpr (240 % N == 0) {
+    constexpr int f = 240 / N;
+    return mk<R>((R)cos240(m * f), (R)(DIR * sin240(m * f)));
+  } else {
+    static_assert(900 % N == 0, "radix must divide 240 or 900");
+    constexpr int f = 900 / N;
+    return mk<R>((R)cos900(m * f), (R)(DIR * sin900(m * f)));
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -172,6 +248,8 @@ template <int DIR> struct Dft<12, DIR> : DftCT<4, 3, DIR> {};
 template <int DIR> struct Dft<16, DIR> : DftCT<4, 4, DIR> {};
 template <int DIR> struct Dft<15, DIR> : DftCT<3, 5, DIR> {};
 template <int DIR> struct Dft<30, DIR> : DftCT<2, 15, DIR> {};
+template <int DIR> struct Dft<9, DIR> : DftCT<3, 3, DIR> {};
+template <int DIR> struct Dft<25, DIR> : DftCT<5, 5, DIR> {};
 
 // ---------------------------------------------------------------------------
 // Plans: radices per stage (first == last so the register distribution of the
@@ -229,8 +307,23 @@ template <> struct FftPlan<375>  { static constexpr int S = 3, r0 = 5, r1 = 15, 
 template <> struct FftPlan<480>  { static constexpr int S = 3, r0 = 4, r1 = 30, T = 60; };
 template <> struct FftPlan<540>  { static constexpr int S = 3, r0 = 6, r1 = 15, T = 90; };
 template <> struct FftPlan<750>  { static constexpr int S = 3, r0 = 5, r1 = 30, T = 75; };
-template <> struct FftPlan<900>  { static constexpr int S = 2, r0 = 30, r1 = 1, T = 30; };
 template <> struct FftPlan<960>  { static constexpr int S = 3, r0 = 8, r1 = 15, T = 120; };
+// with a radix-9 or radix-25 stage
+template <> struct FftPlan<81>   { static constexpr int S = 2, r0 = 9, r1 = 1, T = 9; };
+template <> struct FftPlan<162>  { static constexpr int S = 3, r0 = 9, r1 = 2, T = 18; };
+template <> struct FftPlan<243>  { static constexpr int S = 3, r0 = 9, r1 = 3, T = 27; };
+template <> struct FftPlan<270>  { static constexpr int S = 3, r0 = 3, r1 = 30, T = 45; };
+template <> struct FftPlan<324>  { static constexpr int S = 3, r0 = 9, r1 = 4, T = 36; };
+template <> struct FftPlan<405>  { static constexpr int S = 3, r0 = 9, r1 = 5, T = 45; };
+template <> struct FftPlan<450>  { static constexpr int S = 3, r0 = 15, r1 = 2, T = 30; };
+template <> struct FftPlan<486>  { static constexpr int S = 3, r0 = 9, r1 = 6, T = 54; };
+template <> struct FftPlan<625>  { static constexpr int S = 3, r0 = 5, r1 = 25, T = 125; };
+template <> struct FftPlan<900>  { static constexpr int S = 3, r0 = 10, r1 = 9, T = 90; };
+template <> struct FftPlan<648>  { static constexpr int S = 3, r0 = 9, r1 = 8, T = 72; };
+template <> struct FftPlan<675>  { static constexpr int S = 3, r0 = 15, r1 = 3, T = 45; };
+template <> struct FftPlan<729>  { static constexpr int S = 3, r0 = 9, r1 = 9, T = 81; };
+template <> struct FftPlan<810>  { static constexpr int S = 3, r0 = 9, r1 = 10, T = 90; };
+template <> struct FftPlan<972>  { static constexpr int S = 3, r0 = 9, r1 = 12, T = 108; };
 template <> struct FftPlan<10>   { static constexpr int S = 1, r0 = 10, r1 = 1, T = 1; };
 template <> struct FftPlan<12>   { static constexpr int S = 1, r0 = 12, r1 = 1, T = 1; };
 template <> struct FftPlan<16>   { static constexpr int S = 2, r0 = 4, r1 = 1, T = 4; };

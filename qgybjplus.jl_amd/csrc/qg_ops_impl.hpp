@@ -21,9 +21,9 @@ namespace qg {
 
 // The transform lengths compiled.  Every tile variant has the lengths whose
 // variants the autotuner explores (8, 16, 32, 64, 128, 256, 384, 512); the
-// base variant (the translation units that define QG_ALL_LENGTHS) also has the
-// other even products of the radices 2, 3, 4, 5, 6, 8, 10, 12, 16 with a plan
-// in qg_fft.hpp, which run at the base variant only (fft_size_supported,
+// base variant (the translation units that define QG_ALL_LENGTHS) also has
+// every other length from 1 to 1024 with no prime factor above 5 (a plan in
+// qg_fft.hpp each), which run at the base variant only (fft_size_supported,
 // fft_variants_supported).
 #ifdef QG_ALL_LENGTHS
 #define QG_MORE_LENGTHS(CALL)                                                 \
@@ -91,7 +91,21 @@ namespace qg {
     case 540: CALL(540); break;                                               \
     case 750: CALL(750); break;                                               \
     case 900: CALL(900); break;                                               \
-    case 960: CALL(960); break;
+    case 960: CALL(960); break;                                               \
+    case 81: CALL(81); break;                                                 \
+    case 162: CALL(162); break;                                               \
+    case 243: CALL(243); break;                                               \
+    case 270: CALL(270); break;                                               \
+    case 324: CALL(324); break;                                               \
+    case 405: CALL(405); break;                                               \
+    case 450: CALL(450); break;                                               \
+    case 486: CALL(486); break;                                               \
+    case 625: CALL(625); break;                                               \
+    case 648: CALL(648); break;                                               \
+    case 675: CALL(675); break;                                               \
+    case 729: CALL(729); break;                                               \
+    case 810: CALL(810); break;                                               \
+    case 972: CALL(972); break;
 #else
 #define QG_MORE_LENGTHS(CALL)
 #endif

@@ -5,12 +5,13 @@
 
 namespace qg {
 // Every transform length with a plan in qg_fft.hpp (FftPlan), compiled for
-// the base tile variant.
+// the base tile variant: the lengths 1 to 1024 with no prime factor above 5.
 static const int kLengths[] = {1, 2, 3, 4, 5, 6, 8, 9, 10, 12, 15, 16, 18, 20, 24, 25,
-                               27, 30, 32, 36, 40, 45, 48, 50, 54, 60, 64, 72, 75, 80, 90, 96,
-                               100, 108, 120, 125, 128, 135, 144, 150, 160, 180, 192, 200, 216, 225, 240, 250,
-                               256, 288, 300, 320, 360, 375, 384, 400, 432, 480, 500, 512, 540, 576, 600, 640,
-                               720, 750, 768, 800, 864, 900, 960, 1000, 1024};
+                               27, 30, 32, 36, 40, 45, 48, 50, 54, 60, 64, 72, 75, 80, 81, 90,
+                               96, 100, 108, 120, 125, 128, 135, 144, 150, 160, 162, 180, 192, 200, 216, 225,
+                               240, 243, 250, 256, 270, 288, 300, 320, 324, 360, 375, 384, 400, 405, 432, 450,
+                               480, 486, 500, 512, 540, 576, 600, 625, 640, 648, 675, 720, 729, 750, 768, 800,
+                               810, 864, 900, 960, 972, 1000, 1024};
 bool fft_size_supported(int n) {
   for (int l : kLengths)
     if (l == n) return true;

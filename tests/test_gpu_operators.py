@@ -28,7 +28,10 @@ EXP_N2 = lambda z: 1e-4 * math.exp(z / 800.0)  # noqa: E731
                                   (576, 600, 1), (720, 800, 1), (864, 1000, 1),
                                   # plans with a radix-15 or radix-30 stage
                                   (15, 30, 2), (60, 120, 2), (135, 225, 1), (240, 375, 1),
-                                  (480, 540, 1), (750, 900, 1), (960, 15, 1)])
+                                  (480, 540, 1), (750, 900, 1), (960, 15, 1),
+                                  # plans with a radix-9 or radix-25 stage
+                                  (81, 162, 2), (243, 270, 1), (324, 405, 1), (450, 486, 1),
+                                  (625, 648, 1), (675, 729, 1), (810, 972, 1)])
 def test_fft_matches_dft_and_round_trips(qg, size):
     """KAT test/test_mpi_extension.jl:62-90 (round trip < 1e-12) and the
     transform contract of src/transforms.jl:19-23 against numpy's DFT."""

@@ -223,7 +223,8 @@ def test_other_plans_step_matches_oracle(qg, size):
 
 @pytest.mark.parametrize("size", [(4, 4, 2), (8, 6, 4), (2, 2, 2), (3, 1, 2), (64, 48, 24), (5, 9, 3),
                                   (25, 27, 4), (36, 50, 6), (45, 54, 3), (75, 72, 4), (100, 90, 3),
-                                  (125, 150, 2), (200, 216, 2), (15, 30, 3), (60, 45, 3), (120, 135, 2)])
+                                  (125, 150, 2), (200, 216, 2), (15, 30, 3), (60, 45, 3), (120, 135, 2),
+                                  (81, 90, 2), (162, 243, 2)])
 def test_more_sizes_step_matches_oracle(qg, size):
     """Grids of the reference's own tests ((4, 4, 2), (8, 6, 4), (2, 2, 2),
     (3, 1, 2), (64, 48, 24): test/test_core_components.jl and others), odd
