@@ -168,10 +168,11 @@ inline int scan_tiles(const Geo& g, int QC) {
   return g.pruned() ? g.Kx * scan_row_tiles(g, QC) : (quarter(g).size() + QC - 1) / QC;
 }
 
-// Logical block x: tile of QC quarter columns; y: member (xcd_block()).
+// The solve of operator OP by one block: blk.x is the tile of QC quarter
+// columns, blk.y the member; lds is the block's dynamic LDS (lds_bytes()).
 template <int OP, int M, int QC, typename T>
-__global__ void __launch_bounds__((ScanBounds<M, QC>::MAXT))
-k_colscan(Geo g, Quarter qr, ScanArgs a) {
+__device__ inline void colscan_block(const Geo& g, const Quarter& qr, const ScanArgs& a, const Blk blk,
+                                     double* lds) {
   constexpr int TP = 8 * QC;
   constexpr int R = (M * QC + TP - 1) / TP;  // table entries staged per thread
   constexpr int LR = (M + TP - 1) / TP;      // level entries staged per thread
@@ -181,7 +182,6 @@ k_colscan(Geo g, Quarter qr, ScanArgs a) {
   const int w = threadIdx.x / TP, t = threadIdx.x % TP;
   const int part = t & 1, cs = t >> 1, j = cs % QC, mir = cs / QC;
   const int rplane = qr.size();
-  const Blk blk = xcd_block();
   int rc0 = blk.x * QC, row = -1;
   if (g.pruned()) {  // uniform
     const int tpr = scan_row_tiles(g, QC), ry = blk.x / tpr, rx0 = (blk.x % tpr) * QC;
@@ -198,7 +198,6 @@ k_colscan(Geo g, Quarter qr, ScanArgs a) {
   const double* lv0 = a.lv0 + mem * a.lv0_stride;
   const double* lv1 = (OP == SCAN_OMEGA) ? a.lv1 + mem * a.lv1_stride : nullptr;
 
-  extern __shared__ double lds[];
   double* T0 = lds;               // [kpad][QC]
   double* T1 = T0 + kpad * QC;    // [kpad][QC]
   double* L0 = T1 + kpad * QC;    // [kpad]
@@ -409,10 +408,38 @@ k_colscan(Geo g, Quarter qr, ScanArgs a) {
   }
 }
 
+// Logical block x: tile of QC quarter columns; y: member (xcd_block()).
+template <int OP, int M, int QC, typename T>
+__global__ void __launch_bounds__((ScanBounds<M, QC>::MAXT))
+k_colscan(Geo g, Quarter qr, ScanArgs a) {
+  extern __shared__ double lds[];
+  colscan_block<OP, M, QC, T>(g, qr, a, xcd_block(), lds);
+}
+
+// Two independent solves in one launch: logical blocks x < n0 solve OP0 with
+// a0, the others OP1 with a1 (tile x - n0).  The blocks of both are
+// interleaved over the grid, so the two solves run side by side, as two
+// launches on two streams would, without the streams' synchronization.
+template <int OP0, int OP1, int M, int QC, typename T>
+__global__ void __launch_bounds__((ScanBounds<M, QC>::MAXT))
+k_colscan2(Geo g, Quarter qr, ScanArgs a0, ScanArgs a1, int n0) {
+  extern __shared__ double lds[];
+  Blk b = xcd_block();
+  if (b.x < n0) {
+    colscan_block<OP0, M, QC, T>(g, qr, a0, b, lds);
+  } else {
+    b.x -= n0;
+    colscan_block<OP1, M, QC, T>(g, qr, a1, b, lds);
+  }
+}
+
 // Host side: launch OP with compiled shape `idx` (see kScanShapes) on fields
 // stored in FP32 (f32) or FP64.
 template <int OP>
 void launch_colscan(const Geo& g, hipStream_t st, int idx, const ScanArgs& a, bool f32);
+// OP0 with a0 and OP1 with a1 in one launch (k_colscan2), both at shape idx.
+template <int OP0, int OP1>
+void launch_colscan2(const Geo& g, hipStream_t st, int idx, const ScanArgs& a0, const ScanArgs& a1, bool f32);
 // Raise the dynamic LDS limit of every instantiation (once per process).
 bool colscan_setup();
 

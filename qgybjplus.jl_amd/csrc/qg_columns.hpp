@@ -26,6 +26,12 @@ int thomas_shifted_a_size(int nzc);
 void launch_thomas(const Geo& g, hipStream_t st, const void* rhs, void* out, void* Cout, bool f32,
                    const double* inv, const double* cp, const double* al, int kdiv, unsigned long long* warn,
                    int shape = -1);
+// The q -> psi solve (tables inv_q, cp_q; warn as for kdiv 1) and the
+// B -> A solve (inv_b, cp_b; C when Cout is set) in one launch, both at
+// launch shape `shape`: the same values as the two launch_thomas calls.
+void launch_thomas_pair(const Geo& g, hipStream_t st, bool f32, const double* al, int shape, const void* q,
+                        void* psi, const double* inv_q, const double* cp_q, unsigned long long* warn,
+                        const void* B, void* A, void* Cout, const double* inv_b, const double* cp_b);
 // QG omega-equation interior solves: elimination recorded once per model
 // (tables of thomas_table_size() entries, pivoted flags per quarter column,
 // err[3] set when any column interchanged rows), then applied to every

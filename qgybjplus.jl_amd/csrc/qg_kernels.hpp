@@ -868,21 +868,21 @@ __device__ inline cplx<double> vdiff(const cplx<S>* __restrict__ q, int z, int n
 // the ETD stage/final update of q (timestep.jl:286-295, 362-404).
 // FL >= 0 fixes the flag word at compile time (the launcher passes the same
 // value), which removes the per-point branches on it; FL = -1 reads `flags`.
+// The pass's work for tile blk (blk.x: tile of S columns, blk.y: level);
+// lds: the block's TL::S * TL::LD values.
 template <typename R, int NY, int V, int FL = -1>
-__global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
-k_yb2_q(Geo g, int mode, int flags_rt, const cplx<R>* __restrict__ G1, const cplx<R>* __restrict__ G2,
-        const cplx<R>* __restrict__ qd, const cplx<R>* __restrict__ qn,
-        cplx<R>* __restrict__ rhs0,
-        cplx<R>* __restrict__ qout, Etd etd, double nu, double inv_dz2, int* __restrict__ flag,
-        const cplx<R>* __restrict__ tw) {
+__device__ inline void yb2_q_block(const Geo& g, const Blk blk, cplx<R>* lds, int mode, int flags_rt,
+                                   const cplx<R>* __restrict__ G1, const cplx<R>* __restrict__ G2,
+                                   const cplx<R>* __restrict__ qd, const cplx<R>* __restrict__ qn,
+                                   cplx<R>* __restrict__ rhs0, cplx<R>* __restrict__ qout, const Etd& etd,
+                                   double nu, double inv_dz2, int* __restrict__ flag,
+                                   const cplx<R>* __restrict__ tw) {
   const int flags = FL >= 0 ? FL : flags_rt;
   using TL = Tile<R, NY, V>;
   using F = Fft<R, NY, -1, TL::PL>;
-  __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
   TwTab<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
-  const Blk blk = xcd_block();
   const int x = blk.x * TL::S + s, z = blk.y;
   const bool ok = x < g.Kx;
   cplx<R>* sl = lds + s * TL::LD;
@@ -963,24 +963,38 @@ k_yb2_q(Geo g, int mode, int flags_rt, const cplx<R>* __restrict__ G1, const cpl
   if (flag && !fin) atomicOr(flag, 1);
 }
 
+template <typename R, int NY, int V, int FL = -1>
+__global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
+k_yb2_q(Geo g, int mode, int flags_rt, const cplx<R>* __restrict__ G1, const cplx<R>* __restrict__ G2,
+        const cplx<R>* __restrict__ qd, const cplx<R>* __restrict__ qn,
+        cplx<R>* __restrict__ rhs0,
+        cplx<R>* __restrict__ qout, Etd etd, double nu, double inv_dz2, int* __restrict__ flag,
+        const cplx<R>* __restrict__ tw) {
+  using TL = Tile<R, NY, V>;
+  __shared__ cplx<R> lds[TL::S * TL::LD];
+  yb2_q_block<R, NY, V, FL>(g, xcd_block(), lds, mode, flags_rt, G1, G2, qd, qn, rhs0, qout, etd, nu, inv_dz2,
+                            flag, tw);
+}
+
 // Y-pass (forward, full width) of G3 = -i kx F_x(uB) - (i/2) F_x(zeta B) and
 // G4 = F_x(vB): rhs_B = mask*(F_y(G3) - i ky F_y(G4)) + i (f/2) kh2 A, i.e.
 // -nB - (i/2) rB + i (f/2) kh2 A with nB = i kx F(uB) + i ky F(vB) and
 // rB = F(zeta B); then the ETD update of B and, with T1, the inverse y
 // transforms of the new B that the next pass over B needs.
+// The pass's work for tile blk (blk.x: full-width tile, blk.y: level); lds:
+// the block's TL::S * TL::LD values.
 template <typename R, int NY, int V>
-__global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
-k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R>* __restrict__ G4,
-        const cplx<R>* __restrict__ A, const cplx<R>* __restrict__ Bn, cplx<R>* __restrict__ rhs0,
-        cplx<R>* __restrict__ Bout, cplx<R>* __restrict__ T1, cplx<R>* __restrict__ T3, Etd etd, double alpha,
-        int* __restrict__ flag, const cplx<R>* __restrict__ tw) {
+__device__ inline void yb2_B_block(const Geo& g, const Blk blk, cplx<R>* lds, int mode, int flags,
+                                   const cplx<R>* __restrict__ G3, const cplx<R>* __restrict__ G4,
+                                   const cplx<R>* __restrict__ A, const cplx<R>* __restrict__ Bn,
+                                   cplx<R>* __restrict__ rhs0, cplx<R>* __restrict__ Bout, cplx<R>* __restrict__ T1,
+                                   cplx<R>* __restrict__ T3, const Etd& etd, double alpha, int* __restrict__ flag,
+                                   const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NY, V>;
   using F = Fft<R, NY, -1, TL::PL>;
-  __shared__ cplx<R> lds[TL::S * TL::LD];
   const int s = threadIdx.x % TL::S, lane = threadIdx.x / TL::S;
   TwTab<R, TL::NN, TL::PL> twr;
   twr.load(lane, tw);
-  const Blk blk = xcd_block();
   const int x = fw_x(g, TL::S, blk.x, s), z = blk.y;
   const bool ok = x < g.nx && g.col_kept(x);
   cplx<R>* sl = lds + s * TL::LD;
@@ -1103,6 +1117,43 @@ k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R
           if (T3) T3[ad] = w[it][t];
         }
       }
+  }
+}
+
+template <typename R, int NY, int V>
+__global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
+k_yb2_B(Geo g, int mode, int flags, const cplx<R>* __restrict__ G3, const cplx<R>* __restrict__ G4,
+        const cplx<R>* __restrict__ A, const cplx<R>* __restrict__ Bn, cplx<R>* __restrict__ rhs0,
+        cplx<R>* __restrict__ Bout, cplx<R>* __restrict__ T1, cplx<R>* __restrict__ T3, Etd etd, double alpha,
+        int* __restrict__ flag, const cplx<R>* __restrict__ tw) {
+  using TL = Tile<R, NY, V>;
+  __shared__ cplx<R> lds[TL::S * TL::LD];
+  yb2_B_block<R, NY, V>(g, xcd_block(), lds, mode, flags, G3, G4, A, Bn, rhs0, Bout, T1, T3, etd, alpha, flag, tw);
+}
+
+// The q update (k_yb2_q) and the B update (k_yb2_B) of one ETD stage in one
+// launch: logical blocks x < nq of each level run the q pass's tile x, the
+// others the B pass's tile x - nq.  The two read and write disjoint arrays,
+// so their blocks run side by side, as two launches on two streams would,
+// without the streams' synchronization.  Both share the tile (variant V) and
+// the staged twiddle and wavenumber tables.
+template <typename R, int NY, int V, int FL = -1>
+__global__ void __launch_bounds__((Tile<R, NY, V>::THREADS))
+k_yb2_qB(Geo g, int nq, int mode, int qflags, const cplx<R>* __restrict__ G1, const cplx<R>* __restrict__ G2,
+         const cplx<R>* __restrict__ qd, const cplx<R>* __restrict__ qn, cplx<R>* __restrict__ rq0,
+         cplx<R>* __restrict__ qout, double nu, double inv_dz2, int bflags, const cplx<R>* __restrict__ G3,
+         const cplx<R>* __restrict__ G4, const cplx<R>* __restrict__ A, const cplx<R>* __restrict__ Bn,
+         cplx<R>* __restrict__ rB0, cplx<R>* __restrict__ Bout, cplx<R>* __restrict__ T1,
+         cplx<R>* __restrict__ T3, double alpha, Etd etd, int* __restrict__ flag,
+         const cplx<R>* __restrict__ tw) {
+  using TL = Tile<R, NY, V>;
+  __shared__ cplx<R> lds[TL::S * TL::LD];
+  Blk b = xcd_block();
+  if (b.x < nq) {
+    yb2_q_block<R, NY, V, FL>(g, b, lds, mode, qflags, G1, G2, qd, qn, rq0, qout, etd, nu, inv_dz2, flag, tw);
+  } else {
+    b.x -= nq;
+    yb2_B_block<R, NY, V>(g, b, lds, mode, bflags, G3, G4, A, Bn, rB0, Bout, T1, T3, etd, alpha, flag, tw);
   }
 }
 

@@ -51,6 +51,11 @@ struct OpsV {
   static void yb2_B(const Geo& g, hipStream_t st, int mode, int flags, const C* G3, const C* G4,
                     const C* A, const C* Bn, C* rhs0, C* Bout, C* T1, C* T3, EtdView etd, double alpha,
                     int* flag, const C* twy);
+  // yb2_q (flags YQ_ADV only) and yb2_B in one launch (k_yb2_qB): the same
+  // values as the two launches
+  static void yb2_qB(const Geo& g, hipStream_t st, int mode, const C* G1, const C* G2, const C* qn, C* rq0,
+                     C* qout, int bflags, const C* G3, const C* G4, const C* A, const C* Bn, C* rB0, C* Bout,
+                     C* T1, C* T3, EtdView etd, double alpha, int* flag, const C* twy);
   // Pu != nullptr: also the y-transformed half rows of u and v (as yb_adv);
   // Ppsi != nullptr: also those of psi, and psi's non-finite test into flag;
   // lsum != nullptr (instead of Ppsi): psi's non-finite test and the per-level
@@ -105,6 +110,7 @@ struct Ops {
   QG_OPS_FWD(xb_all)
   QG_OPS_FWD(yb2_q)
   QG_OPS_FWD(yb2_B)
+  QG_OPS_FWD(yb2_qB)
   QG_OPS_FWD(yf_omega)
   QG_OPS_FWD(xf)
   QG_OPS_FWD(y_r2c_mask)

@@ -280,6 +280,25 @@ void OpsV<R, V>::yb2_B(const Geo& g, hipStream_t st, int mode, int flags, const 
 }
 #endif
 
+#if QG_PART_ON(3)
+template <typename R, int V>
+void OpsV<R, V>::yb2_qB(const Geo& g, hipStream_t st, int mode, const C* G1, const C* G2, const C* qn, C* rq0,
+                        C* qout, int bflags, const C* G3, const C* G4, const C* A, const C* Bn, C* rB0, C* Bout,
+                        C* T1, C* T3, EtdView etd, double alpha, int* flag, const C* twy) {
+  Etd e{etd.p};
+#define CALL(N)                                                                                              \
+  do {                                                                                                       \
+    using TL = Tile<R, N, V>;                                                                                \
+    const int nq = (g.Kx + TL::S - 1) / TL::S;                                                               \
+    hipLaunchKernelGGL((k_yb2_qB<R, N, V, YQ_ADV>), dim3(nq + fw_tiles(g, TL::S), g.nz), dim3(TL::THREADS), 0, \
+                       st, g, nq, mode, YQ_ADV, G1, G2, qn, qn, rq0, qout, 0.0, 0.0, bflags, G3, G4, A, Bn,   \
+                       rB0, Bout, T1, T3, alpha, e, flag, twy);                                              \
+  } while (0)
+  QG_SWITCH(g.ny, CALL)
+#undef CALL
+}
+#endif
+
 #if QG_PART_ON(1)
 template <typename R, int V>
 void OpsV<R, V>::yf_omega(const Geo& g, hipStream_t st, const C* psi, C* Pbx, C* Pby, C* Pxx, C* Pxy, C* Pu,

@@ -312,6 +312,17 @@ void solve_A(qg_handle& h, const qg::Geo& g, hipStream_t st, const void* B, void
                     h.tune[C ? qg::P_THOMAS_AC : qg::P_THOMAS_A]);
 }
 
+// solve_psi and solve_A (with C when C is set) in one launch, at the psi
+// solve's launch shape: the same values, side by side without a second
+// stream.
+void solve_psi_A(qg_handle& h, const qg::Geo& g, hipStream_t st, const void* q, void* psi, const void* B, void* A,
+                 void* C) {
+  if (h.sing_psi[0]) raise_singular(h.sing_psi);
+  if (h.sing_A[0]) raise_singular(h.sing_A);
+  qg::launch_thomas_pair(g, st, h.f32(), h.d_al, h.tune[qg::P_THOMAS_PSI], q, psi, h.th_inv[0], h.th_cp[0],
+                         h.d_warn, B, A, C, h.th_inv[1], h.th_cp[1]);
+}
+
 template <typename R>
 struct Engine {
   using C = cplx<R>;
@@ -442,7 +453,13 @@ struct Engine {
   // wave feedback, then q -> psi.  Leaves T1 = IFFTy(mask B) when need_t1.
   // t_ready: T1 (and T3 when q_w is needed) already hold the inverse y
   // transforms of Bsrc, made by the yb2_B pass that produced Bsrc.
-  void diag(const C* qsrc, const C* Bsrc, C* psi_out, bool need_t1, bool t_ready = false) {
+  // Bw: also the wave inversion Bw -> Aw (and Cw when set), independent of
+  // the chain: with concurrency on it is solved in the psi solve's launch
+  // (on the side stream when there is no psi solve); otherwise first.
+  void diag(const C* qsrc, const C* Bsrc, C* psi_out, bool need_t1, bool t_ready = false, const C* Bw = nullptr,
+            C* Aw = nullptr, C* Cw = nullptr) {
+    const bool pair = Bw && concurrent() && !h.fixed_flow && !(h.passive || h.no_disp);
+    if (Bw && !pair) side([&] { wave_inv(Bw, Aw, Cw); });
     bool t1 = t_ready;
     if (!h.fixed_flow) {
       const C* src = qsrc;
@@ -456,10 +473,15 @@ struct Engine {
         src = c(h.qeff);
         t1 = true;
       }
-      K("thomas_psi", qg::P_THOMAS_PSI, Ps * 2 * cS + Tq,
-        [&] { solve_psi(h, *G, S, src, psi_out); });
+      if (pair)
+        K(Cw ? "thomas_psi_AC" : "thomas_psi_A", qg::P_THOMAS_PSI, Ps * 2 * cS + Ps * (Cw ? 3 : 2) * cS + 2 * Tq,
+          [&] { solve_psi_A(h, *G, S, src, psi_out, Bw, Aw, Cw); });
+      else
+        K("thomas_psi", qg::P_THOMAS_PSI, Ps * 2 * cS + Tq,
+          [&] { solve_psi(h, *G, S, src, psi_out); });
     }
     if (need_t1 && !t1) t1_only(Bsrc);
+    if (Bw && !pair) join();
   }
 
   void wave_inv(const C* Bsrc, C* Aout, C* Cout) {
@@ -575,22 +597,31 @@ struct Engine {
     const bool diff = h.nu_z != 0.0 && !h.inviscid && h.nz > 1;
     int qf = ((adv && qdyn) ? qg::YQ_ADV : 0) | (h.fixed_flow ? qg::YQ_FIXED : 0) | (diff ? qg::YQ_DIFF : 0);
     const double inv_dz2 = h.nz > 1 ? 1.0 / (h.dz * h.dz) : 0.0;
-    // the q update beside the B update (disjoint inputs and outputs)
-    side([&] {
-      K(mode == qg::MODE_STAGE ? "yb2_q_stage" : "yb2_q_final", qg::P_YB2_Q,
-        // stage: q_n, the kept bracket and q_S; final: the bracket and q_{n+1}
-        ((qf & qg::YQ_ADV) ? Ph * 2 * cR : 0) +
-            Ps * cS * ((mode == qg::MODE_STAGE || h.fixed_flow ? 3 : 2) + (diff ? 1 : 0)),
-        [&] { O::yb2_q(h.tune[qg::P_YB2_Q], *G, S, mode, qf, c(h.G1), c(h.G2), qS, qn, c(h.rq0), qout, etd(), h.nu_z, inv_dz2, flag, twy()); });
-    });
+    // stage: q_n, the kept bracket and q_S; final: the bracket and q_{n+1}
+    const double qbytes = ((qf & qg::YQ_ADV) ? Ph * 2 * cR : 0) +
+                          Ps * cS * ((mode == qg::MODE_STAGE || h.fixed_flow ? 3 : 2) + (diff ? 1 : 0));
     const bool disp = !(h.passive || h.no_disp);
     int bf = (adv ? qg::YB_ADV : 0) | (refr ? qg::YB_REFR : 0) | (disp ? qg::YB_DISP : 0);
     // the new B's inverse y transforms for the next diagnosis: T1 always
     // (xb_B), T3 when q_w is computed
     C* T3 = (h.feedback && !h.fixed_flow) ? c(h.T3) : nullptr;
-    K(mode == qg::MODE_STAGE ? "yb2_B_stage" : "yb2_B_final", qg::P_YB2_B,
-      Pf * cR * (((adv || refr) ? 1 : 0) + (adv ? 1 : 0) + 1 + (T3 ? 1 : 0)) +
-          Ps * cS * ((mode == qg::MODE_STAGE ? 3 : 2) + (disp ? 1 : 0)),
+    const double bbytes = Pf * cR * (((adv || refr) ? 1 : 0) + (adv ? 1 : 0) + 1 + (T3 ? 1 : 0)) +
+                          Ps * cS * ((mode == qg::MODE_STAGE ? 3 : 2) + (disp ? 1 : 0));
+    // the q update beside the B update (disjoint inputs and outputs): one
+    // launch of both (the evolving flow without vertical diffusion), or the
+    // q update on the side stream
+    if (concurrent() && qf == qg::YQ_ADV) {
+      K(mode == qg::MODE_STAGE ? "yb2_qB_stage" : "yb2_qB_final", qg::P_YB2_B, qbytes + bbytes, [&] {
+        O::yb2_qB(h.tune[qg::P_YB2_B], *G, S, mode, c(h.G1), c(h.G2), qn, c(h.rq0), qout, bf, c(h.G3), c(h.G4), AS,
+                  Bn, c(h.rB0), Bout, c(h.T1), T3, etd(), h.f / 2, flag, twy());
+      });
+      return;
+    }
+    side([&] {
+      K(mode == qg::MODE_STAGE ? "yb2_q_stage" : "yb2_q_final", qg::P_YB2_Q, qbytes,
+        [&] { O::yb2_q(h.tune[qg::P_YB2_Q], *G, S, mode, qf, c(h.G1), c(h.G2), qS, qn, c(h.rq0), qout, etd(), h.nu_z, inv_dz2, flag, twy()); });
+    });
+    K(mode == qg::MODE_STAGE ? "yb2_B_stage" : "yb2_B_final", qg::P_YB2_B, bbytes,
       [&] { O::yb2_B(h.tune[qg::P_YB2_B], *G, S, mode, bf, c(h.G3), c(h.G4), AS, Bn, c(h.rB0), Bout, c(h.T1), T3, etd(), h.f / 2, flag, twy()); });
     join();
   }
@@ -624,19 +655,14 @@ struct Engine {
     // RHS at the stage (yb2_B left T1, T3 of B_S)
     C* psiS = h.fixed_flow ? c(h.psi) : c(h.psis);
     // A_S = L+^-1 B_S beside the q_w -> q_eff -> psi_S chain (independent)
-    side([&] { wave_inv(c(h.Bs), c(h.As), nullptr); });
-    diag(c(h.qs), c(h.Bs), psiS, true, true);
-    join();
+    diag(c(h.qs), c(h.Bs), psiS, true, true, c(h.Bs), c(h.As));
     // the stage's u, v come out of the RHS passes (they are needed nowhere else)
     rhs_update(qg::MODE_FINAL, c(h.qs), psiS, c(h.As), qn, Bn, q1, B1, need_uv && !h.fixed_flow,
                chk ? h.d_flag : nullptr);
     // finalize (src/timestep.jl:300-332); yb2_B left T1, T3 of B_{n+1}.  A
-    // and C beside the diagnosis of psi and the velocity and omega passes,
-    // which read neither.
-    side([&] { wave_inv(B1, c(h.A), c(h.C)); });
-    diag(q1, B1, c(h.psi), false, true);
+    // and C beside the diagnosis of psi, which reads neither.
+    diag(q1, B1, c(h.psi), false, true, B1, c(h.A), c(h.C));
     velocities_w(c(h.psi), chk, true);
-    join();
     if (can && !prune) {
       // The step's outputs (q, B, psi, A, C) are masked and written at every
       // point; the buffers it read from are the next step's outputs, which a

@@ -120,4 +120,27 @@ void launch_thomas(const Geo& g, hipStream_t st, const void* rhs, void* out, voi
   }
 }
 
+void launch_thomas_pair(const Geo& g, hipStream_t st, bool f32, const double* al, int shape, const void* q,
+                        void* psi, const double* inv_q, const double* cp_q, unsigned long long* warn,
+                        const void* B, void* A, void* Cout, const double* inv_b, const double* cp_b) {
+  if (shape < 0) shape = scan_default_shape(g.nzc);
+  ScanArgs a0{}, a1{};
+  a0.rhs = q;
+  a0.out = psi;
+  a0.t0 = inv_q;
+  a0.t1 = cp_q;
+  a0.lv0 = al;
+  a0.lv0_stride = thomas_shifted_a_size(g.nzc);
+  a0.warn = warn;
+  a1.rhs = B;
+  a1.out = A;
+  a1.Cout = Cout;
+  a1.t0 = inv_b;
+  a1.t1 = cp_b;
+  a1.lv0 = al;
+  a1.lv0_stride = a0.lv0_stride;
+  if (Cout) launch_colscan2<SCAN_PSI, SCAN_AC>(g, st, shape, a0, a1, f32);
+  else launch_colscan2<SCAN_PSI, SCAN_A>(g, st, shape, a0, a1, f32);
+}
+
 }  // namespace qg
