@@ -25,18 +25,32 @@ namespace qg {
 // the same in every variant, so variants give bitwise identical results
 // (kTileVariants of them, qg_flags.hpp).
 
+// LDS stride of a sequence, in values: at least the padded length `base`
+// plus one, and congruent to 16/S modulo 16 when S divides 16.  A 64-bit LDS
+// access serves 16 lanes per cycle from the banks; the Y-passes put
+// consecutive threads on consecutive sequences, so a group of 16 lanes holds
+// 16/S elements of each of S sequences, and with this stride the sequences'
+// copies of one element pattern fall on distinct banks (S = 8 at stride 289,
+// 1 mod 16, runs 64-bit accesses 1.6x slower than at 290).
+constexpr int lds_stride(int base, int S) {
+  const int want = (S >= 2 && S < 16 && 16 % S == 0) ? 16 / S : 1;
+  int ld = base + 1;
+  while (ld % 16 != want) ++ld;
+  return ld;
+}
+
 template <typename R, int N, int V = 1>
 struct Tile {
   // V = 3: the alternative FFT plan (FftPlan<N, 1>) in blocks of the half
   // tile (as V = 0).
   static constexpr int PL = V == 3 ? 1 : 0;
   static constexpr int T = FftPlan<N, PL>::T;
-  static constexpr int LD = N + N / FftPlan<N, PL>::r0 + 1;  // padded sequence stride in LDS
   static constexpr int BASE = sizeof(R) == 8 ? 256 : 512;
   static constexpr int TARGET = (V == 0 || V == 3) ? BASE / 2 : (V == 1 ? BASE : 2 * BASE);
   static constexpr int S0 = TARGET / T;
   static constexpr int S1 = S0 < 1 ? 1 : (S0 > 64 ? 64 : S0);
   static constexpr int S2 = S1 * T > 512 ? (512 / T < 1 ? 1 : 512 / T) : S1;
+  static constexpr int LD = lds_stride(N + N / FftPlan<N, PL>::r0, S2);  // padded sequence stride in LDS
   // sequences that fit 64 KB of LDS beside the staged twiddle table and a
   // staged wavenumber table (stage_k)
   static constexpr int SL = (int)((65536 - TwTab<R, N, PL>::bytes() - N * (int)sizeof(double)) /
