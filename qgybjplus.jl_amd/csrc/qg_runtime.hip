@@ -127,6 +127,11 @@ struct qg_handle {
   // replay after a failing chunk), or by its bound (qg_run's chunks)
   hipGraphExec_t graph_chk[2] = {nullptr, nullptr};
   hipGraphExec_t graph_bnd[2] = {nullptr, nullptr};
+  // the same with kLongGraphSteps steps each (fewer graph launches, and
+  // fewer steps whose w passes must finish before their graph ends)
+  hipGraphExec_t graph_l[2] = {nullptr, nullptr};
+  hipGraphExec_t graph_chk_l[2] = {nullptr, nullptr};
+  hipGraphExec_t graph_bnd_l[2] = {nullptr, nullptr};
   // the state a step starts from (q, B and the diagnostics psi, A, T1, u, v
   // it reuses) at the start of a qg_run chunk; allocated by the first qg_run
   void* snap = nullptr;
@@ -198,7 +203,7 @@ struct qg_handle {
     return p;
   }
   void drop_graphs() {
-    for (hipGraphExec_t* set : {graph, graph_chk, graph_bnd})
+    for (hipGraphExec_t* set : {graph, graph_chk, graph_bnd, graph_l, graph_chk_l, graph_bnd_l})
       for (int i = 0; i < 2; ++i)
         if (set[i]) {
           (void)hipGraphExecDestroy(set[i]);
@@ -759,11 +764,17 @@ void fft2_inplace(qg_handle& h, D* f, int dir, D* out = nullptr, int levels = -1
 
 }  // namespace
 
-// `nsteps` steps of the engine: steady-state pairs (diagnostics valid, no
-// per-kernel events) replay a captured two-step graph, the others run
-// eagerly.  With chk = 1 every step is followed by the termination check of
-// the active members, inside the graph too; chk = 2 by its bound (Engine::step).
-// mark(n) follows every n steps.
+// Steps per long graph (even, so that a graph leaves the double buffer as it
+// found it).  Each graph launch costs a few microseconds of idle GPU, and the
+// last step of a graph waits for its w passes instead of overlapping them
+// with the next step.
+constexpr int kLongGraphSteps = 8;
+
+// `nsteps` steps of the engine: steady-state runs of steps (diagnostics
+// valid, no per-kernel events) replay captured graphs of kLongGraphSteps or
+// two steps, the others run eagerly.  With chk = 1 every step is followed by
+// the termination check of the active members, inside the graph too; chk = 2
+// by its bound (Engine::step).  mark(n) follows every n steps.
 template <class E, class Mark>
 void advance(qg_handle& H, E& e, int nsteps, int chk, Mark&& mark) {
   const bool use_graphs = H.graphs && !(H.prof_mode & 1);
@@ -773,7 +784,10 @@ void advance(qg_handle& H, E& e, int nsteps, int chk, Mark&& mark) {
     // an unclean state's step runs unpruned and eagerly (it makes the state clean)
     if (use_graphs && H.diag_valid && (H.clean || !(H.prunable && H.prune_enabled)) && nsteps - i >= 2) {
       const int c = H.cur;
-      hipGraphExec_t& gx = chk == 2 ? H.graph_bnd[c] : chk == 1 ? H.graph_chk[c] : H.graph[c];
+      const int gs = nsteps - i >= kLongGraphSteps ? kLongGraphSteps : 2;
+      hipGraphExec_t* set = gs == 2 ? (chk == 2 ? H.graph_bnd : chk == 1 ? H.graph_chk : H.graph)
+                                    : (chk == 2 ? H.graph_bnd_l : chk == 1 ? H.graph_chk_l : H.graph_l);
+      hipGraphExec_t& gx = set[c];
       // w passes left running by an eager step finish before the graph's
       // steps reuse their buffers
       e.join_w();
@@ -781,8 +795,7 @@ void advance(qg_handle& H, E& e, int nsteps, int chk, Mark&& mark) {
         hipGraph_t g = nullptr;
         HIPCHECK(hipStreamBeginCapture(H.st, hipStreamCaptureModeThreadLocal));
         try {
-          one();
-          one();
+          for (int s = 0; s < gs; ++s) one();
           e.join_w();  // every branch rejoins the captured stream
         } catch (...) {
           H.w_pending = false;
@@ -799,8 +812,8 @@ void advance(qg_handle& H, E& e, int nsteps, int chk, Mark&& mark) {
         }
       }
       HIPCHECK(hipGraphLaunch(gx, H.st));
-      i += 2;
-      mark(2);
+      i += gs;
+      mark(gs);
     } else {
       one();
       ++i;
