@@ -281,9 +281,11 @@ int qg_synchronize(qg_handle* h);
  * reused by later calls unless flags has QG_TUNE_FORCE.  -1 is the default
  * choice of a pass.  choices[] has QG_TUNE_PASSES entries in the order
  * ya_qw, xa_qw, ya2_qeff, yb_adv, x_c2r, xb_q, xb_B, yb2_q, yb2_B, yf_omega,
- * xf, y_r2c, y_herm, thomas_psi, thomas_A, thomas_AC, omega (xb_B also sets
- * the X-pass that advects q and B together when both are advected). */
-#define QG_TUNE_PASSES 17
+ * xf, y_r2c, y_herm, thomas_psi, thomas_A, thomas_AC, omega, thomas_pair
+ * (xb_B also sets the X-pass that advects q and B together when both are
+ * advected, yb2_B the launch of the q and B updates together; thomas_pair is
+ * the launch that solves q -> psi and B -> A (and C) together). */
+#define QG_TUNE_PASSES 18
 #define QG_TUNE_FORCE 1
 int qg_autotune(qg_handle* h, double dt, int steps, int flags);
 int qg_get_tuning(qg_handle* h, int* choices, int n);

@@ -63,7 +63,8 @@ class QgConfig(C.Structure):
 
 # tunable passes, in the order of qg_get_tuning / qg_set_tuning
 TUNE_PASS_NAMES = ("ya_qw", "xa_qw", "ya2_qeff", "yb_adv", "x_c2r", "xb_q", "xb_B", "yb2_q", "yb2_B",
-                   "yf_omega", "xf", "y_r2c", "y_herm", "thomas_psi", "thomas_A", "thomas_AC", "omega")
+                   "yf_omega", "xf", "y_r2c", "y_herm", "thomas_psi", "thomas_A", "thomas_AC", "omega",
+                   "thomas_pair")
 TUNE_PASSES = len(TUNE_PASS_NAMES)  # QG_TUNE_PASSES
 TUNE_FORCE = 1                      # QG_TUNE_FORCE
 

@@ -10,11 +10,11 @@
 //   backward  x_k  = -P_k x_{k+1}  + Q_k(x'_k)    (x_{kpad} = 0)
 //
 // A tile is QC consecutive quarter columns and their mirror images (+-kx,
-// +-ky): 4*QC grid columns, 8*QC column parts (the real and imaginary parts
-// are solved separately, as the reference does).  A block splits the depth
-// into P partitions of M levels; each thread owns one column part of one
-// partition and keeps its M values in registers.  A recurrence is evaluated
-// in three steps:
+// +-ky): 4*QC grid columns.  A block splits the depth into P partitions of M
+// levels; each thread owns one grid column of one partition and keeps its M
+// complex values in registers.  The real and imaginary parts are solved
+// separately, as the reference does, with the factors read once for both.
+// A recurrence is evaluated in three steps:
 //   1. every partition runs it locally from a zero carry, and the product of
 //      its multipliers (-G or -P) over the partition;
 //   2. the partitions' end values and products go through LDS, and each
@@ -67,17 +67,21 @@ __device__ inline QuadCol quad_col(const Geo& g, const Quarter& qr, int rc, int 
 enum : int { SCAN_PSI = 0, SCAN_A = 1, SCAN_AC = 2, SCAN_OMEGA = 3 };
 
 // Launch shape: M levels per thread, QC quarter columns per tile, P
-// partitions (P * M >= nz).  A block has P * 8 * QC threads.
+// partitions (P * M >= nz).  A block has P * 4 * QC threads.  LDS: the two
+// tables of the tile and the two level arrays (kpad levels each) and the
+// forward and backward exchange rows (re, im, product per thread).
 struct ScanShape {
   int M, QC, P;
-  int threads() const { return P * 8 * QC; }
+  int threads() const { return P * 4 * QC; }
   int kpad() const { return P * M; }
   size_t lds_bytes() const {
-    return sizeof(double) * ((size_t)2 * kpad() * QC + 2 * (size_t)kpad() + 4 * (size_t)P * 8 * QC);
+    return sizeof(double) * ((size_t)2 * kpad() * QC + 2 * (size_t)kpad() + 6 * (size_t)P * 4 * QC);
   }
 };
 
-constexpr int kScanMaxThreads = 1024;
+// Blocks of at most 512 threads: two waves per SIMD, so a thread may keep up
+// to 256 VGPRs (2 M doubles of column values plus its staged table entries).
+constexpr int kScanMaxThreads = 512;
 // Word 3 of a raw buffer resource on the gfx9 family (gfx950): dword
 // access, no format conversion.
 constexpr int kRawBufferWord3 = 0x00020000;
@@ -97,8 +101,8 @@ inline ScanShape scan_shape_of(int idx, int nz) {
   return s;
 }
 
-// Default shape for depth nz: among the 8-quarter-column tiles (64 column
-// parts, one wave per partition), M = 8 for nz <= 8, else the smallest of
+// Default shape for depth nz: among the 8-quarter-column tiles (32 columns
+// per partition), M = 8 for nz <= 8, else the smallest of
 // M = 16, 32 that fits (at most 16 partitions); else the deepest
 // 4-quarter-column shape that fits.  Short partitions keep fewer registers
 // per thread and more columns in flight.
@@ -139,24 +143,33 @@ __device__ inline void sweep_chunk_fence(int u) {
 
 template <int M, int QC>
 struct ScanBounds {
-  static constexpr int TP = 8 * QC;
+  static constexpr int TP = 4 * QC;
   static constexpr int MAXP = kScanMaxThreads / TP < 16 ? kScanMaxThreads / TP : 16;
   static constexpr int MAXT = MAXP * TP;
 };
 
-// Loads and stores of one column part of storage type T as FP64.
+// Loads and stores of one column's complex value (both parts) of storage
+// type T as FP64.
 template <typename T>
-__device__ inline double scan_load(__amdgpu_buffer_rsrc_t r, unsigned off, int soff) {
-  if constexpr (sizeof(T) == 8) return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
-  else return (double)__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
+__device__ inline void scan_load2(__amdgpu_buffer_rsrc_t r, unsigned off, int soff, double& re, double& im) {
+  if constexpr (sizeof(T) == 8) {
+    const double2 v = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, soff, 0));
+    re = v.x;
+    im = v.y;
+  } else {
+    const float2 v = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
+    re = (double)v.x;
+    im = (double)v.y;
+  }
 }
 template <typename T>
-__device__ inline void scan_store(double v, __amdgpu_buffer_rsrc_t r, unsigned off, int soff) {
+__device__ inline void scan_store2(double re, double im, __amdgpu_buffer_rsrc_t r, unsigned off, int soff) {
   if constexpr (sizeof(T) == 8) {
-    using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, off, soff, 0);
+    using V4 = decltype(__builtin_amdgcn_raw_buffer_load_b128(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V4, make_double2(re, im)), r, off, soff, 0);
   } else {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, (float)v), r, off, soff, 0);
+    using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, make_float2((float)re, (float)im)), r, off, soff, 0);
   }
 }
 
@@ -173,14 +186,14 @@ inline int scan_tiles(const Geo& g, int QC) {
 template <int OP, int M, int QC, typename T>
 __device__ inline void colscan_block(const Geo& g, const Quarter& qr, const ScanArgs& a, const Blk blk,
                                      double* lds) {
-  constexpr int TP = 8 * QC;
+  constexpr int TP = 4 * QC;
   constexpr int R = (M * QC + TP - 1) / TP;  // table entries staged per thread
   constexpr int LR = (M + TP - 1) / TP;      // level entries staged per thread
   const int nz = g.nzc;
   const int P = blockDim.x / TP;
   const int kpad = P * M;
   const int w = threadIdx.x / TP, t = threadIdx.x % TP;
-  const int part = t & 1, cs = t >> 1, j = cs % QC, mir = cs / QC;
+  const int j = t % QC, mir = t / QC;
   const int rplane = qr.size();
   int rc0 = blk.x * QC, row = -1;
   if (g.pruned()) {  // uniform
@@ -197,20 +210,19 @@ __device__ inline void colscan_block(const Geo& g, const Quarter& qr, const Scan
   const double* t1 = a.t1 + mem * nz * rplane;
   const double* lv0 = a.lv0 + mem * a.lv0_stride;
   const double* lv1 = (OP == SCAN_OMEGA) ? a.lv1 + mem * a.lv1_stride : nullptr;
-
   double* T0 = lds;               // [kpad][QC]
   double* T1 = T0 + kpad * QC;    // [kpad][QC]
   double* L0 = T1 + kpad * QC;    // [kpad]
   double* L1 = L0 + kpad;         // [kpad]
-  double* XF = L1 + kpad;         // [2][P][TP]: forward end values, products
-  double* XB = XF + 2 * P * TP;   // [2][P][TP]: backward end values, products
+  double* XF = L1 + kpad;         // [3][P][TP]: forward end values (re, im), products
+  double* XB = XF + 3 * P * TP;   // [3][P][TP]: backward end values (re, im), products
 
   const QuadCol q = quad_col(g, qr, rc0 + j, mir, row);
-  // This thread's column part within a level, in doubles.  With 64 column
-  // parts per partition a partition is one wave, so its first level is
-  // wave-uniform and every level's address is a scalar base plus this offset.
+  // This thread's column within a level, in bytes.  With 64 columns per
+  // partition a partition is one wave, so its first level is wave-uniform
+  // and every level's address is a scalar base plus this offset.
   constexpr bool UNI = TP % 64 == 0;
-  const unsigned boff = (unsigned)sizeof(T) * (2u * (q.active ? (unsigned)(q.y * g.nx + q.x) : 0u) + (unsigned)part);
+  const unsigned boff = (unsigned)sizeof(T) * 2u * (q.active ? (unsigned)(q.y * g.nx + q.x) : 0u);
   int k0 = w * M;
   if constexpr (UNI) k0 = __builtin_amdgcn_readfirstlane(k0);
   const size_t lvlb = plane * 2 * sizeof(T);  // bytes per level
@@ -246,14 +258,20 @@ __device__ inline void colscan_block(const Geo& g, const Quarter& qr, const Scan
     l0[r] = lv0[e];
     l1[r] = (OP == SCAN_OMEGA) ? lv1[e] : 0.0;
   }
-  double y[M];
+  // the real and imaginary parts of each level (solved with the same
+  // factors, each as the reference solves its parts)
+  double yr[M], yi[M];
   if constexpr (UNI) {
     const auto rs = rsrc(rhs);
 #pragma unroll
-    for (int u = 0; u < M; ++u) y[u] = scan_load<T>(rs, boff, u * (int)lvlb);
+    for (int u = 0; u < M; ++u) scan_load2<T>(rs, boff, u * (int)lvlb, yr[u], yi[u]);
   } else {
 #pragma unroll
-    for (int u = 0; u < M; ++u) y[u] = (double)*at_level(rhs, min(k0 + u, nz - 1));
+    for (int u = 0; u < M; ++u) {
+      const T* p = at_level(rhs, min(k0 + u, nz - 1));
+      yr[u] = (double)p[0];
+      yi[u] = (double)p[1];
+    }
   }
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -300,83 +318,109 @@ __device__ inline void colscan_block(const Geo& g, const Quarter& qr, const Scan
     double m = 0.0;
 #pragma unroll
     for (int u = 0; u < M; ++u) {
-      const double other = __shfl_xor(y[u], 1);
-      const double av = hypot(y[u], other);
+      const double av = hypot(yr[u], yi[u]);
       if (k0 + u < nz) m = av > m ? av : m;
     }
-    if (mean_col && part == 0 && m > 1e-10) atomicMax(a.warn, (unsigned long long)__double_as_longlong(m));
+    if (mean_col && m > 1e-10) atomicMax(a.warn, (unsigned long long)__double_as_longlong(m));
   }
 
   // One-level columns need no special case: their recorded factor is the
   // reciprocal of the operator's only entry (0 where kh = 0) and the omega
   // equation has no interior row.
-  double Y = 0.0;  // final value of the level above this partition
+  double Yr = 0.0, Yi = 0.0;  // final value of the level above this partition
   {
     // forward recurrence, local
-    double xp = 0.0, phi = 1.0;
+    double xr = 0.0, xi = 0.0, phi = 1.0;
 #pragma unroll
     for (int u = 0; u < M; ++u) {
       const int k = k0 + u;
       const double gk = G(k);
-      xp = fma(-gk, xp, H(k, y[u]));
-      y[u] = xp;
+      xr = fma(-gk, xr, H(k, yr[u]));
+      xi = fma(-gk, xi, H(k, yi[u]));
+      yr[u] = xr;
+      yi[u] = xi;
       phi = phi * -gk;
       sweep_chunk_fence(u);
     }
-    XF[w * TP + t] = xp;
-    XF[(P + w) * TP + t] = phi;
+    XF[w * TP + t] = xr;
+    XF[(P + w) * TP + t] = xi;
+    XF[(2 * P + w) * TP + t] = phi;
     __syncthreads();
     if (w > 0) {
-      double X = 0.0;  // x' of the level below this partition
-      for (int p = 0; p < w; ++p) X = fma(XF[(P + p) * TP + t], X, XF[p * TP + t]);
+      double Xr = 0.0, Xi = 0.0;  // x' of the level below this partition
+      for (int p = 0; p < w; ++p) {
+        const double f = XF[(2 * P + p) * TP + t];
+        Xr = fma(f, Xr, XF[p * TP + t]);
+        Xi = fma(f, Xi, XF[(P + p) * TP + t]);
+      }
       double ph = 1.0;
 #pragma unroll
       for (int u = 0; u < M; ++u) {
         ph = ph * -G(k0 + u);
-        y[u] = fma(ph, X, y[u]);
+        yr[u] = fma(ph, Xr, yr[u]);
+        yi[u] = fma(ph, Xi, yi[u]);
         sweep_chunk_fence(u);
       }
     }
     // backward recurrence, local
-    double xn = 0.0, psi = 1.0;
+    double nr = 0.0, ni = 0.0, psi = 1.0;
 #pragma unroll
     for (int u = M - 1; u >= 0; --u) {
       const int k = k0 + u;
       const double pk = Pm(k);
-      xn = fma(-pk, xn, Q(k, y[u]));
-      y[u] = xn;
+      nr = fma(-pk, nr, Q(k, yr[u]));
+      ni = fma(-pk, ni, Q(k, yi[u]));
+      yr[u] = nr;
+      yi[u] = ni;
       psi = psi * -pk;
       sweep_chunk_fence(M - 1 - u);
     }
-    XB[w * TP + t] = xn;
-    XB[(P + w) * TP + t] = psi;
+    XB[w * TP + t] = nr;
+    XB[(P + w) * TP + t] = ni;
+    XB[(2 * P + w) * TP + t] = psi;
     __syncthreads();
     if (w < P - 1) {
-      for (int p = P - 1; p > w; --p) Y = fma(XB[(P + p) * TP + t], Y, XB[p * TP + t]);
+      for (int p = P - 1; p > w; --p) {
+        const double f = XB[(2 * P + p) * TP + t];
+        Yr = fma(f, Yr, XB[p * TP + t]);
+        Yi = fma(f, Yi, XB[(P + p) * TP + t]);
+      }
       double ps = 1.0;
 #pragma unroll
       for (int u = M - 1; u >= 0; --u) {
         ps = ps * -Pm(k0 + u);
-        y[u] = fma(ps, Y, y[u]);
+        yr[u] = fma(ps, Yr, yr[u]);
+        yi[u] = fma(ps, Yi, yi[u]);
         sweep_chunk_fence(M - 1 - u);
       }
     }
     // B -> A: remove the vertical mean of the kh = 0 column
     // (src/elliptic.jl:930-945).  XF is free after the second barrier.
     if ((OP == SCAN_A || OP == SCAN_AC) && blk.x == 0) {
-      double s = 0.0;
+      double sr = 0.0, si = 0.0;
 #pragma unroll
       for (int u = 0; u < M; ++u)
-        if (k0 + u < nz) s = s + y[u];
-      XF[w * TP + t] = s;
+        if (k0 + u < nz) {
+          sr = sr + yr[u];
+          si = si + yi[u];
+        }
+      XF[w * TP + t] = sr;
+      XF[(P + w) * TP + t] = si;
       __syncthreads();
       if (mean_col) {
-        double tot = 0.0;
-        for (int p = 0; p < P; ++p) tot = tot + XF[p * TP + t];
-        const double mean = tot / nz;
+        double tr = 0.0, ti = 0.0;
+        for (int p = 0; p < P; ++p) {
+          tr = tr + XF[p * TP + t];
+          ti = ti + XF[(P + p) * TP + t];
+        }
+        const double mr = tr / nz, mi = ti / nz;
 #pragma unroll
-        for (int u = 0; u < M; ++u) y[u] = y[u] - mean;
-        Y = Y - mean;
+        for (int u = 0; u < M; ++u) {
+          yr[u] = yr[u] - mr;
+          yi[u] = yi[u] - mi;
+        }
+        Yr = Yr - mr;
+        Yi = Yi - mi;
       }
     }
   }
@@ -391,18 +435,25 @@ __device__ inline void colscan_block(const Geo& g, const Quarter& qr, const Scan
   for (int u = 0; u < M; ++u) {
     const int k = k0 + u;
     if (k < nz) {
-      const double r = zero_col ? 0.0 : y[u];
-      double c = 0.0;
+      const double rr = zero_col ? 0.0 : yr[u], ri = zero_col ? 0.0 : yi[u];
+      double cr = 0.0, ci = 0.0;
       if constexpr (OP == SCAN_AC) {
-        const double up = u + 1 < M ? y[u + 1] : Y;
-        c = k < nz - 1 ? (up - r) / g.dz : 0.0;
+        const double ur = u + 1 < M ? yr[u + 1] : Yr, ui = u + 1 < M ? yi[u + 1] : Yi;
+        cr = k < nz - 1 ? (ur - rr) / g.dz : 0.0;
+        ci = k < nz - 1 ? (ui - ri) / g.dz : 0.0;
       }
       if constexpr (UNI) {
-        scan_store<T>(r, ro, boff, u * (int)lvlb);
-        if constexpr (OP == SCAN_AC) scan_store<T>(c, rcz, boff, u * (int)lvlb);
+        scan_store2<T>(rr, ri, ro, boff, u * (int)lvlb);
+        if constexpr (OP == SCAN_AC) scan_store2<T>(cr, ci, rcz, boff, u * (int)lvlb);
       } else {
-        *at_level(out, k) = (T)r;
-        if constexpr (OP == SCAN_AC) *at_level(Co, k) = (T)c;
+        T* p = at_level(out, k);
+        p[0] = (T)rr;
+        p[1] = (T)ri;
+        if constexpr (OP == SCAN_AC) {
+          T* pc = at_level(Co, k);
+          pc[0] = (T)cr;
+          pc[1] = (T)ci;
+        }
       }
     }
   }

@@ -45,7 +45,7 @@ namespace qg {
 enum Pass : int {
   P_YA_QW, P_XA_QW, P_YA2_QEFF, P_YB_ADV, P_X_C2R, P_XB_Q, P_XB_B, P_YB2_Q, P_YB2_B, P_YF_OMEGA, P_XF,
   P_Y_R2C, P_Y_HERM, P_FFT_END,
-  P_THOMAS_PSI = P_FFT_END, P_THOMAS_A, P_THOMAS_AC, P_OMEGA, P_END
+  P_THOMAS_PSI = P_FFT_END, P_THOMAS_A, P_THOMAS_AC, P_OMEGA, P_THOMAS_PAIR, P_END
 };
 }  // namespace qg
 
@@ -317,14 +317,13 @@ void solve_A(qg_handle& h, const qg::Geo& g, hipStream_t st, const void* B, void
                     h.tune[C ? qg::P_THOMAS_AC : qg::P_THOMAS_A]);
 }
 
-// solve_psi and solve_A (with C when C is set) in one launch, at the psi
-// solve's launch shape: the same values, side by side without a second
-// stream.
+// solve_psi and solve_A (with C when C is set) in one launch, at the pair's
+// own launch shape: the same values, side by side without a second stream.
 void solve_psi_A(qg_handle& h, const qg::Geo& g, hipStream_t st, const void* q, void* psi, const void* B, void* A,
                  void* C) {
   if (h.sing_psi[0]) raise_singular(h.sing_psi);
   if (h.sing_A[0]) raise_singular(h.sing_A);
-  qg::launch_thomas_pair(g, st, h.f32(), h.d_al, h.tune[qg::P_THOMAS_PSI], q, psi, h.th_inv[0], h.th_cp[0],
+  qg::launch_thomas_pair(g, st, h.f32(), h.d_al, h.tune[qg::P_THOMAS_PAIR], q, psi, h.th_inv[0], h.th_cp[0],
                          h.d_warn, B, A, C, h.th_inv[1], h.th_cp[1]);
 }
 
@@ -1963,6 +1962,22 @@ void autotune(qg_handle& H, double dt, int steps) {
       H.pool_used = 0;
       H.prof_mode = 1;
       for (int i = 0; i < steps; ++i) e.step();
+      // The paired psi and A solve (the step's launch when concurrency is
+      // on; per-launch profiling runs the two solves apart) at this
+      // candidate's shape, timed on its own from the current q and B into
+      // the stage buffers (restored state is not touched; no mean warning).
+      if (!H.fixed_flow && !(H.passive || H.no_disp) && H.nzc > 1) {
+        const qg::Geo& g = (H.prunable && H.prune_enabled && H.clean) ? H.gp : H.g;
+        for (int i = 0; i < 4; ++i) {
+          hipEvent_t a = H.event(), b = H.event();
+          HIPCHECK(hipEventRecord(a, H.st));
+          qg::launch_thomas_pair(g, H.st, H.f32(), H.d_al, H.tune[qg::P_THOMAS_PAIR], H.q[H.cur], H.psis,
+                                 H.th_inv[0], H.th_cp[0], nullptr, H.B[H.cur], H.As, nullptr, H.th_inv[1],
+                                 H.th_cp[1]);
+          HIPCHECK(hipEventRecord(b, H.st));
+          H.prof.push_back({"thomas_pair", qg::P_THOMAS_PAIR, 0.0, a, b});
+        }
+      }
       H.prof_mode = 0;
     });
     HIPCHECK(hipStreamSynchronize(H.st));

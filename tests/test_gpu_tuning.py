@@ -49,7 +49,7 @@ def _rel(a, b):
 
 FFT = ("ya_qw", "xa_qw", "ya2_qeff", "yb_adv", "x_c2r", "xb_q", "xb_B", "yb2_q", "yb2_B", "yf_omega", "xf",
        "y_r2c", "y_herm")
-COLUMN = ("thomas_psi", "thomas_A", "thomas_AC", "omega")
+COLUMN = ("thomas_psi", "thomas_A", "thomas_AC", "omega", "thomas_pair")
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
