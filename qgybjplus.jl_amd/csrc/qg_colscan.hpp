@@ -431,6 +431,7 @@ __device__ inline void colscan_block(const Geo& g, const Quarter& qr, const Scan
   T* Co = (OP == SCAN_AC) ? static_cast<T*>(a.Cout) + 2 * mem * nz * plane : nullptr;
   const auto ro = rsrc(out);
   const auto rcz = rsrc((OP == SCAN_AC) ? Co : out);
+  const double idz = 1.0 / g.dz;  // C = dA/dz by a multiplication per level
 #pragma unroll
   for (int u = 0; u < M; ++u) {
     const int k = k0 + u;
@@ -439,8 +440,8 @@ __device__ inline void colscan_block(const Geo& g, const Quarter& qr, const Scan
       double cr = 0.0, ci = 0.0;
       if constexpr (OP == SCAN_AC) {
         const double ur = u + 1 < M ? yr[u + 1] : Yr, ui = u + 1 < M ? yi[u + 1] : Yi;
-        cr = k < nz - 1 ? (ur - rr) / g.dz : 0.0;
-        ci = k < nz - 1 ? (ui - ri) / g.dz : 0.0;
+        cr = k < nz - 1 ? (ur - rr) * idz : 0.0;
+        ci = k < nz - 1 ? (ui - ri) * idz : 0.0;
       }
       if constexpr (UNI) {
         scan_store2<T>(rr, ri, ro, boff, u * (int)lvlb);

@@ -1263,8 +1263,10 @@ k_yf_omega(Geo g, const cplx<R>* __restrict__ psi, cplx<R>* __restrict__ Pbx, cp
   // the regular blocks cover every column once (the Nyquist blocks repeat x = nx/2)
   if (CHK == 2 && !nyq) fold_block_sum<TL::THREADS>(bsum, lsum + z);  // uniform
   // psi_z = (psi(zu) - psi(z)) / dz (exactly 0 at the top, where zu = z) and
-  // psi-bar = (psi(zu) + psi(z)) / 2 (psi itself at the top)
-  auto dzf = [&](cplx<R> u1, cplx<R> u0) { return mk<R>((u1.re - u0.re) / rdz, (u1.im - u0.im) / rdz); };
+  // psi-bar = (psi(zu) + psi(z)) / 2 (psi itself at the top); the division
+  // is a multiplication by 1/dz (a division is a full sequence per value)
+  const R idz = (R)1 / rdz;
+  auto dzf = [&](cplx<R> u1, cplx<R> u0) { return mk<R>((u1.re - u0.re) * idz, (u1.im - u0.im) * idz); };
   auto avg = [&](cplx<R> u1, cplx<R> u0) { return scale(u1 + u0, (R)0.5); };
   const double* __restrict__ gky = stage_k<NY>(g.ky);
   // outputs: 0 i kx psi_z, 1 i ky psi_z, 2 -i kx kh2 psi-bar, 3 -i ky kh2 psi-bar,
