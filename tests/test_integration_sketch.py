@@ -1,4 +1,4 @@
-"""The Julia binding sketched in INTEGRATION.md against the C ABI: every
+"""The Julia binding sketched in the integration notes against the C ABI: every
 ``ccall`` names a function include/qgybj.h declares, with the same number of
 arguments and matching C types, and the ``QGConfig`` mirror has the fields of
 ``struct qg_config`` in the same order with the same types.  (No Julia
@@ -13,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DOC = os.path.join(ROOT, "INTEGRATION.md")
 
 # the document is not shipped with every copy of the tree the tests run from
-pytestmark = pytest.mark.skipif(not os.path.exists(DOC), reason="INTEGRATION.md not present")
+pytestmark = pytest.mark.skipif(not os.path.exists(DOC), reason="the integration notes are not present")
 
 
 def _julia_block():
