@@ -5,7 +5,9 @@ exception semantics:
   (src/elliptic.jl:1177-1193) from q -> psi, B -> A and the step;
 * an exactly zero dgtsv pivot of the omega equation raises "... gtsv failed
   for vertical velocity" (src/operators.jl:447-464) from
-  compute_velocities!(compute_w = true) and from the step's finalize.
+  compute_velocities!(compute_w = true) and from the step's finalize;
+* a horizontal length without a compiled transform plan is refused when the
+  model is created (FFTW would take it; the library names what it takes).
 
 Both conditions come from the inputs (the coefficients the host hands over);
 every kernel runs to its normal end and the failure is read back on the
@@ -78,3 +80,22 @@ def test_gtsv_zero_pivot_raises(qg):
         h.step(0.1, 1)
     assert ei.value.code == QG_ERR_GTSV
     h.close()
+
+
+@pytest.mark.parametrize("size", [(14, 16, 4), (16, 21, 4), (1280, 8, 2), (8, 1536, 2), (11, 11, 2)])
+def test_unsupported_transform_length_raises(qg, size):
+    """Lengths with a prime factor above 5 (14, 21, 11) or above 1024 (1280,
+    1536) have no compiled plan: creating the model fails with
+    QG_ERR_UNSUPPORTED (NotImplementedError in Python), the message names
+    the supported lengths, and a supported grid works."""
+    print(f"case unsupported length {size}", flush=True)
+    grid = qg.RectilinearGrid(size=size, extent=(500e3, 500e3, 4000.0))
+    with pytest.raises(NotImplementedError) as ei:
+        qg.QGYBJModel(grid=grid, coriolis=qg.FPlane(1e-4), stratification=qg.ConstantStratification(1e-4))
+    msg = str(ei.value)
+    assert "1000" in msg and "1024" in msg and "15" in msg, msg
+    ok = qg.RectilinearGrid(size=(15, 16, size[2]), extent=(500e3, 500e3, 4000.0))
+    m = qg.QGYBJModel(grid=ok, coriolis=qg.FPlane(1e-4), stratification=qg.ConstantStratification(1e-4))
+    qg.step_(m, qg.ExponentialRungeKutta2(dt=60.0), 2)
+    assert np.all(np.isfinite(m.fields.q))
+    m.finalize()
