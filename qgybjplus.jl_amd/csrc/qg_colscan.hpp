@@ -485,6 +485,18 @@ k_colscan2(Geo g, Quarter qr, ScanArgs a0, ScanArgs a1, int n0) {
   }
 }
 
+// Default shape of the paired solve (k_colscan2, twice the blocks of one
+// solve): the 16-quarter-column tiles (one wave per partition, so each
+// level's address is a scalar base plus a lane offset) at the smallest M
+// that fits; else the single solve's default.
+inline int scan_pair_default_shape(int nz) {
+  for (int i = 0; i < kScanShapeCount; ++i) {
+    ScanShape s = scan_shape_of(i, nz);
+    if (s.QC == 16 && scan_shape_fits(s)) return i;
+  }
+  return scan_default_shape(nz);
+}
+
 // Host side: launch OP with compiled shape `idx` (see kScanShapes) on fields
 // stored in FP32 (f32) or FP64.
 template <int OP>

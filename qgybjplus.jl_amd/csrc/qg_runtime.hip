@@ -1968,7 +1968,7 @@ void autotune(qg_handle& H, double dt, int steps) {
       // the stage buffers (restored state is not touched; no mean warning).
       if (!H.fixed_flow && !(H.passive || H.no_disp) && H.nzc > 1) {
         const qg::Geo& g = (H.prunable && H.prune_enabled && H.clean) ? H.gp : H.g;
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < 8; ++i) {
           hipEvent_t a = H.event(), b = H.event();
           HIPCHECK(hipEventRecord(a, H.st));
           qg::launch_thomas_pair(g, H.st, H.f32(), H.d_al, H.tune[qg::P_THOMAS_PAIR], H.q[H.cur], H.psis,
@@ -1996,7 +1996,9 @@ void autotune(qg_handle& H, double dt, int steps) {
       std::vector<float>& v = ms[p];
       std::nth_element(v.begin(), v.begin() + v.size() / 2, v.end());
       const double med = v[v.size() / 2];
-      const int def = p < qg::P_FFT_END ? 1 : qg::scan_default_shape(H.nzc);
+      const int def = p < qg::P_FFT_END              ? 1
+                      : p == qg::P_THOMAS_PAIR ? qg::scan_pair_default_shape(H.nzc)
+                                               : qg::scan_default_shape(H.nzc);
       if (H.tune[p] == def) dflt[p] = med;
       if (med < best[p]) {
         best[p] = med;

@@ -123,7 +123,7 @@ void launch_thomas(const Geo& g, hipStream_t st, const void* rhs, void* out, voi
 void launch_thomas_pair(const Geo& g, hipStream_t st, bool f32, const double* al, int shape, const void* q,
                         void* psi, const double* inv_q, const double* cp_q, unsigned long long* warn,
                         const void* B, void* A, void* Cout, const double* inv_b, const double* cp_b) {
-  if (shape < 0) shape = scan_default_shape(g.nzc);
+  if (shape < 0) shape = scan_pair_default_shape(g.nzc);
   ScanArgs a0{}, a1{};
   a0.rhs = q;
   a0.out = psi;
