@@ -767,7 +767,7 @@ void fft2_inplace(qg_handle& h, D* f, int dir, D* out = nullptr, int levels = -1
 // found it).  Each graph launch costs a few microseconds of idle GPU, and the
 // last step of a graph waits for its w passes instead of overlapping them
 // with the next step.
-constexpr int kLongGraphSteps = 8;
+constexpr int kLongGraphSteps = 16;
 
 // `nsteps` steps of the engine: steady-state runs of steps (diagnostics
 // valid, no per-kernel events) replay captured graphs of kLongGraphSteps or

@@ -49,11 +49,11 @@ def test_graph_replays_match_eager_steps(qg, precision):
     for m in (g, e):
         qg.set_(m, psi=qg.FieldArray(psi), waves=qg.FieldArray(B))
     ts = qg.ExponentialRungeKutta2(dt=600.0)
-    # pairs replay the two-step graph and runs of 8 the long graph (9: one
-    # long graph and an eager step; 19: two long graphs, a pair and an eager
+    # pairs replay the two-step graph and runs of 16 the long graph (17: one
+    # long graph and an eager step; 35: two long graphs, a pair and an eager
     # step), odd counts end with an eager step, a new dt rewrites the ETD
     # parameter block in place, set_ invalidates the diagnostics
-    plan = [2, 2, 3, 4, 1, 9, ("dt", 300.0), 4, 19, ("set", 5), 6, 2, 8]
+    plan = [2, 2, 3, 4, 1, 17, ("dt", 300.0), 4, 35, ("set", 5), 6, 2, 16]
     for item in plan:
         if isinstance(item, tuple) and item[0] == "dt":
             ts = qg.ExponentialRungeKutta2(dt=item[1])
@@ -79,7 +79,7 @@ def test_graph_replays_match_eager_in_ensemble(qg):
         for m in (g, e):
             m.set_member(i, psi=qg.FieldArray(psi), waves=qg.FieldArray(B))
     ts = qg.ExponentialRungeKutta2(dt=600.0)
-    for n in (2, 4, 2, 8, 10):
+    for n in (2, 4, 2, 16, 18):
         qg.step_(g, ts, n)
         qg.step_(e, ts, n)
         for i in range(3):
