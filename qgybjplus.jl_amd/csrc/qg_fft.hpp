@@ -414,6 +414,7 @@ struct TwTab {
         }
         s[e] = tw[m];
       }
+      __syncthreads();  // the table is shared by every sequence of the block
       tab = s;
     }
   }
@@ -438,7 +439,11 @@ __device__ inline cplx<R> twmul(cplx<R> v, cplx<R> w) {
   return cmul(v, w);
 }
 
-template <typename R, int N, int DIR, int PL = 0>
+// WAVE: the T lanes of every sequence lie in one wave (a team of T
+// consecutive threads, T dividing 64), so the stages' LDS exchanges need only
+// the wave's own order (a fence and a wave barrier) instead of a block
+// barrier, and the block's waves run through their transforms independently.
+template <typename R, int N, int DIR, int PL = 0, bool WAVE = false>
 struct Fft {
   using P = FftPlan<N, PL>;
   static constexpr int R0 = P::r0;
@@ -456,8 +461,22 @@ struct Fft {
   __device__ static inline int elem(int lane, int it, int t) { return lane + it * T + t * NB; }
   __device__ static inline bool valid(int lane, int it) { return (NB % T == 0) || (lane + it * T < NB); }
 
+  static_assert(!WAVE || 64 % P::T == 0, "a sequence of a wave-synchronized transform must lie in one wave");
+  // Orders this sequence's LDS accesses before and after it: a block barrier,
+  // or with WAVE the wave's fence and barrier.
+  __device__ static inline void sync() {
+    if constexpr (WAVE) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+      __syncthreads();
+    }
+  }
+
   // s: this sequence's LDS (LDSZ elements).  Contains barriers: every thread
-  // of the block must call it the same number of times.
+  // of the block (with WAVE: of the wave) must call it the same number of
+  // times.
   template <class TW>
   __device__ static inline void run(cplx<R> (&v)[IT][R0], cplx<R>* s, int lane, const TW& w) {
     if constexpr (P::S == 1) {
@@ -466,7 +485,7 @@ struct Fft {
         if (valid(lane, it)) Dft<R0, DIR>::run(v[it]);
     } else {
       // first stage: registers -> LDS
-      __syncthreads();
+      sync();
 #pragma unroll
       for (int it = 0; it < IT; ++it) {
         if (valid(lane, it)) {
@@ -476,7 +495,7 @@ struct Fft {
           for (int t = 0; t < R0; ++t) s[pad(i * R0 + t)] = v[it][t];
         }
       }
-      __syncthreads();
+      sync();
       if constexpr (P::S == 3) {
         constexpr int NB1 = N / R1;
         constexpr int IT1 = (NB1 + T - 1) / T;
@@ -492,7 +511,7 @@ struct Fft {
             for (int t = 1; t < R1; ++t) wm[it][t] = w.mid(it, t, i % R0);
           }
         }
-        __syncthreads();
+        sync();
 #pragma unroll
         for (int it = 0; it < IT1; ++it) {
           int i = lane + it * T;
@@ -506,7 +525,7 @@ struct Fft {
             for (int t = 0; t < R1; ++t) s[pad(j + t * R0)] = u[it][t];
           }
         }
-        __syncthreads();
+        sync();
       }
       // last stage: LDS -> registers, span NB = N/R0, k = i
 #pragma unroll

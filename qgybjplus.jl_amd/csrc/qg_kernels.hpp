@@ -58,6 +58,9 @@ struct Tile {
   static constexpr int S = S2 > SL ? (SL < 1 ? 1 : SL) : S2;
   static constexpr int THREADS = S * T;
   static constexpr int NN = N;
+  // the X-passes put each sequence on T consecutive threads: within one
+  // wave when T divides 64 (their transforms then synchronize per wave)
+  static constexpr bool XWAVE = 64 % T == 0;
 };
 
 template <typename R>
@@ -227,8 +230,8 @@ __global__ void __launch_bounds__((Tile<R, NX, V>::THREADS))
 k_xa_qw(Geo g, const cplx<R>* __restrict__ T1, const cplx<R>* __restrict__ T3,
         cplx<R>* __restrict__ Qa, cplx<R>* __restrict__ Qb, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NX, V>;
-  using FI = Fft<R, NX, +1, TL::PL>;
-  using FF = Fft<R, NX, -1, TL::PL>;
+  using FI = Fft<R, NX, +1, TL::PL, TL::XWAVE>;
+  using FF = Fft<R, NX, -1, TL::PL, TL::XWAVE>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
   TwTab<R, TL::NN, TL::PL> twr;
@@ -498,7 +501,7 @@ __global__ void __launch_bounds__((Tile<R, NX, V>::THREADS))
 k_x_c2r(Geo g, int nfields, const cplx<R>* __restrict__ P0, const cplx<R>* __restrict__ P1,
         R* __restrict__ o0, R* __restrict__ o1, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NX, V>;
-  using FI = Fft<R, NX, +1, TL::PL>;
+  using FI = Fft<R, NX, +1, TL::PL, TL::XWAVE>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
   TwTab<R, TL::NN, TL::PL> twr;
@@ -535,8 +538,8 @@ k_xb_q(Geo g, const cplx<R>* __restrict__ Pq, const cplx<R>* __restrict__ Pu, co
        R* __restrict__ u, R* __restrict__ vv, cplx<R>* __restrict__ G1, cplx<R>* __restrict__ G2,
        const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NX, V>;
-  using FI = Fft<R, NX, +1, TL::PL>;
-  using FF = Fft<R, NX, -1, TL::PL>;
+  using FI = Fft<R, NX, +1, TL::PL, TL::XWAVE>;
+  using FF = Fft<R, NX, -1, TL::PL, TL::XWAVE>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
   TwTab<R, TL::NN, TL::PL> twr;
@@ -588,13 +591,13 @@ k_xb_q(Geo g, const cplx<R>* __restrict__ Pq, const cplx<R>* __restrict__ Pu, co
       }
   }
   FF::run(v, sl, lane, twr);
-  __syncthreads();  // the last stage's LDS reads are done
+  FI::sync();  // the last stage's LDS reads are done
 #pragma unroll
   for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
     for (int t = 0; t < FI::R0; ++t)
       if (FI::valid(lane, it)) sl[FI::pad(FI::elem(lane, it, t))] = v[it][t];
-  __syncthreads();
+  FI::sync();
 #pragma unroll
   for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
@@ -619,8 +622,8 @@ k_xb_B(Geo g, int flags, const cplx<R>* __restrict__ T1, const cplx<R>* __restri
        const R* __restrict__ u, const R* __restrict__ vv, cplx<R>* __restrict__ G3,
        cplx<R>* __restrict__ G4, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NX, V>;
-  using FI = Fft<R, NX, +1, TL::PL>;
-  using FF = Fft<R, NX, -1, TL::PL>;
+  using FI = Fft<R, NX, +1, TL::PL, TL::XWAVE>;
+  using FF = Fft<R, NX, -1, TL::PL, TL::XWAVE>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
   TwTab<R, TL::NN, TL::PL> twr;
@@ -699,8 +702,8 @@ k_xb_all(Geo g, const cplx<R>* __restrict__ Pq, const cplx<R>* __restrict__ Pz, 
          const R* __restrict__ vv, cplx<R>* __restrict__ G1, cplx<R>* __restrict__ G2, cplx<R>* __restrict__ G3,
          cplx<R>* __restrict__ G4, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NX, V>;
-  using FI = Fft<R, NX, +1, TL::PL>;
-  using FF = Fft<R, NX, -1, TL::PL>;
+  using FI = Fft<R, NX, +1, TL::PL, TL::XWAVE>;
+  using FF = Fft<R, NX, -1, TL::PL, TL::XWAVE>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
   TwTab<R, TL::NN, TL::PL, sizeof(R) == 8> twr;  // FP64: register twiddles (measured faster)
@@ -759,13 +762,13 @@ k_xb_all(Geo g, const cplx<R>* __restrict__ Pq, const cplx<R>* __restrict__ Pz, 
       v[it][t] = mk<R>(ur[it][t] * qv, vr[it][t] * qv);
     }
   FF::run(v, sl, lane, twr);
-  __syncthreads();  // the last stage's LDS reads are done
+  FI::sync();  // the last stage's LDS reads are done
 #pragma unroll
   for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
     for (int t = 0; t < FI::R0; ++t)
       if (FI::valid(lane, it)) sl[FI::pad(FI::elem(lane, it, t))] = v[it][t];
-  __syncthreads();
+  FI::sync();
 #pragma unroll
   for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
@@ -1361,8 +1364,8 @@ k_xf(Geo g, int want_omega, const cplx<R>* __restrict__ Pu, const cplx<R>* __res
      const cplx<R>* __restrict__ Ppsi, unsigned long long* __restrict__ absmax,
      double* __restrict__ lsum, int nlev, const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NX, V>;
-  using FI = Fft<R, NX, +1, TL::PL>;
-  using FF = Fft<R, NX, -1, TL::PL>;
+  using FI = Fft<R, NX, +1, TL::PL, TL::XWAVE>;
+  using FF = Fft<R, NX, -1, TL::PL, TL::XWAVE>;
   if (lsum && blockIdx.x == 0) {  // uniform
     double m = 0.0;
     for (int i = threadIdx.x; i < nlev; i += TL::THREADS) {
@@ -1547,7 +1550,7 @@ __global__ void __launch_bounds__((Tile<R, NX, V>::THREADS))
 k_x_c2c(Geo g, const cplx<double>* __restrict__ in, cplx<double>* __restrict__ out,
         const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NX, V>;
-  using F = Fft<R, NX, DIR, TL::PL>;
+  using F = Fft<R, NX, DIR, TL::PL, TL::XWAVE>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
   TwTab<R, TL::NN, TL::PL> twr;
@@ -1634,7 +1637,7 @@ __global__ void __launch_bounds__((Tile<R, NX, V>::THREADS))
 k_x_psi_max(Geo g, const cplx<R>* __restrict__ P, unsigned long long* __restrict__ absmax,
             const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NX, V>;
-  using FI = Fft<R, NX, +1, TL::PL>;
+  using FI = Fft<R, NX, +1, TL::PL, TL::XWAVE>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
   TwTab<R, TL::NN, TL::PL> twr;
@@ -1700,7 +1703,7 @@ __global__ void __launch_bounds__((Tile<R, NX, V>::THREADS))
 k_x_check(Geo g, const cplx<R>* __restrict__ in, unsigned long long* __restrict__ absmax,
           const cplx<R>* __restrict__ tw) {
   using TL = Tile<R, NX, V>;
-  using F = Fft<R, NX, +1, TL::PL>;
+  using F = Fft<R, NX, +1, TL::PL, TL::XWAVE>;
   __shared__ cplx<R> lds[TL::S * TL::LD];
   const int r = threadIdx.x / TL::T, lane = threadIdx.x % TL::T;
   TwTab<R, TL::NN, TL::PL> twr;
