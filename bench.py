@@ -455,7 +455,7 @@ def fp64_only(qg, args):
     grid, m = build_model(qg, "fp64")
     psi, _ = workload_inputs(grid)
     set_workload(qg, m, psi)
-    choice = m.autotune(DT, steps=16) if args.autotune else None
+    choice = m.autotune(DT, steps=16, in_step=True) if args.autotune else None
     log("timing the FP64 model")
     t, _ = timed_run(qg, m, args.steps, args.warmup)
     kernel_steps = min(args.steps, 200)
@@ -530,7 +530,7 @@ def main():
         set_workload(qg, m, psi)
         if args.autotune:
             t0 = time.perf_counter()
-            tuning[name] = {"choice": m.autotune(DT, steps=16), "seconds": time.perf_counter() - t0}
+            tuning[name] = {"choice": m.autotune(DT, steps=16, in_step=True), "seconds": time.perf_counter() - t0}
             log(f"autotuned the {name} model in {tuning[name]['seconds']:.2f} s")
 
     npts = SIZE[0] * SIZE[1] * SIZE[2]

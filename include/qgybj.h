@@ -287,6 +287,10 @@ int qg_synchronize(qg_handle* h);
  * the launch that solves q -> psi and B -> A (and C) together). */
 #define QG_TUNE_PASSES 18
 #define QG_TUNE_FORCE 1
+/* After the per-launch search, time the candidates of the passes that take
+ * most of the step as whole graph-replayed steps and keep the fastest (about
+ * two seconds more at 256x256x128). */
+#define QG_TUNE_IN_STEP 2
 int qg_autotune(qg_handle* h, double dt, int steps, int flags);
 int qg_get_tuning(qg_handle* h, int* choices, int n);
 int qg_set_tuning(qg_handle* h, const int* choices, int n);

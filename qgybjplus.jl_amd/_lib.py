@@ -67,6 +67,7 @@ TUNE_PASS_NAMES = ("ya_qw", "xa_qw", "ya2_qeff", "yb_adv", "x_c2r", "xb_q", "xb_
                    "thomas_pair")
 TUNE_PASSES = len(TUNE_PASS_NAMES)  # QG_TUNE_PASSES
 TUNE_FORCE = 1                      # QG_TUNE_FORCE
+TUNE_IN_STEP = 2                    # QG_TUNE_IN_STEP
 
 
 class QgError(RuntimeError):
@@ -380,10 +381,12 @@ class Handle:
         return out.reshape(n, 5)
 
     # -- autotuner ---------------------------------------------------------------
-    def autotune(self, dt: float, steps: int = 3, force: bool = False) -> dict:
+    def autotune(self, dt: float, steps: int = 3, force: bool = False, in_step: bool = False) -> dict:
         """Time the tile variants / launch shapes of every tunable pass on this
-        model's state and keep the fastest (cached per grid class)."""
-        self._check(self._lib.qg_autotune(self.h, float(dt), int(steps), TUNE_FORCE if force else 0))
+        model's state and keep the fastest (cached per grid class); in_step:
+        also refine the passes that take most of the step by whole-step times."""
+        flags = (TUNE_FORCE if force else 0) | (TUNE_IN_STEP if in_step else 0)
+        self._check(self._lib.qg_autotune(self.h, float(dt), int(steps), flags))
         return self.tuning()
 
     def tuning(self) -> dict:

@@ -471,12 +471,15 @@ class QGYBJModel:
         reduced on the device from psi = L^-1 q and A, C from B."""
         return dict(zip(ENERGY_NAMES, (float(v) for v in self._h.energies(-1)[0])))
 
-    def autotune(self, dt: float, steps: int = 3, force: bool = False) -> dict:
+    def autotune(self, dt: float, steps: int = 3, force: bool = False, in_step: bool = False) -> dict:
         """Choose the FFT-pass tile variants and column-solve launch shapes by
         timing `steps` steps of size `dt` per candidate on this model's state
         (which is restored).  The choice is cached per grid class in the
-        process; `force` re-times.  Returns the choice per pass."""
-        return self._h.autotune(dt, steps, force)
+        process; `force` re-times.  `in_step`: then time the candidates of the
+        passes that take most of the step as whole graph-replayed steps and
+        keep the fastest (about two seconds more at 256x256x128).  Returns
+        the choice per pass."""
+        return self._h.autotune(dt, steps, force, in_step)
 
     def tuning(self) -> dict:
         return self._h.tuning()

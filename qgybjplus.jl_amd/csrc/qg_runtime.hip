@@ -1858,15 +1858,20 @@ int qg_fft(qg_handle* h, const double* host_in, double* host_out, int direction)
 // operator at the c-th launch shape that fits the depth.  A pass's time for a
 // candidate is the median of its launch durations; a pass leaves its default
 // only for a candidate at least 1 % faster than the default (timing noise
-// between equal variants is about that size).  The state is restored after
-// every candidate and at the end, so tuning leaves the model where it was.
+// between equal variants is about that size).  With QG_TUNE_IN_STEP the
+// passes that take most of the step are then re-chosen one at a time from
+// whole graph-replayed steps.  The state is restored after every candidate
+// and at the end, so tuning leaves the model where it was.
 // ---------------------------------------------------------------------------
 namespace {
 
 std::mutex g_tune_mu;
-std::map<std::tuple<int, int, int, int>, std::array<int, qg::P_END>> g_tune_cache;
+std::map<std::tuple<int, int, int, int, int>, std::array<int, qg::P_END>> g_tune_cache;
 
-std::tuple<int, int, int, int> tune_key(const qg_handle& h) { return {h.nx, h.ny, h.nzc, h.precision}; }
+// the grid class, and whether the choice was refined on whole steps
+std::tuple<int, int, int, int, int> tune_key(const qg_handle& h, bool in_step) {
+  return {h.nx, h.ny, h.nzc, h.precision, (int)in_step};
+}
 
 std::vector<int> column_candidates(int nzc) {
   std::vector<int> c;
@@ -1887,7 +1892,7 @@ bool valid_choice(const qg_handle& h, int pass, int v) {
   return v >= 0 && v < qg::kScanShapeCount && qg::scan_shape_fits(qg::scan_shape_of(v, h.nzc));
 }
 
-void autotune(qg_handle& H, double dt, int steps) {
+void autotune(qg_handle& H, double dt, int steps, bool in_step) {
   if (H.prof_mode) throw QgError(QG_ERR_ARGUMENT, "autotune while a profile is being recorded");
   H.psi64_nm = 0;
   HIPCHECK(hipStreamSynchronize(H.st));
@@ -2010,6 +2015,63 @@ void autotune(qg_handle& H, double dt, int steps) {
     if (choice[p] >= 0 && dflt[p] < std::numeric_limits<double>::infinity() && best[p] > 0.99 * dflt[p])
       choice[p] = -1;
   std::copy(choice, choice + qg::P_END, H.tune);
+  if (!in_step || !H.graphs) return;
+
+  // In-step refinement.  The choices above come from launches timed one at
+  // a time; in the captured step some passes overlap the w passes' stream,
+  // which can favour another variant.  For the passes that take most of the
+  // step, each candidate is timed as whole graph-replayed steps (the other
+  // passes fixed), and one that beats the current choice by 0.3 % replaces it.
+  const int key_passes[] = {qg::P_YB2_B, qg::P_XB_B,   qg::P_YB_ADV,    qg::P_XA_QW,
+                            qg::P_XF,    qg::P_YF_OMEGA, qg::P_YA2_QEFF, qg::P_THOMAS_PAIR};
+  hipEvent_t ea = H.event(), eb = H.event();
+  with_engine(H, [&](auto& e) {
+    auto none = [](int) {};
+    auto step_ms = [&]() -> double {  // every candidate steps from the caller's q and B
+      HIPCHECK(hipStreamSynchronize(H.st));
+      H.cur = cleanup.cur;
+      HIPCHECK(hipMemcpyAsync(H.q[H.cur], save + kept[0].off, fb, hipMemcpyDeviceToDevice, H.st));
+      HIPCHECK(hipMemcpyAsync(H.B[H.cur], save + kept[1].off, fb, hipMemcpyDeviceToDevice, H.st));
+      H.diag_valid = false;
+      H.clean = false;
+      H.t1_valid = false;
+      H.drop_graphs();
+      advance(H, e, 1 + 2 * kLongGraphSteps, 0, none);  // an eager step, then the graphs' capture
+      HIPCHECK(hipEventRecord(ea, H.st));
+      advance(H, e, 2 * kLongGraphSteps, 0, none);
+      HIPCHECK(hipEventRecord(eb, H.st));
+      HIPCHECK(hipEventSynchronize(eb));
+      check_status(H);
+      float ms = 0;
+      HIPCHECK(hipEventElapsedTime(&ms, ea, eb));
+      return ms;
+    };
+    for (int p : key_passes) {
+      std::vector<int> cands;
+      if (p < qg::P_FFT_END) {
+        if (!fft_variants_ok(H)) continue;
+        for (int v = 0; v < qg::kTileVariants; ++v) cands.push_back(v);
+      } else {
+        if (H.fixed_flow || H.passive || H.no_disp || H.nzc <= 1) continue;
+        cands = shapes;
+      }
+      int cur = H.tune[p];
+      double best_ms = step_ms();
+      for (int c : cands) {
+        if (c == cur || (cur < 0 && c == (p < qg::P_FFT_END ? 1 : qg::scan_pair_default_shape(H.nzc)))) continue;
+        H.tune[p] = c;
+        const double t = step_ms();
+        if (t < 0.997 * best_ms) {
+          best_ms = t;
+          cur = c;
+        }
+      }
+      H.tune[p] = cur;
+    }
+  });
+  std::copy(H.tune, H.tune + qg::P_END, choice);
+  H.pool_used = 0;
+  std::copy(choice, choice + qg::P_END, H.tune);
 }
 
 }  // namespace
@@ -2022,7 +2084,8 @@ int qg_autotune(qg_handle* h, double dt, int steps, int flags) {
     if (steps <= 0) steps = 4;
     qg_handle& H = *h;
     if (H.active == 0) return;
-    const auto key = tune_key(H);
+    const bool in_step = (flags & QG_TUNE_IN_STEP) != 0;
+    const auto key = tune_key(H, in_step);
     if (!(flags & QG_TUNE_FORCE)) {
       std::lock_guard<std::mutex> lk(g_tune_mu);
       auto it = g_tune_cache.find(key);
@@ -2033,7 +2096,7 @@ int qg_autotune(qg_handle* h, double dt, int steps, int flags) {
         return;
       }
     }
-    autotune(H, dt, steps);
+    autotune(H, dt, steps, in_step);
     std::array<int, qg::P_END> t;
     std::copy(H.tune, H.tune + qg::P_END, t.begin());
     std::lock_guard<std::mutex> lk(g_tune_mu);

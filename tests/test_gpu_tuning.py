@@ -130,6 +130,33 @@ def test_autotune_restores_state_and_caches(qg):
     u.finalize()
 
 
+def test_autotune_in_step_restores_state(qg):
+    """The in-step refinement replays whole steps per candidate: the state
+    is still put back, the choice is valid, is cached apart from the
+    per-launch choice, and the tuned model steps as an untuned one."""
+    print("case autotune in_step", flush=True)
+    m = _model(qg, (64, 64, 16), "mixed")
+    _init(qg, m)
+    every = ("q", "B", "psi", "A", "C", "u", "v", "w")
+    before = {n: m._h.get_field(n) for n in every}
+    choice = m.autotune(dt=300.0, steps=2, force=True, in_step=True)
+    assert set(choice) == set(FFT + COLUMN)
+    assert all(-1 <= choice[p] <= 3 for p in FFT)
+    for n in every:
+        assert np.array_equal(m._h.get_field(n), before[n]), n
+    u = _model(qg, (64, 64, 16), "mixed")
+    _init(qg, u)
+    assert u.autotune(dt=300.0, in_step=True) == choice  # from the cache
+    u.set_tuning({p: -1 for p in FFT + COLUMN})
+    ts = qg.ExponentialRungeKutta2(dt=300.0)
+    qg.step_(m, ts, 6)
+    qg.step_(u, ts, 6)
+    for n in ("q", "B", "psi", "A"):
+        assert _rel(m._h.get_field(n), u._h.get_field(n)) <= 1e-6, n
+    m.finalize()
+    u.finalize()
+
+
 def test_autotune_on_base_only_lengths(qg):
     """96 x 80 has no tile variants compiled: the tuner keeps every FFT pass
     at the base variant, still tunes the column shapes, refuses other FFT
