@@ -267,6 +267,61 @@ k_xa_qw(Geo g, const cplx<R>* __restrict__ T1, const cplx<R>* __restrict__ T3,
 #pragma unroll
     for (int t = 0; t < FI::R0; ++t) v[it][t] = y3[it][t];
   FI::run(v, sl, lane, twr);  // v = By
+  const size_t hb = (size_t)row * g.Wp;
+  if constexpr (TL::XWAVE) {
+    // qwr and |B|^2 (both real) in one packed forward transform, unpacked
+    // through LDS.  |B|^2 is scaled by the power of two that brings its row
+    // maximum to qwr's: qwr carries an x and a y derivative, so the two
+    // differ by many orders of magnitude, and unscaled the packed transform
+    // would leave round-off of |B|^2's size in qwr's coefficients.  A power
+    // of two scales and unscales exactly.  A row's T lanes lie in one wave.
+    R mq = 0, mb = 0;
+#pragma unroll
+    for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < FI::R0; ++t) {
+        const R q = w[it][t].im * v[it][t].re - w[it][t].re * v[it][t].im;
+        v[it][t].re = q;
+        if (FI::valid(lane, it)) {
+          mq = fmax(mq, fabs(q));
+          mb = fmax(mb, mag[it][t]);
+        }
+      }
+#pragma unroll
+    for (int o = 1; o < TL::T; o <<= 1) {
+      mq = fmax(mq, __shfl_xor(mq, o, TL::T));
+      mb = fmax(mb, __shfl_xor(mb, o, TL::T));
+    }
+    int eq = 0, eb = 0;
+    (void)frexp(mq, &eq);
+    (void)frexp(mb, &eb);
+    const int e = (mq > 0 && mb > 0) ? min(max(eq - eb, -100), 100) : 0;
+    const R s = ldexp((R)1, e), is = ldexp((R)0.5, -e);
+#pragma unroll
+    for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < FI::R0; ++t) v[it][t].im = s * mag[it][t];
+    FF::run(v, sl, lane, twr);
+    FI::sync();  // the last stage's LDS reads are done
+#pragma unroll
+    for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < FI::R0; ++t)
+        if (FI::valid(lane, it)) sl[FI::pad(FI::elem(lane, it, t))] = v[it][t];
+    FI::sync();
+#pragma unroll
+    for (int it = 0; it < FI::IT; ++it)
+#pragma unroll
+      for (int t = 0; t < FI::R0; ++t) {
+        int x = FI::elem(lane, it, t);
+        if (ok && FI::valid(lane, it) && x < g.Kx) {
+          const cplx<R> a = v[it][t], m = conj(sl[FI::pad(x == 0 ? 0 : NX - x)]);
+          Qa[hb + x] = mk<R>((R)0.5 * (a.re + m.re), (R)0.5 * (a.im + m.im));
+          Qb[hb + x] = mk<R>(is * (a.im - m.im), -is * (a.re - m.re));
+        }
+      }
+    return;
+  }
 #pragma unroll
   for (int it = 0; it < FI::IT; ++it)
 #pragma unroll
@@ -277,7 +332,6 @@ k_xa_qw(Geo g, const cplx<R>* __restrict__ T1, const cplx<R>* __restrict__ T3,
     }
   FF::run(v, sl, lane, twr);
   FF::run(w, sl, lane, twr);
-  const size_t hb = (size_t)row * g.Wp;
 #pragma unroll
   for (int it = 0; it < FI::IT; ++it)
 #pragma unroll

@@ -96,8 +96,8 @@ def k3_streamfunction(grid: O.Grid, seed: int, urms: float = 0.1, kpeak: float =
     iy = np.fft.fftfreq(ny) * ny
     K = np.sqrt(ix[:, None] ** 2 + iy[None, :] ** 2)
     k0 = kpeak * math.sqrt(1.5)
-    with np.errstate(divide="ignore"):
-        amp = np.where(K > 0, K ** -1.5 * np.exp(-0.5 * (k0 / np.maximum(K, 1e-30)) ** 2), 0.0)
+    Ks = np.where(K > 0, K, 1.0)  # the mean (K = 0) gets no amplitude
+    amp = np.where(K > 0, Ks ** -1.5 * np.exp(-0.5 * (k0 / Ks) ** 2), 0.0)
     mask = O.dealias_mask(nx, ny)
 
     def one():
