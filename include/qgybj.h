@@ -277,8 +277,9 @@ int qg_synchronize(qg_handle* h);
  * `steps` eager steps of size dt per candidate on the model's own state,
  * restores the state, and keeps per pass the candidate with the least median
  * launch time when it beats the default by at least 1 %; the choice is
- * cached per grid class (nx, ny, nz per member, precision) in the process and
- * reused by later calls unless flags has QG_TUNE_FORCE.  -1 is the default
+ * cached per grid class (nx, ny, nz per member, precision) and
+ * QG_TUNE_IN_STEP flag in the process and reused by later calls unless flags
+ * has QG_TUNE_FORCE.  -1 is the default
  * choice of a pass.  choices[] has QG_TUNE_PASSES entries in the order
  * ya_qw, xa_qw, ya2_qeff, yb_adv, x_c2r, xb_q, xb_B, yb2_q, yb2_B, yf_omega,
  * xf, y_r2c, y_herm, thomas_psi, thomas_A, thomas_AC, omega, thomas_pair
@@ -287,9 +288,12 @@ int qg_synchronize(qg_handle* h);
  * the launch that solves q -> psi and B -> A (and C) together). */
 #define QG_TUNE_PASSES 18
 #define QG_TUNE_FORCE 1
-/* After the per-launch search, time the candidates of the passes that take
- * most of the step as whole graph-replayed steps and keep the fastest (about
- * two seconds more at 256x256x128). */
+/* After the per-launch search, re-choose the passes that take most of the
+ * step (yb2_B, xb_B, yb_adv, xa_qw, xf, yf_omega, ya2_qeff, thomas_pair) one
+ * at a time from whole graph-replayed steps, 32 per candidate from the
+ * caller's state; a candidate replaces the choice when 0.3 % faster (about
+ * two seconds more at 256x256x128).  Without graphs (qg_use_graphs off) the
+ * per-launch choice stands. */
 #define QG_TUNE_IN_STEP 2
 int qg_autotune(qg_handle* h, double dt, int steps, int flags);
 int qg_get_tuning(qg_handle* h, int* choices, int n);
