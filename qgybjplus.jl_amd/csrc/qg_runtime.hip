@@ -1075,7 +1075,10 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
         h->th_cp[o] = h->alloc<double>(tsz);
       }
       int* e = h->alloc<int>(8);
-      HIPCHECK(hipMemset(e, 0, 8 * sizeof(int)));
+      // the error words are cleared on the stream the factor kernels run on:
+      // a plain hipMemset (null stream) is not ordered before work on this
+      // non-blocking stream and could clear an error they recorded
+      HIPCHECK(hipMemsetAsync(e, 0, 8 * sizeof(int), h->st));
       qg::launch_thomas_factor(g, h->st, h->d_a, 1, h->th_inv[0], h->th_cp[0], e);
       qg::launch_thomas_factor(g, h->st, h->d_a, 4, h->th_inv[1], h->th_cp[1], e + 4);
       int he[8];
@@ -1086,7 +1089,7 @@ int qg_create(const qg_config* cfg, qg_handle** out) {
       h->om_M = h->alloc<double>(tsz);
       h->om_D = h->alloc<double>(tsz);
       h->om_piv = h->alloc<unsigned char>(tsz / nzc);
-      HIPCHECK(hipMemset(e, 0, 8 * sizeof(int)));
+      HIPCHECK(hipMemsetAsync(e, 0, 8 * sizeof(int), h->st));
       qg::launch_omega_factor(g, h->st, h->d_cz, h->om_M, h->om_D, h->om_piv, e);
       HIPCHECK(hipMemcpyAsync(he, e, sizeof he, hipMemcpyDeviceToHost, h->st));
       HIPCHECK(hipStreamSynchronize(h->st));
