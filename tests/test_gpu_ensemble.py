@@ -223,3 +223,32 @@ def test_failed_stratification_change_keeps_the_member(qg):
             assert np.max(np.abs(got - ref)) <= 1e-14 * np.max(np.abs(ref)), (i, n)
     m.finalize()
     ens.finalize()
+
+
+def test_in_step_tuning_leaves_members_unchanged(qg):
+    """The tuner's in-step refinement replays whole ensemble steps per
+    candidate: every member's fields are put back, and the tuned ensemble's
+    members step as lone runs do (FP64: tile variants are bitwise, column
+    shapes agree to a few ulp)."""
+    print("case ensemble in-step tuning", flush=True)
+    kw = _kwargs(qg)
+    M = 3
+    ens = qg.QGYBJEnsemble(stratifications=[_strat(qg, i) for i in range(M)], **kw)
+    for i in range(M):
+        psi, B = _initial(qg, kw["grid"], i)
+        ens.set_member(i, psi=qg.FieldArray(psi), waves=qg.FieldArray(B))
+    names = ("q", "B", "psi", "A", "C", "u", "v", "w")
+    before = [{n: getattr(ens.member_fields(i), n).copy() for n in names} for i in range(M)]
+    ens.autotune(600.0, steps=2, force=True, in_step=True)
+    for i in range(M):
+        mf = ens.member_fields(i)
+        for n in names:
+            assert np.array_equal(getattr(mf, n), before[i][n]), (i, n)
+    qg.step_(ens, qg.ExponentialRungeKutta2(dt=600.0), 4)
+    for i in range(M):
+        ref = _alone(qg, i, 4)
+        mf = ens.member_fields(i)
+        for n in ("q", "B", "psi", "A", "C", "w"):
+            e = rel(getattr(mf, n), ref[n])
+            assert e <= 1e-12, (i, n, e)
+    ens.finalize()
