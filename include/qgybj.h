@@ -289,11 +289,12 @@ int qg_synchronize(qg_handle* h);
 #define QG_TUNE_PASSES 18
 #define QG_TUNE_FORCE 1
 /* After the per-launch search, re-choose the passes that take most of the
- * step (yb2_B, xb_B, yb_adv, xa_qw, xf, yf_omega, ya2_qeff, thomas_pair) one
- * at a time from whole graph-replayed steps, 32 per candidate from the
+ * step (yb2_B, xb_B, yb_adv, xa_qw, xf, yf_omega, ya2_qeff, thomas_pair) and
+ * the w passes that run beside the next step (omega, y_r2c, y_herm, x_c2r)
+ * one at a time from whole graph-replayed steps, 32 per candidate from the
  * caller's state; a candidate replaces the choice when 0.3 % faster (about
- * two seconds more at 256x256x128).  Without graphs (qg_use_graphs off) the
- * per-launch choice stands. */
+ * three seconds more at 256x256x128).  Without graphs (qg_use_graphs off)
+ * the per-launch choice stands. */
 #define QG_TUNE_IN_STEP 2
 int qg_autotune(qg_handle* h, double dt, int steps, int flags);
 int qg_get_tuning(qg_handle* h, int* choices, int n);

@@ -477,7 +477,7 @@ class QGYBJModel:
         (which is restored).  The choice is cached per grid class in the
         process; `force` re-times.  `in_step`: then time the candidates of the
         passes that take most of the step as whole graph-replayed steps and
-        keep the fastest (about two seconds more at 256x256x128).  Returns
+        keep the fastest (about three seconds more at 256x256x128).  Returns
         the choice per pass."""
         return self._h.autotune(dt, steps, force, in_step)
 

@@ -2023,10 +2023,12 @@ void autotune(qg_handle& H, double dt, int steps, bool in_step) {
   // In-step refinement.  The choices above come from launches timed one at
   // a time; in the captured step some passes overlap the w passes' stream,
   // which can favour another variant.  For the passes that take most of the
-  // step, each candidate is timed as whole graph-replayed steps (the other
+  // step, and then for the w passes (which run beside the next step's
+  // passes), each candidate is timed as whole graph-replayed steps (the other
   // passes fixed), and one that beats the current choice by 0.3 % replaces it.
-  const int key_passes[] = {qg::P_YB2_B, qg::P_XB_B,   qg::P_YB_ADV,    qg::P_XA_QW,
-                            qg::P_XF,    qg::P_YF_OMEGA, qg::P_YA2_QEFF, qg::P_THOMAS_PAIR};
+  const int key_passes[] = {qg::P_YB2_B,    qg::P_XB_B,     qg::P_YB_ADV,  qg::P_XA_QW,
+                            qg::P_XF,       qg::P_YF_OMEGA, qg::P_YA2_QEFF, qg::P_THOMAS_PAIR,
+                            qg::P_OMEGA,    qg::P_Y_R2C,    qg::P_Y_HERM,  qg::P_X_C2R};
   hipEvent_t ea = H.event(), eb = H.event();
   with_engine(H, [&](auto& e) {
     auto none = [](int) {};
@@ -2054,14 +2056,20 @@ void autotune(qg_handle& H, double dt, int steps, bool in_step) {
       if (p < qg::P_FFT_END) {
         if (!fft_variants_ok(H)) continue;
         for (int v = 0; v < qg::kTileVariants; ++v) cands.push_back(v);
-      } else {
+      } else if (p == qg::P_THOMAS_PAIR) {
         if (H.fixed_flow || H.passive || H.no_disp || H.nzc <= 1) continue;
         cands = shapes;
+      } else {
+        if (H.nzc <= 2) continue;  // no interior level: no omega solve
+        cands = shapes;
       }
+      const int def = p < qg::P_FFT_END              ? 1
+                      : p == qg::P_THOMAS_PAIR ? qg::scan_pair_default_shape(H.nzc)
+                                               : qg::scan_default_shape(H.nzc);
       int cur = H.tune[p];
       double best_ms = step_ms();
       for (int c : cands) {
-        if (c == cur || (cur < 0 && c == (p < qg::P_FFT_END ? 1 : qg::scan_pair_default_shape(H.nzc)))) continue;
+        if (c == cur || (cur < 0 && c == def)) continue;
         H.tune[p] = c;
         const double t = step_ms();
         if (t < 0.997 * best_ms) {
